@@ -1,0 +1,151 @@
+"""Benchmark: ray samples/s and frames/s of the HIP raycaster on the medium
+workload (512^3 uint16, 256^3 bricks + 2-voxel apron, 3 LODs, 4096-entry
+Gaussian-bump TF, 1920x1080 perspective 60 degree FoV, 36-view orbit,
+sampling rate 1.5, min/max empty-space skipping), rendered at LOD 0.
+
+A step is one frame: one raycast of the whole 1080p image for the next
+view of the orbit, with the volume already bricked into the device pool.
+`value` is nominal samples per second (sum over pixels of
+floor(chord * rate) through the LOD-0 box, identical for any renderer);
+frames/s and evaluated samples/s are reported beside it.  The roofline
+line prices the raycast kernel at its algorithmic gather traffic:
+evaluated samples x 8 corners x 2 bytes + 4 bytes per pixel, over the
+kernel's HIP-event time, against 8 TB/s.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gsamples/s + FPS @1080p 512³u16; learned-LOD FPS at ≤1/255 RMSE; p99 ms"
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "raycast_medium_traffic.json")
+
+
+def cpu_baseline(vol, w, cams, threads: int) -> dict:
+    """The oracle's FP32 multi-threaded CPU raycaster on a bounded sample of
+    the same workload (the first `len(cams)` orbit views at full size)."""
+    from oracle import oracle as O
+    from imagevis3d_amd import workloads
+    nominal, secs = 0, 0.0
+    for cam in cams:
+        p = O.make_params(vol, w.tf, **workloads.oracle_kwargs(w, cam))
+        nominal += O.nominal_samples(p)
+        t0 = time.perf_counter()
+        O.baseline_render(p, threads)
+        secs += time.perf_counter() - t0
+    return {"value": nominal / secs / 1e9, "unit": "Gsamples/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{len(cams)} of the 36 orbit views at 1920x1080 on the 512^3 volume, "
+                      f"LOD 0, {secs:.1f} s of CPU time (FP32 OpenMP raycaster, same model)",
+            "fps": len(cams) / secs}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=36)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-views", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
+              "(every GPU box here has one GPU; the sharding design is in DESIGN.md)",
+              file=sys.stderr)
+        return 2
+
+    from imagevis3d_amd import tuvok, workloads
+
+    w = workloads.medium()
+    print(f"[bench] generating {w.name} volume", file=sys.stderr, flush=True)
+    vol = w.volume()
+    r = tuvok.renderer.new(tuvok.renderer.types.OpenGL_Raycaster, False, False, False, False)
+    assert r.initialize(tuvok.createContext(w.width, w.height, device=0)), r.lastError()
+    r.setRendererTarget(tuvok.renderer.types.RT_Headless)
+    t0 = time.perf_counter()
+    r.setVolume(vol, w.brick, w.apron, w.n_lod)
+    brick_s = time.perf_counter() - t0
+    r.set1DTrans(w.tf)
+    r.setLODLimits((0, 0))
+    workloads.apply(r, w, w.cameras[0])
+    print(f"[bench] bricked into the device pool in {brick_s:.2f} s; warmup {args.warmup}",
+          file=sys.stderr, flush=True)
+
+    cams = w.cameras
+    for i in range(args.warmup):
+        workloads.set_camera(r, cams[i % len(cams)])
+        assert r.paint(), r.lastError()
+        r.frameStats()
+    r.synchronize()
+
+    kernel_ms, evaluated, nominal = [], 0, 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        workloads.set_camera(r, cams[i % len(cams)])
+        if not r.paint():
+            raise RuntimeError(r.lastError())
+        st = r.frameStats()  # waits for this frame's events and counters
+        kernel_ms.append(st.kernel_ms)
+        evaluated += st.evaluated_samples
+        nominal += st.nominal_samples
+    r.synchronize()
+    wall = time.perf_counter() - t0
+    print(f"[bench] {args.steps} frames in {wall:.3f} s", file=sys.stderr, flush=True)
+
+    ksum = sum(kernel_ms) / 1e3
+    gather_bytes = evaluated * 8 * 2 + args.steps * w.width * w.height * 4
+    achieved = gather_bytes / ksum / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        if t.get("workload") == w.name:
+            traffic = t.get("bytes_per_launch")
+    out = {
+        "metric": METRIC,
+        "value": nominal / wall / 1e9,
+        "unit": "Gsamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": w.name, "volume": "512x512x512 u16", "image": "1920x1080",
+                   "views": len(cams), "rate": w.rate, "bricks": "256^3 + 2-voxel apron",
+                   "lod": 0, "tf": "4096 x RGBA f32, 3 Gaussian bumps", "fov_deg": 60},
+        "fps": args.steps / wall,
+        "evaluated_gsamples_per_s": evaluated / wall / 1e9,
+        "nominal_samples_per_frame": nominal // args.steps,
+        "evaluated_samples_per_frame": evaluated // args.steps,
+        "kernel_ms_per_frame": ksum / args.steps * 1e3,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "cpu_baseline": None,
+    }
+    r.cleanup()
+    if not args.no_cpu_baseline:
+        try:
+            threads = min(16, len(os.sched_getaffinity(0)))
+        except AttributeError:
+            threads = min(16, os.cpu_count() or 1)
+        print(f"[bench] CPU baseline on {threads} threads", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(vol, w, cams[:args.cpu_views], threads)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

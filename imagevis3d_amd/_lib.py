@@ -1,0 +1,140 @@
+"""ctypes bindings of libtvkhip.so (the C ABI declared in include/tvk_hip.h).
+
+The library is built in-tree by ``imagevis3d_amd/build.py``.  Loading fails
+loudly when it is missing: there is no CPU fallback for the renderer.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtvkhip.so")
+
+TVK_OK = 0
+ERRORS = {
+    -1: "TVK_ERR_ARG", -2: "TVK_ERR_HIP", -3: "TVK_ERR_STATE", -4: "TVK_ERR_IO",
+    -5: "TVK_ERR_OOM", -6: "TVK_ERR_FORMAT", -7: "TVK_ERR_CHECKSUM",
+}
+
+
+class TvkError(RuntimeError):
+    """A failed call of the C ABI (Tuvok's bool-false / LuaError)."""
+
+    def __init__(self, func: str, code: int, msg: str):
+        super().__init__(f"{func} -> {ERRORS.get(code, code)}: {msg}")
+        self.code = code
+        self.func = func
+
+
+class FrameStats(C.Structure):
+    _fields_ = [
+        ("evaluated_samples", C.c_uint64), ("nominal_samples", C.c_uint64),
+        ("bricks_total", C.c_uint32), ("bricks_skipped", C.c_uint32),
+        ("lod", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
+        ("kernel_ms", C.c_float), ("paint_ms", C.c_float),
+    ]
+
+
+class MemStats(C.Structure):
+    _fields_ = [
+        ("gpu_mem", C.c_uint64), ("gpu_budget", C.c_uint64),
+        ("allocated_gpu_mem", C.c_uint64), ("pool_slots", C.c_uint64),
+        ("resident_bricks", C.c_uint64), ("slot_bytes", C.c_uint64),
+        ("bricks_uploaded", C.c_uint64), ("bricks_evicted", C.c_uint64),
+    ]
+
+
+H = C.c_void_p
+u32, u64, i32, f32, f64 = C.c_uint32, C.c_uint64, C.c_int, C.c_float, C.c_double
+P = C.c_void_p
+F3 = C.POINTER(C.c_float)
+
+# name -> argtypes (every function returns int unless listed in _RESTYPE)
+SIGNATURES = {
+    "tvk_last_error": [],
+    "tvk_version": [],
+    "tvk_create": [i32, i32, i32, i32, i32, C.POINTER(H)],
+    "tvk_destroy": [H],
+    "tvk_initialize": [H, i32],
+    "tvk_resize": [H, u32, u32],
+    "tvk_set_renderer_target": [H, i32],
+    "tvk_get_renderer_target": [H, C.POINTER(i32)],
+    "tvk_load_dataset": [H, C.c_char_p, i32],
+    "tvk_set_volume": [H, P, u64, u64, u64, u32, u32, u32, u32],
+    "tvk_get_lod_level_count": [H, C.POINTER(u32)],
+    "tvk_get_domain_size": [H, u32, C.POINTER(u64)],
+    "tvk_get_bit_width": [H, C.POINTER(u32)],
+    "tvk_get_brick_layout": [H, u32, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)],
+    "tvk_read_lod": [H, u32, P, u64],
+    "tvk_read_brick": [H, u32, u32, u32, u32, P, u64],
+    "tvk_get_brick_minmax": [H, u32, P, P, u64],
+    "tvk_compute_histogram_1d": [H, P, u64],
+    "tvk_compute_histogram_2d": [H, P, u32, u32, C.POINTER(f32)],
+    "tvk_set_render_mode": [H, i32],
+    "tvk_set_1d_trans": [H, P, u32],
+    "tvk_set_2d_trans": [H, P, u32, u32, f32],
+    "tvk_set_iso_value": [H, f32],
+    "tvk_set_isosurface_color": [H, f32, f32, f32],
+    "tvk_set_view_pos": [H, f32, f32, f32],
+    "tvk_set_view_dir": [H, f32, f32, f32],
+    "tvk_set_up_dir": [H, f32, f32, f32],
+    "tvk_set_fov": [H, f32],
+    "tvk_set_ortho_view": [H, i32, f32],
+    "tvk_set_rotation_4x4": [H, P],
+    "tvk_set_translation_4x4": [H, P],
+    "tvk_set_rescale_factors": [H, f64, f64, f64],
+    "tvk_set_sample_rate_modifier": [H, f32],
+    "tvk_set_interpolant": [H, i32],
+    "tvk_set_lod_limits": [H, u32, u32],
+    "tvk_set_bg_colors": [H, P, P],
+    "tvk_set_lighting_enabled": [H, i32],
+    "tvk_set_light_colors": [H, P, P, P, P],
+    "tvk_set_early_ray_termination": [H, f32],
+    "tvk_set_empty_space_skipping": [H, i32],
+    "tvk_paint": [H],
+    "tvk_check_for_redraw": [H],
+    "tvk_schedule_complete_redraw": [H],
+    "tvk_capture_single_frame": [H, P, u64, i32],
+    "tvk_get_progress": [H, C.POINTER(u64), C.POINTER(u32), C.POINTER(u32), C.POINTER(u32),
+                         C.POINTER(u64)],
+    "tvk_get_frame_stats": [H, C.POINTER(FrameStats)],
+    "tvk_get_device_framebuffer": [H, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)],
+    "tvk_synchronize": [H],
+    "tvk_set_max_usable_gpu_mem": [H, u64],
+    "tvk_get_mem_stats": [H, C.POINTER(MemStats)],
+    "tvk_generate_volume": [i32, u64, u64, u64, u64, u32, P, u64],
+}
+_RESTYPE = {"tvk_last_error": C.c_char_p, "tvk_version": C.c_char_p}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libtvkhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python imagevis3d_amd/build.py` "
+            "(the HIP backend has no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPE.get(name, C.c_int)
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return (load().tvk_last_error() or b"").decode("utf-8", "replace")
+
+
+def call(name: str, *args) -> int:
+    """Call `name` and raise TvkError on a negative status."""
+    rc = getattr(load(), name)(*args)
+    if rc < 0:
+        raise TvkError(name, rc, last_error())
+    return rc

@@ -1,0 +1,47 @@
+// kernels.hpp -- host-side launchers of the HIP kernels (gfx950).
+// All launchers are asynchronous on `stream` and return the hipError_t of
+// the launch.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include "tvk_common.hpp"
+
+namespace tvk {
+
+// Level l+1 from level l by 2x2x2 integer mean: floor(sum / count) over the
+// existing children.  dst dims are ceil(src / 2).
+hipError_t launch_downsample(const void* src, int bpv, const int64_t src_dims[3], void* dst,
+                             hipStream_t stream);
+
+// Cut `n_jobs` bricks of payload B with apron a out of a flat level into
+// pool slots (clamp-to-edge outside the volume) and write each brick's
+// (min, max) over the slot to brick_range[2*brick_id].  Slot j starts at
+// element j * slot_stride of the pool; scratch_minmax holds 2 u32 per job.
+hipError_t launch_extract_bricks(const void* level, int bpv, const int64_t dims[3],
+                                 const BrickJob* jobs, int n_jobs, void* pool,
+                                 int64_t slot_stride, int B, int apron, float* brick_range,
+                                 uint32_t* scratch_minmax, hipStream_t stream);
+
+// 1D histogram of raw values, n_bins = 2^(8*bpv), u64 counts (cleared here).
+hipError_t launch_histogram_1d(const void* vol, int bpv, int64_t n, unsigned long long* hist,
+                               hipStream_t stream);
+
+// Squared-gradient pass of the 2D histogram: K = integer squared gradient
+// magnitude in units of 1/(2*maxval); kmax receives max K (cleared here).
+hipError_t launch_gradient_kmax(const void* vol, int bpv, const int64_t dims[3],
+                                unsigned long long* kmax, hipStream_t stream);
+
+// Binning pass: hist[v_bin * n_grad + g_bin] (cleared here) with
+// g_bin = floor((n_grad-1) * sqrt(K / kmax)) decided in exact integers.
+hipError_t launch_histogram_2d(const void* vol, int bpv, const int64_t dims[3],
+                               const unsigned long long* kmax, int n_value, int n_grad,
+                               unsigned long long* hist, hipStream_t stream);
+
+// Zero-fill a u64 buffer (a kernel, so the call is safe under graph capture).
+hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v,
+                           hipStream_t stream);
+
+// One raycast of a 1D-TF frame.
+hipError_t launch_raycast_1d(const RaycastParams& p, int bpv, hipStream_t stream);
+
+}  // namespace tvk

@@ -1,0 +1,336 @@
+// kernels_brick.hip -- HBM-streaming kernels of the UVF bricking / LOD path
+// (RasterDataBlock::FlatDataToBrickedLOD + CombineAverage + SimpleMaxMin,
+// UVFReader/DataSource.h:550-570; Histogram1D/2DDataBlock::Compute,
+// UVFReader/DataSource.h:595-636).  Integer in, integer out: every result
+// is bit-exact with the CPU twins in oracle/oracle.c.
+//
+// All of these are byte/integer streaming work bounded by HBM bandwidth:
+// threads walk rows of x so each wavefront reads and writes contiguous
+// runs, and reductions go through wave shuffles before one atomic per
+// workgroup.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace tvk {
+namespace {
+
+constexpr int kThreads = 256;
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void downsample_kernel(const T* __restrict__ src, int64_t nx,
+                                                              int64_t ny, int64_t nz,
+                                                              T* __restrict__ dst, int64_t mx,
+                                                              int64_t my, int64_t mz) {
+  const int64_t x = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (x >= mx) return;
+  // rows of a level stay below 2^31 for every supported volume, so the
+  // per-row index split uses 32-bit division.
+  const uint32_t rows = (uint32_t)(my * mz), umy = (uint32_t)my;
+  const int64_t x0 = 2 * x, x1 = 2 * x + 1 < nx ? 2 * x + 1 : -1;
+  for (uint32_t row = blockIdx.y; row < rows; row += gridDim.y) {
+    const int64_t y = row % umy, z = row / umy;
+    const int64_t ys[2] = {2 * y, 2 * y + 1 < ny ? 2 * y + 1 : -1};
+    const int64_t zs[2] = {2 * z, 2 * z + 1 < nz ? 2 * z + 1 : -1};
+    uint32_t sum = 0, cnt = 0;
+#pragma unroll
+    for (int dz = 0; dz < 2; ++dz) {
+      if (zs[dz] < 0) continue;
+#pragma unroll
+      for (int dy = 0; dy < 2; ++dy) {
+        if (ys[dy] < 0) continue;
+        const T* r = src + (zs[dz] * ny + ys[dy]) * nx;
+        sum += r[x0];
+        ++cnt;
+        if (x1 >= 0) {
+          sum += r[x1];
+          ++cnt;
+        }
+      }
+    }
+    dst[(z * my + y) * mx + x] = (T)(sum / cnt);
+  }
+}
+
+__device__ inline uint32_t wave_min(uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+__device__ inline uint32_t wave_max(uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+__device__ inline unsigned long long wave_max64(unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+__device__ inline int64_t clamp64(int64_t v, int64_t lo, int64_t hi) {
+  return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// grid.x: chunks of one slot, grid.y: brick job.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void extract_bricks_kernel(
+    const T* __restrict__ level, int64_t nx, int64_t ny, int64_t nz,
+    const BrickJob* __restrict__ jobs, T* __restrict__ pool, int64_t slot_stride, int64_t slot_elems,
+    int B, int apron, uint32_t* __restrict__ minmax) {
+  const BrickJob job = jobs[blockIdx.y];
+  const int S = B + 2 * apron;
+  const int64_t ox = (int64_t)job.bx * B - apron;
+  const int64_t oy = (int64_t)job.by * B - apron;
+  const int64_t oz = (int64_t)job.bz * B - apron;
+  T* __restrict__ slot = pool + job.slot * slot_stride;
+  const int64_t chunk = (slot_elems + gridDim.x - 1) / gridDim.x;
+  const int64_t beg = chunk * blockIdx.x;
+  const int64_t end = beg + chunk < slot_elems ? beg + chunk : slot_elems;
+  uint32_t lo = 0xffffffffu, hi = 0;
+  const uint32_t uS = (uint32_t)S;  // S^3 < 2^31 (checked by the host)
+  for (int64_t i = beg + threadIdx.x; i < end; i += kThreads) {
+    const uint32_t ii = (uint32_t)i;
+    const int64_t x = ii % uS;
+    const uint32_t yz = ii / uS;
+    const int64_t y = yz % uS, z = yz / uS;
+    const int64_t gx = clamp64(ox + x, 0, nx - 1);
+    const int64_t gy = clamp64(oy + y, 0, ny - 1);
+    const int64_t gz = clamp64(oz + z, 0, nz - 1);
+    const T v = level[(gz * ny + gy) * nx + gx];
+    slot[i] = v;
+    lo = min(lo, (uint32_t)v);
+    hi = max(hi, (uint32_t)v);
+  }
+  lo = wave_min(lo);
+  hi = wave_max(hi);
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(&minmax[2 * blockIdx.y], lo);
+    atomicMax(&minmax[2 * blockIdx.y + 1], hi);
+  }
+}
+
+__global__ void minmax_init_kernel(uint32_t* mm, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    mm[2 * i] = 0xffffffffu;
+    mm[2 * i + 1] = 0u;
+  }
+}
+
+__global__ void minmax_finish_kernel(const uint32_t* mm, const BrickJob* jobs, int n,
+                                     float* brick_range) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const int id = jobs[i].brick_id;
+    brick_range[2 * id] = (float)mm[2 * i];
+    brick_range[2 * id + 1] = (float)mm[2 * i + 1];
+  }
+}
+
+__global__ void fill_u64_kernel(unsigned long long* p, int64_t n, unsigned long long v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// u8: privatised 256-bin histogram in LDS, one u64 atomic per bin per block.
+__global__ __launch_bounds__(kThreads) void hist1d_u8_kernel(const uint8_t* __restrict__ v,
+                                                             int64_t n,
+                                                             unsigned long long* hist) {
+  __shared__ uint32_t bins[256];
+  bins[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads)
+    atomicAdd(&bins[v[i]], 1u);
+  __syncthreads();
+  if (bins[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)bins[threadIdx.x]);
+}
+
+// u16: 65536 bins do not fit a block's LDS; count in global memory.
+__global__ __launch_bounds__(kThreads) void hist1d_u16_kernel(const uint16_t* __restrict__ v,
+                                                              int64_t n,
+                                                              unsigned long long* hist) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads)
+    atomicAdd(&hist[v[i]], 1ull);
+}
+
+// Integer squared gradient in units of 1/(2*maxval): central differences
+// v[c+1]-v[c-1] inside, 2*(one-sided difference) at the borders (the
+// numpy.gradient definition that reproduces the 2D histogram stored in
+// DemoData/c60.uvf).
+template <typename T>
+__device__ inline long long grad_k(const T* __restrict__ v, int64_t i, int64_t c, int64_t n,
+                                   int64_t stride) {
+  if (n < 2) return 0;
+  if (c == 0) return 2ll * ((long long)v[i + stride] - (long long)v[i]);
+  if (c == n - 1) return 2ll * ((long long)v[i] - (long long)v[i - stride]);
+  return (long long)v[i + stride] - (long long)v[i - stride];
+}
+
+template <typename T>
+__device__ inline unsigned long long grad_K(const T* __restrict__ v, int64_t nx, int64_t ny,
+                                            int64_t nz, int64_t x, int64_t y, int64_t z) {
+  const int64_t i = (z * ny + y) * nx + x;
+  const long long gx = grad_k(v, i, x, nx, 1);
+  const long long gy = grad_k(v, i, y, ny, nx);
+  const long long gz = grad_k(v, i, z, nz, nx * ny);
+  return (unsigned long long)(gx * gx + gy * gy + gz * gz);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void gradient_kmax_kernel(const T* __restrict__ v,
+                                                                 int64_t nx, int64_t ny,
+                                                                 int64_t nz,
+                                                                 unsigned long long* kmax) {
+  unsigned long long m = 0;
+  const int64_t x = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (x < nx) {
+    const uint32_t rows = (uint32_t)(ny * nz), uny = (uint32_t)ny;
+    for (uint32_t row = blockIdx.y; row < rows; row += gridDim.y) {
+      const unsigned long long k = grad_K(v, nx, ny, nz, x, row % uny, row / uny);
+      m = k > m ? k : m;
+    }
+  }
+  m = wave_max64(m);
+  if ((threadIdx.x & 63) == 0 && m) atomicMax(kmax, m);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void hist2d_kernel(const T* __restrict__ v, int64_t nx,
+                                                          int64_t ny, int64_t nz,
+                                                          const unsigned long long* kmax_p,
+                                                          int shift, int n_grad,
+                                                          unsigned long long* hist) {
+  const int64_t x = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (x >= nx) return;
+  const unsigned long long kmax = *kmax_p;
+  const unsigned long long F = (unsigned long long)(n_grad - 1);
+  const uint32_t rows = (uint32_t)(ny * nz), uny = (uint32_t)ny;
+  for (uint32_t row = blockIdx.y; row < rows; row += gridDim.y) {
+    const int64_t y = row % uny, z = row / uny;
+    const unsigned long long K = grad_K(v, nx, ny, nz, x, y, z);
+    long long b = 0;
+    if (kmax > 0) {
+      // float estimate, then the exact integer decision of
+      // b = max{ b : b^2 * kmax <= F^2 * K }.
+      b = (long long)floorf((float)F * sqrtf((float)K / (float)kmax));
+      if (b < 0) b = 0;
+      if (b > (long long)F) b = (long long)F;
+      const unsigned long long rhs = F * F * K;
+      while (b < (long long)F && (unsigned long long)(b + 1) * (unsigned long long)(b + 1) * kmax <= rhs) ++b;
+      while (b > 0 && (unsigned long long)b * (unsigned long long)b * kmax > rhs) --b;
+    }
+    const int64_t vb = (int64_t)(v[(z * ny + y) * nx + x] >> shift);
+    atomicAdd(&hist[vb * n_grad + b], 1ull);
+  }
+}
+
+inline dim3 row_grid(int64_t nx, int64_t rows) {
+  const int64_t gy = rows < 65535 ? rows : 65535;
+  return dim3((unsigned)((nx + kThreads - 1) / kThreads), (unsigned)(gy > 0 ? gy : 1), 1);
+}
+
+}  // namespace
+
+hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v,
+                           hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + kThreads - 1) / kThreads;
+  if (blocks > 65536) blocks = 65536;
+  fill_u64_kernel<<<(unsigned)blocks, kThreads, 0, stream>>>(p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_downsample(const void* src, int bpv, const int64_t d[3], void* dst,
+                             hipStream_t stream) {
+  const int64_t mx = (d[0] + 1) / 2, my = (d[1] + 1) / 2, mz = (d[2] + 1) / 2;
+  const dim3 grid = row_grid(mx, my * mz);
+  if (bpv == 1)
+    downsample_kernel<uint8_t><<<grid, kThreads, 0, stream>>>((const uint8_t*)src, d[0], d[1], d[2],
+                                                              (uint8_t*)dst, mx, my, mz);
+  else
+    downsample_kernel<uint16_t><<<grid, kThreads, 0, stream>>>(
+        (const uint16_t*)src, d[0], d[1], d[2], (uint16_t*)dst, mx, my, mz);
+  return hipGetLastError();
+}
+
+hipError_t launch_extract_bricks(const void* level, int bpv, const int64_t d[3],
+                                 const BrickJob* jobs, int n_jobs, void* pool,
+                                 int64_t slot_stride, int B, int apron, float* brick_range,
+                                 uint32_t* scratch, hipStream_t stream) {
+  const int64_t S = B + 2 * apron;
+  const int64_t slot_elems = S * S * S;
+  if (n_jobs <= 0) return hipSuccess;
+  minmax_init_kernel<<<(n_jobs + 255) / 256, 256, 0, stream>>>(scratch, n_jobs);
+  int64_t chunks = (slot_elems + 8191) / 8192;
+  if (chunks > 1024) chunks = 1024;
+  if (chunks < 1) chunks = 1;
+  for (int j0 = 0; j0 < n_jobs; j0 += 65535) {
+    const int nj = n_jobs - j0 < 65535 ? n_jobs - j0 : 65535;
+    const dim3 grid((unsigned)chunks, (unsigned)nj, 1);
+    if (bpv == 1)
+      extract_bricks_kernel<uint8_t><<<grid, kThreads, 0, stream>>>(
+          (const uint8_t*)level, d[0], d[1], d[2], jobs + j0, (uint8_t*)pool, slot_stride, slot_elems, B, apron,
+          scratch + 2 * j0);
+    else
+      extract_bricks_kernel<uint16_t><<<grid, kThreads, 0, stream>>>(
+          (const uint16_t*)level, d[0], d[1], d[2], jobs + j0, (uint16_t*)pool, slot_stride,
+          slot_elems, B, apron, scratch + 2 * j0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  minmax_finish_kernel<<<(n_jobs + 255) / 256, 256, 0, stream>>>(scratch, jobs, n_jobs,
+                                                                  brick_range);
+  return hipGetLastError();
+}
+
+hipError_t launch_histogram_1d(const void* vol, int bpv, int64_t n, unsigned long long* hist,
+                               hipStream_t stream) {
+  const int64_t nb = bpv == 1 ? 256 : 65536;
+  hipError_t e = launch_fill_u64(hist, nb, 0ull, stream);
+  if (e != hipSuccess) return e;
+  int64_t blocks = (n + kThreads * 16 - 1) / (kThreads * 16);
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  if (bpv == 1)
+    hist1d_u8_kernel<<<(unsigned)blocks, kThreads, 0, stream>>>((const uint8_t*)vol, n, hist);
+  else
+    hist1d_u16_kernel<<<(unsigned)blocks, kThreads, 0, stream>>>((const uint16_t*)vol, n, hist);
+  return hipGetLastError();
+}
+
+hipError_t launch_gradient_kmax(const void* vol, int bpv, const int64_t d[3],
+                                unsigned long long* kmax, hipStream_t stream) {
+  hipError_t e = launch_fill_u64(kmax, 1, 0ull, stream);
+  if (e != hipSuccess) return e;
+  const dim3 grid = row_grid(d[0], d[1] * d[2]);
+  if (bpv == 1)
+    gradient_kmax_kernel<uint8_t><<<grid, kThreads, 0, stream>>>((const uint8_t*)vol, d[0], d[1],
+                                                                 d[2], kmax);
+  else
+    gradient_kmax_kernel<uint16_t><<<grid, kThreads, 0, stream>>>((const uint16_t*)vol, d[0],
+                                                                  d[1], d[2], kmax);
+  return hipGetLastError();
+}
+
+hipError_t launch_histogram_2d(const void* vol, int bpv, const int64_t d[3],
+                               const unsigned long long* kmax, int n_value, int n_grad,
+                               unsigned long long* hist, hipStream_t stream) {
+  hipError_t e = launch_fill_u64(hist, (int64_t)n_value * n_grad, 0ull, stream);
+  if (e != hipSuccess) return e;
+  int shift = 0;
+  while (((int64_t)n_value << shift) < ((int64_t)1 << (8 * bpv))) ++shift;
+  const dim3 grid = row_grid(d[0], d[1] * d[2]);
+  if (bpv == 1)
+    hist2d_kernel<uint8_t><<<grid, kThreads, 0, stream>>>((const uint8_t*)vol, d[0], d[1], d[2],
+                                                          kmax, shift, n_grad, hist);
+  else
+    hist2d_kernel<uint16_t><<<grid, kThreads, 0, stream>>>((const uint16_t*)vol, d[0], d[1], d[2],
+                                                           kmax, shift, n_grad, hist);
+  return hipGetLastError();
+}
+
+}  // namespace tvk

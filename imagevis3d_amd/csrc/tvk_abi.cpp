@@ -1,0 +1,325 @@
+// tvk_abi.cpp -- extern "C" entry points of libtvkhip.so (include/tvk_hip.h).
+// Each function forwards to the AbstrRenderer-shaped HipRenderer; C++
+// exceptions never cross the boundary.
+#include <new>
+#include <string>
+
+#include "../../include/tvk_hip.h"
+#include "errors.hpp"
+#include "renderer.hpp"
+
+namespace tvk {
+namespace {
+thread_local std::string g_last_error;
+}
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace tvk
+
+using tvk::HipRenderer;
+
+struct tvk_renderer {
+  HipRenderer impl;
+  tvk_renderer(int t, bool a, bool b, bool c, bool d) : impl(t, a, b, c, d) {}
+};
+
+#define TVK_GUARD(r)                                  \
+  do {                                                \
+    if (!(r)) {                                       \
+      tvk::set_last_error("null renderer handle");    \
+      return TVK_ERR_ARG;                             \
+    }                                                 \
+  } while (0)
+
+#define TVK_TRY(expr)                                 \
+  try {                                               \
+    return (expr);                                    \
+  } catch (const std::bad_alloc&) {                   \
+    tvk::set_last_error("host allocation failed");    \
+    return TVK_ERR_OOM;                               \
+  } catch (const std::exception& e) {                 \
+    tvk::set_last_error(e.what());                    \
+    return TVK_ERR_ARG;                               \
+  }
+
+extern "C" {
+
+const char* tvk_last_error(void) { return tvk::g_last_error.c_str(); }
+const char* tvk_version(void) { return "tvk-hip 0.1 (gfx950)"; }
+
+int tvk_create(int type, int pow2, int to8, int no_border, int bias_scale, tvk_renderer** out) {
+  if (!out) {
+    tvk::set_last_error("null output handle");
+    return TVK_ERR_ARG;
+  }
+  *out = nullptr;
+  if (type != TVK_OPENGL_RAYCASTER && type != TVK_OPENGL_GRIDLEAPER) {
+    tvk::set_last_error("renderer type not provided by the HIP backend (raycaster or grid leaper only)");
+    return TVK_ERR_ARG;
+  }
+  try {
+    *out = new tvk_renderer(type, pow2 != 0, to8 != 0, no_border != 0, bias_scale != 0);
+  } catch (...) {
+    tvk::set_last_error("host allocation failed");
+    return TVK_ERR_OOM;
+  }
+  return TVK_OK;
+}
+
+int tvk_destroy(tvk_renderer* r) {
+  delete r;
+  return TVK_OK;
+}
+
+int tvk_initialize(tvk_renderer* r, int device) {
+  TVK_GUARD(r);
+  TVK_TRY(r->impl.Initialize(device));
+}
+
+int tvk_resize(tvk_renderer* r, uint32_t w, uint32_t h) {
+  TVK_GUARD(r);
+  TVK_TRY(r->impl.Resize(w, h));
+}
+
+int tvk_set_renderer_target(tvk_renderer* r, int t) {
+  TVK_GUARD(r);
+  return r->impl.SetRendererTarget(t);
+}
+
+int tvk_get_renderer_target(tvk_renderer* r, int* t) {
+  TVK_GUARD(r);
+  if (!t) return TVK_ERR_ARG;
+  *t = r->impl.RendererTarget();
+  return TVK_OK;
+}
+
+int tvk_load_dataset(tvk_renderer* r, const char* path, int verify) {
+  TVK_GUARD(r);
+  TVK_TRY(r->impl.LoadDataset(path, verify != 0));
+}
+
+int tvk_set_volume(tvk_renderer* r, const void* v, uint64_t nx, uint64_t ny, uint64_t nz,
+                   uint32_t bits, uint32_t brick, uint32_t apron, uint32_t n_lod) {
+  TVK_GUARD(r);
+  const uint64_t d[3] = {nx, ny, nz};
+  TVK_TRY(r->impl.SetVolume(v, d, bits, brick, apron, n_lod));
+}
+
+int tvk_get_lod_level_count(tvk_renderer* r, uint32_t* n) {
+  TVK_GUARD(r);
+  if (!n) return TVK_ERR_ARG;
+  *n = (uint32_t)r->impl.LodLevelCount();
+  return TVK_OK;
+}
+
+int tvk_get_bit_width(tvk_renderer* r, uint32_t* bits) {
+  TVK_GUARD(r);
+  if (!bits) return TVK_ERR_ARG;
+  return r->impl.BitWidth(bits);
+}
+
+int tvk_get_domain_size(tvk_renderer* r, uint32_t lod, uint64_t dims[3]) {
+  TVK_GUARD(r);
+  if (!dims) return TVK_ERR_ARG;
+  return r->impl.DomainSize(lod, dims);
+}
+
+int tvk_get_brick_layout(tvk_renderer* r, uint32_t lod, uint32_t grid[3], uint32_t* brick,
+                         uint32_t* apron) {
+  TVK_GUARD(r);
+  if (!grid) return TVK_ERR_ARG;
+  return r->impl.BrickLayout(lod, grid, brick, apron);
+}
+
+int tvk_read_lod(tvk_renderer* r, uint32_t lod, void* out, uint64_t bytes) {
+  TVK_GUARD(r);
+  TVK_TRY(r->impl.ReadLod(lod, out, bytes));
+}
+
+int tvk_read_brick(tvk_renderer* r, uint32_t lod, uint32_t bx, uint32_t by, uint32_t bz, void* out,
+                   uint64_t bytes) {
+  TVK_GUARD(r);
+  TVK_TRY(r->impl.ReadBrick(lod, bx, by, bz, out, bytes));
+}
+
+int tvk_get_brick_minmax(tvk_renderer* r, uint32_t lod, float* mins, float* maxs, uint64_t n) {
+  TVK_GUARD(r);
+  return r->impl.BrickMinMax(lod, mins, maxs, n);
+}
+
+int tvk_compute_histogram_1d(tvk_renderer* r, uint64_t* hist, uint64_t n_bins) {
+  TVK_GUARD(r);
+  TVK_TRY(r->impl.Histogram1D(hist, n_bins));
+}
+
+int tvk_compute_histogram_2d(tvk_renderer* r, uint64_t* hist, uint32_t nv, uint32_t ng,
+                             float* max_grad) {
+  TVK_GUARD(r);
+  TVK_TRY(r->impl.Histogram2D(hist, nv, ng, max_grad));
+}
+
+int tvk_set_render_mode(tvk_renderer* r, int m) {
+  TVK_GUARD(r);
+  return r->impl.SetRenderMode(m);
+}
+
+int tvk_set_1d_trans(tvk_renderer* r, const float* rgba, uint32_t n) {
+  TVK_GUARD(r);
+  TVK_TRY(r->impl.Set1DTrans(rgba, n));
+}
+
+int tvk_set_2d_trans(tvk_renderer* r, const float* rgba, uint32_t w, uint32_t h, float mg) {
+  TVK_GUARD(r);
+  TVK_TRY(r->impl.Set2DTrans(rgba, w, h, mg));
+}
+
+int tvk_set_iso_value(tvk_renderer* r, float iso) {
+  TVK_GUARD(r);
+  return r->impl.SetIsoValue(iso);
+}
+
+int tvk_set_isosurface_color(tvk_renderer* r, float red, float green, float blue) {
+  TVK_GUARD(r);
+  return r->impl.SetIsosurfaceColor(red, green, blue);
+}
+
+int tvk_set_view_pos(tvk_renderer* r, float x, float y, float z) {
+  TVK_GUARD(r);
+  const float v[3] = {x, y, z};
+  return r->impl.SetViewPos(v);
+}
+
+int tvk_set_view_dir(tvk_renderer* r, float x, float y, float z) {
+  TVK_GUARD(r);
+  const float v[3] = {x, y, z};
+  return r->impl.SetViewDir(v);
+}
+
+int tvk_set_up_dir(tvk_renderer* r, float x, float y, float z) {
+  TVK_GUARD(r);
+  const float v[3] = {x, y, z};
+  return r->impl.SetUpDir(v);
+}
+
+int tvk_set_fov(tvk_renderer* r, float fov) {
+  TVK_GUARD(r);
+  return r->impl.SetFoV(fov);
+}
+
+int tvk_set_ortho_view(tvk_renderer* r, int enabled, float hh) {
+  TVK_GUARD(r);
+  return r->impl.SetOrtho(enabled != 0, hh);
+}
+
+int tvk_set_rotation_4x4(tvk_renderer* r, const float m[16]) {
+  TVK_GUARD(r);
+  if (!m) return TVK_ERR_ARG;
+  return r->impl.SetRotation(m);
+}
+
+int tvk_set_translation_4x4(tvk_renderer* r, const float m[16]) {
+  TVK_GUARD(r);
+  if (!m) return TVK_ERR_ARG;
+  return r->impl.SetTranslation(m);
+}
+
+int tvk_set_rescale_factors(tvk_renderer* r, double sx, double sy, double sz) {
+  TVK_GUARD(r);
+  const double s[3] = {sx, sy, sz};
+  return r->impl.SetRescaleFactors(s);
+}
+
+int tvk_set_sample_rate_modifier(tvk_renderer* r, float rate) {
+  TVK_GUARD(r);
+  return r->impl.SetSampleRateModifier(rate);
+}
+
+int tvk_set_interpolant(tvk_renderer* r, int i) {
+  TVK_GUARD(r);
+  return r->impl.SetInterpolant(i);
+}
+
+int tvk_set_lod_limits(tvk_renderer* r, uint32_t mn, uint32_t mx) {
+  TVK_GUARD(r);
+  return r->impl.SetLODLimits(mn, mx);
+}
+
+int tvk_set_bg_colors(tvk_renderer* r, const float top[3], const float bottom[3]) {
+  TVK_GUARD(r);
+  if (!top || !bottom) return TVK_ERR_ARG;
+  return r->impl.SetBGColors(top, bottom);
+}
+
+int tvk_set_lighting_enabled(tvk_renderer* r, int on) {
+  TVK_GUARD(r);
+  return r->impl.SetLightingEnabled(on != 0);
+}
+
+int tvk_set_light_colors(tvk_renderer* r, const float a[4], const float d[4], const float s[4],
+                         const float dir[3]) {
+  TVK_GUARD(r);
+  if (!a || !d || !s || !dir) return TVK_ERR_ARG;
+  return r->impl.SetLightColors(a, d, s, dir);
+}
+
+int tvk_set_early_ray_termination(tvk_renderer* r, float a) {
+  TVK_GUARD(r);
+  return r->impl.SetEarlyRayTermination(a);
+}
+
+int tvk_set_empty_space_skipping(tvk_renderer* r, int on) {
+  TVK_GUARD(r);
+  return r->impl.SetEmptySpaceSkipping(on != 0);
+}
+
+int tvk_paint(tvk_renderer* r) {
+  TVK_GUARD(r);
+  TVK_TRY(r->impl.Paint());
+}
+
+int tvk_check_for_redraw(tvk_renderer* r) {
+  if (!r) return 0;
+  return r->impl.CheckForRedraw();
+}
+
+int tvk_schedule_complete_redraw(tvk_renderer* r) {
+  TVK_GUARD(r);
+  return r->impl.ScheduleCompleteRedraw();
+}
+
+int tvk_capture_single_frame(tvk_renderer* r, uint8_t* rgba, uint64_t bytes, int preserve) {
+  TVK_GUARD(r);
+  TVK_TRY(r->impl.Capture(rgba, bytes, preserve != 0));
+}
+
+int tvk_get_progress(tvk_renderer* r, uint64_t* sf, uint32_t* wsf, uint32_t* bc, uint32_t* wb,
+                     uint64_t* ml) {
+  TVK_GUARD(r);
+  return r->impl.Progress(sf, wsf, bc, wb, ml);
+}
+
+int tvk_get_frame_stats(tvk_renderer* r, tvk_frame_stats* out) {
+  TVK_GUARD(r);
+  return r->impl.FrameStats(out);
+}
+
+int tvk_get_device_framebuffer(tvk_renderer* r, void** dptr, void** stream) {
+  TVK_GUARD(r);
+  return r->impl.DeviceFramebuffer(dptr, stream);
+}
+
+int tvk_synchronize(tvk_renderer* r) {
+  TVK_GUARD(r);
+  return r->impl.Synchronize();
+}
+
+int tvk_set_max_usable_gpu_mem(tvk_renderer* r, uint64_t bytes) {
+  TVK_GUARD(r);
+  return r->impl.SetMaxUsableGPUMem(bytes);
+}
+
+int tvk_get_mem_stats(tvk_renderer* r, tvk_mem_stats* out) {
+  TVK_GUARD(r);
+  return r->impl.MemStats(out);
+}
+
+}  // extern "C"

@@ -1,0 +1,73 @@
+// tvk_common.hpp -- plain structs shared by the host renderer and the HIP
+// kernels of the MI355X Tuvok backend.
+//
+// Device data layout (HBM):
+//   brick pool   one allocation of fixed slots; a slot holds one brick of
+//                payload B^3 plus an apron of `apron` voxels on every side,
+//                (B+2a)^3 voxels, x fastest, bpv bytes per voxel.  Slots are
+//                256-byte aligned.
+//   page table   int32 per brick of every LOD (LOD-major, then z, y, x):
+//                slot index, or -1 when the brick is not resident.
+//   brick ranges float2 (min, max) per brick over its stored slot.
+//   TF           n_tf float4 (RGBA) + int32 prefix count of alpha > 0.
+//   framebuffer  W*H uint32 RGBA8 (r in the low byte), row 0 at the top.
+#pragma once
+#include <stdint.h>
+
+namespace tvk {
+
+constexpr int kMaxLod = 16;
+constexpr int kTile = 8;  // ray tile edge: one 64-lane wavefront = 8x8 rays
+
+struct LodInfo {
+  int32_t dims[3];      // voxels of this level
+  int32_t grid[3];      // bricks of this level
+  int32_t brick_base;   // index of the level's first brick in the page table
+  float to_level[3];    // level texel coords per LOD-0 voxel (N_l / N_0)
+};
+
+// Arguments of one raycast launch (passed by value as kernel arguments).
+struct RaycastParams {
+  // Camera in LOD-0 voxel space.  Perspective: ray origin `eye`, direction
+  // normalize(dir00 + x*du + y*dv) for pixel (x, y).  Orthographic: origin
+  // eye + x*du + y*dv, direction odir.  Pixel-centre offsets are folded in.
+  float eye[3];
+  float dir00[3];
+  float du[3];
+  float dv[3];
+  float odir[3];
+  int32_t ortho;
+  int32_t width, height;
+  int32_t tiles_x, tiles_y;
+  float box[3];         // LOD-0 dims (the volume box is [0, box])
+  float step;           // sample spacing in LOD-0 voxels (2^lod / rate)
+  float nominal_rate;   // user rate: nominal samples per LOD-0 voxel
+  float term_alpha;     // early ray termination threshold
+  int32_t interp;       // 0 nearest, 1 trilinear
+  int32_t skip;         // empty-space skipping on/off
+  LodInfo level;        // the level being sampled
+  int32_t brick;        // payload edge B
+  int32_t apron;        // apron a
+  int32_t slot_edge;    // S = B + 2a
+  int64_t slot_elems;   // slot stride in voxels (S^3 rounded up to 256 bytes)
+  int32_t n_tf;
+  float tf_scale;
+  float bg_top[3], bg_bottom[3];
+  float bg_alpha;
+  const void* pool;
+  const int32_t* page_table;
+  const float* brick_range;     // 2 floats per brick
+  const float* tf;              // 4 floats per entry
+  const int32_t* tf_prefix;     // n_tf + 1
+  uint32_t* framebuffer;
+  unsigned long long* counters; // [0] evaluated samples, [1] nominal samples
+};
+
+// One brick to cut out of a flat level into a pool slot.
+struct BrickJob {
+  int32_t bx, by, bz;
+  int32_t brick_id;  // global brick index (page-table index)
+  int64_t slot;      // destination slot
+};
+
+}  // namespace tvk

@@ -1,0 +1,426 @@
+"""Python mirror of Tuvok's Lua renderer interface over the HIP backend.
+
+``tuvok.renderer.new(...)`` returns a :class:`Renderer` whose methods carry
+the Lua names the ImageVis3D tree calls (``LuaScripts/BatchRenderer.lua``,
+``ImageVis3D/UI/RenderWindow.cpp``), so a batch script reads like
+BatchRenderer.lua:
+
+    r = tuvok.renderer.new(tuvok.renderer.types.OpenGL_Raycaster,
+                           False, False, False, False, False)
+    r.loadDataset("c60.uvf")
+    ctx = tuvok.createContext(640, 480, 32, 24, 8, True, False)
+    r.initialize(ctx); r.resize((640, 480))
+    r.setRendererTarget(tuvok.renderer.types.RT_Headless)
+    r.paint()
+    r.captureSingleFrame("render.png", True)
+    r.cleanup(); deleteClass(r)
+
+Error behaviour follows the reference: ``paint``, ``loadDataset`` and
+``initialize`` return ``False`` on failure (the message is in
+``lastError()``), setters raise :class:`TvkError` (Tuvok's LuaError).
+Every call goes through the C ABI in ``include/tvk_hip.h``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import FrameStats, MemStats, TvkError, call
+
+
+class _Types:
+    """tuvok.renderer.types (SettingsDlg.cpp:486-517, DebugScriptWindow.cpp:219-221)."""
+    OpenGL_SBVR = 0
+    OpenGL_Raycaster = 1
+    DirectX_SBVR = 2
+    DirectX_Raycaster = 3
+    OpenGL_2DSBVR = 4
+    DirectX_2DSBVR = 5
+    OpenGL_GridLeaper = 6
+    DirectX_GridLeaper = 7
+    RT_Interactive = 0
+    RT_Capture = 1
+    RT_Headless = 2
+
+
+class RenderMode:
+    """AbstrRenderer::ERenderMode (RenderWindow.cpp:2051-2059)."""
+    RM_1DTRANS = 0
+    RM_2DTRANS = 1
+    RM_ISOSURFACE = 2
+
+
+class Interpolant:
+    NearestNeighbor = 0
+    Linear = 1
+
+
+class Context:
+    """What tuvok.createContext returns: the HIP device the renderer uses
+    (BatchRenderer/main.cpp:49-65 makes a GL context instead)."""
+
+    def __init__(self, width: int, height: int, device: int = 0):
+        self.width, self.height, self.device = width, height, device
+
+
+def _f4(v: Sequence[float]):
+    a = np.ascontiguousarray(np.asarray(v, np.float32).reshape(-1))
+    return a, a.ctypes.data
+
+
+def _mat(m) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(m, np.float32).reshape(16))
+    return a
+
+
+class Dataset:
+    """The dataset proxy of ``renderer.getDataset()`` (ImageVis3D_Progress.cpp:104-117)."""
+
+    def __init__(self, r: "Renderer", path: str | None):
+        self._r, self._path = r, path
+
+    def getLODLevelCount(self) -> int:
+        n = C.c_uint32()
+        call("tvk_get_lod_level_count", self._r._h, C.byref(n))
+        return n.value
+
+    def getDomainSize(self, lod: int = 0, ts: int = 0):
+        d = (C.c_uint64 * 3)()
+        call("tvk_get_domain_size", self._r._h, lod, d)
+        return tuple(int(x) for x in d)
+
+    def getBitWidth(self) -> int:
+        return self._r._bit_width()
+
+    def getBrickLayout(self, lod: int):
+        g = (C.c_uint32 * 3)()
+        b, a = C.c_uint32(), C.c_uint32()
+        call("tvk_get_brick_layout", self._r._h, lod, g, C.byref(b), C.byref(a))
+        return tuple(int(x) for x in g), b.value, a.value
+
+    def fullpath(self) -> str | None:
+        return self._path
+
+
+class Renderer:
+    """One ``tuvok.renderer`` instance (AbstrRenderer) on the HIP backend."""
+
+    def __init__(self, rtype: int, pow2: bool = False, to8bit: bool = False,
+                 disable_border: bool = False, bias_scale_tf: bool = False, *_extra):
+        h = C.c_void_p()
+        call("tvk_create", int(rtype), int(pow2), int(to8bit), int(disable_border),
+             int(bias_scale_tf), C.byref(h))
+        self._h = h
+        self._pending_dataset: tuple | None = None
+        self._path = None
+        self._bits = 0
+        self._size = (0, 0)
+        self.lastErrorMessage = ""
+
+    # ------------------------------------------------------------ lifecycle
+    def _bool(self, name, *args) -> bool:
+        rc = getattr(_lib.load(), name)(self._h, *args)
+        if rc < 0:
+            self.lastErrorMessage = _lib.last_error()
+            return False
+        return True
+
+    def lastError(self) -> str:
+        return self.lastErrorMessage
+
+    def addShaderPath(self, path: str) -> None:
+        """No shaders: the raycaster is compiled HIP code (kept for script compatibility)."""
+
+    def loadDataset(self, path: str, verify: bool = True) -> bool:
+        """Before initialize() the load is deferred, as BatchRenderer.lua does it."""
+        self._path = path
+        if not self._initialized():
+            self._pending_dataset = ("uvf", path, verify)
+            return True
+        ok = self._bool("tvk_load_dataset", path.encode(), int(verify))
+        if ok:
+            self._bits = self._bit_width()
+        return ok
+
+    def setVolume(self, vol: np.ndarray, brick_size: int, apron: int = 2, n_lod: int = 0) -> None:
+        """Flat (nz, ny, nx) uint8/uint16 volume -> bricked device pool + LOD pyramid."""
+        vol = np.ascontiguousarray(vol)
+        bits = {np.dtype(np.uint8): 8, np.dtype(np.uint16): 16}[vol.dtype]
+        nz, ny, nx = vol.shape
+        call("tvk_set_volume", self._h, vol.ctypes.data, nx, ny, nz, bits, brick_size, apron, n_lod)
+        self._bits = bits
+
+    def _bit_width(self) -> int:
+        b = C.c_uint32()
+        call("tvk_get_bit_width", self._h, C.byref(b))
+        return b.value
+
+    def _initialized(self) -> bool:
+        return getattr(self, "_init", False)
+
+    def initialize(self, context: Context | int = 0) -> bool:
+        dev = context.device if isinstance(context, Context) else int(context)
+        ok = self._bool("tvk_initialize", dev)
+        self._init = ok
+        if ok and self._pending_dataset is not None:
+            _, path, verify = self._pending_dataset
+            self._pending_dataset = None
+            ok = self._bool("tvk_load_dataset", path.encode(), int(verify))
+            if ok:
+                self._bits = self._bit_width()
+        if ok and isinstance(context, Context):
+            self.resize((context.width, context.height))
+        return ok
+
+    def resize(self, size: Sequence[int]) -> None:
+        w, h = int(size[0]), int(size[1])
+        call("tvk_resize", self._h, w, h)
+        self._size = (w, h)
+
+    def setRendererTarget(self, t: int) -> None:
+        call("tvk_set_renderer_target", self._h, int(t))
+
+    def getRendererTarget(self) -> int:
+        t = C.c_int()
+        call("tvk_get_renderer_target", self._h, C.byref(t))
+        return t.value
+
+    def cleanup(self) -> None:
+        if self._h:
+            _lib.load().tvk_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.cleanup()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ rendering
+    def paint(self) -> bool:
+        return self._bool("tvk_paint")
+
+    def checkForRedraw(self) -> bool:
+        return bool(_lib.load().tvk_check_for_redraw(self._h))
+
+    def scheduleCompleteRedraw(self) -> None:
+        call("tvk_schedule_complete_redraw", self._h)
+
+    def capture(self, preserve_transparency: bool = False) -> np.ndarray:
+        """Finish the frame and return it as an (H, W, 4) uint8 array."""
+        w, h = self._size
+        out = np.empty((h, w, 4), np.uint8)
+        call("tvk_capture_single_frame", self._h, out.ctypes.data, out.nbytes,
+             int(preserve_transparency))
+        return out
+
+    def captureSingleFrame(self, filename: str, preserve_transparency: bool = False) -> bool:
+        try:
+            img = self.capture(preserve_transparency)
+        except TvkError as e:
+            self.lastErrorMessage = str(e)
+            return False
+        try:
+            from PIL import Image
+            Image.fromarray(img, "RGBA").save(filename)
+        except ImportError:
+            img.tofile(filename)
+        return True
+
+    def synchronize(self) -> None:
+        call("tvk_synchronize", self._h)
+
+    def frameStats(self) -> FrameStats:
+        s = FrameStats()
+        call("tvk_get_frame_stats", self._h, C.byref(s))
+        return s
+
+    def deviceFramebuffer(self):
+        p, s = C.c_void_p(), C.c_void_p()
+        call("tvk_get_device_framebuffer", self._h, C.byref(p), C.byref(s))
+        return p.value, s.value
+
+    # progress counters (RenderWindow.cpp:453-486)
+    def _progress(self):
+        sf, wsf, bc, wb, ml = C.c_uint64(), C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint64()
+        call("tvk_get_progress", self._h, C.byref(sf), C.byref(wsf), C.byref(bc), C.byref(wb),
+             C.byref(ml))
+        return sf.value, wsf.value, bc.value, wb.value, ml.value
+
+    def getCurrentSubFrameCount(self) -> int:
+        return self._progress()[0]
+
+    def getWorkingSubFrame(self) -> int:
+        return self._progress()[1]
+
+    def getCurrentBrickCount(self) -> int:
+        return self._progress()[2]
+
+    def getWorkingBrick(self) -> int:
+        return self._progress()[3]
+
+    def getMinLODIndex(self) -> int:
+        return self._progress()[4]
+
+    # ------------------------------------------------------- mode and TFs
+    def setRenderMode(self, mode: int) -> None:
+        call("tvk_set_render_mode", self._h, int(mode))
+
+    def set1DTrans(self, rgba: np.ndarray) -> None:
+        """Upload a 1D TF (n x RGBA floats): TransferFunction1D + tuvok.gpu.changed1DTrans."""
+        a = np.ascontiguousarray(np.asarray(rgba, np.float32).reshape(-1, 4))
+        call("tvk_set_1d_trans", self._h, a.ctypes.data, a.shape[0])
+
+    def set2DTrans(self, rgba: np.ndarray, max_grad: float) -> None:
+        a = np.ascontiguousarray(np.asarray(rgba, np.float32))
+        h, w = a.shape[:2]
+        call("tvk_set_2d_trans", self._h, a.ctypes.data, w, h, float(max_grad))
+
+    def setIsoValue(self, v: float) -> None:
+        call("tvk_set_iso_value", self._h, float(v))
+
+    def setIsosurfaceColor(self, c: Sequence[float]) -> None:
+        call("tvk_set_isosurface_color", self._h, *[float(x) for x in c[:3]])
+
+    # --------------------------------------------------------------- camera
+    def setViewPos(self, v: Sequence[float]) -> None:
+        call("tvk_set_view_pos", self._h, *[float(x) for x in v])
+
+    def setViewDir(self, v: Sequence[float]) -> None:
+        call("tvk_set_view_dir", self._h, *[float(x) for x in v])
+
+    def setUpDir(self, v: Sequence[float]) -> None:
+        call("tvk_set_up_dir", self._h, *[float(x) for x in v])
+
+    def setFoV(self, deg: float) -> None:
+        call("tvk_set_fov", self._h, float(deg))
+
+    def setOrthoViewEnabled(self, on: bool, half_height: float = 0.5) -> None:
+        call("tvk_set_ortho_view", self._h, int(on), float(half_height))
+
+    def setRotation4x4(self, m) -> None:
+        a = _mat(m)
+        call("tvk_set_rotation_4x4", self._h, a.ctypes.data)
+
+    def setTranslation4x4(self, m) -> None:
+        a = _mat(m)
+        call("tvk_set_translation_4x4", self._h, a.ctypes.data)
+
+    def setRescaleFactors(self, s: Sequence[float]) -> None:
+        call("tvk_set_rescale_factors", self._h, *[float(x) for x in s])
+
+    # ------------------------------------------------------ quality and LOD
+    def setSampleRateModifier(self, r: float) -> None:
+        call("tvk_set_sample_rate_modifier", self._h, float(r))
+
+    def setInterpolant(self, i: int) -> None:
+        call("tvk_set_interpolant", self._h, int(i))
+
+    def setLODLimits(self, lim: Sequence[int]) -> None:
+        call("tvk_set_lod_limits", self._h, int(lim[0]), int(lim[1]))
+
+    def setBGColors(self, top: Sequence[float], bottom: Sequence[float]) -> None:
+        t, tp = _f4(top)
+        b, bp = _f4(bottom)
+        call("tvk_set_bg_colors", self._h, tp, bp)
+
+    def setLightingEnabled(self, on: bool) -> None:
+        call("tvk_set_lighting_enabled", self._h, int(on))
+
+    def setLightColors(self, ambient, diffuse, specular, light_dir) -> None:
+        a, ap = _f4(ambient)
+        d, dp = _f4(diffuse)
+        s, sp = _f4(specular)
+        l, lp = _f4(light_dir)
+        call("tvk_set_light_colors", self._h, ap, dp, sp, lp)
+
+    def setEarlyRayTermination(self, alpha: float) -> None:
+        call("tvk_set_early_ray_termination", self._h, float(alpha))
+
+    def setEmptySpaceSkipping(self, on: bool) -> None:
+        call("tvk_set_empty_space_skipping", self._h, int(on))
+
+    # --------------------------------------------------------------- data
+    def getDataset(self) -> Dataset:
+        return Dataset(self, self._path)
+
+    def readLOD(self, lod: int) -> np.ndarray:
+        nx, ny, nz = self.getDataset().getDomainSize(lod)
+        dt = np.uint8 if self._bits == 8 else np.uint16
+        out = np.empty((nz, ny, nx), dt)
+        call("tvk_read_lod", self._h, lod, out.ctypes.data, out.nbytes)
+        return out
+
+    def readBrick(self, lod: int, bx: int, by: int, bz: int) -> np.ndarray:
+        _, B, a = self.getDataset().getBrickLayout(lod)
+        S = B + 2 * a
+        dt = np.uint8 if self._bits == 8 else np.uint16
+        out = np.empty((S, S, S), dt)
+        call("tvk_read_brick", self._h, lod, bx, by, bz, out.ctypes.data, out.nbytes)
+        return out
+
+    def brickMinMax(self, lod: int):
+        g, _, _ = self.getDataset().getBrickLayout(lod)
+        n = g[0] * g[1] * g[2]
+        mn = np.empty(n, np.float32)
+        mx = np.empty(n, np.float32)
+        call("tvk_get_brick_minmax", self._h, lod, mn.ctypes.data, mx.ctypes.data, n)
+        return mn.reshape(g[2], g[1], g[0]), mx.reshape(g[2], g[1], g[0])
+
+    def histogram1D(self) -> np.ndarray:
+        h = np.empty(1 << self._bits, np.uint64)
+        call("tvk_compute_histogram_1d", self._h, h.ctypes.data, h.size)
+        return h
+
+    def histogram2D(self, n_value: int = 256, n_grad: int = 256):
+        h = np.empty((n_value, n_grad), np.uint64)
+        mg = C.c_float()
+        call("tvk_compute_histogram_2d", self._h, h.ctypes.data, n_value, n_grad, C.byref(mg))
+        return h, mg.value
+
+    # ------------------------------------------------------------- memory
+    def setMaxUsableGPUMem(self, nbytes: int) -> None:
+        call("tvk_set_max_usable_gpu_mem", self._h, int(nbytes))
+
+    def memStats(self) -> MemStats:
+        s = MemStats()
+        call("tvk_get_mem_stats", self._h, C.byref(s))
+        return s
+
+
+class _RendererNS:
+    types = _Types
+
+    @staticmethod
+    def new(rtype: int, *flags) -> Renderer:
+        return Renderer(rtype, *flags)
+
+
+class tuvok:  # noqa: N801 -- the Lua namespace name
+    """The `tuvok.*` names BatchRenderer.lua uses."""
+    renderer = _RendererNS
+
+    @staticmethod
+    def createContext(width: int, height: int, color_bits: int = 32, depth_bits: int = 24,
+                      stencil_bits: int = 8, double_buffer: bool = True, visible: bool = False,
+                      device: int = 0) -> Context:
+        return Context(width, height, device)
+
+
+def deleteClass(r: Renderer) -> None:  # noqa: N802 -- Lua name
+    r.cleanup()
+
+
+def generate_volume(kind: int, seed: int, dims: Sequence[int], bits: int) -> np.ndarray:
+    """Seeded synthetic volume from the library's host generators, (nz, ny, nx)."""
+    nx, ny, nz = (int(d) for d in dims)
+    out = np.empty((nz, ny, nx), np.uint8 if bits == 8 else np.uint16)
+    call("tvk_generate_volume", int(kind), int(seed), nx, ny, nz, int(bits), out.ctypes.data,
+         out.nbytes)
+    return out
+
+
+VOL_CONST, VOL_RANDOM, VOL_SPHERE, VOL_BLOBS, VOL_PERLIN_SPHERES = 0, 1, 2, 3, 4

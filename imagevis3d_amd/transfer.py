@@ -1,0 +1,97 @@
+"""Transfer functions: the reference's .1dt/.2dt files and the seeded TFs of
+the benchmark workloads.
+
+``.1dt`` (DemoData/c60.1dt): first line N, then N lines ``r g b a``
+(TransferFunction1D, Q1DTransferFunction.cpp:573-589).  ``.2dt``
+(DemoData/c60.2dt): ``W H``, an embedded 1D TF, then a swatch count and
+swatches (TransferFunction2D, Q2DTransferFunction.cpp:426-470).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def load_1dt(path: str) -> np.ndarray:
+    """Read a .1dt file -> (N, 4) float32 RGBA."""
+    with open(path) as f:
+        toks = f.read().split()
+    n = int(toks[0])
+    vals = np.asarray([float(t) for t in toks[1:1 + 4 * n]], np.float32)
+    if vals.size != 4 * n:
+        raise ValueError(f"{path}: expected {n} RGBA entries")
+    return vals.reshape(n, 4)
+
+
+def resample_1d(tf: np.ndarray, size: int) -> np.ndarray:
+    """Linear resampling of a 1D TF to `size` entries (end points kept), the
+    operation loadFromFileWithSize applies to match the histogram size."""
+    tf = np.asarray(tf, np.float32)
+    n = tf.shape[0]
+    if n == size:
+        return tf.copy()
+    x = np.linspace(0.0, n - 1.0, size)
+    out = np.empty((size, 4), np.float32)
+    for c in range(4):
+        out[:, c] = np.interp(x, np.arange(n), tf[:, c])
+    return out
+
+
+def parse_2dt(path: str) -> dict:
+    """Read a .2dt file: raster size, embedded 1D TF and swatches."""
+    with open(path) as f:
+        toks = f.read().split()
+    it = iter(toks)
+    w, h = int(next(it)), int(next(it))
+    n1 = int(next(it))
+    tf1 = np.asarray([float(next(it)) for _ in range(4 * n1)], np.float32).reshape(n1, 4)
+    ns = int(next(it))
+    swatches = []
+    for _ in range(ns):
+        radial = int(next(it))
+        npts = int(next(it))
+        pts = [(float(next(it)), float(next(it))) for _ in range(npts)]
+        grad = tuple(float(next(it)) for _ in range(4))
+        nstops = int(next(it))
+        stops = []
+        for _ in range(nstops):
+            pos = float(next(it))
+            rgba = tuple(float(next(it)) for _ in range(4))
+            stops.append((pos, rgba))
+        swatches.append({"radial": radial, "points": pts, "gradient": grad, "stops": stops})
+    return {"size": (w, h), "tf1d": tf1, "swatches": swatches}
+
+
+def ramp_tf(n: int = 256, lo: float = 40.0, hi: float = 220.0) -> np.ndarray:
+    """Linear alpha ramp from value `lo` (alpha 0) to `hi` (alpha 1); colour
+    ramp r = t, g = t^2, b = 1 - t with t = i / (n - 1)."""
+    i = np.arange(n, dtype=np.float64)
+    t = i / (n - 1)
+    tf = np.empty((n, 4), np.float32)
+    tf[:, 0] = t
+    tf[:, 1] = t * t
+    tf[:, 2] = 1.0 - t
+    tf[:, 3] = np.clip((i - lo) / (hi - lo), 0.0, 1.0)
+    return tf
+
+
+def gaussian_bumps_tf(n: int = 4096, seed: int = 2, bumps: int = 3) -> np.ndarray:
+    """`bumps` seeded Gaussian alpha bumps with compact support (+-3 sigma):
+    centre U[0.15, 0.93]*n, sigma U[0.015, 0.05]*n, peak U[0.02, 0.08];
+    colour = bump colours weighted by the bump profiles."""
+    rng = np.random.default_rng(seed)
+    mu = rng.uniform(0.15, 0.93, bumps) * n
+    sig = rng.uniform(0.015, 0.05, bumps) * n
+    amp = rng.uniform(0.02, 0.08, bumps)
+    col = rng.uniform(0.2, 1.0, (bumps, 3))
+    i = np.arange(n, dtype=np.float64)[:, None]
+    prof = np.exp(-0.5 * ((i - mu) / sig) ** 2)
+    prof[np.abs(i - mu) > 3.0 * sig] = 0.0
+    alpha = np.clip((prof * amp).sum(1), 0.0, 1.0)
+    wsum = prof.sum(1, keepdims=True)
+    rgb = np.where(wsum > 0, (prof @ col) / np.maximum(wsum, 1e-30), 0.0)
+    tf = np.concatenate([rgb, alpha[:, None]], 1).astype(np.float32)
+    return tf
+
+
+def constant_tf(n: int, rgba) -> np.ndarray:
+    return np.tile(np.asarray(rgba, np.float32), (n, 1))

@@ -1,0 +1,115 @@
+"""The benchmark workloads: seeded volume, TF, cameras and render settings.
+
+``small``  -- 64^3 uint8, 60 Gaussian blobs on a sphere + noise, one 64^3
+              brick (apron 2, no LOD), 256-entry ramp TF (alpha 0 at 40 to
+              1 at 220), 512x512 orthographic face-filling view, rate 1.0,
+              early termination at alpha >= 0.99.
+``medium`` -- 512^3 uint16, 5-octave Perlin fBm in [0,4095] + 20 spheres,
+              256^3 bricks with a 2-voxel apron, 3 LOD levels, 4096-entry TF
+              with three seeded Gaussian bumps, 1920x1080 perspective (60
+              degree vertical FoV), 36-view orbit (10 degree azimuth steps,
+              20 degree elevation, bounding sphere filling the FoV), rate 1.5,
+              min/max empty-space skipping.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import transfer
+from .renderer import VOL_BLOBS, VOL_PERLIN_SPHERES, generate_volume
+
+
+@dataclass
+class Camera:
+    view_pos: tuple
+    view_dir: tuple
+    up_dir: tuple = (0.0, 1.0, 0.0)
+    fov: float = 60.0
+    ortho: bool = False
+    ortho_half_height: float = 0.5
+
+
+@dataclass
+class Workload:
+    name: str
+    dims: tuple            # (nx, ny, nz)
+    bits: int
+    volume_kind: int
+    seed: int
+    brick: int
+    apron: int
+    n_lod: int
+    tf: np.ndarray
+    width: int
+    height: int
+    rate: float
+    cameras: list = field(default_factory=list)
+    term_alpha: float = 0.99
+    skip: bool = True
+    bg_top: tuple = (0.0, 0.0, 0.0)
+    bg_bottom: tuple = (0.0, 0.0, 0.0)
+
+    def volume(self) -> np.ndarray:
+        return generate_volume(self.volume_kind, self.seed, self.dims, self.bits)
+
+
+def orbit_cameras(n_views: int = 36, fov: float = 60.0, elevation_deg: float = 20.0,
+                  extent=(1.0, 1.0, 1.0)) -> list:
+    """Cameras on a circle around the volume centre, looking at it, at the
+    distance where the bounding sphere exactly fills the vertical FoV."""
+    r = 0.5 * math.sqrt(sum(e * e for e in extent))
+    d = r / math.sin(math.radians(fov) / 2.0)
+    el = math.radians(elevation_deg)
+    cams = []
+    for k in range(n_views):
+        az = math.radians(10.0 * k * 36.0 / n_views)
+        pos = (d * math.cos(el) * math.sin(az), d * math.sin(el), d * math.cos(el) * math.cos(az))
+        cams.append(Camera(pos, tuple(-p for p in pos), (0.0, 1.0, 0.0), fov))
+    return cams
+
+
+def small(seed: int = 1) -> Workload:
+    return Workload(
+        name="small-64^3-u8-ortho512", dims=(64, 64, 64), bits=8, volume_kind=VOL_BLOBS,
+        seed=seed, brick=64, apron=2, n_lod=1, tf=transfer.ramp_tf(256, 40, 220),
+        width=512, height=512, rate=1.0,
+        cameras=[Camera((0.0, 0.0, 1.5), (0.0, 0.0, -1.0), (0.0, 1.0, 0.0), 60.0, True, 0.5)])
+
+
+def medium(seed: int = 2, n_views: int = 36, width: int = 1920, height: int = 1080,
+           dims=(512, 512, 512)) -> Workload:
+    return Workload(
+        name="medium-512^3-u16-persp1080p-36views", dims=tuple(dims), bits=16,
+        volume_kind=VOL_PERLIN_SPHERES, seed=seed, brick=256, apron=2, n_lod=3,
+        tf=transfer.gaussian_bumps_tf(4096, seed, 3), width=width, height=height, rate=1.5,
+        cameras=orbit_cameras(n_views, 60.0, 20.0))
+
+
+def apply(r, w: Workload, cam: Camera) -> None:
+    """Push a workload's settings and one camera into a renderer."""
+    r.setSampleRateModifier(w.rate)
+    r.setEarlyRayTermination(w.term_alpha)
+    r.setEmptySpaceSkipping(w.skip)
+    r.setBGColors(w.bg_top, w.bg_bottom)
+    set_camera(r, cam)
+
+
+def set_camera(r, cam: Camera) -> None:
+    r.setViewPos(cam.view_pos)
+    r.setViewDir(cam.view_dir)
+    r.setUpDir(cam.up_dir)
+    r.setFoV(cam.fov)
+    r.setOrthoViewEnabled(cam.ortho, cam.ortho_half_height)
+
+
+def oracle_kwargs(w: Workload, cam: Camera, lod: int = 0, level_dims=None) -> dict:
+    """Keyword arguments of oracle.make_params for the same render."""
+    return dict(dims0=w.dims, lod=lod, view_pos=cam.view_pos, view_dir=cam.view_dir,
+                up_dir=cam.up_dir, fov=cam.fov, ortho=cam.ortho,
+                ortho_half_height=cam.ortho_half_height, width=w.width, height=w.height,
+                rate=w.rate, term_alpha=w.term_alpha, interp=1, bg_top=w.bg_top,
+                bg_bottom=w.bg_bottom, bg_alpha=1.0,
+                brick_size=w.brick if w.skip else 0, apron=w.apron)

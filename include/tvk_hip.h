@@ -1,0 +1,277 @@
+/*
+ * tvk_hip.h -- C ABI of the MI355X (gfx950) renderer backend for Tuvok.
+ *
+ * This is the drop-in boundary for Tuvok's raycasting + brick/LOD hot path.
+ * Every entry point replaces a method of Tuvok's AbstrRenderer (reached from
+ * Lua through `cexec("<renderer>.<method>")`) or of the IO/UVF layer that
+ * feeds it.  The reference interface each one stands in for is cited as
+ * `file:line` of the ImageVis3D tree (Tuvok itself is an un-vendored
+ * submodule, so its call sites are the evidence of the interface).
+ *
+ * Conventions
+ *  - Plain C: opaque handle, int status (TVK_OK == 0, negative on error),
+ *    caller-owned output buffers, no C++ or torch types.
+ *  - One handle == one AbstrRenderer instance == one HIP stream on one
+ *    device.  A handle is thread-compatible (not thread-safe), like the
+ *    single-threaded Tuvok renderer (ImageVis3D/UI/ImageVis3D.cpp:976-983).
+ *  - tvk_last_error() returns a message for the last failed call on this
+ *    thread (the log-macro channel of Tuvok's T_ERROR).
+ *  - Matrices are float[16], row-major, applied to row vectors
+ *    (p' = p * M, translation in elements 12..14), which is the layout of
+ *    Tuvok's FLOATMATRIX4 passed by setRotation4x4/setTranslation4x4.
+ *  - World space: the volume's bounding box is centred at the origin and
+ *    scaled so its largest extent (dims * rescale factors) is 1.  This is
+ *    the space of setViewPos/setViewDir/setUpDir.
+ *  - Images are RGBA8, row 0 at the top, W*H*4 bytes.
+ */
+#ifndef TVK_HIP_H
+#define TVK_HIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+enum tvk_status {
+  TVK_OK = 0,
+  TVK_ERR_ARG = -1,     /* invalid argument (bool false / LuaError)        */
+  TVK_ERR_HIP = -2,     /* a HIP runtime call failed                       */
+  TVK_ERR_STATE = -3,   /* call out of order (e.g. paint before a volume)  */
+  TVK_ERR_IO = -4,      /* file could not be read                          */
+  TVK_ERR_OOM = -5,     /* brick pool / device memory budget exhausted     */
+  TVK_ERR_FORMAT = -6,  /* malformed or unsupported UVF / TF file          */
+  TVK_ERR_CHECKSUM = -7 /* UVF checksum mismatch (UVFReader/main.cpp:151)  */
+};
+
+/* EVolumeRendererType (ImageVis3D/UI/SettingsDlg.cpp:486-517,
+ * RenderWindowGL.cpp:176-188).  This backend implements the two raycasters;
+ * both map to the HIP raycaster, GRIDLEAPER additionally pages bricks on
+ * demand from the host (out-of-core). */
+enum tvk_renderer_type {
+  TVK_OPENGL_SBVR = 0,
+  TVK_OPENGL_RAYCASTER = 1,
+  TVK_DIRECTX_SBVR = 2,
+  TVK_DIRECTX_RAYCASTER = 3,
+  TVK_OPENGL_2DSBVR = 4,
+  TVK_DIRECTX_2DSBVR = 5,
+  TVK_OPENGL_GRIDLEAPER = 6,
+  TVK_DIRECTX_GRIDLEAPER = 7
+};
+
+/* AbstrRenderer::ERenderMode (RenderWindow.cpp:2051-2059). */
+enum tvk_render_mode { TVK_RM_1DTRANS = 0, TVK_RM_2DTRANS = 1, TVK_RM_ISOSURFACE = 2 };
+
+/* AbstrRenderer::ERendererTarget (DebugScriptWindow.cpp:219-221,
+ * LuaScripts/BatchRenderer.lua:34,37). */
+enum tvk_target { TVK_RT_INTERACTIVE = 0, TVK_RT_CAPTURE = 1, TVK_RT_HEADLESS = 2 };
+
+/* Interpolant (RenderWindow.cpp:437-446). */
+enum tvk_interpolant { TVK_NEAREST = 0, TVK_LINEAR = 1 };
+
+typedef struct tvk_renderer tvk_renderer;
+
+/* Per-paint statistics (the progress counters of RenderWindow.cpp:453-486
+ * plus the measurement counters of this backend). */
+typedef struct tvk_frame_stats {
+  uint64_t evaluated_samples; /* samples fetched+composited (after skipping) */
+  uint64_t nominal_samples;   /* sum over pixels of floor(chord * rate) on LOD 0 */
+  uint32_t bricks_total;      /* bricks of the rendered LOD                  */
+  uint32_t bricks_skipped;    /* bricks empty under the current TF           */
+  uint32_t lod;               /* LOD index rendered by this paint            */
+  uint32_t width, height;
+  float kernel_ms;            /* raycast kernel time (HIP events)            */
+  float paint_ms;             /* whole paint() on the device (HIP events)    */
+} tvk_frame_stats;
+
+/* Brick-pool / GPUMemMan accounting (ImageVis3D_DebugWindow.cpp:172-206). */
+typedef struct tvk_mem_stats {
+  uint64_t gpu_mem;            /* device memory total (SysInfo GetGPUMem)     */
+  uint64_t gpu_budget;         /* SetMaxUsableGPUMem                          */
+  uint64_t allocated_gpu_mem;  /* bytes held by this renderer's pool + FB     */
+  uint64_t pool_slots;         /* brick slots in the pool                     */
+  uint64_t resident_bricks;    /* slots holding a brick                       */
+  uint64_t slot_bytes;         /* bytes per slot ((B+2*apron)^3 * bpv)        */
+  uint64_t bricks_uploaded;    /* host->device brick uploads since create     */
+  uint64_t bricks_evicted;     /* LRU evictions since create                  */
+} tvk_mem_stats;
+
+/* --------------------------------------------------------------- errors */
+const char* tvk_last_error(void);
+const char* tvk_version(void);
+
+/* -------------------------------------------------------------- lifecycle */
+/* tuvok.renderer.new(type, useOnlyPowerOfTwo, downSampleTo8Bits,
+ * disableBorder, biasAndScaleTF) -- RenderWindowGL.cpp:97-100,
+ * LuaScripts/BatchRenderer.lua:18-22.  Only TVK_OPENGL_RAYCASTER and
+ * TVK_OPENGL_GRIDLEAPER are accepted; other types return TVK_ERR_ARG. */
+int tvk_create(int renderer_type, int use_only_pow2, int downsample_to_8bits,
+               int disable_border, int bias_and_scale_tf, tvk_renderer** out);
+/* renderer.cleanup() + deleteClass (RenderWindow.cpp:1590-1602). */
+int tvk_destroy(tvk_renderer* r);
+/* renderer.initialize(context) -- RenderWindowGL.cpp:361-362.  The context
+ * is a HIP device ordinal; creates the stream, events and framebuffer. */
+int tvk_initialize(tvk_renderer* r, int device);
+/* renderer.resize(UINTVECTOR2) -- RenderWindow.cpp:2446. */
+int tvk_resize(tvk_renderer* r, uint32_t width, uint32_t height);
+/* setRendererTarget/getRendererTarget -- RenderWindow.cpp:151-163. */
+int tvk_set_renderer_target(tvk_renderer* r, int target);
+int tvk_get_renderer_target(tvk_renderer* r, int* target);
+
+/* ------------------------------------------------------------------ data */
+/* renderer.loadDataset(file) -- RenderWindowGL.cpp:111.  Reads a UVF v2
+ * file with an uncompressed RasterDataBlock (doc/import.adoc:904-1102),
+ * verifies its MD5 when `verify` is set (UVFReader/main.cpp:67-68,151),
+ * and builds the device brick pool from it. */
+int tvk_load_dataset(tvk_renderer* r, const char* uvf_path, int verify);
+/* In-memory replacement of loadDataset for a flat scalar volume, x fastest:
+ * bricks it into the device pool with `apron` voxels of overlap per side
+ * and builds `n_lod` levels by 2x2x2 integer mean (RasterDataBlock::
+ * FlatDataToBrickedLOD + CombineAverage, UVFReader/DataSource.h:550-570).
+ * bits is 8 or 16.  n_lod == 0 picks the RasterDataBlock rule
+ * (1 + halvings until the largest dimension fits one brick,
+ * UVFReader/DataSource.h:521-529). */
+int tvk_set_volume(tvk_renderer* r, const void* voxels, uint64_t nx, uint64_t ny,
+                   uint64_t nz, uint32_t bits, uint32_t brick_size, uint32_t apron,
+                   uint32_t n_lod);
+/* dataset.getLODLevelCount / getDomainSize(lod) -- ImageVis3D_Progress.cpp:104-117. */
+int tvk_get_lod_level_count(tvk_renderer* r, uint32_t* n_lod);
+int tvk_get_domain_size(tvk_renderer* r, uint32_t lod, uint64_t dims[3]);
+/* dataset.getBitWidth -- RenderWindow.cpp:2124-2132 (8 or 16). */
+int tvk_get_bit_width(tvk_renderer* r, uint32_t* bits);
+/* dataset.getBrickCount(lod) and the brick grid -- BlockInfo.h:49-93. */
+int tvk_get_brick_layout(tvk_renderer* r, uint32_t lod, uint32_t grid[3],
+                         uint32_t* brick_size, uint32_t* apron);
+/* Read back one LOD level as a flat volume (x fastest), bpv bytes/voxel:
+ * the device pyramid that RasterDataBlock::FlatDataToBrickedLOD builds. */
+int tvk_read_lod(tvk_renderer* r, uint32_t lod, void* out, uint64_t out_bytes);
+/* Read back one stored brick slot including its apron ((B+2a)^3 voxels):
+ * UVFDataset::GetBrick(BrickKey) (doc/import.adoc:761-775). */
+int tvk_read_brick(tvk_renderer* r, uint32_t lod, uint32_t bx, uint32_t by, uint32_t bz,
+                   void* out, uint64_t out_bytes);
+/* Per-brick min/max of one LOD (MaxMinDataBlock, BlockInfo.h:255-271),
+ * over the brick's stored voxels (payload + apron), in brick-grid order. */
+int tvk_get_brick_minmax(tvk_renderer* r, uint32_t lod, float* mins, float* maxs,
+                         uint64_t n_bricks);
+/* Histogram1DDataBlock::Compute over LOD 0 (UVFReader/DataSource.h:595-610):
+ * n_bins = 2^bits (u64 counts). */
+int tvk_compute_histogram_1d(tvk_renderer* r, uint64_t* hist, uint64_t n_bins);
+/* Histogram2DDataBlock::Compute over LOD 0 (DataSource.h:611-636): value x
+ * gradient-magnitude bins, 256 x 256 (value bin = raw value scaled to 256
+ * bins, X slowest), gradient of v/(2^bits-1) by central differences.
+ * max_grad receives the maximum gradient magnitude. */
+int tvk_compute_histogram_2d(tvk_renderer* r, uint64_t* hist, uint32_t n_value_bins,
+                             uint32_t n_grad_bins, float* max_grad);
+
+/* ------------------------------------------------------- mode and TFs */
+/* setRenderMode/getRenderMode -- RenderWindow.cpp:2051-2053. */
+int tvk_set_render_mode(tvk_renderer* r, int mode);
+/* TransferFunction1D colour data (Q1DTransferFunction.cpp:254-258) and
+ * tuvok.gpu.changed1DTrans (Q1DTransferFunction.cpp:465-469): n RGBA floats.
+ * Data value v indexes entry v * (n-1) / (2^bits - 1) when biasAndScaleTF is
+ * set, otherwise entry v (clamped to n-1). */
+int tvk_set_1d_trans(tvk_renderer* r, const float* rgba, uint32_t n);
+/* TransferFunction2D raster (Q2DTransferFunction.cpp:426-470): w (value) x h
+ * (gradient magnitude) RGBA floats, value fastest; max_grad is the gradient
+ * magnitude that maps to the last row. */
+int tvk_set_2d_trans(tvk_renderer* r, const float* rgba, uint32_t w, uint32_t h,
+                     float max_grad);
+/* setIsoValue -- RenderWindow.cpp:2117-2147 (raw data units). */
+int tvk_set_iso_value(tvk_renderer* r, float iso);
+/* setIsosurfaceColor -- RenderWindow.cpp:2329. */
+int tvk_set_isosurface_color(tvk_renderer* r, float red, float green, float blue);
+
+/* ---------------------------------------------------------------- camera */
+/* setViewPos/setViewDir/setUpDir -- RenderWindow.cpp:1033-1082 (world space). */
+int tvk_set_view_pos(tvk_renderer* r, float x, float y, float z);
+int tvk_set_view_dir(tvk_renderer* r, float x, float y, float z);
+int tvk_set_up_dir(tvk_renderer* r, float x, float y, float z);
+/* setFoV -- RenderWindow.cpp:2105-2115 (vertical, degrees). */
+int tvk_set_fov(tvk_renderer* r, float fov_degrees);
+/* setOrthoViewEnabled -- RenderWindow.cpp:1699-1700.  half_height is the
+ * world-space half height of the orthographic view volume. */
+int tvk_set_ortho_view(tvk_renderer* r, int enabled, float half_height);
+/* renderRegion3D setRotation4x4/setTranslation4x4 -- RenderWindow.cpp:166-182. */
+int tvk_set_rotation_4x4(tvk_renderer* r, const float m[16]);
+int tvk_set_translation_4x4(tvk_renderer* r, const float m[16]);
+/* setRescaleFactors -- RenderWindow.cpp:592-599 (voxel aspect). */
+int tvk_set_rescale_factors(tvk_renderer* r, double sx, double sy, double sz);
+
+/* ------------------------------------------------------- quality and LOD */
+/* setSampleRateModifier -- RenderWindow.cpp:2090-2103: samples per voxel of
+ * the LOD being rendered. */
+int tvk_set_sample_rate_modifier(tvk_renderer* r, float rate);
+/* setInterpolant -- RenderWindow.cpp:437-440. */
+int tvk_set_interpolant(tvk_renderer* r, int interpolant);
+/* setLODLimits -- RenderWindow.cpp:447-451: finest (min) and coarsest (max)
+ * LOD index a paint may use (0 = full resolution). */
+int tvk_set_lod_limits(tvk_renderer* r, uint32_t min_lod, uint32_t max_lod);
+/* setBGColors -- RenderWindow.cpp:401-406 (vertical gradient, top/bottom). */
+int tvk_set_bg_colors(tvk_renderer* r, const float top[3], const float bottom[3]);
+/* setLightingEnabled / setLightColors -- RenderWindow.cpp:2076,408-415. */
+int tvk_set_lighting_enabled(tvk_renderer* r, int enabled);
+int tvk_set_light_colors(tvk_renderer* r, const float ambient[4], const float diffuse[4],
+                         const float specular[4], const float light_dir[3]);
+/* Early ray termination threshold on accumulated opacity (default 0.99). */
+int tvk_set_early_ray_termination(tvk_renderer* r, float alpha_threshold);
+/* Empty-space skipping on per-brick min/max (default on); the image is the
+ * same either way, only the work changes. */
+int tvk_set_empty_space_skipping(tvk_renderer* r, int enabled);
+
+/* ---------------------------------------------------------------- render */
+/* renderer.paint() -- RenderWindow.cpp:2464, BatchRenderer.lua:35.  Renders
+ * one subframe into the device framebuffer.  In RT_CAPTURE/RT_HEADLESS the
+ * finest allowed LOD is rendered directly; in RT_INTERACTIVE successive
+ * paints refine from the coarsest to the finest allowed LOD. */
+int tvk_paint(tvk_renderer* r);
+/* checkForRedraw -- RenderWindow.cpp:392,1605-1610: 1 while finer subframes
+ * remain, 0 when the frame is complete. */
+int tvk_check_for_redraw(tvk_renderer* r);
+/* scheduleCompleteRedraw -- RenderWindow.cpp:385-393. */
+int tvk_schedule_complete_redraw(tvk_renderer* r);
+/* captureSingleFrame(file, preserveTransparency) readback part --
+ * RenderWindow.cpp:1642-1671, BatchRenderer.lua:38.  Renders until the
+ * frame is complete and copies W*H*4 bytes to `rgba`.  With
+ * preserve_transparency the background is black with alpha = opacity. */
+int tvk_capture_single_frame(tvk_renderer* r, uint8_t* rgba, uint64_t rgba_bytes,
+                             int preserve_transparency);
+/* Progress counters -- RenderWindow.cpp:453-486. */
+int tvk_get_progress(tvk_renderer* r, uint64_t* subframe_count, uint32_t* working_subframe,
+                     uint32_t* brick_count, uint32_t* working_brick, uint64_t* min_lod_index);
+int tvk_get_frame_stats(tvk_renderer* r, tvk_frame_stats* out);
+/* Device pointer of the RGBA8 framebuffer and the handle's HIP stream, for
+ * callers that keep frames on the device. */
+int tvk_get_device_framebuffer(tvk_renderer* r, void** dptr, void** hip_stream);
+/* Block until this handle's stream has drained. */
+int tvk_synchronize(tvk_renderer* r);
+
+/* ---------------------------------------------------------------- memory */
+/* SysInfo()->SetMaxUsableGPUMem + MemMan()->MemSizesChanged()
+ * (ImageVis3D_Settings.cpp:557-559). */
+int tvk_set_max_usable_gpu_mem(tvk_renderer* r, uint64_t bytes);
+/* MemMan()->GetGPUMem/GetAllocatedGPUMem (ImageVis3D_DebugWindow.cpp:172-206). */
+int tvk_get_mem_stats(tvk_renderer* r, tvk_mem_stats* out);
+
+/* ------------------------------------------------------ synthetic data */
+/* Seeded synthetic scalar volumes: UVFReader's GenerateVolumeData
+ * (UVFReader/DataSource.h:335-396, kinds CT_CONST_VALUE / CT_RANDOM /
+ * CT_SPHERE at :30-35) plus the benchmark workloads.  Host code only (no
+ * device needed); x fastest, bits 8 or 16.  For TVK_VOL_CONST the value is
+ * seed modulo 2^bits. */
+enum tvk_volume_kind {
+  TVK_VOL_CONST = 0,
+  TVK_VOL_RANDOM = 1,
+  TVK_VOL_SPHERE = 2,
+  TVK_VOL_BLOBS = 3,          /* 60 Gaussian blobs on a sphere + noise     */
+  TVK_VOL_PERLIN_SPHERES = 4  /* 5-octave Perlin fBm in [0,4095] + spheres */
+};
+int tvk_generate_volume(int kind, uint64_t seed, uint64_t nx, uint64_t ny, uint64_t nz,
+                        uint32_t bits, void* out, uint64_t out_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TVK_HIP_H */

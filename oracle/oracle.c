@@ -1,0 +1,301 @@
+/*
+ * oracle.c -- FP64 CPU restatement of the Tuvok raycast + brick/LOD path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker and the timed CPU baseline); the
+ * product path never loads it.  See oracle.h for the reference anchors and
+ * for which functions are pinned by what.
+ *
+ * Render model (shared with the HIP kernels, stated once here):
+ *  world   : volume box centred at 0, largest extent (dims*rescale) = 1.
+ *  voxel   : LOD-0 voxel i covers [i, i+1); its centre is i + 0.5.  A point
+ *            p in model space maps to v = p * (m / s) + N0 / 2 with
+ *            m = max_j(N0_j * s_j) (GL texture convention, clamp-to-edge).
+ *  rays    : pixel centres, row 0 at the top; perspective through the eye
+ *            with vertical FoV, or orthographic with a stated half height.
+ *            Rays start at t >= 0 (eye / image plane) and are clipped to the
+ *            LOD-0 box; t is voxel-space length.
+ *  samples : step D = 2^lod / rate; the chord L is cut into n = ceil(L/D)
+ *            segments, the last one of length L-(n-1)D; each segment is
+ *            sampled at its midpoint and its opacity is corrected with its
+ *            own length: a' = 1 - (1-a)^len (len in LOD-0 voxels).  The
+ *            composite is therefore continuous in the entry/exit distances.
+ *  level   : sample position in level texel space q = v * N_l / N0.
+ *  TF      : x = clamp(value * tf_scale, 0, n-1), linear between entries.
+ *  blend   : front to back, C += (1-A) a' c, A += (1-A) a'; stop once
+ *            A >= term_alpha.  Pixel = C + (1-A) bg, alpha A + (1-A) bg_a.
+ *  bg      : vertical gradient, top colour at the top row.
+ */
+#include "oracle_common.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static double sample_lin(const oracle_render_params* p, const double q[3]) {
+  int n[3] = {p->nx, p->ny, p->nz};
+  int i0[3], i1[3];
+  double f[3];
+  for (int a = 0; a < 3; ++a) {
+    double g = q[a] - 0.5;
+    double fl = floor(g);
+    f[a] = g - fl;
+    i0[a] = clampi((int)fl, 0, n[a] - 1);
+    i1[a] = clampi((int)fl + 1, 0, n[a] - 1);
+  }
+  int64_t sx = 1, sy = n[0], sz = (int64_t)n[0] * n[1];
+  double c000 = vox_at(p->voxels, p->bits, i0[0] * sx + i0[1] * sy + i0[2] * sz);
+  double c100 = vox_at(p->voxels, p->bits, i1[0] * sx + i0[1] * sy + i0[2] * sz);
+  double c010 = vox_at(p->voxels, p->bits, i0[0] * sx + i1[1] * sy + i0[2] * sz);
+  double c110 = vox_at(p->voxels, p->bits, i1[0] * sx + i1[1] * sy + i0[2] * sz);
+  double c001 = vox_at(p->voxels, p->bits, i0[0] * sx + i0[1] * sy + i1[2] * sz);
+  double c101 = vox_at(p->voxels, p->bits, i1[0] * sx + i0[1] * sy + i1[2] * sz);
+  double c011 = vox_at(p->voxels, p->bits, i0[0] * sx + i1[1] * sy + i1[2] * sz);
+  double c111 = vox_at(p->voxels, p->bits, i1[0] * sx + i1[1] * sy + i1[2] * sz);
+  double c00 = c000 + (c100 - c000) * f[0];
+  double c10 = c010 + (c110 - c010) * f[0];
+  double c01 = c001 + (c101 - c001) * f[0];
+  double c11 = c011 + (c111 - c011) * f[0];
+  double c0 = c00 + (c10 - c00) * f[1];
+  double c1 = c01 + (c11 - c01) * f[1];
+  return c0 + (c1 - c0) * f[2];
+}
+
+static double sample_nn(const oracle_render_params* p, const double q[3], int* near_face) {
+  int n[3] = {p->nx, p->ny, p->nz};
+  int i[3];
+  for (int a = 0; a < 3; ++a) {
+    double fl = floor(q[a]);
+    double fr = q[a] - fl;
+    if (fr < 1e-4 || fr > 1.0 - 1e-4) *near_face = 1;
+    i[a] = clampi((int)fl, 0, n[a] - 1);
+  }
+  return vox_at(p->voxels, p->bits, i[0] + (int64_t)n[0] * (i[1] + (int64_t)n[1] * i[2]));
+}
+
+static void tf_lookup(const oracle_render_params* p, double v, double rgba[4]) {
+  double x = clampd(v * p->tf_scale, 0.0, (double)(p->n_tf - 1));
+  int i = (int)floor(x);
+  double f = x - i;
+  int j = i + 1 < p->n_tf ? i + 1 : p->n_tf - 1;
+  for (int c = 0; c < 4; ++c) rgba[c] = (double)p->tf[4 * i + c] * (1.0 - f) + (double)p->tf[4 * j + c] * f;
+}
+
+/* TF-empty flag per brick of the sampled level: brick empty iff every TF
+ * entry its [min, max] range can reach has alpha == 0 (exact skip). */
+uint8_t* oracle_brick_empty_flags(const oracle_render_params* p, int nb[3]) {
+  int B = p->brick_size, a = p->apron;
+  nb[0] = (p->nx + B - 1) / B; nb[1] = (p->ny + B - 1) / B; nb[2] = (p->nz + B - 1) / B;
+  int64_t cnt = (int64_t)nb[0] * nb[1] * nb[2];
+  uint8_t* e = (uint8_t*)malloc(cnt);
+  for (int bz = 0; bz < nb[2]; ++bz)
+    for (int by = 0; by < nb[1]; ++by)
+      for (int bx = 0; bx < nb[0]; ++bx) {
+        double mn, mx;
+        oracle_brick_minmax(p->voxels, p->bits, p->nx, p->ny, p->nz, B, a, bx, by, bz, &mn, &mx);
+        double x0 = clampd(mn * p->tf_scale, 0, p->n_tf - 1), x1 = clampd(mx * p->tf_scale, 0, p->n_tf - 1);
+        int i0 = (int)floor(x0), i1 = (int)floor(x1) + 1;
+        if (i1 > p->n_tf - 1) i1 = p->n_tf - 1;
+        int empty = 1;
+        for (int i = i0; i <= i1; ++i)
+          if (p->tf[4 * i + 3] > 0.0f) { empty = 0; break; }
+        e[(bz * nb[1] + by) * nb[0] + bx] = (uint8_t)empty;
+      }
+  return e;
+}
+
+int oracle_render_1d(const oracle_render_params* p, double* out, uint32_t* out_samples,
+                     uint8_t* out_flags) {
+  cam_ctx c;
+  if (cam_setup(p, &c)) return -1;
+  int nb[3] = {0, 0, 0};
+  uint8_t* empty = p->brick_size > 0 ? oracle_brick_empty_flags(p, nb) : NULL;
+  const double lo[3] = {0, 0, 0};
+  const double hi[3] = {p->n0x, p->n0y, p->n0z};
+  const double lsc[3] = {(double)p->nx / p->n0x, (double)p->ny / p->n0y, (double)p->nz / p->n0z};
+  const double step = ldexp(1.0, p->lod) / p->rate;
+  for (int y = 0; y < p->height; ++y) {
+    double wy = 1.0 - (y + 0.5) / p->height;
+    double bg[3];
+    for (int i = 0; i < 3; ++i) bg[i] = p->bg_bottom[i] + (p->bg_top[i] - p->bg_bottom[i]) * wy;
+    for (int x = 0; x < p->width; ++x) {
+      double o[3], d[3];
+      cam_ray(p, &c, x, y, o, d);
+      double C[3] = {0, 0, 0}, A = 0;
+      uint32_t ns = 0;
+      uint8_t flags = 0;
+      double t0, t1;
+      if (box_hit(o, d, lo, hi, &t0, &t1)) {
+        if (t0 < 0) t0 = 0;
+        double L = t1 - t0;
+        if (L > 0) {
+          int64_t n = (int64_t)ceil(L / step);
+          for (int64_t k = 0; k < n; ++k) {
+            double len = (k == n - 1) ? L - (double)(n - 1) * step : step;
+            double t = t0 + (double)k * step + 0.5 * len;
+            double q[3];
+            for (int i = 0; i < 3; ++i) q[i] = (o[i] + t * d[i]) * lsc[i];
+            if (empty) {
+              int b[3];
+              for (int i = 0; i < 3; ++i) b[i] = clampi((int)floor(q[i] / p->brick_size), 0, nb[i] - 1);
+              if (empty[(b[2] * nb[1] + b[1]) * nb[0] + b[0]]) continue;
+            }
+            int nf = 0;
+            double v = p->interp ? sample_lin(p, q) : sample_nn(p, q, &nf);
+            if (nf) flags |= 2;
+            double rgba[4];
+            tf_lookup(p, v, rgba);
+            ++ns;
+            double a = rgba[3];
+            if (a <= 0.0) continue;
+            double ac = a >= 1.0 ? 1.0 : 1.0 - pow(1.0 - a, len);
+            double w = (1.0 - A) * ac;
+            for (int i = 0; i < 3; ++i) C[i] += w * rgba[i];
+            A += w;
+            if (fabs(A - p->term_alpha) < 1e-5) flags |= 1;
+            if (A >= p->term_alpha) break;
+          }
+        }
+      }
+      double* px = out + 4 * ((int64_t)y * p->width + x);
+      for (int i = 0; i < 3; ++i) px[i] = C[i] + (1.0 - A) * bg[i];
+      px[3] = A + (1.0 - A) * p->bg_alpha;
+      if (out_samples) out_samples[(int64_t)y * p->width + x] = ns;
+      if (out_flags) out_flags[(int64_t)y * p->width + x] = flags;
+    }
+  }
+  free(empty);
+  return 0;
+}
+
+uint64_t oracle_nominal_samples(const oracle_render_params* p) {
+  cam_ctx c;
+  if (cam_setup(p, &c)) return 0;
+  const double lo[3] = {0, 0, 0};
+  const double hi[3] = {p->n0x, p->n0y, p->n0z};
+  uint64_t total = 0;
+  for (int y = 0; y < p->height; ++y)
+    for (int x = 0; x < p->width; ++x) {
+      double o[3], d[3], t0, t1;
+      cam_ray(p, &c, x, y, o, d);
+      if (!box_hit(o, d, lo, hi, &t0, &t1)) continue;
+      if (t0 < 0) t0 = 0;
+      if (t1 > t0) total += (uint64_t)floor((t1 - t0) * p->rate);
+    }
+  return total;
+}
+
+/* ------------------------------------------------------- bricking / LOD */
+void oracle_downsample(const void* src, int32_t bits, int32_t nx, int32_t ny, int32_t nz, void* dst) {
+  int mx = (nx + 1) / 2, my = (ny + 1) / 2, mz = (nz + 1) / 2;
+  for (int z = 0; z < mz; ++z)
+    for (int y = 0; y < my; ++y)
+      for (int x = 0; x < mx; ++x) {
+        uint64_t sum = 0, cnt = 0;
+        for (int dz = 0; dz < 2; ++dz)
+          for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+              int sx = 2 * x + dx, sy = 2 * y + dy, sz = 2 * z + dz;
+              if (sx >= nx || sy >= ny || sz >= nz) continue;
+              int64_t i = sx + (int64_t)nx * (sy + (int64_t)ny * sz);
+              sum += bits == 8 ? ((const uint8_t*)src)[i] : ((const uint16_t*)src)[i];
+              ++cnt;
+            }
+        int64_t o = x + (int64_t)mx * (y + (int64_t)my * z);
+        if (bits == 8) ((uint8_t*)dst)[o] = (uint8_t)(sum / cnt);
+        else ((uint16_t*)dst)[o] = (uint16_t)(sum / cnt);
+      }
+}
+
+void oracle_extract_brick(const void* src, int32_t bits, int32_t nx, int32_t ny, int32_t nz,
+                          int32_t B, int32_t a, int32_t bx, int32_t by, int32_t bz, void* slot) {
+  int S = B + 2 * a;
+  for (int z = 0; z < S; ++z)
+    for (int y = 0; y < S; ++y)
+      for (int x = 0; x < S; ++x) {
+        int gx = clampi(bx * B - a + x, 0, nx - 1);
+        int gy = clampi(by * B - a + y, 0, ny - 1);
+        int gz = clampi(bz * B - a + z, 0, nz - 1);
+        int64_t i = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
+        int64_t o = x + (int64_t)S * (y + (int64_t)S * z);
+        if (bits == 8) ((uint8_t*)slot)[o] = ((const uint8_t*)src)[i];
+        else ((uint16_t*)slot)[o] = ((const uint16_t*)src)[i];
+      }
+}
+
+void oracle_brick_minmax(const void* src, int32_t bits, int32_t nx, int32_t ny, int32_t nz,
+                         int32_t B, int32_t a, int32_t bx, int32_t by, int32_t bz, double* mn,
+                         double* mx) {
+  /* clamp-to-edge replication adds no new values, so the slot's range is
+   * the range of the in-volume part of [b*B - a, (b+1)*B + a). */
+  int x0 = clampi(bx * B - a, 0, nx - 1), x1 = clampi((bx + 1) * B + a - 1, 0, nx - 1);
+  int y0 = clampi(by * B - a, 0, ny - 1), y1 = clampi((by + 1) * B + a - 1, 0, ny - 1);
+  int z0 = clampi(bz * B - a, 0, nz - 1), z1 = clampi((bz + 1) * B + a - 1, 0, nz - 1);
+  double lo = INFINITY, hi = -INFINITY;
+  for (int z = z0; z <= z1; ++z)
+    for (int y = y0; y <= y1; ++y)
+      for (int x = x0; x <= x1; ++x) {
+        double v = vox_at(src, bits, x + (int64_t)nx * (y + (int64_t)ny * z));
+        if (v < lo) lo = v;
+        if (v > hi) hi = v;
+      }
+  *mn = lo;
+  *mx = hi;
+}
+
+void oracle_histogram_1d(const void* src, int32_t bits, uint64_t n, uint64_t* hist) {
+  uint64_t nb = (uint64_t)1 << bits;
+  memset(hist, 0, nb * sizeof(uint64_t));
+  for (uint64_t i = 0; i < n; ++i) hist[bits == 8 ? ((const uint8_t*)src)[i] : ((const uint16_t*)src)[i]]++;
+}
+
+static inline float vnorm(const void* src, int bits, int64_t i) {
+  volatile float s = bits == 8 ? 255.0f : 65535.0f;
+  return vox_f(src, bits, i) / s;
+}
+
+/* One-sided at the borders, central inside (numpy.gradient, unit spacing). */
+static inline float grad1(const void* src, int bits, int64_t i, int c, int n, int64_t stride) {
+  if (n < 2) return 0.0f;
+  if (c == 0) return vnorm(src, bits, i + stride) - vnorm(src, bits, i);
+  if (c == n - 1) return vnorm(src, bits, i) - vnorm(src, bits, i - stride);
+  return (vnorm(src, bits, i + stride) - vnorm(src, bits, i - stride)) * 0.5f;
+}
+
+static inline float grad_mag(const void* src, int bits, int nx, int ny, int nz, int x, int y, int z) {
+  int64_t i = x + (int64_t)nx * (y + (int64_t)ny * z);
+  volatile float gx = grad1(src, bits, i, x, nx, 1);
+  volatile float gy = grad1(src, bits, i, y, ny, nx);
+  volatile float gz = grad1(src, bits, i, z, nz, (int64_t)nx * ny);
+  volatile float sx = gx * gx, sy = gy * gy, sz = gz * gz;
+  volatile float s = sx + sy;
+  s = s + sz;
+  return sqrtf(s);
+}
+
+float oracle_histogram_2d(const void* src, int32_t bits, int32_t nx, int32_t ny, int32_t nz,
+                          uint32_t nv, uint32_t ng, uint64_t* hist) {
+  float gmax = 0.0f;
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        float g = grad_mag(src, bits, nx, ny, nz, x, y, z);
+        if (g > gmax) gmax = g;
+      }
+  memset(hist, 0, (size_t)nv * ng * sizeof(uint64_t));
+  int shift = 0;
+  while (((uint64_t)nv << shift) < ((uint64_t)1 << bits)) ++shift;
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        int64_t i = x + (int64_t)nx * (y + (int64_t)ny * z);
+        uint32_t v = bits == 8 ? ((const uint8_t*)src)[i] : ((const uint16_t*)src)[i];
+        uint32_t bx = v >> shift;
+        float g = grad_mag(src, bits, nx, ny, nz, x, y, z);
+        volatile float r = gmax > 0 ? g / gmax : 0.0f;
+        volatile float rs = r * (float)(ng - 1);
+        uint32_t by = (uint32_t)floorf(rs);
+        if (by > ng - 1) by = ng - 1;
+        hist[(uint64_t)bx * ng + by]++;
+      }
+  return gmax;
+}

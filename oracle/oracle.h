@@ -1,0 +1,108 @@
+/*
+ * oracle.h -- CPU restatement of Tuvok's raycasting + brick/LOD hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the
+ * checker or as the timed CPU baseline -- never as the product path.
+ *
+ * Reference anchors (the Tuvok sources themselves are an un-vendored
+ * submodule, absent from the reference tree, so the GLSL raycaster cannot be
+ * compiled or run; see the notes on each function):
+ *  - render model: the GLRaycaster parameters exposed through Lua
+ *    (ImageVis3D/UI/RenderWindow.cpp:2090-2103 sample rate, :2051-2059 render
+ *    mode, :437-440 interpolant, :1033-1082 camera, :166-182 region matrices,
+ *    :401-406 background, :1642-1671 capture) -- the shader maths is
+ *    "parity unpinned" against the reference and pinned here by closed
+ *    forms and invariants instead (tests/test_oracle_render.py).
+ *  - bricking / LOD: UVFReader/DataSource.h:502-570 (brick size, overlap,
+ *    2x LOD decimation, LOD count rule), CombineAverage rounding unpinned.
+ *  - histograms: UVFReader/DataSource.h:595-636, pinned exactly by the
+ *    blocks stored in DemoData/c60.uvf.
+ */
+#ifndef TVK_ORACLE_H
+#define TVK_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_render_params {
+  /* volume actually sampled (one LOD level), x fastest */
+  const void* voxels;
+  int32_t bits;            /* 8 or 16 */
+  int32_t nx, ny, nz;      /* dims of the sampled level */
+  int32_t n0x, n0y, n0z;   /* LOD-0 dims: they define the world box */
+  int32_t lod;             /* level index: step = 2^lod / rate LOD-0 voxels */
+  double rescale[3];       /* setRescaleFactors */
+  /* 1D transfer function, n_tf RGBA floats; entry = value * tf_scale */
+  const float* tf;
+  int32_t n_tf;
+  double tf_scale;
+  /* camera (world space), region matrix (row-vector convention) */
+  double view_pos[3], view_dir[3], up_dir[3];
+  double fov_deg;
+  int32_t ortho;
+  double ortho_half_height;
+  double model[16];
+  /* image + quality */
+  int32_t width, height;
+  double rate;             /* samples per voxel of the sampled level */
+  double term_alpha;       /* early ray termination threshold; >1 disables */
+  int32_t interp;          /* 0 nearest, 1 trilinear */
+  double bg_top[3], bg_bottom[3], bg_alpha;
+  /* brick-granular empty-space skipping (sample counting only) */
+  int32_t brick_size;      /* payload B of the sampled level, 0 = no bricks */
+  int32_t apron;
+} oracle_render_params;
+
+/* FP64 reference raycast.  out_rgba: W*H*4 doubles (composited over the
+ * background, [0,1]); out_samples: per-pixel evaluated samples (may be NULL);
+ * out_flags: per-pixel bits, 1 = the early-termination decision was within
+ * 1e-5 of the threshold, 2 = a nearest-neighbour sample lay within 1e-4 of
+ * a voxel face (may be NULL).  Returns 0 on success. */
+int oracle_render_1d(const oracle_render_params* p, double* out_rgba,
+                     uint32_t* out_samples, uint8_t* out_flags);
+
+/* Nominal samples: sum over pixels of floor(chord * rate) through the
+ * LOD-0 box (the LOD-independent work unit of the throughput metric). */
+uint64_t oracle_nominal_samples(const oracle_render_params* p);
+
+/* FP32 multi-threaded CPU baseline of the same model (OpenMP over 8x8 pixel
+ * tiles, brick-granular empty-space skipping).  out_rgba8: W*H*4 bytes.
+ * Returns evaluated samples. */
+uint64_t baseline_render_1d_f32(const oracle_render_params* p, uint8_t* out_rgba8,
+                                int32_t n_threads);
+
+/* LOD level l+1 from level l: each output voxel is floor(sum / count) over
+ * its existing 2x2x2 children (count = 8 inside, fewer at odd edges).
+ * Output dims are ceil(n/2). */
+void oracle_downsample(const void* src, int32_t bits, int32_t nx, int32_t ny, int32_t nz,
+                       void* dst);
+
+/* Copy brick (bx,by,bz) of payload B with `apron` voxels per side into a
+ * (B+2a)^3 slot; voxels outside the volume replicate the nearest edge
+ * voxel (clamp-to-edge). */
+void oracle_extract_brick(const void* src, int32_t bits, int32_t nx, int32_t ny, int32_t nz,
+                          int32_t B, int32_t apron, int32_t bx, int32_t by, int32_t bz,
+                          void* slot);
+
+/* min/max over a brick's stored slot (payload + apron). */
+void oracle_brick_minmax(const void* src, int32_t bits, int32_t nx, int32_t ny, int32_t nz,
+                         int32_t B, int32_t apron, int32_t bx, int32_t by, int32_t bz,
+                         double* mn, double* mx);
+
+/* Histogram1DDataBlock: counts per raw value, n_bins = 2^bits. */
+void oracle_histogram_1d(const void* src, int32_t bits, uint64_t n_voxels, uint64_t* hist);
+
+/* Histogram2DDataBlock: gradient = central differences of v/(2^bits-1)
+ * ((f[i+1]-f[i-1])/2 inside, one-sided f[1]-f[0] at the borders), in
+ * float; bin_y = floor(|g| / max|g| * (n_grad-1)); bin_x = value scaled to
+ * n_value bins.  hist is n_value x n_grad, X slowest.  Returns max|g|. */
+float oracle_histogram_2d(const void* src, int32_t bits, int32_t nx, int32_t ny, int32_t nz,
+                          uint32_t n_value, uint32_t n_grad, uint64_t* hist);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

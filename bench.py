@@ -49,13 +49,34 @@ def cpu_baseline(vol, w, cams, threads: int) -> dict:
             "fps": len(cams) / secs}
 
 
+def calibrate_copy(nbytes: int = 1 << 30) -> int:
+    """A plain device copy of known size (nbytes read + nbytes written), the
+    reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings."""
+    import torch
+    n = nbytes // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read": nbytes, "bytes_written": nbytes,
+                      "dispatches": 3}))
+    return 0
+
+
 def main() -> int:
+    if "--calibrate-copy" in sys.argv:
+        return calibrate_copy()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=36)
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-views", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=8, help="samples per march step")
+    ap.add_argument("--tile-group", type=int, default=4, help="ray tiles per XCD turn")
+    ap.add_argument("--xcd-split", type=int, default=0, help="0 or frame regions per row")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
@@ -76,6 +97,7 @@ def main() -> int:
     brick_s = time.perf_counter() - t0
     r.set1DTrans(w.tf)
     r.setLODLimits((0, 0))
+    r.setKernelTuning(args.batch, args.tile_group, args.xcd_split)
     workloads.apply(r, w, w.cameras[0])
     print(f"[bench] bricked into the device pool in {brick_s:.2f} s; warmup {args.warmup}",
           file=sys.stderr, flush=True)

@@ -92,6 +92,7 @@ SIGNATURES = {
     "tvk_set_light_colors": [H, P, P, P, P],
     "tvk_set_early_ray_termination": [H, f32],
     "tvk_set_empty_space_skipping": [H, i32],
+    "tvk_set_kernel_tuning": [H, u32, u32, u32],
     "tvk_paint": [H],
     "tvk_check_for_redraw": [H],
     "tvk_schedule_complete_redraw": [H],

@@ -2,16 +2,26 @@
 // (the work of Tuvok's GLRaycaster / GLGridLeaper fragment shaders,
 // reached through AbstrRenderer::Paint, ImageVis3D/UI/RenderWindow.cpp:2464).
 //
-// Mapping to CDNA4: one 64-lane wavefront renders one 8x8 ray tile, so the
-// 64 rays of a wave march through neighbouring voxels and their gathers hit
-// the same L1/L2 lines.  Work-groups are remapped so each of the 8 XCDs
-// renders one contiguous band of tiles (its own L2 holds the bricks that
-// band sees).  Each ray walks the brick grid of the chosen LOD; per brick it
-// looks up the pool slot in the page table, skips the brick when the 1D TF
-// is transparent over the brick's [min, max] (exact: such samples add
-// nothing), and otherwise marches the brick's samples with trilinear
-// fetches from the slot, whose apron makes every fetch brick-local.  Rays
-// stop at the early-termination opacity.
+// Mapping to CDNA4
+//  - One 64-lane wavefront renders one 8x8 ray tile, so the 64 rays of a
+//    wave march through neighbouring voxels and their gathers share L1/L2
+//    lines.
+//  - The kernel is persistent: one grid of work-groups fills the chip and
+//    every wave pulls tiles from per-XCD queues (work-group w runs on XCD
+//    w%8).  Tiles are cut into runs of `tile_group` neighbours and run g
+//    belongs to XCD g%8, so each XCD's L2 sees neighbouring rays while all
+//    XCDs get an even share of the frame; a wave whose own queue is empty
+//    steals from the other XCDs' queues, which absorbs the long-ray tail.
+//  - The 1D TF is staged once per work-group into LDS (up to 4096 RGBA
+//    entries = 64 KB), so the two TF reads per sample are LDS reads and the
+//    vector-memory path only carries voxel gathers.
+//  - Each ray walks the brick grid of the chosen LOD; per brick it looks up
+//    the pool slot in the page table, skips the brick when the TF is
+//    transparent over the brick's [min, max] (exact: such samples add
+//    nothing), and otherwise marches the brick's samples with trilinear
+//    fetches from the slot, whose apron keeps every fetch brick-local.
+//    Samples are gathered BATCH at a time so a wave keeps several gathers in
+//    flight.  Rays stop at the early-termination opacity.
 //
 // The sample positions, segment lengths, opacity correction and blending
 // follow the model stated in oracle/oracle.c (the FP64 oracle it is tested
@@ -25,6 +35,9 @@ namespace tvk {
 namespace {
 
 constexpr float kInf = __builtin_huge_valf();
+constexpr int kWaves = 8;                  // waves per work-group
+constexpr int kThreads = 64 * kWaves;
+constexpr int kLdsTfEntries = 4096;        // 64 KB of float4
 
 __device__ inline float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 __device__ inline int clampi(int x, int lo, int hi) { return min(max(x, lo), hi); }
@@ -42,178 +55,239 @@ __device__ inline float ld(const T* p) {
   return (float)*p;
 }
 
+// One sample from a brick slot at slot-local texel coordinate (gx, gy, gz):
+// trilinear over the 8 surrounding voxels, or the nearest voxel.  Indices
+// are clamped to the slot, so no fetch leaves the slot.
 template <typename T, bool LINEAR>
-__global__ __launch_bounds__(64) void raycast1d_kernel(const RaycastParams P) {
-  // XCD-aware tile order: the dispatcher deals work-groups round-robin to
-  // the 8 XCDs, so work-group w runs on XCD w%8; give that XCD the
-  // contiguous tile band [xcd*per_xcd, (xcd+1)*per_xcd).
-  const int ntiles = P.tiles_x * P.tiles_y;
-  const int per_xcd = (ntiles + 7) >> 3;
-  const int tile = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
-  if (tile >= ntiles) return;  // uniform per wave
-  const int tx = tile % P.tiles_x, ty = tile / P.tiles_x;
-  const int px = tx * kTile + (int)(threadIdx.x & 7);
-  const int py = ty * kTile + (int)(threadIdx.x >> 3);
+__device__ inline float fetch(const T* __restrict__ base, int S, int S2, float gx, float gy,
+                              float gz) {
+  if (LINEAR) {
+    const int ix = clampi((int)floorf(gx), 0, S - 2);
+    const int iy = clampi((int)floorf(gy), 0, S - 2);
+    const int iz = clampi((int)floorf(gz), 0, S - 2);
+    const float wx = clampf(gx - (float)ix, 0.f, 1.f);
+    const float wy = clampf(gy - (float)iy, 0.f, 1.f);
+    const float wz = clampf(gz - (float)iz, 0.f, 1.f);
+    const T* p = base + (iz * S2 + iy * S + ix);
+    const float c000 = ld(p), c100 = ld(p + 1);
+    const float c010 = ld(p + S), c110 = ld(p + S + 1);
+    const float c001 = ld(p + S2), c101 = ld(p + S2 + 1);
+    const float c011 = ld(p + S2 + S), c111 = ld(p + S2 + S + 1);
+    const float c00 = fmaf(c100 - c000, wx, c000);
+    const float c10 = fmaf(c110 - c010, wx, c010);
+    const float c01 = fmaf(c101 - c001, wx, c001);
+    const float c11 = fmaf(c111 - c011, wx, c011);
+    const float c0 = fmaf(c10 - c00, wy, c00);
+    const float c1 = fmaf(c11 - c01, wy, c01);
+    return fmaf(c1 - c0, wz, c0);
+  }
+  const int ix = clampi((int)floorf(gx), 0, S - 1);
+  const int iy = clampi((int)floorf(gy), 0, S - 1);
+  const int iz = clampi((int)floorf(gz), 0, S - 1);
+  return ld(base + (iz * S2 + iy * S + ix));
+}
 
-  uint32_t n_eval = 0, n_nom = 0;
-  if (px < P.width && py < P.height) {
-    float o[3], d[3];
-    const float fx = (float)px, fy = (float)py;
-    if (P.ortho) {
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        o[i] = P.eye[i] + fx * P.du[i] + fy * P.dv[i];
-        d[i] = P.odir[i];
-      }
-    } else {
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        o[i] = P.eye[i];
-        d[i] = P.dir00[i] + fx * P.du[i] + fy * P.dv[i];
-      }
-      const float l = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-#pragma unroll
-      for (int i = 0; i < 3; ++i) d[i] = d[i] / l;
-    }
-    // ray vs LOD-0 box [0, box]; axis-parallel rays use an inside test
-    float t0 = -kInf, t1 = kInf, inv[3];
-    bool hit = true;
+// Render the 64 rays of ray tile `tile` (lane = ray within the 8x8 tile).
+// `tf4` points at the TF (LDS or global).
+template <typename T, bool LINEAR, int BATCH>
+__device__ inline void render_tile(const RaycastParams& P, const float4* __restrict__ tf4, int tile,
+                                   int lane, uint32_t& n_eval, uint32_t& n_nom) {
+  const int tx = tile % P.tiles_x, ty = tile / P.tiles_x;
+  const int px = tx * kTile + (lane & 7);
+  const int py = ty * kTile + (lane >> 3);
+  if (px >= P.width || py >= P.height) return;
+
+  float o[3], d[3];
+  const float fx = (float)px, fy = (float)py;
+  if (P.ortho) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      if (d[i] == 0.f) {
-        inv[i] = kInf;
-        if (o[i] < 0.f || o[i] > P.box[i]) hit = false;
-      } else {
-        inv[i] = 1.f / d[i];
-        float ta = (0.f - o[i]) * inv[i], tb = (P.box[i] - o[i]) * inv[i];
-        if (ta > tb) {
-          const float s = ta;
-          ta = tb;
-          tb = s;
-        }
-        t0 = fmaxf(t0, ta);
-        t1 = fminf(t1, tb);
-      }
+      o[i] = P.eye[i] + fx * P.du[i] + fy * P.dv[i];
+      d[i] = P.odir[i];
     }
-    t0 = fmaxf(t0, 0.f);
-    const float L = t1 - t0;
-    float C0 = 0.f, C1 = 0.f, C2 = 0.f, A = 0.f;
-    if (hit && L > 0.f) {
-      n_nom = (uint32_t)floorf(L * P.nominal_rate);
-      const float step = P.step;
-      const int n = (int)ceilf(L / step);
-      const float lastlen = fmaxf(L - (float)(n - 1) * step, 0.f);
-      const LodInfo& lv = P.level;
-      const float invB = 1.f / (float)P.brick;
-      float od[3], oo[3], bl0[3];
+  } else {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      o[i] = P.eye[i];
+      d[i] = P.dir00[i] + fx * P.du[i] + fy * P.dv[i];
+    }
+    const float l = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) d[i] = d[i] / l;
+  }
+  // ray vs LOD-0 box [0, box]; axis-parallel rays use an inside test
+  float t0 = -kInf, t1 = kInf, inv[3];
+  bool hit = true;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (d[i] == 0.f) {
+      inv[i] = kInf;
+      if (o[i] < 0.f || o[i] > P.box[i]) hit = false;
+    } else {
+      inv[i] = 1.f / d[i];
+      float ta = (0.f - o[i]) * inv[i], tb = (P.box[i] - o[i]) * inv[i];
+      if (ta > tb) {
+        const float s = ta;
+        ta = tb;
+        tb = s;
+      }
+      t0 = fmaxf(t0, ta);
+      t1 = fminf(t1, tb);
+    }
+  }
+  t0 = fmaxf(t0, 0.f);
+  const float L = t1 - t0;
+  float C0 = 0.f, C1 = 0.f, C2 = 0.f, A = 0.f;
+  if (hit && L > 0.f) {
+    n_nom += (uint32_t)floorf(L * P.nominal_rate);
+    const float step = P.step;
+    const int n = (int)ceilf(L / step);
+    const float lastlen = fmaxf(L - (float)(n - 1) * step, 0.f);
+    const LodInfo& lv = P.level;
+    const float invB = 1.f / (float)P.brick;
+    float od[3], oo[3], bl0[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      od[i] = d[i] * lv.to_level[i];
+      oo[i] = o[i] * lv.to_level[i];
+      bl0[i] = (float)P.brick / lv.to_level[i];  // brick edge in LOD-0 voxels
+    }
+    const float tfmax = (float)(P.n_tf - 1);
+    const int S = P.slot_edge;
+    const int S2 = S * S;
+    int k = 0;
+    while (k < n) {
+      const float len0 = (k == n - 1) ? lastlen : step;
+      const float tk = fmaf((float)k, step, t0) + 0.5f * len0;
+      int b[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        b[i] = clampi((int)floorf(fmaf(tk, od[i], oo[i]) * invB), 0, lv.grid[i] - 1);
+      float tb = kInf;
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        od[i] = d[i] * lv.to_level[i];
-        oo[i] = o[i] * lv.to_level[i];
-        bl0[i] = (float)P.brick / lv.to_level[i];  // brick edge in LOD-0 voxels
+        if (d[i] != 0.f) {
+          const float face = (float)(d[i] > 0.f ? b[i] + 1 : b[i]) * bl0[i];
+          tb = fminf(tb, (face - o[i]) * inv[i]);
+        }
       }
-      const float4* __restrict__ tf4 = (const float4*)P.tf;
-      const float tfmax = (float)(P.n_tf - 1);
-      const int S = P.slot_edge;
-      const int S2 = S * S;
-      int k = 0;
-      while (k < n) {
-        const float len0 = (k == n - 1) ? lastlen : step;
-        const float tk = fmaf((float)k, step, t0) + 0.5f * len0;
-        int b[3];
+      int kend = (int)ceilf((tb - t0) / step - 0.5f);
+      kend = min(max(kend, k + 1), n);
+      const int gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
+      const int slot = P.page_table[gid];
+      bool empty = slot < 0;
+      if (!empty && P.skip) {
+        const float2 r = ((const float2*)P.brick_range)[gid];
+        const int i0 = (int)clampf(r.x * P.tf_scale, 0.f, tfmax);
+        const int i1 = min((int)clampf(r.y * P.tf_scale, 0.f, tfmax) + 1, P.n_tf - 1);
+        empty = (P.tf_prefix[i1 + 1] - P.tf_prefix[i0]) == 0;
+      }
+      if (empty) {
+        k = kend;
+        continue;
+      }
+      const T* __restrict__ base = (const T*)P.pool + (int64_t)slot * P.slot_elems;
+      float g0[3];  // slot-local texel coordinate at t = 0
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
-          b[i] = clampi((int)floorf(fmaf(tk, od[i], oo[i]) * invB), 0, lv.grid[i] - 1);
-        float tb = kInf;
+      for (int i = 0; i < 3; ++i)
+        g0[i] = oo[i] - (float)(b[i] * P.brick - P.apron) - (LINEAR ? 0.5f : 0.f);
+      // Batched march: gather BATCH samples (clamped to the brick's last
+      // sample), look all of them up in the TF, then composite in order.
+      bool live = true;
+      for (; k < kend; k += BATCH) {
+        float v[BATCH], len[BATCH];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          if (d[i] != 0.f) {
-            const float face = (float)(d[i] > 0.f ? b[i] + 1 : b[i]) * bl0[i];
-            tb = fminf(tb, (face - o[i]) * inv[i]);
-          }
+        for (int j = 0; j < BATCH; ++j) {
+          const int kk = min(k + j, kend - 1);
+          len[j] = (kk == n - 1) ? lastlen : step;
+          const float t = fmaf((float)kk, step, t0) + 0.5f * len[j];
+          v[j] = fetch<T, LINEAR>(base, S, S2, fmaf(t, od[0], g0[0]), fmaf(t, od[1], g0[1]),
+                                  fmaf(t, od[2], g0[2]));
         }
-        int kend = (int)ceilf((tb - t0) / step - 0.5f);
-        kend = min(max(kend, k + 1), n);
-        const int gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
-        const int slot = P.page_table[gid];
-        bool empty = slot < 0;
-        if (!empty && P.skip) {
-          const float2 r = ((const float2*)P.brick_range)[gid];
-          const int i0 = (int)clampf(r.x * P.tf_scale, 0.f, tfmax);
-          const int i1 = min((int)clampf(r.y * P.tf_scale, 0.f, tfmax) + 1, P.n_tf - 1);
-          empty = (P.tf_prefix[i1 + 1] - P.tf_prefix[i0]) == 0;
-        }
-        if (empty) {
-          k = kend;
-          continue;
-        }
-        const T* __restrict__ base = (const T*)P.pool + (int64_t)slot * P.slot_elems;
-        float g0[3];  // slot-local texel coordinate at t = 0
+        float4 e0[BATCH], e1[BATCH];
+        float fr[BATCH];
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
-          g0[i] = oo[i] - (float)(b[i] * P.brick - P.apron) - (LINEAR ? 0.5f : 0.f);
-        for (; k < kend; ++k) {
-          const float len = (k == n - 1) ? lastlen : step;
-          const float t = fmaf((float)k, step, t0) + 0.5f * len;
-          const float gx = fmaf(t, od[0], g0[0]);
-          const float gy = fmaf(t, od[1], g0[1]);
-          const float gz = fmaf(t, od[2], g0[2]);
-          float v;
-          if (LINEAR) {
-            const float fx0 = floorf(gx), fy0 = floorf(gy), fz0 = floorf(gz);
-            const int ix = clampi((int)fx0, 0, S - 2);
-            const int iy = clampi((int)fy0, 0, S - 2);
-            const int iz = clampi((int)fz0, 0, S - 2);
-            const float wx = clampf(gx - (float)ix, 0.f, 1.f);
-            const float wy = clampf(gy - (float)iy, 0.f, 1.f);
-            const float wz = clampf(gz - (float)iz, 0.f, 1.f);
-            const T* p = base + (iz * S2 + iy * S + ix);
-            const float c000 = ld(p), c100 = ld(p + 1);
-            const float c010 = ld(p + S), c110 = ld(p + S + 1);
-            const float c001 = ld(p + S2), c101 = ld(p + S2 + 1);
-            const float c011 = ld(p + S2 + S), c111 = ld(p + S2 + S + 1);
-            const float c00 = fmaf(c100 - c000, wx, c000);
-            const float c10 = fmaf(c110 - c010, wx, c010);
-            const float c01 = fmaf(c101 - c001, wx, c001);
-            const float c11 = fmaf(c111 - c011, wx, c011);
-            const float c0 = fmaf(c10 - c00, wy, c00);
-            const float c1 = fmaf(c11 - c01, wy, c01);
-            v = fmaf(c1 - c0, wz, c0);
-          } else {
-            const int ix = clampi((int)floorf(gx), 0, S - 1);
-            const int iy = clampi((int)floorf(gy), 0, S - 1);
-            const int iz = clampi((int)floorf(gz), 0, S - 1);
-            v = ld(base + (iz * S2 + iy * S + ix));
-          }
-          ++n_eval;
-          const float x = clampf(v * P.tf_scale, 0.f, tfmax);
+        for (int j = 0; j < BATCH; ++j) {
+          const float x = clampf(v[j] * P.tf_scale, 0.f, tfmax);
           const int ti = (int)x;
-          const float fr = x - (float)ti;
-          const int tj = min(ti + 1, P.n_tf - 1);
-          const float4 e0 = tf4[ti], e1 = tf4[tj];
-          const float a = fmaf(e1.w - e0.w, fr, e0.w);
-          if (a > 0.f) {
-            const float ac = a >= 1.f ? 1.f : 1.f - exp2f(len * log2f(1.f - a));
-            const float w = (1.f - A) * ac;
-            C0 = fmaf(w, fmaf(e1.x - e0.x, fr, e0.x), C0);
-            C1 = fmaf(w, fmaf(e1.y - e0.y, fr, e0.y), C1);
-            C2 = fmaf(w, fmaf(e1.z - e0.z, fr, e0.z), C2);
-            A += w;
-            if (A >= P.term_alpha) {
-              k = n;
-              break;
+          fr[j] = x - (float)ti;
+          e0[j] = tf4[ti];
+          e1[j] = tf4[min(ti + 1, P.n_tf - 1)];
+        }
+#pragma unroll
+        for (int j = 0; j < BATCH; ++j) {
+          if (k + j < kend && live) {
+            ++n_eval;
+            const float a = fmaf(e1[j].w - e0[j].w, fr[j], e0[j].w);
+            if (a > 0.f) {
+              const float ac = a >= 1.f ? 1.f : 1.f - exp2f(len[j] * log2f(1.f - a));
+              const float w = (1.f - A) * ac;
+              C0 = fmaf(w, fmaf(e1[j].x - e0[j].x, fr[j], e0[j].x), C0);
+              C1 = fmaf(w, fmaf(e1[j].y - e0[j].y, fr[j], e0[j].y), C1);
+              C2 = fmaf(w, fmaf(e1[j].z - e0[j].z, fr[j], e0[j].z), C2);
+              A += w;
+              live = A < P.term_alpha;
             }
           }
         }
+        if (!live) break;
       }
+      k = live ? kend : n;
     }
-    const float wy = 1.f - ((float)py + 0.5f) / (float)P.height;
-    const float bg0 = P.bg_bottom[0] + (P.bg_top[0] - P.bg_bottom[0]) * wy;
-    const float bg1 = P.bg_bottom[1] + (P.bg_top[1] - P.bg_bottom[1]) * wy;
-    const float bg2 = P.bg_bottom[2] + (P.bg_top[2] - P.bg_bottom[2]) * wy;
-    const float T1 = 1.f - A;
-    P.framebuffer[(int64_t)py * P.width + px] =
-        pack_rgba8(fmaf(T1, bg0, C0), fmaf(T1, bg1, C1), fmaf(T1, bg2, C2), fmaf(T1, P.bg_alpha, A));
+  }
+  const float wy = 1.f - ((float)py + 0.5f) / (float)P.height;
+  const float bg0 = P.bg_bottom[0] + (P.bg_top[0] - P.bg_bottom[0]) * wy;
+  const float bg1 = P.bg_bottom[1] + (P.bg_top[1] - P.bg_bottom[1]) * wy;
+  const float bg2 = P.bg_bottom[2] + (P.bg_top[2] - P.bg_bottom[2]) * wy;
+  const float T1 = 1.f - A;
+  P.framebuffer[(int64_t)py * P.width + px] =
+      pack_rgba8(fmaf(T1, bg0, C0), fmaf(T1, bg1, C1), fmaf(T1, bg2, C2), fmaf(T1, P.bg_alpha, A));
+}
+
+// Persistent raycast: every wave loops over ray tiles taken from the
+// per-XCD queues P.counters[2 + xcd] until all queues are drained.
+template <typename T, bool LINEAR, int BATCH, bool TF_LDS>
+__global__ __launch_bounds__(kThreads, 2) void raycast1d_kernel(const RaycastParams P) {
+  extern __shared__ float4 s_tf[];  // n_tf entries when TF_LDS
+  const float4* tf4 = (const float4*)P.tf;
+  if (TF_LDS) {
+    for (int i = (int)threadIdx.x; i < P.n_tf; i += kThreads) s_tf[i] = tf4[i];
+    __syncthreads();
+    tf4 = s_tf;
+  }
+  const int lane = (int)(threadIdx.x & 63);
+  const int xcd = (int)(blockIdx.x & 7);
+  unsigned long long* queues = P.counters + 2;
+  uint32_t n_eval = 0, n_nom = 0;
+  for (int q = 0; q < 8; ++q) {
+    const int x = (xcd + q) & 7;  // own queue first, then steal
+    // Queue x enumerates either runs of tile_group tiles dealt round-robin
+    // (xcd_split == 0) or, with xcd_split = rx in {1,2,4,8}, the tiles of
+    // region x of an rx by (8/rx) split of the frame in row-major order.
+    int rx0 = 0, rw = 0, ry0 = 0, rh = 0, slots;
+    const int G = P.tile_group;
+    if (P.xcd_split == 0) {
+      const int ngroups = (P.tiles_x * P.tiles_y + G - 1) / G;
+      slots = ((ngroups + 7) / 8) * G;
+    } else {
+      const int rx = P.xcd_split, ry = 8 / P.xcd_split;
+      const int i = x % rx, jr = x / rx;
+      rx0 = i * P.tiles_x / rx;
+      rw = (i + 1) * P.tiles_x / rx - rx0;
+      ry0 = jr * P.tiles_y / ry;
+      rh = (jr + 1) * P.tiles_y / ry - ry0;
+      slots = rw * rh;
+    }
+    while (true) {
+      int pos = 0;
+      if (lane == 0) pos = (int)atomicAdd(&queues[x], 1ull);
+      pos = __shfl(pos, 0, 64);
+      if (pos >= slots) break;
+      int tile;
+      if (P.xcd_split == 0) tile = ((pos / G) * 8 + x) * G + (pos % G);
+      else tile = (ry0 + pos / rw) * P.tiles_x + rx0 + pos % rw;
+      if (tile < P.tiles_x * P.tiles_y) render_tile<T, LINEAR, BATCH>(P, tf4, tile, lane, n_eval, n_nom);
+    }
   }
   // one atomic per wave for each counter
   unsigned long long e = n_eval, m = n_nom;
@@ -221,29 +295,50 @@ __global__ __launch_bounds__(64) void raycast1d_kernel(const RaycastParams P) {
     e += __shfl_xor(e, off, 64);
     m += __shfl_xor(m, off, 64);
   }
-  if (threadIdx.x == 0) {
+  if (lane == 0) {
     atomicAdd(&P.counters[0], e);
     atomicAdd(&P.counters[1], m);
   }
 }
 
+template <typename T, bool LINEAR, int BATCH, bool TF_LDS>
+hipError_t launch_one(const RaycastParams& p, hipStream_t stream) {
+  auto kern = raycast1d_kernel<T, LINEAR, BATCH, TF_LDS>;
+  const size_t lds = TF_LDS ? (size_t)p.n_tf * sizeof(float4) : 0;
+  int per_cu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads, lds);
+  if (e != hipSuccess) return e;
+  if (per_cu < 1) per_cu = 1;
+  int grid = p.compute_units * per_cu;
+  grid = ((grid + 7) / 8) * 8;
+  kern<<<grid, kThreads, lds, stream>>>(p);
+  return hipGetLastError();
+}
+
+template <typename T, bool LINEAR, int BATCH>
+hipError_t launch_tf(const RaycastParams& p, hipStream_t stream) {
+  if (p.n_tf <= kLdsTfEntries) return launch_one<T, LINEAR, BATCH, true>(p, stream);
+  return launch_one<T, LINEAR, BATCH, false>(p, stream);
+}
+
+template <int BATCH>
+hipError_t launch_batch(const RaycastParams& p, int bpv, hipStream_t stream) {
+  if (bpv == 1)
+    return p.interp ? launch_tf<uint8_t, true, BATCH>(p, stream)
+                    : launch_tf<uint8_t, false, BATCH>(p, stream);
+  return p.interp ? launch_tf<uint16_t, true, BATCH>(p, stream)
+                  : launch_tf<uint16_t, false, BATCH>(p, stream);
+}
+
 }  // namespace
 
 hipError_t launch_raycast_1d(const RaycastParams& p, int bpv, hipStream_t stream) {
-  const int ntiles = p.tiles_x * p.tiles_y;
-  const int grid = 8 * ((ntiles + 7) / 8);
-  if (bpv == 1) {
-    if (p.interp)
-      raycast1d_kernel<uint8_t, true><<<grid, 64, 0, stream>>>(p);
-    else
-      raycast1d_kernel<uint8_t, false><<<grid, 64, 0, stream>>>(p);
-  } else {
-    if (p.interp)
-      raycast1d_kernel<uint16_t, true><<<grid, 64, 0, stream>>>(p);
-    else
-      raycast1d_kernel<uint16_t, false><<<grid, 64, 0, stream>>>(p);
+  switch (p.batch) {
+    case 1: return launch_batch<1>(p, bpv, stream);
+    case 2: return launch_batch<2>(p, bpv, stream);
+    case 4: return launch_batch<4>(p, bpv, stream);
+    default: return launch_batch<8>(p, bpv, stream);
   }
-  return hipGetLastError();
 }
 
 }  // namespace tvk

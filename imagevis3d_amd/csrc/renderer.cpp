@@ -144,7 +144,9 @@ int HipRenderer::Initialize(int device) {
   TVK_HIP(hipSetDevice(device_));
   TVK_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   for (auto& e : ev_) TVK_HIP(hipEventCreate(&e));
-  TVK_HIP(counters_.alloc(4 * sizeof(unsigned long long)));
+  // [0] evaluated samples, [1] nominal samples, [2..9] per-XCD tile queues
+  TVK_HIP(counters_.alloc(16 * sizeof(unsigned long long)));
+  TVK_HIP(hipDeviceGetAttribute(&compute_units_, hipDeviceAttributeMultiprocessorCount, device_));
   TVK_HIP(hipHostMalloc((void**)&counters_h_, 4 * sizeof(unsigned long long), hipHostMallocDefault));
   memset(counters_h_, 0, 4 * sizeof(unsigned long long));
   size_t free_b = 0, total_b = 0;
@@ -606,6 +608,17 @@ int HipRenderer::SetEmptySpaceSkipping(bool on) {
   return TVK_OK;
 }
 
+int HipRenderer::SetKernelTuning(uint32_t batch, uint32_t tile_group, uint32_t xcd_split) {
+  if (batch != 1 && batch != 2 && batch != 4 && batch != 8) return arg_fail("batch must be 1, 2, 4 or 8");
+  if (tile_group < 1 || tile_group > 4096) return arg_fail("tile group must be in [1, 4096]");
+  if (xcd_split != 0 && xcd_split != 1 && xcd_split != 2 && xcd_split != 4 && xcd_split != 8)
+    return arg_fail("xcd split must be 0, 1, 2, 4 or 8");
+  batch_ = (int)batch;
+  tile_group_ = (int)tile_group;
+  xcd_split_ = (int)xcd_split;
+  return TVK_OK;
+}
+
 // ----------------------------------------------------------------- render
 int HipRenderer::check_ready() const {
   if (!initialized_) return state_fail("paint before initialize");
@@ -682,6 +695,10 @@ void HipRenderer::build_params(RaycastParams* p, int lod) const {
   p->nominal_rate = rate_;
   p->term_alpha = term_alpha_;
   p->interp = interp_;
+  p->batch = batch_;
+  p->tile_group = tile_group_;
+  p->xcd_split = xcd_split_;
+  p->compute_units = compute_units_;
   p->skip = skip_ ? 1 : 0;
   p->level = lods_[lod];
   p->brick = brick_;
@@ -723,7 +740,7 @@ int HipRenderer::Paint() {
   RaycastParams p;
   build_params(&p, lod);
   TVK_HIP(hipEventRecord(ev_[0], stream_));
-  TVK_HIP(launch_fill_u64(counters_.as<unsigned long long>(), 2, 0ull, stream_));
+  TVK_HIP(launch_fill_u64(counters_.as<unsigned long long>(), 10, 0ull, stream_));
   TVK_HIP(hipEventRecord(ev_[1], stream_));
   TVK_HIP(launch_raycast_1d(p, (int)bits_ / 8, stream_));
   TVK_HIP(hipEventRecord(ev_[2], stream_));

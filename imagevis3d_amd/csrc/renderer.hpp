@@ -93,6 +93,7 @@ class HipRenderer {
   int SetLightColors(const float amb[4], const float dif[4], const float spec[4], const float dir[3]);
   int SetEarlyRayTermination(float a);
   int SetEmptySpaceSkipping(bool on);
+  int SetKernelTuning(uint32_t batch, uint32_t tile_group, uint32_t xcd_split);
 
   int Paint();
   int CheckForRedraw() const { return frame_complete_ ? 0 : 1; }
@@ -127,6 +128,7 @@ class HipRenderer {
   DevBuf counters_;
   unsigned long long* counters_h_ = nullptr;  // pinned
   uint64_t gpu_total_ = 0, gpu_budget_ = 0;
+  int compute_units_ = 0;
 
   // volume
   bool has_volume_ = false;
@@ -168,6 +170,9 @@ class HipRenderer {
   float light_[15] = {0};
   float term_alpha_ = 0.99f;
   bool skip_ = true;
+  int batch_ = 8;
+  int tile_group_ = 4;
+  int xcd_split_ = 0;
   int target_ = TVK_RT_INTERACTIVE;
 
   // progress

@@ -271,6 +271,12 @@ int tvk_set_empty_space_skipping(tvk_renderer* r, int on) {
   return r->impl.SetEmptySpaceSkipping(on != 0);
 }
 
+int tvk_set_kernel_tuning(tvk_renderer* r, uint32_t batch, uint32_t tile_group,
+                          uint32_t xcd_split) {
+  TVK_GUARD(r);
+  return r->impl.SetKernelTuning(batch, tile_group, xcd_split);
+}
+
 int tvk_paint(tvk_renderer* r) {
   TVK_GUARD(r);
   TVK_TRY(r->impl.Paint());

@@ -44,6 +44,10 @@ struct RaycastParams {
   float nominal_rate;   // user rate: nominal samples per LOD-0 voxel
   float term_alpha;     // early ray termination threshold
   int32_t interp;       // 0 nearest, 1 trilinear
+  int32_t batch;        // samples gathered per step of the march (1, 2, 4, 8)
+  int32_t tile_group;   // consecutive ray tiles dealt to one XCD in turn
+  int32_t xcd_split;    // 0: round-robin tile runs; rx: rx x (8/rx) frame regions
+  int32_t compute_units;  // CUs of the device (sizes the persistent grid)
   int32_t skip;         // empty-space skipping on/off
   LodInfo level;        // the level being sampled
   int32_t brick;        // payload edge B

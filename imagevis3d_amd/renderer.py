@@ -343,6 +343,12 @@ class Renderer:
     def setEmptySpaceSkipping(self, on: bool) -> None:
         call("tvk_set_empty_space_skipping", self._h, int(on))
 
+    def setKernelTuning(self, samples_per_batch: int = 8, tile_group: int = 4,
+                        xcd_split: int = 0) -> None:
+        """Launch tuning picked by the autotuner (no image change)."""
+        call("tvk_set_kernel_tuning", self._h, int(samples_per_batch), int(tile_group),
+             int(xcd_split))
+
     # --------------------------------------------------------------- data
     def getDataset(self) -> Dataset:
         return Dataset(self, self._path)

@@ -220,6 +220,16 @@ int tvk_set_early_ray_termination(tvk_renderer* r, float alpha_threshold);
  * same either way, only the work changes. */
 int tvk_set_empty_space_skipping(tvk_renderer* r, int enabled);
 
+/* Kernel launch tuning (no counterpart in the reference; chosen by the
+ * autotuner): samples gathered per march step (1, 2, 4 or 8; default 8);
+ * how 8x8 ray tiles are dealt to the 8 XCDs: xcd_split 0 deals runs of
+ * tile_group (1..4096) consecutive tiles round-robin, xcd_split rx in
+ * {1, 2, 4, 8} gives each XCD one region of an rx x (8/rx) split of the
+ * frame.  Idle XCDs steal tiles either way.  The image does not depend on
+ * any of them. */
+int tvk_set_kernel_tuning(tvk_renderer* r, uint32_t samples_per_batch, uint32_t tile_group,
+                          uint32_t xcd_split);
+
 /* ---------------------------------------------------------------- render */
 /* renderer.paint() -- RenderWindow.cpp:2464, BatchRenderer.lua:35.  Renders
  * one subframe into the device framebuffer.  In RT_CAPTURE/RT_HEADLESS the

@@ -22,6 +22,11 @@ hipError_t launch_extract_bricks(const void* level, int bpv, const int64_t dims[
                                  int64_t slot_stride, int B, int apron, float* brick_range,
                                  uint32_t* scratch_minmax, hipStream_t stream);
 
+// Empty-space cell ranges of one flat level: out[(cz*ncy + cy)*ncx + cx] =
+// min | max << 16 over voxels [c*C - 1, (c+1)*C] (clamped to the level).
+hipError_t launch_cell_ranges(const void* level, int bpv, const int64_t dims[3], int C,
+                              uint32_t* out, hipStream_t stream);
+
 // 1D histogram of raw values, n_bins = 2^(8*bpv), u64 counts (cleared here).
 hipError_t launch_histogram_1d(const void* vol, int bpv, int64_t n, unsigned long long* hist,
                                hipStream_t stream);

@@ -110,6 +110,33 @@ __global__ __launch_bounds__(kThreads) void extract_bricks_kernel(
   }
 }
 
+// One wave per empty-space cell: lanes stride over the cell's (C+2)^3
+// neighbourhood row by row, then reduce.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void cell_range_kernel(const T* __restrict__ level,
+                                                              int64_t nx, int64_t ny, int64_t nz,
+                                                              int C, int ncx, int ncy, int ncz,
+                                                              uint32_t* __restrict__ out) {
+  const int wave = (int)(blockIdx.x * (kThreads / 64) + threadIdx.x / 64);
+  const int lane = (int)(threadIdx.x & 63);
+  if (wave >= ncx * ncy * ncz) return;
+  const int cx = wave % ncx, cy = (wave / ncx) % ncy, cz = wave / (ncx * ncy);
+  const int64_t x0 = clamp64((int64_t)cx * C - 1, 0, nx - 1), x1 = clamp64((int64_t)(cx + 1) * C, 0, nx - 1);
+  const int64_t y0 = clamp64((int64_t)cy * C - 1, 0, ny - 1), y1 = clamp64((int64_t)(cy + 1) * C, 0, ny - 1);
+  const int64_t z0 = clamp64((int64_t)cz * C - 1, 0, nz - 1), z1 = clamp64((int64_t)(cz + 1) * C, 0, nz - 1);
+  const int w = (int)(x1 - x0 + 1), h = (int)(y1 - y0 + 1), dz = (int)(z1 - z0 + 1);
+  uint32_t lo = 0xffffffffu, hi = 0;
+  for (int i = lane; i < w * h * dz; i += 64) {
+    const int x = i % w, y = (i / w) % h, z = i / (w * h);
+    const uint32_t v = level[((z0 + z) * ny + (y0 + y)) * nx + (x0 + x)];
+    lo = min(lo, v);
+    hi = max(hi, v);
+  }
+  lo = wave_min(lo);
+  hi = wave_max(hi);
+  if (lane == 0) out[wave] = lo | (hi << 16);
+}
+
 __global__ void minmax_init_kernel(uint32_t* mm, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
@@ -284,6 +311,21 @@ hipError_t launch_extract_bricks(const void* level, int bpv, const int64_t d[3],
   }
   minmax_finish_kernel<<<(n_jobs + 255) / 256, 256, 0, stream>>>(scratch, jobs, n_jobs,
                                                                   brick_range);
+  return hipGetLastError();
+}
+
+hipError_t launch_cell_ranges(const void* level, int bpv, const int64_t d[3], int C, uint32_t* out,
+                              hipStream_t stream) {
+  const int ncx = (int)((d[0] + C - 1) / C), ncy = (int)((d[1] + C - 1) / C),
+            ncz = (int)((d[2] + C - 1) / C);
+  const int64_t cells = (int64_t)ncx * ncy * ncz;
+  const int64_t blocks = (cells + (kThreads / 64) - 1) / (kThreads / 64);
+  if (bpv == 1)
+    cell_range_kernel<uint8_t><<<(unsigned)blocks, kThreads, 0, stream>>>(
+        (const uint8_t*)level, d[0], d[1], d[2], C, ncx, ncy, ncz, out);
+  else
+    cell_range_kernel<uint16_t><<<(unsigned)blocks, kThreads, 0, stream>>>(
+        (const uint16_t*)level, d[0], d[1], d[2], C, ncx, ncy, ncz, out);
   return hipGetLastError();
 }
 

@@ -15,10 +15,12 @@
 //  - The 1D TF is staged once per work-group into LDS (up to 4096 RGBA
 //    entries = 64 KB), so the two TF reads per sample are LDS reads and the
 //    vector-memory path only carries voxel gathers.
-//  - Each ray walks the brick grid of the chosen LOD; per brick it looks up
-//    the pool slot in the page table, skips the brick when the TF is
-//    transparent over the brick's [min, max] (exact: such samples add
-//    nothing), and otherwise marches the brick's samples with trilinear
+//  - Each ray walks a grid of march units of the chosen LOD: empty-space
+//    cells (default 16^3 voxels, always inside one brick) when skipping is
+//    on, whole bricks otherwise.  A cell is skipped when the TF is
+//    transparent over the value range its samples can read (exact: such
+//    samples add nothing); otherwise the ray looks up the brick's pool slot
+//    in the page table and marches the cell's samples with trilinear
 //    fetches from the slot, whose apron keeps every fetch brick-local.
 //    Samples are gathered BATCH at a time so a wave keeps several gathers in
 //    flight.  Rays stop at the early-termination opacity.
@@ -144,14 +146,20 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
     const int n = (int)ceilf(L / step);
     const float lastlen = fmaxf(L - (float)(n - 1) * step, 0.f);
     const LodInfo& lv = P.level;
-    const float invB = 1.f / (float)P.brick;
-    float od[3], oo[3], bl0[3];
+    // The ray walks march units of P.cell level voxels: the empty-space
+    // cells when skipping (a cell lies inside one brick), else the bricks.
+    const int U = P.cell;
+    const int per_brick = P.brick / U;
+    const float invU = 1.f / (float)U;
+    float od[3], oo[3], ul0[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       od[i] = d[i] * lv.to_level[i];
       oo[i] = o[i] * lv.to_level[i];
-      bl0[i] = (float)P.brick / lv.to_level[i];  // brick edge in LOD-0 voxels
+      ul0[i] = (float)U / lv.to_level[i];  // unit edge in LOD-0 voxels
     }
+    const int ucount[3] = {(lv.dims[0] + U - 1) / U, (lv.dims[1] + U - 1) / U,
+                           (lv.dims[2] + U - 1) / U};
     const float tfmax = (float)(P.n_tf - 1);
     const int S = P.slot_edge;
     const int S2 = S * S;
@@ -159,30 +167,35 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
     while (k < n) {
       const float len0 = (k == n - 1) ? lastlen : step;
       const float tk = fmaf((float)k, step, t0) + 0.5f * len0;
-      int b[3];
+      int c[3];
 #pragma unroll
       for (int i = 0; i < 3; ++i)
-        b[i] = clampi((int)floorf(fmaf(tk, od[i], oo[i]) * invB), 0, lv.grid[i] - 1);
+        c[i] = clampi((int)floorf(fmaf(tk, od[i], oo[i]) * invU), 0, ucount[i] - 1);
       float tb = kInf;
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
         if (d[i] != 0.f) {
-          const float face = (float)(d[i] > 0.f ? b[i] + 1 : b[i]) * bl0[i];
+          const float face = (float)(d[i] > 0.f ? c[i] + 1 : c[i]) * ul0[i];
           tb = fminf(tb, (face - o[i]) * inv[i]);
         }
       }
       int kend = (int)ceilf((tb - t0) / step - 0.5f);
       kend = min(max(kend, k + 1), n);
+      if (P.skip) {
+        const uint32_t r = P.cell_range[lv.cell_base + (c[2] * ucount[1] + c[1]) * ucount[0] + c[0]];
+        const int i0 = (int)clampf((float)(r & 0xffffu) * P.tf_scale, 0.f, tfmax);
+        const int i1 = min((int)clampf((float)(r >> 16) * P.tf_scale, 0.f, tfmax) + 1, P.n_tf - 1);
+        if (P.tf_prefix[i1 + 1] - P.tf_prefix[i0] == 0) {
+          k = kend;
+          continue;
+        }
+      }
+      int b[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) b[i] = c[i] / per_brick;
       const int gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
       const int slot = P.page_table[gid];
-      bool empty = slot < 0;
-      if (!empty && P.skip) {
-        const float2 r = ((const float2*)P.brick_range)[gid];
-        const int i0 = (int)clampf(r.x * P.tf_scale, 0.f, tfmax);
-        const int i1 = min((int)clampf(r.y * P.tf_scale, 0.f, tfmax) + 1, P.n_tf - 1);
-        empty = (P.tf_prefix[i1 + 1] - P.tf_prefix[i0]) == 0;
-      }
-      if (empty) {
+      if (slot < 0) {  // not resident: nothing to sample
         k = kend;
         continue;
       }

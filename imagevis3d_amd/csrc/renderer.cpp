@@ -127,6 +127,7 @@ HipRenderer::~HipRenderer() {
     pool_.page_table.reset();
     pool_.brick_range.reset();
     scratch_.reset();
+    cell_range_.reset();
     tf_dev_.reset();
     tf_prefix_dev_.reset();
     framebuffer_.reset();
@@ -230,21 +231,27 @@ int HipRenderer::SetVolume(const void* voxels, const uint64_t dims[3], uint32_t 
   for (int l = 1; l < nl; ++l)
     for (int i = 0; i < 3; ++i) ld[l][i] = (ld[l - 1][i] + 1) / 2;
 
+  // empty-space cell edge: 16 voxels when it divides the brick, else the brick
+  const int cell = (brick % 16 == 0) ? 16 : (int)brick;
   LodInfo lods[kMaxLod];
-  int total = 0;
+  int total = 0, total_cells = 0;
   uint64_t level_bytes = 0;
   for (int l = 0; l < nl; ++l) {
     for (int i = 0; i < 3; ++i) {
       lods[l].dims[i] = (int32_t)ld[l][i];
       lods[l].grid[i] = (int32_t)((ld[l][i] + brick - 1) / brick);
       lods[l].to_level[i] = (float)((double)ld[l][i] / (double)dims[i]);
+      lods[l].cells[i] = (int32_t)((ld[l][i] + cell - 1) / cell);
     }
     lods[l].brick_base = total;
+    lods[l].cell_base = total_cells;
     total += lods[l].grid[0] * lods[l].grid[1] * lods[l].grid[2];
+    total_cells += lods[l].cells[0] * lods[l].cells[1] * lods[l].cells[2];
     level_bytes += ld[l][0] * ld[l][1] * ld[l][2] * bpv;
   }
   const uint64_t slot_bytes = ((S * S * S * bpv) + 255) & ~255ull;
   const uint64_t need = level_bytes + (uint64_t)total * slot_bytes + (uint64_t)total * 12 +
+                        (uint64_t)total_cells * 4 +
                         (uint64_t)total * sizeof(BrickJob) + (size_t)width_ * height_ * 4;
   if (need > gpu_budget_) {
     set_last_error("volume needs " + std::to_string(need) + " bytes of device memory, budget is " +
@@ -298,6 +305,12 @@ int HipRenderer::SetVolume(const void* voxels, const uint64_t dims[3], uint32_t 
                                   pool_.slots.p, stride, (int)brick, (int)apron,
                                   pool_.brick_range.as<float>(), scratch_.as<uint32_t>(), stream_));
   }
+  TVK_HIP(cell_range_.alloc((size_t)total_cells * sizeof(uint32_t)));
+  for (int l = 0; l < nl; ++l) {
+    const int64_t d[3] = {lods[l].dims[0], lods[l].dims[1], lods[l].dims[2]};
+    TVK_HIP(launch_cell_ranges(levels_[l].p, bpv, d, cell,
+                               cell_range_.as<uint32_t>() + lods[l].cell_base, stream_));
+  }
   pool_.page_table_h.resize(total);
   for (int i = 0; i < total; ++i) pool_.page_table_h[i] = i;
   TVK_HIP(hipMemcpyAsync(pool_.page_table.p, pool_.page_table_h.data(), total * sizeof(int32_t),
@@ -315,6 +328,7 @@ int HipRenderer::SetVolume(const void* voxels, const uint64_t dims[3], uint32_t 
   brick_ = (int)brick;
   apron_ = (int)apron;
   slot_edge_ = (int)S;
+  cell_ = cell;
   for (int l = 0; l < nl; ++l) lods_[l] = lods[l];
   total_bricks_ = total;
   lod_min_ = 0;
@@ -715,6 +729,8 @@ void HipRenderer::build_params(RaycastParams* p, int lod) const {
   p->pool = pool_.slots.p;
   p->page_table = pool_.page_table.as<int32_t>();
   p->brick_range = pool_.brick_range.as<float>();
+  p->cell_range = cell_range_.as<uint32_t>();
+  p->cell = skip_ ? cell_ : brick_;
   p->tf = tf_dev_.as<float>();
   p->tf_prefix = tf_prefix_dev_.as<int32_t>();
   p->framebuffer = framebuffer_.as<uint32_t>();
@@ -874,7 +890,8 @@ int HipRenderer::MemStats(tvk_mem_stats* out) {
   out->gpu_mem = gpu_total_;
   out->gpu_budget = gpu_budget_;
   uint64_t a = framebuffer_.bytes + counters_.bytes + tf_dev_.bytes + tf_prefix_dev_.bytes +
-               pool_.slots.bytes + pool_.page_table.bytes + pool_.brick_range.bytes + scratch_.bytes;
+               pool_.slots.bytes + pool_.page_table.bytes + pool_.brick_range.bytes + scratch_.bytes +
+               cell_range_.bytes;
   for (auto& l : levels_) a += l.bytes;
   out->allocated_gpu_mem = a;
   out->pool_slots = pool_.n_slots;

@@ -141,6 +141,8 @@ class HipRenderer {
   DevBuf levels_[kMaxLod];
   BrickPool pool_;
   DevBuf scratch_;
+  DevBuf cell_range_;  // empty-space cell ranges of every level
+  int cell_ = 16;      // empty-space cell edge (divides the brick size)
 
   // transfer functions
   int render_mode_ = TVK_RM_1DTRANS;

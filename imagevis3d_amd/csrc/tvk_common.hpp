@@ -9,6 +9,9 @@
 //   page table   int32 per brick of every LOD (LOD-major, then z, y, x):
 //                slot index, or -1 when the brick is not resident.
 //   brick ranges float2 (min, max) per brick over its stored slot.
+//   cell ranges  uint32 (min | max << 16) per empty-space cell of edge C
+//                (C divides the brick size), over the voxels
+//                [c*C - 1, (c+1)*C] that trilinear samples in the cell read.
 //   TF           n_tf float4 (RGBA) + int32 prefix count of alpha > 0.
 //   framebuffer  W*H uint32 RGBA8 (r in the low byte), row 0 at the top.
 #pragma once
@@ -24,6 +27,8 @@ struct LodInfo {
   int32_t grid[3];      // bricks of this level
   int32_t brick_base;   // index of the level's first brick in the page table
   float to_level[3];    // level texel coords per LOD-0 voxel (N_l / N_0)
+  int32_t cells[3];     // empty-space cells of this level (edge = cell size)
+  int32_t cell_base;    // index of the level's first cell in the cell-range grid
 };
 
 // Arguments of one raycast launch (passed by value as kernel arguments).
@@ -61,6 +66,8 @@ struct RaycastParams {
   const void* pool;
   const int32_t* page_table;
   const float* brick_range;     // 2 floats per brick
+  const uint32_t* cell_range;   // per cell: min | max << 16 (raw values)
+  int32_t cell;                 // march unit edge in level voxels (divides brick)
   const float* tf;              // 4 floats per entry
   const int32_t* tf_prefix;     // n_tf + 1
   uint32_t* framebuffer;

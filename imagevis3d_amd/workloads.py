@@ -105,11 +105,18 @@ def set_camera(r, cam: Camera) -> None:
     r.setOrthoViewEnabled(cam.ortho, cam.ortho_half_height)
 
 
+def skip_cell(brick: int) -> int:
+    """Edge of the renderer's empty-space cells for a brick size."""
+    return 16 if brick % 16 == 0 else brick
+
+
 def oracle_kwargs(w: Workload, cam: Camera, lod: int = 0, level_dims=None) -> dict:
-    """Keyword arguments of oracle.make_params for the same render."""
+    """Keyword arguments of oracle.make_params for the same render.  The
+    oracle counts skipped samples on the renderer's empty-space cells: edge
+    skip_cell(brick), value range over the cell plus one voxel (apron 1)."""
     return dict(dims0=w.dims, lod=lod, view_pos=cam.view_pos, view_dir=cam.view_dir,
                 up_dir=cam.up_dir, fov=cam.fov, ortho=cam.ortho,
                 ortho_half_height=cam.ortho_half_height, width=w.width, height=w.height,
                 rate=w.rate, term_alpha=w.term_alpha, interp=1, bg_top=w.bg_top,
                 bg_bottom=w.bg_bottom, bg_alpha=1.0,
-                brick_size=w.brick if w.skip else 0, apron=w.apron)
+                brick_size=skip_cell(w.brick) if w.skip else 0, apron=1)

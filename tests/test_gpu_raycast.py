@@ -128,7 +128,7 @@ def test_empty_space_skipping_exact_and_counts():
     assert np.array_equal(a, b)
     assert sa.evaluated_samples < 0.5 * sb.evaluated_samples
     assert sa.bricks_skipped > 0 and sb.bricks_skipped == 0
-    ref8, ns, fl, _ = oracle_ref(vol, w, w.cameras[0], brick_size=16)
+    ref8, ns, fl, _ = oracle_ref(vol, w, w.cameras[0])
     assert_parity(a, ref8, fl, "skip")
     assert abs(int(sa.evaluated_samples) - int(ns.sum())) <= 2e-3 * ns.sum() + 64
     r.cleanup()
@@ -179,7 +179,7 @@ def test_nearest_interpolant():
     r.set1DTrans(w.tf)
     r.setInterpolant(0)
     img, _ = render_gpu(r, w, cam)
-    ref8, _, fl, _ = oracle_ref(vol, w, cam, interp=0, brick_size=32)
+    ref8, _, fl, _ = oracle_ref(vol, w, cam, interp=0)
     assert_parity(img, ref8, fl, "nearest")
     r.cleanup()
 
@@ -220,7 +220,7 @@ def test_region_rotation_matches_oracle():
     r.setRotation4x4(R)
     r.setTranslation4x4(T)
     img, _ = render_gpu(r, w, cam)
-    ref8, _, fl, _ = oracle_ref(vol, w, cam, model=R @ T, brick_size=32)
+    ref8, _, fl, _ = oracle_ref(vol, w, cam, model=R @ T)
     assert_parity(img, ref8, fl, "region rotation+translation")
     r.cleanup()
 

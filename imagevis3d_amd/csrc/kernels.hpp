@@ -46,7 +46,8 @@ hipError_t launch_histogram_2d(const void* vol, int bpv, const int64_t dims[3],
 hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v,
                            hipStream_t stream);
 
-// One raycast of a 1D-TF frame.
-hipError_t launch_raycast_1d(const RaycastParams& p, int bpv, hipStream_t stream);
+// One raycast frame (1D TF, 2D TF, lighting or isosurface per p.mode and
+// p.lighting).
+hipError_t launch_raycast(const RaycastParams& p, int bpv, hipStream_t stream);
 
 }  // namespace tvk

@@ -89,9 +89,78 @@ __device__ inline float fetch(const T* __restrict__ base, int S, int S2, float g
   return ld(base + (iz * S2 + iy * S + ix));
 }
 
+// Render modes of the kernel template: 1D TF without a gradient, any mode
+// that needs the gradient (1D TF with lighting, 2D TF), isosurface.
+constexpr int kMode1D = 0, kModeGrad = 1, kModeIso = 2;
+
+// Sample at level texel coordinate q through the page table (clamp to
+// edge): used off the march's current brick (isosurface refinement and
+// normal), where the neighbouring brick may hold the point.
+template <typename T, bool LINEAR>
+__device__ inline float fetch_level(const RaycastParams& P, float qx, float qy, float qz) {
+  const LodInfo& lv = P.level;
+  const float q[3] = {qx, qy, qz};
+  int b[3];
+  float g[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float qi = clampf(q[i], 0.f, (float)lv.dims[i]);
+    b[i] = clampi((int)floorf(qi / (float)P.brick), 0, lv.grid[i] - 1);
+    g[i] = qi - (float)(b[i] * P.brick - P.apron) - (LINEAR ? 0.5f : 0.f);
+  }
+  const int slot = P.page_table[lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0]];
+  if (slot < 0) return 0.f;
+  const T* base = (const T*)P.pool + (int64_t)slot * P.slot_elems;
+  return fetch<T, LINEAR>(base, P.slot_edge, P.slot_edge * P.slot_edge, g[0], g[1], g[2]);
+}
+
+// 1D TF: linear between the two entries around value * tf_scale.
+__device__ inline void tf1d_lookup(const RaycastParams& P, const float4* __restrict__ tf4, float v,
+                                   float c[4]) {
+  const float x = clampf(v * P.tf_scale, 0.f, (float)(P.n_tf - 1));
+  const int i = (int)x;
+  const float f = x - (float)i;
+  const float4 e0 = tf4[i], e1 = tf4[min(i + 1, P.n_tf - 1)];
+  c[0] = fmaf(e1.x - e0.x, f, e0.x);
+  c[1] = fmaf(e1.y - e0.y, f, e0.y);
+  c[2] = fmaf(e1.z - e0.z, f, e0.z);
+  c[3] = fmaf(e1.w - e0.w, f, e0.w);
+}
+
+// 2D TF (value x gradient magnitude), bilinear, value fastest.
+__device__ inline void tf2d_lookup(const RaycastParams& P, float v, float gm, float c[4]) {
+  const int W = P.tf2d_w, H = P.tf2d_h;
+  const float x = clampf(v * P.tf_scale, 0.f, (float)(W - 1));
+  const float y = clampf(gm * P.tf2d_row_scale, 0.f, (float)(H - 1));
+  const int i = (int)x, j = (int)y;
+  const float fx = x - (float)i, fy = y - (float)j;
+  const int i1 = min(i + 1, W - 1), j1 = min(j + 1, H - 1);
+  const float4* t = (const float4*)P.tf2d;
+  const float4 a = t[j * W + i], b = t[j * W + i1], e = t[j1 * W + i], f = t[j1 * W + i1];
+  const float ax[4] = {a.x, a.y, a.z, a.w}, bx[4] = {b.x, b.y, b.z, b.w};
+  const float ex[4] = {e.x, e.y, e.z, e.w}, fx4[4] = {f.x, f.y, f.z, f.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float r0 = fmaf(bx[k] - ax[k], fx, ax[k]);
+    const float r1 = fmaf(fx4[k] - ex[k], fx, ex[k]);
+    c[k] = fmaf(r1 - r0, fy, r0);
+  }
+}
+
+// Two-sided Blinn-Phong: c*(ka + kd|N.L|) + ks|N.H|^shininess with
+// N = g / sqrt(|g|^2 + 1e-8); Lh = light direction and half vector.
+__device__ inline void shade(const RaycastParams& P, const float g[3], const float Lh[6], float c[4]) {
+  const float s = 1.f / sqrtf(g[0] * g[0] + g[1] * g[1] + g[2] * g[2] + 1e-8f);
+  const float nl = fabsf((g[0] * Lh[0] + g[1] * Lh[1] + g[2] * Lh[2]) * s);
+  const float nh = fabsf((g[0] * Lh[3] + g[1] * Lh[4] + g[2] * Lh[5]) * s);
+  const float sp = nh > 0.f ? exp2f(P.shininess * log2f(nh)) : 0.f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) c[i] = c[i] * fmaf(P.kd[i], nl, P.ka[i]) + P.ks[i] * sp;
+}
+
 // Render the 64 rays of ray tile `tile` (lane = ray within the 8x8 tile).
-// `tf4` points at the TF (LDS or global).
-template <typename T, bool LINEAR, int BATCH>
+// `tf4` points at the 1D TF (LDS or global).
+template <typename T, bool LINEAR, int BATCH, int MODE>
 __device__ inline void render_tile(const RaycastParams& P, const float4* __restrict__ tf4, int tile,
                                    int lane, uint32_t& n_eval, uint32_t& n_nom) {
   const int tx = tile % P.tiles_x, ty = tile / P.tiles_x;
@@ -163,6 +232,20 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
     const float tfmax = (float)(P.n_tf - 1);
     const int S = P.slot_edge;
     const int S2 = S * S;
+    float Lh[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // light direction, half vector
+    if (MODE != kMode1D) {
+      float h[3], hl = 0.f;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        Lh[i] = P.light[i];
+        h[i] = P.light[i] - d[i];
+        hl += h[i] * h[i];
+      }
+      hl = sqrtf(hl);
+      if (hl > 1e-12f)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) Lh[3 + i] = h[i] / hl;
+    }
     int k = 0;
     while (k < n) {
       const float len0 = (k == n - 1) ? lastlen : step;
@@ -183,9 +266,16 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
       kend = min(max(kend, k + 1), n);
       if (P.skip) {
         const uint32_t r = P.cell_range[lv.cell_base + (c[2] * ucount[1] + c[1]) * ucount[0] + c[0]];
-        const int i0 = (int)clampf((float)(r & 0xffffu) * P.tf_scale, 0.f, tfmax);
-        const int i1 = min((int)clampf((float)(r >> 16) * P.tf_scale, 0.f, tfmax) + 1, P.n_tf - 1);
-        if (P.tf_prefix[i1 + 1] - P.tf_prefix[i0] == 0) {
+        bool empty;
+        if (MODE == kModeIso) {
+          empty = (float)(r >> 16) < P.iso_value;
+        } else {
+          // TF entries (1D) or columns (2D) the cell's value range reaches
+          const int i0 = (int)clampf((float)(r & 0xffffu) * P.tf_scale, 0.f, tfmax);
+          const int i1 = min((int)clampf((float)(r >> 16) * P.tf_scale, 0.f, tfmax) + 1, P.n_tf - 1);
+          empty = P.tf_prefix[i1 + 1] - P.tf_prefix[i0] == 0;
+        }
+        if (empty) {
           k = kend;
           continue;
         }
@@ -204,9 +294,110 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
 #pragma unroll
       for (int i = 0; i < 3; ++i)
         g0[i] = oo[i] - (float)(b[i] * P.brick - P.apron) - (LINEAR ? 0.5f : 0.f);
-      // Batched march: gather BATCH samples (clamped to the brick's last
-      // sample), look all of them up in the TF, then composite in order.
       bool live = true;
+      if (MODE == kModeGrad) {
+        // 1D TF with lighting, or 2D TF (value x |gradient|) with or
+        // without lighting: 7 trilinear fetches per sample.
+        for (; k < kend && live; k += BATCH) {
+          float v[BATCH], len[BATCH], g[BATCH][3];
+#pragma unroll
+          for (int j = 0; j < BATCH; ++j) {
+            const int kk = min(k + j, kend - 1);
+            len[j] = (kk == n - 1) ? lastlen : step;
+            const float t = fmaf((float)kk, step, t0) + 0.5f * len[j];
+            const float gx = fmaf(t, od[0], g0[0]), gy = fmaf(t, od[1], g0[1]),
+                        gz = fmaf(t, od[2], g0[2]);
+            v[j] = fetch<T, LINEAR>(base, S, S2, gx, gy, gz);
+            // trilinear coordinates of the same point for the gradient
+            const float lx = LINEAR ? gx : gx - 0.5f, ly = LINEAR ? gy : gy - 0.5f,
+                        lz = LINEAR ? gz : gz - 0.5f;
+            g[j][0] = (fetch<T, true>(base, S, S2, lx + 1.f, ly, lz) -
+                       fetch<T, true>(base, S, S2, lx - 1.f, ly, lz)) * P.grad_scale[0];
+            g[j][1] = (fetch<T, true>(base, S, S2, lx, ly + 1.f, lz) -
+                       fetch<T, true>(base, S, S2, lx, ly - 1.f, lz)) * P.grad_scale[1];
+            g[j][2] = (fetch<T, true>(base, S, S2, lx, ly, lz + 1.f) -
+                       fetch<T, true>(base, S, S2, lx, ly, lz - 1.f)) * P.grad_scale[2];
+          }
+#pragma unroll
+          for (int j = 0; j < BATCH; ++j) {
+            if (k + j < kend && live) {
+              ++n_eval;
+              float c[4];
+              if (P.mode == 1) {
+                const float gm = sqrtf(g[j][0] * g[j][0] + g[j][1] * g[j][1] + g[j][2] * g[j][2]);
+                tf2d_lookup(P, v[j], gm, c);
+              } else {
+                tf1d_lookup(P, tf4, v[j], c);
+              }
+              if (c[3] > 0.f) {
+                if (P.lighting) shade(P, g[j], Lh, c);
+                const float ac = c[3] >= 1.f ? 1.f : 1.f - exp2f(len[j] * log2f(1.f - c[3]));
+                const float w = (1.f - A) * ac;
+                C0 = fmaf(w, c[0], C0);
+                C1 = fmaf(w, c[1], C1);
+                C2 = fmaf(w, c[2], C2);
+                A += w;
+                live = A < P.term_alpha;
+              }
+            }
+          }
+        }
+        k = live ? kend : n;
+        continue;
+      }
+      if (MODE == kModeIso) {
+        // first sample at or above the iso value; refine against the
+        // previous sample, shade at the refined point
+        for (; k < kend && live; k += BATCH) {
+          float v[BATCH];
+#pragma unroll
+          for (int j = 0; j < BATCH; ++j) {
+            const int kk = min(k + j, kend - 1);
+            const float lenj = (kk == n - 1) ? lastlen : step;
+            const float t = fmaf((float)kk, step, t0) + 0.5f * lenj;
+            v[j] = fetch<T, LINEAR>(base, S, S2, fmaf(t, od[0], g0[0]), fmaf(t, od[1], g0[1]),
+                                    fmaf(t, od[2], g0[2]));
+          }
+#pragma unroll
+          for (int j = 0; j < BATCH; ++j) {
+            if (k + j < kend && live) {
+              ++n_eval;
+              if (v[j] >= P.iso_value) {
+                const int kh = k + j;
+                const float lenh = (kh == n - 1) ? lastlen : step;
+                const float th0 = fmaf((float)kh, step, t0) + 0.5f * lenh;
+                float th = th0;
+                if (kh > 0) {
+                  const float tp = fmaf((float)(kh - 1), step, t0) + 0.5f * step;
+                  const float vp = fetch_level<T, LINEAR>(P, (o[0] + tp * d[0]) * lv.to_level[0],
+                                                          (o[1] + tp * d[1]) * lv.to_level[1],
+                                                          (o[2] + tp * d[2]) * lv.to_level[2]);
+                  if (vp < P.iso_value && v[j] > vp)
+                    th = tp + (P.iso_value - vp) / (v[j] - vp) * (th0 - tp);
+                }
+                const float qx = (o[0] + th * d[0]) * lv.to_level[0];
+                const float qy = (o[1] + th * d[1]) * lv.to_level[1];
+                const float qz = (o[2] + th * d[2]) * lv.to_level[2];
+                float gh[3];
+                gh[0] = (fetch_level<T, true>(P, qx + 1.f, qy, qz) - fetch_level<T, true>(P, qx - 1.f, qy, qz)) * P.grad_scale[0];
+                gh[1] = (fetch_level<T, true>(P, qx, qy + 1.f, qz) - fetch_level<T, true>(P, qx, qy - 1.f, qz)) * P.grad_scale[1];
+                gh[2] = (fetch_level<T, true>(P, qx, qy, qz + 1.f) - fetch_level<T, true>(P, qx, qy, qz - 1.f)) * P.grad_scale[2];
+                float c[4] = {P.iso_color[0], P.iso_color[1], P.iso_color[2], 1.f};
+                shade(P, gh, Lh, c);
+                C0 = c[0];
+                C1 = c[1];
+                C2 = c[2];
+                A = 1.f;
+                live = false;
+              }
+            }
+          }
+        }
+        k = live ? kend : n;
+        continue;
+      }
+      // Batched 1D march: gather BATCH samples (clamped to the brick's last
+      // sample), look all of them up in the TF, then composite in order.
       for (; k < kend; k += BATCH) {
         float v[BATCH], len[BATCH];
 #pragma unroll
@@ -259,8 +450,8 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
 
 // Persistent raycast: every wave loops over ray tiles taken from the
 // per-XCD queues P.counters[2 + xcd] until all queues are drained.
-template <typename T, bool LINEAR, int BATCH, bool TF_LDS>
-__global__ __launch_bounds__(kThreads, 2) void raycast1d_kernel(const RaycastParams P) {
+template <typename T, bool LINEAR, int BATCH, bool TF_LDS, int MODE>
+__global__ __launch_bounds__(kThreads, 2) void raycast_kernel(const RaycastParams P) {
   extern __shared__ float4 s_tf[];  // n_tf entries when TF_LDS
   const float4* tf4 = (const float4*)P.tf;
   if (TF_LDS) {
@@ -299,7 +490,8 @@ __global__ __launch_bounds__(kThreads, 2) void raycast1d_kernel(const RaycastPar
       int tile;
       if (P.xcd_split == 0) tile = ((pos / G) * 8 + x) * G + (pos % G);
       else tile = (ry0 + pos / rw) * P.tiles_x + rx0 + pos % rw;
-      if (tile < P.tiles_x * P.tiles_y) render_tile<T, LINEAR, BATCH>(P, tf4, tile, lane, n_eval, n_nom);
+      if (tile < P.tiles_x * P.tiles_y)
+        render_tile<T, LINEAR, BATCH, MODE>(P, tf4, tile, lane, n_eval, n_nom);
     }
   }
   // one atomic per wave for each counter
@@ -314,9 +506,9 @@ __global__ __launch_bounds__(kThreads, 2) void raycast1d_kernel(const RaycastPar
   }
 }
 
-template <typename T, bool LINEAR, int BATCH, bool TF_LDS>
+template <typename T, bool LINEAR, int BATCH, bool TF_LDS, int MODE>
 hipError_t launch_one(const RaycastParams& p, hipStream_t stream) {
-  auto kern = raycast1d_kernel<T, LINEAR, BATCH, TF_LDS>;
+  auto kern = raycast_kernel<T, LINEAR, BATCH, TF_LDS, MODE>;
   const size_t lds = TF_LDS ? (size_t)p.n_tf * sizeof(float4) : 0;
   int per_cu = 0;
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads, lds);
@@ -328,30 +520,36 @@ hipError_t launch_one(const RaycastParams& p, hipStream_t stream) {
   return hipGetLastError();
 }
 
-template <typename T, bool LINEAR, int BATCH>
-hipError_t launch_tf(const RaycastParams& p, hipStream_t stream) {
-  if (p.n_tf <= kLdsTfEntries) return launch_one<T, LINEAR, BATCH, true>(p, stream);
-  return launch_one<T, LINEAR, BATCH, false>(p, stream);
-}
-
-template <int BATCH>
-hipError_t launch_batch(const RaycastParams& p, int bpv, hipStream_t stream) {
-  if (bpv == 1)
-    return p.interp ? launch_tf<uint8_t, true, BATCH>(p, stream)
-                    : launch_tf<uint8_t, false, BATCH>(p, stream);
-  return p.interp ? launch_tf<uint16_t, true, BATCH>(p, stream)
-                  : launch_tf<uint16_t, false, BATCH>(p, stream);
+template <typename T, bool LINEAR>
+hipError_t launch_typed(const RaycastParams& p, hipStream_t stream) {
+  if (p.mode == 2) return launch_one<T, LINEAR, 4, false, kModeIso>(p, stream);
+  if (p.mode == 1) return launch_one<T, LINEAR, 2, false, kModeGrad>(p, stream);
+  const bool lds = p.n_tf <= kLdsTfEntries;
+  if (p.lighting)
+    return lds ? launch_one<T, LINEAR, 2, true, kModeGrad>(p, stream)
+               : launch_one<T, LINEAR, 2, false, kModeGrad>(p, stream);
+  switch (p.batch) {
+    case 1:
+      return lds ? launch_one<T, LINEAR, 1, true, kMode1D>(p, stream)
+                 : launch_one<T, LINEAR, 1, false, kMode1D>(p, stream);
+    case 2:
+      return lds ? launch_one<T, LINEAR, 2, true, kMode1D>(p, stream)
+                 : launch_one<T, LINEAR, 2, false, kMode1D>(p, stream);
+    case 4:
+      return lds ? launch_one<T, LINEAR, 4, true, kMode1D>(p, stream)
+                 : launch_one<T, LINEAR, 4, false, kMode1D>(p, stream);
+    default:
+      return lds ? launch_one<T, LINEAR, 8, true, kMode1D>(p, stream)
+                 : launch_one<T, LINEAR, 8, false, kMode1D>(p, stream);
+  }
 }
 
 }  // namespace
 
-hipError_t launch_raycast_1d(const RaycastParams& p, int bpv, hipStream_t stream) {
-  switch (p.batch) {
-    case 1: return launch_batch<1>(p, bpv, stream);
-    case 2: return launch_batch<2>(p, bpv, stream);
-    case 4: return launch_batch<4>(p, bpv, stream);
-    default: return launch_batch<8>(p, bpv, stream);
-  }
+hipError_t launch_raycast(const RaycastParams& p, int bpv, hipStream_t stream) {
+  if (bpv == 1)
+    return p.interp ? launch_typed<uint8_t, true>(p, stream) : launch_typed<uint8_t, false>(p, stream);
+  return p.interp ? launch_typed<uint16_t, true>(p, stream) : launch_typed<uint16_t, false>(p, stream);
 }
 
 }  // namespace tvk

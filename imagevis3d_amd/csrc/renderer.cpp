@@ -460,9 +460,19 @@ int HipRenderer::Set1DTrans(const float* rgba, uint32_t n) {
   return TVK_OK;
 }
 
-int HipRenderer::Set2DTrans(const float*, uint32_t, uint32_t, float) {
-  set_last_error("2D transfer functions are not implemented by this backend yet");
-  return TVK_ERR_ARG;
+int HipRenderer::Set2DTrans(const float* rgba, uint32_t w, uint32_t h, float max_grad) {
+  if (!rgba || w < 1 || h < 1 || w > 65536 || h > 4096 || (uint64_t)w * h > (1ull << 24))
+    return arg_fail("2D TF size out of range");
+  if (!(max_grad > 0.f) || !isfinite(max_grad)) return arg_fail("2D TF max gradient must be > 0");
+  for (uint64_t i = 0; i < 4ull * w * h; ++i)
+    if (!isfinite(rgba[i])) return arg_fail("2D TF holds a non-finite value");
+  tf2d_.assign(rgba, rgba + 4ull * w * h);
+  tf2d_w_ = w;
+  tf2d_h_ = h;
+  tf2d_max_grad_ = max_grad;
+  tf2d_dirty_ = true;
+  invalidate();
+  return TVK_OK;
 }
 
 int HipRenderer::SetIsoValue(float v) {
@@ -481,6 +491,23 @@ int HipRenderer::SetIsosurfaceColor(float r, float g, float b) {
 }
 
 int HipRenderer::upload_tf() {
+  if (tf2d_dirty_) {
+    // 2D raster + prefix count over columns holding any alpha > 0
+    const size_t W = tf2d_w_, H = tf2d_h_;
+    std::vector<int32_t> prefix(W + 1, 0);
+    for (size_t i = 0; i < W; ++i) {
+      int any = 0;
+      for (size_t j = 0; j < H && !any; ++j) any = tf2d_[4 * (j * W + i) + 3] > 0.f;
+      prefix[i + 1] = prefix[i] + any;
+    }
+    if (tf2d_dev_.bytes < W * H * 16) TVK_HIP(tf2d_dev_.alloc(W * H * 16));
+    if (tf2d_prefix_dev_.bytes < (W + 1) * 4) TVK_HIP(tf2d_prefix_dev_.alloc((W + 1) * 4));
+    TVK_HIP(hipMemcpyAsync(tf2d_dev_.p, tf2d_.data(), W * H * 16, hipMemcpyHostToDevice, stream_));
+    TVK_HIP(hipMemcpyAsync(tf2d_prefix_dev_.p, prefix.data(), (W + 1) * 4, hipMemcpyHostToDevice,
+                           stream_));
+    TVK_HIP(hipStreamSynchronize(stream_));
+    tf2d_dirty_ = false;
+  }
   if (!tf_dirty_) return TVK_OK;
   const size_t n = tf1d_.size() / 4;
   std::vector<int32_t> prefix(n + 1, 0);
@@ -639,7 +666,17 @@ int HipRenderer::check_ready() const {
   if (!has_volume_) return state_fail("paint before a dataset is loaded");
   if (width_ == 0) return state_fail("paint before resize");
   if (render_mode_ == TVK_RM_1DTRANS && tf1d_.empty()) return state_fail("paint before a 1D TF is set");
+  if (render_mode_ == TVK_RM_2DTRANS && tf2d_.empty()) return state_fail("paint before a 2D TF is set");
   return TVK_OK;
+}
+
+double HipRenderer::value_max() const {
+  // the largest value of the finest level (the ranges of its bricks)
+  double m = 0.0;
+  const LodInfo& L = lods_[0];
+  const int nb = L.grid[0] * L.grid[1] * L.grid[2];
+  for (int b = 0; b < nb; ++b) m = std::max(m, (double)pool_.brick_range_h[2 * (L.brick_base + b) + 1]);
+  return m > 0.0 ? m : 1.0;
 }
 
 void HipRenderer::build_params(RaycastParams* p, int lod) const {
@@ -720,7 +757,7 @@ void HipRenderer::build_params(RaycastParams* p, int lod) const {
   p->slot_edge = slot_edge_;
   p->slot_elems = (int64_t)(pool_.slot_bytes / (bits_ / 8));
   p->n_tf = (int32_t)(tf1d_.size() / 4);
-  p->tf_scale = bias_scale_tf_ ? (float)((double)(p->n_tf - 1) / (double)((1u << bits_) - 1)) : 1.0f;
+  p->tf_scale = bias_scale_tf_ ? (float)((double)(p->n_tf - 1) / value_max()) : 1.0f;
   for (int i = 0; i < 3; ++i) {
     p->bg_top[i] = bg_top_[i];
     p->bg_bottom[i] = bg_bottom_[i];
@@ -735,14 +772,38 @@ void HipRenderer::build_params(RaycastParams* p, int lod) const {
   p->tf_prefix = tf_prefix_dev_.as<int32_t>();
   p->framebuffer = framebuffer_.as<uint32_t>();
   p->counters = counters_.as<unsigned long long>();
+
+  // render mode, gradient units, lighting
+  p->mode = render_mode_ == TVK_RM_2DTRANS ? 1 : (render_mode_ == TVK_RM_ISOSURFACE ? 2 : 0);
+  p->lighting = (lighting_ && p->mode == 0) ? 1 : 0;
+  const double maxval = (double)((1u << bits_) - 1);
+  for (int i = 0; i < 3; ++i) p->grad_scale[i] = (float)((double)lods_[lod].to_level[i] / (2.0 * maxval));
+  for (int i = 0; i < 3; ++i) {
+    p->ka[i] = light_[i];
+    p->kd[i] = light_[4 + i];
+    p->ks[i] = light_[8 + i];
+  }
+  p->shininess = light_[11];
+  double ldir[3] = {light_[12], light_[13], light_[14]}, lv[3];
+  vox_dir(ldir, lv);
+  normalize(lv);
+  for (int i = 0; i < 3; ++i) p->light[i] = (float)lv[i];
+  if (p->mode == 1) {
+    p->lighting = lighting_ ? 1 : 0;
+    p->tf2d = tf2d_dev_.as<float>();
+    p->tf2d_w = (int32_t)tf2d_w_;
+    p->tf2d_h = (int32_t)tf2d_h_;
+    p->tf2d_row_scale = (float)((double)(tf2d_h_ - 1) / (double)tf2d_max_grad_);
+    p->n_tf = (int32_t)tf2d_w_;  // the skip test runs over the value columns
+    p->tf_scale = bias_scale_tf_ ? (float)((double)(tf2d_w_ - 1) / value_max()) : 1.0f;
+    p->tf_prefix = tf2d_prefix_dev_.as<int32_t>();
+  }
+  p->iso_value = iso_value_;
+  for (int i = 0; i < 3; ++i) p->iso_color[i] = iso_color_[i];
 }
 
 int HipRenderer::Paint() {
   if (int e = check_ready()) return e;
-  if (render_mode_ != TVK_RM_1DTRANS) {
-    set_last_error("only the 1D transfer-function render mode is implemented by this backend yet");
-    return TVK_ERR_ARG;
-  }
   TVK_HIP(hipSetDevice(device_));
   if (int e = upload_tf()) return e;
   const int lo = std::min<int>((int)lod_min_, n_lod_ - 1);
@@ -758,7 +819,7 @@ int HipRenderer::Paint() {
   TVK_HIP(hipEventRecord(ev_[0], stream_));
   TVK_HIP(launch_fill_u64(counters_.as<unsigned long long>(), 10, 0ull, stream_));
   TVK_HIP(hipEventRecord(ev_[1], stream_));
-  TVK_HIP(launch_raycast_1d(p, (int)bits_ / 8, stream_));
+  TVK_HIP(launch_raycast(p, (int)bits_ / 8, stream_));
   TVK_HIP(hipEventRecord(ev_[2], stream_));
   TVK_HIP(hipMemcpyAsync(counters_h_, counters_.p, 2 * sizeof(unsigned long long),
                          hipMemcpyDeviceToHost, stream_));
@@ -769,12 +830,23 @@ int HipRenderer::Paint() {
   const int nb = L.grid[0] * L.grid[1] * L.grid[2];
   uint32_t skipped = 0;
   if (skip_) {
-    const int ntf = (int)(tf1d_.size() / 4);
+    // bricks whose whole value range is transparent (below the iso value)
+    const int ntf = p.n_tf;
     std::vector<int32_t> prefix(ntf + 1, 0);
-    for (int i = 0; i < ntf; ++i) prefix[i + 1] = prefix[i] + (tf1d_[4 * i + 3] > 0.f ? 1 : 0);
+    for (int i = 0; i < ntf; ++i) {
+      int any = 0;
+      if (p.mode == 0) any = tf1d_[4 * i + 3] > 0.f;
+      else if (p.mode == 1)
+        for (uint32_t j = 0; j < tf2d_h_ && !any; ++j) any = tf2d_[4 * ((size_t)j * tf2d_w_ + i) + 3] > 0.f;
+      prefix[i + 1] = prefix[i] + any;
+    }
     for (int b = 0; b < nb; ++b) {
       const float mn = pool_.brick_range_h[2 * (L.brick_base + b)];
       const float mx = pool_.brick_range_h[2 * (L.brick_base + b) + 1];
+      if (p.mode == 2) {
+        if (mx < p.iso_value) ++skipped;
+        continue;
+      }
       const int i0 = (int)std::min(std::max(mn * p.tf_scale, 0.f), (float)(ntf - 1));
       const int i1 = std::min((int)std::min(std::max(mx * p.tf_scale, 0.f), (float)(ntf - 1)) + 1, ntf - 1);
       if (prefix[i1 + 1] - prefix[i0] == 0) ++skipped;

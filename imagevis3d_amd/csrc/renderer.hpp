@@ -112,6 +112,7 @@ class HipRenderer {
   int check_ready() const;
   int upload_tf();
   void build_params(RaycastParams* p, int lod) const;
+  double value_max() const;
   void invalidate() { frame_complete_ = false; next_lod_ = -1; }
 
   // construction flags (tuvok.renderer.new)
@@ -151,6 +152,11 @@ class HipRenderer {
   DevBuf tf_dev_, tf_prefix_dev_;
   float iso_value_ = 0.f;
   float iso_color_[3] = {1.f, 1.f, 1.f};
+  std::vector<float> tf2d_;
+  uint32_t tf2d_w_ = 0, tf2d_h_ = 0;
+  float tf2d_max_grad_ = 1.f;
+  bool tf2d_dirty_ = false;
+  DevBuf tf2d_dev_, tf2d_prefix_dev_;
 
   // camera
   double view_pos_[3] = {0.0, 0.0, 2.0};
@@ -169,7 +175,9 @@ class HipRenderer {
   float bg_top_[3] = {0.f, 0.f, 0.f}, bg_bottom_[3] = {0.f, 0.f, 0.f};
   float bg_alpha_ = 1.f;
   bool lighting_ = false;
-  float light_[15] = {0};
+  // ambient rgba, diffuse rgba, specular rgb + shininess, light direction
+  float light_[15] = {0.1f, 0.1f, 0.1f, 1.f, 0.9f, 0.9f, 0.9f, 1.f,
+                      0.3f, 0.3f, 0.3f, 16.f, 0.f, 0.f, 1.f};
   float term_alpha_ = 0.99f;
   bool skip_ = true;
   int batch_ = 8;

@@ -63,6 +63,17 @@ struct RaycastParams {
   float tf_scale;
   float bg_top[3], bg_bottom[3];
   float bg_alpha;
+  // render mode: 0 = 1D TF, 1 = 2D TF, 2 = isosurface; lighting for 0/1
+  int32_t mode;
+  int32_t lighting;
+  float grad_scale[3];  // gradient units: (N_l/N_0) / (2 (2^bits - 1)) per axis
+  float ka[3], kd[3], ks[3], shininess;
+  float light[3];       // unit light direction in voxel space
+  const float* tf2d;    // tf2d_w x tf2d_h float4, value fastest
+  int32_t tf2d_w, tf2d_h;
+  float tf2d_row_scale; // (tf2d_h - 1) / max gradient
+  float iso_value;      // raw data units
+  float iso_color[3];
   const void* pool;
   const int32_t* page_table;
   const float* brick_range;     // 2 floats per brick

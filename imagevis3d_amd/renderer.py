@@ -275,6 +275,9 @@ class Renderer:
         call("tvk_set_1d_trans", self._h, a.ctypes.data, a.shape[0])
 
     def set2DTrans(self, rgba: np.ndarray, max_grad: float) -> None:
+        """Upload a 2D TF raster (H gradient rows x W value columns x RGBA);
+        `max_grad` is the gradient magnitude of the last row (the 2D
+        histogram's maximum)."""
         a = np.ascontiguousarray(np.asarray(rgba, np.float32))
         h, w = a.shape[:2]
         call("tvk_set_2d_trans", self._h, a.ctypes.data, w, h, float(max_grad))

@@ -93,5 +93,29 @@ def gaussian_bumps_tf(n: int = 4096, seed: int = 2, bumps: int = 3) -> np.ndarra
     return tf
 
 
+def gaussian_lobes_2d(w: int = 256, h: int = 256, seed: int = 3, lobes: int = 4) -> np.ndarray:
+    """2D TF raster (h gradient rows x w value columns x RGBA) of `lobes`
+    seeded Gaussian lobes with compact support (+-3 sigma per axis): centre
+    U[0.15, 0.9] x U[0.1, 0.8], sigma U[0.03, 0.12] x U[0.05, 0.25] of the
+    axis, peak alpha U[0.02, 0.1]; colour weighted by the lobe profiles."""
+    rng = np.random.default_rng(seed)
+    mx = rng.uniform(0.15, 0.9, lobes) * (w - 1)
+    my = rng.uniform(0.1, 0.8, lobes) * (h - 1)
+    sx = rng.uniform(0.03, 0.12, lobes) * w
+    sy = rng.uniform(0.05, 0.25, lobes) * h
+    amp = rng.uniform(0.02, 0.1, lobes)
+    col = rng.uniform(0.2, 1.0, (lobes, 3))
+    x = np.arange(w, dtype=np.float64)[None, :, None]
+    y = np.arange(h, dtype=np.float64)[:, None, None]
+    dx = (x - mx) / sx
+    dy = (y - my) / sy
+    prof = np.exp(-0.5 * (dx ** 2 + dy ** 2))
+    prof[(np.abs(dx) > 3) | (np.abs(dy) > 3)] = 0.0
+    alpha = np.clip((prof * amp).sum(-1), 0.0, 1.0)
+    wsum = prof.sum(-1, keepdims=True)
+    rgb = np.where(wsum > 0, (prof @ col) / np.maximum(wsum, 1e-30), 0.0)
+    return np.concatenate([rgb, alpha[..., None]], -1).astype(np.float32)
+
+
 def constant_tf(n: int, rgba) -> np.ndarray:
     return np.tile(np.asarray(rgba, np.float32), (n, 1))

@@ -170,8 +170,9 @@ int tvk_compute_histogram_2d(tvk_renderer* r, uint64_t* hist, uint32_t n_value_b
 int tvk_set_render_mode(tvk_renderer* r, int mode);
 /* TransferFunction1D colour data (Q1DTransferFunction.cpp:254-258) and
  * tuvok.gpu.changed1DTrans (Q1DTransferFunction.cpp:465-469): n RGBA floats.
- * Data value v indexes entry v * (n-1) / (2^bits - 1) when biasAndScaleTF is
- * set, otherwise entry v (clamped to n-1). */
+ * Data value v indexes entry v * (n-1) / vmax when biasAndScaleTF is set
+ * (vmax = the largest value of the volume, so the TF spans the data range
+ * of e.g. 12-bit data in 16-bit voxels), otherwise entry v (clamped to n-1). */
 int tvk_set_1d_trans(tvk_renderer* r, const float* rgba, uint32_t n);
 /* TransferFunction2D raster (Q2DTransferFunction.cpp:426-470): w (value) x h
  * (gradient magnitude) RGBA floats, value fastest; max_grad is the gradient
@@ -210,7 +211,13 @@ int tvk_set_interpolant(tvk_renderer* r, int interpolant);
 int tvk_set_lod_limits(tvk_renderer* r, uint32_t min_lod, uint32_t max_lod);
 /* setBGColors -- RenderWindow.cpp:401-406 (vertical gradient, top/bottom). */
 int tvk_set_bg_colors(tvk_renderer* r, const float top[3], const float bottom[3]);
-/* setLightingEnabled / setLightColors -- RenderWindow.cpp:2076,408-415. */
+/* setLightingEnabled / setLightColors -- RenderWindow.cpp:2076,408-415.
+ * Two-sided Blinn-Phong on the TF colour c with the central-difference
+ * gradient normal: c*(ambient + diffuse*|N.L|) + specular*|N.H|^s, where
+ * s = specular[3] (shininess) and light_dir points towards the light in
+ * world space.  Defaults: ambient 0.1, diffuse 0.9, specular 0.3, s = 16,
+ * light_dir (0, 0, 1).  Lighting applies to the 1D and 2D TF modes; the
+ * isosurface mode is always lit. */
 int tvk_set_lighting_enabled(tvk_renderer* r, int enabled);
 int tvk_set_light_colors(tvk_renderer* r, const float ambient[4], const float diffuse[4],
                          const float specular[4], const float light_dir[3]);

@@ -80,26 +80,90 @@ static void tf_lookup(const oracle_render_params* p, double v, double rgba[4]) {
   for (int c = 0; c < 4; ++c) rgba[c] = (double)p->tf[4 * i + c] * (1.0 - f) + (double)p->tf[4 * j + c] * f;
 }
 
+/* 2D TF lookup, bilinear: x = value * tf_scale (columns), y = |g| / max_grad
+ * * (rows - 1), both clamped to the raster (value fastest). */
+static void tf2d_lookup(const oracle_render_params* p, double v, double gmag, double rgba[4]) {
+  const int W = p->tf2d_w, H = p->tf2d_h;
+  double x = clampd(v * p->tf_scale, 0.0, (double)(W - 1));
+  double y = p->max_grad > 0 ? clampd(gmag / p->max_grad * (H - 1), 0.0, (double)(H - 1)) : 0.0;
+  int i = (int)floor(x), j = (int)floor(y);
+  double fx = x - i, fy = y - j;
+  int i1 = i + 1 < W ? i + 1 : W - 1, j1 = j + 1 < H ? j + 1 : H - 1;
+  for (int c = 0; c < 4; ++c) {
+    double a = p->tf2d[4 * ((int64_t)j * W + i) + c], b = p->tf2d[4 * ((int64_t)j * W + i1) + c];
+    double e = p->tf2d[4 * ((int64_t)j1 * W + i) + c], f = p->tf2d[4 * ((int64_t)j1 * W + i1) + c];
+    double r0 = a + (b - a) * fx, r1 = e + (f - e) * fx;
+    rgba[c] = r0 + (r1 - r0) * fy;
+  }
+}
+
+/* Gradient at level texel position q: central differences of the trilinear
+ * interpolant one level voxel apart, in value/(2^bits-1) per LOD-0 voxel. */
+static void gradient(const oracle_render_params* p, const double q[3], double g[3]) {
+  const double maxval = p->bits == 8 ? 255.0 : 65535.0;
+  const double lsc[3] = {(double)p->nx / p->n0x, (double)p->ny / p->n0y, (double)p->nz / p->n0z};
+  for (int a = 0; a < 3; ++a) {
+    double qp[3] = {q[0], q[1], q[2]}, qm[3] = {q[0], q[1], q[2]};
+    qp[a] += 1.0;
+    qm[a] -= 1.0;
+    g[a] = (sample_lin(p, qp) - sample_lin(p, qm)) / (2.0 * maxval) * lsc[a];
+  }
+}
+
+/* Two-sided Blinn-Phong on colour c: c*(ka + kd|N.L|) + ks|N.H|^shininess
+ * with N = g / sqrt(|g|^2 + eps^2) (eps = 1e-4: flat regions fade to the
+ * ambient term instead of taking a noisy direction). */
+static void shade(const oracle_render_params* p, const double g[3], const double V[3],
+                  const double L[3], double rgb[3]) {
+  const double eps = 1e-4;
+  double s = 1.0 / sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2] + eps * eps);
+  double N[3] = {g[0] * s, g[1] * s, g[2] * s};
+  double H[3] = {L[0] + V[0], L[1] + V[1], L[2] + V[2]};
+  double hl = sqrt(H[0] * H[0] + H[1] * H[1] + H[2] * H[2]);
+  if (hl > 1e-12) for (int i = 0; i < 3; ++i) H[i] /= hl;
+  double nl = fabs(N[0] * L[0] + N[1] * L[1] + N[2] * L[2]);
+  double nh = fabs(N[0] * H[0] + N[1] * H[1] + N[2] * H[2]);
+  double sp = pow(nh, p->shininess);
+  for (int i = 0; i < 3; ++i) rgb[i] = rgb[i] * (p->ambient[i] + p->diffuse[i] * nl) + p->specular[i] * sp;
+}
+
 /* TF-empty flag per brick of the sampled level: brick empty iff every TF
- * entry its [min, max] range can reach has alpha == 0 (exact skip). */
+ * entry (1D) or column (2D, any row) its [min, max] range can reach has
+ * alpha == 0 (exact skip); for isosurfaces, iff its max is below the iso
+ * value. */
 uint8_t* oracle_brick_empty_flags(const oracle_render_params* p, int nb[3]) {
   int B = p->brick_size, a = p->apron;
   nb[0] = (p->nx + B - 1) / B; nb[1] = (p->ny + B - 1) / B; nb[2] = (p->nz + B - 1) / B;
   int64_t cnt = (int64_t)nb[0] * nb[1] * nb[2];
   uint8_t* e = (uint8_t*)malloc(cnt);
+  const int ncol = p->mode == 1 ? p->tf2d_w : p->n_tf;
+  uint8_t* col = (uint8_t*)calloc(ncol > 0 ? ncol : 1, 1);
+  for (int i = 0; i < ncol; ++i) {
+    if (p->mode == 1) {
+      for (int j = 0; j < p->tf2d_h; ++j)
+        if (p->tf2d[4 * ((int64_t)j * p->tf2d_w + i) + 3] > 0.0f) { col[i] = 1; break; }
+    } else if (p->mode == 0) {
+      col[i] = p->tf[4 * i + 3] > 0.0f;
+    }
+  }
   for (int bz = 0; bz < nb[2]; ++bz)
     for (int by = 0; by < nb[1]; ++by)
       for (int bx = 0; bx < nb[0]; ++bx) {
         double mn, mx;
         oracle_brick_minmax(p->voxels, p->bits, p->nx, p->ny, p->nz, B, a, bx, by, bz, &mn, &mx);
-        double x0 = clampd(mn * p->tf_scale, 0, p->n_tf - 1), x1 = clampd(mx * p->tf_scale, 0, p->n_tf - 1);
-        int i0 = (int)floor(x0), i1 = (int)floor(x1) + 1;
-        if (i1 > p->n_tf - 1) i1 = p->n_tf - 1;
         int empty = 1;
-        for (int i = i0; i <= i1; ++i)
-          if (p->tf[4 * i + 3] > 0.0f) { empty = 0; break; }
+        if (p->mode == 2) {
+          empty = mx < p->iso_value;
+        } else {
+          double x0 = clampd(mn * p->tf_scale, 0, ncol - 1), x1 = clampd(mx * p->tf_scale, 0, ncol - 1);
+          int i0 = (int)floor(x0), i1 = (int)floor(x1) + 1;
+          if (i1 > ncol - 1) i1 = ncol - 1;
+          for (int i = i0; i <= i1; ++i)
+            if (col[i]) { empty = 0; break; }
+        }
         e[(bz * nb[1] + by) * nb[0] + bx] = (uint8_t)empty;
       }
+  free(col);
   return e;
 }
 
@@ -113,6 +177,16 @@ int oracle_render_1d(const oracle_render_params* p, double* out, uint32_t* out_s
   const double hi[3] = {p->n0x, p->n0y, p->n0z};
   const double lsc[3] = {(double)p->nx / p->n0x, (double)p->ny / p->n0y, (double)p->nz / p->n0z};
   const double step = ldexp(1.0, p->lod) / p->rate;
+  const int grad = p->mode == 1 || p->mode == 2 || p->lighting;
+  /* light direction in voxel space (a direction: linear part only) */
+  double Lv[3] = {0, 0, 1};
+  if (grad) {
+    double lm[3];
+    xform(c.minv, p->light_dir, 0.0, lm);
+    for (int i = 0; i < 3; ++i) Lv[i] = lm[i] * c.vscale[i];
+    norm3(Lv);
+  }
+  const double iso_eps = 1e-2;
   for (int y = 0; y < p->height; ++y) {
     double wy = 1.0 - (y + 0.5) / p->height;
     double bg[3];
@@ -120,6 +194,7 @@ int oracle_render_1d(const oracle_render_params* p, double* out, uint32_t* out_s
     for (int x = 0; x < p->width; ++x) {
       double o[3], d[3];
       cam_ray(p, &c, x, y, o, d);
+      const double V[3] = {-d[0], -d[1], -d[2]};
       double C[3] = {0, 0, 0}, A = 0;
       uint32_t ns = 0;
       uint8_t flags = 0;
@@ -142,11 +217,36 @@ int oracle_render_1d(const oracle_render_params* p, double* out, uint32_t* out_s
             int nf = 0;
             double v = p->interp ? sample_lin(p, q) : sample_nn(p, q, &nf);
             if (nf) flags |= 2;
-            double rgba[4];
-            tf_lookup(p, v, rgba);
             ++ns;
+            if (p->mode == 2) {
+              /* first sample at or above the iso value; the hit is refined
+               * linearly against the previous sample (re-fetched, so the
+               * result does not depend on skipping) */
+              if (fabs(v - p->iso_value) < iso_eps) flags |= 4;
+              if (v < p->iso_value) continue;
+              double th = t;
+              if (k > 0) {
+                double tp = t0 + ((double)(k - 1) + 0.5) * step, qp[3];
+                for (int i = 0; i < 3; ++i) qp[i] = (o[i] + tp * d[i]) * lsc[i];
+                double vp = p->interp ? sample_lin(p, qp) : sample_nn(p, qp, &nf);
+                if (fabs(vp - p->iso_value) < iso_eps) flags |= 4;
+                if (vp < p->iso_value && v > vp) th = tp + (p->iso_value - vp) / (v - vp) * (t - tp);
+              }
+              double qh[3], g[3], rgb[3] = {p->iso_color[0], p->iso_color[1], p->iso_color[2]};
+              for (int i = 0; i < 3; ++i) qh[i] = (o[i] + th * d[i]) * lsc[i];
+              gradient(p, qh, g);
+              shade(p, g, V, Lv, rgb);
+              for (int i = 0; i < 3; ++i) C[i] = rgb[i];
+              A = 1.0;
+              break;
+            }
+            double rgba[4], g[3] = {0, 0, 0};
+            if (grad) gradient(p, q, g);
+            if (p->mode == 1) tf2d_lookup(p, v, sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]), rgba);
+            else tf_lookup(p, v, rgba);
             double a = rgba[3];
             if (a <= 0.0) continue;
+            if (p->lighting) shade(p, g, V, Lv, rgba);
             double ac = a >= 1.0 ? 1.0 : 1.0 - pow(1.0 - a, len);
             double w = (1.0 - A) * ac;
             for (int i = 0; i < 3; ++i) C[i] += w * rgba[i];

@@ -54,13 +54,35 @@ typedef struct oracle_render_params {
   /* brick-granular empty-space skipping (sample counting only) */
   int32_t brick_size;      /* payload B of the sampled level, 0 = no bricks */
   int32_t apron;
+  /* render mode: 0 = 1D TF, 1 = 2D TF (value x gradient magnitude),
+   * 2 = isosurface (first hit) */
+  int32_t mode;
+  const float* tf2d;       /* tf2d_w x tf2d_h RGBA floats, value fastest */
+  int32_t tf2d_w, tf2d_h;
+  double max_grad;         /* gradient magnitude of the last TF row */
+  /* Phong lighting (modes 0/1 when lighting != 0; always for mode 2) */
+  int32_t lighting;
+  double ambient[3], diffuse[3], specular[3], shininess;
+  double light_dir[3];     /* world space, towards the light */
+  /* isosurface */
+  double iso_value;        /* raw data units */
+  double iso_color[3];
 } oracle_render_params;
 
 /* FP64 reference raycast.  out_rgba: W*H*4 doubles (composited over the
  * background, [0,1]); out_samples: per-pixel evaluated samples (may be NULL);
  * out_flags: per-pixel bits, 1 = the early-termination decision was within
  * 1e-5 of the threshold, 2 = a nearest-neighbour sample lay within 1e-4 of
- * a voxel face (may be NULL).  Returns 0 on success. */
+ * a voxel face, 4 = an isosurface sample lay within 1e-2 of the iso value
+ * (may be NULL).  Returns 0 on success.
+ *
+ * Gradient (2D TF axis and lighting): central differences of the trilinear
+ * interpolant one level voxel apart, g_a = (f(q+e_a) - f(q-e_a)) /
+ * (2 (2^bits-1)) * N_l/N_0 (units: normalised value per LOD-0 voxel).
+ * Lighting: two-sided Blinn-Phong, c (ka + kd |N.L|) + ks |N.H|^shininess,
+ * N = g / sqrt(|g|^2 + 1e-8), V = -ray, L = light direction, H = (L+V)/|L+V|,
+ * all in voxel space.  Isosurface: first sample with value >= iso, hit
+ * refined linearly against the previous sample, shaded iso colour, opaque. */
 int oracle_render_1d(const oracle_render_params* p, double* out_rgba,
                      uint32_t* out_samples, uint8_t* out_flags);
 

@@ -46,6 +46,11 @@ class RenderParams(C.Structure):
         ("bg_top", C.c_double * 3), ("bg_bottom", C.c_double * 3),
         ("bg_alpha", C.c_double),
         ("brick_size", C.c_int32), ("apron", C.c_int32),
+        ("mode", C.c_int32), ("tf2d", C.c_void_p), ("tf2d_w", C.c_int32), ("tf2d_h", C.c_int32),
+        ("max_grad", C.c_double), ("lighting", C.c_int32),
+        ("ambient", C.c_double * 3), ("diffuse", C.c_double * 3), ("specular", C.c_double * 3),
+        ("shininess", C.c_double), ("light_dir", C.c_double * 3),
+        ("iso_value", C.c_double), ("iso_color", C.c_double * 3),
     ]
 
 
@@ -95,10 +100,18 @@ def make_params(vol: np.ndarray, tf: np.ndarray, *, dims0=None, lod=0, rescale=(
                 fov=60.0, ortho=False, ortho_half_height=0.5, model=None,
                 width=64, height=64, rate=1.0, term_alpha=0.99, interp=1,
                 bg_top=(0, 0, 0), bg_bottom=(0, 0, 0), bg_alpha=1.0,
-                brick_size=0, apron=2):
-    """Parameters of one render; ``vol`` is (nz, ny, nx) x-fastest."""
+                brick_size=0, apron=2, mode=0, tf2d=None, max_grad=1.0, lighting=False,
+                ambient=(0.1, 0.1, 0.1), diffuse=(0.9, 0.9, 0.9), specular=(0.3, 0.3, 0.3),
+                shininess=16.0, light_dir=(0.0, 0.0, 1.0), iso_value=0.0,
+                iso_color=(1.0, 1.0, 1.0)):
+    """Parameters of one render; ``vol`` is (nz, ny, nx) x-fastest.  For
+    mode 1 ``tf2d`` is (H, W, 4) float32 (value along W) and ``tf_scale``
+    maps values to its columns."""
     vol = np.ascontiguousarray(vol)
+    if tf is None:
+        tf = np.zeros((1, 4), np.float32)
     tf = np.ascontiguousarray(tf, dtype=np.float32)
+    t2 = np.ascontiguousarray(tf2d if tf2d is not None else np.zeros((1, 1, 4)), np.float32)
     p = RenderParams()
     p.voxels = _ptr(vol)
     p.bits = bits_of(vol)
@@ -127,8 +140,20 @@ def make_params(vol: np.ndarray, tf: np.ndarray, *, dims0=None, lod=0, rescale=(
     p.bg_alpha = bg_alpha
     p.brick_size = brick_size
     p.apron = apron
+    p.mode = mode
+    p.tf2d = _ptr(t2)
+    p.tf2d_h, p.tf2d_w = t2.shape[:2]
+    p.max_grad = max_grad
+    p.lighting = int(lighting)
+    p.ambient[:] = ambient
+    p.diffuse[:] = diffuse
+    p.specular[:] = specular
+    p.shininess = shininess
+    p.light_dir[:] = light_dir
+    p.iso_value = iso_value
+    p.iso_color[:] = iso_color
     # keep the buffers alive as long as the params
-    p._keep = (vol, tf)
+    p._keep = (vol, tf, t2)
     return p
 
 

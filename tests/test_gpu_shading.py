@@ -1,0 +1,109 @@
+"""2D transfer functions, Phong lighting and isosurfaces on the HIP
+raycaster against the FP64 oracle (same parity bar as the 1D renders;
+isosurface hit/miss decisions within 1e-2 of the iso value are flagged)."""
+import numpy as np
+import pytest
+
+from conftest import assert_parity
+from imagevis3d_amd import RenderMode, transfer, tuvok, workloads
+from imagevis3d_amd.workloads import Camera
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+LIGHT = dict(ambient=(0.15, 0.1, 0.1), diffuse=(0.7, 0.8, 0.9), specular=(0.3, 0.3, 0.2),
+             shininess=12.0, light_dir=(0.3, 0.8, 0.5))
+
+
+def make_renderer(w, vol, bias_scale_tf=False, lighting=False):
+    r = tuvok.renderer.new(tuvok.renderer.types.OpenGL_Raycaster, False, False, False,
+                           bias_scale_tf)
+    assert r.initialize(tuvok.createContext(w.width, w.height, device=0)), r.lastError()
+    r.setRendererTarget(tuvok.renderer.types.RT_Capture)
+    r.setVolume(vol, w.brick, w.apron, 1)
+    if lighting:
+        r.setLightingEnabled(True)
+        r.setLightColors((*LIGHT["ambient"], 1.0), (*LIGHT["diffuse"], 1.0),
+                         (*LIGHT["specular"], LIGHT["shininess"]), LIGHT["light_dir"])
+    return r
+
+
+def render(r, w, cam):
+    workloads.apply(r, w, cam)
+    assert r.paint(), r.lastError()
+    return r.capture(), r.frameStats()
+
+
+def oracle_ref(vol, w, cam, tf, **over):
+    kw = workloads.oracle_kwargs(w, cam)
+    kw.update(over)
+    img, ns, fl = O.render(O.make_params(vol, tf, **kw))
+    return O.to_rgba8(img), ns, fl
+
+
+def medium_small(width=320, height=180):
+    w = workloads.medium(dims=(128, 128, 128), width=width, height=height)
+    w.brick = 32
+    return w, w.volume()
+
+
+@pytest.mark.parametrize("lighting", [False, True])
+def test_2d_tf_parity(lighting):
+    w, vol = medium_small()
+    print(f"case 2D TF lighting={lighting}", flush=True)
+    _, max_grad = O.histogram_2d_exact(vol, 256, 256)
+    tf2 = transfer.gaussian_lobes_2d(256, 256, seed=3)
+    r = make_renderer(w, vol, bias_scale_tf=True, lighting=lighting)
+    r.set2DTrans(tf2, max_grad)
+    r.setRenderMode(RenderMode.RM_2DTRANS)
+    cam = w.cameras[4]
+    img, st = render(r, w, cam)
+    scale = float(np.float32(255.0 / float(vol.max())))
+    ref8, ns, fl = oracle_ref(vol, w, cam, None, mode=1, tf2d=tf2, max_grad=max_grad,
+                              tf_scale=scale, lighting=lighting, **LIGHT)
+    assert_parity(img, ref8, fl, f"2D TF lighting={lighting}")
+    assert abs(int(st.evaluated_samples) - int(ns.sum())) <= 2e-3 * ns.sum() + 64
+    # skipping on/off gives the same image
+    r.setEmptySpaceSkipping(False)
+    assert r.paint()
+    assert np.array_equal(r.capture(), img)
+    r.cleanup()
+
+
+def test_1d_tf_with_lighting_parity():
+    w = workloads.small()
+    w.width, w.height = 256, 192
+    vol = w.volume()
+    cam = Camera((0.7, 0.5, 1.4), (-0.7, -0.5, -1.4), fov=45.0)
+    r = make_renderer(w, vol, lighting=True)
+    r.set1DTrans(w.tf)
+    img, _ = render(r, w, cam)
+    ref8, _, fl = oracle_ref(vol, w, cam, w.tf, lighting=True, **LIGHT)
+    assert_parity(img, ref8, fl, "1D TF + Phong")
+    r.cleanup()
+
+
+@pytest.mark.parametrize("interp", [1, 0])
+def test_isosurface_parity(interp):
+    w, vol = medium_small(256, 144)
+    print(f"case isosurface interp={interp}", flush=True)
+    r = make_renderer(w, vol)
+    r.setLightColors((*LIGHT["ambient"], 1.0), (*LIGHT["diffuse"], 1.0),
+                     (*LIGHT["specular"], LIGHT["shininess"]), LIGHT["light_dir"])
+    r.setRenderMode(RenderMode.RM_ISOSURFACE)
+    r.setIsoValue(2700.0)
+    r.setIsosurfaceColor((0.9, 0.7, 0.4))
+    r.setInterpolant(interp)
+    cam = w.cameras[9]
+    img, st = render(r, w, cam)
+    ref8, ns, fl = oracle_ref(vol, w, cam, None, mode=2, iso_value=2700.0,
+                              iso_color=(0.9, 0.7, 0.4), interp=interp, **LIGHT)
+    hit = ref8[..., 3] == 255
+    assert hit.mean() > 0.05
+    # nearest sampling also flags samples within 1e-4 of a voxel face
+    assert_parity(img, ref8, fl, f"isosurface interp={interp}", flagged_max=255,
+                  max_flagged_frac=0.02 if interp else 0.3)
+    r.setEmptySpaceSkipping(False)
+    assert r.paint()
+    assert np.array_equal(r.capture(), img)
+    r.cleanup()

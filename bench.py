@@ -77,6 +77,9 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=8, help="samples per march step")
     ap.add_argument("--tile-group", type=int, default=4, help="ray tiles per XCD turn")
     ap.add_argument("--xcd-split", type=int, default=0, help="0 or frame regions per row")
+    ap.add_argument("--tile-width", type=int, default=8, help="ray tile width (4, 8, 16)")
+    ap.add_argument("--cell", type=int, default=16, help="empty-space cell edge in voxels")
+    ap.add_argument("--no-skip", action="store_true", help="march whole bricks (no cell skipping)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
@@ -92,6 +95,8 @@ def main() -> int:
     r = tuvok.renderer.new(tuvok.renderer.types.OpenGL_Raycaster, False, False, False, False)
     assert r.initialize(tuvok.createContext(w.width, w.height, device=0)), r.lastError()
     r.setRendererTarget(tuvok.renderer.types.RT_Headless)
+    r.setEmptySpaceCell(args.cell)
+    r.setRayTileShape(args.tile_width)
     t0 = time.perf_counter()
     r.setVolume(vol, w.brick, w.apron, w.n_lod)
     brick_s = time.perf_counter() - t0
@@ -99,6 +104,8 @@ def main() -> int:
     r.setLODLimits((0, 0))
     r.setKernelTuning(args.batch, args.tile_group, args.xcd_split)
     workloads.apply(r, w, w.cameras[0])
+    if args.no_skip:
+        r.setEmptySpaceSkipping(False)
     print(f"[bench] bricked into the device pool in {brick_s:.2f} s; warmup {args.warmup}",
           file=sys.stderr, flush=True)
 
@@ -148,6 +155,9 @@ def main() -> int:
         "config": {"workload": w.name, "volume": "512x512x512 u16", "image": "1920x1080",
                    "views": len(cams), "rate": w.rate, "bricks": "256^3 + 2-voxel apron",
                    "lod": 0, "tf": "4096 x RGBA f32, 3 Gaussian bumps", "fov_deg": 60},
+        "launch": {"samples_per_batch": args.batch, "tile_group": args.tile_group,
+                   "xcd_split": args.xcd_split, "tile_width": args.tile_width,
+                   "cell": 0 if args.no_skip else args.cell},
         "fps": args.steps / wall,
         "evaluated_gsamples_per_s": evaluated / wall / 1e9,
         "nominal_samples_per_frame": nominal // args.steps,

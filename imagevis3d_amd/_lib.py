@@ -45,7 +45,27 @@ class MemStats(C.Structure):
     ]
 
 
+class View(C.Structure):
+    """tvk_view: one view of a batched launch."""
+    _fields_ = [
+        ("view_pos", C.c_float * 3), ("view_dir", C.c_float * 3), ("up_dir", C.c_float * 3),
+        ("fov_deg", C.c_float), ("ortho", C.c_int32), ("ortho_half_height", C.c_float),
+        ("width", C.c_uint32), ("height", C.c_uint32), ("tf_slot", C.c_uint32),
+        ("rgba", C.c_void_p),
+    ]
+
+
+class ViewStats(C.Structure):
+    _fields_ = [
+        ("evaluated_samples", C.c_uint64), ("nominal_samples", C.c_uint64),
+        ("lod", C.c_uint32), ("reserved", C.c_uint32),
+    ]
+
+
 H = C.c_void_p
+# tvk_brick_source_fn
+BRICK_SOURCE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                           C.c_void_p, C.c_uint64)
 u32, u64, i32, f32, f64 = C.c_uint32, C.c_uint64, C.c_int, C.c_float, C.c_double
 P = C.c_void_p
 F3 = C.POINTER(C.c_float)
@@ -62,6 +82,8 @@ SIGNATURES = {
     "tvk_get_renderer_target": [H, C.POINTER(i32)],
     "tvk_load_dataset": [H, C.c_char_p, i32],
     "tvk_set_volume": [H, P, u64, u64, u64, u32, u32, u32, u32],
+    "tvk_set_paged_volume": [H, u64, u64, u64, u32, u32, u32, u32, C.c_void_p, P, P, u64],
+    "tvk_set_paged_flat_volume": [H, P, u64, u64, u64, u32, u32, u32, u32, u64],
     "tvk_get_lod_level_count": [H, C.POINTER(u32)],
     "tvk_get_domain_size": [H, u32, C.POINTER(u64)],
     "tvk_get_bit_width": [H, C.POINTER(u32)],
@@ -93,6 +115,13 @@ SIGNATURES = {
     "tvk_set_early_ray_termination": [H, f32],
     "tvk_set_empty_space_skipping": [H, i32],
     "tvk_set_kernel_tuning": [H, u32, u32, u32],
+    "tvk_set_ray_tile_shape": [H, u32],
+    "tvk_set_empty_space_cell": [H, u32],
+    "tvk_set_1d_trans_slot": [H, u32, P, u32],
+    "tvk_set_2d_trans_slot": [H, u32, P, u32, u32, f32],
+    "tvk_render_batch": [H, C.POINTER(View), u32, C.POINTER(ViewStats), C.POINTER(f32)],
+    "tvk_get_batch_framebuffer": [H, u32, C.POINTER(C.c_void_p)],
+    "tvk_set_launch_graphs": [H, i32],
     "tvk_paint": [H],
     "tvk_check_for_redraw": [H],
     "tvk_schedule_complete_redraw": [H],

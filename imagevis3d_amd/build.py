@@ -21,6 +21,8 @@ SOURCES = [
     "kernels_brick.hip",
     "kernels_raycast.hip",
     "renderer.cpp",
+    "paging.cpp",
+    "host_volume.cpp",
     "uvf.cpp",
     "volume_gen.cpp",
     "tvk_abi.cpp",

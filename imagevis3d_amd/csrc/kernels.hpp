@@ -27,6 +27,13 @@ hipError_t launch_extract_bricks(const void* level, int bpv, const int64_t dims[
 hipError_t launch_cell_ranges(const void* level, int bpv, const int64_t dims[3], int C,
                               uint32_t* out, hipStream_t stream);
 
+// Cell ranges of the cells inside one uploaded brick, computed from its pool
+// slot (slot edge S = B + 2*apron; `cells` = the level's cell grid; out =
+// the level's cell-range array).
+hipError_t launch_slot_cell_ranges(const void* slot, int bpv, int S, int apron, int B, int C,
+                                   const int brick[3], const int cells[3], uint32_t* out,
+                                   hipStream_t stream);
+
 // 1D histogram of raw values, n_bins = 2^(8*bpv), u64 counts (cleared here).
 hipError_t launch_histogram_1d(const void* vol, int bpv, int64_t n, unsigned long long* hist,
                                hipStream_t stream);
@@ -46,8 +53,25 @@ hipError_t launch_histogram_2d(const void* vol, int bpv, const int64_t dims[3],
 hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v,
                            hipStream_t stream);
 
-// One raycast frame (1D TF, 2D TF, lighting or isosurface per p.mode and
-// p.lighting).
-hipError_t launch_raycast(const RaycastParams& p, int bpv, hipStream_t stream);
+// Empty flags (u8) of n_cells empty-space cells for one TF (mode 0/1: TF
+// prefix counts over n_tf entries or columns; mode 2: iso value).
+hipError_t launch_cell_empty(const uint32_t* cell_range, int64_t n_cells, const int32_t* prefix,
+                             int n_tf, float tf_scale, int mode, float iso_value, uint8_t* out,
+                             hipStream_t stream);
+
+// Kernel variant of a raycast launch: every job of the table shares it.
+struct LaunchInfo {
+  int bpv = 1;          // bytes per voxel of every job's volume
+  int interp = 1;       // 0 nearest, 1 trilinear
+  int mode = 0;         // 0 1D TF, 1 2D TF, 2 isosurface
+  int lighting = 0;     // 1D TF with lighting
+  int batch = 8;        // samples per gather batch (1D TF without lighting)
+  int n_tf_lds = 0;     // > 0: the table's shared 1D TF goes to LDS
+  int compute_units = 0;
+};
+
+// One raycast launch over a device job table `tab` (its header counters
+// must be zeroed first).
+hipError_t launch_raycast(const LaunchInfo& li, const JobTable* tab, hipStream_t stream);
 
 }  // namespace tvk

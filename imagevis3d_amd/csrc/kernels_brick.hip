@@ -137,6 +137,36 @@ __global__ __launch_bounds__(kThreads) void cell_range_kernel(const T* __restric
   if (lane == 0) out[wave] = lo | (hi << 16);
 }
 
+// Cell ranges of one brick from its pool slot (paged pools, after upload):
+// one wave per cell of the brick; the slot's clamp-to-edge apron gives the
+// same values as clamping to the level.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void slot_cell_range_kernel(
+    const T* __restrict__ slot, int S, int apron, int B, int C, int bx, int by, int bz,
+    int ncx, int ncy, int ncz, uint32_t* __restrict__ out) {
+  const int per = B / C;
+  const int wave = (int)(blockIdx.x * (kThreads / 64) + threadIdx.x / 64);
+  const int lane = (int)(threadIdx.x & 63);
+  if (wave >= per * per * per) return;
+  const int cx = bx * per + wave % per, cy = by * per + (wave / per) % per, cz = bz * per + wave / (per * per);
+  if (cx >= ncx || cy >= ncy || cz >= ncz) return;
+  // level voxels [c*C - 1, (c+1)*C] -> slot-local (clamped into the slot)
+  const int x0 = max(cx * C - 1 - (bx * B - apron), 0), x1 = min((cx + 1) * C - (bx * B - apron), S - 1);
+  const int y0 = max(cy * C - 1 - (by * B - apron), 0), y1 = min((cy + 1) * C - (by * B - apron), S - 1);
+  const int z0 = max(cz * C - 1 - (bz * B - apron), 0), z1 = min((cz + 1) * C - (bz * B - apron), S - 1);
+  const int w = x1 - x0 + 1, h = y1 - y0 + 1, dz = z1 - z0 + 1;
+  uint32_t lo = 0xffffffffu, hi = 0;
+  for (int i = lane; i < w * h * dz; i += 64) {
+    const int x = x0 + i % w, y = y0 + (i / w) % h, z = z0 + i / (w * h);
+    const uint32_t v = slot[((int64_t)z * S + y) * S + x];
+    lo = min(lo, v);
+    hi = max(hi, v);
+  }
+  lo = wave_min(lo);
+  hi = wave_max(hi);
+  if (lane == 0) out[((int64_t)cz * ncy + cy) * ncx + cx] = lo | (hi << 16);
+}
+
 __global__ void minmax_init_kernel(uint32_t* mm, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
@@ -326,6 +356,23 @@ hipError_t launch_cell_ranges(const void* level, int bpv, const int64_t d[3], in
   else
     cell_range_kernel<uint16_t><<<(unsigned)blocks, kThreads, 0, stream>>>(
         (const uint16_t*)level, d[0], d[1], d[2], C, ncx, ncy, ncz, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_slot_cell_ranges(const void* slot, int bpv, int S, int apron, int B, int C,
+                                   const int brick[3], const int cells[3], uint32_t* out,
+                                   hipStream_t stream) {
+  const int per = B / C;
+  const int waves = per * per * per;
+  const unsigned blocks = (unsigned)((waves + (kThreads / 64) - 1) / (kThreads / 64));
+  if (bpv == 1)
+    slot_cell_range_kernel<uint8_t><<<blocks, kThreads, 0, stream>>>(
+        (const uint8_t*)slot, S, apron, B, C, brick[0], brick[1], brick[2], cells[0], cells[1],
+        cells[2], out);
+  else
+    slot_cell_range_kernel<uint16_t><<<blocks, kThreads, 0, stream>>>(
+        (const uint16_t*)slot, S, apron, B, C, brick[0], brick[1], brick[2], cells[0], cells[1],
+        cells[2], out);
   return hipGetLastError();
 }
 

@@ -108,8 +108,13 @@ __device__ inline float fetch_level(const RaycastParams& P, float qx, float qy, 
     b[i] = clampi((int)floorf(qi / (float)P.brick), 0, lv.grid[i] - 1);
     g[i] = qi - (float)(b[i] * P.brick - P.apron) - (LINEAR ? 0.5f : 0.f);
   }
-  const int slot = P.page_table[lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0]];
-  if (slot < 0) return 0.f;
+  const int gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
+  const int slot = P.page_table[gid];
+  if (slot < 0) {  // paged pool: request the brick; the frame is re-rendered
+    if (P.requests) P.requests[gid] = 1u;
+    return 0.f;
+  }
+  if (P.touch) P.touch[gid] = P.frame;
   const T* base = (const T*)P.pool + (int64_t)slot * P.slot_elems;
   return fetch<T, LINEAR>(base, P.slot_edge, P.slot_edge * P.slot_edge, g[0], g[1], g[2]);
 }
@@ -164,8 +169,9 @@ template <typename T, bool LINEAR, int BATCH, int MODE>
 __device__ inline void render_tile(const RaycastParams& P, const float4* __restrict__ tf4, int tile,
                                    int lane, uint32_t& n_eval, uint32_t& n_nom) {
   const int tx = tile % P.tiles_x, ty = tile / P.tiles_x;
-  const int px = tx * kTile + (lane & 7);
-  const int py = ty * kTile + (lane >> 3);
+  const int sh = P.tile_shift;
+  const int px = (tx << sh) + (lane & ((1 << sh) - 1));
+  const int py = ty * (64 >> sh) + (lane >> sh);
   if (px >= P.width || py >= P.height) return;
 
   float o[3], d[3];
@@ -264,31 +270,24 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
       }
       int kend = (int)ceilf((tb - t0) / step - 0.5f);
       kend = min(max(kend, k + 1), n);
-      if (P.skip) {
-        const uint32_t r = P.cell_range[lv.cell_base + (c[2] * ucount[1] + c[1]) * ucount[0] + c[0]];
-        bool empty;
-        if (MODE == kModeIso) {
-          empty = (float)(r >> 16) < P.iso_value;
-        } else {
-          // TF entries (1D) or columns (2D) the cell's value range reaches
-          const int i0 = (int)clampf((float)(r & 0xffffu) * P.tf_scale, 0.f, tfmax);
-          const int i1 = min((int)clampf((float)(r >> 16) * P.tf_scale, 0.f, tfmax) + 1, P.n_tf - 1);
-          empty = P.tf_prefix[i1 + 1] - P.tf_prefix[i0] == 0;
-        }
-        if (empty) {
-          k = kend;
-          continue;
-        }
-      }
       int b[3];
 #pragma unroll
       for (int i = 0; i < 3; ++i) b[i] = c[i] / per_brick;
       const int gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
+      // the cell's empty flag and the brick's slot are independent loads
+      const uint8_t empty =
+          P.skip ? P.cell_empty[lv.cell_base + (c[2] * ucount[1] + c[1]) * ucount[0] + c[0]] : 0;
       const int slot = P.page_table[gid];
-      if (slot < 0) {  // not resident: nothing to sample
+      if (empty) {
         k = kend;
         continue;
       }
+      if (slot < 0) {  // not resident: ask the host for it, sample nothing
+        if (P.requests) P.requests[gid] = 1u;
+        k = kend;
+        continue;
+      }
+      if (P.touch) P.touch[gid] = P.frame;
       const T* __restrict__ base = (const T*)P.pool + (int64_t)slot * P.slot_elems;
       float g0[3];  // slot-local texel coordinate at t = 0
 #pragma unroll
@@ -448,108 +447,156 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
       pack_rgba8(fmaf(T1, bg0, C0), fmaf(T1, bg1, C1), fmaf(T1, bg2, C2), fmaf(T1, P.bg_alpha, A));
 }
 
-// Persistent raycast: every wave loops over ray tiles taken from the
-// per-XCD queues P.counters[2 + xcd] until all queues are drained.
+// Empty flag per empty-space cell for one TF: the TF entries (1D) or
+// columns (2D) that the cell's value range reaches hold no alpha > 0, or
+// (isosurface) the cell's maximum is below the iso value.  Such cells
+// contribute nothing to any ray, so the raycast skips them.
+__global__ void cell_empty_kernel(const uint32_t* __restrict__ cell_range, int64_t n_cells,
+                                  const int32_t* __restrict__ prefix, int n_tf, float tf_scale,
+                                  int mode, float iso_value, uint8_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_cells) return;
+  const uint32_t r = cell_range[i];
+  bool empty;
+  if (mode == kModeIso) {
+    empty = (float)(r >> 16) < iso_value;
+  } else {
+    const float tfmax = (float)(n_tf - 1);
+    const int i0 = (int)clampf((float)(r & 0xffffu) * tf_scale, 0.f, tfmax);
+    const int i1 = min((int)clampf((float)(r >> 16) * tf_scale, 0.f, tfmax) + 1, n_tf - 1);
+    empty = prefix[i1 + 1] - prefix[i0] == 0;
+  }
+  out[i] = empty ? 1 : 0;
+}
+
+// Persistent raycast over a job table: every wave loops over ray tiles
+// taken from the per-XCD queues until all queues are drained.  Tile t of
+// the launch belongs to job j with tile_start[j] <= t < tile_start[j + 1].
 template <typename T, bool LINEAR, int BATCH, bool TF_LDS, int MODE>
-__global__ __launch_bounds__(kThreads, 2) void raycast_kernel(const RaycastParams P) {
-  extern __shared__ float4 s_tf[];  // n_tf entries when TF_LDS
-  const float4* tf4 = (const float4*)P.tf;
+__global__ __launch_bounds__(kThreads, 2) void raycast_kernel(const JobTable* __restrict__ tab) {
+  extern __shared__ float4 s_tf[];  // n_tf_lds entries when TF_LDS
+  const JobTableHeader& H = tab->h;
+  const float4* tf_shared = (const float4*)H.tf_lds;
   if (TF_LDS) {
-    for (int i = (int)threadIdx.x; i < P.n_tf; i += kThreads) s_tf[i] = tf4[i];
+    for (int i = (int)threadIdx.x; i < H.n_tf_lds; i += kThreads) s_tf[i] = tf_shared[i];
     __syncthreads();
-    tf4 = s_tf;
+    tf_shared = s_tf;
   }
   const int lane = (int)(threadIdx.x & 63);
   const int xcd = (int)(blockIdx.x & 7);
-  unsigned long long* queues = P.counters + 2;
-  uint32_t n_eval = 0, n_nom = 0;
-  for (int q = 0; q < 8; ++q) {
+  unsigned long long* queues = H.counters;
+  const int n_jobs = H.n_jobs;
+  const int total = H.total_tiles;
+  const int G = H.tile_group;
+  // xcd_split regions apply to one-job launches only
+  const int split = n_jobs == 1 ? H.xcd_split : 0;
+  const int tiles_x0 = tab->jobs[0].tiles_x, tiles_y0 = tab->jobs[0].tiles_y;
+  for (int q = 0; q < kQueues; ++q) {
     const int x = (xcd + q) & 7;  // own queue first, then steal
-    // Queue x enumerates either runs of tile_group tiles dealt round-robin
-    // (xcd_split == 0) or, with xcd_split = rx in {1,2,4,8}, the tiles of
-    // region x of an rx by (8/rx) split of the frame in row-major order.
-    int rx0 = 0, rw = 0, ry0 = 0, rh = 0, slots;
-    const int G = P.tile_group;
-    if (P.xcd_split == 0) {
-      const int ngroups = (P.tiles_x * P.tiles_y + G - 1) / G;
+    // Queue x enumerates either runs of G tiles dealt round-robin
+    // (split == 0) or, with split = rx in {1,2,4,8}, the tiles of region x
+    // of an rx by (8/rx) split of the frame in row-major order.
+    int rx0 = 0, rw = 1, ry0 = 0, slots;
+    if (split == 0) {
+      const int ngroups = (total + G - 1) / G;
       slots = ((ngroups + 7) / 8) * G;
     } else {
-      const int rx = P.xcd_split, ry = 8 / P.xcd_split;
+      const int rx = split, ry = 8 / split;
       const int i = x % rx, jr = x / rx;
-      rx0 = i * P.tiles_x / rx;
-      rw = (i + 1) * P.tiles_x / rx - rx0;
-      ry0 = jr * P.tiles_y / ry;
-      rh = (jr + 1) * P.tiles_y / ry - ry0;
+      rx0 = i * tiles_x0 / rx;
+      rw = (i + 1) * tiles_x0 / rx - rx0;
+      ry0 = jr * tiles_y0 / ry;
+      const int rh = (jr + 1) * tiles_y0 / ry - ry0;
       slots = rw * rh;
     }
     while (true) {
       int pos = 0;
       if (lane == 0) pos = (int)atomicAdd(&queues[x], 1ull);
-      pos = __shfl(pos, 0, 64);
+      pos = __builtin_amdgcn_readfirstlane(__shfl(pos, 0, 64));
       if (pos >= slots) break;
       int tile;
-      if (P.xcd_split == 0) tile = ((pos / G) * 8 + x) * G + (pos % G);
-      else tile = (ry0 + pos / rw) * P.tiles_x + rx0 + pos % rw;
-      if (tile < P.tiles_x * P.tiles_y)
-        render_tile<T, LINEAR, BATCH, MODE>(P, tf4, tile, lane, n_eval, n_nom);
+      if (split == 0) tile = ((pos / G) * 8 + x) * G + (pos % G);
+      else tile = (ry0 + pos / rw) * tiles_x0 + rx0 + pos % rw;
+      if (tile >= total) continue;
+      // the job of this tile (wave-uniform binary search over the prefix)
+      int lo = 0, hi = n_jobs - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab->tile_start[mid] <= tile) lo = mid;
+        else hi = mid - 1;
+      }
+      const int job = __builtin_amdgcn_readfirstlane(lo);
+      const RaycastParams& P = tab->jobs[job];
+      const float4* tf4 = TF_LDS ? tf_shared : (const float4*)P.tf;
+      uint32_t n_eval = 0, n_nom = 0;
+      render_tile<T, LINEAR, BATCH, MODE>(P, tf4, tile - tab->tile_start[job], lane, n_eval, n_nom);
+      // one atomic per tile for each of the job's counters
+      unsigned long long e = n_eval, m = n_nom;
+      for (int off = 32; off > 0; off >>= 1) {
+        e += __shfl_xor(e, off, 64);
+        m += __shfl_xor(m, off, 64);
+      }
+      if (lane == 0) {
+        atomicAdd(&H.counters[kQueues + 2 * job], e);
+        atomicAdd(&H.counters[kQueues + 2 * job + 1], m);
+      }
     }
-  }
-  // one atomic per wave for each counter
-  unsigned long long e = n_eval, m = n_nom;
-  for (int off = 32; off > 0; off >>= 1) {
-    e += __shfl_xor(e, off, 64);
-    m += __shfl_xor(m, off, 64);
-  }
-  if (lane == 0) {
-    atomicAdd(&P.counters[0], e);
-    atomicAdd(&P.counters[1], m);
   }
 }
 
 template <typename T, bool LINEAR, int BATCH, bool TF_LDS, int MODE>
-hipError_t launch_one(const RaycastParams& p, hipStream_t stream) {
+hipError_t launch_one(const LaunchInfo& li, const JobTable* tab, hipStream_t stream) {
   auto kern = raycast_kernel<T, LINEAR, BATCH, TF_LDS, MODE>;
-  const size_t lds = TF_LDS ? (size_t)p.n_tf * sizeof(float4) : 0;
+  const size_t lds = TF_LDS ? (size_t)li.n_tf_lds * sizeof(float4) : 0;
   int per_cu = 0;
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads, lds);
   if (e != hipSuccess) return e;
   if (per_cu < 1) per_cu = 1;
-  int grid = p.compute_units * per_cu;
+  int grid = li.compute_units * per_cu;
   grid = ((grid + 7) / 8) * 8;
-  kern<<<grid, kThreads, lds, stream>>>(p);
+  kern<<<grid, kThreads, lds, stream>>>(tab);
   return hipGetLastError();
 }
 
 template <typename T, bool LINEAR>
-hipError_t launch_typed(const RaycastParams& p, hipStream_t stream) {
-  if (p.mode == 2) return launch_one<T, LINEAR, 4, false, kModeIso>(p, stream);
-  if (p.mode == 1) return launch_one<T, LINEAR, 2, false, kModeGrad>(p, stream);
-  const bool lds = p.n_tf <= kLdsTfEntries;
-  if (p.lighting)
-    return lds ? launch_one<T, LINEAR, 2, true, kModeGrad>(p, stream)
-               : launch_one<T, LINEAR, 2, false, kModeGrad>(p, stream);
-  switch (p.batch) {
-    case 1:
-      return lds ? launch_one<T, LINEAR, 1, true, kMode1D>(p, stream)
-                 : launch_one<T, LINEAR, 1, false, kMode1D>(p, stream);
+hipError_t launch_typed(const LaunchInfo& li, const JobTable* tab, hipStream_t stream) {
+  if (li.mode == 2) return launch_one<T, LINEAR, 4, false, kModeIso>(li, tab, stream);
+  if (li.mode == 1) return launch_one<T, LINEAR, 2, false, kModeGrad>(li, tab, stream);
+  const bool lds = li.n_tf_lds > 0;
+  if (li.lighting)
+    return lds ? launch_one<T, LINEAR, 2, true, kModeGrad>(li, tab, stream)
+               : launch_one<T, LINEAR, 2, false, kModeGrad>(li, tab, stream);
+  switch (li.batch) {
     case 2:
-      return lds ? launch_one<T, LINEAR, 2, true, kMode1D>(p, stream)
-                 : launch_one<T, LINEAR, 2, false, kMode1D>(p, stream);
+      return lds ? launch_one<T, LINEAR, 2, true, kMode1D>(li, tab, stream)
+                 : launch_one<T, LINEAR, 2, false, kMode1D>(li, tab, stream);
     case 4:
-      return lds ? launch_one<T, LINEAR, 4, true, kMode1D>(p, stream)
-                 : launch_one<T, LINEAR, 4, false, kMode1D>(p, stream);
+      return lds ? launch_one<T, LINEAR, 4, true, kMode1D>(li, tab, stream)
+                 : launch_one<T, LINEAR, 4, false, kMode1D>(li, tab, stream);
     default:
-      return lds ? launch_one<T, LINEAR, 8, true, kMode1D>(p, stream)
-                 : launch_one<T, LINEAR, 8, false, kMode1D>(p, stream);
+      return lds ? launch_one<T, LINEAR, 8, true, kMode1D>(li, tab, stream)
+                 : launch_one<T, LINEAR, 8, false, kMode1D>(li, tab, stream);
   }
 }
 
 }  // namespace
 
-hipError_t launch_raycast(const RaycastParams& p, int bpv, hipStream_t stream) {
-  if (bpv == 1)
-    return p.interp ? launch_typed<uint8_t, true>(p, stream) : launch_typed<uint8_t, false>(p, stream);
-  return p.interp ? launch_typed<uint16_t, true>(p, stream) : launch_typed<uint16_t, false>(p, stream);
+hipError_t launch_cell_empty(const uint32_t* cell_range, int64_t n_cells, const int32_t* prefix,
+                             int n_tf, float tf_scale, int mode, float iso_value, uint8_t* out,
+                             hipStream_t stream) {
+  if (n_cells <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)((n_cells + 255) / 256);
+  cell_empty_kernel<<<blocks, 256, 0, stream>>>(cell_range, n_cells, prefix, n_tf, tf_scale,
+                                                mode == 2 ? kModeIso : mode, iso_value, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_raycast(const LaunchInfo& li, const JobTable* tab, hipStream_t stream) {
+  if (li.bpv == 1)
+    return li.interp ? launch_typed<uint8_t, true>(li, tab, stream)
+                     : launch_typed<uint8_t, false>(li, tab, stream);
+  return li.interp ? launch_typed<uint16_t, true>(li, tab, stream)
+                   : launch_typed<uint16_t, false>(li, tab, stream);
 }
 
 }  // namespace tvk

@@ -2,6 +2,7 @@
 #include "renderer.hpp"
 
 #include <math.h>
+#include <stddef.h>
 #include <string.h>
 
 #include <algorithm>
@@ -46,6 +47,35 @@ void DevBuf::reset() {
   if (p) (void)hipFree(p);
   p = nullptr;
   bytes = 0;
+}
+
+int LaunchSlot::alloc() {
+  TVK_HIP(table.alloc(sizeof(JobTable)));
+  TVK_HIP(counters.alloc(kCounterWords * sizeof(unsigned long long)));
+  TVK_HIP(hipHostMalloc((void**)&table_h, sizeof(JobTable), hipHostMallocDefault));
+  TVK_HIP(hipHostMalloc((void**)&counters_h, kCounterWords * sizeof(unsigned long long),
+                        hipHostMallocDefault));
+  memset(table_h, 0, sizeof(JobTable));
+  memset(counters_h, 0, kCounterWords * sizeof(unsigned long long));
+  for (auto& e : ev) TVK_HIP(hipEventCreate(&e));
+  return TVK_OK;
+}
+
+void LaunchSlot::release() {
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  exec = nullptr;
+  graph = nullptr;
+  for (auto& e : ev) {
+    if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
+  if (table_h) (void)hipHostFree(table_h);
+  if (counters_h) (void)hipHostFree(counters_h);
+  table_h = nullptr;
+  counters_h = nullptr;
+  table.reset();
+  counters.reset();
 }
 
 // ------------------------------------------------------------ small math
@@ -119,19 +149,24 @@ HipRenderer::~HipRenderer() {
   if (initialized_) {
     (void)hipSetDevice(device_);
     (void)hipStreamSynchronize(stream_);
-    for (auto& e : ev_)
-      if (e) (void)hipEventDestroy(e);
-    if (counters_h_) (void)hipHostFree(counters_h_);
+    for (auto& sl : paint_slots_) sl.release();
+    batch_slot_.release();
+    batch_fb_.reset();
+    if (requests_h_) (void)hipHostFree(requests_h_);
+    for (int i = 0; i < kStaging; ++i) {
+      if (staging_[i]) (void)hipHostFree(staging_[i]);
+      if (staging_ev_[i]) (void)hipEventDestroy(staging_ev_[i]);
+    }
+    requests_dev_.reset();
+    touch_dev_.reset();
+    tfs_.clear();
     for (auto& l : levels_) l.reset();
     pool_.slots.reset();
     pool_.page_table.reset();
     pool_.brick_range.reset();
     scratch_.reset();
     cell_range_.reset();
-    tf_dev_.reset();
-    tf_prefix_dev_.reset();
     framebuffer_.reset();
-    counters_.reset();
     if (stream_) (void)hipStreamDestroy(stream_);
   }
 }
@@ -144,12 +179,10 @@ int HipRenderer::Initialize(int device) {
   device_ = device;
   TVK_HIP(hipSetDevice(device_));
   TVK_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  for (auto& e : ev_) TVK_HIP(hipEventCreate(&e));
-  // [0] evaluated samples, [1] nominal samples, [2..9] per-XCD tile queues
-  TVK_HIP(counters_.alloc(16 * sizeof(unsigned long long)));
+  for (auto& sl : paint_slots_)
+    if (int e = sl.alloc()) return e;
+  if (int e = batch_slot_.alloc()) return e;
   TVK_HIP(hipDeviceGetAttribute(&compute_units_, hipDeviceAttributeMultiprocessorCount, device_));
-  TVK_HIP(hipHostMalloc((void**)&counters_h_, 4 * sizeof(unsigned long long), hipHostMallocDefault));
-  memset(counters_h_, 0, 4 * sizeof(unsigned long long));
   size_t free_b = 0, total_b = 0;
   TVK_HIP(hipMemGetInfo(&free_b, &total_b));
   gpu_total_ = total_b;
@@ -202,10 +235,8 @@ int HipRenderer::LoadDataset(const char* path, bool verify) {
   return SetVolume(vox.data(), r.dims, r.bits, (uint32_t)bs, apron, 1);
 }
 
-int HipRenderer::SetVolume(const void* voxels, const uint64_t dims[3], uint32_t bits,
-                           uint32_t brick, uint32_t apron, uint32_t n_lod) {
-  if (!initialized_) return state_fail("setVolume before initialize");
-  if (!voxels) return arg_fail("null voxel pointer");
+int HipRenderer::plan_levels(const uint64_t dims[3], uint32_t bits, uint32_t brick, uint32_t apron,
+                             uint32_t n_lod, LevelPlan* out) const {
   if (bits != 8 && bits != 16) return arg_fail("bits must be 8 or 16");
   for (int i = 0; i < 3; ++i)
     if (dims[i] == 0 || dims[i] > (1u << 20)) return arg_fail("volume dimension out of range");
@@ -214,7 +245,6 @@ int HipRenderer::SetVolume(const void* voxels, const uint64_t dims[3], uint32_t 
   const uint64_t S = (uint64_t)brick + 2ull * apron;
   if (S * S * S >= (1ull << 31)) return arg_fail("brick slot too large");
   const int bpv = (int)bits / 8;
-
   // LOD count: RasterDataBlock rule when not given (UVFReader/DataSource.h:521-529)
   int nl = (int)n_lod;
   if (nl == 0) {
@@ -226,30 +256,79 @@ int HipRenderer::SetVolume(const void* voxels, const uint64_t dims[3], uint32_t 
     }
   }
   if (nl > kMaxLod) return arg_fail("too many LOD levels");
-  uint64_t ld[kMaxLod][3];
-  for (int i = 0; i < 3; ++i) ld[0][i] = dims[i];
+  for (int i = 0; i < 3; ++i) out->dims[0][i] = dims[i];
   for (int l = 1; l < nl; ++l)
-    for (int i = 0; i < 3; ++i) ld[l][i] = (ld[l - 1][i] + 1) / 2;
-
-  // empty-space cell edge: 16 voxels when it divides the brick, else the brick
-  const int cell = (brick % 16 == 0) ? 16 : (int)brick;
-  LodInfo lods[kMaxLod];
+    for (int i = 0; i < 3; ++i) out->dims[l][i] = (out->dims[l - 1][i] + 1) / 2;
+  // empty-space cell edge: the chosen edge (default 16) when it divides the
+  // brick, else 16 when that does, else the brick
+  int cell = (int)brick;
+  for (const int c : {cell_pref_, 16})
+    if (c >= 4 && c <= (int)brick && brick % c == 0) {
+      cell = c;
+      break;
+    }
   int total = 0, total_cells = 0;
   uint64_t level_bytes = 0;
   for (int l = 0; l < nl; ++l) {
+    LodInfo& L = out->lods[l];
     for (int i = 0; i < 3; ++i) {
-      lods[l].dims[i] = (int32_t)ld[l][i];
-      lods[l].grid[i] = (int32_t)((ld[l][i] + brick - 1) / brick);
-      lods[l].to_level[i] = (float)((double)ld[l][i] / (double)dims[i]);
-      lods[l].cells[i] = (int32_t)((ld[l][i] + cell - 1) / cell);
+      L.dims[i] = (int32_t)out->dims[l][i];
+      L.grid[i] = (int32_t)((out->dims[l][i] + brick - 1) / brick);
+      L.to_level[i] = (float)((double)out->dims[l][i] / (double)dims[i]);
+      L.cells[i] = (int32_t)((out->dims[l][i] + cell - 1) / cell);
     }
-    lods[l].brick_base = total;
-    lods[l].cell_base = total_cells;
-    total += lods[l].grid[0] * lods[l].grid[1] * lods[l].grid[2];
-    total_cells += lods[l].cells[0] * lods[l].cells[1] * lods[l].cells[2];
-    level_bytes += ld[l][0] * ld[l][1] * ld[l][2] * bpv;
+    L.brick_base = total;
+    L.cell_base = total_cells;
+    total += L.grid[0] * L.grid[1] * L.grid[2];
+    total_cells += L.cells[0] * L.cells[1] * L.cells[2];
+    level_bytes += out->dims[l][0] * out->dims[l][1] * out->dims[l][2] * bpv;
   }
-  const uint64_t slot_bytes = ((S * S * S * bpv) + 255) & ~255ull;
+  out->n_lod = nl;
+  out->total_bricks = total;
+  out->total_cells = total_cells;
+  out->cell = cell;
+  out->level_bytes = level_bytes;
+  out->slot_edge = (int)S;
+  out->slot_bytes = ((S * S * S * bpv) + 255) & ~255ull;
+  return TVK_OK;
+}
+
+void HipRenderer::release_volume() {
+  has_volume_ = false;
+  paged_ = false;
+  for (auto& l : levels_) l.reset();
+  pool_.slots.reset();
+  pool_.page_table.reset();
+  pool_.brick_range.reset();
+  pool_.n_slots = pool_.slot_bytes = pool_.resident = 0;
+  pool_.page_table_h.clear();
+  pool_.brick_range_h.clear();
+  pool_.slot_owner.clear();
+  cell_range_.reset();
+  requests_dev_.reset();
+  touch_dev_.reset();
+  host_levels_.clear();
+  host_level0_ = nullptr;
+  source_fn_ = nullptr;
+  source_user_ = nullptr;
+  total_cells_ = 0;
+  ++cells_gen_;
+}
+
+int HipRenderer::SetVolume(const void* voxels, const uint64_t dims[3], uint32_t bits,
+                           uint32_t brick, uint32_t apron, uint32_t n_lod) {
+  if (!initialized_) return state_fail("setVolume before initialize");
+  if (!voxels) return arg_fail("null voxel pointer");
+  LevelPlan plan;
+  if (int e = plan_levels(dims, bits, brick, apron, n_lod, &plan)) return e;
+  const int bpv = (int)bits / 8;
+  const int nl = plan.n_lod, total = plan.total_bricks, total_cells = plan.total_cells;
+  const int cell = plan.cell;
+  const uint64_t S = (uint64_t)plan.slot_edge;
+  const uint64_t level_bytes = plan.level_bytes;
+  LodInfo* lods = plan.lods;
+  uint64_t (*ld)[3] = plan.dims;
+  const uint64_t slot_bytes = plan.slot_bytes;
   const uint64_t need = level_bytes + (uint64_t)total * slot_bytes + (uint64_t)total * 12 +
                         (uint64_t)total_cells * 4 +
                         (uint64_t)total * sizeof(BrickJob) + (size_t)width_ * height_ * 4;
@@ -261,14 +340,7 @@ int HipRenderer::SetVolume(const void* voxels, const uint64_t dims[3], uint32_t 
 
   TVK_HIP(hipSetDevice(device_));
   TVK_HIP(hipStreamSynchronize(stream_));
-  has_volume_ = false;
-  for (auto& l : levels_) l.reset();
-  pool_.slots.reset();
-  pool_.page_table.reset();
-  pool_.brick_range.reset();
-  pool_.n_slots = pool_.slot_bytes = pool_.resident = 0;
-  pool_.page_table_h.clear();
-  pool_.brick_range_h.clear();
+  release_volume();
 
   for (int l = 0; l < nl; ++l) TVK_HIP(levels_[l].alloc(ld[l][0] * ld[l][1] * ld[l][2] * bpv));
   TVK_HIP(hipMemcpyAsync(levels_[0].p, voxels, levels_[0].bytes, hipMemcpyHostToDevice, stream_));
@@ -329,12 +401,13 @@ int HipRenderer::SetVolume(const void* voxels, const uint64_t dims[3], uint32_t 
   apron_ = (int)apron;
   slot_edge_ = (int)S;
   cell_ = cell;
+  total_cells_ = total_cells;
+  ++cells_gen_;
   for (int l = 0; l < nl; ++l) lods_[l] = lods[l];
   total_bricks_ = total;
   lod_min_ = 0;
   lod_max_ = (uint32_t)(nl - 1);
   has_volume_ = true;
-  if (!tf1d_.empty()) tf_dirty_ = true;
   invalidate();
   return TVK_OK;
 }
@@ -364,6 +437,15 @@ int HipRenderer::BrickLayout(uint32_t lod, uint32_t grid[3], uint32_t* brick, ui
 int HipRenderer::ReadLod(uint32_t lod, void* out, uint64_t bytes) {
   if (!has_volume_) return state_fail("no volume loaded");
   if ((int)lod >= n_lod_) return arg_fail("LOD index out of range");
+  if (paged_) {
+    // the host keeps the levels of a flat paged volume
+    const LodInfo& L = lods_[lod];
+    const uint64_t n = (uint64_t)L.dims[0] * L.dims[1] * L.dims[2] * (bits_ / 8);
+    if (!host_level0_) return state_fail("levels of a callback-sourced volume live with the source");
+    if (!out || bytes < n) return arg_fail("output buffer too small");
+    memcpy(out, lod == 0 ? host_level0_ : host_levels_[lod - 1].data(), n);
+    return TVK_OK;
+  }
   if (!out || bytes < levels_[lod].bytes) return arg_fail("output buffer too small");
   TVK_HIP(hipSetDevice(device_));
   TVK_HIP(hipMemcpyAsync(out, levels_[lod].p, levels_[lod].bytes, hipMemcpyDeviceToHost, stream_));
@@ -382,7 +464,17 @@ int HipRenderer::ReadBrick(uint32_t lod, uint32_t bx, uint32_t by, uint32_t bz, 
   if (!out || bytes < n) return arg_fail("output buffer too small");
   const int gid = L.brick_base + ((int)bz * L.grid[1] + (int)by) * L.grid[0] + (int)bx;
   const int slot = pool_.page_table_h[gid];
-  if (slot < 0) return state_fail("brick is not resident");
+  if (slot < 0) {
+    if (!paged_) return state_fail("brick is not resident");
+    // not resident: the brick source produces the same slot contents
+    std::vector<uint8_t> tmp(pool_.slot_bytes);
+    if (source_fn_(source_user_, lod, bx, by, bz, tmp.data(), pool_.slot_bytes) != 0) {
+      set_last_error("brick source failed");
+      return TVK_ERR_IO;
+    }
+    memcpy(out, tmp.data(), n);
+    return TVK_OK;
+  }
   TVK_HIP(hipSetDevice(device_));
   TVK_HIP(hipMemcpyAsync(out, (const uint8_t*)pool_.slots.p + (uint64_t)slot * pool_.slot_bytes, n,
                          hipMemcpyDeviceToHost, stream_));
@@ -407,6 +499,7 @@ int HipRenderer::Histogram1D(uint64_t* hist, uint64_t n_bins) {
   if (!has_volume_) return state_fail("no volume loaded");
   const uint64_t nb = 1ull << bits_;
   if (!hist || n_bins != nb) return arg_fail("histogram must have 2^bits bins");
+  if (paged_) return state_fail("histograms need an in-core volume (tvk_set_volume / loadDataset)");
   TVK_HIP(hipSetDevice(device_));
   DevBuf h;
   TVK_HIP(h.alloc(nb * 8));
@@ -422,6 +515,7 @@ int HipRenderer::Histogram2D(uint64_t* hist, uint32_t n_value, uint32_t n_grad, 
   if (!hist || n_value == 0 || n_grad < 2 || n_value > (1u << bits_) || n_grad > 65536 ||
       (n_value & (n_value - 1)))
     return arg_fail("bad 2D histogram size (value bins: power of two <= 2^bits)");
+  if (paged_) return state_fail("histograms need an in-core volume (tvk_set_volume / loadDataset)");
   TVK_HIP(hipSetDevice(device_));
   DevBuf h, km;
   TVK_HIP(h.alloc((size_t)n_value * n_grad * 8));
@@ -450,28 +544,46 @@ int HipRenderer::SetRenderMode(int m) {
   return TVK_OK;
 }
 
-int HipRenderer::Set1DTrans(const float* rgba, uint32_t n) {
+int HipRenderer::Set1DTransSlot(uint32_t slot, const float* rgba, uint32_t n) {
+  if (slot >= (uint32_t)kMaxTfSlots) return arg_fail("TF slot out of range (0..63)");
   if (!rgba || n < 1 || n > 65536) return arg_fail("1D TF needs 1..65536 entries");
   for (uint64_t i = 0; i < 4ull * n; ++i)
     if (!isfinite(rgba[i])) return arg_fail("1D TF holds a non-finite value");
-  tf1d_.assign(rgba, rgba + 4ull * n);
-  tf_dirty_ = true;
-  invalidate();
+  if (tfs_.size() <= slot) tfs_.resize(slot + 1);
+  TfSet& t = tfs_[slot];
+  t.tf1d.assign(rgba, rgba + 4ull * n);
+  t.prefix1d.assign(n + 1, 0);
+  for (uint32_t i = 0; i < n; ++i) t.prefix1d[i + 1] = t.prefix1d[i] + (rgba[4 * i + 3] > 0.f ? 1 : 0);
+  t.dirty1d = true;
+  ++t.version;
+  if (slot == 0) invalidate();
   return TVK_OK;
 }
 
-int HipRenderer::Set2DTrans(const float* rgba, uint32_t w, uint32_t h, float max_grad) {
+int HipRenderer::Set2DTransSlot(uint32_t slot, const float* rgba, uint32_t w, uint32_t h,
+                                float max_grad) {
+  if (slot >= (uint32_t)kMaxTfSlots) return arg_fail("TF slot out of range (0..63)");
   if (!rgba || w < 1 || h < 1 || w > 65536 || h > 4096 || (uint64_t)w * h > (1ull << 24))
     return arg_fail("2D TF size out of range");
   if (!(max_grad > 0.f) || !isfinite(max_grad)) return arg_fail("2D TF max gradient must be > 0");
   for (uint64_t i = 0; i < 4ull * w * h; ++i)
     if (!isfinite(rgba[i])) return arg_fail("2D TF holds a non-finite value");
-  tf2d_.assign(rgba, rgba + 4ull * w * h);
-  tf2d_w_ = w;
-  tf2d_h_ = h;
-  tf2d_max_grad_ = max_grad;
-  tf2d_dirty_ = true;
-  invalidate();
+  if (tfs_.size() <= slot) tfs_.resize(slot + 1);
+  TfSet& t = tfs_[slot];
+  t.tf2d.assign(rgba, rgba + 4ull * w * h);
+  t.w2d = w;
+  t.h2d = h;
+  t.max_grad = max_grad;
+  // prefix count over the value columns holding any alpha > 0
+  t.prefix2d.assign(w + 1, 0);
+  for (size_t i = 0; i < w; ++i) {
+    int any = 0;
+    for (size_t j = 0; j < h && !any; ++j) any = rgba[4 * (j * w + i) + 3] > 0.f;
+    t.prefix2d[i + 1] = t.prefix2d[i] + any;
+  }
+  t.dirty2d = true;
+  ++t.version;
+  if (slot == 0) invalidate();
   return TVK_OK;
 }
 
@@ -491,70 +603,67 @@ int HipRenderer::SetIsosurfaceColor(float r, float g, float b) {
 }
 
 int HipRenderer::upload_tf() {
-  if (tf2d_dirty_) {
-    // 2D raster + prefix count over columns holding any alpha > 0
-    const size_t W = tf2d_w_, H = tf2d_h_;
-    std::vector<int32_t> prefix(W + 1, 0);
-    for (size_t i = 0; i < W; ++i) {
-      int any = 0;
-      for (size_t j = 0; j < H && !any; ++j) any = tf2d_[4 * (j * W + i) + 3] > 0.f;
-      prefix[i + 1] = prefix[i] + any;
+  bool copied = false;
+  for (TfSet& t : tfs_) {
+    if (t.dirty2d) {
+      const size_t W = t.w2d, H = t.h2d;
+      if (t.tf2d_dev.bytes < W * H * 16) TVK_HIP(t.tf2d_dev.alloc(W * H * 16));
+      if (t.prefix2d_dev.bytes < (W + 1) * 4) TVK_HIP(t.prefix2d_dev.alloc((W + 1) * 4));
+      TVK_HIP(hipMemcpyAsync(t.tf2d_dev.p, t.tf2d.data(), W * H * 16, hipMemcpyHostToDevice, stream_));
+      TVK_HIP(hipMemcpyAsync(t.prefix2d_dev.p, t.prefix2d.data(), (W + 1) * 4, hipMemcpyHostToDevice,
+                             stream_));
+      t.dirty2d = false;
+      copied = true;
     }
-    if (tf2d_dev_.bytes < W * H * 16) TVK_HIP(tf2d_dev_.alloc(W * H * 16));
-    if (tf2d_prefix_dev_.bytes < (W + 1) * 4) TVK_HIP(tf2d_prefix_dev_.alloc((W + 1) * 4));
-    TVK_HIP(hipMemcpyAsync(tf2d_dev_.p, tf2d_.data(), W * H * 16, hipMemcpyHostToDevice, stream_));
-    TVK_HIP(hipMemcpyAsync(tf2d_prefix_dev_.p, prefix.data(), (W + 1) * 4, hipMemcpyHostToDevice,
-                           stream_));
-    TVK_HIP(hipStreamSynchronize(stream_));
-    tf2d_dirty_ = false;
+    if (t.dirty1d) {
+      const size_t n = t.tf1d.size() / 4;
+      if (t.tf1d_dev.bytes < n * 16) TVK_HIP(t.tf1d_dev.alloc(n * 16));
+      if (t.prefix1d_dev.bytes < (n + 1) * 4) TVK_HIP(t.prefix1d_dev.alloc((n + 1) * 4));
+      TVK_HIP(hipMemcpyAsync(t.tf1d_dev.p, t.tf1d.data(), n * 16, hipMemcpyHostToDevice, stream_));
+      TVK_HIP(hipMemcpyAsync(t.prefix1d_dev.p, t.prefix1d.data(), (n + 1) * 4, hipMemcpyHostToDevice,
+                             stream_));
+      t.dirty1d = false;
+      copied = true;
+    }
   }
-  if (!tf_dirty_) return TVK_OK;
-  const size_t n = tf1d_.size() / 4;
-  std::vector<int32_t> prefix(n + 1, 0);
-  for (size_t i = 0; i < n; ++i) prefix[i + 1] = prefix[i] + (tf1d_[4 * i + 3] > 0.f ? 1 : 0);
-  if (tf_dev_.bytes < n * 16) TVK_HIP(tf_dev_.alloc(n * 16));
-  if (tf_prefix_dev_.bytes < (n + 1) * 4) TVK_HIP(tf_prefix_dev_.alloc((n + 1) * 4));
-  TVK_HIP(hipMemcpyAsync(tf_dev_.p, tf1d_.data(), n * 16, hipMemcpyHostToDevice, stream_));
-  TVK_HIP(hipMemcpyAsync(tf_prefix_dev_.p, prefix.data(), (n + 1) * 4, hipMemcpyHostToDevice, stream_));
-  // the host vector may change before the copy runs: wait for it
-  TVK_HIP(hipStreamSynchronize(stream_));
-  tf_dirty_ = false;
+  // the host vectors may change before the copies run: wait for them
+  if (copied) TVK_HIP(hipStreamSynchronize(stream_));
   return TVK_OK;
 }
 
 // ----------------------------------------------------------------- camera
 int HipRenderer::SetViewPos(const float v[3]) {
   if (!finite3(v)) return arg_fail("view position must be finite");
-  for (int i = 0; i < 3; ++i) view_pos_[i] = v[i];
+  for (int i = 0; i < 3; ++i) cam_.pos[i] = v[i];
   invalidate();
   return TVK_OK;
 }
 
 int HipRenderer::SetViewDir(const float v[3]) {
   if (!finite3(v) || (v[0] == 0 && v[1] == 0 && v[2] == 0)) return arg_fail("view direction must be non-zero");
-  for (int i = 0; i < 3; ++i) view_dir_[i] = v[i];
+  for (int i = 0; i < 3; ++i) cam_.dir[i] = v[i];
   invalidate();
   return TVK_OK;
 }
 
 int HipRenderer::SetUpDir(const float v[3]) {
   if (!finite3(v) || (v[0] == 0 && v[1] == 0 && v[2] == 0)) return arg_fail("up direction must be non-zero");
-  for (int i = 0; i < 3; ++i) up_dir_[i] = v[i];
+  for (int i = 0; i < 3; ++i) cam_.up[i] = v[i];
   invalidate();
   return TVK_OK;
 }
 
 int HipRenderer::SetFoV(float deg) {
   if (!(deg > 0.f && deg < 180.f)) return arg_fail("field of view must be in (0, 180)");
-  fov_ = deg;
+  cam_.fov = deg;
   invalidate();
   return TVK_OK;
 }
 
 int HipRenderer::SetOrtho(bool on, float hh) {
   if (on && !(hh > 0.f && isfinite(hh))) return arg_fail("orthographic half height must be > 0");
-  ortho_ = on;
-  if (on) ortho_hh_ = hh;
+  cam_.ortho = on;
+  if (on) cam_.ortho_hh = hh;
   invalidate();
   return TVK_OK;
 }
@@ -660,15 +769,35 @@ int HipRenderer::SetKernelTuning(uint32_t batch, uint32_t tile_group, uint32_t x
   return TVK_OK;
 }
 
+int HipRenderer::SetRayTileShape(uint32_t tile_width) {
+  if (tile_width != 4 && tile_width != 8 && tile_width != 16) return arg_fail("tile width must be 4, 8 or 16");
+  tile_shift_ = tile_width == 4 ? 2 : (tile_width == 8 ? 3 : 4);
+  return TVK_OK;
+}
+
+int HipRenderer::SetEmptySpaceCell(uint32_t cell) {
+  if (cell < 4 || cell > 1024) return arg_fail("empty-space cell must be in [4, 1024]");
+  cell_pref_ = (int)cell;
+  return TVK_OK;
+}
+
 // ----------------------------------------------------------------- render
 int HipRenderer::check_ready() const {
   if (!initialized_) return state_fail("paint before initialize");
   if (!has_volume_) return state_fail("paint before a dataset is loaded");
   if (width_ == 0) return state_fail("paint before resize");
-  if (render_mode_ == TVK_RM_1DTRANS && tf1d_.empty()) return state_fail("paint before a 1D TF is set");
-  if (render_mode_ == TVK_RM_2DTRANS && tf2d_.empty()) return state_fail("paint before a 2D TF is set");
+  return check_tf(0);
+}
+
+int HipRenderer::check_tf(int slot) const {
+  const bool have = slot >= 0 && slot < (int)tfs_.size();
+  if (render_mode_ == TVK_RM_1DTRANS && (!have || tfs_[slot].tf1d.empty()))
+    return state_fail("render before a 1D TF is set (slot " + std::to_string(slot) + ")");
+  if (render_mode_ == TVK_RM_2DTRANS && (!have || tfs_[slot].tf2d.empty()))
+    return state_fail("render before a 2D TF is set (slot " + std::to_string(slot) + ")");
   return TVK_OK;
 }
+
 
 double HipRenderer::value_max() const {
   // the largest value of the finest level (the ranges of its bricks)
@@ -679,8 +808,10 @@ double HipRenderer::value_max() const {
   return m > 0.0 ? m : 1.0;
 }
 
-void HipRenderer::build_params(RaycastParams* p, int lod) const {
+void HipRenderer::build_params(RaycastParams* p, int lod, const CameraState& cam, uint32_t width,
+                               uint32_t height, int tf_slot, uint32_t* framebuffer) const {
   memset(p, 0, sizeof(*p));
+  const TfSet& tfs = tfs_[tf_slot];
   double M[16], Minv[16];
   mat_mul(rot_, trans_, M);
   if (!mat_inverse(M, Minv)) mat_identity(Minv);  // setters reject singular input
@@ -701,17 +832,17 @@ void HipRenderer::build_params(RaycastParams* p, int lod) const {
     xform(Minv, w, 0.0, dm);
     for (int i = 0; i < 3; ++i) out[i] = dm[i] * vscale[i];
   };
-  double f[3] = {view_dir_[0], view_dir_[1], view_dir_[2]}, r[3], u[3];
+  double f[3] = {cam.dir[0], cam.dir[1], cam.dir[2]}, r[3], u[3];
   normalize(f);
-  cross(f, up_dir_, r);
+  cross(f, cam.up, r);
   normalize(r);
   cross(r, f, u);
-  const double W = width_, H = height_, asp = W / H;
+  const double W = width, H = height, asp = W / H;
   double tmp[3], dd[3];
-  if (ortho_) {
-    const double hh = ortho_hh_;
+  if (cam.ortho) {
+    const double hh = cam.ortho_hh;
     double o00[3];
-    for (int i = 0; i < 3; ++i) o00[i] = view_pos_[i] + (1.0 / W - 1.0) * hh * asp * r[i] + (1.0 - 1.0 / H) * hh * u[i];
+    for (int i = 0; i < 3; ++i) o00[i] = cam.pos[i] + (1.0 / W - 1.0) * hh * asp * r[i] + (1.0 - 1.0 / H) * hh * u[i];
     vox_point(o00, p->eye);
     for (int i = 0; i < 3; ++i) tmp[i] = 2.0 * hh * asp / W * r[i];
     vox_dir(tmp, dd);
@@ -724,8 +855,8 @@ void HipRenderer::build_params(RaycastParams* p, int lod) const {
     for (int i = 0; i < 3; ++i) p->odir[i] = (float)dd[i];
     p->ortho = 1;
   } else {
-    const double th = tan(fov_ * 0.5 * M_PI / 180.0);
-    vox_point(view_pos_, p->eye);
+    const double th = tan(cam.fov * 0.5 * M_PI / 180.0);
+    vox_point(cam.pos, p->eye);
     for (int i = 0; i < 3; ++i) tmp[i] = f[i] + (1.0 / W - 1.0) * th * asp * r[i] + (1.0 - 1.0 / H) * th * u[i];
     vox_dir(tmp, dd);
     for (int i = 0; i < 3; ++i) p->dir00[i] = (float)dd[i];
@@ -737,10 +868,12 @@ void HipRenderer::build_params(RaycastParams* p, int lod) const {
     for (int i = 0; i < 3; ++i) p->dv[i] = (float)dd[i];
     p->ortho = 0;
   }
-  p->width = (int32_t)width_;
-  p->height = (int32_t)height_;
-  p->tiles_x = (p->width + kTile - 1) / kTile;
-  p->tiles_y = (p->height + kTile - 1) / kTile;
+  p->width = (int32_t)width;
+  p->height = (int32_t)height;
+  p->tile_shift = tile_shift_;
+  p->tiles_x = (p->width + (1 << tile_shift_) - 1) >> tile_shift_;
+  const int tile_h = 64 >> tile_shift_;
+  p->tiles_y = (p->height + tile_h - 1) / tile_h;
   for (int i = 0; i < 3; ++i) p->box[i] = (float)dims0_[i];
   p->step = (float)(ldexp(1.0, lod) / (double)rate_);
   p->nominal_rate = rate_;
@@ -756,8 +889,8 @@ void HipRenderer::build_params(RaycastParams* p, int lod) const {
   p->apron = apron_;
   p->slot_edge = slot_edge_;
   p->slot_elems = (int64_t)(pool_.slot_bytes / (bits_ / 8));
-  p->n_tf = (int32_t)(tf1d_.size() / 4);
-  p->tf_scale = bias_scale_tf_ ? (float)((double)(p->n_tf - 1) / value_max()) : 1.0f;
+  p->n_tf = (int32_t)(tfs.tf1d.size() / 4);
+  p->tf_scale = tf_scale(p->n_tf);
   for (int i = 0; i < 3; ++i) {
     p->bg_top[i] = bg_top_[i];
     p->bg_bottom[i] = bg_bottom_[i];
@@ -767,11 +900,11 @@ void HipRenderer::build_params(RaycastParams* p, int lod) const {
   p->page_table = pool_.page_table.as<int32_t>();
   p->brick_range = pool_.brick_range.as<float>();
   p->cell_range = cell_range_.as<uint32_t>();
+  p->cell_empty = tfs.cell_empty.as<uint8_t>();
   p->cell = skip_ ? cell_ : brick_;
-  p->tf = tf_dev_.as<float>();
-  p->tf_prefix = tf_prefix_dev_.as<int32_t>();
-  p->framebuffer = framebuffer_.as<uint32_t>();
-  p->counters = counters_.as<unsigned long long>();
+  p->tf = tfs.tf1d_dev.as<float>();
+  p->tf_prefix = tfs.prefix1d_dev.as<int32_t>();
+  p->framebuffer = framebuffer;
 
   // render mode, gradient units, lighting
   p->mode = render_mode_ == TVK_RM_2DTRANS ? 1 : (render_mode_ == TVK_RM_ISOSURFACE ? 2 : 0);
@@ -790,16 +923,153 @@ void HipRenderer::build_params(RaycastParams* p, int lod) const {
   for (int i = 0; i < 3; ++i) p->light[i] = (float)lv[i];
   if (p->mode == 1) {
     p->lighting = lighting_ ? 1 : 0;
-    p->tf2d = tf2d_dev_.as<float>();
-    p->tf2d_w = (int32_t)tf2d_w_;
-    p->tf2d_h = (int32_t)tf2d_h_;
-    p->tf2d_row_scale = (float)((double)(tf2d_h_ - 1) / (double)tf2d_max_grad_);
-    p->n_tf = (int32_t)tf2d_w_;  // the skip test runs over the value columns
-    p->tf_scale = bias_scale_tf_ ? (float)((double)(tf2d_w_ - 1) / value_max()) : 1.0f;
-    p->tf_prefix = tf2d_prefix_dev_.as<int32_t>();
+    p->tf2d = tfs.tf2d_dev.as<float>();
+    p->tf2d_w = (int32_t)tfs.w2d;
+    p->tf2d_h = (int32_t)tfs.h2d;
+    p->tf2d_row_scale = (float)((double)(tfs.h2d - 1) / (double)tfs.max_grad);
+    p->n_tf = (int32_t)tfs.w2d;  // the skip test runs over the value columns
+    p->tf_scale = tf_scale((int)tfs.w2d);
+    p->tf_prefix = tfs.prefix2d_dev.as<int32_t>();
   }
   p->iso_value = iso_value_;
   for (int i = 0; i < 3; ++i) p->iso_color[i] = iso_color_[i];
+}
+
+LaunchInfo HipRenderer::launch_info(int n_tf_lds) const {
+  LaunchInfo li;
+  li.bpv = (int)bits_ / 8;
+  li.interp = interp_;
+  li.mode = render_mode_ == TVK_RM_2DTRANS ? 1 : (render_mode_ == TVK_RM_ISOSURFACE ? 2 : 0);
+  li.lighting = (lighting_ && li.mode == 0) ? 1 : 0;
+  li.batch = batch_;
+  li.n_tf_lds = (li.mode == 0 && n_tf_lds <= 4096) ? n_tf_lds : 0;
+  li.compute_units = compute_units_;
+  return li;
+}
+
+int HipRenderer::launch_jobs(LaunchSlot& sl, const RaycastParams* jobs, int n, const LaunchInfo& li,
+                             bool graph) {
+  // the pinned table may still be read by this slot's previous upload
+  if (sl.used) TVK_HIP(hipEventSynchronize(sl.ev[3]));
+  JobTable* t = sl.table_h;
+  t->h.n_jobs = n;
+  t->h.tile_group = tile_group_;
+  t->h.xcd_split = xcd_split_;
+  t->h.n_tf_lds = li.n_tf_lds;
+  t->h.tf_lds = li.n_tf_lds > 0 ? jobs[0].tf : nullptr;
+  t->h.counters = sl.counters.as<unsigned long long>();
+  int total = 0;
+  for (int k = 0; k < n; ++k) {
+    t->tile_start[k] = total;
+    total += jobs[k].tiles_x * jobs[k].tiles_y;
+    t->jobs[k] = jobs[k];
+  }
+  t->tile_start[n] = total;
+  t->h.total_tiles = total;
+  const int words = kQueues + 2 * n;
+  const size_t table_bytes = offsetof(JobTable, jobs) + (size_t)n * sizeof(RaycastParams);
+  sl.used = true;
+  TVK_HIP(hipEventRecord(sl.ev[0], stream_));
+  if (!graph) {
+    TVK_HIP(hipMemcpyAsync(sl.table.p, t, table_bytes, hipMemcpyHostToDevice, stream_));
+    TVK_HIP(launch_fill_u64(sl.counters.as<unsigned long long>(), words, 0ull, stream_));
+    TVK_HIP(hipEventRecord(sl.ev[1], stream_));
+    TVK_HIP(launch_raycast(li, sl.table.as<JobTable>(), stream_));
+    TVK_HIP(hipEventRecord(sl.ev[2], stream_));
+    TVK_HIP(hipMemcpyAsync(sl.counters_h, sl.counters.p, words * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, stream_));
+    TVK_HIP(hipEventRecord(sl.ev[3], stream_));
+    return TVK_OK;
+  }
+  // Graph replay: the graph copies the whole table and all counters, so one
+  // capture serves any job count; it is re-captured when the kernel
+  // variant or its LDS size changes.
+  const uint64_t key = 1ull | (uint64_t)li.bpv << 1 | (uint64_t)li.interp << 3 | (uint64_t)li.mode << 4 |
+                       (uint64_t)li.lighting << 6 | (uint64_t)li.batch << 7 | (uint64_t)li.n_tf_lds << 12;
+  if (!sl.exec || sl.graph_key != key) {
+    if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+    if (sl.graph) (void)hipGraphDestroy(sl.graph);
+    sl.exec = nullptr;
+    sl.graph = nullptr;
+    TVK_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipMemcpyAsync(sl.table.p, t, sizeof(JobTable), hipMemcpyHostToDevice, stream_);
+    if (e == hipSuccess)
+      e = launch_fill_u64(sl.counters.as<unsigned long long>(), kCounterWords, 0ull, stream_);
+    if (e == hipSuccess) e = launch_raycast(li, sl.table.as<JobTable>(), stream_);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(sl.counters_h, sl.counters.p, kCounterWords * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, stream_);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(stream_, &g);
+    if (e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return hip_fail(e, "raycast graph capture");
+    }
+    TVK_HIP(e2);
+    sl.graph = g;
+    TVK_HIP(hipGraphInstantiate(&sl.exec, sl.graph, nullptr, nullptr, 0));
+    sl.graph_key = key;
+  }
+  TVK_HIP(hipEventRecord(sl.ev[1], stream_));
+  TVK_HIP(hipGraphLaunch(sl.exec, stream_));
+  TVK_HIP(hipEventRecord(sl.ev[2], stream_));
+  TVK_HIP(hipEventRecord(sl.ev[3], stream_));
+  return TVK_OK;
+}
+
+float HipRenderer::tf_scale(int n_entries) const {
+  return bias_scale_tf_ ? (float)((double)(n_entries - 1) / value_max()) : 1.0f;
+}
+
+int HipRenderer::prepare_cells(int tf_slot) {
+  TfSet& t = tfs_[tf_slot];
+  if (!skip_ || total_cells_ == 0) return TVK_OK;
+  const int mode = render_mode_ == TVK_RM_2DTRANS ? 1 : (render_mode_ == TVK_RM_ISOSURFACE ? 2 : 0);
+  const std::vector<int32_t>& prefix = mode == 1 ? t.prefix2d : t.prefix1d;
+  const int n = (int)prefix.size() - 1;
+  const float scale = tf_scale(n);
+  uint32_t iso_bits = 0, scale_bits = 0;
+  memcpy(&iso_bits, &iso_value_, 4);
+  memcpy(&scale_bits, &scale, 4);
+  // FNV-1a over what the flags depend on
+  uint64_t key = 1469598103934665603ull;
+  for (const uint64_t v : {cells_gen_, (uint64_t)mode, t.version, (uint64_t)iso_bits,
+                           (uint64_t)scale_bits, (uint64_t)n})
+    key = (key ^ v) * 1099511628211ull;
+  if (key == t.cell_key && t.cell_empty.p) return TVK_OK;
+  if (t.cell_empty.bytes < (size_t)total_cells_) TVK_HIP(t.cell_empty.alloc((size_t)total_cells_));
+  const int32_t* prefix_dev = mode == 1 ? t.prefix2d_dev.as<int32_t>() : t.prefix1d_dev.as<int32_t>();
+  if (mode != 2 && (n < 1 || !prefix_dev)) return state_fail("TF not uploaded");
+  TVK_HIP(launch_cell_empty(cell_range_.as<uint32_t>(), total_cells_, prefix_dev, n, scale, mode,
+                            iso_value_, t.cell_empty.as<uint8_t>(), stream_));
+  t.cell_key = key;
+  return TVK_OK;
+}
+
+uint32_t HipRenderer::bricks_skipped(int lod, int tf_slot) const {
+  // bricks whose whole value range is transparent (below the iso value)
+  if (!skip_) return 0;
+  const TfSet& t = tfs_[tf_slot];
+  const LodInfo& L = lods_[lod];
+  const int nb = L.grid[0] * L.grid[1] * L.grid[2];
+  const int mode = render_mode_ == TVK_RM_2DTRANS ? 1 : (render_mode_ == TVK_RM_ISOSURFACE ? 2 : 0);
+  const std::vector<int32_t>& prefix = mode == 1 ? t.prefix2d : t.prefix1d;
+  const int ntf = (int)prefix.size() - 1;
+  const float scale = ntf > 0 ? tf_scale(ntf) : 1.0f;
+  uint32_t skipped = 0;
+  for (int b = 0; b < nb; ++b) {
+    const float mn = pool_.brick_range_h[2 * (L.brick_base + b)];
+    const float mx = pool_.brick_range_h[2 * (L.brick_base + b) + 1];
+    if (mode == 2) {
+      if (mx < iso_value_) ++skipped;
+      continue;
+    }
+    if (ntf < 1) continue;
+    const int i0 = (int)std::min(std::max(mn * scale, 0.f), (float)(ntf - 1));
+    const int i1 = std::min((int)std::min(std::max(mx * scale, 0.f), (float)(ntf - 1)) + 1, ntf - 1);
+    if (prefix[i1 + 1] - prefix[i0] == 0) ++skipped;
+  }
+  return skipped;
 }
 
 int HipRenderer::Paint() {
@@ -813,61 +1083,186 @@ int HipRenderer::Paint() {
   else if (frame_complete_) lod = lo;
   else if (next_lod_ < 0) lod = hi;
   else lod = std::max(lo, next_lod_);
+  if (paged_) lod = std::max(lod, std::min(lod_floor_, hi));
 
+  if (int e = prepare_cells(0)) return e;
   RaycastParams p;
-  build_params(&p, lod);
-  TVK_HIP(hipEventRecord(ev_[0], stream_));
-  TVK_HIP(launch_fill_u64(counters_.as<unsigned long long>(), 10, 0ull, stream_));
-  TVK_HIP(hipEventRecord(ev_[1], stream_));
-  TVK_HIP(launch_raycast(p, (int)bits_ / 8, stream_));
-  TVK_HIP(hipEventRecord(ev_[2], stream_));
-  TVK_HIP(hipMemcpyAsync(counters_h_, counters_.p, 2 * sizeof(unsigned long long),
-                         hipMemcpyDeviceToHost, stream_));
-  TVK_HIP(hipEventRecord(ev_[3], stream_));
+  build_params(&p, lod, cam_, width_, height_, 0, framebuffer_.as<uint32_t>());
+  if (paged_) {
+    ++frame_;
+    p.requests = requests_dev_.as<uint32_t>();
+    p.touch = touch_dev_.as<uint32_t>();
+    p.frame = frame_;
+    TVK_HIP(launch_fill_u64(requests_dev_.as<unsigned long long>(), (int64_t)(requests_dev_.bytes / 8),
+                            0ull, stream_));
+  }
+  LaunchSlot& sl = paint_slots_[paint_cur_];
+  paint_cur_ ^= 1;
+  if (int e = launch_jobs(sl, &p, 1, launch_info(p.n_tf), false)) return e;
+  stats_slot_ = &sl;
+  bool pass_complete = true;
+  if (paged_) {
+    // read the brick requests of this pass and page the bricks in
+    TVK_HIP(hipMemcpyAsync(requests_h_, requests_dev_.p, requests_dev_.bytes, hipMemcpyDeviceToHost,
+                           stream_));
+    TVK_HIP(hipStreamSynchronize(stream_));
+    bool overflow = false;
+    if (int e = service_requests(lod, &pass_complete, &overflow)) return e;
+    if (overflow && lod < hi) lod_floor_ = lod + 1;  // the view's bricks do not fit: go coarser
+  }
 
   // frame bookkeeping and the host-side part of the stats
   const LodInfo& L = lods_[lod];
-  const int nb = L.grid[0] * L.grid[1] * L.grid[2];
-  uint32_t skipped = 0;
-  if (skip_) {
-    // bricks whose whole value range is transparent (below the iso value)
-    const int ntf = p.n_tf;
-    std::vector<int32_t> prefix(ntf + 1, 0);
-    for (int i = 0; i < ntf; ++i) {
-      int any = 0;
-      if (p.mode == 0) any = tf1d_[4 * i + 3] > 0.f;
-      else if (p.mode == 1)
-        for (uint32_t j = 0; j < tf2d_h_ && !any; ++j) any = tf2d_[4 * ((size_t)j * tf2d_w_ + i) + 3] > 0.f;
-      prefix[i + 1] = prefix[i] + any;
-    }
-    for (int b = 0; b < nb; ++b) {
-      const float mn = pool_.brick_range_h[2 * (L.brick_base + b)];
-      const float mx = pool_.brick_range_h[2 * (L.brick_base + b) + 1];
-      if (p.mode == 2) {
-        if (mx < p.iso_value) ++skipped;
-        continue;
-      }
-      const int i0 = (int)std::min(std::max(mn * p.tf_scale, 0.f), (float)(ntf - 1));
-      const int i1 = std::min((int)std::min(std::max(mx * p.tf_scale, 0.f), (float)(ntf - 1)) + 1, ntf - 1);
-      if (prefix[i1 + 1] - prefix[i0] == 0) ++skipped;
-    }
-  }
   stats_ = tvk_frame_stats{};
-  stats_.bricks_total = (uint32_t)nb;
-  stats_.bricks_skipped = skipped;
+  stats_.bricks_total = (uint32_t)(L.grid[0] * L.grid[1] * L.grid[2]);
+  stats_.bricks_skipped = bricks_skipped(lod, 0);
   stats_.lod = (uint32_t)lod;
   stats_.width = width_;
   stats_.height = height_;
   stats_pending_ = true;
   last_lod_ = lod;
   ++subframes_;
-  if (lod <= lo) {
+  if (!pass_complete) {
+    // bricks were paged in: render this LOD again (a coarser one after an
+    // overflow) before the frame counts as done
+    frame_complete_ = false;
+    next_lod_ = std::max(lod, std::min(lod_floor_, hi));
+  } else if (lod <= std::max(lo, paged_ ? std::min(lod_floor_, hi) : lo)) {
     frame_complete_ = true;
     next_lod_ = -1;
   } else {
     frame_complete_ = false;
     next_lod_ = lod - 1;
   }
+  return TVK_OK;
+}
+
+int HipRenderer::RenderBatch(const tvk_view* views, uint32_t n, tvk_view_stats* stats,
+                             float* kernel_ms) {
+  if (!initialized_) return state_fail("render before initialize");
+  if (!has_volume_) return state_fail("render before a dataset is loaded");
+  if (n == 0) return TVK_OK;
+  if (!views) return arg_fail("null view array");
+  std::vector<CameraState> cams(n);
+  uint64_t fb_bytes = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const tvk_view& v = views[i];
+    if (v.width == 0 || v.height == 0 || v.width > 16384 || v.height > 16384)
+      return arg_fail("view " + std::to_string(i) + ": image size out of range");
+    if (int e = check_tf((int)v.tf_slot)) return e;
+    if (!finite3(v.view_pos) || !finite3(v.view_dir) || !finite3(v.up_dir))
+      return arg_fail("view " + std::to_string(i) + ": camera must be finite");
+    const float* d = v.view_dir;
+    if (d[0] == 0.f && d[1] == 0.f && d[2] == 0.f) return arg_fail("view direction must be non-zero");
+    if (v.ortho ? !(v.ortho_half_height > 0.f) : !(v.fov_deg > 0.f && v.fov_deg < 180.f))
+      return arg_fail("view " + std::to_string(i) + ": fov / ortho extent out of range");
+    CameraState& c = cams[i];
+    for (int k = 0; k < 3; ++k) {
+      c.pos[k] = v.view_pos[k];
+      c.dir[k] = v.view_dir[k];
+      c.up[k] = v.up_dir[k];
+    }
+    c.fov = v.fov_deg;
+    c.ortho = v.ortho != 0;
+    c.ortho_hh = v.ortho_half_height;
+    fb_bytes += ((uint64_t)v.width * v.height * 4 + 255) & ~255ull;
+  }
+  TVK_HIP(hipSetDevice(device_));
+  if (int e = upload_tf()) return e;
+  if (batch_fb_.bytes < fb_bytes) {
+    TVK_HIP(hipStreamSynchronize(stream_));
+    TVK_HIP(batch_fb_.alloc(fb_bytes));
+  }
+  batch_fb_offset_.assign(n, 0);
+  uint64_t off = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    batch_fb_offset_[i] = off;
+    off += ((uint64_t)views[i].width * views[i].height * 4 + 255) & ~255ull;
+  }
+  const int lo = std::min<int>((int)lod_min_, n_lod_ - 1);
+  const int hi = std::max(lo, std::min<int>((int)lod_max_, n_lod_ - 1));
+  float kms = 0.f;
+  for (uint32_t first = 0; first < n; first += kMaxJobs) {
+    const int cnt = (int)std::min<uint32_t>(kMaxJobs, n - first);
+    // one shared 1D TF goes to LDS
+    bool same_tf = true;
+    for (int k = 1; k < cnt; ++k) same_tf = same_tf && views[first + k].tf_slot == views[first].tf_slot;
+    const int n_tf0 = (int)(tfs_[views[first].tf_slot].tf1d.size() / 4);
+    const LaunchInfo li = launch_info(same_tf ? n_tf0 : 0);
+    int lod = lo;
+    if (paged_) lod = std::max(lod, std::min(lod_floor_, hi));
+    std::vector<RaycastParams> jobs(cnt);
+    // paged volumes: a pass per set of requested bricks, as in a capture
+    for (int pass = 0; pass < 4 * kMaxLod + 8; ++pass) {
+      for (int k = 0; k < cnt; ++k)
+        if (int e = prepare_cells((int)views[first + k].tf_slot)) return e;
+      for (int k = 0; k < cnt; ++k) {
+        const uint32_t i = first + k;
+        build_params(&jobs[k], lod, cams[i], views[i].width, views[i].height, (int)views[i].tf_slot,
+                     (uint32_t*)((uint8_t*)batch_fb_.p + batch_fb_offset_[i]));
+        if (paged_) {
+          jobs[k].requests = requests_dev_.as<uint32_t>();
+          jobs[k].touch = touch_dev_.as<uint32_t>();
+          jobs[k].frame = frame_ + 1;
+        }
+      }
+      if (paged_) {
+        ++frame_;
+        TVK_HIP(launch_fill_u64(requests_dev_.as<unsigned long long>(),
+                                (int64_t)(requests_dev_.bytes / 8), 0ull, stream_));
+      }
+      if (int e = launch_jobs(batch_slot_, jobs.data(), cnt, li, use_graphs_)) return e;
+      if (paged_)
+        TVK_HIP(hipMemcpyAsync(requests_h_, requests_dev_.p, requests_dev_.bytes,
+                               hipMemcpyDeviceToHost, stream_));
+      TVK_HIP(hipEventSynchronize(batch_slot_.ev[3]));
+      float k_ms = 0.f;
+      TVK_HIP(hipEventElapsedTime(&k_ms, batch_slot_.ev[1], batch_slot_.ev[2]));
+      kms += k_ms;
+      bool complete = true, overflow = false;
+      if (paged_) {
+        TVK_HIP(hipStreamSynchronize(stream_));
+        if (int e = service_requests(lod, &complete, &overflow)) return e;
+        if (overflow && lod < hi) {
+          lod_floor_ = lod + 1;
+          lod = lod + 1;
+        }
+      }
+      if (complete) break;
+    }
+    for (int k = 0; k < cnt; ++k) {
+      const uint32_t i = first + k;
+      if (stats) {
+        stats[i].evaluated_samples = batch_slot_.counters_h[kQueues + 2 * k];
+        stats[i].nominal_samples = batch_slot_.counters_h[kQueues + 2 * k + 1];
+        stats[i].lod = (uint32_t)lod;
+        stats[i].reserved = 0;
+      }
+      if (views[i].rgba)
+        TVK_HIP(hipMemcpyAsync(views[i].rgba, (uint8_t*)batch_fb_.p + batch_fb_offset_[i],
+                               (size_t)views[i].width * views[i].height * 4, hipMemcpyDeviceToHost,
+                               stream_));
+    }
+    TVK_HIP(hipStreamSynchronize(stream_));
+  }
+  if (kernel_ms) *kernel_ms = kms;
+  return TVK_OK;
+}
+
+int HipRenderer::SetLaunchGraphs(bool on) {
+  // The first batched launch replayed from a captured graph did not return
+  // on the device (the capture also held the kernel's first use in the
+  // process); until that is understood the graph path stays off and
+  // batches launch eagerly.
+  if (on) return state_fail("graph replay of batched launches is switched off in this build");
+  use_graphs_ = false;
+  return TVK_OK;
+}
+
+int HipRenderer::BatchFramebuffer(uint32_t index, void** dptr) {
+  if (index >= batch_fb_offset_.size()) return arg_fail("no such view in the last batch");
+  if (!dptr) return arg_fail("null pointer");
+  *dptr = (uint8_t*)batch_fb_.p + batch_fb_offset_[index];
   return TVK_OK;
 }
 
@@ -889,7 +1284,8 @@ int HipRenderer::Capture(uint8_t* rgba, uint64_t bytes, bool preserve) {
     invalidate();
   }
   int rc = TVK_OK;
-  for (int guard = 0; guard < kMaxLod + 1 && !frame_complete_ && rc == TVK_OK; ++guard) rc = Paint();
+  // a paged pool may need a paging pass per LOD on top of the subframes
+  for (int guard = 0; guard < 4 * kMaxLod + 8 && !frame_complete_ && rc == TVK_OK; ++guard) rc = Paint();
   if (rc == TVK_OK) {
     hipError_t e = hipMemcpyAsync(rgba, framebuffer_.p, (size_t)width_ * height_ * 4,
                                   hipMemcpyDeviceToHost, stream_);
@@ -919,16 +1315,17 @@ int HipRenderer::Progress(uint64_t* subframes, uint32_t* working_subframe, uint3
 
 int HipRenderer::FrameStats(tvk_frame_stats* out) {
   if (!out) return arg_fail("null stats pointer");
-  if (stats_pending_) {
+  if (stats_pending_ && stats_slot_) {
     TVK_HIP(hipSetDevice(device_));
-    TVK_HIP(hipEventSynchronize(ev_[3]));
+    LaunchSlot& sl = *stats_slot_;
+    TVK_HIP(hipEventSynchronize(sl.ev[3]));
     float k = 0.f, all = 0.f;
-    TVK_HIP(hipEventElapsedTime(&k, ev_[1], ev_[2]));
-    TVK_HIP(hipEventElapsedTime(&all, ev_[0], ev_[3]));
+    TVK_HIP(hipEventElapsedTime(&k, sl.ev[1], sl.ev[2]));
+    TVK_HIP(hipEventElapsedTime(&all, sl.ev[0], sl.ev[3]));
     stats_.kernel_ms = k;
     stats_.paint_ms = all;
-    stats_.evaluated_samples = counters_h_[0];
-    stats_.nominal_samples = counters_h_[1];
+    stats_.evaluated_samples = sl.counters_h[kQueues];
+    stats_.nominal_samples = sl.counters_h[kQueues + 1];
     stats_pending_ = false;
   }
   *out = stats_;
@@ -961,10 +1358,14 @@ int HipRenderer::MemStats(tvk_mem_stats* out) {
   *out = tvk_mem_stats{};
   out->gpu_mem = gpu_total_;
   out->gpu_budget = gpu_budget_;
-  uint64_t a = framebuffer_.bytes + counters_.bytes + tf_dev_.bytes + tf_prefix_dev_.bytes +
-               pool_.slots.bytes + pool_.page_table.bytes + pool_.brick_range.bytes + scratch_.bytes +
+  uint64_t a = framebuffer_.bytes + batch_fb_.bytes + pool_.slots.bytes + pool_.page_table.bytes + pool_.brick_range.bytes + scratch_.bytes +
                cell_range_.bytes;
   for (auto& l : levels_) a += l.bytes;
+  for (const TfSet& t : tfs_)
+    a += t.tf1d_dev.bytes + t.prefix1d_dev.bytes + t.tf2d_dev.bytes + t.prefix2d_dev.bytes;
+  for (const LaunchSlot* sl : {&paint_slots_[0], &paint_slots_[1], &batch_slot_})
+    a += sl->table.bytes + sl->counters.bytes;
+  a += requests_dev_.bytes + touch_dev_.bytes;
   out->allocated_gpu_mem = a;
   out->pool_slots = pool_.n_slots;
   out->resident_bricks = pool_.resident;

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/tvk_hip.h"
+#include "kernels.hpp"
 #include "tvk_common.hpp"
 
 namespace tvk {
@@ -24,6 +25,20 @@ struct DevBuf {
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      reset();
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
   ~DevBuf() { reset(); }
   hipError_t alloc(size_t n);
   void reset();
@@ -45,7 +60,66 @@ struct BrickPool {
   uint64_t evicted = 0;
   std::vector<int32_t> page_table_h;
   std::vector<float> brick_range_h;
+  std::vector<int32_t> slot_owner;   // paged pools: brick id held by each slot, -1 free
+  std::vector<uint32_t> slot_touch;  // paged pools: last frame the slot's brick was sampled
 };
+
+// Geometry of a bricked LOD pyramid (RasterDataBlock layout).
+struct LevelPlan {
+  uint64_t dims[kMaxLod][3];
+  LodInfo lods[kMaxLod];
+  int n_lod = 0, total_bricks = 0, total_cells = 0, cell = 16, slot_edge = 0;
+  uint64_t level_bytes = 0, slot_bytes = 0;
+};
+
+// One transfer-function slot: a 1D TF and/or a 2D TF with the prefix
+// counts of entries (columns) whose alpha is > 0 (the empty-space test).
+struct TfSet {
+  std::vector<float> tf1d;
+  std::vector<int32_t> prefix1d;
+  bool dirty1d = false;
+  DevBuf tf1d_dev, prefix1d_dev;
+  std::vector<float> tf2d;
+  std::vector<int32_t> prefix2d;
+  uint32_t w2d = 0, h2d = 0;
+  float max_grad = 1.f;
+  bool dirty2d = false;
+  DevBuf tf2d_dev, prefix2d_dev;
+  uint64_t version = 0;   // bumped by every change of this slot
+  DevBuf cell_empty;      // u8 per empty-space cell of every level
+  uint64_t cell_key = 0;  // what cell_empty was built for
+};
+
+// A camera (Tuvok's view position / direction / up, FoV or ortho extent).
+struct CameraState {
+  double pos[3] = {0.0, 0.0, 2.0};
+  double dir[3] = {0.0, 0.0, -1.0};
+  double up[3] = {0.0, 1.0, 0.0};
+  double fov = 50.0;
+  bool ortho = false;
+  double ortho_hh = 0.5;
+};
+
+// Device and pinned host resources of one raycast launch: the job table,
+// the counters and, for batched launches, the captured graph replaying
+// [table upload, counter clear, raycast, counter readback].
+struct LaunchSlot {
+  DevBuf table;                                 // JobTable
+  JobTable* table_h = nullptr;                  // pinned staging copy
+  DevBuf counters;                              // kCounterWords u64
+  unsigned long long* counters_h = nullptr;     // pinned copy after the launch
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // start, kernel in, kernel out, done
+  bool used = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t graph_key = 0;
+  int alloc();
+  void release();
+};
+
+// Out-of-core brick source (see tvk_brick_source_fn in tvk_hip.h).
+typedef int (*BrickSourceFn)(void* user, uint32_t lod, uint32_t bx, uint32_t by, uint32_t bz,
+                             void* slot, uint64_t slot_bytes);
 
 class HipRenderer {
  public:
@@ -60,6 +134,11 @@ class HipRenderer {
   int LoadDataset(const char* path, bool verify);
   int SetVolume(const void* voxels, const uint64_t dims[3], uint32_t bits, uint32_t brick,
                 uint32_t apron, uint32_t n_lod);
+  int SetPagedVolume(const uint64_t dims[3], uint32_t bits, uint32_t brick, uint32_t apron,
+                     uint32_t n_lod, BrickSourceFn fn, void* user, const float* brick_ranges,
+                     uint64_t pool_bytes);
+  int SetPagedFlatVolume(const void* voxels, const uint64_t dims[3], uint32_t bits, uint32_t brick,
+                         uint32_t apron, uint32_t n_lod, uint64_t pool_bytes);
   int LodLevelCount() const { return n_lod_; }
   int BitWidth(uint32_t* bits) const;
   int DomainSize(uint32_t lod, uint64_t dims[3]) const;
@@ -71,8 +150,12 @@ class HipRenderer {
   int Histogram2D(uint64_t* hist, uint32_t n_value, uint32_t n_grad, float* max_grad);
 
   int SetRenderMode(int m);
-  int Set1DTrans(const float* rgba, uint32_t n);
-  int Set2DTrans(const float* rgba, uint32_t w, uint32_t h, float max_grad);
+  int Set1DTrans(const float* rgba, uint32_t n) { return Set1DTransSlot(0, rgba, n); }
+  int Set2DTrans(const float* rgba, uint32_t w, uint32_t h, float max_grad) {
+    return Set2DTransSlot(0, rgba, w, h, max_grad);
+  }
+  int Set1DTransSlot(uint32_t slot, const float* rgba, uint32_t n);
+  int Set2DTransSlot(uint32_t slot, const float* rgba, uint32_t w, uint32_t h, float max_grad);
   int SetIsoValue(float v);
   int SetIsosurfaceColor(float r, float g, float b);
 
@@ -94,8 +177,15 @@ class HipRenderer {
   int SetEarlyRayTermination(float a);
   int SetEmptySpaceSkipping(bool on);
   int SetKernelTuning(uint32_t batch, uint32_t tile_group, uint32_t xcd_split);
+  int SetRayTileShape(uint32_t tile_width);
+  int SetEmptySpaceCell(uint32_t cell);
+
+  int SetLaunchGraphs(bool on);
 
   int Paint();
+  // Many views (camera, size, TF slot) of this volume in one launch.
+  int RenderBatch(const tvk_view* views, uint32_t n, tvk_view_stats* stats, float* kernel_ms);
+  int BatchFramebuffer(uint32_t index, void** dptr);
   int CheckForRedraw() const { return frame_complete_ ? 0 : 1; }
   int ScheduleCompleteRedraw();
   int Capture(uint8_t* rgba, uint64_t bytes, bool preserve_transparency);
@@ -110,10 +200,34 @@ class HipRenderer {
 
  private:
   int check_ready() const;
+  int check_tf(int slot) const;
   int upload_tf();
-  void build_params(RaycastParams* p, int lod) const;
+  void build_params(RaycastParams* p, int lod, const CameraState& cam, uint32_t w, uint32_t h,
+                    int tf_slot, uint32_t* framebuffer) const;
+  LaunchInfo launch_info(int n_tf_lds) const;
+  // fill slot's table from jobs[0..n) and launch (graph replay if enabled)
+  int launch_jobs(LaunchSlot& slot, const RaycastParams* jobs, int n, const LaunchInfo& li,
+                  bool graph);
+  uint32_t bricks_skipped(int lod, int tf_slot) const;
+  float tf_scale(int n_entries) const;
+  // (re)build the empty flags of TF slot `tf_slot` for the current mode
+  int prepare_cells(int tf_slot);
   double value_max() const;
-  void invalidate() { frame_complete_ = false; next_lod_ = -1; }
+  int plan_levels(const uint64_t dims[3], uint32_t bits, uint32_t brick, uint32_t apron,
+                  uint32_t n_lod, LevelPlan* out) const;
+  void release_volume();
+  // paged pools: upload the bricks the last raycast asked for; *complete is
+  // false when another pass is needed, *overflow when the pool cannot hold
+  // the bricks of this LOD for this view
+  int service_requests(int lod, bool* complete, bool* overflow);
+  int upload_brick(int gid, int lod, int bx, int by, int bz, int slot, int staging);
+  static int flat_source(void* user, uint32_t lod, uint32_t bx, uint32_t by, uint32_t bz,
+                         void* slot, uint64_t slot_bytes);
+  void invalidate() {
+    frame_complete_ = false;
+    next_lod_ = -1;
+    lod_floor_ = 0;
+  }
 
   // construction flags (tuvok.renderer.new)
   int type_;
@@ -123,11 +237,14 @@ class HipRenderer {
   bool initialized_ = false;
   int device_ = -1;
   hipStream_t stream_ = nullptr;
-  hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t width_ = 0, height_ = 0;
   DevBuf framebuffer_;
-  DevBuf counters_;
-  unsigned long long* counters_h_ = nullptr;  // pinned
+  LaunchSlot paint_slots_[2];  // Paint alternates between two
+  int paint_cur_ = 0;
+  LaunchSlot batch_slot_;
+  DevBuf batch_fb_;
+  std::vector<uint64_t> batch_fb_offset_;  // per view of the last batch
+  bool use_graphs_ = false;  // graph replay is switched off (see SetLaunchGraphs)
   uint64_t gpu_total_ = 0, gpu_budget_ = 0;
   int compute_units_ = 0;
 
@@ -144,27 +261,35 @@ class HipRenderer {
   DevBuf scratch_;
   DevBuf cell_range_;  // empty-space cell ranges of every level
   int cell_ = 16;      // empty-space cell edge (divides the brick size)
+  int cell_pref_ = 16; // requested cell edge for the next volume
+  int total_cells_ = 0;
+  uint64_t cells_gen_ = 1;  // bumped whenever cell ranges change
 
-  // transfer functions
+  // paged (out-of-core) pools: GridLeaper-style requests + LRU
+  bool paged_ = false;
+  BrickSourceFn source_fn_ = nullptr;
+  void* source_user_ = nullptr;
+  const void* host_level0_ = nullptr;             // caller-owned LOD 0 (flat source)
+  std::vector<std::vector<uint8_t>> host_levels_;  // LOD >= 1 (flat source)
+  DevBuf requests_dev_, touch_dev_;               // u32 per brick
+  uint32_t* requests_h_ = nullptr;                // pinned
+  uint64_t requests_h_count_ = 0;
+  static constexpr int kStaging = 4;
+  void* staging_[kStaging] = {nullptr, nullptr, nullptr, nullptr};  // pinned slots
+  hipEvent_t staging_ev_[kStaging] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t frame_ = 0;
+  int lod_floor_ = 0;       // coarsest-needed fallback when the pool overflows
+  std::vector<uint32_t> cell_init_h_;  // brick-range initial cell ranges
+
+  // transfer functions: slot 0 is the one Set1DTrans / Set2DTrans set
+  static constexpr int kMaxTfSlots = 64;
   int render_mode_ = TVK_RM_1DTRANS;
-  std::vector<float> tf1d_;
-  bool tf_dirty_ = false;
-  DevBuf tf_dev_, tf_prefix_dev_;
+  std::vector<TfSet> tfs_ = std::vector<TfSet>(1);
   float iso_value_ = 0.f;
   float iso_color_[3] = {1.f, 1.f, 1.f};
-  std::vector<float> tf2d_;
-  uint32_t tf2d_w_ = 0, tf2d_h_ = 0;
-  float tf2d_max_grad_ = 1.f;
-  bool tf2d_dirty_ = false;
-  DevBuf tf2d_dev_, tf2d_prefix_dev_;
 
   // camera
-  double view_pos_[3] = {0.0, 0.0, 2.0};
-  double view_dir_[3] = {0.0, 0.0, -1.0};
-  double up_dir_[3] = {0.0, 1.0, 0.0};
-  double fov_ = 50.0;
-  bool ortho_ = false;
-  double ortho_hh_ = 0.5;
+  CameraState cam_;
   double rot_[16], trans_[16];
   double rescale_[3] = {1.0, 1.0, 1.0};
 
@@ -183,6 +308,7 @@ class HipRenderer {
   int batch_ = 8;
   int tile_group_ = 4;
   int xcd_split_ = 0;
+  int tile_shift_ = 3;  // 8 x 8 ray tiles
   int target_ = TVK_RT_INTERACTIVE;
 
   // progress
@@ -192,6 +318,7 @@ class HipRenderer {
   uint64_t subframes_ = 0;
   tvk_frame_stats stats_{};
   bool stats_pending_ = false;
+  LaunchSlot* stats_slot_ = nullptr;
 };
 
 }  // namespace tvk

@@ -104,6 +104,22 @@ int tvk_set_volume(tvk_renderer* r, const void* v, uint64_t nx, uint64_t ny, uin
   TVK_TRY(r->impl.SetVolume(v, d, bits, brick, apron, n_lod));
 }
 
+int tvk_set_paged_volume(tvk_renderer* r, uint64_t nx, uint64_t ny, uint64_t nz, uint32_t bits,
+                         uint32_t brick, uint32_t apron, uint32_t n_lod, tvk_brick_source_fn src,
+                         void* user, const float* ranges, uint64_t pool_bytes) {
+  TVK_GUARD(r);
+  const uint64_t d[3] = {nx, ny, nz};
+  TVK_TRY(r->impl.SetPagedVolume(d, bits, brick, apron, n_lod, src, user, ranges, pool_bytes));
+}
+
+int tvk_set_paged_flat_volume(tvk_renderer* r, const void* v, uint64_t nx, uint64_t ny, uint64_t nz,
+                              uint32_t bits, uint32_t brick, uint32_t apron, uint32_t n_lod,
+                              uint64_t pool_bytes) {
+  TVK_GUARD(r);
+  const uint64_t d[3] = {nx, ny, nz};
+  TVK_TRY(r->impl.SetPagedFlatVolume(v, d, bits, brick, apron, n_lod, pool_bytes));
+}
+
 int tvk_get_lod_level_count(tvk_renderer* r, uint32_t* n) {
   TVK_GUARD(r);
   if (!n) return TVK_ERR_ARG;
@@ -277,6 +293,16 @@ int tvk_set_kernel_tuning(tvk_renderer* r, uint32_t batch, uint32_t tile_group,
   return r->impl.SetKernelTuning(batch, tile_group, xcd_split);
 }
 
+int tvk_set_ray_tile_shape(tvk_renderer* r, uint32_t tile_width) {
+  TVK_GUARD(r);
+  return r->impl.SetRayTileShape(tile_width);
+}
+
+int tvk_set_empty_space_cell(tvk_renderer* r, uint32_t cell) {
+  TVK_GUARD(r);
+  return r->impl.SetEmptySpaceCell(cell);
+}
+
 int tvk_paint(tvk_renderer* r) {
   TVK_GUARD(r);
   TVK_TRY(r->impl.Paint());
@@ -316,6 +342,33 @@ int tvk_get_device_framebuffer(tvk_renderer* r, void** dptr, void** stream) {
 int tvk_synchronize(tvk_renderer* r) {
   TVK_GUARD(r);
   return r->impl.Synchronize();
+}
+
+int tvk_set_1d_trans_slot(tvk_renderer* r, uint32_t slot, const float* rgba, uint32_t n) {
+  TVK_GUARD(r);
+  TVK_TRY(r->impl.Set1DTransSlot(slot, rgba, n));
+}
+
+int tvk_set_2d_trans_slot(tvk_renderer* r, uint32_t slot, const float* rgba, uint32_t w, uint32_t h,
+                          float max_grad) {
+  TVK_GUARD(r);
+  TVK_TRY(r->impl.Set2DTransSlot(slot, rgba, w, h, max_grad));
+}
+
+int tvk_render_batch(tvk_renderer* r, const tvk_view* views, uint32_t n, tvk_view_stats* stats,
+                     float* kernel_ms) {
+  TVK_GUARD(r);
+  TVK_TRY(r->impl.RenderBatch(views, n, stats, kernel_ms));
+}
+
+int tvk_get_batch_framebuffer(tvk_renderer* r, uint32_t index, void** dptr) {
+  TVK_GUARD(r);
+  return r->impl.BatchFramebuffer(index, dptr);
+}
+
+int tvk_set_launch_graphs(tvk_renderer* r, int enabled) {
+  TVK_GUARD(r);
+  return r->impl.SetLaunchGraphs(enabled != 0);
 }
 
 int tvk_set_max_usable_gpu_mem(tvk_renderer* r, uint64_t bytes) {

@@ -12,6 +12,9 @@
 //   cell ranges  uint32 (min | max << 16) per empty-space cell of edge C
 //                (C divides the brick size), over the voxels
 //                [c*C - 1, (c+1)*C] that trilinear samples in the cell read.
+//   cell flags   uint8 per cell and TF slot: 1 when the TF is transparent
+//                over the cell's range (rebuilt when the TF, mode or the
+//                cell ranges change).
 //   TF           n_tf float4 (RGBA) + int32 prefix count of alpha > 0.
 //   framebuffer  W*H uint32 RGBA8 (r in the low byte), row 0 at the top.
 #pragma once
@@ -31,7 +34,7 @@ struct LodInfo {
   int32_t cell_base;    // index of the level's first cell in the cell-range grid
 };
 
-// Arguments of one raycast launch (passed by value as kernel arguments).
+// One job of a raycast launch (an entry of the device job table).
 struct RaycastParams {
   // Camera in LOD-0 voxel space.  Perspective: ray origin `eye`, direction
   // normalize(dir00 + x*du + y*dv) for pixel (x, y).  Orthographic: origin
@@ -43,6 +46,7 @@ struct RaycastParams {
   float odir[3];
   int32_t ortho;
   int32_t width, height;
+  int32_t tile_shift;   // ray tile = (1 << tile_shift) x (64 >> tile_shift) rays
   int32_t tiles_x, tiles_y;
   float box[3];         // LOD-0 dims (the volume box is [0, box])
   float step;           // sample spacing in LOD-0 voxels (2^lod / rate)
@@ -78,12 +82,44 @@ struct RaycastParams {
   const int32_t* page_table;
   const float* brick_range;     // 2 floats per brick
   const uint32_t* cell_range;   // per cell: min | max << 16 (raw values)
+  const uint8_t* cell_empty;    // per cell: 1 when this job's TF is transparent over it
   int32_t cell;                 // march unit edge in level voxels (divides brick)
   const float* tf;              // 4 floats per entry
   const int32_t* tf_prefix;     // n_tf + 1
+  // paged pools: a ray that needs a non-resident brick sets requests[brick]
+  // = 1; every brick a ray samples gets touch[brick] = frame (LRU).  Both
+  // are null for in-core volumes.
+  uint32_t* requests;
+  uint32_t* touch;
+  uint32_t frame;
   uint32_t* framebuffer;
-  unsigned long long* counters; // [0] evaluated samples, [1] nominal samples
 };
+
+// Launch-wide part of a raycast launch.  One launch renders any number of
+// jobs (views x TFs x volumes that share a kernel variant); their ray tiles
+// form one index space that the persistent kernel's waves drain.
+constexpr int kMaxJobs = 64;
+constexpr int kQueues = 8;  // one tile queue per XCD
+struct JobTableHeader {
+  int32_t n_jobs;
+  int32_t total_tiles;
+  int32_t tile_group;   // consecutive ray tiles dealt to one XCD in turn
+  int32_t xcd_split;    // one-job launches: 0 round-robin runs; rx: rx x (8/rx) regions
+  int32_t n_tf_lds;     // > 0: every job reads this 1D TF, staged into LDS
+  int32_t pad_[3];
+  const float* tf_lds;
+  // [0, 8) per-XCD tile queue heads, then per job j: [8 + 2j] evaluated
+  // samples, [9 + 2j] nominal samples
+  unsigned long long* counters;
+};
+
+// The job table as the kernel reads it from device memory.
+struct JobTable {
+  JobTableHeader h;
+  int32_t tile_start[kMaxJobs + 8];  // prefix sum of the jobs' tile counts
+  RaycastParams jobs[kMaxJobs];
+};
+constexpr int kCounterWords = kQueues + 2 * kMaxJobs;
 
 // One brick to cut out of a flat level into a pool slot.
 struct BrickJob {

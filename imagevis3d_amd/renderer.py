@@ -28,7 +28,7 @@ from typing import Sequence
 import numpy as np
 
 from . import _lib
-from ._lib import FrameStats, MemStats, TvkError, call
+from ._lib import FrameStats, MemStats, TvkError, View, ViewStats, call
 
 
 class _Types:
@@ -151,6 +151,38 @@ class Renderer:
         bits = {np.dtype(np.uint8): 8, np.dtype(np.uint16): 16}[vol.dtype]
         nz, ny, nx = vol.shape
         call("tvk_set_volume", self._h, vol.ctypes.data, nx, ny, nz, bits, brick_size, apron, n_lod)
+        self._bits = bits
+
+    def setPagedFlatVolume(self, vol: np.ndarray, brick_size: int, pool_bytes: int,
+                           apron: int = 2, n_lod: int = 0) -> None:
+        """Out-of-core volume: LOD 0 stays in this host array (kept alive by
+        the renderer), bricks are paged into a pool of `pool_bytes`."""
+        vol = np.ascontiguousarray(vol)
+        bits = {np.dtype(np.uint8): 8, np.dtype(np.uint16): 16}[vol.dtype]
+        nz, ny, nx = vol.shape
+        call("tvk_set_paged_flat_volume", self._h, vol.ctypes.data, nx, ny, nz, bits, brick_size,
+             apron, n_lod, int(pool_bytes))
+        self._paged_keep = vol
+        self._bits = bits
+
+    def setPagedVolume(self, dims, bits: int, brick_size: int, source, pool_bytes: int,
+                       brick_ranges=None, apron: int = 2, n_lod: int = 0) -> None:
+        """Out-of-core volume from a brick source:
+        ``source(lod, bx, by, bz) -> ndarray`` of the (B+2a)^3 stored brick."""
+        def fill(user, lod, bx, by, bz, slot, nbytes):
+            try:
+                b = np.ascontiguousarray(source(lod, bx, by, bz))
+                C.memmove(slot, b.ctypes.data, min(b.nbytes, nbytes))
+                return 0
+            except Exception:  # noqa: BLE001 -- reported to the C side as a failure
+                return -1
+        cb = _lib.BRICK_SOURCE(fill)
+        rng = None if brick_ranges is None else np.ascontiguousarray(brick_ranges, np.float32)
+        nx, ny, nz = (int(d) for d in dims)
+        call("tvk_set_paged_volume", self._h, nx, ny, nz, bits, brick_size, apron, n_lod,
+             C.cast(cb, C.c_void_p), None, None if rng is None else rng.ctypes.data,
+             int(pool_bytes))
+        self._paged_keep = (cb, rng)
         self._bits = bits
 
     def _bit_width(self) -> int:
@@ -339,6 +371,64 @@ class Renderer:
         s, sp = _f4(specular)
         l, lp = _f4(light_dir)
         call("tvk_set_light_colors", self._h, ap, dp, sp, lp)
+
+    def set1DTransSlot(self, slot: int, rgba: np.ndarray) -> None:
+        """1D TF of TF slot `slot` (0 = set1DTrans) for batched views."""
+        a = np.ascontiguousarray(np.asarray(rgba, np.float32).reshape(-1, 4))
+        call("tvk_set_1d_trans_slot", self._h, int(slot), a.ctypes.data, a.shape[0])
+
+    def set2DTransSlot(self, slot: int, rgba: np.ndarray, max_grad: float) -> None:
+        """2D TF of TF slot `slot` (0 = set2DTrans) for batched views."""
+        a = np.ascontiguousarray(np.asarray(rgba, np.float32))
+        h, w = a.shape[:2]
+        call("tvk_set_2d_trans_slot", self._h, int(slot), a.ctypes.data, w, h, float(max_grad))
+
+    def renderBatch(self, views, readback: bool = True):
+        """Render many views in one launch.  `views` holds objects with
+        view_pos, view_dir, up_dir, fov, ortho, ortho_half_height, width,
+        height and tf_slot (a workloads.Camera plus size/slot, or dicts).
+        Returns (images or None, [ViewStats], kernel_ms)."""
+        n = len(views)
+        arr = (View * max(n, 1))()
+        imgs = []
+        for i, v in enumerate(views):
+            g = v.get if isinstance(v, dict) else (lambda k, d=None, v=v: getattr(v, k, d))
+            x = arr[i]
+            x.view_pos[:] = [float(t) for t in g("view_pos")]
+            x.view_dir[:] = [float(t) for t in g("view_dir")]
+            x.up_dir[:] = [float(t) for t in g("up_dir", (0.0, 1.0, 0.0))]
+            x.fov_deg = float(g("fov", 50.0))
+            x.ortho = int(bool(g("ortho", False)))
+            x.ortho_half_height = float(g("ortho_half_height", 0.5))
+            x.width, x.height = int(g("width")), int(g("height"))
+            x.tf_slot = int(g("tf_slot", 0))
+            if readback:
+                img = np.empty((x.height, x.width, 4), np.uint8)
+                imgs.append(img)
+                x.rgba = img.ctypes.data
+            else:
+                x.rgba = None
+        stats = (ViewStats * max(n, 1))()
+        kms = C.c_float(0.0)
+        call("tvk_render_batch", self._h, arr, n, stats, C.byref(kms))
+        return (imgs if readback else None), list(stats)[:n], kms.value
+
+    def batchFramebuffer(self, index: int) -> int:
+        """Device pointer of view `index` of the last batch."""
+        p = C.c_void_p()
+        call("tvk_get_batch_framebuffer", self._h, int(index), C.byref(p))
+        return p.value
+
+    def setLaunchGraphs(self, on: bool) -> None:
+        call("tvk_set_launch_graphs", self._h, int(bool(on)))
+
+    def setRayTileShape(self, tile_width: int) -> None:
+        """Ray tile of one wavefront: tile_width x 64/tile_width (no image change)."""
+        call("tvk_set_ray_tile_shape", self._h, int(tile_width))
+
+    def setEmptySpaceCell(self, cell: int) -> None:
+        """Empty-space cell edge for the next setVolume (no image change)."""
+        call("tvk_set_empty_space_cell", self._h, int(cell))
 
     def setEarlyRayTermination(self, alpha: float) -> None:
         call("tvk_set_early_ray_termination", self._h, float(alpha))

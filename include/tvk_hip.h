@@ -136,6 +136,31 @@ int tvk_load_dataset(tvk_renderer* r, const char* uvf_path, int verify);
 int tvk_set_volume(tvk_renderer* r, const void* voxels, uint64_t nx, uint64_t ny,
                    uint64_t nz, uint32_t bits, uint32_t brick_size, uint32_t apron,
                    uint32_t n_lod);
+/* Out-of-core datasets on a fixed HBM brick pool (GLGridLeaper paging,
+ * RenderWindowGL.cpp:153-163; the data side is UVFDataset::GetBrick,
+ * doc/import.adoc:761-775).  A brick source fills one stored brick:
+ * (B+2a)^3 voxels of LOD `lod`, brick (bx,by,bz), x fastest, clamp-to-edge
+ * outside the level, the same contents tvk_read_brick returns; it returns 0
+ * on success.  It is called from the painting thread. */
+typedef int (*tvk_brick_source_fn)(void* user, uint32_t lod, uint32_t bx, uint32_t by,
+                                   uint32_t bz, void* slot, uint64_t slot_bytes);
+/* Paged volume from a brick source.  brick_ranges: 2 floats (min, max) per
+ * brick of every LOD (LOD-major, then z, y, x) over its stored voxels, or
+ * NULL (no empty-space skipping before a brick is resident).  The pool
+ * holds pool_bytes / slot_bytes bricks; paint() pages bricks in on demand
+ * (LRU eviction), needs more passes while bricks arrive (checkForRedraw
+ * stays 1) and falls back to a coarser LOD when one view's bricks exceed
+ * the pool. */
+int tvk_set_paged_volume(tvk_renderer* r, uint64_t nx, uint64_t ny, uint64_t nz, uint32_t bits,
+                         uint32_t brick_size, uint32_t apron, uint32_t n_lod,
+                         tvk_brick_source_fn source, void* user, const float* brick_ranges,
+                         uint64_t pool_bytes);
+/* Paged volume whose LOD 0 stays in caller memory (kept valid until the
+ * volume is replaced); coarser levels and brick ranges are built on host
+ * threads with the in-core rules. */
+int tvk_set_paged_flat_volume(tvk_renderer* r, const void* voxels, uint64_t nx, uint64_t ny,
+                              uint64_t nz, uint32_t bits, uint32_t brick_size, uint32_t apron,
+                              uint32_t n_lod, uint64_t pool_bytes);
 /* dataset.getLODLevelCount / getDomainSize(lod) -- ImageVis3D_Progress.cpp:104-117. */
 int tvk_get_lod_level_count(tvk_renderer* r, uint32_t* n_lod);
 int tvk_get_domain_size(tvk_renderer* r, uint32_t lod, uint64_t dims[3]);
@@ -236,6 +261,15 @@ int tvk_set_empty_space_skipping(tvk_renderer* r, int enabled);
  * any of them. */
 int tvk_set_kernel_tuning(tvk_renderer* r, uint32_t samples_per_batch, uint32_t tile_group,
                           uint32_t xcd_split);
+/* Ray tile shape of one 64-lane wavefront (no counterpart in the
+ * reference; chosen by the autotuner): tile_width x (64 / tile_width) rays,
+ * tile_width in {4, 8, 16} (default 8).  The image does not depend on it. */
+int tvk_set_ray_tile_shape(tvk_renderer* r, uint32_t tile_width);
+/* Empty-space cell edge in voxels of the rendered LOD (no counterpart in the
+ * reference): 4..brick size, a divisor of the brick size (default 16, or the
+ * brick size when that is smaller).  Takes effect at the next setVolume /
+ * loadDataset.  The image does not depend on it. */
+int tvk_set_empty_space_cell(tvk_renderer* r, uint32_t cell);
 
 /* ---------------------------------------------------------------- render */
 /* renderer.paint() -- RenderWindow.cpp:2464, BatchRenderer.lua:35.  Renders
@@ -263,6 +297,54 @@ int tvk_get_frame_stats(tvk_renderer* r, tvk_frame_stats* out);
 int tvk_get_device_framebuffer(tvk_renderer* r, void** dptr, void** hip_stream);
 /* Block until this handle's stream has drained. */
 int tvk_synchronize(tvk_renderer* r);
+
+/* -------------------------------------------------------------- batching */
+/* Batched rendering (no counterpart in the reference, which renders one
+ * view per paint; BatchRenderer.lua:22-41 loops paint + capture per image).
+ * Many views of the handle's volume -- each with its own camera, image size
+ * and transfer-function slot -- are rendered by one persistent kernel
+ * launch whose waves drain the ray tiles of all views; all other state
+ * (render mode, lighting, sampling rate, background, region transforms,
+ * finest LOD limit) is the handle's.  Each view's image equals what
+ * tvk_capture_single_frame gives for the same camera and TF, bit for bit. */
+typedef struct tvk_view {
+  float view_pos[3];
+  float view_dir[3];
+  float up_dir[3];
+  float fov_deg;            /* perspective vertical field of view        */
+  int32_t ortho;            /* 1: orthographic with ortho_half_height     */
+  float ortho_half_height;
+  uint32_t width, height;
+  uint32_t tf_slot;         /* transfer function slot (0 = setXDTrans)    */
+  uint8_t* rgba;            /* host RGBA8 output (W*H*4 bytes) or NULL     */
+} tvk_view;
+
+typedef struct tvk_view_stats {
+  uint64_t evaluated_samples;
+  uint64_t nominal_samples;
+  uint32_t lod;
+  uint32_t reserved;
+} tvk_view_stats;
+
+/* Transfer-function slots 0..63 for batched views (slot 0 is the one
+ * tvk_set_1d_trans / tvk_set_2d_trans set); same conventions as those. */
+int tvk_set_1d_trans_slot(tvk_renderer* r, uint32_t slot, const float* rgba, uint32_t n_entries);
+int tvk_set_2d_trans_slot(tvk_renderer* r, uint32_t slot, const float* rgba, uint32_t width,
+                          uint32_t height, float max_grad);
+/* Render n views (any n; up to 64 share one launch).  `stats` (may be NULL)
+ * receives n entries; `kernel_ms` (may be NULL) the summed raycast launch
+ * time.  Views with rgba == NULL stay on the device
+ * (tvk_get_batch_framebuffer).  Paged volumes are paged in as for a
+ * capture. */
+int tvk_render_batch(tvk_renderer* r, const tvk_view* views, uint32_t n, tvk_view_stats* stats,
+                     float* kernel_ms);
+/* Device pointer of view `index` of the last batch (RGBA8, row 0 at top). */
+int tvk_get_batch_framebuffer(tvk_renderer* r, uint32_t index, void** dptr);
+/* Replay batched launches from captured HIP graphs.  Currently switched
+ * off: enabling returns TVK_ERR_STATE and batches launch eagerly (the
+ * first graph-replayed launch did not return on the device and the cause
+ * is not yet known).  The images do not depend on it. */
+int tvk_set_launch_graphs(tvk_renderer* r, int enabled);
 
 /* ---------------------------------------------------------------- memory */
 /* SysInfo()->SetMaxUsableGPUMem + MemMan()->MemSizesChanged()

@@ -31,6 +31,8 @@ SOURCES = [
 COMMON = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # wave-uniform atomics are reduced with DPP row operations, not a loop
+    "-mllvm", "-amdgpu-atomic-optimizer-strategy=DPP",
     "-I", os.path.join(HERE, "..", "include"),
 ]
 

@@ -52,9 +52,34 @@ __device__ inline uint32_t pack_rgba8(float r, float g, float b, float a) {
   return R | (G << 8) | (B << 16) | (A << 24);
 }
 
+// Relaxed device-scope atomic add on global memory.  Called by every lane
+// of a wave with a wave-uniform address, it is reduced by the compiler to
+// one atomic per wave; each lane gets the old value plus the sum of the
+// lower lanes' operands.
+typedef __attribute__((address_space(1))) unsigned long long global_u64;
+__device__ inline unsigned long long wave_atomic_add(unsigned long long* p, unsigned long long v) {
+  return __hip_atomic_fetch_add((global_u64*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <typename T>
 __device__ inline float ld(const T* p) {
   return (float)*p;
+}
+
+// Two x-neighbours (p[0], p[1]) with one load: a dword for 16-bit voxels, a
+// short for 8-bit ones (unaligned global loads are legal on gfx950), which
+// halves the vector-memory instructions of a trilinear fetch.
+__device__ inline void ld2(const uint16_t* p, float& a, float& b) {
+  uint32_t v;
+  __builtin_memcpy(&v, p, 4);
+  a = (float)(v & 0xffffu);
+  b = (float)(v >> 16);
+}
+__device__ inline void ld2(const uint8_t* p, float& a, float& b) {
+  uint16_t v;
+  __builtin_memcpy(&v, p, 2);
+  a = (float)(v & 0xffu);
+  b = (float)(v >> 8);
 }
 
 // One sample from a brick slot at slot-local texel coordinate (gx, gy, gz):
@@ -71,10 +96,11 @@ __device__ inline float fetch(const T* __restrict__ base, int S, int S2, float g
     const float wy = clampf(gy - (float)iy, 0.f, 1.f);
     const float wz = clampf(gz - (float)iz, 0.f, 1.f);
     const T* p = base + (iz * S2 + iy * S + ix);
-    const float c000 = ld(p), c100 = ld(p + 1);
-    const float c010 = ld(p + S), c110 = ld(p + S + 1);
-    const float c001 = ld(p + S2), c101 = ld(p + S2 + 1);
-    const float c011 = ld(p + S2 + S), c111 = ld(p + S2 + S + 1);
+    float c000, c100, c010, c110, c001, c101, c011, c111;
+    ld2(p, c000, c100);
+    ld2(p + S, c010, c110);
+    ld2(p + S2, c001, c101);
+    ld2(p + S2 + S, c011, c111);
     const float c00 = fmaf(c100 - c000, wx, c000);
     const float c10 = fmaf(c110 - c010, wx, c010);
     const float c01 = fmaf(c101 - c001, wx, c001);
@@ -475,13 +501,15 @@ __global__ void cell_empty_kernel(const uint32_t* __restrict__ cell_range, int64
 template <typename T, bool LINEAR, int BATCH, bool TF_LDS, int MODE>
 __global__ __launch_bounds__(kThreads, 2) void raycast_kernel(const JobTable* __restrict__ tab) {
   extern __shared__ float4 s_tf[];  // n_tf_lds entries when TF_LDS
+  __shared__ unsigned long long s_count[2 * kMaxJobs];  // per job: evaluated, nominal
   const JobTableHeader& H = tab->h;
   const float4* tf_shared = (const float4*)H.tf_lds;
   if (TF_LDS) {
     for (int i = (int)threadIdx.x; i < H.n_tf_lds; i += kThreads) s_tf[i] = tf_shared[i];
-    __syncthreads();
     tf_shared = s_tf;
   }
+  for (int i = (int)threadIdx.x; i < 2 * kMaxJobs; i += kThreads) s_count[i] = 0;
+  __syncthreads();
   const int lane = (int)(threadIdx.x & 63);
   const int xcd = (int)(blockIdx.x & 7);
   unsigned long long* queues = H.counters;
@@ -491,6 +519,11 @@ __global__ __launch_bounds__(kThreads, 2) void raycast_kernel(const JobTable* __
   // xcd_split regions apply to one-job launches only
   const int split = n_jobs == 1 ? H.xcd_split : 0;
   const int tiles_x0 = tab->jobs[0].tiles_x, tiles_y0 = tab->jobs[0].tiles_y;
+  // per-lane sample counts of the job the wave is on; added to the
+  // work-group's LDS counters when the wave moves to another job and at the
+  // end, then the work-group adds its counters to the job's global ones
+  int cur_job = -1;
+  uint32_t n_eval = 0, n_nom = 0;
   for (int q = 0; q < kQueues; ++q) {
     const int x = (xcd + q) & 7;  // own queue first, then steal
     // Queue x enumerates either runs of G tiles dealt round-robin
@@ -510,9 +543,13 @@ __global__ __launch_bounds__(kThreads, 2) void raycast_kernel(const JobTable* __
       slots = rw * rh;
     }
     while (true) {
-      int pos = 0;
-      if (lane == 0) pos = (int)atomicAdd(&queues[x], 1ull);
-      pos = __builtin_amdgcn_readfirstlane(__shfl(pos, 0, 64));
+      // Every lane takes part in the queue atomic (lanes other than 0 add
+      // 0; the compiler reduces it to one atomic per wave) and all branches
+      // in this loop are wave-uniform: no lane-0-only block in the source
+      // that the compiler could split off the loop around the wave-wide
+      // broadcast of the ticket.
+      const unsigned long long ticket = wave_atomic_add(&queues[x], lane == 0 ? 1ull : 0ull);
+      const int pos = __builtin_amdgcn_readfirstlane((int)__shfl((long long)ticket, 0, 64));
       if (pos >= slots) break;
       int tile;
       if (split == 0) tile = ((pos / G) * 8 + x) * G + (pos % G);
@@ -528,20 +565,24 @@ __global__ __launch_bounds__(kThreads, 2) void raycast_kernel(const JobTable* __
       const int job = __builtin_amdgcn_readfirstlane(lo);
       const RaycastParams& P = tab->jobs[job];
       const float4* tf4 = TF_LDS ? tf_shared : (const float4*)P.tf;
-      uint32_t n_eval = 0, n_nom = 0;
+      if (job != cur_job) {  // wave-uniform
+        if (cur_job >= 0) {
+          atomicAdd(&s_count[2 * cur_job], (unsigned long long)n_eval);
+          atomicAdd(&s_count[2 * cur_job + 1], (unsigned long long)n_nom);
+        }
+        cur_job = job;
+        n_eval = n_nom = 0;
+      }
       render_tile<T, LINEAR, BATCH, MODE>(P, tf4, tile - tab->tile_start[job], lane, n_eval, n_nom);
-      // one atomic per tile for each of the job's counters
-      unsigned long long e = n_eval, m = n_nom;
-      for (int off = 32; off > 0; off >>= 1) {
-        e += __shfl_xor(e, off, 64);
-        m += __shfl_xor(m, off, 64);
-      }
-      if (lane == 0) {
-        atomicAdd(&H.counters[kQueues + 2 * job], e);
-        atomicAdd(&H.counters[kQueues + 2 * job + 1], m);
-      }
     }
   }
+  if (cur_job >= 0) {
+    atomicAdd(&s_count[2 * cur_job], (unsigned long long)n_eval);
+    atomicAdd(&s_count[2 * cur_job + 1], (unsigned long long)n_nom);
+  }
+  __syncthreads();
+  for (int i = (int)threadIdx.x; i < 2 * n_jobs; i += kThreads)
+    if (s_count[i] != 0) atomicAdd(&H.counters[kQueues + i], s_count[i]);
 }
 
 template <typename T, bool LINEAR, int BATCH, bool TF_LDS, int MODE>

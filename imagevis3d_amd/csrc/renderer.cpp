@@ -1250,12 +1250,7 @@ int HipRenderer::RenderBatch(const tvk_view* views, uint32_t n, tvk_view_stats* 
 }
 
 int HipRenderer::SetLaunchGraphs(bool on) {
-  // The first batched launch replayed from a captured graph did not return
-  // on the device (the capture also held the kernel's first use in the
-  // process); until that is understood the graph path stays off and
-  // batches launch eagerly.
-  if (on) return state_fail("graph replay of batched launches is switched off in this build");
-  use_graphs_ = false;
+  use_graphs_ = on;
   return TVK_OK;
 }
 

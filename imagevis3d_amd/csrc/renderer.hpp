@@ -244,7 +244,7 @@ class HipRenderer {
   LaunchSlot batch_slot_;
   DevBuf batch_fb_;
   std::vector<uint64_t> batch_fb_offset_;  // per view of the last batch
-  bool use_graphs_ = false;  // graph replay is switched off (see SetLaunchGraphs)
+  bool use_graphs_ = true;  // batched launches replay captured graphs
   uint64_t gpu_total_ = 0, gpu_budget_ = 0;
   int compute_units_ = 0;
 

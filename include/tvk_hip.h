@@ -340,10 +340,10 @@ int tvk_render_batch(tvk_renderer* r, const tvk_view* views, uint32_t n, tvk_vie
                      float* kernel_ms);
 /* Device pointer of view `index` of the last batch (RGBA8, row 0 at top). */
 int tvk_get_batch_framebuffer(tvk_renderer* r, uint32_t index, void** dptr);
-/* Replay batched launches from captured HIP graphs.  Currently switched
- * off: enabling returns TVK_ERR_STATE and batches launch eagerly (the
- * first graph-replayed launch did not return on the device and the cause
- * is not yet known).  The images do not depend on it. */
+/* Replay batched launches from captured HIP graphs (default on): table
+ * upload, counter clear, raycast and counter readback are one graph per
+ * kernel variant, re-captured when the variant changes.  Off runs the same
+ * sequence eagerly.  The images do not depend on it. */
 int tvk_set_launch_graphs(tvk_renderer* r, int enabled);
 
 /* ---------------------------------------------------------------- memory */

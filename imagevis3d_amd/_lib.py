@@ -9,7 +9,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtvkhip.so")
+# IMAGEVIS3D_AMD_LIB selects another build of the same library (A/B runs)
+LIB_PATH = os.environ.get("IMAGEVIS3D_AMD_LIB") or os.path.join(HERE, "libtvkhip.so")
 
 TVK_OK = 0
 ERRORS = {

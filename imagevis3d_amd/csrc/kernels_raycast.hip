@@ -422,38 +422,35 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
         continue;
       }
       // Batched 1D march: gather BATCH samples (clamped to the brick's last
-      // sample), look all of them up in the TF, then composite in order.
+      // sample) so their loads are in flight together, then look each up in
+      // the TF and composite in order (TF entries are read per sample, which
+      // keeps the register count down).
       for (; k < kend; k += BATCH) {
-        float v[BATCH], len[BATCH];
+        float v[BATCH];
 #pragma unroll
         for (int j = 0; j < BATCH; ++j) {
           const int kk = min(k + j, kend - 1);
-          len[j] = (kk == n - 1) ? lastlen : step;
-          const float t = fmaf((float)kk, step, t0) + 0.5f * len[j];
+          const float lenj = (kk == n - 1) ? lastlen : step;
+          const float t = fmaf((float)kk, step, t0) + 0.5f * lenj;
           v[j] = fetch<T, LINEAR>(base, S, S2, fmaf(t, od[0], g0[0]), fmaf(t, od[1], g0[1]),
                                   fmaf(t, od[2], g0[2]));
-        }
-        float4 e0[BATCH], e1[BATCH];
-        float fr[BATCH];
-#pragma unroll
-        for (int j = 0; j < BATCH; ++j) {
-          const float x = clampf(v[j] * P.tf_scale, 0.f, tfmax);
-          const int ti = (int)x;
-          fr[j] = x - (float)ti;
-          e0[j] = tf4[ti];
-          e1[j] = tf4[min(ti + 1, P.n_tf - 1)];
         }
 #pragma unroll
         for (int j = 0; j < BATCH; ++j) {
           if (k + j < kend && live) {
             ++n_eval;
-            const float a = fmaf(e1[j].w - e0[j].w, fr[j], e0[j].w);
+            const float x = clampf(v[j] * P.tf_scale, 0.f, tfmax);
+            const int ti = (int)x;
+            const float fr = x - (float)ti;
+            const float4 e0 = tf4[ti], e1 = tf4[min(ti + 1, P.n_tf - 1)];
+            const float a = fmaf(e1.w - e0.w, fr, e0.w);
             if (a > 0.f) {
-              const float ac = a >= 1.f ? 1.f : 1.f - exp2f(len[j] * log2f(1.f - a));
+              const float lenj = (k + j == n - 1) ? lastlen : step;
+              const float ac = a >= 1.f ? 1.f : 1.f - exp2f(lenj * log2f(1.f - a));
               const float w = (1.f - A) * ac;
-              C0 = fmaf(w, fmaf(e1[j].x - e0[j].x, fr[j], e0[j].x), C0);
-              C1 = fmaf(w, fmaf(e1[j].y - e0[j].y, fr[j], e0[j].y), C1);
-              C2 = fmaf(w, fmaf(e1[j].z - e0[j].z, fr[j], e0[j].z), C2);
+              C0 = fmaf(w, fmaf(e1.x - e0.x, fr, e0.x), C0);
+              C1 = fmaf(w, fmaf(e1.y - e0.y, fr, e0.y), C1);
+              C2 = fmaf(w, fmaf(e1.z - e0.z, fr, e0.z), C2);
               A += w;
               live = A < P.term_alpha;
             }

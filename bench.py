@@ -80,6 +80,8 @@ def main() -> int:
     ap.add_argument("--tile-width", type=int, default=8, help="ray tile width (4, 8, 16)")
     ap.add_argument("--cell", type=int, default=16, help="empty-space cell edge in voxels")
     ap.add_argument("--no-skip", action="store_true", help="march whole bricks (no cell skipping)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the given launch settings instead of autotuning them")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
@@ -106,6 +108,20 @@ def main() -> int:
     workloads.apply(r, w, w.cameras[0])
     if args.no_skip:
         r.setEmptySpaceSkipping(False)
+    tuning = None
+    if not args.no_autotune:
+        # empty-space cell and launch shape for this workload (the brick
+        # size stays the configured 256^3), timed on 4 orbit views at full
+        # size (untimed setup)
+        tv = [dict(view_pos=c.view_pos, view_dir=c.view_dir, up_dir=c.up_dir, fov=c.fov,
+                   width=w.width, height=w.height) for c in w.cameras[::9]]
+        t0 = time.perf_counter()
+        t = r.autotune(tv, flags=r.TUNE_CELL | r.TUNE_LAUNCH)
+        tuning = {"brick": t.brick_size, "cell": t.cell, "samples_per_batch": t.samples_per_batch,
+                  "tile_width": t.tile_width, "tile_group": t.tile_group,
+                  "tuned_ms": round(t.kernel_ms, 4), "start_ms": round(t.baseline_ms, 4),
+                  "settings_timed": t.candidates, "seconds": round(time.perf_counter() - t0, 2)}
+        print(f"[bench] autotuned {tuning}", file=sys.stderr, flush=True)
     print(f"[bench] bricked into the device pool in {brick_s:.2f} s; warmup {args.warmup}",
           file=sys.stderr, flush=True)
 
@@ -155,9 +171,9 @@ def main() -> int:
         "config": {"workload": w.name, "volume": "512x512x512 u16", "image": "1920x1080",
                    "views": len(cams), "rate": w.rate, "bricks": "256^3 + 2-voxel apron",
                    "lod": 0, "tf": "4096 x RGBA f32, 3 Gaussian bumps", "fov_deg": 60},
-        "launch": {"samples_per_batch": args.batch, "tile_group": args.tile_group,
-                   "xcd_split": args.xcd_split, "tile_width": args.tile_width,
-                   "cell": 0 if args.no_skip else args.cell},
+        "launch": tuning or {"samples_per_batch": args.batch, "tile_group": args.tile_group,
+                             "xcd_split": args.xcd_split, "tile_width": args.tile_width,
+                             "brick": w.brick, "cell": 0 if args.no_skip else args.cell},
         "fps": args.steps / wall,
         "evaluated_gsamples_per_s": evaluated / wall / 1e9,
         "nominal_samples_per_frame": nominal // args.steps,

@@ -63,6 +63,15 @@ class ViewStats(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [
+        ("brick_size", C.c_uint32), ("cell", C.c_uint32), ("samples_per_batch", C.c_uint32),
+        ("tile_width", C.c_uint32), ("tile_group", C.c_uint32), ("xcd_split", C.c_uint32),
+        ("kernel_ms", C.c_float), ("baseline_ms", C.c_float), ("candidates", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
 H = C.c_void_p
 # tvk_brick_source_fn
 BRICK_SOURCE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -123,6 +132,7 @@ SIGNATURES = {
     "tvk_render_batch": [H, C.POINTER(View), u32, C.POINTER(ViewStats), C.POINTER(f32)],
     "tvk_get_batch_framebuffer": [H, u32, C.POINTER(C.c_void_p)],
     "tvk_set_launch_graphs": [H, i32],
+    "tvk_autotune": [H, C.POINTER(View), u32, u32, C.POINTER(Tuning)],
     "tvk_paint": [H],
     "tvk_check_for_redraw": [H],
     "tvk_schedule_complete_redraw": [H],

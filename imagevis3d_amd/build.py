@@ -22,6 +22,7 @@ SOURCES = [
     "kernels_raycast.hip",
     "renderer.cpp",
     "paging.cpp",
+    "autotune.cpp",
     "host_volume.cpp",
     "uvf.cpp",
     "volume_gen.cpp",

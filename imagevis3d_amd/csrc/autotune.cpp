@@ -1,0 +1,117 @@
+// autotune.cpp -- HipRenderer::Autotune: coordinate descent over brick
+// size, empty-space cell edge and launch shape, timed with the renderer's
+// own batched launches (HIP events around each raycast launch).
+#include <algorithm>
+#include <vector>
+
+#include "errors.hpp"
+#include "renderer.hpp"
+
+namespace tvk {
+
+namespace {
+
+struct Setting {
+  uint32_t brick, cell, batch, tile_width, tile_group;
+};
+
+}  // namespace
+
+int HipRenderer::Autotune(const tvk_view* views, uint32_t n, uint32_t flags, tvk_tuning* out) {
+  if (!has_volume_) {
+    set_last_error("autotune before a volume is set");
+    return TVK_ERR_STATE;
+  }
+  if (!views || n == 0) {
+    set_last_error("autotune needs at least one view");
+    return TVK_ERR_ARG;
+  }
+  // images stay on the device while tuning
+  std::vector<tvk_view> v(views, views + n);
+  for (auto& x : v) x.rgba = nullptr;
+  int tried = 0;
+  // median of three timed repeats after one warm-up (graph capture, paging)
+  auto time_now = [&](float* ms) -> int {
+    float t[3];
+    if (int e = RenderBatch(v.data(), n, nullptr, &t[0])) return e;
+    for (float& x : t)
+      if (int e = RenderBatch(v.data(), n, nullptr, &x)) return e;
+    std::sort(t, t + 3);
+    *ms = t[1];
+    ++tried;
+    return TVK_OK;
+  };
+  Setting cur{(uint32_t)brick_, (uint32_t)cell_, (uint32_t)batch_, 1u << tile_shift_,
+              (uint32_t)tile_group_};
+  auto apply = [&](const Setting& s, bool rebrick) -> int {
+    if (rebrick) {
+      if (int e = Rebrick(s.brick, s.cell)) return e;
+    }
+    if (int e = SetKernelTuning(s.batch, s.tile_group, (uint32_t)xcd_split_)) return e;
+    return SetRayTileShape(s.tile_width);
+  };
+  float best = 0.f;
+  if (int e = time_now(&best)) return e;
+  const float baseline = best;
+
+  auto try_values = [&](const std::vector<uint32_t>& values, uint32_t Setting::*field,
+                        bool rebrick) -> int {
+    for (uint32_t val : values) {
+      if (cur.*field == val) continue;
+      Setting s = cur;
+      s.*field = val;
+      if (field == &Setting::brick) {
+        // keep the cell edge when it divides the new brick, else the brick
+        if (s.brick % s.cell != 0 || s.cell > s.brick) s.cell = std::min<uint32_t>(s.brick, 16);
+      }
+      if (int e = apply(s, rebrick)) {
+        // an unusable candidate (e.g. out of device memory): restore, go on
+        if (int e2 = apply(cur, rebrick)) return e2;
+        (void)e;
+        continue;
+      }
+      float ms = 0.f;
+      if (int e = time_now(&ms)) return e;
+      if (ms < best) {
+        best = ms;
+        cur = s;
+      }
+    }
+    return apply(cur, rebrick);
+  };
+
+  const uint64_t maxdim = std::max(dims0_[0], std::max(dims0_[1], dims0_[2]));
+  if ((flags & TVK_TUNE_BRICK) && !paged_) {
+    std::vector<uint32_t> bricks;
+    for (uint32_t b : {32u, 64u, 128u, 256u})
+      if (b <= std::max<uint64_t>(32, maxdim)) bricks.push_back(b);
+    if (int e = try_values(bricks, &Setting::brick, true)) return e;
+  }
+  if ((flags & TVK_TUNE_CELL) && !paged_) {
+    std::vector<uint32_t> cells;
+    for (uint32_t c = 8; c <= cur.brick; c *= 2)
+      if (cur.brick % c == 0) cells.push_back(c);
+    if (std::find(cells.begin(), cells.end(), cur.brick) == cells.end()) cells.push_back(cur.brick);
+    if (int e = try_values(cells, &Setting::cell, true)) return e;
+  }
+  if (flags & TVK_TUNE_LAUNCH) {
+    if (int e = try_values({4u, 8u}, &Setting::batch, false)) return e;
+    if (int e = try_values({4u, 8u, 16u}, &Setting::tile_width, false)) return e;
+    if (int e = try_values({1u, 2u, 4u, 8u, 16u}, &Setting::tile_group, false)) return e;
+  }
+  if (out) {
+    *out = tvk_tuning{};
+    out->brick_size = (uint32_t)brick_;
+    out->cell = (uint32_t)cell_;
+    out->samples_per_batch = (uint32_t)batch_;
+    out->tile_width = 1u << tile_shift_;
+    out->tile_group = (uint32_t)tile_group_;
+    out->xcd_split = (uint32_t)xcd_split_;
+    out->kernel_ms = best;
+    out->baseline_ms = baseline;
+    out->candidates = (uint32_t)tried;
+  }
+  return TVK_OK;
+}
+
+}  // namespace tvk

@@ -287,6 +287,7 @@ int HipRenderer::plan_levels(const uint64_t dims[3], uint32_t bits, uint32_t bri
   out->total_bricks = total;
   out->total_cells = total_cells;
   out->cell = cell;
+  out->brick = (int)brick;
   out->level_bytes = level_bytes;
   out->slot_edge = (int)S;
   out->slot_bytes = ((S * S * S * bpv) + 255) & ~255ull;
@@ -323,11 +324,8 @@ int HipRenderer::SetVolume(const void* voxels, const uint64_t dims[3], uint32_t 
   if (int e = plan_levels(dims, bits, brick, apron, n_lod, &plan)) return e;
   const int bpv = (int)bits / 8;
   const int nl = plan.n_lod, total = plan.total_bricks, total_cells = plan.total_cells;
-  const int cell = plan.cell;
-  const uint64_t S = (uint64_t)plan.slot_edge;
   const uint64_t level_bytes = plan.level_bytes;
-  LodInfo* lods = plan.lods;
-  uint64_t (*ld)[3] = plan.dims;
+  const uint64_t (*ld)[3] = plan.dims;
   const uint64_t slot_bytes = plan.slot_bytes;
   const uint64_t need = level_bytes + (uint64_t)total * slot_bytes + (uint64_t)total * 12 +
                         (uint64_t)total_cells * 4 +
@@ -349,6 +347,26 @@ int HipRenderer::SetVolume(const void* voxels, const uint64_t dims[3], uint32_t 
     TVK_HIP(launch_downsample(levels_[l - 1].p, bpv, sd, levels_[l].p, stream_));
   }
 
+  bits_ = bits;
+  for (int i = 0; i < 3; ++i) dims0_[i] = dims[i];
+  apron_ = (int)apron;
+  if (int e = brick_levels(plan)) return e;
+  lod_min_ = 0;
+  lod_max_ = (uint32_t)(nl - 1);
+  has_volume_ = true;
+  invalidate();
+  return TVK_OK;
+}
+
+// Cut the flat levels on the device (levels_) into a fresh brick pool of
+// the plan's brick size; per-brick and per-cell ranges; identity page table.
+int HipRenderer::brick_levels(const LevelPlan& plan) {
+  const int bpv = (int)bits_ / 8;
+  const int nl = plan.n_lod, total = plan.total_bricks, total_cells = plan.total_cells;
+  const int cell = plan.cell;
+  const uint64_t slot_bytes = plan.slot_bytes;
+  const int brick = plan.brick, apron = apron_;
+  const LodInfo* lods = plan.lods;
   TVK_HIP(pool_.slots.alloc((size_t)total * slot_bytes));
   TVK_HIP(pool_.page_table.alloc((size_t)total * sizeof(int32_t)));
   TVK_HIP(pool_.brick_range.alloc((size_t)total * 2 * sizeof(float)));
@@ -374,7 +392,7 @@ int HipRenderer::SetVolume(const void* voxels, const uint64_t dims[3], uint32_t 
     const int n = L.grid[0] * L.grid[1] * L.grid[2];
     const int64_t d[3] = {L.dims[0], L.dims[1], L.dims[2]};
     TVK_HIP(launch_extract_bricks(levels_[l].p, bpv, d, djobs.as<BrickJob>() + L.brick_base, n,
-                                  pool_.slots.p, stride, (int)brick, (int)apron,
+                                  pool_.slots.p, stride, brick, apron,
                                   pool_.brick_range.as<float>(), scratch_.as<uint32_t>(), stream_));
   }
   TVK_HIP(cell_range_.alloc((size_t)total_cells * sizeof(uint32_t)));
@@ -390,24 +408,39 @@ int HipRenderer::SetVolume(const void* voxels, const uint64_t dims[3], uint32_t 
   pool_.brick_range_h.resize((size_t)total * 2);
   TVK_HIP(hipMemcpyAsync(pool_.brick_range_h.data(), pool_.brick_range.p, total * 2 * sizeof(float),
                          hipMemcpyDeviceToHost, stream_));
+  // djobs and the host vectors must outlive the copies
   TVK_HIP(hipStreamSynchronize(stream_));
   pool_.resident = (uint64_t)total;
   pool_.uploaded += (uint64_t)total;
-
-  bits_ = bits;
-  for (int i = 0; i < 3; ++i) dims0_[i] = dims[i];
   n_lod_ = nl;
-  brick_ = (int)brick;
-  apron_ = (int)apron;
-  slot_edge_ = (int)S;
+  brick_ = brick;
+  slot_edge_ = plan.slot_edge;
   cell_ = cell;
   total_cells_ = total_cells;
   ++cells_gen_;
   for (int l = 0; l < nl; ++l) lods_[l] = lods[l];
   total_bricks_ = total;
-  lod_min_ = 0;
-  lod_max_ = (uint32_t)(nl - 1);
-  has_volume_ = true;
+  return TVK_OK;
+}
+
+int HipRenderer::Rebrick(uint32_t brick, uint32_t cell) {
+  if (!has_volume_) return state_fail("rebrick before a volume is set");
+  if (paged_) return state_fail("a paged volume keeps its brick size");
+  const int saved = cell_pref_;
+  if (cell) cell_pref_ = (int)cell;
+  LevelPlan plan;
+  const int e0 = plan_levels(dims0_, bits_, brick, (uint32_t)apron_, (uint32_t)n_lod_, &plan);
+  if (e0) {
+    cell_pref_ = saved;
+    return e0;
+  }
+  TVK_HIP(hipSetDevice(device_));
+  TVK_HIP(hipStreamSynchronize(stream_));
+  pool_.slots.reset();
+  pool_.page_table.reset();
+  pool_.brick_range.reset();
+  cell_range_.reset();
+  if (int e = brick_levels(plan)) return e;
   invalidate();
   return TVK_OK;
 }

@@ -68,7 +68,7 @@ struct BrickPool {
 struct LevelPlan {
   uint64_t dims[kMaxLod][3];
   LodInfo lods[kMaxLod];
-  int n_lod = 0, total_bricks = 0, total_cells = 0, cell = 16, slot_edge = 0;
+  int n_lod = 0, total_bricks = 0, total_cells = 0, cell = 16, slot_edge = 0, brick = 0;
   uint64_t level_bytes = 0, slot_bytes = 0;
 };
 
@@ -178,6 +178,10 @@ class HipRenderer {
   int SetEmptySpaceSkipping(bool on);
   int SetKernelTuning(uint32_t batch, uint32_t tile_group, uint32_t xcd_split);
   int SetRayTileShape(uint32_t tile_width);
+  // In-core volumes: cut the device-resident levels into bricks of another
+  // size / empty-space cell edge (cell 0 keeps the current preference).
+  int Rebrick(uint32_t brick, uint32_t cell);
+  int Autotune(const tvk_view* views, uint32_t n, uint32_t flags, tvk_tuning* out);
   int SetEmptySpaceCell(uint32_t cell);
 
   int SetLaunchGraphs(bool on);
@@ -216,6 +220,7 @@ class HipRenderer {
   int plan_levels(const uint64_t dims[3], uint32_t bits, uint32_t brick, uint32_t apron,
                   uint32_t n_lod, LevelPlan* out) const;
   void release_volume();
+  int brick_levels(const LevelPlan& plan);
   // paged pools: upload the bricks the last raycast asked for; *complete is
   // false when another pass is needed, *overflow when the pool cannot hold
   // the bricks of this LOD for this view

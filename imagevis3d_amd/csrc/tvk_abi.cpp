@@ -366,6 +366,12 @@ int tvk_get_batch_framebuffer(tvk_renderer* r, uint32_t index, void** dptr) {
   return r->impl.BatchFramebuffer(index, dptr);
 }
 
+int tvk_autotune(tvk_renderer* r, const tvk_view* views, uint32_t n, uint32_t flags,
+                 tvk_tuning* out) {
+  TVK_GUARD(r);
+  TVK_TRY(r->impl.Autotune(views, n, flags, out));
+}
+
 int tvk_set_launch_graphs(tvk_renderer* r, int enabled) {
   TVK_GUARD(r);
   return r->impl.SetLaunchGraphs(enabled != 0);

@@ -28,7 +28,7 @@ from typing import Sequence
 import numpy as np
 
 from . import _lib
-from ._lib import FrameStats, MemStats, TvkError, View, ViewStats, call
+from ._lib import FrameStats, MemStats, Tuning, TvkError, View, ViewStats, call
 
 
 class _Types:
@@ -383,11 +383,8 @@ class Renderer:
         h, w = a.shape[:2]
         call("tvk_set_2d_trans_slot", self._h, int(slot), a.ctypes.data, w, h, float(max_grad))
 
-    def renderBatch(self, views, readback: bool = True):
-        """Render many views in one launch.  `views` holds objects with
-        view_pos, view_dir, up_dir, fov, ortho, ortho_half_height, width,
-        height and tf_slot (a workloads.Camera plus size/slot, or dicts).
-        Returns (images or None, [ViewStats], kernel_ms)."""
+    @staticmethod
+    def _views(views, readback: bool):
         n = len(views)
         arr = (View * max(n, 1))()
         imgs = []
@@ -408,6 +405,25 @@ class Renderer:
                 x.rgba = img.ctypes.data
             else:
                 x.rgba = None
+        return arr, imgs
+
+    TUNE_BRICK, TUNE_CELL, TUNE_LAUNCH = 1, 2, 4
+
+    def autotune(self, views, flags: int = 7) -> Tuning:
+        """Pick brick size / empty-space cell / launch shape for these views
+        (fastest summed kernel time) and leave them applied."""
+        arr, _ = self._views(views, False)
+        t = Tuning()
+        call("tvk_autotune", self._h, arr, len(views), int(flags), C.byref(t))
+        return t
+
+    def renderBatch(self, views, readback: bool = True):
+        """Render many views in one launch.  `views` holds objects with
+        view_pos, view_dir, up_dir, fov, ortho, ortho_half_height, width,
+        height and tf_slot (a workloads.Camera plus size/slot, or dicts).
+        Returns (images or None, [ViewStats], kernel_ms)."""
+        n = len(views)
+        arr, imgs = self._views(views, readback)
         stats = (ViewStats * max(n, 1))()
         kms = C.c_float(0.0)
         call("tvk_render_batch", self._h, arr, n, stats, C.byref(kms))

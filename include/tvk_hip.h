@@ -340,6 +340,33 @@ int tvk_render_batch(tvk_renderer* r, const tvk_view* views, uint32_t n, tvk_vie
                      float* kernel_ms);
 /* Device pointer of view `index` of the last batch (RGBA8, row 0 at top). */
 int tvk_get_batch_framebuffer(tvk_renderer* r, uint32_t index, void** dptr);
+/* ------------------------------------------------------------ autotuner */
+/* Launch autotuner (no counterpart in the reference).  Times the given
+ * views (tvk_render_batch, images stay on the device) under candidate
+ * settings by coordinate descent and leaves the fastest applied:
+ *   TVK_TUNE_BRICK  brick size in {32, 64, 128, 256} (in-core volumes;
+ *                   the device-resident levels are re-bricked),
+ *   TVK_TUNE_CELL   empty-space cell edge (divisors of the brick size),
+ *   TVK_TUNE_LAUNCH samples per batch, ray tile width, XCD tile group.
+ * Cell and launch settings leave the images bit-identical; a different
+ * brick size changes rounding in the slot-local coordinates (tested:
+ * at most 1/255 per channel). */
+enum { TVK_TUNE_BRICK = 1, TVK_TUNE_CELL = 2, TVK_TUNE_LAUNCH = 4 };
+typedef struct tvk_tuning {
+  uint32_t brick_size;
+  uint32_t cell;
+  uint32_t samples_per_batch;
+  uint32_t tile_width;
+  uint32_t tile_group;
+  uint32_t xcd_split;
+  float kernel_ms;          /* best summed launch time over the views   */
+  float baseline_ms;        /* the same views under the starting settings */
+  uint32_t candidates;      /* settings timed                            */
+  uint32_t reserved;
+} tvk_tuning;
+int tvk_autotune(tvk_renderer* r, const tvk_view* views, uint32_t n, uint32_t flags,
+                 tvk_tuning* out);
+
 /* Replay batched launches from captured HIP graphs (default on): table
  * upload, counter clear, raycast and counter readback are one graph per
  * kernel variant, re-captured when the variant changes.  Off runs the same

@@ -9,7 +9,7 @@ ctypes binding, and a Python mirror of Tuvok's Lua renderer interface.
 """
 from ._lib import TvkError, load  # noqa: F401
 from .renderer import (Context, Interpolant, RenderMode, Renderer, deleteClass,  # noqa: F401
-                       generate_volume, tuvok)
+                       generate_volume, generate_volume_gpu, tuvok)
 
 __all__ = ["TvkError", "load", "Context", "Interpolant", "RenderMode", "Renderer", "deleteClass",
-           "generate_volume", "tuvok"]
+           "generate_volume", "generate_volume_gpu", "tuvok"]

@@ -72,6 +72,27 @@ class Tuning(C.Structure):
     ]
 
 
+class Request(C.Structure):
+    """tvk_request: one render request of the server."""
+    _fields_ = [("volume", C.c_uint32), ("render_mode", C.c_int32), ("view", View)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("launch_jobs", C.c_uint32),
+        ("evaluated_samples", C.c_uint64), ("nominal_samples", C.c_uint64),
+        ("queue_ms", C.c_double), ("latency_ms", C.c_double),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64), ("completed", C.c_uint64), ("launches", C.c_uint64),
+        ("max_jobs_per_launch", C.c_uint64), ("max_lanes_busy", C.c_uint64),
+        ("nominal_samples", C.c_uint64),
+    ]
+
+
 H = C.c_void_p
 # tvk_brick_source_fn
 BRICK_SOURCE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -133,6 +154,12 @@ SIGNATURES = {
     "tvk_get_batch_framebuffer": [H, u32, C.POINTER(C.c_void_p)],
     "tvk_set_launch_graphs": [H, i32],
     "tvk_autotune": [H, C.POINTER(View), u32, u32, C.POINTER(Tuning)],
+    "tvk_server_create": [i32, u64, u32, C.POINTER(C.c_void_p)],
+    "tvk_server_destroy": [C.c_void_p],
+    "tvk_server_add_volume": [C.c_void_p, H, C.POINTER(u32)],
+    "tvk_server_submit": [C.c_void_p, C.POINTER(Request), C.POINTER(u64)],
+    "tvk_server_wait": [C.c_void_p, u64, u32, C.POINTER(RequestResult)],
+    "tvk_server_get_stats": [C.c_void_p, C.POINTER(ServerStats)],
     "tvk_paint": [H],
     "tvk_check_for_redraw": [H],
     "tvk_schedule_complete_redraw": [H],
@@ -145,6 +172,8 @@ SIGNATURES = {
     "tvk_set_max_usable_gpu_mem": [H, u64],
     "tvk_get_mem_stats": [H, C.POINTER(MemStats)],
     "tvk_generate_volume": [i32, u64, u64, u64, u64, u32, P, u64],
+    "tvk_generate_volume_gpu": [i32, i32, u64, u64, u64, u64, u32, P, u64],
+    "tvk_set_generated_volume": [H, i32, u64, u64, u64, u64, u32, u32, u32, u32],
 }
 _RESTYPE = {"tvk_last_error": C.c_char_p, "tvk_version": C.c_char_p}
 

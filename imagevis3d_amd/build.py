@@ -23,9 +23,11 @@ SOURCES = [
     "renderer.cpp",
     "paging.cpp",
     "autotune.cpp",
+    "server.cpp",
     "host_volume.cpp",
     "uvf.cpp",
     "volume_gen.cpp",
+    "volume_gen_gpu.hip",
     "tvk_abi.cpp",
 ]
 

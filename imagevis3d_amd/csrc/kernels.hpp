@@ -59,6 +59,12 @@ hipError_t launch_cell_empty(const uint32_t* cell_range, int64_t n_cells, const 
                              int n_tf, float tf_scale, int mode, float iso_value, uint8_t* out,
                              hipStream_t stream);
 
+// Device-side synthetic volumes (TVK_VOL_FBM_PLANES, _NOISE_TUBES,
+// _FBM_SPHERES): z slab [z0, z0 + nz) of a dims volume into `out`.
+bool is_gpu_volume_kind(int kind);
+hipError_t launch_generate_volume(int kind, uint64_t seed, const int64_t dims[3], int bits,
+                                  int64_t z0, int64_t nz, void* out, hipStream_t stream);
+
 // Kernel variant of a raycast launch: every job of the table shares it.
 struct LaunchInfo {
   int bpv = 1;          // bytes per voxel of every job's volume

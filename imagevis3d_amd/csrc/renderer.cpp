@@ -318,8 +318,20 @@ void HipRenderer::release_volume() {
 
 int HipRenderer::SetVolume(const void* voxels, const uint64_t dims[3], uint32_t bits,
                            uint32_t brick, uint32_t apron, uint32_t n_lod) {
-  if (!initialized_) return state_fail("setVolume before initialize");
   if (!voxels) return arg_fail("null voxel pointer");
+  return set_volume_impl(voxels, -1, 0, dims, bits, brick, apron, n_lod);
+}
+
+int HipRenderer::SetGeneratedVolume(int kind, uint64_t seed, const uint64_t dims[3], uint32_t bits,
+                                    uint32_t brick, uint32_t apron, uint32_t n_lod) {
+  if (!is_gpu_volume_kind(kind)) return arg_fail("not a device-generated volume kind");
+  return set_volume_impl(nullptr, kind, seed, dims, bits, brick, apron, n_lod);
+}
+
+int HipRenderer::set_volume_impl(const void* voxels, int gen_kind, uint64_t seed,
+                                 const uint64_t dims[3], uint32_t bits, uint32_t brick,
+                                 uint32_t apron, uint32_t n_lod) {
+  if (!initialized_) return state_fail("setVolume before initialize");
   LevelPlan plan;
   if (int e = plan_levels(dims, bits, brick, apron, n_lod, &plan)) return e;
   const int bpv = (int)bits / 8;
@@ -341,7 +353,17 @@ int HipRenderer::SetVolume(const void* voxels, const uint64_t dims[3], uint32_t 
   release_volume();
 
   for (int l = 0; l < nl; ++l) TVK_HIP(levels_[l].alloc(ld[l][0] * ld[l][1] * ld[l][2] * bpv));
-  TVK_HIP(hipMemcpyAsync(levels_[0].p, voxels, levels_[0].bytes, hipMemcpyHostToDevice, stream_));
+  if (voxels) {
+    TVK_HIP(hipMemcpyAsync(levels_[0].p, voxels, levels_[0].bytes, hipMemcpyHostToDevice, stream_));
+  } else {
+    const int64_t d[3] = {(int64_t)dims[0], (int64_t)dims[1], (int64_t)dims[2]};
+    const int64_t slab = std::max<int64_t>(1, (int64_t)((1ull << 30) / (dims[0] * dims[1])));
+    for (int64_t z0 = 0; z0 < d[2]; z0 += slab) {
+      uint8_t* dst = (uint8_t*)levels_[0].p + (uint64_t)z0 * dims[0] * dims[1] * bpv;
+      TVK_HIP(launch_generate_volume(gen_kind, seed, d, (int)bits, z0, std::min(slab, d[2] - z0),
+                                     dst, stream_));
+    }
+  }
   for (int l = 1; l < nl; ++l) {
     const int64_t sd[3] = {(int64_t)ld[l - 1][0], (int64_t)ld[l - 1][1], (int64_t)ld[l - 1][2]};
     TVK_HIP(launch_downsample(levels_[l - 1].p, bpv, sd, levels_[l].p, stream_));
@@ -933,7 +955,7 @@ void HipRenderer::build_params(RaycastParams* p, int lod, const CameraState& cam
   p->page_table = pool_.page_table.as<int32_t>();
   p->brick_range = pool_.brick_range.as<float>();
   p->cell_range = cell_range_.as<uint32_t>();
-  p->cell_empty = tfs.cell_empty.as<uint8_t>();
+  p->cell_empty = tfs.cell_empty[mode_index()].as<uint8_t>();
   p->cell = skip_ ? cell_ : brick_;
   p->tf = tfs.tf1d_dev.as<float>();
   p->tf_prefix = tfs.prefix1d_dev.as<int32_t>();
@@ -980,14 +1002,14 @@ LaunchInfo HipRenderer::launch_info(int n_tf_lds) const {
   return li;
 }
 
-int HipRenderer::launch_jobs(LaunchSlot& sl, const RaycastParams* jobs, int n, const LaunchInfo& li,
-                             bool graph) {
+int launch_job_table(LaunchSlot& sl, const RaycastParams* jobs, int n, const LaunchInfo& li,
+                     int tile_group, int xcd_split, hipStream_t stream_, bool graph) {
   // the pinned table may still be read by this slot's previous upload
   if (sl.used) TVK_HIP(hipEventSynchronize(sl.ev[3]));
   JobTable* t = sl.table_h;
   t->h.n_jobs = n;
-  t->h.tile_group = tile_group_;
-  t->h.xcd_split = xcd_split_;
+  t->h.tile_group = tile_group;
+  t->h.xcd_split = xcd_split;
   t->h.n_tf_lds = li.n_tf_lds;
   t->h.tf_lds = li.n_tf_lds > 0 ? jobs[0].tf : nullptr;
   t->h.counters = sl.counters.as<unsigned long long>();
@@ -1050,14 +1072,18 @@ int HipRenderer::launch_jobs(LaunchSlot& sl, const RaycastParams* jobs, int n, c
   return TVK_OK;
 }
 
+int HipRenderer::launch_jobs(LaunchSlot& sl, const RaycastParams* jobs, int n, const LaunchInfo& li,
+                             bool graph) {
+  return launch_job_table(sl, jobs, n, li, tile_group_, xcd_split_, stream_, graph);
+}
+
 float HipRenderer::tf_scale(int n_entries) const {
   return bias_scale_tf_ ? (float)((double)(n_entries - 1) / value_max()) : 1.0f;
 }
 
-int HipRenderer::prepare_cells(int tf_slot) {
+int HipRenderer::prepare_cells(int tf_slot, int mode) {
   TfSet& t = tfs_[tf_slot];
   if (!skip_ || total_cells_ == 0) return TVK_OK;
-  const int mode = render_mode_ == TVK_RM_2DTRANS ? 1 : (render_mode_ == TVK_RM_ISOSURFACE ? 2 : 0);
   const std::vector<int32_t>& prefix = mode == 1 ? t.prefix2d : t.prefix1d;
   const int n = (int)prefix.size() - 1;
   const float scale = tf_scale(n);
@@ -1066,16 +1092,17 @@ int HipRenderer::prepare_cells(int tf_slot) {
   memcpy(&scale_bits, &scale, 4);
   // FNV-1a over what the flags depend on
   uint64_t key = 1469598103934665603ull;
-  for (const uint64_t v : {cells_gen_, (uint64_t)mode, t.version, (uint64_t)iso_bits,
+  for (const uint64_t v : {cells_gen_, (uint64_t)mode, t.version, (uint64_t)(mode == 2 ? iso_bits : 0),
                            (uint64_t)scale_bits, (uint64_t)n})
     key = (key ^ v) * 1099511628211ull;
-  if (key == t.cell_key && t.cell_empty.p) return TVK_OK;
-  if (t.cell_empty.bytes < (size_t)total_cells_) TVK_HIP(t.cell_empty.alloc((size_t)total_cells_));
+  DevBuf& flags = t.cell_empty[mode];
+  if (key == t.cell_key[mode] && flags.p) return TVK_OK;
+  if (flags.bytes < (size_t)total_cells_) TVK_HIP(flags.alloc((size_t)total_cells_));
   const int32_t* prefix_dev = mode == 1 ? t.prefix2d_dev.as<int32_t>() : t.prefix1d_dev.as<int32_t>();
   if (mode != 2 && (n < 1 || !prefix_dev)) return state_fail("TF not uploaded");
   TVK_HIP(launch_cell_empty(cell_range_.as<uint32_t>(), total_cells_, prefix_dev, n, scale, mode,
-                            iso_value_, t.cell_empty.as<uint8_t>(), stream_));
-  t.cell_key = key;
+                            iso_value_, flags.as<uint8_t>(), stream_));
+  t.cell_key[mode] = key;
   return TVK_OK;
 }
 
@@ -1118,7 +1145,7 @@ int HipRenderer::Paint() {
   else lod = std::max(lo, next_lod_);
   if (paged_) lod = std::max(lod, std::min(lod_floor_, hi));
 
-  if (int e = prepare_cells(0)) return e;
+  if (int e = prepare_cells(0, mode_index())) return e;
   RaycastParams p;
   build_params(&p, lod, cam_, width_, height_, 0, framebuffer_.as<uint32_t>());
   if (paged_) {
@@ -1179,26 +1206,9 @@ int HipRenderer::RenderBatch(const tvk_view* views, uint32_t n, tvk_view_stats* 
   std::vector<CameraState> cams(n);
   uint64_t fb_bytes = 0;
   for (uint32_t i = 0; i < n; ++i) {
-    const tvk_view& v = views[i];
-    if (v.width == 0 || v.height == 0 || v.width > 16384 || v.height > 16384)
-      return arg_fail("view " + std::to_string(i) + ": image size out of range");
-    if (int e = check_tf((int)v.tf_slot)) return e;
-    if (!finite3(v.view_pos) || !finite3(v.view_dir) || !finite3(v.up_dir))
-      return arg_fail("view " + std::to_string(i) + ": camera must be finite");
-    const float* d = v.view_dir;
-    if (d[0] == 0.f && d[1] == 0.f && d[2] == 0.f) return arg_fail("view direction must be non-zero");
-    if (v.ortho ? !(v.ortho_half_height > 0.f) : !(v.fov_deg > 0.f && v.fov_deg < 180.f))
-      return arg_fail("view " + std::to_string(i) + ": fov / ortho extent out of range");
-    CameraState& c = cams[i];
-    for (int k = 0; k < 3; ++k) {
-      c.pos[k] = v.view_pos[k];
-      c.dir[k] = v.view_dir[k];
-      c.up[k] = v.up_dir[k];
-    }
-    c.fov = v.fov_deg;
-    c.ortho = v.ortho != 0;
-    c.ortho_hh = v.ortho_half_height;
-    fb_bytes += ((uint64_t)v.width * v.height * 4 + 255) & ~255ull;
+    if (int e = view_camera(views[i], i, &cams[i])) return e;
+    if (int e = check_tf((int)views[i].tf_slot)) return e;
+    fb_bytes += ((uint64_t)views[i].width * views[i].height * 4 + 255) & ~255ull;
   }
   TVK_HIP(hipSetDevice(device_));
   if (int e = upload_tf()) return e;
@@ -1228,7 +1238,7 @@ int HipRenderer::RenderBatch(const tvk_view* views, uint32_t n, tvk_view_stats* 
     // paged volumes: a pass per set of requested bricks, as in a capture
     for (int pass = 0; pass < 4 * kMaxLod + 8; ++pass) {
       for (int k = 0; k < cnt; ++k)
-        if (int e = prepare_cells((int)views[first + k].tf_slot)) return e;
+        if (int e = prepare_cells((int)views[first + k].tf_slot, mode_index())) return e;
       for (int k = 0; k < cnt; ++k) {
         const uint32_t i = first + k;
         build_params(&jobs[k], lod, cams[i], views[i].width, views[i].height, (int)views[i].tf_slot,
@@ -1284,6 +1294,55 @@ int HipRenderer::RenderBatch(const tvk_view* views, uint32_t n, tvk_view_stats* 
 
 int HipRenderer::SetLaunchGraphs(bool on) {
   use_graphs_ = on;
+  return TVK_OK;
+}
+
+int HipRenderer::view_camera(const tvk_view& v, uint32_t i, CameraState* c) const {
+  if (v.width == 0 || v.height == 0 || v.width > 16384 || v.height > 16384)
+    return arg_fail("view " + std::to_string(i) + ": image size out of range");
+  if (!finite3(v.view_pos) || !finite3(v.view_dir) || !finite3(v.up_dir))
+    return arg_fail("view " + std::to_string(i) + ": camera must be finite");
+  const float* d = v.view_dir;
+  if (d[0] == 0.f && d[1] == 0.f && d[2] == 0.f) return arg_fail("view direction must be non-zero");
+  if (v.ortho ? !(v.ortho_half_height > 0.f) : !(v.fov_deg > 0.f && v.fov_deg < 180.f))
+    return arg_fail("view " + std::to_string(i) + ": fov / ortho extent out of range");
+  for (int k = 0; k < 3; ++k) {
+    c->pos[k] = v.view_pos[k];
+    c->dir[k] = v.view_dir[k];
+    c->up[k] = v.up_dir[k];
+  }
+  c->fov = v.fov_deg;
+  c->ortho = v.ortho != 0;
+  c->ortho_hh = v.ortho_half_height;
+  return TVK_OK;
+}
+
+int HipRenderer::ServePrepare(uint32_t tf_slot, int mode) {
+  if (!initialized_ || !has_volume_) return state_fail("serving needs an initialized renderer with a volume");
+  if (paged_) return state_fail("the server renders in-core volumes only");
+  if (mode != TVK_RM_1DTRANS && mode != TVK_RM_2DTRANS) return arg_fail("served render mode must be 1D or 2D TF");
+  const int saved = render_mode_;
+  render_mode_ = mode;
+  int e = check_tf((int)tf_slot);
+  render_mode_ = saved;
+  if (e) return e;
+  TVK_HIP(hipSetDevice(device_));
+  if (int e2 = upload_tf()) return e2;
+  if (int e2 = prepare_cells((int)tf_slot, mode == TVK_RM_2DTRANS ? 1 : 0)) return e2;
+  TVK_HIP(hipStreamSynchronize(stream_));
+  return TVK_OK;
+}
+
+int HipRenderer::ServeJob(const tvk_view& v, int mode, uint32_t* fb, RaycastParams* out,
+                          LaunchInfo* li) {
+  CameraState cam;
+  if (int e = view_camera(v, 0, &cam)) return e;
+  const int saved = render_mode_;
+  render_mode_ = mode;
+  const int lo = std::min<int>((int)lod_min_, n_lod_ - 1);
+  build_params(out, lo, cam, v.width, v.height, (int)v.tf_slot, fb);
+  *li = launch_info(0);
+  render_mode_ = saved;
   return TVK_OK;
 }
 
@@ -1389,8 +1448,10 @@ int HipRenderer::MemStats(tvk_mem_stats* out) {
   uint64_t a = framebuffer_.bytes + batch_fb_.bytes + pool_.slots.bytes + pool_.page_table.bytes + pool_.brick_range.bytes + scratch_.bytes +
                cell_range_.bytes;
   for (auto& l : levels_) a += l.bytes;
-  for (const TfSet& t : tfs_)
+  for (const TfSet& t : tfs_) {
     a += t.tf1d_dev.bytes + t.prefix1d_dev.bytes + t.tf2d_dev.bytes + t.prefix2d_dev.bytes;
+    for (const DevBuf& f : t.cell_empty) a += f.bytes;
+  }
   for (const LaunchSlot* sl : {&paint_slots_[0], &paint_slots_[1], &batch_slot_})
     a += sl->table.bytes + sl->counters.bytes;
   a += requests_dev_.bytes + touch_dev_.bytes;

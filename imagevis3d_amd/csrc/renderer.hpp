@@ -86,8 +86,8 @@ struct TfSet {
   bool dirty2d = false;
   DevBuf tf2d_dev, prefix2d_dev;
   uint64_t version = 0;   // bumped by every change of this slot
-  DevBuf cell_empty;      // u8 per empty-space cell of every level
-  uint64_t cell_key = 0;  // what cell_empty was built for
+  DevBuf cell_empty[3];      // per render mode: u8 per empty-space cell of every level
+  uint64_t cell_key[3] = {};  // what each cell_empty was built for
 };
 
 // A camera (Tuvok's view position / direction / up, FoV or ortho extent).
@@ -117,6 +117,11 @@ struct LaunchSlot {
   void release();
 };
 
+// Fill `sl`'s job table from jobs[0..n) and launch it on `stream` (graph
+// replay when `graph`; one graph per kernel variant, re-captured on change).
+int launch_job_table(LaunchSlot& sl, const RaycastParams* jobs, int n, const LaunchInfo& li,
+                     int tile_group, int xcd_split, hipStream_t stream, bool graph);
+
 // Out-of-core brick source (see tvk_brick_source_fn in tvk_hip.h).
 typedef int (*BrickSourceFn)(void* user, uint32_t lod, uint32_t bx, uint32_t by, uint32_t bz,
                              void* slot, uint64_t slot_bytes);
@@ -134,6 +139,8 @@ class HipRenderer {
   int LoadDataset(const char* path, bool verify);
   int SetVolume(const void* voxels, const uint64_t dims[3], uint32_t bits, uint32_t brick,
                 uint32_t apron, uint32_t n_lod);
+  int SetGeneratedVolume(int kind, uint64_t seed, const uint64_t dims[3], uint32_t bits,
+                         uint32_t brick, uint32_t apron, uint32_t n_lod);
   int SetPagedVolume(const uint64_t dims[3], uint32_t bits, uint32_t brick, uint32_t apron,
                      uint32_t n_lod, BrickSourceFn fn, void* user, const float* brick_ranges,
                      uint64_t pool_bytes);
@@ -190,6 +197,14 @@ class HipRenderer {
   // Many views (camera, size, TF slot) of this volume in one launch.
   int RenderBatch(const tvk_view* views, uint32_t n, tvk_view_stats* stats, float* kernel_ms);
   int BatchFramebuffer(uint32_t index, void** dptr);
+
+  // Serving (RenderServer): ServePrepare checks the TF slot for `mode`,
+  // uploads TFs and builds the slot's empty-cell flags (device work on this
+  // renderer's stream is finished when it returns); ServeJob builds the job
+  // of one view in `mode` (finest LOD limit) and its kernel variant.
+  int ServePrepare(uint32_t tf_slot, int mode);
+  int ServeJob(const tvk_view& v, int mode, uint32_t* fb, RaycastParams* out, LaunchInfo* li);
+  int Device() const { return device_; }
   int CheckForRedraw() const { return frame_complete_ ? 0 : 1; }
   int ScheduleCompleteRedraw();
   int Capture(uint8_t* rgba, uint64_t bytes, bool preserve_transparency);
@@ -205,6 +220,7 @@ class HipRenderer {
  private:
   int check_ready() const;
   int check_tf(int slot) const;
+  int view_camera(const tvk_view& v, uint32_t index, CameraState* c) const;
   int upload_tf();
   void build_params(RaycastParams* p, int lod, const CameraState& cam, uint32_t w, uint32_t h,
                     int tf_slot, uint32_t* framebuffer) const;
@@ -215,11 +231,16 @@ class HipRenderer {
   uint32_t bricks_skipped(int lod, int tf_slot) const;
   float tf_scale(int n_entries) const;
   // (re)build the empty flags of TF slot `tf_slot` for the current mode
-  int prepare_cells(int tf_slot);
+  int prepare_cells(int tf_slot, int mode);
+  int mode_index() const {
+    return render_mode_ == TVK_RM_2DTRANS ? 1 : (render_mode_ == TVK_RM_ISOSURFACE ? 2 : 0);
+  }
   double value_max() const;
   int plan_levels(const uint64_t dims[3], uint32_t bits, uint32_t brick, uint32_t apron,
                   uint32_t n_lod, LevelPlan* out) const;
   void release_volume();
+  int set_volume_impl(const void* voxels, int gen_kind, uint64_t seed, const uint64_t dims[3],
+                      uint32_t bits, uint32_t brick, uint32_t apron, uint32_t n_lod);
   int brick_levels(const LevelPlan& plan);
   // paged pools: upload the bricks the last raycast asked for; *complete is
   // false when another pass is needed, *overflow when the pool cannot hold

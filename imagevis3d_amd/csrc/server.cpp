@@ -1,0 +1,330 @@
+// server.cpp -- RenderServer (see server.hpp).
+#include "server.hpp"
+
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <memory>
+
+#include "errors.hpp"
+#include "kernels.hpp"
+
+namespace tvk {
+
+namespace {
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+int fail(int code, const std::string& msg) {
+  set_last_error(msg);
+  return code;
+}
+
+constexpr uint64_t kAlign = 256;
+
+}  // namespace
+
+// ------------------------------------------------------------------ arena
+void Arena::reset(uint64_t size) {
+  size_ = size & ~(kAlign - 1);
+  free_.clear();
+  if (size_) free_[0] = size_;
+}
+
+bool Arena::alloc(uint64_t bytes, uint64_t* off) {
+  bytes = (bytes + kAlign - 1) & ~(kAlign - 1);
+  for (auto it = free_.begin(); it != free_.end(); ++it) {
+    if (it->second < bytes) continue;
+    *off = it->first;
+    const uint64_t rest = it->second - bytes, at = it->first + bytes;
+    free_.erase(it);
+    if (rest) free_[at] = rest;
+    return true;
+  }
+  return false;
+}
+
+void Arena::free(uint64_t off, uint64_t bytes) {
+  bytes = (bytes + kAlign - 1) & ~(kAlign - 1);
+  auto it = free_.emplace(off, bytes).first;
+  auto next = std::next(it);
+  if (next != free_.end() && it->first + it->second == next->first) {
+    it->second += next->second;
+    free_.erase(next);
+  }
+  if (it != free_.begin()) {
+    auto prev = std::prev(it);
+    if (prev->first + prev->second == it->first) {
+      prev->second += it->second;
+      free_.erase(it);
+    }
+  }
+}
+
+// ----------------------------------------------------------------- server
+RenderServer::~RenderServer() {
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    stop_ = true;
+  }
+  cv_work_.notify_all();
+  if (worker_.joinable()) worker_.join();
+  if (device_ >= 0) {
+    (void)hipSetDevice(device_);
+    for (Lane& L : lanes_) {
+      if (L.stream) (void)hipStreamSynchronize(L.stream);
+      for (auto& kv : L.slots) kv.second->release();
+      if (L.done) (void)hipEventDestroy(L.done);
+      if (L.stream) (void)hipStreamDestroy(L.stream);
+    }
+    dev_arena_mem_.reset();
+    if (host_arena_mem_) (void)hipHostFree(host_arena_mem_);
+  }
+}
+
+int RenderServer::Init(int device, uint64_t arena_bytes, uint32_t lanes) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n)
+    return fail(TVK_ERR_ARG, "device ordinal out of range");
+  if (lanes < 1 || lanes > 16) return fail(TVK_ERR_ARG, "lanes must be in [1, 16]");
+  if (arena_bytes < (1u << 20)) return fail(TVK_ERR_ARG, "arena must hold at least 1 MiB");
+  device_ = device;
+  if (hipSetDevice(device_) != hipSuccess) return fail(TVK_ERR_HIP, "hipSetDevice failed");
+  lanes_.resize(lanes);
+  for (Lane& L : lanes_) {
+    if (hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&L.done, hipEventDisableTiming) != hipSuccess)
+      return fail(TVK_ERR_HIP, "stream / event creation failed");
+  }
+  if (dev_arena_mem_.alloc(arena_bytes) != hipSuccess)
+    return fail(TVK_ERR_OOM, "device output arena allocation failed");
+  if (hipHostMalloc((void**)&host_arena_mem_, arena_bytes, hipHostMallocDefault) != hipSuccess)
+    return fail(TVK_ERR_OOM, "pinned host output arena allocation failed");
+  dev_arena_.reset(arena_bytes);
+  host_arena_.reset(arena_bytes);
+  worker_ = std::thread([this] { loop(); });
+  return TVK_OK;
+}
+
+int RenderServer::AddVolume(HipRenderer* r, uint32_t* id) {
+  if (!r || !id) return fail(TVK_ERR_ARG, "null renderer or id");
+  if (r->Device() != device_) return fail(TVK_ERR_ARG, "renderer is on another device");
+  std::lock_guard<std::mutex> lk(mu_);
+  volumes_.push_back(r);
+  *id = (uint32_t)(volumes_.size() - 1);
+  return TVK_OK;
+}
+
+uint64_t RenderServer::variant_key(const LaunchInfo& li) {
+  return (uint64_t)li.bpv | (uint64_t)li.interp << 4 | (uint64_t)li.mode << 8 |
+         (uint64_t)li.lighting << 12 | (uint64_t)li.batch << 16;
+}
+
+int RenderServer::Submit(const tvk_request& q, uint64_t* ticket) {
+  if (!ticket) return fail(TVK_ERR_ARG, "null ticket");
+  std::unique_lock<std::mutex> lk(mu_);
+  if (stop_) return fail(TVK_ERR_STATE, "server is stopping");
+  if (q.volume >= volumes_.size()) return fail(TVK_ERR_ARG, "unknown volume id");
+  HipRenderer* r = volumes_[q.volume];
+  if (int e = r->ServePrepare(q.view.tf_slot, q.render_mode)) return e;
+  Request req;
+  req.q = q;
+  RaycastParams probe;
+  LaunchInfo li;
+  if (int e = r->ServeJob(q.view, q.render_mode, nullptr, &probe, &li)) return e;
+  req.key = variant_key(li);
+  req.bytes = (uint64_t)q.view.width * q.view.height * 4;
+  if (req.bytes > dev_arena_.size()) return fail(TVK_ERR_OOM, "image larger than the output arena");
+  req.t_submit = now_ms();
+  const uint64_t t = next_ticket_++;
+  reqs_.emplace(t, req);
+  pending_.push_back(t);
+  ++stats_.submitted;
+  *ticket = t;
+  lk.unlock();
+  cv_work_.notify_one();
+  return TVK_OK;
+}
+
+bool RenderServer::retire_locked() {
+  bool any = false;
+  for (Lane& L : lanes_) {
+    if (!L.busy) continue;
+    const hipError_t e = hipEventQuery(L.done);
+    if (e == hipErrorNotReady) continue;
+    const double t = now_ms();
+    for (size_t k = 0; k < L.tickets.size(); ++k) {
+      Request& r = reqs_[L.tickets[k]];
+      r.done = true;
+      r.t_done = t;
+      if (e == hipSuccess) {
+        r.evaluated = L.cur->counters_h[kQueues + 2 * k];
+        r.nominal = L.cur->counters_h[kQueues + 2 * k + 1];
+        stats_.nominal_samples += r.nominal;
+      } else {
+        r.status = TVK_ERR_HIP;
+      }
+      dev_arena_.free(r.dev_off, r.bytes);
+      ++stats_.completed;
+    }
+    L.tickets.clear();
+    L.busy = false;
+    any = true;
+  }
+  if (any) cv_done_.notify_all();
+  return any;
+}
+
+bool RenderServer::launch_locked(Lane& L) {
+  const uint64_t key = reqs_[pending_.front()].key;
+  std::vector<uint64_t> pick;
+  for (auto it = pending_.begin(); it != pending_.end() && (int)pick.size() < kMaxJobs;) {
+    Request& r = reqs_[*it];
+    if (r.key != key) {
+      ++it;
+      continue;
+    }
+    uint64_t doff = 0, hoff = 0;
+    if (!dev_arena_.alloc(r.bytes, &doff)) break;  // admission by output memory, in order
+    if (!host_arena_.alloc(r.bytes, &hoff)) {
+      dev_arena_.free(doff, r.bytes);
+      break;
+    }
+    r.dev_off = doff;
+    r.host_off = hoff;
+    r.host_held = true;
+    pick.push_back(*it);
+    it = pending_.erase(it);
+  }
+  if (pick.empty()) return false;
+  std::vector<RaycastParams> jobs(pick.size());
+  LaunchInfo li;
+  bool ok = true;
+  for (size_t k = 0; k < pick.size() && ok; ++k) {
+    Request& r = reqs_[pick[k]];
+    uint32_t* fb = (uint32_t*)((uint8_t*)dev_arena_mem_.p + r.dev_off);
+    ok = volumes_[r.q.volume]->ServeJob(r.q.view, r.q.render_mode, fb, &jobs[k], &li) == TVK_OK;
+  }
+  // a 1D TF shared by every job of the launch is staged in LDS
+  bool shared = ok && li.mode == 0 && jobs[0].n_tf <= 4096;
+  for (size_t k = 1; k < jobs.size() && shared; ++k)
+    shared = jobs[k].tf == jobs[0].tf && jobs[k].n_tf == jobs[0].n_tf;
+  li.n_tf_lds = shared ? jobs[0].n_tf : 0;
+  const uint64_t slot_key = variant_key(li) ^ ((uint64_t)li.n_tf_lds << 32);
+  auto& slot = L.slots[slot_key];
+  if (ok && !slot) {
+    slot.reset(new LaunchSlot());
+    ok = slot->alloc() == TVK_OK;
+  }
+  if (ok) {
+    L.cur = slot.get();
+    ok = launch_job_table(*slot, jobs.data(), (int)jobs.size(), li, 4, 0, L.stream, true) == TVK_OK;
+  }
+  for (size_t k = 0; k < pick.size() && ok; ++k) {
+    Request& r = reqs_[pick[k]];
+    ok = hipMemcpyAsync(host_arena_mem_ + r.host_off, (uint8_t*)dev_arena_mem_.p + r.dev_off, r.bytes,
+                        hipMemcpyDeviceToHost, L.stream) == hipSuccess;
+  }
+  if (ok) ok = hipEventRecord(L.done, L.stream) == hipSuccess;
+  const double t = now_ms();
+  if (!ok) {
+    for (uint64_t tk : pick) {
+      Request& r = reqs_[tk];
+      r.status = TVK_ERR_HIP;
+      r.done = true;
+      r.t_launch = r.t_done = t;
+      dev_arena_.free(r.dev_off, r.bytes);
+      ++stats_.completed;
+    }
+    cv_done_.notify_all();
+    return true;
+  }
+  for (uint64_t tk : pick) {
+    Request& r = reqs_[tk];
+    r.t_launch = t;
+    r.launch_jobs = (uint32_t)pick.size();
+  }
+  L.tickets = pick;
+  L.busy = true;
+  ++stats_.launches;
+  stats_.max_jobs_per_launch = std::max<uint64_t>(stats_.max_jobs_per_launch, pick.size());
+  uint64_t busy = 0;
+  for (const Lane& x : lanes_) busy += x.busy ? 1 : 0;
+  stats_.max_lanes_busy = std::max(stats_.max_lanes_busy, busy);
+  return true;
+}
+
+void RenderServer::loop() {
+  (void)hipSetDevice(device_);
+  std::unique_lock<std::mutex> lk(mu_);
+  while (true) {
+    bool progress = retire_locked();
+    for (Lane& L : lanes_)
+      if (!L.busy && !pending_.empty() && !stop_) progress |= launch_locked(L);
+    bool busy = false;
+    for (const Lane& L : lanes_) busy |= L.busy;
+    if (stop_ && !busy) break;
+    if (progress) continue;
+    if (busy) {
+      // lanes in flight: poll their events at a short interval
+      cv_work_.wait_for(lk, std::chrono::microseconds(50));
+    } else {
+      cv_work_.wait(lk, [&] { return stop_ || !pending_.empty(); });
+    }
+  }
+  // requests never launched fail with a state error
+  for (uint64_t t : pending_) {
+    Request& r = reqs_[t];
+    r.status = TVK_ERR_STATE;
+    r.done = true;
+  }
+  pending_.clear();
+  cv_done_.notify_all();
+}
+
+int RenderServer::Wait(uint64_t ticket, uint32_t timeout_ms, tvk_request_result* out) {
+  std::unique_lock<std::mutex> lk(mu_);
+  auto it = reqs_.find(ticket);
+  if (it == reqs_.end()) return fail(TVK_ERR_ARG, "unknown or retired ticket");
+  auto ready = [&] { return reqs_[ticket].done; };
+  if (timeout_ms == 0) {
+    cv_done_.wait(lk, ready);
+  } else if (!cv_done_.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) {
+    return fail(TVK_ERR_STATE, "request not finished within the time-out");
+  }
+  Request r = reqs_[ticket];
+  reqs_.erase(ticket);
+  const bool copy = r.status == TVK_OK && r.q.view.rgba;
+  lk.unlock();
+  // the host block stays reserved until the copy into the caller's buffer
+  if (copy) memcpy(r.q.view.rgba, host_arena_mem_ + r.host_off, r.bytes);
+  if (r.host_held) {
+    std::lock_guard<std::mutex> g(mu_);
+    host_arena_.free(r.host_off, r.bytes);
+  }
+  cv_work_.notify_one();  // output memory was freed
+  if (out) {
+    *out = tvk_request_result{};
+    out->status = r.status;
+    out->launch_jobs = r.launch_jobs;
+    out->evaluated_samples = r.evaluated;
+    out->nominal_samples = r.nominal;
+    out->queue_ms = r.t_launch > 0 ? r.t_launch - r.t_submit : 0.0;
+    out->latency_ms = (copy ? now_ms() : r.t_done) - r.t_submit;
+  }
+  if (r.status != TVK_OK) return fail(r.status, "request failed");
+  return TVK_OK;
+}
+
+int RenderServer::Stats(tvk_server_stats* out) {
+  if (!out) return fail(TVK_ERR_ARG, "null stats");
+  std::lock_guard<std::mutex> lk(mu_);
+  *out = stats_;
+  return TVK_OK;
+}
+
+}  // namespace tvk

@@ -1,0 +1,96 @@
+// server.hpp -- RenderServer: continuous batching of render requests over
+// several resident volumes (tvk_server_* in tvk_hip.h).
+//
+// A worker thread owns `lanes` HIP streams, each with its own job table.
+// Requests wait in arrival order; whenever a lane is free the oldest
+// request's kernel variant picks the launch, and every waiting request of
+// that variant whose output fits the arenas joins it (up to kMaxJobs).  A
+// launch renders into a device arena block per request and copies each
+// image to a pinned host arena block on the lane's stream; the lane is
+// retired when its event completes, which frees the device blocks and
+// wakes the waiters.  Lanes run concurrently, so requests that arrive while
+// others are on the GPU start as soon as a lane and memory free up.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/tvk_hip.h"
+#include "renderer.hpp"
+
+namespace tvk {
+
+// First-fit allocator over [0, size) in 256-byte units, with coalescing.
+class Arena {
+ public:
+  void reset(uint64_t size);
+  bool alloc(uint64_t bytes, uint64_t* off);
+  void free(uint64_t off, uint64_t bytes);
+  uint64_t size() const { return size_; }
+
+ private:
+  uint64_t size_ = 0;
+  std::map<uint64_t, uint64_t> free_;  // offset -> length
+};
+
+class RenderServer {
+ public:
+  ~RenderServer();
+  int Init(int device, uint64_t arena_bytes, uint32_t lanes);
+  int AddVolume(HipRenderer* r, uint32_t* id);
+  int Submit(const tvk_request& q, uint64_t* ticket);
+  int Wait(uint64_t ticket, uint32_t timeout_ms, tvk_request_result* out);
+  int Stats(tvk_server_stats* out);
+
+ private:
+  struct Request {
+    tvk_request q;
+    uint64_t key = 0;        // kernel variant
+    double t_submit = 0, t_launch = 0, t_done = 0;
+    uint64_t dev_off = 0, host_off = 0, bytes = 0;
+    uint64_t evaluated = 0, nominal = 0;
+    uint32_t launch_jobs = 0;
+    int status = TVK_OK;
+    bool done = false;
+    bool host_held = false;  // owns a pinned host arena block
+  };
+  struct Lane {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    // one job table (and captured graph) per kernel variant
+    std::map<uint64_t, std::unique_ptr<LaunchSlot>> slots;
+    LaunchSlot* cur = nullptr;  // the slot of the launch in flight
+    bool busy = false;
+    std::vector<uint64_t> tickets;
+  };
+
+  void loop();
+  bool retire_locked();  // true when a lane finished
+  bool launch_locked(Lane& lane);
+  static uint64_t variant_key(const LaunchInfo& li);
+
+  int device_ = -1;
+  std::vector<HipRenderer*> volumes_;
+  std::vector<Lane> lanes_;
+  DevBuf dev_arena_mem_;
+  uint8_t* host_arena_mem_ = nullptr;  // pinned
+  Arena dev_arena_, host_arena_;
+
+  std::mutex mu_;
+  std::condition_variable cv_work_, cv_done_;
+  std::deque<uint64_t> pending_;
+  std::unordered_map<uint64_t, Request> reqs_;
+  uint64_t next_ticket_ = 1;
+  bool stop_ = false;
+  std::thread worker_;
+  tvk_server_stats stats_{};
+};
+
+}  // namespace tvk

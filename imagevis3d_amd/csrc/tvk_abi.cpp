@@ -1,12 +1,15 @@
 // tvk_abi.cpp -- extern "C" entry points of libtvkhip.so (include/tvk_hip.h).
 // Each function forwards to the AbstrRenderer-shaped HipRenderer; C++
 // exceptions never cross the boundary.
+#include <algorithm>
 #include <new>
 #include <string>
 
 #include "../../include/tvk_hip.h"
 #include "errors.hpp"
+#include "kernels.hpp"
 #include "renderer.hpp"
+#include "server.hpp"
 
 namespace tvk {
 namespace {
@@ -20,6 +23,10 @@ using tvk::HipRenderer;
 struct tvk_renderer {
   HipRenderer impl;
   tvk_renderer(int t, bool a, bool b, bool c, bool d) : impl(t, a, b, c, d) {}
+};
+
+struct tvk_server {
+  tvk::RenderServer impl;
 };
 
 #define TVK_GUARD(r)                                  \
@@ -95,6 +102,59 @@ int tvk_get_renderer_target(tvk_renderer* r, int* t) {
 int tvk_load_dataset(tvk_renderer* r, const char* path, int verify) {
   TVK_GUARD(r);
   TVK_TRY(r->impl.LoadDataset(path, verify != 0));
+}
+
+int tvk_set_generated_volume(tvk_renderer* r, int kind, uint64_t seed, uint64_t nx, uint64_t ny,
+                             uint64_t nz, uint32_t bits, uint32_t brick, uint32_t apron,
+                             uint32_t n_lod) {
+  TVK_GUARD(r);
+  const uint64_t d[3] = {nx, ny, nz};
+  TVK_TRY(r->impl.SetGeneratedVolume(kind, seed, d, bits, brick, apron, n_lod));
+}
+
+int tvk_generate_volume_gpu(int device, int kind, uint64_t seed, uint64_t nx, uint64_t ny,
+                            uint64_t nz, uint32_t bits, void* out, uint64_t out_bytes) {
+  if (!out || (bits != 8 && bits != 16) || nx == 0 || ny == 0 || nz == 0 ||
+      !tvk::is_gpu_volume_kind(kind)) {
+    tvk::set_last_error("bad device volume generator arguments");
+    return TVK_ERR_ARG;
+  }
+  const uint64_t bpv = bits / 8, plane = nx * ny * bpv;
+  if (out_bytes < plane * nz) {
+    tvk::set_last_error("output buffer too small");
+    return TVK_ERR_ARG;
+  }
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+    tvk::set_last_error("device ordinal out of range");
+    return TVK_ERR_ARG;
+  }
+  if (hipSetDevice(device) != hipSuccess) {
+    tvk::set_last_error("hipSetDevice failed");
+    return TVK_ERR_HIP;
+  }
+  // z slabs of at most 1 GiB, generated on the device and copied back
+  const uint64_t slab = std::max<uint64_t>(1, (1ull << 30) / plane);
+  void* buf = nullptr;
+  if (hipMalloc(&buf, std::min(slab, nz) * plane) != hipSuccess) {
+    tvk::set_last_error("device allocation for the generator failed");
+    return TVK_ERR_OOM;
+  }
+  const int64_t d[3] = {(int64_t)nx, (int64_t)ny, (int64_t)nz};
+  int rc = TVK_OK;
+  for (uint64_t z0 = 0; z0 < nz && rc == TVK_OK; z0 += slab) {
+    const uint64_t cnt = std::min(slab, nz - z0);
+    hipError_t e = tvk::launch_generate_volume(kind, seed, d, (int)bits, (int64_t)z0, (int64_t)cnt,
+                                               buf, nullptr);
+    if (e == hipSuccess)
+      e = hipMemcpy((uint8_t*)out + z0 * plane, buf, cnt * plane, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      tvk::set_last_error(std::string("device volume generator: ") + hipGetErrorString(e));
+      rc = TVK_ERR_HIP;
+    }
+  }
+  (void)hipFree(buf);
+  return rc;
 }
 
 int tvk_set_volume(tvk_renderer* r, const void* v, uint64_t nx, uint64_t ny, uint64_t nz,
@@ -370,6 +430,63 @@ int tvk_autotune(tvk_renderer* r, const tvk_view* views, uint32_t n, uint32_t fl
                  tvk_tuning* out) {
   TVK_GUARD(r);
   TVK_TRY(r->impl.Autotune(views, n, flags, out));
+}
+
+int tvk_server_create(int device, uint64_t arena_bytes, uint32_t lanes, tvk_server** out) {
+  if (!out) {
+    tvk::set_last_error("null output pointer");
+    return TVK_ERR_ARG;
+  }
+  *out = nullptr;
+  try {
+    tvk_server* s = new tvk_server();
+    if (int e = s->impl.Init(device, arena_bytes, lanes)) {
+      delete s;
+      return e;
+    }
+    *out = s;
+    return TVK_OK;
+  } catch (const std::bad_alloc&) {
+    tvk::set_last_error("host allocation failed");
+    return TVK_ERR_OOM;
+  }
+}
+
+int tvk_server_destroy(tvk_server* s) {
+  delete s;
+  return TVK_OK;
+}
+
+int tvk_server_add_volume(tvk_server* s, tvk_renderer* r, uint32_t* id) {
+  if (!s || !r) {
+    tvk::set_last_error("null server or renderer handle");
+    return TVK_ERR_ARG;
+  }
+  TVK_TRY(s->impl.AddVolume(&r->impl, id));
+}
+
+int tvk_server_submit(tvk_server* s, const tvk_request* q, uint64_t* ticket) {
+  if (!s || !q) {
+    tvk::set_last_error("null server handle or request");
+    return TVK_ERR_ARG;
+  }
+  TVK_TRY(s->impl.Submit(*q, ticket));
+}
+
+int tvk_server_wait(tvk_server* s, uint64_t ticket, uint32_t timeout_ms, tvk_request_result* out) {
+  if (!s) {
+    tvk::set_last_error("null server handle");
+    return TVK_ERR_ARG;
+  }
+  TVK_TRY(s->impl.Wait(ticket, timeout_ms, out));
+}
+
+int tvk_server_get_stats(tvk_server* s, tvk_server_stats* out) {
+  if (!s) {
+    tvk::set_last_error("null server handle");
+    return TVK_ERR_ARG;
+  }
+  return s->impl.Stats(out);
 }
 
 int tvk_set_launch_graphs(tvk_renderer* r, int enabled) {

@@ -153,6 +153,15 @@ class Renderer:
         call("tvk_set_volume", self._h, vol.ctypes.data, nx, ny, nz, bits, brick_size, apron, n_lod)
         self._bits = bits
 
+    def setGeneratedVolume(self, kind: int, seed: int, dims, bits: int, brick_size: int,
+                           apron: int = 2, n_lod: int = 0) -> None:
+        """setVolume with a volume generated straight into device memory."""
+        nx, ny, nz = (int(d) for d in dims)
+        call("tvk_set_generated_volume", self._h, int(kind), int(seed), nx, ny, nz, int(bits),
+             int(brick_size), int(apron), int(n_lod))
+        self._bits = int(bits)
+        self._path = None
+
     def setPagedFlatVolume(self, vol: np.ndarray, brick_size: int, pool_bytes: int,
                            apron: int = 2, n_lod: int = 0) -> None:
         """Out-of-core volume: LOD 0 stays in this host array (kept alive by
@@ -538,4 +547,17 @@ def generate_volume(kind: int, seed: int, dims: Sequence[int], bits: int) -> np.
     return out
 
 
+def generate_volume_gpu(kind: int, seed: int, dims: Sequence[int], bits: int,
+                        device: int = 0, out: np.ndarray | None = None) -> np.ndarray:
+    """Device-generated synthetic volume (kinds VOL_FBM_PLANES, VOL_NOISE_TUBES,
+    VOL_FBM_SPHERES) copied into host memory, (nz, ny, nx)."""
+    nx, ny, nz = (int(d) for d in dims)
+    if out is None:
+        out = np.empty((nz, ny, nx), np.uint8 if bits == 8 else np.uint16)
+    call("tvk_generate_volume_gpu", int(device), int(kind), int(seed), nx, ny, nz, int(bits),
+         out.ctypes.data, out.nbytes)
+    return out
+
+
 VOL_CONST, VOL_RANDOM, VOL_SPHERE, VOL_BLOBS, VOL_PERLIN_SPHERES = 0, 1, 2, 3, 4
+VOL_FBM_PLANES, VOL_NOISE_TUBES, VOL_FBM_SPHERES = 5, 6, 7

@@ -367,6 +367,49 @@ typedef struct tvk_tuning {
 int tvk_autotune(tvk_renderer* r, const tvk_view* views, uint32_t n, uint32_t flags,
                  tvk_tuning* out);
 
+/* --------------------------------------------------------------- serving */
+/* Render server (no counterpart in the reference, where each paint serves
+ * one view of one renderer): a worker thread batches requests as they
+ * arrive into persistent-kernel launches over several volumes, keeps every
+ * volume's bricks resident, admits a request when a launch lane and output
+ * memory (device arena + pinned host arena of `arena_bytes` each) are free,
+ * and overlaps launches on up to `lanes` HIP streams.  Requests that share a
+ * kernel variant (bytes per voxel, interpolant, render mode, lighting) share
+ * a launch.  A request's image is the same whether it runs alone or in a
+ * mix.  Registered renderers must hold in-core volumes and must not be
+ * changed or used by other threads while the server runs. */
+typedef struct tvk_server tvk_server;
+typedef struct tvk_request {
+  uint32_t volume;        /* id from tvk_server_add_volume                */
+  int32_t render_mode;    /* TVK_RM_1DTRANS or TVK_RM_2DTRANS             */
+  tvk_view view;          /* camera, size, TF slot; rgba = host output    */
+} tvk_request;
+typedef struct tvk_request_result {
+  int32_t status;         /* TVK_OK or the error of this request          */
+  uint32_t launch_jobs;   /* jobs in the launch that rendered it          */
+  uint64_t evaluated_samples;
+  uint64_t nominal_samples;
+  double queue_ms;        /* submit -> launch                             */
+  double latency_ms;      /* submit -> image in host memory               */
+} tvk_request_result;
+typedef struct tvk_server_stats {
+  uint64_t submitted;
+  uint64_t completed;
+  uint64_t launches;
+  uint64_t max_jobs_per_launch;
+  uint64_t max_lanes_busy;
+  uint64_t nominal_samples;
+} tvk_server_stats;
+int tvk_server_create(int device, uint64_t arena_bytes, uint32_t lanes, tvk_server** out);
+int tvk_server_destroy(tvk_server* s);
+int tvk_server_add_volume(tvk_server* s, tvk_renderer* r, uint32_t* volume_id);
+int tvk_server_submit(tvk_server* s, const tvk_request* req, uint64_t* ticket);
+/* Wait up to timeout_ms (0 = forever) for a request; on completion copies
+ * its image to req.view.rgba (when not NULL), fills `out` and retires the
+ * ticket.  Returns TVK_ERR_STATE on time-out (the ticket stays valid). */
+int tvk_server_wait(tvk_server* s, uint64_t ticket, uint32_t timeout_ms, tvk_request_result* out);
+int tvk_server_get_stats(tvk_server* s, tvk_server_stats* out);
+
 /* Replay batched launches from captured HIP graphs (default on): table
  * upload, counter clear, raycast and counter readback are one graph per
  * kernel variant, re-captured when the variant changes.  Off runs the same
@@ -391,10 +434,24 @@ enum tvk_volume_kind {
   TVK_VOL_RANDOM = 1,
   TVK_VOL_SPHERE = 2,
   TVK_VOL_BLOBS = 3,          /* 60 Gaussian blobs on a sphere + noise     */
-  TVK_VOL_PERLIN_SPHERES = 4  /* 5-octave Perlin fBm in [0,4095] + spheres */
+  TVK_VOL_PERLIN_SPHERES = 4, /* 5-octave Perlin fBm in [0,4095] + spheres */
+  /* generated on the device (tvk_generate_volume_gpu,
+   * tvk_set_generated_volume): */
+  TVK_VOL_FBM_PLANES = 5,     /* value-noise fBm + 12 layered slabs        */
+  TVK_VOL_NOISE_TUBES = 6,    /* multi-scale noise + 64 thin tubes          */
+  TVK_VOL_FBM_SPHERES = 7     /* fBm + 20 solid spheres                     */
 };
 int tvk_generate_volume(int kind, uint64_t seed, uint64_t nx, uint64_t ny, uint64_t nz,
                         uint32_t bits, void* out, uint64_t out_bytes);
+/* Device-generated kinds (5..7) into host memory, made on `device` in z
+ * slabs and copied back. */
+int tvk_generate_volume_gpu(int device, int kind, uint64_t seed, uint64_t nx, uint64_t ny,
+                            uint64_t nz, uint32_t bits, void* out, uint64_t out_bytes);
+/* setVolume with a device-generated volume: level 0 is generated straight
+ * into HBM (no host copy), then downsampled and bricked as tvk_set_volume. */
+int tvk_set_generated_volume(tvk_renderer* r, int kind, uint64_t seed, uint64_t nx, uint64_t ny,
+                             uint64_t nz, uint32_t bits, uint32_t brick_size, uint32_t apron,
+                             uint32_t n_lod);
 
 #ifdef __cplusplus
 }

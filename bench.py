@@ -65,6 +65,99 @@ def calibrate_copy(nbytes: int = 1 << 30) -> int:
     return 0
 
 
+def run_serving(duration_s: float, rate_hz: float, seed: int = 5) -> dict:
+    """The serving workload: seeded Poisson arrivals at `rate_hz` for
+    `duration_s` over a pool of 7 device-generated volumes (64^3, 256^3 and
+    512^3 in uint8 and uint16, one 2048^3 uint16), each request a random
+    camera, a random 1D or 2D TF (two 1D slots and one 2D slot per volume)
+    and an image of 256^2, 512^2, 1920x1080 or 3840x2160.  Returns latency
+    percentiles (submit -> image in host memory) and aggregate samples/s."""
+    import math
+    import threading
+
+    import numpy as np
+
+    from imagevis3d_amd import transfer, tuvok
+    from imagevis3d_amd.renderer import VOL_FBM_SPHERES, RenderMode
+    from imagevis3d_amd.serving import RenderServer
+
+    pool = [((64, 64, 64), 8, 64), ((64, 64, 64), 16, 64), ((256, 256, 256), 8, 128),
+            ((256, 256, 256), 16, 128), ((512, 512, 512), 8, 128), ((512, 512, 512), 16, 128),
+            ((2048, 2048, 2048), 16, 128)]
+    rs = []
+    t0 = time.perf_counter()
+    for i, (dims, bits, brick) in enumerate(pool):
+        r = tuvok.renderer.new(tuvok.renderer.types.OpenGL_Raycaster, False, False, False, False)
+        assert r.initialize(tuvok.createContext(256, 256, device=0)), r.lastError()
+        r.setGeneratedVolume(VOL_FBM_SPHERES, 100 + i, dims, bits, brick, 2, 1)
+        n = 256 if bits == 8 else 4096
+        r.set1DTransSlot(0, transfer.gaussian_bumps_tf(n, seed=200 + i))
+        r.set1DTransSlot(1, transfer.ramp_tf(n, 0.45 * n, 0.95 * n))
+        _, mg = r.histogram2D(256, 256)
+        r.set2DTransSlot(2, transfer.gaussian_lobes_2d(256, 256, seed=300 + i), mg)
+        r.setSampleRateModifier(1.0)
+        rs.append(r)
+    setup_s = time.perf_counter() - t0
+    srv = RenderServer(arena_bytes=4 << 30, lanes=4)
+    ids = [srv.addVolume(r) for r in rs]
+    rng = np.random.default_rng(seed)
+    n_req = int(duration_s * rate_hz)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate_hz, n_req))
+    sizes = [(256, 256), (512, 512), (1920, 1080), (3840, 2160)]
+    reqs = []
+    for _ in range(n_req):
+        th, ph = rng.uniform(0, 2 * math.pi), rng.uniform(-1.3, 1.3)
+        d = rng.uniform(1.5, 2.4)
+        eye = (d * math.cos(ph) * math.sin(th), d * math.sin(ph), d * math.cos(ph) * math.cos(th))
+        w, h = sizes[rng.integers(4)]
+        two_d = rng.uniform() < 0.5
+        view = dict(view_pos=eye, view_dir=tuple(-x for x in eye), fov=50.0, width=w, height=h,
+                    tf_slot=2 if two_d else int(rng.integers(2)))
+        reqs.append((ids[rng.integers(len(ids))], view,
+                     RenderMode.RM_2DTRANS if two_d else RenderMode.RM_1DTRANS))
+    # warm every kernel variant and graph once (not timed)
+    for v in range(len(ids)):
+        for mode, slot in ((RenderMode.RM_1DTRANS, 0), (RenderMode.RM_2DTRANS, 2)):
+            srv.wait(srv.submit(ids[v], dict(view_pos=(0, 0, 2), view_dir=(0, 0, -1), fov=50.0,
+                                             width=64, height=64, tf_slot=slot), mode))
+    tickets = [None] * n_req
+    results = [None] * n_req
+
+    def waiter():
+        for i in range(n_req):
+            while tickets[i] is None:
+                time.sleep(0.0002)
+            results[i] = srv.wait(tickets[i])[1]
+
+    th_wait = threading.Thread(target=waiter)
+    th_wait.start()
+    start = time.perf_counter()
+    for i, (vid, view, mode) in enumerate(reqs):
+        delay = start + arrivals[i] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        tickets[i] = srv.submit(vid, view, mode, readback=False)
+    th_wait.join()
+    wall = time.perf_counter() - start
+    st = srv.stats()
+    lat = np.array([r.latency_ms for r in results])
+    nominal = sum(r.nominal_samples for r in results)
+    out = {
+        "workload": f"serving-poisson{rate_hz:g}hz-{duration_s:g}s-7vols",
+        "requests": n_req, "offered_rate_hz": rate_hz, "completed_rate_hz": n_req / wall,
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "mean_queue_ms": float(np.mean([r.queue_ms for r in results])),
+        "gsamples_per_s": nominal / wall / 1e9, "launches": int(st.launches),
+        "max_jobs_per_launch": int(st.max_jobs_per_launch), "max_lanes_busy": int(st.max_lanes_busy),
+        "volume_setup_s": round(setup_s, 2),
+        "note": "image in pinned host memory counts as done; images 256^2..3840x2160",
+    }
+    srv.close()
+    for r in rs:
+        r.cleanup()
+    return out
+
+
 def main() -> int:
     if "--calibrate-copy" in sys.argv:
         return calibrate_copy()
@@ -82,6 +175,10 @@ def main() -> int:
     ap.add_argument("--no-skip", action="store_true", help="march whole bricks (no cell skipping)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the given launch settings instead of autotuning them")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
+    ap.add_argument("--serving-seconds", type=float, default=10.0)
+    ap.add_argument("--serving-rate", type=float, default=200.0)
+    ap.add_argument("--serving-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
@@ -90,6 +187,10 @@ def main() -> int:
         return 2
 
     from imagevis3d_amd import tuvok, workloads
+
+    if args.serving_only:
+        print(json.dumps(run_serving(args.serving_seconds, args.serving_rate)))
+        return 0
 
     w = workloads.medium()
     print(f"[bench] generating {w.name} volume", file=sys.stderr, flush=True)
@@ -184,6 +285,13 @@ def main() -> int:
         "cpu_baseline": None,
     }
     r.cleanup()
+    if not args.no_serving:
+        print(f"[bench] serving workload ({args.serving_seconds:g} s at {args.serving_rate:g} req/s)",
+              file=sys.stderr, flush=True)
+        sv = run_serving(args.serving_seconds, args.serving_rate)
+        out["serving"] = sv
+        out["p99_ms"] = sv["p99_ms"]
+        print(f"[bench] serving {sv}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         try:
             threads = min(16, len(os.sched_getaffinity(0)))

@@ -158,6 +158,76 @@ def run_serving(duration_s: float, rate_hz: float, seed: int = 5) -> dict:
     return out
 
 
+def run_batched(batches: int = 3, seed: int = 3) -> dict:
+    """The out-of-core batched workload: a 2048x2048x1024 uint16 volume
+    (device-generated fBm + layered slabs, copied to host memory) streamed
+    into a 64 GB device brick pool (128^3 bricks + 2-voxel apron, 5 LODs);
+    4 seeded 2D TFs (256x256 Gaussian lobes over value x gradient
+    magnitude) with Phong shading; 8 orbit cameras x 4 TFs = 32 views at
+    1920x1080 per persistent-kernel launch; launch shape autotuned."""
+    from imagevis3d_amd import transfer, tuvok, workloads
+    from imagevis3d_amd.renderer import VOL_FBM_PLANES, RenderMode
+
+    dims, bits, brick = (2048, 2048, 1024), 16, 128
+    t0 = time.perf_counter()
+    # exact gradient range from an in-core pass, then the host copy
+    g = tuvok.renderer.new(tuvok.renderer.types.OpenGL_Raycaster, False, False, False, False)
+    assert g.initialize(tuvok.createContext(64, 64, device=0)), g.lastError()
+    g.setGeneratedVolume(VOL_FBM_PLANES, seed, dims, bits, 256, 2, 1)
+    _, max_grad = g.histogram2D(256, 256)
+    vol = g.readLOD(0)
+    g.cleanup()
+    gen_s = time.perf_counter() - t0
+    r = tuvok.renderer.new(tuvok.renderer.types.OpenGL_Raycaster, False, False, False, False)
+    assert r.initialize(tuvok.createContext(64, 64, device=0)), r.lastError()
+    t0 = time.perf_counter()
+    r.setPagedFlatVolume(vol, brick, 64 * 10**9, n_lod=5)
+    plan_s = time.perf_counter() - t0
+    r.setRenderMode(RenderMode.RM_2DTRANS)
+    for k in range(4):
+        r.set2DTransSlot(k, transfer.gaussian_lobes_2d(256, 256, seed=40 + k), max_grad)
+    r.setLightingEnabled(True)
+    r.setSampleRateModifier(1.0)
+    r.setLODLimits((0, 0))
+    ext = tuple(d / max(dims) for d in dims)
+    cams = workloads.orbit_cameras(8, fov=60.0, elevation_deg=25.0, extent=ext)
+    views = [dict(view_pos=c.view_pos, view_dir=c.view_dir, up_dir=c.up_dir, fov=c.fov,
+                  width=1920, height=1080, tf_slot=k) for k in range(4) for c in cams]
+    t0 = time.perf_counter()
+    _, st0, _ = r.renderBatch(views, readback=False)  # pages the view set's bricks in
+    page_s = time.perf_counter() - t0
+    t = r.autotune(views, flags=r.TUNE_LAUNCH)
+    kms, nominal, evaluated = [], 0, 0
+    t0 = time.perf_counter()
+    for _ in range(batches):
+        _, st, k = r.renderBatch(views, readback=False)
+        kms.append(k)
+        nominal += sum(x.nominal_samples for x in st)
+        evaluated += sum(x.evaluated_samples for x in st)
+    wall = time.perf_counter() - t0
+    ms = r.memStats()
+    ksum = sum(kms) / 1e3
+    # gather bytes: value + 6 gradient neighbours, 8 corners x 2 B each
+    achieved = (evaluated * 7 * 16 + batches * len(views) * 1920 * 1080 * 4) / ksum / 1e9
+    out = {
+        "workload": "batched-2048x2048x1024-u16-paged64GB-2dtf-phong-8cams-x-4tfs-1080p",
+        "views_per_launch": len(views), "launches": batches,
+        "gsamples_per_s": nominal / wall / 1e9, "views_per_s": batches * len(views) / wall,
+        "ms_per_launch": wall / batches * 1e3, "kernel_ms_per_launch": ksum / batches * 1e3,
+        "evaluated_gsamples_per_s": evaluated / wall / 1e9,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS},
+        "tuned": {"samples_per_batch": t.samples_per_batch, "tile_width": t.tile_width,
+                  "tile_group": t.tile_group},
+        "pool": {"slots": int(ms.pool_slots), "resident": int(ms.resident_bricks),
+                 "uploaded": int(ms.bricks_uploaded)},
+        "setup_s": {"generate": round(gen_s, 2), "host_pyramid": round(plan_s, 2),
+                    "first_batch_paging": round(page_s, 2)},
+    }
+    r.cleanup()
+    return out
+
+
 def main() -> int:
     if "--calibrate-copy" in sys.argv:
         return calibrate_copy()
@@ -179,6 +249,8 @@ def main() -> int:
     ap.add_argument("--serving-seconds", type=float, default=10.0)
     ap.add_argument("--serving-rate", type=float, default=200.0)
     ap.add_argument("--serving-only", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the out-of-core batched workload")
+    ap.add_argument("--batched-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
@@ -190,6 +262,9 @@ def main() -> int:
 
     if args.serving_only:
         print(json.dumps(run_serving(args.serving_seconds, args.serving_rate)))
+        return 0
+    if args.batched_only:
+        print(json.dumps(run_batched()))
         return 0
 
     w = workloads.medium()
@@ -285,6 +360,10 @@ def main() -> int:
         "cpu_baseline": None,
     }
     r.cleanup()
+    if not args.no_batched:
+        print("[bench] out-of-core batched workload", file=sys.stderr, flush=True)
+        out["batched"] = run_batched()
+        print(f"[bench] batched {out['batched']}", file=sys.stderr, flush=True)
     if not args.no_serving:
         print(f"[bench] serving workload ({args.serving_seconds:g} s at {args.serving_rate:g} req/s)",
               file=sys.stderr, flush=True)

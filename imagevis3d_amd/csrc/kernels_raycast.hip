@@ -115,6 +115,87 @@ __device__ inline float fetch(const T* __restrict__ base, int S, int S2, float g
   return ld(base + (iz * S2 + iy * S + ix));
 }
 
+// Four x-neighbours p[0..3] with one load (8 bytes for 16-bit voxels, 4
+// for 8-bit ones).
+__device__ inline void ld4(const uint16_t* p, float o[4]) {
+  uint64_t v;
+  __builtin_memcpy(&v, p, 8);
+  o[0] = (float)(uint32_t)(v & 0xffffu);
+  o[1] = (float)(uint32_t)((v >> 16) & 0xffffu);
+  o[2] = (float)(uint32_t)((v >> 32) & 0xffffu);
+  o[3] = (float)(uint32_t)(v >> 48);
+}
+__device__ inline void ld4(const uint8_t* p, float o[4]) {
+  uint32_t v;
+  __builtin_memcpy(&v, p, 4);
+  o[0] = (float)(v & 0xffu);
+  o[1] = (float)((v >> 8) & 0xffu);
+  o[2] = (float)((v >> 16) & 0xffu);
+  o[3] = (float)(v >> 24);
+}
+
+__device__ inline float trilerp(const float c[2][2][2], float wx, float wy, float wz) {
+  const float c00 = fmaf(c[0][0][1] - c[0][0][0], wx, c[0][0][0]);
+  const float c10 = fmaf(c[0][1][1] - c[0][1][0], wx, c[0][1][0]);
+  const float c01 = fmaf(c[1][0][1] - c[1][0][0], wx, c[1][0][0]);
+  const float c11 = fmaf(c[1][1][1] - c[1][1][0], wx, c[1][1][0]);
+  const float c0 = fmaf(c10 - c00, wy, c00);
+  const float c1 = fmaf(c11 - c01, wy, c01);
+  return fmaf(c1 - c0, wz, c0);
+}
+
+// Trilinear value at slot-local texel coordinate (gx, gy, gz) and the
+// central differences of the trilinear interpolant at +-1 voxel along each
+// axis (unscaled).  The seven interpolations share their weights, so each
+// difference is the interpolation of voxel differences over the 2x2x2
+// cell; the 32 voxels they read (the cell and one voxel beyond it along
+// each axis) come in 12 loads.  The cell index is clamped so that every
+// read stays in the slot (apron >= 2).
+template <typename T>
+__device__ inline void fetch_value_grad(const T* __restrict__ base, int S, int S2, float gx,
+                                        float gy, float gz, float& val, float g[3]) {
+  const int ix = clampi((int)floorf(gx), 1, S - 3);
+  const int iy = clampi((int)floorf(gy), 1, S - 3);
+  const int iz = clampi((int)floorf(gz), 1, S - 3);
+  const float wx = clampf(gx - (float)ix, 0.f, 1.f);
+  const float wy = clampf(gy - (float)iy, 0.f, 1.f);
+  const float wz = clampf(gz - (float)iz, 0.f, 1.f);
+  const T* p = base + (iz * S2 + iy * S + ix);
+  float row[2][2][4];  // [z][y][x - 1 .. x + 2]
+  float ya[2][2][2];   // [z][y - 1, y + 2][x, x + 1]
+  float za[2][2][2];   // [z - 1, z + 2][y][x, x + 1]
+#pragma unroll
+  for (int z = 0; z < 2; ++z)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) ld4(p + z * S2 + y * S - 1, row[z][y]);
+#pragma unroll
+  for (int z = 0; z < 2; ++z) {
+    ld2(p + z * S2 - S, ya[z][0][0], ya[z][0][1]);
+    ld2(p + z * S2 + 2 * S, ya[z][1][0], ya[z][1][1]);
+  }
+#pragma unroll
+  for (int y = 0; y < 2; ++y) {
+    ld2(p - S2 + y * S, za[0][y][0], za[0][y][1]);
+    ld2(p + 2 * S2 + y * S, za[1][y][0], za[1][y][1]);
+  }
+  float c[2][2][2], dx[2][2][2], dy[2][2][2], dz[2][2][2];
+#pragma unroll
+  for (int z = 0; z < 2; ++z)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        c[z][y][x] = row[z][y][x + 1];
+        dx[z][y][x] = row[z][y][x + 2] - row[z][y][x];
+        dy[z][y][x] = (y == 0 ? row[z][1][x + 1] : ya[z][1][x]) - (y == 0 ? ya[z][0][x] : row[z][0][x + 1]);
+        dz[z][y][x] = (z == 0 ? row[1][y][x + 1] : za[1][y][x]) - (z == 0 ? za[0][y][x] : row[0][y][x + 1]);
+      }
+  val = trilerp(c, wx, wy, wz);
+  g[0] = trilerp(dx, wx, wy, wz);
+  g[1] = trilerp(dy, wx, wy, wz);
+  g[2] = trilerp(dz, wx, wy, wz);
+}
+
 // Render modes of the kernel template: 1D TF without a gradient, any mode
 // that needs the gradient (1D TF with lighting, 2D TF), isosurface.
 constexpr int kMode1D = 0, kModeGrad = 1, kModeIso = 2;
@@ -322,7 +403,8 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
       bool live = true;
       if (MODE == kModeGrad) {
         // 1D TF with lighting, or 2D TF (value x |gradient|) with or
-        // without lighting: 7 trilinear fetches per sample.
+        // without lighting: value and central-difference gradient per
+        // sample (fetch_value_grad).
         for (; k < kend && live; k += BATCH) {
           float v[BATCH], len[BATCH], g[BATCH][3];
 #pragma unroll
@@ -332,16 +414,21 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
             const float t = fmaf((float)kk, step, t0) + 0.5f * len[j];
             const float gx = fmaf(t, od[0], g0[0]), gy = fmaf(t, od[1], g0[1]),
                         gz = fmaf(t, od[2], g0[2]);
-            v[j] = fetch<T, LINEAR>(base, S, S2, gx, gy, gz);
-            // trilinear coordinates of the same point for the gradient
-            const float lx = LINEAR ? gx : gx - 0.5f, ly = LINEAR ? gy : gy - 0.5f,
-                        lz = LINEAR ? gz : gz - 0.5f;
-            g[j][0] = (fetch<T, true>(base, S, S2, lx + 1.f, ly, lz) -
-                       fetch<T, true>(base, S, S2, lx - 1.f, ly, lz)) * P.grad_scale[0];
-            g[j][1] = (fetch<T, true>(base, S, S2, lx, ly + 1.f, lz) -
-                       fetch<T, true>(base, S, S2, lx, ly - 1.f, lz)) * P.grad_scale[1];
-            g[j][2] = (fetch<T, true>(base, S, S2, lx, ly, lz + 1.f) -
-                       fetch<T, true>(base, S, S2, lx, ly, lz - 1.f)) * P.grad_scale[2];
+            float d[3];
+            if (LINEAR) {
+              fetch_value_grad<T>(base, S, S2, gx, gy, gz, v[j], d);
+            } else {
+              // nearest value; the gradient of the trilinear interpolant at
+              // the same point (texel coordinates shifted by half a voxel)
+              v[j] = fetch<T, false>(base, S, S2, gx, gy, gz);
+              const float lx = gx - 0.5f, ly = gy - 0.5f, lz = gz - 0.5f;
+              d[0] = fetch<T, true>(base, S, S2, lx + 1.f, ly, lz) - fetch<T, true>(base, S, S2, lx - 1.f, ly, lz);
+              d[1] = fetch<T, true>(base, S, S2, lx, ly + 1.f, lz) - fetch<T, true>(base, S, S2, lx, ly - 1.f, lz);
+              d[2] = fetch<T, true>(base, S, S2, lx, ly, lz + 1.f) - fetch<T, true>(base, S, S2, lx, ly, lz - 1.f);
+            }
+            g[j][0] = d[0] * P.grad_scale[0];
+            g[j][1] = d[1] * P.grad_scale[1];
+            g[j][2] = d[2] * P.grad_scale[2];
           }
 #pragma unroll
           for (int j = 0; j < BATCH; ++j) {

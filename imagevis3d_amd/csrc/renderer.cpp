@@ -850,6 +850,10 @@ int HipRenderer::check_tf(int slot) const {
     return state_fail("render before a 1D TF is set (slot " + std::to_string(slot) + ")");
   if (render_mode_ == TVK_RM_2DTRANS && (!have || tfs_[slot].tf2d.empty()))
     return state_fail("render before a 2D TF is set (slot " + std::to_string(slot) + ")");
+  // the value + gradient fetch reads one voxel beyond the trilinear cell
+  if ((render_mode_ == TVK_RM_2DTRANS || (lighting_ && render_mode_ == TVK_RM_1DTRANS)) &&
+      has_volume_ && apron_ < 2)
+    return state_fail("gradient-based rendering (2D TF, lighting) needs a brick apron >= 2");
   return TVK_OK;
 }
 

@@ -153,6 +153,9 @@ SIGNATURES = {
     "tvk_render_batch": [H, C.POINTER(View), u32, C.POINTER(ViewStats), C.POINTER(f32)],
     "tvk_get_batch_framebuffer": [H, u32, C.POINTER(C.c_void_p)],
     "tvk_set_launch_graphs": [H, i32],
+    "tvk_set_region_lods": [H, P, u64],
+    "tvk_set_region_output": [H, i32],
+    "tvk_read_region_output": [H, P, u64],
     "tvk_autotune": [H, C.POINTER(View), u32, u32, C.POINTER(Tuning)],
     "tvk_server_create": [i32, u64, u32, C.POINTER(C.c_void_p)],
     "tvk_server_destroy": [C.c_void_p],

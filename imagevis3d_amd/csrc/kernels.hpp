@@ -73,6 +73,7 @@ struct LaunchInfo {
   int lighting = 0;     // 1D TF with lighting
   int batch = 8;        // samples per gather batch (1D TF without lighting)
   int n_tf_lds = 0;     // > 0: the table's shared 1D TF goes to LDS
+  int regions = 0;      // 1D TF with per-region LOD (jobs carry region codes)
   int compute_units = 0;
 };
 

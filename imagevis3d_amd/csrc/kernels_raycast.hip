@@ -197,8 +197,9 @@ __device__ inline void fetch_value_grad(const T* __restrict__ base, int S, int S
 }
 
 // Render modes of the kernel template: 1D TF without a gradient, any mode
-// that needs the gradient (1D TF with lighting, 2D TF), isosurface.
-constexpr int kMode1D = 0, kModeGrad = 1, kModeIso = 2;
+// that needs the gradient (1D TF with lighting, 2D TF), isosurface, 1D TF
+// with a per-region LOD.
+constexpr int kMode1D = 0, kModeGrad = 1, kModeIso = 2, kModeRegions = 3;
 
 // Sample at level texel coordinate q through the page table (clamp to
 // edge): used off the march's current brick (isosurface refinement and
@@ -272,6 +273,110 @@ __device__ inline void shade(const RaycastParams& P, const float g[3], const flo
 
 // Render the 64 rays of ray tile `tile` (lane = ray within the 8x8 tile).
 // `tf4` points at the 1D TF (LDS or global).
+// Per-region LOD march (1D TF): the ray walks the LOD-0 brick grid; region
+// r is rendered from level l = region_code[r] & 15 with step 2^l / rate,
+// its own segments restarting at the region entry (the last one partial),
+// or skipped when region_code[r] & 0x80 marks the level-l brick as
+// transparent under the TF.  Returns the region that added the most
+// opacity (-1 if none) for error attribution.
+template <typename T, bool LINEAR, int BATCH>
+__device__ inline int march_regions(const RaycastParams& P, const float4* __restrict__ tf4,
+                                    const float o[3], const float d[3], const float inv[3], float t0,
+                                    float t1, float& C0, float& C1, float& C2, float& A,
+                                    uint32_t& n_eval) {
+  const LodInfo& L0 = P.lods[0];
+  const float B = (float)P.brick;
+  const float tfmax = (float)(P.n_tf - 1);
+  const float base_step = P.step;
+  int dominant = -1;
+  float best = 0.f;
+  float t = t0;
+  bool live = true;
+  while (t < t1 && live) {
+    const float tm = t + 0.5f * fminf(base_step, t1 - t);
+    int c[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) c[i] = clampi((int)floorf(fmaf(tm, d[i], o[i]) / B), 0, L0.grid[i] - 1);
+    float tb = t1;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      if (d[i] != 0.f) {
+        const float face = (float)(d[i] > 0.f ? c[i] + 1 : c[i]) * B;
+        tb = fminf(tb, (face - o[i]) * inv[i]);
+      }
+    tb = fminf(fmaxf(tb, t + 1e-3f), t1);  // always advance
+    const int region = (c[2] * L0.grid[1] + c[1]) * L0.grid[0] + c[0];
+    const uint32_t code = P.region_code[region];
+    if (!(code & 0x80u)) {
+      const int l = (int)(code & 15u);
+      const LodInfo& lv = P.lods[l];
+      // the level-l brick holding the region (from its centre)
+      int b[3];
+      float oo[3], od[3], g0[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const float qc = ((float)c[i] + 0.5f) * B * lv.to_level[i];
+        b[i] = clampi((int)floorf(qc / B), 0, lv.grid[i] - 1);
+        oo[i] = o[i] * lv.to_level[i];
+        od[i] = d[i] * lv.to_level[i];
+        g0[i] = oo[i] - (float)(b[i] * P.brick - P.apron) - (LINEAR ? 0.5f : 0.f);
+      }
+      const int gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
+      const int slot = P.page_table[gid];
+      if (slot < 0) {
+        if (P.requests) P.requests[gid] = 1u;
+      } else {
+        if (P.touch) P.touch[gid] = P.frame;
+        const T* __restrict__ base = (const T*)P.pool + (int64_t)slot * P.slot_elems;
+        const int S = P.slot_edge, S2 = S * S;
+        const float step = ldexpf(base_step, l);
+        const float span = tb - t;
+        const int m = max((int)ceilf(span / step), 1);
+        const float last = fmaxf(span - (float)(m - 1) * step, 0.f);
+        const float A_in = A;
+        for (int k = 0; k < m && live; k += BATCH) {
+          float v[BATCH];
+#pragma unroll
+          for (int j = 0; j < BATCH; ++j) {
+            const int kk = min(k + j, m - 1);
+            const float len = kk == m - 1 ? last : step;
+            const float ts = fmaf((float)kk, step, t) + 0.5f * len;
+            v[j] = fetch<T, LINEAR>(base, S, S2, fmaf(ts, od[0], g0[0]), fmaf(ts, od[1], g0[1]),
+                                    fmaf(ts, od[2], g0[2]));
+          }
+#pragma unroll
+          for (int j = 0; j < BATCH; ++j) {
+            if (k + j < m && live) {
+              ++n_eval;
+              const float x = clampf(v[j] * P.tf_scale, 0.f, tfmax);
+              const int ti = (int)x;
+              const float fr = x - (float)ti;
+              const float4 e0 = tf4[ti], e1 = tf4[min(ti + 1, P.n_tf - 1)];
+              const float a = fmaf(e1.w - e0.w, fr, e0.w);
+              if (a > 0.f) {
+                const float len = (k + j == m - 1) ? last : step;
+                const float ac = a >= 1.f ? 1.f : 1.f - exp2f(len * log2f(1.f - a));
+                const float w = (1.f - A) * ac;
+                C0 = fmaf(w, fmaf(e1.x - e0.x, fr, e0.x), C0);
+                C1 = fmaf(w, fmaf(e1.y - e0.y, fr, e0.y), C1);
+                C2 = fmaf(w, fmaf(e1.z - e0.z, fr, e0.z), C2);
+                A += w;
+                live = A < P.term_alpha;
+              }
+            }
+          }
+        }
+        if (A - A_in > best) {
+          best = A - A_in;
+          dominant = region;
+        }
+      }
+    }
+    t = tb;
+  }
+  return dominant;
+}
+
 template <typename T, bool LINEAR, int BATCH, int MODE>
 __device__ inline void render_tile(const RaycastParams& P, const float4* __restrict__ tf4, int tile,
                                    int lane, uint32_t& n_eval, uint32_t& n_nom) {
@@ -322,7 +427,14 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
   t0 = fmaxf(t0, 0.f);
   const float L = t1 - t0;
   float C0 = 0.f, C1 = 0.f, C2 = 0.f, A = 0.f;
-  if (hit && L > 0.f) {
+  if (MODE == kModeRegions) {
+    int dominant = -1;
+    if (hit && L > 0.f) {
+      n_nom += (uint32_t)floorf(L * P.nominal_rate);
+      dominant = march_regions<T, LINEAR, BATCH>(P, tf4, o, d, inv, t0, t1, C0, C1, C2, A, n_eval);
+    }
+    if (P.region_out) P.region_out[(int64_t)py * P.width + px] = dominant;
+  } else if (hit && L > 0.f) {
     n_nom += (uint32_t)floorf(L * P.nominal_rate);
     const float step = P.step;
     const int n = (int)ceilf(L / step);
@@ -685,6 +797,9 @@ hipError_t launch_one(const LaunchInfo& li, const JobTable* tab, hipStream_t str
 
 template <typename T, bool LINEAR>
 hipError_t launch_typed(const LaunchInfo& li, const JobTable* tab, hipStream_t stream) {
+  if (li.regions)
+    return li.n_tf_lds > 0 ? launch_one<T, LINEAR, 4, true, kModeRegions>(li, tab, stream)
+                           : launch_one<T, LINEAR, 4, false, kModeRegions>(li, tab, stream);
   if (li.mode == 2) return launch_one<T, LINEAR, 4, false, kModeIso>(li, tab, stream);
   if (li.mode == 1) return launch_one<T, LINEAR, 2, false, kModeGrad>(li, tab, stream);
   const bool lds = li.n_tf_lds > 0;

@@ -128,7 +128,9 @@ int HipRenderer::SetPagedVolume(const uint64_t dims[3], uint32_t bits, uint32_t 
   cell_ = plan.cell;
   total_cells_ = plan.total_cells;
   ++cells_gen_;
+  region_lod_h_.clear();
   for (int l = 0; l < plan.n_lod; ++l) lods_[l] = plan.lods[l];
+  if (int e = upload_lods()) return e;
   total_bricks_ = total;
   lod_min_ = 0;
   lod_max_ = (uint32_t)(plan.n_lod - 1);
@@ -204,13 +206,22 @@ int HipRenderer::upload_brick(int gid, int lod, int bx, int by, int bz, int slot
 }
 
 int HipRenderer::service_requests(int lod, bool* complete, bool* overflow) {
+  return service_requests(lod, lod, complete, overflow);
+}
+
+int HipRenderer::service_requests(int lod_lo, int lod_hi, bool* complete, bool* overflow) {
   *complete = true;
   *overflow = false;
-  const LodInfo& L = lods_[lod];
-  const int nb = L.grid[0] * L.grid[1] * L.grid[2];
-  std::vector<int> want;
-  for (int b = 0; b < nb; ++b)
-    if (requests_h_[L.brick_base + b]) want.push_back(L.brick_base + b);
+  std::vector<int> want, want_lod;
+  for (int l = lod_lo; l <= lod_hi; ++l) {
+    const LodInfo& L = lods_[l];
+    const int nb = L.grid[0] * L.grid[1] * L.grid[2];
+    for (int b = 0; b < nb; ++b)
+      if (requests_h_[L.brick_base + b]) {
+        want.push_back(L.brick_base + b);
+        want_lod.push_back(l);
+      }
+  }
   if (want.empty()) return TVK_OK;
   *complete = false;
   // which resident bricks did this frame sample?
@@ -232,7 +243,8 @@ int HipRenderer::service_requests(int lod, bool* complete, bool* overflow) {
   if (want.size() > cand.size()) *overflow = true;
   const size_t n_up = std::min(want.size(), cand.size());
   for (size_t i = 0; i < n_up; ++i) {
-    const int gid = want[i], s = cand[i];
+    const int gid = want[i], s = cand[i], lod = want_lod[i];
+    const LodInfo& L = lods_[lod];
     const int local = gid - L.brick_base;
     const int bx = local % L.grid[0], by = (local / L.grid[0]) % L.grid[1],
               bz = local / (L.grid[0] * L.grid[1]);

@@ -442,6 +442,95 @@ int HipRenderer::brick_levels(const LevelPlan& plan) {
   ++cells_gen_;
   for (int l = 0; l < nl; ++l) lods_[l] = lods[l];
   total_bricks_ = total;
+  region_lod_h_.clear();
+  return upload_lods();
+}
+
+int HipRenderer::upload_lods() {
+  if (!lods_dev_.p) TVK_HIP(lods_dev_.alloc(sizeof(LodInfo) * kMaxLod));
+  TVK_HIP(hipMemcpyAsync(lods_dev_.p, lods_, sizeof(LodInfo) * kMaxLod, hipMemcpyHostToDevice, stream_));
+  TVK_HIP(hipStreamSynchronize(stream_));
+  return TVK_OK;
+}
+
+int HipRenderer::SetRegionLods(const uint8_t* levels, uint64_t n) {
+  if (!has_volume_) return state_fail("setRegionLODs before a volume is set");
+  if (n == 0 || !levels) {
+    region_lod_h_.clear();
+    ++region_version_;
+    invalidate();
+    return TVK_OK;
+  }
+  const LodInfo& L0 = lods_[0];
+  const uint64_t regions = (uint64_t)L0.grid[0] * L0.grid[1] * L0.grid[2];
+  if (n != regions) return arg_fail("region LOD table needs one entry per LOD-0 brick (" + std::to_string(regions) + ")");
+  for (uint64_t i = 0; i < n; ++i)
+    if (levels[i] >= (uint8_t)n_lod_) return arg_fail("region LOD beyond the coarsest level");
+  region_lod_h_.assign(levels, levels + n);
+  ++region_version_;
+  invalidate();
+  return TVK_OK;
+}
+
+int HipRenderer::SetRegionOutput(bool on) {
+  region_out_on_ = on;
+  return TVK_OK;
+}
+
+int HipRenderer::ReadRegionOutput(int32_t* out, uint64_t n) {
+  if (!region_out_on_ || !region_out_dev_.p) return state_fail("region output is off (or nothing painted)");
+  if (!out || n < (uint64_t)width_ * height_) return arg_fail("region output buffer too small");
+  TVK_HIP(hipSetDevice(device_));
+  TVK_HIP(hipMemcpyAsync(out, region_out_dev_.p, (size_t)width_ * height_ * 4, hipMemcpyDeviceToHost,
+                         stream_));
+  TVK_HIP(hipStreamSynchronize(stream_));
+  return TVK_OK;
+}
+
+// Region codes of TF slot `tf_slot`: the chosen level of every LOD-0 region,
+// with 0x80 when the level's brick holding the region centre (the kernel's
+// rule) is transparent under the 1D TF over its stored range.
+int HipRenderer::prepare_regions(int tf_slot) {
+  TfSet& t = tfs_[tf_slot];
+  const int n = (int)t.prefix1d.size() - 1;
+  const float scale = tf_scale(n);
+  uint32_t scale_bits = 0;
+  memcpy(&scale_bits, &scale, 4);
+  uint64_t key = 1469598103934665603ull;
+  for (const uint64_t v : {region_version_, t.version, cells_gen_, (uint64_t)scale_bits,
+                           (uint64_t)skip_})
+    key = (key ^ v) * 1099511628211ull;
+  if (key == t.region_key && t.region_code.p) return TVK_OK;
+  const LodInfo& L0 = lods_[0];
+  const int g[3] = {L0.grid[0], L0.grid[1], L0.grid[2]};
+  std::vector<uint8_t> code(region_lod_h_.size());
+  const float B = (float)brick_;
+  for (int cz = 0; cz < g[2]; ++cz)
+    for (int cy = 0; cy < g[1]; ++cy)
+      for (int cx = 0; cx < g[0]; ++cx) {
+        const size_t r = ((size_t)cz * g[1] + cy) * g[0] + cx;
+        const int l = region_lod_h_[r];
+        const LodInfo& lv = lods_[l];
+        const int c[3] = {cx, cy, cz};
+        int b[3];
+        for (int i = 0; i < 3; ++i) {
+          const float qc = ((float)c[i] + 0.5f) * B * lv.to_level[i];
+          b[i] = std::min(std::max((int)floorf(qc / B), 0), lv.grid[i] - 1);
+        }
+        const int gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
+        bool empty = false;
+        if (skip_ && n > 0) {
+          const float mn = pool_.brick_range_h[2 * gid], mx = pool_.brick_range_h[2 * gid + 1];
+          const int i0 = (int)std::min(std::max(mn * scale, 0.f), (float)(n - 1));
+          const int i1 = std::min((int)std::min(std::max(mx * scale, 0.f), (float)(n - 1)) + 1, n - 1);
+          empty = t.prefix1d[i1 + 1] - t.prefix1d[i0] == 0;
+        }
+        code[r] = (uint8_t)(l | (empty ? 0x80 : 0));
+      }
+  if (t.region_code.bytes < code.size()) TVK_HIP(t.region_code.alloc(code.size()));
+  TVK_HIP(hipMemcpyAsync(t.region_code.p, code.data(), code.size(), hipMemcpyHostToDevice, stream_));
+  TVK_HIP(hipStreamSynchronize(stream_));
+  t.region_key = key;
   return TVK_OK;
 }
 
@@ -992,6 +1081,8 @@ void HipRenderer::build_params(RaycastParams* p, int lod, const CameraState& cam
   }
   p->iso_value = iso_value_;
   for (int i = 0; i < 3; ++i) p->iso_color[i] = iso_color_[i];
+  p->lods = lods_dev_.as<LodInfo>();
+  if (regions_active() && p->mode == 0 && !p->lighting) p->region_code = tfs.region_code.as<uint8_t>();
 }
 
 LaunchInfo HipRenderer::launch_info(int n_tf_lds) const {
@@ -1002,6 +1093,7 @@ LaunchInfo HipRenderer::launch_info(int n_tf_lds) const {
   li.lighting = (lighting_ && li.mode == 0) ? 1 : 0;
   li.batch = batch_;
   li.n_tf_lds = (li.mode == 0 && n_tf_lds <= 4096) ? n_tf_lds : 0;
+  li.regions = (regions_active() && li.mode == 0 && !li.lighting) ? 1 : 0;
   li.compute_units = compute_units_;
   return li;
 }
@@ -1044,7 +1136,8 @@ int launch_job_table(LaunchSlot& sl, const RaycastParams* jobs, int n, const Lau
   // capture serves any job count; it is re-captured when the kernel
   // variant or its LDS size changes.
   const uint64_t key = 1ull | (uint64_t)li.bpv << 1 | (uint64_t)li.interp << 3 | (uint64_t)li.mode << 4 |
-                       (uint64_t)li.lighting << 6 | (uint64_t)li.batch << 7 | (uint64_t)li.n_tf_lds << 12;
+                       (uint64_t)li.lighting << 6 | (uint64_t)li.batch << 7 | (uint64_t)li.regions << 11 |
+                       (uint64_t)li.n_tf_lds << 12;
   if (!sl.exec || sl.graph_key != key) {
     if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
     if (sl.graph) (void)hipGraphDestroy(sl.graph);
@@ -1150,8 +1243,18 @@ int HipRenderer::Paint() {
   if (paged_) lod = std::max(lod, std::min(lod_floor_, hi));
 
   if (int e = prepare_cells(0, mode_index())) return e;
+  if (regions_active()) {
+    if (render_mode_ != TVK_RM_1DTRANS || lighting_)
+      return state_fail("per-region LOD renders the 1D TF without lighting");
+    if (int e = prepare_regions(0)) return e;
+  }
   RaycastParams p;
   build_params(&p, lod, cam_, width_, height_, 0, framebuffer_.as<uint32_t>());
+  if (regions_active() && region_out_on_) {
+    const size_t need = (size_t)width_ * height_ * 4;
+    if (region_out_dev_.bytes < need) TVK_HIP(region_out_dev_.alloc(need));
+    p.region_out = region_out_dev_.as<int32_t>();
+  }
   if (paged_) {
     ++frame_;
     p.requests = requests_dev_.as<uint32_t>();
@@ -1171,8 +1274,16 @@ int HipRenderer::Paint() {
                            stream_));
     TVK_HIP(hipStreamSynchronize(stream_));
     bool overflow = false;
-    if (int e = service_requests(lod, &pass_complete, &overflow)) return e;
-    if (overflow && lod < hi) lod_floor_ = lod + 1;  // the view's bricks do not fit: go coarser
+    if (regions_active()) {
+      if (int e = service_requests(0, n_lod_ - 1, &pass_complete, &overflow)) return e;
+      if (overflow) {
+        set_last_error("the pool cannot hold this view's region bricks");
+        return TVK_ERR_OOM;
+      }
+    } else {
+      if (int e = service_requests(lod, &pass_complete, &overflow)) return e;
+      if (overflow && lod < hi) lod_floor_ = lod + 1;  // the view's bricks do not fit: go coarser
+    }
   }
 
   // frame bookkeeping and the host-side part of the stats
@@ -1241,8 +1352,11 @@ int HipRenderer::RenderBatch(const tvk_view* views, uint32_t n, tvk_view_stats* 
     std::vector<RaycastParams> jobs(cnt);
     // paged volumes: a pass per set of requested bricks, as in a capture
     for (int pass = 0; pass < 4 * kMaxLod + 8; ++pass) {
-      for (int k = 0; k < cnt; ++k)
+      for (int k = 0; k < cnt; ++k) {
         if (int e = prepare_cells((int)views[first + k].tf_slot, mode_index())) return e;
+        if (regions_active() && mode_index() == 0 && !lighting_)
+          if (int e = prepare_regions((int)views[first + k].tf_slot)) return e;
+      }
       for (int k = 0; k < cnt; ++k) {
         const uint32_t i = first + k;
         build_params(&jobs[k], lod, cams[i], views[i].width, views[i].height, (int)views[i].tf_slot,
@@ -1269,7 +1383,9 @@ int HipRenderer::RenderBatch(const tvk_view* views, uint32_t n, tvk_view_stats* 
       bool complete = true, overflow = false;
       if (paged_) {
         TVK_HIP(hipStreamSynchronize(stream_));
-        if (int e = service_requests(lod, &complete, &overflow)) return e;
+        if (int e = regions_active() ? service_requests(0, n_lod_ - 1, &complete, &overflow)
+                                     : service_requests(lod, &complete, &overflow))
+          return e;
         if (overflow && lod < hi) {
           lod_floor_ = lod + 1;
           lod = lod + 1;

@@ -86,6 +86,8 @@ struct TfSet {
   bool dirty2d = false;
   DevBuf tf2d_dev, prefix2d_dev;
   uint64_t version = 0;   // bumped by every change of this slot
+  DevBuf region_code;     // per LOD-0 region: chosen level | 0x80 when TF-empty
+  uint64_t region_key = 0;
   DevBuf cell_empty[3];      // per render mode: u8 per empty-space cell of every level
   uint64_t cell_key[3] = {};  // what each cell_empty was built for
 };
@@ -185,6 +187,10 @@ class HipRenderer {
   int SetEmptySpaceSkipping(bool on);
   int SetKernelTuning(uint32_t batch, uint32_t tile_group, uint32_t xcd_split);
   int SetRayTileShape(uint32_t tile_width);
+  // Per-region LOD (1D TF): one level per LOD-0 brick; n == 0 turns it off.
+  int SetRegionLods(const uint8_t* levels, uint64_t n);
+  int SetRegionOutput(bool on);
+  int ReadRegionOutput(int32_t* out, uint64_t n);
   // In-core volumes: cut the device-resident levels into bricks of another
   // size / empty-space cell edge (cell 0 keeps the current preference).
   int Rebrick(uint32_t brick, uint32_t cell);
@@ -230,6 +236,9 @@ class HipRenderer {
                   bool graph);
   uint32_t bricks_skipped(int lod, int tf_slot) const;
   float tf_scale(int n_entries) const;
+  bool regions_active() const { return !region_lod_h_.empty(); }
+  int prepare_regions(int tf_slot);
+  int upload_lods();
   // (re)build the empty flags of TF slot `tf_slot` for the current mode
   int prepare_cells(int tf_slot, int mode);
   int mode_index() const {
@@ -246,6 +255,7 @@ class HipRenderer {
   // false when another pass is needed, *overflow when the pool cannot hold
   // the bricks of this LOD for this view
   int service_requests(int lod, bool* complete, bool* overflow);
+  int service_requests(int lod_lo, int lod_hi, bool* complete, bool* overflow);
   int upload_brick(int gid, int lod, int bx, int by, int bz, int slot, int staging);
   static int flat_source(void* user, uint32_t lod, uint32_t bx, uint32_t by, uint32_t bz,
                          void* slot, uint64_t slot_bytes);
@@ -289,6 +299,11 @@ class HipRenderer {
   int cell_ = 16;      // empty-space cell edge (divides the brick size)
   int cell_pref_ = 16; // requested cell edge for the next volume
   int total_cells_ = 0;
+  DevBuf lods_dev_;                    // LodInfo of every level
+  std::vector<uint8_t> region_lod_h_;  // per LOD-0 brick (empty: uniform LOD)
+  uint64_t region_version_ = 0;
+  bool region_out_on_ = false;
+  DevBuf region_out_dev_;              // int32 per pixel of the framebuffer
   uint64_t cells_gen_ = 1;  // bumped whenever cell ranges change
 
   // paged (out-of-core) pools: GridLeaper-style requests + LRU

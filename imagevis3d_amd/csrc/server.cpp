@@ -121,7 +121,7 @@ int RenderServer::AddVolume(HipRenderer* r, uint32_t* id) {
 
 uint64_t RenderServer::variant_key(const LaunchInfo& li) {
   return (uint64_t)li.bpv | (uint64_t)li.interp << 4 | (uint64_t)li.mode << 8 |
-         (uint64_t)li.lighting << 12 | (uint64_t)li.batch << 16;
+         (uint64_t)li.lighting << 12 | (uint64_t)li.batch << 16 | (uint64_t)li.regions << 24;
 }
 
 int RenderServer::Submit(const tvk_request& q, uint64_t* ticket) {

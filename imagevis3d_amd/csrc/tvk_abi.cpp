@@ -426,6 +426,21 @@ int tvk_get_batch_framebuffer(tvk_renderer* r, uint32_t index, void** dptr) {
   return r->impl.BatchFramebuffer(index, dptr);
 }
 
+int tvk_set_region_lods(tvk_renderer* r, const uint8_t* levels, uint64_t n) {
+  TVK_GUARD(r);
+  TVK_TRY(r->impl.SetRegionLods(levels, n));
+}
+
+int tvk_set_region_output(tvk_renderer* r, int enabled) {
+  TVK_GUARD(r);
+  return r->impl.SetRegionOutput(enabled != 0);
+}
+
+int tvk_read_region_output(tvk_renderer* r, int32_t* out, uint64_t n) {
+  TVK_GUARD(r);
+  return r->impl.ReadRegionOutput(out, n);
+}
+
 int tvk_autotune(tvk_renderer* r, const tvk_view* views, uint32_t n, uint32_t flags,
                  tvk_tuning* out) {
   TVK_GUARD(r);

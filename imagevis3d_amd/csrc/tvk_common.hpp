@@ -93,6 +93,12 @@ struct RaycastParams {
   uint32_t* touch;
   uint32_t frame;
   uint32_t* framebuffer;
+  // per-region LOD (1D TF only): code per LOD-0 brick (level | 0x80 when
+  // transparent), all levels' geometry, and an optional per-pixel output of
+  // the region that added the most opacity.  region_code null: uniform LOD.
+  const uint8_t* region_code;
+  const LodInfo* lods;
+  int32_t* region_out;
 };
 
 // Launch-wide part of a raycast launch.  One launch renders any number of

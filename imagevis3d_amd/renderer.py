@@ -444,6 +444,25 @@ class Renderer:
         call("tvk_get_batch_framebuffer", self._h, int(index), C.byref(p))
         return p.value
 
+    def setRegionLODs(self, levels) -> None:
+        """Per-region LOD: one level per LOD-0 brick ((gz, gy, gx) or flat);
+        None turns it off."""
+        if levels is None:
+            call("tvk_set_region_lods", self._h, None, 0)
+            return
+        a = np.ascontiguousarray(np.asarray(levels, np.uint8).ravel())
+        call("tvk_set_region_lods", self._h, a.ctypes.data, a.size)
+
+    def setRegionOutput(self, on: bool) -> None:
+        call("tvk_set_region_output", self._h, int(bool(on)))
+
+    def readRegionOutput(self) -> np.ndarray:
+        """(H, W) int32: per pixel the LOD-0 region adding the most opacity."""
+        w, h = self._size
+        out = np.empty((h, w), np.int32)
+        call("tvk_read_region_output", self._h, out.ctypes.data, out.size)
+        return out
+
     def setLaunchGraphs(self, on: bool) -> None:
         call("tvk_set_launch_graphs", self._h, int(bool(on)))
 

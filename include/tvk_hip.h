@@ -340,6 +340,21 @@ int tvk_render_batch(tvk_renderer* r, const tvk_view* views, uint32_t n, tvk_vie
                      float* kernel_ms);
 /* Device pointer of view `index` of the last batch (RGBA8, row 0 at top). */
 int tvk_get_batch_framebuffer(tvk_renderer* r, uint32_t index, void** dptr);
+/* ---------------------------------------------------------- per-region LOD */
+/* Per-region LOD (no direct counterpart in the reference, whose LOD choice
+ * is per frame: setLODLimits, RenderWindow.cpp:447-451; this is what the
+ * learned LOD selector drives).  levels[r] is the LOD level used for LOD-0
+ * brick region r (x fastest, one entry per LOD-0 brick); the ray samples
+ * that region from that level with step 2^level / rate, its segments
+ * restarting at the region entry.  Applies to the 1D TF without lighting;
+ * n = 0 turns it off. */
+int tvk_set_region_lods(tvk_renderer* r, const uint8_t* levels, uint64_t n);
+/* With region output on, each paint also records per pixel the LOD-0
+ * region that added the most opacity (-1: none); read it back with
+ * tvk_read_region_output (W*H int32). */
+int tvk_set_region_output(tvk_renderer* r, int enabled);
+int tvk_read_region_output(tvk_renderer* r, int32_t* out, uint64_t n);
+
 /* ------------------------------------------------------------ autotuner */
 /* Launch autotuner (no counterpart in the reference).  Times the given
  * views (tvk_render_batch, images stay on the device) under candidate

@@ -267,6 +267,120 @@ int oracle_render_1d(const oracle_render_params* p, double* out, uint32_t* out_s
   return 0;
 }
 
+/* Per-region LOD model (1D TF, no lighting): the ray walks the grid of
+ * LOD-0 regions of edge `region_edge` (the LOD-0 brick size); region r is
+ * sampled from level l = region_lod[r] with step 2^l / rate, its segments
+ * restarting at the region entry (last one partial, midpoints, opacity
+ * correction by segment length in LOD-0 voxels).  A region whose level-l
+ * brick (the brick holding the region centre, stored range over payload +
+ * apron) is TF-transparent contributes nothing and is not counted.  The
+ * region boundary of a ray is found at the midpoint of a short probe step
+ * and crossed with at least 1e-3 voxel of progress, as the kernel does. */
+int oracle_render_regions(const oracle_render_params* p, int32_t n_levels,
+                          const void* const* levels, const int32_t* level_dims,
+                          int32_t region_edge, const uint8_t* region_lod, double* out,
+                          uint32_t* out_samples, uint8_t* out_flags) {
+  cam_ctx c;
+  if (cam_setup(p, &c)) return -1;
+  if (n_levels < 1 || n_levels > 16 || region_edge < 1) return -1;
+  oracle_render_params lp[16];
+  uint8_t* empty[16];
+  int nb[16][3];
+  for (int l = 0; l < n_levels; ++l) {
+    lp[l] = *p;
+    lp[l].voxels = levels[l];
+    lp[l].nx = level_dims[3 * l];
+    lp[l].ny = level_dims[3 * l + 1];
+    lp[l].nz = level_dims[3 * l + 2];
+    lp[l].brick_size = region_edge;
+    lp[l].mode = 0;
+    empty[l] = oracle_brick_empty_flags(&lp[l], nb[l]);
+  }
+  const double E = region_edge;
+  const int g0[3] = {(p->n0x + region_edge - 1) / region_edge, (p->n0y + region_edge - 1) / region_edge,
+                     (p->n0z + region_edge - 1) / region_edge};
+  const double lo[3] = {0, 0, 0};
+  const double hi[3] = {p->n0x, p->n0y, p->n0z};
+  const double base_step = 1.0 / p->rate;
+  for (int y = 0; y < p->height; ++y) {
+    double wy = 1.0 - (y + 0.5) / p->height;
+    double bg[3];
+    for (int i = 0; i < 3; ++i) bg[i] = p->bg_bottom[i] + (p->bg_top[i] - p->bg_bottom[i]) * wy;
+    for (int x = 0; x < p->width; ++x) {
+      double o[3], d[3];
+      cam_ray(p, &c, x, y, o, d);
+      double C[3] = {0, 0, 0}, A = 0;
+      uint32_t ns = 0;
+      uint8_t flags = 0;
+      double t0, t1;
+      if (box_hit(o, d, lo, hi, &t0, &t1)) {
+        if (t0 < 0) t0 = 0;
+        double t = t0;
+        int live = t1 > t0;
+        while (t < t1 && live) {
+          const double tm = t + 0.5 * fmin(base_step, t1 - t);
+          int cc[3];
+          for (int i = 0; i < 3; ++i) cc[i] = clampi((int)floor((o[i] + tm * d[i]) / E), 0, g0[i] - 1);
+          double tb = t1;
+          for (int i = 0; i < 3; ++i)
+            if (d[i] != 0.0) {
+              const double face = (d[i] > 0 ? cc[i] + 1 : cc[i]) * E;
+              tb = fmin(tb, (face - o[i]) / d[i]);
+            }
+          tb = fmin(fmax(tb, t + 1e-3), t1);
+          const int l = region_lod[((int64_t)cc[2] * g0[1] + cc[1]) * g0[0] + cc[0]] & 15;
+          if (l >= n_levels) return -1;
+          const oracle_render_params* q = &lp[l];
+          const double lsc[3] = {(double)q->nx / p->n0x, (double)q->ny / p->n0y, (double)q->nz / p->n0z};
+          int b[3];
+          for (int i = 0; i < 3; ++i) {
+            const double qc = (cc[i] + 0.5) * E * lsc[i];
+            b[i] = clampi((int)floor(qc / E), 0, nb[l][i] - 1);
+          }
+          if (!empty[l][((int64_t)b[2] * nb[l][1] + b[1]) * nb[l][0] + b[0]]) {
+            const double step = ldexp(base_step, l);
+            const double span = tb - t;
+            int64_t m = (int64_t)ceil(span / step);
+            if (m < 1) m = 1;
+            const double last = fmax(span - (double)(m - 1) * step, 0.0);
+            for (int64_t k = 0; k < m; ++k) {
+              const double len = k == m - 1 ? last : step;
+              const double ts = t + (double)k * step + 0.5 * len;
+              double qq[3];
+              for (int i = 0; i < 3; ++i) qq[i] = (o[i] + ts * d[i]) * lsc[i];
+              int nf = 0;
+              const double v = p->interp ? sample_lin(q, qq) : sample_nn(q, qq, &nf);
+              if (nf) flags |= 2;
+              ++ns;
+              double rgba[4];
+              tf_lookup(q, v, rgba);
+              const double a = rgba[3];
+              if (a <= 0.0) continue;
+              const double ac = a >= 1.0 ? 1.0 : 1.0 - pow(1.0 - a, len);
+              const double w = (1.0 - A) * ac;
+              for (int i = 0; i < 3; ++i) C[i] += w * rgba[i];
+              A += w;
+              if (fabs(A - p->term_alpha) < 1e-5) flags |= 1;
+              if (A >= p->term_alpha) {
+                live = 0;
+                break;
+              }
+            }
+          }
+          t = tb;
+        }
+      }
+      double* px = out + 4 * ((int64_t)y * p->width + x);
+      for (int i = 0; i < 3; ++i) px[i] = C[i] + (1.0 - A) * bg[i];
+      px[3] = A + (1.0 - A) * p->bg_alpha;
+      if (out_samples) out_samples[(int64_t)y * p->width + x] = ns;
+      if (out_flags) out_flags[(int64_t)y * p->width + x] = flags;
+    }
+  }
+  for (int l = 0; l < n_levels; ++l) free(empty[l]);
+  return 0;
+}
+
 uint64_t oracle_nominal_samples(const oracle_render_params* p) {
   cam_ctx c;
   if (cam_setup(p, &c)) return 0;

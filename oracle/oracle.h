@@ -83,6 +83,13 @@ typedef struct oracle_render_params {
  * N = g / sqrt(|g|^2 + 1e-8), V = -ray, L = light direction, H = (L+V)/|L+V|,
  * all in voxel space.  Isosurface: first sample with value >= iso, hit
  * refined linearly against the previous sample, shaded iso colour, opaque. */
+/* Per-region LOD render (1D TF): levels[l] with level_dims[3l..3l+2],
+ * region_lod per LOD-0 region of edge region_edge (x fastest); p supplies
+ * camera, TF and quality (its voxels / dims / lod are not used). */
+int oracle_render_regions(const oracle_render_params* p, int32_t n_levels,
+                          const void* const* levels, const int32_t* level_dims,
+                          int32_t region_edge, const uint8_t* region_lod, double* out_rgba,
+                          uint32_t* out_samples, uint8_t* out_flags);
 int oracle_render_1d(const oracle_render_params* p, double* out_rgba,
                      uint32_t* out_samples, uint8_t* out_flags);
 

@@ -65,6 +65,9 @@ def lib():
         P = C.POINTER(RenderParams)
         L.oracle_render_1d.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_render_1d.restype = C.c_int
+        L.oracle_render_regions.argtypes = [P, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
+                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_render_regions.restype = C.c_int
         L.oracle_nominal_samples.argtypes = [P]
         L.oracle_nominal_samples.restype = C.c_uint64
         L.baseline_render_1d_f32.argtypes = [P, C.c_void_p, C.c_int32]
@@ -165,6 +168,24 @@ def render(p: RenderParams):
     rc = lib().oracle_render_1d(C.byref(p), _ptr(out), _ptr(ns), _ptr(fl))
     if rc != 0:
         raise RuntimeError("oracle_render_1d failed (singular region matrix?)")
+    return out, ns, fl
+
+
+def render_regions(p: RenderParams, levels, region_lod: np.ndarray, region_edge: int):
+    """FP64 per-region LOD render (1D TF): `levels` = LOD pyramid (z, y, x
+    arrays), `region_lod` = level per LOD-0 region of edge `region_edge`
+    ((gz, gy, gx) or flat, x fastest).  Returns (rgba, samples, flags)."""
+    levels = [np.ascontiguousarray(v) for v in levels]
+    ptrs = (C.c_void_p * len(levels))(*[v.ctypes.data for v in levels])
+    dims = np.array([[v.shape[2], v.shape[1], v.shape[0]] for v in levels], np.int32).ravel()
+    rl = np.ascontiguousarray(region_lod, np.uint8).ravel()
+    out = np.zeros((p.height, p.width, 4), np.float64)
+    ns = np.zeros((p.height, p.width), np.uint32)
+    fl = np.zeros((p.height, p.width), np.uint8)
+    rc = lib().oracle_render_regions(C.byref(p), len(levels), ptrs, _ptr(dims), int(region_edge),
+                                     _ptr(rl), _ptr(out), _ptr(ns), _ptr(fl))
+    if rc != 0:
+        raise RuntimeError("oracle_render_regions failed")
     return out, ns, fl
 
 

@@ -154,6 +154,10 @@ SIGNATURES = {
     "tvk_get_batch_framebuffer": [H, u32, C.POINTER(C.c_void_p)],
     "tvk_set_launch_graphs": [H, i32],
     "tvk_set_region_lods": [H, P, u64],
+    "tvk_brick_histograms": [H, u32, u32, P, u64],
+    "tvk_lod_mlp_create": [i32, P, P, C.POINTER(C.c_void_p)],
+    "tvk_lod_mlp_predict": [C.c_void_p, P, u64, P, C.POINTER(f32)],
+    "tvk_lod_mlp_destroy": [C.c_void_p],
     "tvk_set_region_output": [H, i32],
     "tvk_read_region_output": [H, P, u64],
     "tvk_autotune": [H, C.POINTER(View), u32, u32, C.POINTER(Tuning)],

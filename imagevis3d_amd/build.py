@@ -28,6 +28,7 @@ SOURCES = [
     "uvf.cpp",
     "volume_gen.cpp",
     "volume_gen_gpu.hip",
+    "lod_mlp.hip",
     "tvk_abi.cpp",
 ]
 

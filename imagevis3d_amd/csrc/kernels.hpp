@@ -49,6 +49,11 @@ hipError_t launch_histogram_2d(const void* vol, int bpv, const int64_t dims[3],
                                const unsigned long long* kmax, int n_value, int n_grad,
                                unsigned long long* hist, hipStream_t stream);
 
+// Per-brick histograms of a flat level (payload only): bins in {2, 4, ...,
+// 64} equal bins over the value range; out = n_bricks x bins u32.
+hipError_t launch_brick_histograms(const void* level, int bpv, const int64_t dims[3], int B, int bins,
+                                   uint32_t* out, hipStream_t stream);
+
 // Zero-fill a u64 buffer (a kernel, so the call is safe under graph capture).
 hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v,
                            hipStream_t stream);
@@ -64,6 +69,12 @@ hipError_t launch_cell_empty(const uint32_t* cell_range, int64_t n_cells, const 
 bool is_gpu_volume_kind(int kind);
 hipError_t launch_generate_volume(int kind, uint64_t seed, const int64_t dims[3], int bits,
                                   int64_t z0, int64_t nz, void* out, hipStream_t stream);
+
+// Learned-LOD error predictor (bf16 MLP 32 -> 4 x 64 -> 1 on MFMA): w = bf16
+// weights [in][out] per layer then the 64 output weights, b = f32 biases
+// (4 x 64 + 1), x = n x 32 f32 inputs, y = n outputs.
+hipError_t launch_lod_mlp(const uint16_t* w, const float* b, const float* x, float* y, int n,
+                          hipStream_t stream);
 
 // Kernel variant of a raycast launch: every job of the table shares it.
 struct LaunchInfo {

@@ -376,6 +376,47 @@ hipError_t launch_slot_cell_ranges(const void* slot, int bpv, int S, int apron, 
   return hipGetLastError();
 }
 
+// Per-brick value histogram of a flat level: one work-group per brick of
+// payload B (no apron), `bins` equal bins over [0, 2^(8 bpv)), LDS counts.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void brick_hist_kernel(const T* __restrict__ level, int64_t nx,
+                                                              int64_t ny, int64_t nz, int B, int gx,
+                                                              int gy, int bins, int shift,
+                                                              uint32_t* __restrict__ out) {
+  __shared__ uint32_t h[64];
+  for (int i = (int)threadIdx.x; i < bins; i += kThreads) h[i] = 0;
+  __syncthreads();
+  const int b = (int)blockIdx.x;
+  const int bx = b % gx, by = (b / gx) % gy, bz = b / (gx * gy);
+  const int64_t x0 = (int64_t)bx * B, y0 = (int64_t)by * B, z0 = (int64_t)bz * B;
+  const int w = (int)min<int64_t>(B, nx - x0), hh = (int)min<int64_t>(B, ny - y0),
+            d = (int)min<int64_t>(B, nz - z0);
+  const int64_t cnt = (int64_t)w * hh * d;
+  for (int64_t i = threadIdx.x; i < cnt; i += kThreads) {
+    const int x = (int)(i % w), y = (int)((i / w) % hh), z = (int)(i / ((int64_t)w * hh));
+    const uint32_t v = level[((z0 + z) * ny + (y0 + y)) * nx + (x0 + x)];
+    atomicAdd(&h[v >> shift], 1u);
+  }
+  __syncthreads();
+  for (int i = (int)threadIdx.x; i < bins; i += kThreads) out[(int64_t)b * bins + i] = h[i];
+}
+
+hipError_t launch_brick_histograms(const void* level, int bpv, const int64_t d[3], int B, int bins,
+                                   uint32_t* out, hipStream_t stream) {
+  const int gx = (int)((d[0] + B - 1) / B), gy = (int)((d[1] + B - 1) / B),
+            gz = (int)((d[2] + B - 1) / B);
+  int shift = 0;
+  while ((1 << (8 * bpv - shift)) > bins) ++shift;
+  const unsigned blocks = (unsigned)(gx * gy * gz);
+  if (bpv == 1)
+    brick_hist_kernel<uint8_t><<<blocks, kThreads, 0, stream>>>((const uint8_t*)level, d[0], d[1], d[2],
+                                                                 B, gx, gy, bins, shift, out);
+  else
+    brick_hist_kernel<uint16_t><<<blocks, kThreads, 0, stream>>>((const uint16_t*)level, d[0], d[1],
+                                                                  d[2], B, gx, gy, bins, shift, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_histogram_1d(const void* vol, int bpv, int64_t n, unsigned long long* hist,
                                hipStream_t stream) {
   const int64_t nb = bpv == 1 ? 256 : 65536;

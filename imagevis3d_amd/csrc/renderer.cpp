@@ -639,6 +639,25 @@ int HipRenderer::BrickMinMax(uint32_t lod, float* mins, float* maxs, uint64_t n)
   return TVK_OK;
 }
 
+int HipRenderer::BrickHistograms(uint32_t lod, uint32_t bins, uint32_t* out, uint64_t n) {
+  if (!has_volume_) return state_fail("no volume loaded");
+  if (paged_) return state_fail("brick histograms need an in-core volume");
+  if (lod >= (uint32_t)n_lod_) return arg_fail("no such LOD level");
+  if (bins < 2 || bins > 64 || (bins & (bins - 1))) return arg_fail("bins must be a power of two in [2, 64]");
+  const LodInfo& L = lods_[lod];
+  const uint64_t nb = (uint64_t)L.grid[0] * L.grid[1] * L.grid[2];
+  if (!out || n < nb * bins) return arg_fail("output needs n_bricks x bins entries");
+  TVK_HIP(hipSetDevice(device_));
+  DevBuf h;
+  TVK_HIP(h.alloc(nb * bins * 4));
+  const int64_t d[3] = {L.dims[0], L.dims[1], L.dims[2]};
+  TVK_HIP(launch_brick_histograms(levels_[lod].p, (int)bits_ / 8, d, brick_, (int)bins, h.as<uint32_t>(),
+                                  stream_));
+  TVK_HIP(hipMemcpyAsync(out, h.p, nb * bins * 4, hipMemcpyDeviceToHost, stream_));
+  TVK_HIP(hipStreamSynchronize(stream_));
+  return TVK_OK;
+}
+
 int HipRenderer::Histogram1D(uint64_t* hist, uint64_t n_bins) {
   if (!has_volume_) return state_fail("no volume loaded");
   const uint64_t nb = 1ull << bits_;

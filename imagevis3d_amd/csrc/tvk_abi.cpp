@@ -29,6 +29,13 @@ struct tvk_server {
   tvk::RenderServer impl;
 };
 
+struct tvk_lod_mlp {
+  int device = -1;
+  tvk::DevBuf w, b, x, y;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
 #define TVK_GUARD(r)                                  \
   do {                                                \
     if (!(r)) {                                       \
@@ -426,6 +433,11 @@ int tvk_get_batch_framebuffer(tvk_renderer* r, uint32_t index, void** dptr) {
   return r->impl.BatchFramebuffer(index, dptr);
 }
 
+int tvk_brick_histograms(tvk_renderer* r, uint32_t lod, uint32_t bins, uint32_t* out, uint64_t n) {
+  TVK_GUARD(r);
+  return r->impl.BrickHistograms(lod, bins, out, n);
+}
+
 int tvk_set_region_lods(tvk_renderer* r, const uint8_t* levels, uint64_t n) {
   TVK_GUARD(r);
   TVK_TRY(r->impl.SetRegionLods(levels, n));
@@ -502,6 +514,81 @@ int tvk_server_get_stats(tvk_server* s, tvk_server_stats* out) {
     return TVK_ERR_ARG;
   }
   return s->impl.Stats(out);
+}
+
+static const uint64_t kMlpWeights = 32 * 64 + 3 * 64 * 64 + 64, kMlpBiases = 4 * 64 + 1;
+
+int tvk_lod_mlp_create(int device, const uint16_t* w, const float* b, tvk_lod_mlp** out) {
+  if (!w || !b || !out) {
+    tvk::set_last_error("null weights, biases or output");
+    return TVK_ERR_ARG;
+  }
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+    tvk::set_last_error("device ordinal out of range");
+    return TVK_ERR_ARG;
+  }
+  tvk_lod_mlp* m = new (std::nothrow) tvk_lod_mlp();
+  if (!m) return TVK_ERR_OOM;
+  m->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&m->ev[0]);
+  if (e == hipSuccess) e = hipEventCreate(&m->ev[1]);
+  if (e == hipSuccess) e = m->w.alloc(kMlpWeights * 2);
+  if (e == hipSuccess) e = m->b.alloc(kMlpBiases * 4);
+  if (e == hipSuccess) e = hipMemcpy(m->w.p, w, kMlpWeights * 2, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(m->b.p, b, kMlpBiases * 4, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    tvk::set_last_error(std::string("lod mlp setup: ") + hipGetErrorString(e));
+    tvk_lod_mlp_destroy(m);
+    return TVK_ERR_HIP;
+  }
+  *out = m;
+  return TVK_OK;
+}
+
+int tvk_lod_mlp_predict(tvk_lod_mlp* m, const float* x, uint64_t n, float* y, float* kernel_ms) {
+  if (!m || (n && (!x || !y)) || n > (1ull << 30)) {
+    tvk::set_last_error("bad lod mlp arguments");
+    return TVK_ERR_ARG;
+  }
+  if (n == 0) return TVK_OK;
+  hipError_t e = hipSetDevice(m->device);
+  if (e == hipSuccess && m->x.bytes < n * 32 * 4) e = m->x.alloc(n * 32 * 4);
+  if (e == hipSuccess && m->y.bytes < n * 4) e = m->y.alloc(n * 4);
+  if (e == hipSuccess) e = hipMemcpyAsync(m->x.p, x, n * 32 * 4, hipMemcpyHostToDevice, m->stream);
+  if (e == hipSuccess) e = hipEventRecord(m->ev[0], m->stream);
+  if (e == hipSuccess)
+    e = tvk::launch_lod_mlp(m->w.as<uint16_t>(), m->b.as<float>(), m->x.as<float>(), m->y.as<float>(),
+                            (int)n, m->stream);
+  if (e == hipSuccess) e = hipEventRecord(m->ev[1], m->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(y, m->y.p, n * 4, hipMemcpyDeviceToHost, m->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  if (e == hipSuccess && kernel_ms) e = hipEventElapsedTime(kernel_ms, m->ev[0], m->ev[1]);
+  if (e != hipSuccess) {
+    tvk::set_last_error(std::string("lod mlp predict: ") + hipGetErrorString(e));
+    return TVK_ERR_HIP;
+  }
+  return TVK_OK;
+}
+
+int tvk_lod_mlp_destroy(tvk_lod_mlp* m) {
+  if (!m) return TVK_OK;
+  if (m->device >= 0) {
+    (void)hipSetDevice(m->device);
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    m->w.reset();
+    m->b.reset();
+    m->x.reset();
+    m->y.reset();
+    for (auto& ev : m->ev)
+      if (ev) (void)hipEventDestroy(ev);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+  }
+  delete m;
+  return TVK_OK;
 }
 
 int tvk_set_launch_graphs(tvk_renderer* r, int enabled) {

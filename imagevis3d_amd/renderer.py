@@ -453,6 +453,13 @@ class Renderer:
         a = np.ascontiguousarray(np.asarray(levels, np.uint8).ravel())
         call("tvk_set_region_lods", self._h, a.ctypes.data, a.size)
 
+    def brickHistograms(self, lod: int = 0, bins: int = 16) -> np.ndarray:
+        """(n_bricks, bins) uint32 value histograms of a level's bricks."""
+        grid, _, _ = self.getDataset().getBrickLayout(lod)
+        out = np.empty((grid[0] * grid[1] * grid[2], bins), np.uint32)
+        call("tvk_brick_histograms", self._h, int(lod), int(bins), out.ctypes.data, out.size)
+        return out
+
     def setRegionOutput(self, on: bool) -> None:
         call("tvk_set_region_output", self._h, int(bool(on)))
 

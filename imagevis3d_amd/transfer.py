@@ -117,5 +117,27 @@ def gaussian_lobes_2d(w: int = 256, h: int = 256, seed: int = 3, lobes: int = 4)
     return np.concatenate([rgb, alpha[..., None]], -1).astype(np.float32)
 
 
+def tubes_tf(n: int = 256, seed: int = 0) -> np.ndarray:
+    """1D TF of the learned-LOD workload (noise + thin tubes): the tube
+    values (top 22% of the range) nearly opaque in a seeded warm colour, a
+    seeded low-alpha Gaussian bump (peak 0.01..0.04, sigma 4..10% of the
+    range) over the noise values (centre 15..50% of the range) in a seeded
+    cool colour."""
+    rng = np.random.default_rng(seed)
+    x = np.arange(n, dtype=np.float64) / (n - 1)
+    tf = np.zeros((n, 4), np.float32)
+    warm = np.array([rng.uniform(0.8, 1.0), rng.uniform(0.3, 0.7), rng.uniform(0.0, 0.3)])
+    cool = np.array([rng.uniform(0.0, 0.3), rng.uniform(0.4, 0.8), rng.uniform(0.7, 1.0)])
+    c, s, peak = rng.uniform(0.15, 0.5), rng.uniform(0.04, 0.10), rng.uniform(0.01, 0.04)
+    bump = peak * np.exp(-0.5 * ((x - c) / s) ** 2)
+    bump[np.abs(x - c) > 3 * s] = 0.0
+    tube = np.clip((x - 0.78) / 0.04, 0.0, 1.0) * rng.uniform(0.7, 0.95)
+    a = np.maximum(bump, tube)
+    w = np.where(tube > bump, 1.0, 0.0)[:, None]
+    tf[:, :3] = w * warm + (1 - w) * cool
+    tf[:, 3] = a
+    return tf
+
+
 def constant_tf(n: int, rgba) -> np.ndarray:
     return np.tile(np.asarray(rgba, np.float32), (n, 1))

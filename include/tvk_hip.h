@@ -355,6 +355,25 @@ int tvk_set_region_lods(tvk_renderer* r, const uint8_t* levels, uint64_t n);
 int tvk_set_region_output(tvk_renderer* r, int enabled);
 int tvk_read_region_output(tvk_renderer* r, int32_t* out, uint64_t n);
 
+/* Per-brick value histograms of LOD level `lod` (payload voxels, `bins` =
+ * 2..64 equal bins over [0, 2^bits)): n_bricks x bins counts, bricks in
+ * page-table order (the learned-LOD features; in-core volumes). */
+int tvk_brick_histograms(tvk_renderer* r, uint32_t lod, uint32_t bins, uint32_t* out, uint64_t n);
+
+/* Learned-LOD error predictor (no counterpart in the reference): a bf16 MLP
+ * 32 inputs -> 4 hidden layers of 64 (ReLU) -> 1 output evaluated on the
+ * MFMA units with FP32 accumulation.  weights_bf16: layer-major, each layer
+ * [in][out] row-major (32x64, then 3 x 64x64), then the 64 output weights
+ * (32*64 + 3*64*64 + 64 values, bfloat16 bits); biases: 4 x 64 + 1 floats.
+ * predict: n rows of 32 floats (host) -> n floats (host); kernel_ms (may be
+ * NULL) gets the kernel time. */
+typedef struct tvk_lod_mlp tvk_lod_mlp;
+int tvk_lod_mlp_create(int device, const uint16_t* weights_bf16, const float* biases,
+                       tvk_lod_mlp** out);
+int tvk_lod_mlp_predict(tvk_lod_mlp* m, const float* inputs, uint64_t n, float* outputs,
+                        float* kernel_ms);
+int tvk_lod_mlp_destroy(tvk_lod_mlp* m);
+
 /* ------------------------------------------------------------ autotuner */
 /* Launch autotuner (no counterpart in the reference).  Times the given
  * views (tvk_render_batch, images stay on the device) under candidate

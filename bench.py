@@ -228,6 +228,111 @@ def run_batched(batches: int = 3, seed: int = 3) -> dict:
     return out
 
 
+def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: int = 24,
+                    eval_size=(3840, 2160)) -> dict:
+    """The learned-LOD workload: an n^3 uint8 volume of multi-scale noise
+    with 64 thin tubes (device-generated, in HBM), 128^3 bricks, 6 LODs,
+    1D TF (tubes_tf: opaque tubes over a faint noise bump), rate 2.  The
+    bf16 MLP is trained on the renderer's own fine-vs-coarse renders (32
+    views x 4 TFs at 480x270), its selection budget calibrated on 4 views,
+    then 24 new random views at 3840x2160 with a new TF are rendered at full
+    resolution and with the learned per-region LOD (the brick budget is a
+    quarter of the LOD-0 bricks, as 16 GB of a 64 GiB volume).  Frame time
+    of the learned path includes features, MLP and selection."""
+    import math
+
+    import numpy as np
+
+    from imagevis3d_amd import learned_lod as LL
+    from imagevis3d_amd import transfer, tuvok
+    from imagevis3d_amd.renderer import VOL_NOISE_TUBES
+
+    def views(k, seed, w, h):
+        rng = np.random.default_rng(seed)
+        out = []
+        for _ in range(k):
+            th, ph = rng.uniform(0, 2 * math.pi), rng.uniform(-1.0, 1.0)
+            d = rng.uniform(1.2, 1.9)
+            eye = (d * math.cos(ph) * math.sin(th), d * math.sin(ph), d * math.cos(ph) * math.cos(th))
+            out.append(dict(view_pos=eye, view_dir=tuple(-e for e in eye), up_dir=(0.0, 1.0, 0.0),
+                            fov=50.0, width=w, height=h))
+        return out
+
+    dims = (n, n, n)
+    r = tuvok.renderer.new(tuvok.renderer.types.OpenGL_Raycaster, False, False, False, False)
+    assert r.initialize(tuvok.createContext(480, 270, device=0)), r.lastError()
+    r.setRendererTarget(tuvok.renderer.types.RT_Capture)
+    t0 = time.perf_counter()
+    r.setGeneratedVolume(VOL_NOISE_TUBES, 4, dims, 8, edge, 2, n_lod)
+    setup_s = time.perf_counter() - t0
+    r.setSampleRateModifier(2.0)
+    hist = LL.region_histograms(r)
+    t0 = time.perf_counter()
+    X, Y = LL.training_rows(r, dims, edge, n_lod, views(32, 11, 480, 270),
+                            [transfer.tubes_tf(256, seed=s) for s in range(4)], hist)
+    data_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    wbits, b, loss = LL.train(X, Y, steps=3000)
+    train_s = time.perf_counter() - t0
+    pred = LL.Predictor(wbits, b)
+    tf = transfer.tubes_tf(256, seed=9)
+    r.set1DTrans(tf)
+    mom = LL.tf_moments(hist, tf)
+    s_slot = edge + 4
+    slot_bytes = (s_slot ** 3 + 255) // 256 * 256
+    n_reg = (n // edge) ** 3
+    mem_budget = n_reg * slot_bytes // 4
+    safety, worst = LL.calibrate_safety(r, pred, views(4, 12, *eval_size), dims, edge, n_lod, hist,
+                                        mom, 1.0, slot_bytes, mem_budget)
+    full_ms, learned_ms, learned_kernel_ms, sel_ms, rm, rm_uniform, mem = [], [], [], [], [], [], []
+    s_full = s_learned = 0
+    for v in views(eval_views, 13, *eval_size):
+        r.resize(eval_size)
+        LL.set_view(r, v)
+        r.setRegionLODs(None)
+        assert r.paint(), r.lastError()
+        st = r.frameStats()
+        full_ms.append(st.kernel_ms)
+        s_full += st.evaluated_samples
+        uni = r.capture()
+        ref = LL.full_resolution(r, dims, edge)
+        t0 = time.perf_counter()
+        lev, _, m = LL.choose_levels(pred, v, dims, edge, n_lod, hist, mom, safety, slot_bytes,
+                                     mem_budget)
+        r.setRegionLODs(lev)
+        sel = (time.perf_counter() - t0) * 1e3
+        assert r.paint(), r.lastError()
+        st = r.frameStats()
+        s_learned += st.evaluated_samples
+        img = r.capture()
+        sel_ms.append(sel)
+        learned_kernel_ms.append(st.kernel_ms)
+        learned_ms.append(st.kernel_ms + sel)
+        rm.append(LL.rmse255(img, ref))
+        rm_uniform.append(LL.rmse255(img, uni))
+        mem.append(m)
+        r.setRegionLODs(None)
+    out = {
+        "workload": f"learned-lod-{n}^3-u8-noise-tubes-{edge}^3-bricks-{n_lod}lods-"
+                    f"{eval_size[0]}x{eval_size[1]}-{eval_views}views",
+        "fps_full": 1e3 / float(np.mean(full_ms)),
+        "fps_learned": 1e3 / float(np.mean(learned_ms)),
+        "speedup": float(np.mean(full_ms) / np.mean(learned_ms)),
+        "rmse_mean": float(np.mean(rm)), "rmse_max": float(np.max(rm)), "rmse_target": 1.0,
+        "rmse_vs_global_sampling_mean": float(np.mean(rm_uniform)),
+        "evaluated_sample_ratio": s_learned / max(s_full, 1),
+        "selection_ms_mean": float(np.mean(sel_ms)), "mlp_kernel_ms": pred.kernel_ms,
+        "brick_bytes_mean": float(np.mean(mem)), "brick_budget": mem_budget,
+        "train_rows": int(len(X)), "train_loss": loss, "budget_factor": safety,
+        "calibration_worst_rmse": worst,
+        "seconds": {"volume": round(setup_s, 2), "training_data": round(data_s, 2),
+                    "training": round(train_s, 2)},
+    }
+    pred.close()
+    r.cleanup()
+    return out
+
+
 def main() -> int:
     if "--calibrate-copy" in sys.argv:
         return calibrate_copy()
@@ -251,6 +356,9 @@ def main() -> int:
     ap.add_argument("--serving-only", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the out-of-core batched workload")
     ap.add_argument("--batched-only", action="store_true")
+    ap.add_argument("--no-learned-lod", action="store_true", help="skip the learned-LOD workload")
+    ap.add_argument("--learned-lod-only", action="store_true")
+    ap.add_argument("--learned-lod-size", type=int, default=2048)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
@@ -265,6 +373,9 @@ def main() -> int:
         return 0
     if args.batched_only:
         print(json.dumps(run_batched()))
+        return 0
+    if args.learned_lod_only:
+        print(json.dumps(run_learned_lod(args.learned_lod_size)))
         return 0
 
     w = workloads.medium()
@@ -360,6 +471,12 @@ def main() -> int:
         "cpu_baseline": None,
     }
     r.cleanup()
+    if not args.no_learned_lod:
+        print("[bench] learned-LOD workload", file=sys.stderr, flush=True)
+        ll = run_learned_lod(args.learned_lod_size)
+        out["learned_lod"] = ll
+        out["learned_lod_fps"] = ll["fps_learned"]
+        print(f"[bench] learned LOD {ll}", file=sys.stderr, flush=True)
     if not args.no_batched:
         print("[bench] out-of-core batched workload", file=sys.stderr, flush=True)
         out["batched"] = run_batched()

@@ -17,10 +17,12 @@ Training data come from the renderer itself: for each (view, TF) the frame
 is rendered with every region at level 0 (recording per pixel the region
 that added the most opacity) and with every region at level l; the per-pixel
 squared error of level l is attributed to that region.  The selector then
-picks, per region, the coarsest level whose predicted error, times the
-region's footprint, fits an even share of the frame's error budget (RMSE
-<= 1/255 on RGB), found by bisection; full resolution (level 0) is the
-fallback for every region the budget cannot cover.
+cuts the brick octree: starting from the coarsest bricks it splits the
+brick whose children reduce the predicted error (mse x footprint) the most
+per added brick, until the predicted error fits the frame's budget (RMSE
+1/255, scaled by a factor calibrated against full-resolution renders) or
+the brick budget is spent; full resolution (level 0) is where splitting
+ends at the finest level.
 """
 from __future__ import annotations
 
@@ -270,6 +272,81 @@ def bricks_needed(levels: np.ndarray, grid) -> int:
     return int(np.unique(key).size)
 
 
+def select_octree(pred_log_mse: np.ndarray, area: np.ndarray, budget_sse: float, grid,
+                  brick_bytes: int = 1, mem_budget: int = 0):
+    """Octree cut of the brick pyramid: start from the coarsest level's
+    bricks and repeatedly split the brick whose children cut the predicted
+    SSE the most per added brick, until the predicted SSE fits `budget_sse`
+    or the next split would exceed `mem_budget` bytes (0: no memory cap).
+    A level-l brick covers the LOD-0 regions c with c >> l equal to its
+    index; its predicted SSE is the sum over those regions of mse_l x
+    footprint.  Returns (levels per region, predicted SSE, brick bytes)."""
+    import heapq
+
+    L = pred_log_mse.shape[0] + 1
+    gx, gy, gz = (int(v) for v in grid)
+    mse = np.exp(pred_log_mse.astype(np.float64))
+    cost = np.vstack([np.zeros((1, mse.shape[1])), mse * area[None, :]]).reshape(L, gz, gy, gx)
+    # group sums per level: E[l][bz, by, bx]
+    E = []
+    for l in range(L):
+        s = 1 << l
+        ez, ey, ex = -(-gz // s), -(-gy // s), -(-gx // s)
+        e = np.zeros((ez, ey, ex))
+        np.add.at(e, tuple(np.meshgrid(np.arange(gz) >> l, np.arange(gy) >> l, np.arange(gx) >> l,
+                                       indexing="ij")), cost[l])
+        E.append(e)
+    top = L - 1
+    active = {(top, z, y, x) for z in range(E[top].shape[0]) for y in range(E[top].shape[1])
+              for x in range(E[top].shape[2])}
+    total = float(E[top].sum())
+    n_bricks = len(active)
+
+    def children(l, z, y, x):
+        out = []
+        for dz in (0, 1):
+            for dy in (0, 1):
+                for dx in (0, 1):
+                    c = (l - 1, 2 * z + dz, 2 * y + dy, 2 * x + dx)
+                    if c[1] < E[l - 1].shape[0] and c[2] < E[l - 1].shape[1] and c[3] < E[l - 1].shape[2]:
+                        out.append(c)
+        return out
+
+    def gain(b):
+        l = b[0]
+        if l == 0:
+            return None
+        ch = children(*b)
+        g = E[l][b[1], b[2], b[3]] - sum(E[c[0]][c[1], c[2], c[3]] for c in ch)
+        return g / max(len(ch) - 1, 1), ch
+
+    heap = []
+    for b in active:
+        gc = gain(b)
+        if gc and gc[0] > 0:
+            heapq.heappush(heap, (-gc[0], b))
+    while heap and total > budget_sse:
+        neg, b = heapq.heappop(heap)
+        gc = gain(b)
+        ch = gc[1]
+        if mem_budget and (n_bricks + len(ch) - 1) * brick_bytes > mem_budget:
+            break
+        active.discard(b)
+        total -= E[b[0]][b[1], b[2], b[3]]
+        for c in ch:
+            active.add(c)
+            total += E[c[0]][c[1], c[2], c[3]]
+            g2 = gain(c)
+            if g2 and g2[0] > 0:
+                heapq.heappush(heap, (-g2[0], c))
+        n_bricks += len(ch) - 1
+    lev = np.zeros((gz, gy, gx), np.uint8)
+    for (l, z, y, x) in active:
+        s = 1 << l
+        lev[z * s:(z + 1) * s, y * s:(y + 1) * s, x * s:(x + 1) * s] = l
+    return lev.ravel(), max(total, 0.0), n_bricks * brick_bytes
+
+
 def select_levels(pred_log_mse: np.ndarray, area: np.ndarray, budget_sse: float,
                   grid=None, brick_bytes: int = 0, mem_budget: int = 0):
     """pred_log_mse: (n_levels - 1, n_regions) predictions for levels 1..L-1.
@@ -338,7 +415,7 @@ def choose_levels(pred, view, dims, edge, n_lod, hist, moments, safety: float,
     y = pred(x).reshape(n_lod - 1, -1)
     budget = safety * (1.0 / 255.0) ** 2 * 4 * view["width"] * view["height"]
     grid = tuple(int(v) for v in np.ceil(np.asarray(dims) / edge))
-    return select_levels(y, area, budget, grid, brick_bytes, mem_budget)
+    return select_octree(y, area, budget, grid, max(brick_bytes, 1), mem_budget)
 
 
 def calibrate_safety(renderer, pred, views, dims, edge, n_lod, hist, moments, target: float = 1.0,

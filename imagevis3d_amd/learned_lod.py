@@ -320,11 +320,16 @@ def select_octree(pred_log_mse: np.ndarray, area: np.ndarray, budget_sse: float,
         g = E[l][b[1], b[2], b[3]] - sum(E[c[0]][c[1], c[2], c[3]] for c in ch)
         return g / max(len(ch) - 1, 1), ch
 
+    # bricks with error left are split in order of gain per added brick; a
+    # split that gains nothing now may still open finer levels that do
+    def push(b):
+        gc = gain(b)
+        if gc and E[b[0]][b[1], b[2], b[3]] > 0:
+            heapq.heappush(heap, (-gc[0], b))
+
     heap = []
     for b in active:
-        gc = gain(b)
-        if gc and gc[0] > 0:
-            heapq.heappush(heap, (-gc[0], b))
+        push(b)
     while heap and total > budget_sse:
         neg, b = heapq.heappop(heap)
         gc = gain(b)
@@ -336,9 +341,7 @@ def select_octree(pred_log_mse: np.ndarray, area: np.ndarray, budget_sse: float,
         for c in ch:
             active.add(c)
             total += E[c[0]][c[1], c[2], c[3]]
-            g2 = gain(c)
-            if g2 and g2[0] > 0:
-                heapq.heappush(heap, (-g2[0], c))
+            push(c)
         n_bricks += len(ch) - 1
     lev = np.zeros((gz, gy, gx), np.uint8)
     for (l, z, y, x) in active:

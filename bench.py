@@ -267,9 +267,10 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
     setup_s = time.perf_counter() - t0
     r.setSampleRateModifier(2.0)
     hist = LL.region_histograms(r)
+    detail = LL.region_detail(r)
     t0 = time.perf_counter()
     X, Y = LL.training_rows(r, dims, edge, n_lod, views(32, 11, 480, 270),
-                            [transfer.tubes_tf(256, seed=s) for s in range(4)], hist)
+                            [transfer.tubes_tf(256, seed=s) for s in range(4)], hist, detail)
     data_s = time.perf_counter() - t0
     t0 = time.perf_counter()
     wbits, b, loss = LL.train(X, Y, steps=3000)
@@ -283,7 +284,7 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
     n_reg = (n // edge) ** 3
     mem_budget = n_reg * slot_bytes // 4
     safety, worst = LL.calibrate_safety(r, pred, views(4, 12, *eval_size), dims, edge, n_lod, hist,
-                                        mom, 1.0, slot_bytes, mem_budget)
+                                        mom, 1.0, slot_bytes, mem_budget, detail)
     full_ms, learned_ms, learned_kernel_ms, sel_ms, rm, rm_uniform, mem = [], [], [], [], [], [], []
     s_full = s_learned = 0
     for v in views(eval_views, 13, *eval_size):
@@ -298,7 +299,7 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
         ref = LL.full_resolution(r, dims, edge)
         t0 = time.perf_counter()
         lev, _, m = LL.choose_levels(pred, v, dims, edge, n_lod, hist, mom, safety, slot_bytes,
-                                     mem_budget)
+                                     mem_budget, detail)
         r.setRegionLODs(lev)
         sel = (time.perf_counter() - t0) * 1e3
         assert r.paint(), r.lastError()

@@ -155,6 +155,7 @@ SIGNATURES = {
     "tvk_set_launch_graphs": [H, i32],
     "tvk_set_region_lods": [H, P, u64],
     "tvk_brick_histograms": [H, u32, u32, P, u64],
+    "tvk_region_detail": [H, P, u64],
     "tvk_lod_mlp_create": [i32, P, P, C.POINTER(C.c_void_p)],
     "tvk_lod_mlp_predict": [C.c_void_p, P, u64, P, C.POINTER(f32)],
     "tvk_lod_mlp_destroy": [C.c_void_p],

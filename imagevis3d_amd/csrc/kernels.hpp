@@ -54,6 +54,11 @@ hipError_t launch_histogram_2d(const void* vol, int bpv, const int64_t dims[3],
 hipError_t launch_brick_histograms(const void* level, int bpv, const int64_t dims[3], int B, int bins,
                                    uint32_t* out, hipStream_t stream);
 
+// Mean |LOD-0 voxel - its level-l ancestor| per LOD-0 brick region of edge
+// B and level (out = n_regions x n_lod floats, level 0 = 0).
+hipError_t launch_region_detail(const void* const* levels, const int64_t (*dims)[3], int n_lod, int bpv,
+                                int B, float* out, hipStream_t stream);
+
 // Zero-fill a u64 buffer (a kernel, so the call is safe under graph capture).
 hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v,
                            hipStream_t stream);

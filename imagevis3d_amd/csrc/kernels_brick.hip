@@ -401,6 +401,70 @@ __global__ __launch_bounds__(kThreads) void brick_hist_kernel(const T* __restric
   for (int i = (int)threadIdx.x; i < bins; i += kThreads) out[(int64_t)b * bins + i] = h[i];
 }
 
+// Per LOD-0 brick region and level l >= 1: mean |v0(x) - v_l(x >> l)| over
+// the region's voxels (each LOD-0 voxel against its level-l ancestor), the
+// detail a region loses when it is rendered from level l.  One work-group
+// per region; out[region * n_lod + l], out[.. + 0] = 0.
+struct DetailLevels {
+  const void* level[16];
+  int64_t dims[16][3];
+};
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void region_detail_kernel(const DetailLevels D, int n_lod, int B,
+                                                                 int gx, int gy, float* __restrict__ out) {
+  __shared__ float red[kThreads / 64][16];
+  const int r = (int)blockIdx.x;
+  const int bx = r % gx, by = (r / gx) % gy, bz = r / (gx * gy);
+  const int64_t nx = D.dims[0][0], ny = D.dims[0][1], nz = D.dims[0][2];
+  const int64_t x0 = (int64_t)bx * B, y0 = (int64_t)by * B, z0 = (int64_t)bz * B;
+  const int w = (int)min<int64_t>(B, nx - x0), h = (int)min<int64_t>(B, ny - y0),
+            d = (int)min<int64_t>(B, nz - z0);
+  const int64_t cnt = (int64_t)w * h * d;
+  float acc[16];
+  for (int l = 0; l < 16; ++l) acc[l] = 0.f;
+  const T* v0 = (const T*)D.level[0];
+  for (int64_t i = threadIdx.x; i < cnt; i += kThreads) {
+    const int64_t x = x0 + i % w, y = y0 + (i / w) % h, z = z0 + i / ((int64_t)w * h);
+    const float a = (float)v0[(z * ny + y) * nx + x];
+    for (int l = 1; l < n_lod; ++l) {
+      const T* vl = (const T*)D.level[l];
+      const int64_t lx = x >> l, ly = y >> l, lz = z >> l;
+      acc[l] += fabsf(a - (float)vl[(lz * D.dims[l][1] + ly) * D.dims[l][0] + lx]);
+    }
+  }
+  const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+  for (int l = 1; l < n_lod; ++l) {
+    float v = acc[l];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[wave][l] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < (unsigned)n_lod) {
+    float s = 0.f;
+    if (threadIdx.x > 0)
+      for (int k = 0; k < kThreads / 64; ++k) s += red[k][threadIdx.x];
+    out[(int64_t)r * n_lod + threadIdx.x] = s / (float)max<int64_t>(cnt, 1);
+  }
+}
+
+hipError_t launch_region_detail(const void* const* levels, const int64_t (*dims)[3], int n_lod, int bpv,
+                                int B, float* out, hipStream_t stream) {
+  DetailLevels D{};
+  for (int l = 0; l < n_lod && l < 16; ++l) {
+    D.level[l] = levels[l];
+    for (int i = 0; i < 3; ++i) D.dims[l][i] = dims[l][i];
+  }
+  const int gx = (int)((dims[0][0] + B - 1) / B), gy = (int)((dims[0][1] + B - 1) / B),
+            gz = (int)((dims[0][2] + B - 1) / B);
+  const unsigned blocks = (unsigned)(gx * gy * gz);
+  if (bpv == 1)
+    region_detail_kernel<uint8_t><<<blocks, kThreads, 0, stream>>>(D, n_lod, B, gx, gy, out);
+  else
+    region_detail_kernel<uint16_t><<<blocks, kThreads, 0, stream>>>(D, n_lod, B, gx, gy, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_brick_histograms(const void* level, int bpv, const int64_t d[3], int B, int bins,
                                    uint32_t* out, hipStream_t stream) {
   const int gx = (int)((d[0] + B - 1) / B), gy = (int)((d[1] + B - 1) / B),

@@ -658,6 +658,27 @@ int HipRenderer::BrickHistograms(uint32_t lod, uint32_t bins, uint32_t* out, uin
   return TVK_OK;
 }
 
+int HipRenderer::RegionDetail(float* out, uint64_t n) {
+  if (!has_volume_) return state_fail("no volume loaded");
+  if (paged_) return state_fail("region detail needs an in-core volume");
+  const LodInfo& L = lods_[0];
+  const uint64_t nr = (uint64_t)L.grid[0] * L.grid[1] * L.grid[2];
+  if (!out || n < nr * n_lod_) return arg_fail("output needs n_regions x n_lod entries");
+  TVK_HIP(hipSetDevice(device_));
+  DevBuf d;
+  TVK_HIP(d.alloc(nr * n_lod_ * 4));
+  const void* lv[kMaxLod];
+  int64_t dims[kMaxLod][3];
+  for (int l = 0; l < n_lod_; ++l) {
+    lv[l] = levels_[l].p;
+    for (int i = 0; i < 3; ++i) dims[l][i] = lods_[l].dims[i];
+  }
+  TVK_HIP(launch_region_detail(lv, dims, n_lod_, (int)bits_ / 8, brick_, d.as<float>(), stream_));
+  TVK_HIP(hipMemcpyAsync(out, d.p, nr * n_lod_ * 4, hipMemcpyDeviceToHost, stream_));
+  TVK_HIP(hipStreamSynchronize(stream_));
+  return TVK_OK;
+}
+
 int HipRenderer::Histogram1D(uint64_t* hist, uint64_t n_bins) {
   if (!has_volume_) return state_fail("no volume loaded");
   const uint64_t nb = 1ull << bits_;

@@ -157,6 +157,7 @@ class HipRenderer {
   int BrickMinMax(uint32_t lod, float* mins, float* maxs, uint64_t n);
   int Histogram1D(uint64_t* hist, uint64_t n_bins);
   int BrickHistograms(uint32_t lod, uint32_t bins, uint32_t* out, uint64_t n);
+  int RegionDetail(float* out, uint64_t n);
   int Histogram2D(uint64_t* hist, uint32_t n_value, uint32_t n_grad, float* max_grad);
 
   int SetRenderMode(int m);

@@ -438,6 +438,11 @@ int tvk_brick_histograms(tvk_renderer* r, uint32_t lod, uint32_t bins, uint32_t*
   return r->impl.BrickHistograms(lod, bins, out, n);
 }
 
+int tvk_region_detail(tvk_renderer* r, float* out, uint64_t n) {
+  TVK_GUARD(r);
+  return r->impl.RegionDetail(out, n);
+}
+
 int tvk_set_region_lods(tvk_renderer* r, const uint8_t* levels, uint64_t n) {
   TVK_GUARD(r);
   TVK_TRY(r->impl.SetRegionLods(levels, n));

@@ -5,13 +5,16 @@ made per brick and learned).
 A bf16 MLP (32 inputs -> 4 x 64 ReLU -> 1, evaluated on MFMA by
 ``tvk_lod_mlp_predict``) predicts, for every LOD-0 brick region and every
 coarser level, the mean squared RGB error per screen pixel that region
-causes when it is rendered from that level.  Inputs (27 used, padded to 32):
+causes when it is rendered from that level.  Inputs (29 used, padded to 32):
 
     level / 8, log2(1 + projected footprint in pixels) / 24,
     the region's 16-bin value histogram (fractions),
     8 TF moments over that histogram (mean alpha, mean alpha^2, alpha-
     weighted r, g, b, max alpha, alpha variance, alpha roughness),
-    view distance (world units).
+    view distance (world units),
+    and (slots 27-28) the region's detail at that level -- mean |LOD-0 voxel
+    - level-l ancestor| (tvk_region_detail) -- plain and times the region's
+    maximum TF alpha.
 
 Training data come from the renderer itself: for each (view, TF) the frame
 is rendered with every region at level 0 (recording per pixel the region
@@ -105,7 +108,7 @@ def tf_moments(hist_frac: np.ndarray, tf: np.ndarray) -> np.ndarray:
     return np.stack([ma, m2, cr, cg, cb, amax, var, rough], 1)
 
 
-def features(level: int, area, dist, hist_frac, moments) -> np.ndarray:
+def features(level: int, area, dist, hist_frac, moments, detail=None) -> np.ndarray:
     n = len(area)
     x = np.zeros((n, N_IN), np.float32)
     x[:, 0] = level / 8.0
@@ -113,6 +116,10 @@ def features(level: int, area, dist, hist_frac, moments) -> np.ndarray:
     x[:, 2:18] = hist_frac
     x[:, 18:26] = moments
     x[:, 26] = dist
+    if detail is not None:
+        # the region's detail lost at this level, plain and opacity-weighted
+        x[:, 27] = detail[:, level]
+        x[:, 28] = detail[:, level] * moments[:, 5]
     return x
 
 
@@ -121,8 +128,14 @@ def region_histograms(renderer) -> np.ndarray:
     return h / np.maximum(h.sum(1, keepdims=True), 1.0)
 
 
+def region_detail(renderer) -> np.ndarray:
+    """(n_regions, n_lod) detail per level, in units of the value range."""
+    bits = renderer._bit_width()
+    return renderer.regionDetail() * (16.0 / ((1 << bits) - 1))
+
+
 # --------------------------------------------------------- training data
-def training_rows(renderer, dims, edge, n_lod, views, tfs, hist_frac):
+def training_rows(renderer, dims, edge, n_lod, views, tfs, hist_frac, detail=None):
     """Render every (view, TF) at all-level-0 and all-level-l and attribute
     the per-pixel squared RGB error of level l to the dominant region.
     Returns (inputs, log targets)."""
@@ -148,7 +161,7 @@ def training_rows(renderer, dims, edge, n_lod, views, tfs, hist_frac):
                 e = ((img[..., :3] - base[..., :3]) ** 2).sum(-1).ravel()
                 sse = np.bincount(ids[ids >= 0], weights=e[ids >= 0], minlength=n_reg)
                 mse = sse / np.maximum(cnt, 1)
-                X.append(features(lev, area, dist, hist_frac, mom)[seen])
+                X.append(features(lev, area, dist, hist_frac, mom, detail)[seen])
                 Y.append(np.log(MSE_FLOOR + mse[seen]).astype(np.float32))
     renderer.setRegionOutput(False)
     renderer.setRegionLODs(None)
@@ -409,12 +422,12 @@ def full_resolution(renderer, dims, edge) -> np.ndarray:
 
 
 def choose_levels(pred, view, dims, edge, n_lod, hist, moments, safety: float,
-                  brick_bytes: int = 0, mem_budget: int = 0):
+                  brick_bytes: int = 0, mem_budget: int = 0, detail=None):
     """Features -> MLP -> selection for one view.  The SSE budget is
     safety x (1/255)^2 x 4 x W x H (an RGBA RMSE of sqrt(safety)/255 if the
     predictions were exact)."""
     area, dist = region_geometry(dims, edge, view)
-    x = np.concatenate([features(l, area, dist, hist, moments) for l in range(1, n_lod)])
+    x = np.concatenate([features(l, area, dist, hist, moments, detail) for l in range(1, n_lod)])
     y = pred(x).reshape(n_lod - 1, -1)
     budget = safety * (1.0 / 255.0) ** 2 * 4 * view["width"] * view["height"]
     grid = tuple(int(v) for v in np.ceil(np.asarray(dims) / edge))
@@ -422,7 +435,7 @@ def choose_levels(pred, view, dims, edge, n_lod, hist, moments, safety: float,
 
 
 def calibrate_safety(renderer, pred, views, dims, edge, n_lod, hist, moments, target: float = 1.0,
-                     brick_bytes: int = 0, mem_budget: int = 0):
+                     brick_bytes: int = 0, mem_budget: int = 0, detail=None):
     """Largest budget factor (halving from 4) whose learned frames stay within
     `target` RMSE of full resolution on every calibration view."""
     refs = []
@@ -437,7 +450,7 @@ def calibrate_safety(renderer, pred, views, dims, edge, n_lod, hist, moments, ta
             renderer.resize((v["width"], v["height"]))
             set_view(renderer, v)
             lev, _, _ = choose_levels(pred, v, dims, edge, n_lod, hist, moments, safety,
-                                      brick_bytes, mem_budget)
+                                      brick_bytes, mem_budget, detail)
             renderer.setRegionLODs(lev)
             worst = max(worst, rmse255(renderer.capture(), ref))
             renderer.setRegionLODs(None)

@@ -460,6 +460,14 @@ class Renderer:
         call("tvk_brick_histograms", self._h, int(lod), int(bins), out.ctypes.data, out.size)
         return out
 
+    def regionDetail(self) -> np.ndarray:
+        """(n_regions, n_lod) mean |LOD-0 voxel - level-l ancestor| per region."""
+        grid, _, _ = self.getDataset().getBrickLayout(0)
+        nl = self.getDataset().getLODLevelCount()
+        out = np.empty((grid[0] * grid[1] * grid[2], nl), np.float32)
+        call("tvk_region_detail", self._h, out.ctypes.data, out.size)
+        return out
+
     def setRegionOutput(self, on: bool) -> None:
         call("tvk_set_region_output", self._h, int(bool(on)))
 

@@ -360,6 +360,12 @@ int tvk_read_region_output(tvk_renderer* r, int32_t* out, uint64_t n);
  * page-table order (the learned-LOD features; in-core volumes). */
 int tvk_brick_histograms(tvk_renderer* r, uint32_t lod, uint32_t bins, uint32_t* out, uint64_t n);
 
+/* Per LOD-0 brick region and LOD level l: mean |v0 - v_l| between each
+ * LOD-0 voxel and its level-l ancestor (x >> l, y >> l, z >> l) -- the
+ * detail a region loses at level l (learned-LOD feature; in-core volumes).
+ * out: n_regions x n_lod floats (level 0 = 0), regions x fastest. */
+int tvk_region_detail(tvk_renderer* r, float* out, uint64_t n);
+
 /* Learned-LOD error predictor (no counterpart in the reference): a bf16 MLP
  * 32 inputs -> 4 hidden layers of 64 (ReLU) -> 1 output evaluated on the
  * MFMA units with FP32 accumulation.  weights_bf16: layer-major, each layer

@@ -64,8 +64,12 @@ def test_pipeline_small():
     r.setGeneratedVolume(VOL_NOISE_TUBES, 4, (n, n, n), 8, edge, 2, n_lod)
     r.setSampleRateModifier(2.0)
     hist = LL.region_histograms(r)
+    detail = LL.region_detail(r)
+    assert detail.shape == (512, n_lod) and (detail[:, 0] == 0).all()
+    assert (detail[:, 1:].mean(0) > 0).all()
     tfs = [transfer.tubes_tf(256, seed=s) for s in range(2)]
-    X, Y = LL.training_rows(r, (n, n, n), edge, n_lod, random_views(12, 1, 160, 90), tfs, hist)
+    X, Y = LL.training_rows(r, (n, n, n), edge, n_lod, random_views(12, 1, 160, 90), tfs, hist,
+                            detail)
     assert len(X) > 2000
     wbits, b, loss = LL.train(X, Y, steps=2000)
     print(f"rows {len(X)} loss {loss:.3f}")
@@ -75,7 +79,7 @@ def test_pipeline_small():
     mom = LL.tf_moments(hist, tf)
     dims = (n, n, n)
     safety, worst = LL.calibrate_safety(r, pred, random_views(3, 3, 320, 180), dims, edge, n_lod,
-                                        hist, mom)
+                                        hist, mom, detail=detail)
     print(f"calibrated budget factor {safety} (worst calibration rmse {worst:.3f}/255)")
     assert safety > 0
     rm = []
@@ -84,7 +88,8 @@ def test_pipeline_small():
         LL.set_view(r, view)
         full = LL.full_resolution(r, dims, edge)
         st_full = r.frameStats()
-        lev, sse, mem = LL.choose_levels(pred, view, dims, edge, n_lod, hist, mom, safety)
+        lev, sse, mem = LL.choose_levels(pred, view, dims, edge, n_lod, hist, mom, safety,
+                                         detail=detail)
         r.setRegionLODs(lev)
         img = r.capture()
         st = r.frameStats()

@@ -45,7 +45,8 @@ def cpu_baseline(vol, w, cams, threads: int) -> dict:
     return {"value": nominal / secs / 1e9, "unit": "Gsamples/s", "cores": threads,
             "kind": "port",
             "sample": f"{len(cams)} of the 36 orbit views at 1920x1080 on the 512^3 volume, "
-                      f"LOD 0, {secs:.1f} s of CPU time (FP32 OpenMP raycaster, same model)",
+                      f"LOD 0, {secs:.1f} s wall on {threads} threads (FP32 OpenMP raycaster, "
+                      f"same model and empty-space cells)",
             "fps": len(cams) / secs}
 
 
@@ -236,9 +237,10 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
     bf16 MLP is trained on the renderer's own fine-vs-coarse renders (32
     views x 4 TFs at 480x270), its selection budget calibrated on 4 views,
     then 24 new random views at 3840x2160 with a new TF are rendered at full
-    resolution and with the learned per-region LOD (the brick budget is a
-    quarter of the LOD-0 bricks, as 16 GB of a 64 GiB volume).  Frame time
-    of the learned path includes features, MLP and selection."""
+    resolution and with the learned per-region LOD at two operating points:
+    the error target alone ("target") and under a brick budget of a quarter
+    of the LOD-0 bricks, as 16 GB of a 64 GiB volume ("budgeted").  Frame
+    time of the learned path includes features, MLP and selection."""
     import math
 
     import numpy as np
@@ -283,10 +285,15 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
     slot_bytes = (s_slot ** 3 + 255) // 256 * 256
     n_reg = (n // edge) ** 3
     mem_budget = n_reg * slot_bytes // 4
-    safety, worst = LL.calibrate_safety(r, pred, views(4, 12, *eval_size), dims, edge, n_lod, hist,
-                                        mom, 1.0, slot_bytes, mem_budget, detail)
-    full_ms, learned_ms, learned_kernel_ms, sel_ms, rm, rm_uniform, mem = [], [], [], [], [], [], []
-    s_full = s_learned = 0
+    cal_views = views(4, 12, *eval_size)
+    # two operating points: under the brick budget, and with the error
+    # target alone (no brick cap)
+    points = {"budgeted": mem_budget, "target": 0}
+    cal = {k: LL.calibrate_safety(r, pred, cal_views, dims, edge, n_lod, hist, mom, 1.0,
+                                  slot_bytes, mb, detail) for k, mb in points.items()}
+    full_ms, s_full = [], 0
+    acc = {k: {"ms": [], "kernel_ms": [], "sel_ms": [], "rmse": [], "rmse_uniform": [], "mem": [],
+               "samples": 0} for k in points}
     for v in views(eval_views, 13, *eval_size):
         r.resize(eval_size)
         LL.set_view(r, v)
@@ -297,35 +304,47 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
         s_full += st.evaluated_samples
         uni = r.capture()
         ref = LL.full_resolution(r, dims, edge)
-        t0 = time.perf_counter()
-        lev, _, m = LL.choose_levels(pred, v, dims, edge, n_lod, hist, mom, safety, slot_bytes,
-                                     mem_budget, detail)
-        r.setRegionLODs(lev)
-        sel = (time.perf_counter() - t0) * 1e3
-        assert r.paint(), r.lastError()
-        st = r.frameStats()
-        s_learned += st.evaluated_samples
-        img = r.capture()
-        sel_ms.append(sel)
-        learned_kernel_ms.append(st.kernel_ms)
-        learned_ms.append(st.kernel_ms + sel)
-        rm.append(LL.rmse255(img, ref))
-        rm_uniform.append(LL.rmse255(img, uni))
-        mem.append(m)
-        r.setRegionLODs(None)
+        for k, mb in points.items():
+            a = acc[k]
+            t0 = time.perf_counter()
+            lev, _, m = LL.choose_levels(pred, v, dims, edge, n_lod, hist, mom, cal[k][0],
+                                         slot_bytes, mb, detail)
+            r.setRegionLODs(lev)
+            sel = (time.perf_counter() - t0) * 1e3
+            assert r.paint(), r.lastError()
+            st = r.frameStats()
+            a["samples"] += st.evaluated_samples
+            img = r.capture()
+            a["sel_ms"].append(sel)
+            a["kernel_ms"].append(st.kernel_ms)
+            a["ms"].append(st.kernel_ms + sel)
+            a["rmse"].append(LL.rmse255(img, ref))
+            a["rmse_uniform"].append(LL.rmse255(img, uni))
+            a["mem"].append(m)
+            r.setRegionLODs(None)
+
+    def summary(k):
+        a = acc[k]
+        return {"fps_learned": 1e3 / float(np.mean(a["ms"])),
+                "speedup": float(np.mean(full_ms) / np.mean(a["ms"])),
+                "kernel_speedup": float(np.mean(full_ms) / np.mean(a["kernel_ms"])),
+                "rmse_mean": float(np.mean(a["rmse"])), "rmse_max": float(np.max(a["rmse"])),
+                "rmse_vs_global_sampling_mean": float(np.mean(a["rmse_uniform"])),
+                "evaluated_sample_ratio": a["samples"] / max(s_full, 1),
+                "selection_ms_mean": float(np.mean(a["sel_ms"])),
+                "brick_bytes_mean": float(np.mean(a["mem"])),
+                "brick_budget": points[k] or None,
+                "budget_factor": cal[k][0], "calibration_worst_rmse": cal[k][1]}
+
     out = {
         "workload": f"learned-lod-{n}^3-u8-noise-tubes-{edge}^3-bricks-{n_lod}lods-"
                     f"{eval_size[0]}x{eval_size[1]}-{eval_views}views",
         "fps_full": 1e3 / float(np.mean(full_ms)),
-        "fps_learned": 1e3 / float(np.mean(learned_ms)),
-        "speedup": float(np.mean(full_ms) / np.mean(learned_ms)),
-        "rmse_mean": float(np.mean(rm)), "rmse_max": float(np.max(rm)), "rmse_target": 1.0,
-        "rmse_vs_global_sampling_mean": float(np.mean(rm_uniform)),
-        "evaluated_sample_ratio": s_learned / max(s_full, 1),
-        "selection_ms_mean": float(np.mean(sel_ms)), "mlp_kernel_ms": pred.kernel_ms,
-        "brick_bytes_mean": float(np.mean(mem)), "brick_budget": mem_budget,
-        "train_rows": int(len(X)), "train_loss": loss, "budget_factor": safety,
-        "calibration_worst_rmse": worst,
+        "rmse_target": 1.0,
+        "target": summary("target"),
+        "budgeted": summary("budgeted"),
+        "mlp_kernel_ms": pred.kernel_ms,
+        "train_rows": int(len(X)), "train_loss": loss,
         "seconds": {"volume": round(setup_s, 2), "training_data": round(data_s, 2),
                     "training": round(train_s, 2)},
     }
@@ -342,7 +361,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=36)
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-views", type=int, default=3)
+    ap.add_argument("--cpu-views", type=int, default=36)
     ap.add_argument("--batch", type=int, default=8, help="samples per march step")
     ap.add_argument("--tile-group", type=int, default=4, help="ray tiles per XCD turn")
     ap.add_argument("--xcd-split", type=int, default=0, help="0 or frame regions per row")
@@ -476,7 +495,8 @@ def main() -> int:
         print("[bench] learned-LOD workload", file=sys.stderr, flush=True)
         ll = run_learned_lod(args.learned_lod_size)
         out["learned_lod"] = ll
-        out["learned_lod_fps"] = ll["fps_learned"]
+        # FPS of the operating point held to the 1/255 RMSE target
+        out["learned_lod_fps"] = ll["target"]["fps_learned"]
         print(f"[bench] learned LOD {ll}", file=sys.stderr, flush=True)
     if not args.no_batched:
         print("[bench] out-of-core batched workload", file=sys.stderr, flush=True)

@@ -159,6 +159,9 @@ SIGNATURES = {
     "tvk_lod_mlp_create": [i32, P, P, C.POINTER(C.c_void_p)],
     "tvk_lod_mlp_predict": [C.c_void_p, P, u64, P, C.POINTER(f32)],
     "tvk_lod_mlp_destroy": [C.c_void_p],
+    "tvk_lod_features": [C.POINTER(View), P, u32, u32, P, P, P, P, P, P],
+    "tvk_lod_select_octree": [P, P, P, u32, C.c_double, u64, u64, P, C.POINTER(C.c_double),
+                              C.POINTER(u64)],
     "tvk_set_region_output": [H, i32],
     "tvk_read_region_output": [H, P, u64],
     "tvk_autotune": [H, C.POINTER(View), u32, u32, C.POINTER(Tuning)],

@@ -24,6 +24,7 @@ SOURCES = [
     "paging.cpp",
     "autotune.cpp",
     "server.cpp",
+    "lod_select.cpp",
     "host_volume.cpp",
     "uvf.cpp",
     "volume_gen.cpp",

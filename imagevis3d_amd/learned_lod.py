@@ -123,6 +123,54 @@ def features(level: int, area, dist, hist_frac, moments, detail=None) -> np.ndar
     return x
 
 
+def plan_features(view, dims, edge, n_lod, hist_frac, moments, detail=None, out=None):
+    """Predictor inputs of every region at levels 1..n_lod-1 (level-major
+    rows, 32 columns as `features`), footprints and distances, computed by
+    the library (tvk_lod_features) -- the path the renderer plans with.
+    `region_geometry` and `features` are the numpy statement of the same.
+    `out`, if given, is a float32 array of the inputs' shape to fill."""
+    from .renderer import Renderer
+    g = np.ceil(np.asarray(dims, np.float64) / edge).astype(np.int64)
+    n_reg = int(np.prod(g))
+    arr, _ = Renderer._views([view], False)
+    d = (C.c_uint64 * 3)(*[int(v) for v in dims])
+    hist = np.ascontiguousarray(hist_frac, np.float64)
+    mom = np.ascontiguousarray(moments, np.float64)
+    det = None if detail is None else np.ascontiguousarray(detail, np.float64)
+    if hist.shape != (n_reg, 16) or mom.shape != (n_reg, 8):
+        raise ValueError("hist_frac must be (regions, 16) and moments (regions, 8)")
+    if det is not None and det.shape != (n_reg, n_lod):
+        raise ValueError("detail must be (regions, n_lod)")
+    shape = ((n_lod - 1) * n_reg, N_IN)
+    if out is not None and (out.shape != shape or out.dtype != np.float32
+                            or not out.flags.c_contiguous):
+        raise ValueError(f"out must be a C-contiguous float32 array of shape {shape}")
+    x = np.empty(shape, np.float32) if out is None else out
+    area = np.empty(n_reg, np.float64)
+    dist = np.empty(n_reg, np.float64)
+    call("tvk_lod_features", arr, d, edge, n_lod, hist.ctypes.data, mom.ctypes.data,
+         None if det is None else det.ctypes.data, x.ctypes.data, area.ctypes.data,
+         dist.ctypes.data)
+    return x, area, dist
+
+
+def plan_octree(pred_log_mse: np.ndarray, area: np.ndarray, budget_sse: float, grid,
+                brick_bytes: int = 1, mem_budget: int = 0):
+    """`select_octree` computed by the library (tvk_lod_select_octree)."""
+    pred = np.ascontiguousarray(pred_log_mse, np.float32)
+    area = np.ascontiguousarray(area, np.float64)
+    gx, gy, gz = (int(v) for v in grid)
+    if pred.shape != (pred.shape[0], gx * gy * gz) or area.shape != (gx * gy * gz,):
+        raise ValueError("pred must be (levels - 1, regions) and area (regions,)")
+    lev = np.empty(gx * gy * gz, np.uint8)
+    g = (C.c_uint32 * 3)(gx, gy, gz)
+    sse, nbytes = C.c_double(), C.c_uint64()
+    call("tvk_lod_select_octree", pred.ctypes.data, area.ctypes.data, g, pred.shape[0] + 1,
+         float(budget_sse), int(brick_bytes), int(mem_budget), lev.ctypes.data, C.byref(sse),
+         C.byref(nbytes))
+    return lev, float(sse.value), int(nbytes.value)
+
+
 def region_histograms(renderer) -> np.ndarray:
     h = renderer.brickHistograms(0, 16).astype(np.float64)
     return h / np.maximum(h.sum(1, keepdims=True), 1.0)
@@ -146,10 +194,11 @@ def training_rows(renderer, dims, edge, n_lod, views, tfs, hist_frac, detail=Non
     for view in views:
         renderer.resize((view["width"], view["height"]))
         set_view(renderer, view)
-        area, dist = region_geometry(dims, edge, view)
         for tf in tfs:
             renderer.set1DTrans(tf)
             mom = tf_moments(hist_frac, tf)
+            xs = plan_features(view, dims, edge, n_lod, hist_frac, mom, detail)[0]
+            xs = xs.reshape(n_lod - 1, n_reg, N_IN)
             renderer.setRegionLODs(np.zeros(n_reg, np.uint8))
             base = renderer.capture().astype(np.float64) / 255.0
             ids = renderer.readRegionOutput().ravel()
@@ -161,7 +210,7 @@ def training_rows(renderer, dims, edge, n_lod, views, tfs, hist_frac, detail=Non
                 e = ((img[..., :3] - base[..., :3]) ** 2).sum(-1).ravel()
                 sse = np.bincount(ids[ids >= 0], weights=e[ids >= 0], minlength=n_reg)
                 mse = sse / np.maximum(cnt, 1)
-                X.append(features(lev, area, dist, hist_frac, mom, detail)[seen])
+                X.append(xs[lev - 1][seen])
                 Y.append(np.log(MSE_FLOOR + mse[seen]).astype(np.float32))
     renderer.setRegionOutput(False)
     renderer.setRegionLODs(None)
@@ -421,17 +470,25 @@ def full_resolution(renderer, dims, edge) -> np.ndarray:
     return img
 
 
+_PLAN_BUF: dict = {}  # the predictor-input buffer of choose_levels, reused per shape
+
+
 def choose_levels(pred, view, dims, edge, n_lod, hist, moments, safety: float,
                   brick_bytes: int = 0, mem_budget: int = 0, detail=None):
     """Features -> MLP -> selection for one view.  The SSE budget is
     safety x (1/255)^2 x 4 x W x H (an RGBA RMSE of sqrt(safety)/255 if the
     predictions were exact)."""
-    area, dist = region_geometry(dims, edge, view)
-    x = np.concatenate([features(l, area, dist, hist, moments, detail) for l in range(1, n_lod)])
+    n_reg = int(np.prod(np.ceil(np.asarray(dims, np.float64) / edge)))
+    shape = ((n_lod - 1) * n_reg, N_IN)
+    buf = _PLAN_BUF.get(shape)
+    if buf is None:
+        _PLAN_BUF.clear()
+        buf = _PLAN_BUF[shape] = np.empty(shape, np.float32)
+    x, area, _ = plan_features(view, dims, edge, n_lod, hist, moments, detail, out=buf)
     y = pred(x).reshape(n_lod - 1, -1)
     budget = safety * (1.0 / 255.0) ** 2 * 4 * view["width"] * view["height"]
     grid = tuple(int(v) for v in np.ceil(np.asarray(dims) / edge))
-    return select_octree(y, area, budget, grid, max(brick_bytes, 1), mem_budget)
+    return plan_octree(y, area, budget, grid, max(brick_bytes, 1), mem_budget)
 
 
 def calibrate_safety(renderer, pred, views, dims, edge, n_lod, hist, moments, target: float = 1.0,

@@ -380,6 +380,37 @@ int tvk_lod_mlp_predict(tvk_lod_mlp* m, const float* inputs, uint64_t n, float* 
                         float* kernel_ms);
 int tvk_lod_mlp_destroy(tvk_lod_mlp* m);
 
+/* Learned-LOD frame planning on the host (Tuvok's per-frame LOD choice,
+ * RenderWindow.cpp:447-451, made per LOD-0 brick region).  Both run on
+ * host threads and touch no device.
+ *
+ * tvk_lod_features: per LOD-0 region r (x fastest over the brick grid
+ * ceil(dims / edge)) its projected footprint in pixels (screen box of the
+ * region's corners, clipped to the view; 0 behind the camera) and centre
+ * distance from the eye, then the predictor's 32 inputs for every coarser
+ * level l = 1 .. n_lod-1 into x[((l-1) * R + r) * 32 + k]:
+ *   l/8, log2(1 + footprint)/24, hist[r][0..16), moments[r][0..8),
+ *   distance, detail[r][l], detail[r][l] * moments[r][5], zeros.
+ * detail may be NULL (inputs 27-28 zero); dist_out may be NULL.  The view
+ * is perspective (ortho views are rejected); world box centred at 0 with
+ * largest extent 1. */
+int tvk_lod_features(const tvk_view* view, const uint64_t dims[3], uint32_t edge, uint32_t n_lod,
+                     const double* hist16, const double* moments8, const double* detail,
+                     float* x, double* area_out, double* dist_out);
+
+/* tvk_lod_select_octree: octree cut of the brick pyramid from predicted
+ * log mean-squared errors pred[(l-1) * R + r] (levels 1 .. n_levels-1):
+ * start from the coarsest level's bricks and repeatedly split the brick
+ * whose children cut the predicted SSE (sum over its regions of
+ * exp(pred) x footprint) the most per added brick, ties to the lower
+ * (level, z, y, x), until the SSE fits budget_sse or the next split would
+ * pass mem_budget bytes at brick_bytes per brick (0: no cap).  Writes one
+ * level per region (x fastest) and the predicted SSE and brick bytes. */
+int tvk_lod_select_octree(const float* pred, const double* area, const uint32_t grid[3],
+                          uint32_t n_levels, double budget_sse, uint64_t brick_bytes,
+                          uint64_t mem_budget, uint8_t* levels, double* sse_out,
+                          uint64_t* bytes_out);
+
 /* ------------------------------------------------------------ autotuner */
 /* Launch autotuner (no counterpart in the reference).  Times the given
  * views (tvk_render_batch, images stay on the device) under candidate

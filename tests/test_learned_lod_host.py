@@ -64,3 +64,54 @@ def test_bf16_round_trip_and_features():
     assert m[0, 0] == 0.0 and np.isclose(m[1, 0], 0.5)
     x = LL.features(2, area[:2], dist[:2], h, m)
     assert x.shape == (2, 32) and (x[:, 27:] == 0).all()
+
+
+def _planner_case(seed, dims, edge, n_lod):
+    rng = np.random.default_rng(seed)
+    g = tuple(int(v) for v in np.ceil(np.asarray(dims) / edge))
+    n_reg = int(np.prod(g))
+    hist = rng.dirichlet(np.ones(16), n_reg)
+    tf = np.zeros((256, 4), np.float32)
+    tf[:, 3] = np.clip((np.arange(256) - 90) / 120.0, 0, 1)
+    tf[:, :3] = rng.uniform(0, 1, 3)
+    mom = LL.tf_moments(hist, tf)
+    detail = rng.uniform(0, 0.2, (n_reg, n_lod))
+    # coordinates exact in float32 (the library's view struct holds floats)
+    view = dict(view_pos=(1.25, 0.5, -0.75), view_dir=(-1.25, -0.5, 0.75), up_dir=(0.0, 1.0, 0.0),
+                fov=45.0, width=640, height=360)
+    return g, n_reg, hist, mom, detail, view
+
+
+def test_native_features_match_numpy_statement():
+    """tvk_lod_features against region_geometry + features, with a ragged
+    grid (dims not a multiple of the brick edge) and a view that puts some
+    regions behind the camera."""
+    for dims, edge, n_lod in (((96, 80, 72), 16, 4), ((128, 128, 128), 32, 3)):
+        g, n_reg, hist, mom, detail, view = _planner_case(1, dims, edge, n_lod)
+        for v in (view, dict(view, view_pos=(0.125, 0.0, 0.25))):
+            x, area, dist = LL.plan_features(v, dims, edge, n_lod, hist, mom, detail)
+            a_ref, d_ref = LL.region_geometry(dims, edge, v)
+            np.testing.assert_allclose(area, a_ref, rtol=1e-12, atol=1e-9)
+            np.testing.assert_allclose(dist, d_ref, rtol=1e-12)
+            ref = np.concatenate([LL.features(l, a_ref, d_ref, hist, mom, detail)
+                                  for l in range(1, n_lod)])
+            np.testing.assert_allclose(x, ref, rtol=1e-6, atol=1e-7)
+            assert (area == 0).any() or v is view
+        x0, _, _ = LL.plan_features(view, dims, edge, n_lod, hist, mom, None)
+        assert (x0[:, 27:] == 0).all()
+
+
+def test_native_octree_cut_matches_numpy_statement():
+    """tvk_lod_select_octree makes the same cut as select_octree on random
+    predictions, with and without error and brick budgets."""
+    rng = np.random.default_rng(3)
+    for g, L in (((8, 8, 8), 4), ((6, 5, 7), 4), ((16, 16, 16), 5)):
+        n = int(np.prod(g))
+        y = np.log(rng.uniform(1e-6, 1e-3, (L - 1, n)) * np.arange(1, L)[:, None]).astype(np.float32)
+        area = rng.uniform(0, 400, n) * (rng.uniform(size=n) > 0.2)
+        for budget, bb, mb in ((1e9, 1, 0), (2.0, 1, 0), (0.0, 1, 0), (0.0, 1000, 1000 * n // 5),
+                               (0.5, 7, 7 * n // 3)):
+            lev, sse, mem = LL.plan_octree(y, area, budget, g, bb, mb)
+            lr, sr, mr = LL.select_octree(y, area, budget, g, bb, mb)
+            assert np.array_equal(lev, lr), (g, budget, mb)
+            assert mem == mr and np.isclose(sse, sr, rtol=1e-9, atol=1e-12)

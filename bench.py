@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Gsamples/s + FPS @1080p 512³u16; learned-LOD FPS at ≤1/255 RMSE; p99 ms"
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "raycast_medium_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "raycast_medium_orbit_batch_traffic.json")
 
 
 def cpu_baseline(vol, w, cams, threads: int) -> dict:
@@ -64,6 +64,34 @@ def calibrate_copy(nbytes: int = 1 << 30) -> int:
     print(json.dumps({"calibration": "copy", "bytes_read": nbytes, "bytes_written": nbytes,
                       "dispatches": 3}))
     return 0
+
+
+def run_interactive(r, w, views, tune=None, warmup: int = 4) -> dict:
+    """Sequential interactive frames on the medium workload: the 36 orbit
+    cameras painted one per frame (paint + frame stats), after retuning the
+    launch shape for single-view launches."""
+    from imagevis3d_amd import workloads
+    tuning = tune(views[4:5], "single-view paints") if tune else None
+    cams = w.cameras
+    for i in range(warmup):
+        workloads.set_camera(r, cams[i % len(cams)])
+        assert r.paint(), r.lastError()
+        r.frameStats()
+    r.synchronize()
+    kernel_ms, nominal = [], 0
+    t0 = time.perf_counter()
+    for cam in cams:
+        workloads.set_camera(r, cam)
+        if not r.paint():
+            raise RuntimeError(r.lastError())
+        st = r.frameStats()  # waits for this frame's events and counters
+        kernel_ms.append(st.kernel_ms)
+        nominal += st.nominal_samples
+    r.synchronize()
+    wall = time.perf_counter() - t0
+    return {"frames": len(cams), "fps": len(cams) / wall, "ms_per_frame": wall / len(cams) * 1e3,
+            "kernel_ms_per_frame": sum(kernel_ms) / len(cams), "gsamples_per_s": nominal / wall / 1e9,
+            "launch": tuning}
 
 
 def run_serving(duration_s: float, rate_hz: float, seed: int = 5) -> dict:
@@ -247,7 +275,7 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
 
     from imagevis3d_amd import learned_lod as LL
     from imagevis3d_amd import transfer, tuvok
-    from imagevis3d_amd.renderer import VOL_NOISE_TUBES
+    from imagevis3d_amd.renderer import VOL_NOISE_TUBES, brick_slot_bytes
 
     def views(k, seed, w, h):
         rng = np.random.default_rng(seed)
@@ -281,8 +309,7 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
     tf = transfer.tubes_tf(256, seed=9)
     r.set1DTrans(tf)
     mom = LL.tf_moments(hist, tf)
-    s_slot = edge + 4
-    slot_bytes = (s_slot ** 3 + 255) // 256 * 256
+    slot_bytes = brick_slot_bytes(edge, 2, 8)
     n_reg = (n // edge) ** 3
     mem_budget = n_reg * slot_bytes // 4
     cal_views = views(4, 12, *eval_size)
@@ -358,8 +385,11 @@ def main() -> int:
         return calibrate_copy()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=36)
-    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed steps; a step renders the 36-view orbit in one batched launch")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-interactive", action="store_true",
+                    help="skip the sequential one-view-per-paint leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-views", type=int, default=36)
     ap.add_argument("--batch", type=int, default=8, help="samples per march step")
@@ -378,7 +408,7 @@ def main() -> int:
     ap.add_argument("--batched-only", action="store_true")
     ap.add_argument("--no-learned-lod", action="store_true", help="skip the learned-LOD workload")
     ap.add_argument("--learned-lod-only", action="store_true")
-    ap.add_argument("--learned-lod-size", type=int, default=2048)
+    ap.add_argument("--learned-lod-size", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
@@ -415,52 +445,52 @@ def main() -> int:
     workloads.apply(r, w, w.cameras[0])
     if args.no_skip:
         r.setEmptySpaceSkipping(False)
-    tuning = None
-    if not args.no_autotune:
-        # empty-space cell and launch shape for this workload (the brick
-        # size stays the configured 256^3), timed on 4 orbit views at full
-        # size (untimed setup)
-        tv = [dict(view_pos=c.view_pos, view_dir=c.view_dir, up_dir=c.up_dir, fov=c.fov,
-                   width=w.width, height=w.height) for c in w.cameras[::9]]
+    cams = w.cameras
+    views = [dict(view_pos=c.view_pos, view_dir=c.view_dir, up_dir=c.up_dir, fov=c.fov,
+                  width=w.width, height=w.height) for c in cams]
+
+    def tune(tv, what):
+        # empty-space cell and launch shape (the brick size stays the
+        # configured 256^3), timed on the given views (untimed setup)
         t0 = time.perf_counter()
         t = r.autotune(tv, flags=r.TUNE_CELL | r.TUNE_LAUNCH)
-        tuning = {"brick": t.brick_size, "cell": t.cell, "samples_per_batch": t.samples_per_batch,
-                  "tile_width": t.tile_width, "tile_group": t.tile_group,
-                  "tuned_ms": round(t.kernel_ms, 4), "start_ms": round(t.baseline_ms, 4),
-                  "settings_timed": t.candidates, "seconds": round(time.perf_counter() - t0, 2)}
-        print(f"[bench] autotuned {tuning}", file=sys.stderr, flush=True)
+        out = {"brick": t.brick_size, "cell": t.cell, "samples_per_batch": t.samples_per_batch,
+               "tile_width": t.tile_width, "tile_group": t.tile_group,
+               "tuned_ms": round(t.kernel_ms, 4), "start_ms": round(t.baseline_ms, 4),
+               "settings_timed": t.candidates, "seconds": round(time.perf_counter() - t0, 2)}
+        print(f"[bench] autotuned for {what}: {out}", file=sys.stderr, flush=True)
+        return out
+
+    tuning = None if args.no_autotune else tune(views, "the 36-view orbit launch")
     print(f"[bench] bricked into the device pool in {brick_s:.2f} s; warmup {args.warmup}",
           file=sys.stderr, flush=True)
 
-    cams = w.cameras
-    for i in range(args.warmup):
-        workloads.set_camera(r, cams[i % len(cams)])
-        assert r.paint(), r.lastError()
-        r.frameStats()
+    # A step renders the whole orbit: 36 views of 1920x1080 in one persistent
+    # launch (BatchRenderer-style capture of an orbit), images left in HBM.
+    for _ in range(args.warmup):
+        r.renderBatch(views, readback=False)
     r.synchronize()
-
     kernel_ms, evaluated, nominal = [], 0, 0
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        workloads.set_camera(r, cams[i % len(cams)])
-        if not r.paint():
-            raise RuntimeError(r.lastError())
-        st = r.frameStats()  # waits for this frame's events and counters
-        kernel_ms.append(st.kernel_ms)
-        evaluated += st.evaluated_samples
-        nominal += st.nominal_samples
+    for _ in range(args.steps):
+        _, st, kms = r.renderBatch(views, readback=False)  # waits for the launch's counters
+        kernel_ms.append(kms)
+        evaluated += sum(v.evaluated_samples for v in st)
+        nominal += sum(v.nominal_samples for v in st)
     r.synchronize()
     wall = time.perf_counter() - t0
-    print(f"[bench] {args.steps} frames in {wall:.3f} s", file=sys.stderr, flush=True)
+    frames = args.steps * len(views)
+    print(f"[bench] {args.steps} orbit launches ({frames} frames) in {wall:.3f} s",
+          file=sys.stderr, flush=True)
 
     ksum = sum(kernel_ms) / 1e3
-    gather_bytes = evaluated * 8 * 2 + args.steps * w.width * w.height * 4
+    gather_bytes = evaluated * 8 * 2 + frames * w.width * w.height * 4
     achieved = gather_bytes / ksum / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
-        if t.get("workload") == w.name:
+        if t.get("workload") == w.name and t.get("launch") == "orbit-batch-36":
             traffic = t.get("bytes_per_launch")
     out = {
         "metric": METRIC,
@@ -476,20 +506,27 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": w.name, "volume": "512x512x512 u16", "image": "1920x1080",
-                   "views": len(cams), "rate": w.rate, "bricks": "256^3 + 2-voxel apron",
-                   "lod": 0, "tf": "4096 x RGBA f32, 3 Gaussian bumps", "fov_deg": 60},
+                   "views": len(cams), "views_per_launch": len(views), "rate": w.rate,
+                   "bricks": "256^3 + 2-voxel apron", "lod": 0,
+                   "tf": "4096 x RGBA f32, 3 Gaussian bumps", "fov_deg": 60},
         "launch": tuning or {"samples_per_batch": args.batch, "tile_group": args.tile_group,
                              "xcd_split": args.xcd_split, "tile_width": args.tile_width,
                              "brick": w.brick, "cell": 0 if args.no_skip else args.cell},
-        "fps": args.steps / wall,
+        "fps": frames / wall,
         "evaluated_gsamples_per_s": evaluated / wall / 1e9,
-        "nominal_samples_per_frame": nominal // args.steps,
-        "evaluated_samples_per_frame": evaluated // args.steps,
-        "kernel_ms_per_frame": ksum / args.steps * 1e3,
+        "nominal_samples_per_frame": nominal // frames,
+        "evaluated_samples_per_frame": evaluated // frames,
+        "kernel_ms_per_launch": ksum / args.steps * 1e3,
+        "kernel_ms_per_frame": ksum / frames * 1e3,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": None,
     }
+    if not args.no_interactive:
+        # interactive frames: one view per paint, cameras in orbit order,
+        # launch shape tuned for a single view
+        out["interactive"] = run_interactive(r, w, views, None if args.no_autotune else tune)
+        print(f"[bench] interactive {out['interactive']}", file=sys.stderr, flush=True)
     r.cleanup()
     if not args.no_learned_lod:
         print("[bench] learned-LOD workload", file=sys.stderr, flush=True)

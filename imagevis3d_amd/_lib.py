@@ -159,6 +159,7 @@ SIGNATURES = {
     "tvk_lod_mlp_create": [i32, P, P, C.POINTER(C.c_void_p)],
     "tvk_lod_mlp_predict": [C.c_void_p, P, u64, P, C.POINTER(f32)],
     "tvk_lod_mlp_destroy": [C.c_void_p],
+    "tvk_brick_slot_bytes": [u32, u32, u32],
     "tvk_lod_features": [C.POINTER(View), P, u32, u32, P, P, P, P, P, P],
     "tvk_lod_select_octree": [P, P, P, u32, C.c_double, u64, u64, P, C.POINTER(C.c_double),
                               C.POINTER(u64)],
@@ -186,7 +187,8 @@ SIGNATURES = {
     "tvk_generate_volume_gpu": [i32, i32, u64, u64, u64, u64, u32, P, u64],
     "tvk_set_generated_volume": [H, i32, u64, u64, u64, u64, u32, u32, u32, u32],
 }
-_RESTYPE = {"tvk_last_error": C.c_char_p, "tvk_version": C.c_char_p}
+_RESTYPE = {"tvk_last_error": C.c_char_p, "tvk_version": C.c_char_p,
+            "tvk_brick_slot_bytes": C.c_uint64}
 
 _lib = None
 

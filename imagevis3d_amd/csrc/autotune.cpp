@@ -95,7 +95,7 @@ int HipRenderer::Autotune(const tvk_view* views, uint32_t n, uint32_t flags, tvk
     if (int e = try_values(cells, &Setting::cell, true)) return e;
   }
   if (flags & TVK_TUNE_LAUNCH) {
-    if (int e = try_values({4u, 8u}, &Setting::batch, false)) return e;
+    if (int e = try_values({2u, 4u, 8u}, &Setting::batch, false)) return e;
     if (int e = try_values({4u, 8u, 16u}, &Setting::tile_width, false)) return e;
     if (int e = try_values({1u, 2u, 4u, 8u, 16u}, &Setting::tile_group, false)) return e;
   }

@@ -14,13 +14,19 @@ hipError_t launch_downsample(const void* src, int bpv, const int64_t src_dims[3]
                              hipStream_t stream);
 
 // Cut `n_jobs` bricks of payload B with apron a out of a flat level into
-// pool slots (clamp-to-edge outside the volume) and write each brick's
-// (min, max) over the slot to brick_range[2*brick_id].  Slot j starts at
-// element j * slot_stride of the pool; scratch_minmax holds 2 u32 per job.
+// pool slots (blocked slot layout, clamp-to-edge outside the volume) and
+// write each brick's (min, max) over the slot to brick_range[2*brick_id].
+// Slot j starts at element j * slot_stride of the pool; scratch_minmax
+// holds 2 u32 per job.
 hipError_t launch_extract_bricks(const void* level, int bpv, const int64_t dims[3],
                                  const BrickJob* jobs, int n_jobs, void* pool,
                                  int64_t slot_stride, int B, int apron, float* brick_range,
                                  uint32_t* scratch_minmax, hipStream_t stream);
+
+// An x-fastest S^3 brick (src) into one slot of the blocked slot layout,
+// and one slot back into an x-fastest S^3 brick (dst).
+hipError_t launch_linear_to_slot(const void* src, void* slot, int bpv, int S, hipStream_t stream);
+hipError_t launch_slot_to_linear(const void* slot, void* dst, int bpv, int S, hipStream_t stream);
 
 // Empty-space cell ranges of one flat level: out[(cz*ncy + cy)*ncx + cx] =
 // min | max << 16 over voxels [c*C - 1, (c+1)*C] (clamped to the level).

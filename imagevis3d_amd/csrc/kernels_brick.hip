@@ -72,7 +72,20 @@ __device__ inline int64_t clamp64(int64_t v, int64_t lo, int64_t hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }
 
-// grid.x: chunks of one slot, grid.y: brick job.
+// Slot-local voxel of element i of a slot in the blocked slot layout
+// (tvk_common.hpp); the overlap voxel past a block row's end and padding
+// past the slot's edge repeat the slot's last voxel along that axis.
+__device__ inline void slot_element_voxel(uint32_t i, int S, int& x, int& y, int& z) {
+  const uint32_t n = (uint32_t)slot_blocks(S);
+  const uint32_t b = i / kSlotBlockElems, r = i % kSlotBlockElems;
+  const uint32_t row = r / 5, e = r % 5;
+  x = min((int)((b % n) * 4 + e), S - 1);
+  y = min((int)(((b / n) % n) * 4 + (row & 3)), S - 1);
+  z = min((int)((b / (n * n)) * 4 + (row >> 2)), S - 1);
+}
+
+// grid.x: chunks of one slot, grid.y: brick job.  Writes the slot in
+// element order (coalesced), reading the level voxel each element holds.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void extract_bricks_kernel(
     const T* __restrict__ level, int64_t nx, int64_t ny, int64_t nz,
@@ -88,12 +101,9 @@ __global__ __launch_bounds__(kThreads) void extract_bricks_kernel(
   const int64_t beg = chunk * blockIdx.x;
   const int64_t end = beg + chunk < slot_elems ? beg + chunk : slot_elems;
   uint32_t lo = 0xffffffffu, hi = 0;
-  const uint32_t uS = (uint32_t)S;  // S^3 < 2^31 (checked by the host)
   for (int64_t i = beg + threadIdx.x; i < end; i += kThreads) {
-    const uint32_t ii = (uint32_t)i;
-    const int64_t x = ii % uS;
-    const uint32_t yz = ii / uS;
-    const int64_t y = yz % uS, z = yz / uS;
+    int x, y, z;
+    slot_element_voxel((uint32_t)i, S, x, y, z);  // slot_elems < 2^31 (checked by the host)
     const int64_t gx = clamp64(ox + x, 0, nx - 1);
     const int64_t gy = clamp64(oy + y, 0, ny - 1);
     const int64_t gz = clamp64(oz + z, 0, nz - 1);
@@ -108,6 +118,28 @@ __global__ __launch_bounds__(kThreads) void extract_bricks_kernel(
     atomicMin(&minmax[2 * blockIdx.y], lo);
     atomicMax(&minmax[2 * blockIdx.y + 1], hi);
   }
+}
+
+// An x-fastest S^3 brick into the blocked slot layout, and back.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void linear_to_slot_kernel(const T* __restrict__ src,
+                                                                  T* __restrict__ slot, int S,
+                                                                  int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  int x, y, z;
+  slot_element_voxel((uint32_t)i, S, x, y, z);
+  slot[i] = src[((int64_t)z * S + y) * S + x];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void slot_to_linear_kernel(const T* __restrict__ slot,
+                                                                  T* __restrict__ dst, int S,
+                                                                  int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const int x = (int)(i % S), y = (int)((i / S) % S), z = (int)(i / ((int64_t)S * S));
+  dst[i] = slot[slot_offset(slot_geo(S), x, y, z)];
 }
 
 // One wave per empty-space cell: lanes stride over the cell's (C+2)^3
@@ -155,10 +187,11 @@ __global__ __launch_bounds__(kThreads) void slot_cell_range_kernel(
   const int y0 = max(cy * C - 1 - (by * B - apron), 0), y1 = min((cy + 1) * C - (by * B - apron), S - 1);
   const int z0 = max(cz * C - 1 - (bz * B - apron), 0), z1 = min((cz + 1) * C - (bz * B - apron), S - 1);
   const int w = x1 - x0 + 1, h = y1 - y0 + 1, dz = z1 - z0 + 1;
+  const SlotGeo G = slot_geo(S);
   uint32_t lo = 0xffffffffu, hi = 0;
   for (int i = lane; i < w * h * dz; i += 64) {
     const int x = x0 + i % w, y = y0 + (i / w) % h, z = z0 + i / (w * h);
-    const uint32_t v = slot[((int64_t)z * S + y) * S + x];
+    const uint32_t v = slot[slot_offset(G, x, y, z)];
     lo = min(lo, v);
     hi = max(hi, v);
   }
@@ -318,8 +351,7 @@ hipError_t launch_extract_bricks(const void* level, int bpv, const int64_t d[3],
                                  const BrickJob* jobs, int n_jobs, void* pool,
                                  int64_t slot_stride, int B, int apron, float* brick_range,
                                  uint32_t* scratch, hipStream_t stream) {
-  const int64_t S = B + 2 * apron;
-  const int64_t slot_elems = S * S * S;
+  const int64_t slot_elems = slot_block_elems(B + 2 * apron);
   if (n_jobs <= 0) return hipSuccess;
   minmax_init_kernel<<<(n_jobs + 255) / 256, 256, 0, stream>>>(scratch, n_jobs);
   int64_t chunks = (slot_elems + 8191) / 8192;
@@ -341,6 +373,28 @@ hipError_t launch_extract_bricks(const void* level, int bpv, const int64_t d[3],
   }
   minmax_finish_kernel<<<(n_jobs + 255) / 256, 256, 0, stream>>>(scratch, jobs, n_jobs,
                                                                   brick_range);
+  return hipGetLastError();
+}
+
+hipError_t launch_linear_to_slot(const void* src, void* slot, int bpv, int S, hipStream_t stream) {
+  const int64_t n = slot_block_elems(S);
+  const unsigned blocks = (unsigned)((n + kThreads - 1) / kThreads);
+  if (bpv == 1)
+    linear_to_slot_kernel<uint8_t><<<blocks, kThreads, 0, stream>>>((const uint8_t*)src, (uint8_t*)slot, S, n);
+  else
+    linear_to_slot_kernel<uint16_t><<<blocks, kThreads, 0, stream>>>((const uint16_t*)src, (uint16_t*)slot,
+                                                                      S, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_slot_to_linear(const void* slot, void* dst, int bpv, int S, hipStream_t stream) {
+  const int64_t n = (int64_t)S * S * S;
+  const unsigned blocks = (unsigned)((n + kThreads - 1) / kThreads);
+  if (bpv == 1)
+    slot_to_linear_kernel<uint8_t><<<blocks, kThreads, 0, stream>>>((const uint8_t*)slot, (uint8_t*)dst, S, n);
+  else
+    slot_to_linear_kernel<uint16_t><<<blocks, kThreads, 0, stream>>>((const uint16_t*)slot, (uint16_t*)dst,
+                                                                      S, n);
   return hipGetLastError();
 }
 

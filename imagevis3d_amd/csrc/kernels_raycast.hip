@@ -82,12 +82,14 @@ __device__ inline void ld2(const uint8_t* p, float& a, float& b) {
   b = (float)(v >> 8);
 }
 
-// One sample from a brick slot at slot-local texel coordinate (gx, gy, gz):
-// trilinear over the 8 surrounding voxels, or the nearest voxel.  Indices
+// One sample from a brick slot (blocked slot layout) at slot-local texel
+// coordinate (gx, gy, gz): trilinear over the 8 surrounding voxels -- four
+// loads of an x-pair, one per (y, z) row -- or the nearest voxel.  Indices
 // are clamped to the slot, so no fetch leaves the slot.
 template <typename T, bool LINEAR>
-__device__ inline float fetch(const T* __restrict__ base, int S, int S2, float gx, float gy,
+__device__ inline float fetch(const T* __restrict__ base, const SlotGeo& G, float gx, float gy,
                               float gz) {
+  const int S = G.S;
   if (LINEAR) {
     const int ix = clampi((int)floorf(gx), 0, S - 2);
     const int iy = clampi((int)floorf(gy), 0, S - 2);
@@ -95,12 +97,14 @@ __device__ inline float fetch(const T* __restrict__ base, int S, int S2, float g
     const float wx = clampf(gx - (float)ix, 0.f, 1.f);
     const float wy = clampf(gy - (float)iy, 0.f, 1.f);
     const float wz = clampf(gz - (float)iz, 0.f, 1.f);
-    const T* p = base + (iz * S2 + iy * S + ix);
+    const T* p = base + slot_ox(ix);
+    const int y0 = slot_oy(G, iy), y1 = slot_oy(G, iy + 1);
+    const int z0 = slot_oz(G, iz), z1 = slot_oz(G, iz + 1);
     float c000, c100, c010, c110, c001, c101, c011, c111;
-    ld2(p, c000, c100);
-    ld2(p + S, c010, c110);
-    ld2(p + S2, c001, c101);
-    ld2(p + S2 + S, c011, c111);
+    ld2(p + (y0 + z0), c000, c100);
+    ld2(p + (y1 + z0), c010, c110);
+    ld2(p + (y0 + z1), c001, c101);
+    ld2(p + (y1 + z1), c011, c111);
     const float c00 = fmaf(c100 - c000, wx, c000);
     const float c10 = fmaf(c110 - c010, wx, c010);
     const float c01 = fmaf(c101 - c001, wx, c001);
@@ -112,26 +116,7 @@ __device__ inline float fetch(const T* __restrict__ base, int S, int S2, float g
   const int ix = clampi((int)floorf(gx), 0, S - 1);
   const int iy = clampi((int)floorf(gy), 0, S - 1);
   const int iz = clampi((int)floorf(gz), 0, S - 1);
-  return ld(base + (iz * S2 + iy * S + ix));
-}
-
-// Four x-neighbours p[0..3] with one load (8 bytes for 16-bit voxels, 4
-// for 8-bit ones).
-__device__ inline void ld4(const uint16_t* p, float o[4]) {
-  uint64_t v;
-  __builtin_memcpy(&v, p, 8);
-  o[0] = (float)(uint32_t)(v & 0xffffu);
-  o[1] = (float)(uint32_t)((v >> 16) & 0xffffu);
-  o[2] = (float)(uint32_t)((v >> 32) & 0xffffu);
-  o[3] = (float)(uint32_t)(v >> 48);
-}
-__device__ inline void ld4(const uint8_t* p, float o[4]) {
-  uint32_t v;
-  __builtin_memcpy(&v, p, 4);
-  o[0] = (float)(v & 0xffu);
-  o[1] = (float)((v >> 8) & 0xffu);
-  o[2] = (float)((v >> 16) & 0xffu);
-  o[3] = (float)(v >> 24);
+  return ld(base + slot_offset(G, ix, iy, iz));
 }
 
 __device__ inline float trilerp(const float c[2][2][2], float wx, float wy, float wz) {
@@ -149,34 +134,41 @@ __device__ inline float trilerp(const float c[2][2][2], float wx, float wy, floa
 // axis (unscaled).  The seven interpolations share their weights, so each
 // difference is the interpolation of voxel differences over the 2x2x2
 // cell; the 32 voxels they read (the cell and one voxel beyond it along
-// each axis) come in 12 loads.  The cell index is clamped so that every
-// read stays in the slot (apron >= 2).
+// each axis) come in 16 x-pair loads.  The cell index is clamped so that
+// every read stays in the slot (apron >= 2).
 template <typename T>
-__device__ inline void fetch_value_grad(const T* __restrict__ base, int S, int S2, float gx,
+__device__ inline void fetch_value_grad(const T* __restrict__ base, const SlotGeo& G, float gx,
                                         float gy, float gz, float& val, float g[3]) {
+  const int S = G.S;
   const int ix = clampi((int)floorf(gx), 1, S - 3);
   const int iy = clampi((int)floorf(gy), 1, S - 3);
   const int iz = clampi((int)floorf(gz), 1, S - 3);
   const float wx = clampf(gx - (float)ix, 0.f, 1.f);
   const float wy = clampf(gy - (float)iy, 0.f, 1.f);
   const float wz = clampf(gz - (float)iz, 0.f, 1.f);
-  const T* p = base + (iz * S2 + iy * S + ix);
+  const int xm = slot_ox(ix - 1), x0 = slot_ox(ix), xp = slot_ox(ix + 1);
+  const int yo[4] = {slot_oy(G, iy - 1), slot_oy(G, iy), slot_oy(G, iy + 1), slot_oy(G, iy + 2)};
+  const int zo[4] = {slot_oz(G, iz - 1), slot_oz(G, iz), slot_oz(G, iz + 1), slot_oz(G, iz + 2)};
   float row[2][2][4];  // [z][y][x - 1 .. x + 2]
   float ya[2][2][2];   // [z][y - 1, y + 2][x, x + 1]
   float za[2][2][2];   // [z - 1, z + 2][y][x, x + 1]
 #pragma unroll
   for (int z = 0; z < 2; ++z)
 #pragma unroll
-    for (int y = 0; y < 2; ++y) ld4(p + z * S2 + y * S - 1, row[z][y]);
+    for (int y = 0; y < 2; ++y) {
+      const T* q = base + (yo[1 + y] + zo[1 + z]);
+      ld2(q + xm, row[z][y][0], row[z][y][1]);
+      ld2(q + xp, row[z][y][2], row[z][y][3]);
+    }
 #pragma unroll
   for (int z = 0; z < 2; ++z) {
-    ld2(p + z * S2 - S, ya[z][0][0], ya[z][0][1]);
-    ld2(p + z * S2 + 2 * S, ya[z][1][0], ya[z][1][1]);
+    ld2(base + (x0 + yo[0] + zo[1 + z]), ya[z][0][0], ya[z][0][1]);
+    ld2(base + (x0 + yo[3] + zo[1 + z]), ya[z][1][0], ya[z][1][1]);
   }
 #pragma unroll
   for (int y = 0; y < 2; ++y) {
-    ld2(p - S2 + y * S, za[0][y][0], za[0][y][1]);
-    ld2(p + 2 * S2 + y * S, za[1][y][0], za[1][y][1]);
+    ld2(base + (x0 + yo[1 + y] + zo[0]), za[0][y][0], za[0][y][1]);
+    ld2(base + (x0 + yo[1 + y] + zo[3]), za[1][y][0], za[1][y][1]);
   }
   float c[2][2][2], dx[2][2][2], dy[2][2][2], dz[2][2][2];
 #pragma unroll
@@ -224,7 +216,7 @@ __device__ inline float fetch_level(const RaycastParams& P, float qx, float qy, 
   }
   if (P.touch) P.touch[gid] = P.frame;
   const T* base = (const T*)P.pool + (int64_t)slot * P.slot_elems;
-  return fetch<T, LINEAR>(base, P.slot_edge, P.slot_edge * P.slot_edge, g[0], g[1], g[2]);
+  return fetch<T, LINEAR>(base, slot_geo(P.slot_edge), g[0], g[1], g[2]);
 }
 
 // 1D TF: linear between the two entries around value * tf_scale.
@@ -328,7 +320,7 @@ __device__ inline int march_regions(const RaycastParams& P, const float4* __rest
       } else {
         if (P.touch) P.touch[gid] = P.frame;
         const T* __restrict__ base = (const T*)P.pool + (int64_t)slot * P.slot_elems;
-        const int S = P.slot_edge, S2 = S * S;
+        const SlotGeo G = slot_geo(P.slot_edge);
         const float step = ldexpf(base_step, l);
         const float span = tb - t;
         const int m = max((int)ceilf(span / step), 1);
@@ -341,7 +333,7 @@ __device__ inline int march_regions(const RaycastParams& P, const float4* __rest
             const int kk = min(k + j, m - 1);
             const float len = kk == m - 1 ? last : step;
             const float ts = fmaf((float)kk, step, t) + 0.5f * len;
-            v[j] = fetch<T, LINEAR>(base, S, S2, fmaf(ts, od[0], g0[0]), fmaf(ts, od[1], g0[1]),
+            v[j] = fetch<T, LINEAR>(base, G, fmaf(ts, od[0], g0[0]), fmaf(ts, od[1], g0[1]),
                                     fmaf(ts, od[2], g0[2]));
           }
 #pragma unroll
@@ -455,8 +447,7 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
     const int ucount[3] = {(lv.dims[0] + U - 1) / U, (lv.dims[1] + U - 1) / U,
                            (lv.dims[2] + U - 1) / U};
     const float tfmax = (float)(P.n_tf - 1);
-    const int S = P.slot_edge;
-    const int S2 = S * S;
+    const SlotGeo G = slot_geo(P.slot_edge);
     float Lh[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // light direction, half vector
     if (MODE != kMode1D) {
       float h[3], hl = 0.f;
@@ -528,15 +519,15 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
                         gz = fmaf(t, od[2], g0[2]);
             float d[3];
             if (LINEAR) {
-              fetch_value_grad<T>(base, S, S2, gx, gy, gz, v[j], d);
+              fetch_value_grad<T>(base, G, gx, gy, gz, v[j], d);
             } else {
               // nearest value; the gradient of the trilinear interpolant at
               // the same point (texel coordinates shifted by half a voxel)
-              v[j] = fetch<T, false>(base, S, S2, gx, gy, gz);
+              v[j] = fetch<T, false>(base, G, gx, gy, gz);
               const float lx = gx - 0.5f, ly = gy - 0.5f, lz = gz - 0.5f;
-              d[0] = fetch<T, true>(base, S, S2, lx + 1.f, ly, lz) - fetch<T, true>(base, S, S2, lx - 1.f, ly, lz);
-              d[1] = fetch<T, true>(base, S, S2, lx, ly + 1.f, lz) - fetch<T, true>(base, S, S2, lx, ly - 1.f, lz);
-              d[2] = fetch<T, true>(base, S, S2, lx, ly, lz + 1.f) - fetch<T, true>(base, S, S2, lx, ly, lz - 1.f);
+              d[0] = fetch<T, true>(base, G, lx + 1.f, ly, lz) - fetch<T, true>(base, G, lx - 1.f, ly, lz);
+              d[1] = fetch<T, true>(base, G, lx, ly + 1.f, lz) - fetch<T, true>(base, G, lx, ly - 1.f, lz);
+              d[2] = fetch<T, true>(base, G, lx, ly, lz + 1.f) - fetch<T, true>(base, G, lx, ly, lz - 1.f);
             }
             g[j][0] = d[0] * P.grad_scale[0];
             g[j][1] = d[1] * P.grad_scale[1];
@@ -579,7 +570,7 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
             const int kk = min(k + j, kend - 1);
             const float lenj = (kk == n - 1) ? lastlen : step;
             const float t = fmaf((float)kk, step, t0) + 0.5f * lenj;
-            v[j] = fetch<T, LINEAR>(base, S, S2, fmaf(t, od[0], g0[0]), fmaf(t, od[1], g0[1]),
+            v[j] = fetch<T, LINEAR>(base, G, fmaf(t, od[0], g0[0]), fmaf(t, od[1], g0[1]),
                                     fmaf(t, od[2], g0[2]));
           }
 #pragma unroll
@@ -631,7 +622,7 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
           const int kk = min(k + j, kend - 1);
           const float lenj = (kk == n - 1) ? lastlen : step;
           const float t = fmaf((float)kk, step, t0) + 0.5f * lenj;
-          v[j] = fetch<T, LINEAR>(base, S, S2, fmaf(t, od[0], g0[0]), fmaf(t, od[1], g0[1]),
+          v[j] = fetch<T, LINEAR>(base, G, fmaf(t, od[0], g0[0]), fmaf(t, od[1], g0[1]),
                                   fmaf(t, od[2], g0[2]));
         }
 #pragma unroll

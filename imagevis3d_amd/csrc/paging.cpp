@@ -7,8 +7,9 @@
 // non-empty brick writes requests[brick] = 1 and samples nothing there;
 // every brick it does sample gets touch[brick] = frame.  After the launch
 // the host reads the requests, fills bricks from the brick source into
-// pinned staging buffers, copies them into free slots or into the least
-// recently touched slots not used by this frame, refreshes those bricks'
+// pinned staging buffers, copies them to the device and rearranges them
+// (blocked slot layout) into free slots or into the least recently touched
+// slots not used by this frame, refreshes those bricks'
 // empty-space cell ranges on the device, and asks for another pass.  When
 // a view needs more bricks of a LOD than the pool holds, the frame falls
 // back to the next coarser LOD.  Empty bricks (by the per-brick ranges the
@@ -53,7 +54,9 @@ int HipRenderer::SetPagedVolume(const uint64_t dims[3], uint32_t bits, uint32_t 
     return TVK_ERR_OOM;
   }
   const int total = plan.total_bricks;
-  const uint64_t need = n_slots * plan.slot_bytes + (uint64_t)total * 20 +
+  const uint64_t S = (uint64_t)plan.slot_edge;
+  const uint64_t brick_bytes = S * S * S * (bits / 8);  // one brick as the source gives it
+  const uint64_t need = n_slots * plan.slot_bytes + brick_bytes + (uint64_t)total * 20 +
                         (uint64_t)plan.total_cells * 4 + (uint64_t)width_ * height_ * 4;
   if (need > gpu_budget_) {
     set_last_error("paged pool needs " + std::to_string(need) + " bytes of device memory, budget is " +
@@ -86,6 +89,7 @@ int HipRenderer::SetPagedVolume(const uint64_t dims[3], uint32_t bits, uint32_t 
   }
 
   TVK_HIP(pool_.slots.alloc(n_slots * plan.slot_bytes));
+  TVK_HIP(upload_stage_.alloc(brick_bytes));
   TVK_HIP(pool_.page_table.alloc((size_t)total * sizeof(int32_t)));
   TVK_HIP(pool_.brick_range.alloc((size_t)total * 2 * sizeof(float)));
   TVK_HIP(cell_range_.alloc((size_t)plan.total_cells * sizeof(uint32_t)));
@@ -195,8 +199,12 @@ int HipRenderer::upload_brick(int gid, int lod, int bx, int by, int bz, int slot
     return TVK_ERR_IO;
   }
   uint8_t* dst = (uint8_t*)pool_.slots.p + (uint64_t)slot * pool_.slot_bytes;
-  TVK_HIP(hipMemcpyAsync(dst, staging_[st], pool_.slot_bytes, hipMemcpyHostToDevice, stream_));
+  // the source's x-fastest brick -> device staging -> the slot's blocked
+  // layout (copies and kernels are ordered on the one stream)
+  const uint64_t n = (uint64_t)slot_edge_ * slot_edge_ * slot_edge_ * (bits_ / 8);
+  TVK_HIP(hipMemcpyAsync(upload_stage_.p, staging_[st], n, hipMemcpyHostToDevice, stream_));
   TVK_HIP(hipEventRecord(staging_ev_[st], stream_));
+  TVK_HIP(launch_linear_to_slot(upload_stage_.p, dst, (int)bits_ / 8, slot_edge_, stream_));
   const LodInfo& L = lods_[lod];
   const int b[3] = {bx, by, bz};
   TVK_HIP(launch_slot_cell_ranges(dst, (int)bits_ / 8, slot_edge_, apron_, brick_, cell_, b, L.cells,

@@ -243,7 +243,7 @@ int HipRenderer::plan_levels(const uint64_t dims[3], uint32_t bits, uint32_t bri
   if (brick < 8 || brick > 1024) return arg_fail("brick size must be in [8, 1024]");
   if (apron < 1 || apron > 8) return arg_fail("apron must be in [1, 8]");
   const uint64_t S = (uint64_t)brick + 2ull * apron;
-  if (S * S * S >= (1ull << 31)) return arg_fail("brick slot too large");
+  if ((uint64_t)slot_block_elems((int)S) >= (1ull << 31)) return arg_fail("brick slot too large");
   const int bpv = (int)bits / 8;
   // LOD count: RasterDataBlock rule when not given (UVFReader/DataSource.h:521-529)
   int nl = (int)n_lod;
@@ -290,7 +290,7 @@ int HipRenderer::plan_levels(const uint64_t dims[3], uint32_t bits, uint32_t bri
   out->brick = (int)brick;
   out->level_bytes = level_bytes;
   out->slot_edge = (int)S;
-  out->slot_bytes = ((S * S * S * bpv) + 255) & ~255ull;
+  out->slot_bytes = ((uint64_t)slot_block_elems((int)S) * bpv + 255) & ~255ull;
   return TVK_OK;
 }
 
@@ -299,6 +299,7 @@ void HipRenderer::release_volume() {
   paged_ = false;
   for (auto& l : levels_) l.reset();
   pool_.slots.reset();
+  upload_stage_.reset();
   pool_.page_table.reset();
   pool_.brick_range.reset();
   pool_.n_slots = pool_.slot_bytes = pool_.resident = 0;
@@ -620,8 +621,12 @@ int HipRenderer::ReadBrick(uint32_t lod, uint32_t bx, uint32_t by, uint32_t bz, 
     return TVK_OK;
   }
   TVK_HIP(hipSetDevice(device_));
-  TVK_HIP(hipMemcpyAsync(out, (const uint8_t*)pool_.slots.p + (uint64_t)slot * pool_.slot_bytes, n,
-                         hipMemcpyDeviceToHost, stream_));
+  // the slot's blocked layout back to an x-fastest brick, then to the host
+  DevBuf lin;
+  TVK_HIP(lin.alloc(n));
+  TVK_HIP(launch_slot_to_linear((const uint8_t*)pool_.slots.p + (uint64_t)slot * pool_.slot_bytes, lin.p,
+                                (int)bits_ / 8, slot_edge_, stream_));
+  TVK_HIP(hipMemcpyAsync(out, lin.p, n, hipMemcpyDeviceToHost, stream_));
   TVK_HIP(hipStreamSynchronize(stream_));
   return TVK_OK;
 }

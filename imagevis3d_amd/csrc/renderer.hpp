@@ -320,6 +320,7 @@ class HipRenderer {
   static constexpr int kStaging = 4;
   void* staging_[kStaging] = {nullptr, nullptr, nullptr, nullptr};  // pinned slots
   hipEvent_t staging_ev_[kStaging] = {nullptr, nullptr, nullptr, nullptr};
+  DevBuf upload_stage_;  // one x-fastest brick on its way into a slot
   uint32_t frame_ = 0;
   int lod_floor_ = 0;       // coarsest-needed fallback when the pool overflows
   std::vector<uint32_t> cell_init_h_;  // brick-range initial cell ranges

@@ -224,6 +224,12 @@ int tvk_read_brick(tvk_renderer* r, uint32_t lod, uint32_t bx, uint32_t by, uint
   TVK_TRY(r->impl.ReadBrick(lod, bx, by, bz, out, bytes));
 }
 
+uint64_t tvk_brick_slot_bytes(uint32_t brick_size, uint32_t apron, uint32_t bits) {
+  if (brick_size < 1 || brick_size > (1u << 20) || apron > 64 || (bits != 8 && bits != 16)) return 0;
+  const int S = (int)(brick_size + 2 * apron);
+  return ((uint64_t)tvk::slot_block_elems(S) * (bits / 8) + 255) & ~255ull;
+}
+
 int tvk_get_brick_minmax(tvk_renderer* r, uint32_t lod, float* mins, float* maxs, uint64_t n) {
   TVK_GUARD(r);
   return r->impl.BrickMinMax(lod, mins, maxs, n);

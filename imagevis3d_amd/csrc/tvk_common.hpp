@@ -4,8 +4,8 @@
 // Device data layout (HBM):
 //   brick pool   one allocation of fixed slots; a slot holds one brick of
 //                payload B^3 plus an apron of `apron` voxels on every side,
-//                (B+2a)^3 voxels, x fastest, bpv bytes per voxel.  Slots are
-//                256-byte aligned.
+//                S^3 voxels (S = B+2a), bpv bytes per voxel, in the blocked
+//                slot layout below.  Slots are 256-byte aligned.
 //   page table   int32 per brick of every LOD (LOD-major, then z, y, x):
 //                slot index, or -1 when the brick is not resident.
 //   brick ranges float2 (min, max) per brick over its stored slot.
@@ -20,7 +20,48 @@
 #pragma once
 #include <stdint.h>
 
+#if defined(__HIPCC__)
+#define TVK_HD __host__ __device__
+#else
+#define TVK_HD
+#endif
+
 namespace tvk {
+
+// Blocked slot layout.  The slot's S^3 voxels are stored in 4x4x4 blocks,
+// ceil(S/4) per axis, blocks x fastest, then y, then z.  Inside a block the
+// 16 rows of x (y, then z within the block) are 5 voxels long: the row's 4
+// voxels and the voxel after them in x (the first voxel of the next block's
+// row; past the slot's last voxel, the last voxel again).  So the two
+// x-neighbours (x, x+1) of any voxel are adjacent in memory: a trilinear
+// sample is 4 two-voxel loads, and the 64 lanes of a wave, whose samples
+// lie in a small patch of the volume, touch few cache lines per load (the
+// x-fastest layout spreads the same patch over one line per row).  Costs
+// 1.25x the brick's voxels.
+constexpr int kSlotBlockElems = 80;  // 16 rows of 5
+
+TVK_HD inline int slot_blocks(int S) { return (S + 3) >> 2; }
+TVK_HD inline int64_t slot_block_elems(int S) {
+  const int64_t n = slot_blocks(S);
+  return n * n * n * kSlotBlockElems;
+}
+
+// Element offsets of slot voxel (x, y, z) = sx(x) + sy(y) + sz(z).
+struct SlotGeo {
+  int S;   // slot edge B + 2a
+  int by;  // elements per block step in y (blocks per row x 80)
+  int bz;  // elements per block step in z
+};
+TVK_HD inline SlotGeo slot_geo(int S) {
+  const int n = slot_blocks(S);
+  return SlotGeo{S, n * kSlotBlockElems, n * n * kSlotBlockElems};
+}
+TVK_HD inline int slot_ox(int x) { return (x >> 2) * kSlotBlockElems + (x & 3); }
+TVK_HD inline int slot_oy(const SlotGeo& g, int y) { return (y >> 2) * g.by + (y & 3) * 5; }
+TVK_HD inline int slot_oz(const SlotGeo& g, int z) { return (z >> 2) * g.bz + (z & 3) * 20; }
+TVK_HD inline int64_t slot_offset(const SlotGeo& g, int x, int y, int z) {
+  return (int64_t)slot_ox(x) + slot_oy(g, y) + slot_oz(g, z);
+}
 
 constexpr int kMaxLod = 16;
 constexpr int kTile = 8;  // ray tile edge: one 64-lane wavefront = 8x8 rays

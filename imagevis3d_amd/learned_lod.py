@@ -492,16 +492,19 @@ def choose_levels(pred, view, dims, edge, n_lod, hist, moments, safety: float,
 
 
 def calibrate_safety(renderer, pred, views, dims, edge, n_lod, hist, moments, target: float = 1.0,
-                     brick_bytes: int = 0, mem_budget: int = 0, detail=None):
-    """Largest budget factor (halving from 4) whose learned frames stay within
-    `target` RMSE of full resolution on every calibration view."""
+                     brick_bytes: int = 0, mem_budget: int = 0, detail=None, refine: int = 4):
+    """Largest budget factor whose learned frames stay within `target` RMSE
+    of full resolution on every calibration view: doubled (or halved) from 4
+    until the worst view crosses the target, then bisected geometrically
+    `refine` times.  Returns (factor, worst RMSE at it); (0, worst) when even
+    the smallest factor misses the target (a binding brick budget)."""
     refs = []
     for v in views:
         renderer.resize((v["width"], v["height"]))
         set_view(renderer, v)
         refs.append(full_resolution(renderer, dims, edge))
-    safety = 4.0
-    for _ in range(12):
+
+    def worst_at(safety):
         worst = 0.0
         for v, ref in zip(views, refs):
             renderer.resize((v["width"], v["height"]))
@@ -511,7 +514,37 @@ def calibrate_safety(renderer, pred, views, dims, edge, n_lod, hist, moments, ta
             renderer.setRegionLODs(lev)
             worst = max(worst, rmse255(renderer.capture(), ref))
             renderer.setRegionLODs(None)
-        if worst <= target:
-            return safety, worst
-        safety *= 0.5
-    return 0.0, worst
+        return worst
+
+    s = 4.0
+    w = worst_at(s)
+    if w <= target:
+        good, good_w, bad = s, w, None
+        while bad is None and good < 2.0 ** 24:
+            s = good * 2.0
+            w = worst_at(s)
+            if w <= target:
+                good, good_w = s, w
+            else:
+                bad = s
+        if bad is None:
+            return good, good_w
+    else:
+        bad, good, good_w = s, None, w
+        while good is None and bad > 2.0 ** -12:
+            s = bad * 0.5
+            w = worst_at(s)
+            if w <= target:
+                good, good_w = s, w
+            else:
+                bad = s
+        if good is None:
+            return 0.0, w
+    for _ in range(refine):
+        s = math.sqrt(good * bad)
+        w = worst_at(s)
+        if w <= target:
+            good, good_w = s, w
+        else:
+            bad = s
+    return good, good_w

@@ -581,6 +581,15 @@ def generate_volume(kind: int, seed: int, dims: Sequence[int], bits: int) -> np.
     return out
 
 
+def brick_slot_bytes(brick: int, apron: int = 2, bits: int = 16) -> int:
+    """Device bytes of one brick-pool slot (tvk_brick_slot_bytes): pools of
+    pool_bytes hold pool_bytes // this many bricks."""
+    n = int(_lib.load().tvk_brick_slot_bytes(int(brick), int(apron), int(bits)))
+    if n == 0:
+        raise ValueError("invalid brick size, apron or bit width")
+    return n
+
+
 def generate_volume_gpu(kind: int, seed: int, dims: Sequence[int], bits: int,
                         device: int = 0, out: np.ndarray | None = None) -> np.ndarray:
     """Device-generated synthetic volume (kinds VOL_FBM_PLANES, VOL_NOISE_TUBES,

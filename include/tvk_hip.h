@@ -93,7 +93,7 @@ typedef struct tvk_mem_stats {
   uint64_t allocated_gpu_mem;  /* bytes held by this renderer's pool + FB     */
   uint64_t pool_slots;         /* brick slots in the pool                     */
   uint64_t resident_bricks;    /* slots holding a brick                       */
-  uint64_t slot_bytes;         /* bytes per slot ((B+2*apron)^3 * bpv)        */
+  uint64_t slot_bytes;         /* bytes per slot (tvk_brick_slot_bytes)       */
   uint64_t bricks_uploaded;    /* host->device brick uploads since create     */
   uint64_t bricks_evicted;     /* LRU evictions since create                  */
 } tvk_mem_stats;
@@ -172,14 +172,20 @@ int tvk_get_brick_layout(tvk_renderer* r, uint32_t lod, uint32_t grid[3],
 /* Read back one LOD level as a flat volume (x fastest), bpv bytes/voxel:
  * the device pyramid that RasterDataBlock::FlatDataToBrickedLOD builds. */
 int tvk_read_lod(tvk_renderer* r, uint32_t lod, void* out, uint64_t out_bytes);
-/* Read back one stored brick slot including its apron ((B+2a)^3 voxels):
- * UVFDataset::GetBrick(BrickKey) (doc/import.adoc:761-775). */
+/* Read back one stored brick slot including its apron ((B+2a)^3 voxels, x
+ * fastest): UVFDataset::GetBrick(BrickKey) (doc/import.adoc:761-775). */
 int tvk_read_brick(tvk_renderer* r, uint32_t lod, uint32_t bx, uint32_t by, uint32_t bz,
                    void* out, uint64_t out_bytes);
 /* Per-brick min/max of one LOD (MaxMinDataBlock, BlockInfo.h:255-271),
  * over the brick's stored voxels (payload + apron), in brick-grid order. */
 int tvk_get_brick_minmax(tvk_renderer* r, uint32_t lod, float* mins, float* maxs,
                          uint64_t n_bricks);
+/* Device bytes of one brick-pool slot for bricks of payload brick_size and
+ * the given apron at 8 or 16 bits (GPUMemMan's per-brick texture size;
+ * pool budgets hold pool_bytes / this many bricks).  A slot keeps the
+ * (B+2a)^3 voxels in 4x4x4 blocks whose x-rows carry one extra voxel
+ * (1.25x the voxels), 256-byte aligned; 0 for invalid arguments. */
+uint64_t tvk_brick_slot_bytes(uint32_t brick_size, uint32_t apron, uint32_t bits);
 /* Histogram1DDataBlock::Compute over LOD 0 (UVFReader/DataSource.h:595-610):
  * n_bins = 2^bits (u64 counts). */
 int tvk_compute_histogram_1d(tvk_renderer* r, uint64_t* hist, uint64_t n_bins);

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from conftest import assert_parity, gpu_renderer
-from imagevis3d_amd import transfer, workloads
+from imagevis3d_amd import brick_slot_bytes, transfer, workloads
 from imagevis3d_amd.workloads import Camera
 from oracle import oracle as O
 
@@ -159,8 +159,7 @@ def test_tile_shapes_and_cells_do_not_change_the_image(tile_width):
 def test_batch_on_paged_volume_matches_in_core():
     w, vol = medium_small()
     print("case batch on a paged pool", flush=True)
-    s = w.brick + 4
-    slot_bytes = (s ** 3 * 2 + 255) // 256 * 256
+    slot_bytes = brick_slot_bytes(w.brick, 2, 16)
     pg = gpu_renderer(w)
     pg.setPagedFlatVolume(vol, w.brick, 70 * slot_bytes, n_lod=1)
     pg.set1DTrans(w.tf)

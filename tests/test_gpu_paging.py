@@ -5,7 +5,7 @@ bricks are never paged in."""
 import numpy as np
 import pytest
 
-from imagevis3d_amd import tuvok, workloads
+from imagevis3d_amd import brick_slot_bytes, tuvok, workloads
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -30,8 +30,7 @@ def setup(n=128, brick=32):
 
 
 def slot_bytes(brick, apron=2, bpv=2):
-    s = brick + 2 * apron
-    return (s ** 3 * bpv + 255) // 256 * 256
+    return brick_slot_bytes(brick, apron, 8 * bpv)
 
 
 def test_paged_render_equals_in_core():

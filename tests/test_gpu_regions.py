@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from conftest import assert_parity, gpu_renderer, pixel_diff
-from imagevis3d_amd import workloads
+from imagevis3d_amd import brick_slot_bytes, workloads
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -80,9 +80,8 @@ def test_paged_region_lods_match_in_core():
     cam = w.cameras[27]
     workloads.apply(r, w, cam)
     a = r.capture()
-    s = 36
     pg = gpu_renderer(w)
-    pg.setPagedFlatVolume(vol, 32, 80 * ((s ** 3 * 2 + 255) // 256 * 256), n_lod=3)
+    pg.setPagedFlatVolume(vol, 32, 80 * brick_slot_bytes(32, 2, 16), n_lod=3)
     pg.set1DTrans(w.tf)
     pg.setRegionLODs(rl)
     workloads.apply(pg, w, cam)

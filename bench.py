@@ -258,13 +258,15 @@ def run_batched(batches: int = 3, seed: int = 3) -> dict:
 
 
 def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: int = 24,
-                    eval_size=(3840, 2160)) -> dict:
+                    eval_size=(3840, 2160), train_size=(1920, 1080), train_views: int = 16,
+                    cal_views_n: int = 8, cal_target: float = 0.9) -> dict:
     """The learned-LOD workload: an n^3 uint8 volume of multi-scale noise
     with 64 thin tubes (device-generated, in HBM), 128^3 bricks, 6 LODs,
     1D TF (tubes_tf: opaque tubes over a faint noise bump), rate 2.  The
-    bf16 MLP is trained on the renderer's own fine-vs-coarse renders (32
-    views x 4 TFs at 480x270), its selection budget calibrated on 4 views,
-    then 24 new random views at 3840x2160 with a new TF are rendered at full
+    bf16 MLP is trained on the renderer's own fine-vs-coarse renders
+    (`train_views` views x 4 TFs at `train_size`), its selection budget
+    calibrated on `cal_views_n` views (mean RMSE `cal_target`/255), then 24
+    new random views at 3840x2160 with a new TF are rendered at full
     resolution and with the learned per-region LOD at two operating points:
     the error target alone ("target") and under a brick budget of a quarter
     of the LOD-0 bricks, as 16 GB of a 64 GiB volume ("budgeted").  Frame
@@ -299,7 +301,7 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
     hist = LL.region_histograms(r)
     detail = LL.region_detail(r)
     t0 = time.perf_counter()
-    X, Y = LL.training_rows(r, dims, edge, n_lod, views(32, 11, 480, 270),
+    X, Y = LL.training_rows(r, dims, edge, n_lod, views(train_views, 11, *train_size),
                             [transfer.tubes_tf(256, seed=s) for s in range(4)], hist, detail)
     data_s = time.perf_counter() - t0
     t0 = time.perf_counter()
@@ -312,12 +314,14 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
     slot_bytes = brick_slot_bytes(edge, 2, 8)
     n_reg = (n // edge) ** 3
     mem_budget = n_reg * slot_bytes // 4
-    cal_views = views(4, 12, *eval_size)
+    cal_views = views(cal_views_n, 12, *eval_size)
     # two operating points: under the brick budget, and with the error
-    # target alone (no brick cap)
+    # target alone (no brick cap); the budget factor is calibrated so that
+    # the mean RMSE over the calibration views stays at cal_target/255
     points = {"budgeted": mem_budget, "target": 0}
-    cal = {k: LL.calibrate_safety(r, pred, cal_views, dims, edge, n_lod, hist, mom, 1.0,
-                                  slot_bytes, mb, detail) for k, mb in points.items()}
+    cal = {k: LL.calibrate_safety(r, pred, cal_views, dims, edge, n_lod, hist, mom, cal_target,
+                                  slot_bytes, mb, detail, reduce="mean")
+           for k, mb in points.items()}
     full_ms, s_full = [], 0
     acc = {k: {"ms": [], "kernel_ms": [], "sel_ms": [], "rmse": [], "rmse_uniform": [], "mem": [],
                "samples": 0} for k in points}
@@ -368,6 +372,8 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
                     f"{eval_size[0]}x{eval_size[1]}-{eval_views}views",
         "fps_full": 1e3 / float(np.mean(full_ms)),
         "rmse_target": 1.0,
+        "calibration": {"views": cal_views_n, "mean_rmse_target": cal_target},
+        "training": {"views": train_views, "tfs": 4, "size": list(train_size)},
         "target": summary("target"),
         "budgeted": summary("budgeted"),
         "mlp_kernel_ms": pred.kernel_ms,

@@ -492,12 +492,16 @@ def choose_levels(pred, view, dims, edge, n_lod, hist, moments, safety: float,
 
 
 def calibrate_safety(renderer, pred, views, dims, edge, n_lod, hist, moments, target: float = 1.0,
-                     brick_bytes: int = 0, mem_budget: int = 0, detail=None, refine: int = 4):
+                     brick_bytes: int = 0, mem_budget: int = 0, detail=None, refine: int = 4,
+                     reduce: str = "max"):
     """Largest budget factor whose learned frames stay within `target` RMSE
-    of full resolution on every calibration view: doubled (or halved) from 4
-    until the worst view crosses the target, then bisected geometrically
-    `refine` times.  Returns (factor, worst RMSE at it); (0, worst) when even
+    of full resolution over the calibration views -- on every view
+    (reduce="max") or on average (reduce="mean"): doubled (or halved) from 4
+    until the views cross the target, then bisected geometrically `refine`
+    times.  Returns (factor, RMSE statistic at it); (0, statistic) when even
     the smallest factor misses the target (a binding brick budget)."""
+    if reduce not in ("max", "mean"):
+        raise ValueError("reduce must be 'max' or 'mean'")
     refs = []
     for v in views:
         renderer.resize((v["width"], v["height"]))
@@ -505,16 +509,16 @@ def calibrate_safety(renderer, pred, views, dims, edge, n_lod, hist, moments, ta
         refs.append(full_resolution(renderer, dims, edge))
 
     def worst_at(safety):
-        worst = 0.0
+        errs = []
         for v, ref in zip(views, refs):
             renderer.resize((v["width"], v["height"]))
             set_view(renderer, v)
             lev, _, _ = choose_levels(pred, v, dims, edge, n_lod, hist, moments, safety,
                                       brick_bytes, mem_budget, detail)
             renderer.setRegionLODs(lev)
-            worst = max(worst, rmse255(renderer.capture(), ref))
+            errs.append(rmse255(renderer.capture(), ref))
             renderer.setRegionLODs(None)
-        return worst
+        return max(errs) if reduce == "max" else float(np.mean(errs))
 
     s = 4.0
     w = worst_at(s)

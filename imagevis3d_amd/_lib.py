@@ -72,6 +72,17 @@ class Tuning(C.Structure):
     ]
 
 
+class TocInfo(C.Structure):
+    """tvk_toc_info."""
+    _fields_ = [
+        ("dims", C.c_uint64 * 3), ("bits", C.c_uint32), ("max_brick", C.c_uint32),
+        ("overlap", C.c_uint32), ("n_lod", C.c_uint32), ("use_median", C.c_uint32),
+        ("clamp_to_edge", C.c_uint32), ("compression", C.c_uint32), ("level", C.c_uint32),
+        ("layout", C.c_uint32), ("reserved", C.c_uint32), ("n_bricks", C.c_uint64),
+        ("stored_bytes", C.c_uint64), ("raw_bytes", C.c_uint64),
+    ]
+
+
 class Request(C.Structure):
     """tvk_request: one render request of the server."""
     _fields_ = [("volume", C.c_uint32), ("render_mode", C.c_int32), ("view", View)]
@@ -160,6 +171,15 @@ SIGNATURES = {
     "tvk_lod_mlp_predict": [C.c_void_p, P, u64, P, C.POINTER(f32)],
     "tvk_lod_mlp_destroy": [C.c_void_p],
     "tvk_brick_slot_bytes": [u32, u32, u32],
+    "tvk_toc_build": [i32, P, u64, u64, u64, u32, u32, u32, i32, i32, u32, u32, u32, C.c_char_p,
+                      C.POINTER(TocInfo)],
+    "tvk_toc_open": [C.c_char_p, C.POINTER(C.c_void_p)],
+    "tvk_toc_close": [C.c_void_p],
+    "tvk_toc_get_info": [C.c_void_p, C.POINTER(TocInfo)],
+    "tvk_toc_get_lod": [C.c_void_p, u32, P, P],
+    "tvk_toc_read_brick": [C.c_void_p, u32, u32, u32, u32, P, u64, C.POINTER(u32), C.POINTER(f32),
+                           C.POINTER(f32)],
+    "tvk_set_toc_volume": [H, C.c_void_p, u64],
     "tvk_lod_features": [C.POINTER(View), P, u32, u32, P, P, P, P, P, P],
     "tvk_lod_select_octree": [P, P, P, u32, C.c_double, u64, u64, P, C.POINTER(C.c_double),
                               C.POINTER(u64)],

@@ -25,6 +25,7 @@ SOURCES = [
     "autotune.cpp",
     "server.cpp",
     "lod_select.cpp",
+    "toc.cpp",
     "host_volume.cpp",
     "uvf.cpp",
     "volume_gen.cpp",
@@ -57,7 +58,7 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", tmp, *objs, "-lpthread"]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", tmp, *objs, "-lpthread", "-lz", "-ldl"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

@@ -9,9 +9,10 @@
 namespace tvk {
 
 // Level l+1 from level l by 2x2x2 integer mean: floor(sum / count) over the
-// existing children.  dst dims are ceil(src / 2).
+// existing children, or (median) the lower median of the existing children.
+// dst dims are ceil(src / 2).
 hipError_t launch_downsample(const void* src, int bpv, const int64_t src_dims[3], void* dst,
-                             hipStream_t stream);
+                             hipStream_t stream, bool median = false);
 
 // Cut `n_jobs` bricks of payload B with apron a out of a flat level into
 // pool slots (blocked slot layout, clamp-to-edge outside the volume) and
@@ -22,6 +23,14 @@ hipError_t launch_extract_bricks(const void* level, int bpv, const int64_t dims[
                                  const BrickJob* jobs, int n_jobs, void* pool,
                                  int64_t slot_stride, int B, int apron, float* brick_range,
                                  uint32_t* scratch_minmax, hipStream_t stream);
+
+// Overlap-inclusive x-fastest bricks (S = P + 2o per axis) of a flat level
+// for the TOC builder: job j (its `slot` field = index into `out` and into
+// minmax) is brick (bx, by, bz); outside the level the edge voxel repeats
+// (clamp) or 0 is stored.  minmax gets 2 u32 (min, max) per job.
+hipError_t launch_extract_linear_bricks(const void* level, int bpv, const int64_t dims[3],
+                                        const BrickJob* jobs, int n_jobs, void* out, int P, int o,
+                                        bool clamp, uint32_t* minmax, hipStream_t stream);
 
 // An x-fastest S^3 brick (src) into one slot of the blocked slot layout,
 // and one slot back into an x-fastest S^3 brick (dst).

@@ -17,7 +17,11 @@ namespace {
 
 constexpr int kThreads = 256;
 
-template <typename T>
+// 2x2x2 reduction of the existing children of each parent voxel: integer
+// mean floor(sum / count) (RasterDataBlock's CombineAverage rule), or with
+// MEDIAN the lower median -- element (count-1)/2 of the sorted children
+// (the UseMedian option of the TOC builder).
+template <typename T, bool MEDIAN>
 __global__ __launch_bounds__(kThreads) void downsample_kernel(const T* __restrict__ src, int64_t nx,
                                                               int64_t ny, int64_t nz,
                                                               T* __restrict__ dst, int64_t mx,
@@ -33,6 +37,7 @@ __global__ __launch_bounds__(kThreads) void downsample_kernel(const T* __restric
     const int64_t ys[2] = {2 * y, 2 * y + 1 < ny ? 2 * y + 1 : -1};
     const int64_t zs[2] = {2 * z, 2 * z + 1 < nz ? 2 * z + 1 : -1};
     uint32_t sum = 0, cnt = 0;
+    uint32_t v[8];
 #pragma unroll
     for (int dz = 0; dz < 2; ++dz) {
       if (zs[dz] < 0) continue;
@@ -40,15 +45,36 @@ __global__ __launch_bounds__(kThreads) void downsample_kernel(const T* __restric
       for (int dy = 0; dy < 2; ++dy) {
         if (ys[dy] < 0) continue;
         const T* r = src + (zs[dz] * ny + ys[dy]) * nx;
-        sum += r[x0];
-        ++cnt;
-        if (x1 >= 0) {
-          sum += r[x1];
+        if (MEDIAN) {
+          v[cnt++] = r[x0];
+          if (x1 >= 0) v[cnt++] = r[x1];
+        } else {
+          sum += r[x0];
           ++cnt;
+          if (x1 >= 0) {
+            sum += r[x1];
+            ++cnt;
+          }
         }
       }
     }
-    dst[(z * my + y) * mx + x] = (T)(sum / cnt);
+    uint32_t out;
+    if (MEDIAN) {
+      // insertion sort of the (at most 8) children
+      for (uint32_t i = 1; i < cnt; ++i) {
+        const uint32_t key = v[i];
+        uint32_t j = i;
+        while (j > 0 && v[j - 1] > key) {
+          v[j] = v[j - 1];
+          --j;
+        }
+        v[j] = key;
+      }
+      out = v[(cnt - 1) / 2];
+    } else {
+      out = sum / cnt;
+    }
+    dst[(z * my + y) * mx + x] = (T)out;
   }
 }
 
@@ -117,6 +143,47 @@ __global__ __launch_bounds__(kThreads) void extract_bricks_kernel(
   if ((threadIdx.x & 63) == 0) {
     atomicMin(&minmax[2 * blockIdx.y], lo);
     atomicMax(&minmax[2 * blockIdx.y + 1], hi);
+  }
+}
+
+// Overlap-inclusive bricks of a flat level, x fastest (the TOC builder's
+// stored bricks): job j's brick covers level voxels [b*P - o, b*P - o + S)
+// per axis (S = P + 2o); voxels outside the level repeat the nearest edge
+// voxel (clamp) or are 0.  Brick j goes to out + j * S^3 (j = the job's
+// slot field); its (min, max) over the stored voxels to minmax[2j], [2j+1].
+// grid.x: chunks of one brick, grid.y: job.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void extract_linear_bricks_kernel(
+    const T* __restrict__ level, int64_t nx, int64_t ny, int64_t nz,
+    const BrickJob* __restrict__ jobs, T* __restrict__ out, int P, int o, int clamp,
+    uint32_t* __restrict__ minmax) {
+  const BrickJob job = jobs[blockIdx.y];
+  const int64_t S = P + 2 * o;
+  const int64_t n = S * S * S;
+  const int64_t ox = (int64_t)job.bx * P - o, oy = (int64_t)job.by * P - o, oz = (int64_t)job.bz * P - o;
+  T* __restrict__ dst = out + job.slot * n;
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t beg = chunk * blockIdx.x;
+  const int64_t end = beg + chunk < n ? beg + chunk : n;
+  uint32_t lo = 0xffffffffu, hi = 0;
+  for (int64_t i = beg + threadIdx.x; i < end; i += kThreads) {
+    const int64_t gx = ox + i % S, gy = oy + (i / S) % S, gz = oz + i / (S * S);
+    T v;
+    if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz)
+      v = level[(gz * ny + gy) * nx + gx];
+    else if (clamp)
+      v = level[(clamp64(gz, 0, nz - 1) * ny + clamp64(gy, 0, ny - 1)) * nx + clamp64(gx, 0, nx - 1)];
+    else
+      v = (T)0;
+    dst[i] = v;
+    lo = min(lo, (uint32_t)v);
+    hi = max(hi, (uint32_t)v);
+  }
+  lo = wave_min(lo);
+  hi = wave_max(hi);
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(&minmax[2 * job.slot], lo);
+    atomicMax(&minmax[2 * job.slot + 1], hi);
   }
 }
 
@@ -335,15 +402,24 @@ hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long 
 }
 
 hipError_t launch_downsample(const void* src, int bpv, const int64_t d[3], void* dst,
-                             hipStream_t stream) {
+                             hipStream_t stream, bool median) {
   const int64_t mx = (d[0] + 1) / 2, my = (d[1] + 1) / 2, mz = (d[2] + 1) / 2;
   const dim3 grid = row_grid(mx, my * mz);
-  if (bpv == 1)
-    downsample_kernel<uint8_t><<<grid, kThreads, 0, stream>>>((const uint8_t*)src, d[0], d[1], d[2],
-                                                              (uint8_t*)dst, mx, my, mz);
-  else
-    downsample_kernel<uint16_t><<<grid, kThreads, 0, stream>>>(
-        (const uint16_t*)src, d[0], d[1], d[2], (uint16_t*)dst, mx, my, mz);
+  if (bpv == 1) {
+    if (median)
+      downsample_kernel<uint8_t, true><<<grid, kThreads, 0, stream>>>(
+          (const uint8_t*)src, d[0], d[1], d[2], (uint8_t*)dst, mx, my, mz);
+    else
+      downsample_kernel<uint8_t, false><<<grid, kThreads, 0, stream>>>(
+          (const uint8_t*)src, d[0], d[1], d[2], (uint8_t*)dst, mx, my, mz);
+  } else {
+    if (median)
+      downsample_kernel<uint16_t, true><<<grid, kThreads, 0, stream>>>(
+          (const uint16_t*)src, d[0], d[1], d[2], (uint16_t*)dst, mx, my, mz);
+    else
+      downsample_kernel<uint16_t, false><<<grid, kThreads, 0, stream>>>(
+          (const uint16_t*)src, d[0], d[1], d[2], (uint16_t*)dst, mx, my, mz);
+  }
   return hipGetLastError();
 }
 
@@ -374,6 +450,30 @@ hipError_t launch_extract_bricks(const void* level, int bpv, const int64_t d[3],
   minmax_finish_kernel<<<(n_jobs + 255) / 256, 256, 0, stream>>>(scratch, jobs, n_jobs,
                                                                   brick_range);
   return hipGetLastError();
+}
+
+hipError_t launch_extract_linear_bricks(const void* level, int bpv, const int64_t d[3],
+                                        const BrickJob* jobs, int n_jobs, void* out, int P, int o,
+                                        bool clamp, uint32_t* minmax, hipStream_t stream) {
+  if (n_jobs <= 0) return hipSuccess;
+  minmax_init_kernel<<<(n_jobs + 255) / 256, 256, 0, stream>>>(minmax, n_jobs);
+  const int64_t S = P + 2 * o;
+  int64_t chunks = (S * S * S + 8191) / 8192;
+  if (chunks > 1024) chunks = 1024;
+  if (chunks < 1) chunks = 1;
+  for (int j0 = 0; j0 < n_jobs; j0 += 65535) {
+    const int nj = n_jobs - j0 < 65535 ? n_jobs - j0 : 65535;
+    const dim3 grid((unsigned)chunks, (unsigned)nj, 1);
+    if (bpv == 1)
+      extract_linear_bricks_kernel<uint8_t><<<grid, kThreads, 0, stream>>>(
+          (const uint8_t*)level, d[0], d[1], d[2], jobs + j0, (uint8_t*)out, P, o, clamp ? 1 : 0, minmax);
+    else
+      extract_linear_bricks_kernel<uint16_t><<<grid, kThreads, 0, stream>>>(
+          (const uint16_t*)level, d[0], d[1], d[2], jobs + j0, (uint16_t*)out, P, o, clamp ? 1 : 0, minmax);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_linear_to_slot(const void* src, void* slot, int bpv, int S, hipStream_t stream) {

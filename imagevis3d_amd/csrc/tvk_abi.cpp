@@ -2,14 +2,17 @@
 // Each function forwards to the AbstrRenderer-shaped HipRenderer; C++
 // exceptions never cross the boundary.
 #include <algorithm>
+#include <memory>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/tvk_hip.h"
 #include "errors.hpp"
 #include "kernels.hpp"
 #include "renderer.hpp"
 #include "server.hpp"
+#include "toc.hpp"
 
 namespace tvk {
 namespace {
@@ -27,6 +30,10 @@ struct tvk_renderer {
 
 struct tvk_server {
   tvk::RenderServer impl;
+};
+
+struct tvk_toc {
+  tvk::TocVolume impl;
 };
 
 struct tvk_lod_mlp {
@@ -222,6 +229,107 @@ int tvk_read_brick(tvk_renderer* r, uint32_t lod, uint32_t bx, uint32_t by, uint
                    uint64_t bytes) {
   TVK_GUARD(r);
   TVK_TRY(r->impl.ReadBrick(lod, bx, by, bz, out, bytes));
+}
+
+int tvk_toc_build(int device, const void* voxels, uint64_t nx, uint64_t ny, uint64_t nz, uint32_t bits,
+                  uint32_t max_brick, uint32_t overlap, int use_median, int clamp_to_edge,
+                  uint32_t compression, uint32_t level, uint32_t layout, const char* path,
+                  tvk_toc_info* info) {
+  if (!path) {
+    tvk::set_last_error("null path");
+    return TVK_ERR_ARG;
+  }
+  const uint64_t d[3] = {nx, ny, nz};
+  TVK_TRY(tvk::toc_build(device, voxels, d, bits, max_brick, overlap, use_median != 0, clamp_to_edge != 0,
+                         compression, level, layout, path, info));
+}
+
+int tvk_toc_open(const char* path, tvk_toc** out) {
+  if (!path || !out) {
+    tvk::set_last_error("null argument");
+    return TVK_ERR_ARG;
+  }
+  *out = nullptr;
+  try {
+    std::unique_ptr<tvk_toc> t(new tvk_toc);
+    if (int e = t->impl.Open(path)) return e;
+    *out = t.release();
+    return TVK_OK;
+  } catch (const std::bad_alloc&) {
+    tvk::set_last_error("host allocation failed");
+    return TVK_ERR_OOM;
+  }
+}
+
+int tvk_toc_close(tvk_toc* t) {
+  delete t;
+  return TVK_OK;
+}
+
+int tvk_toc_get_info(tvk_toc* t, tvk_toc_info* info) {
+  if (!t || !info) {
+    tvk::set_last_error("null argument");
+    return TVK_ERR_ARG;
+  }
+  t->impl.Info(info);
+  return TVK_OK;
+}
+
+int tvk_toc_get_lod(tvk_toc* t, uint32_t lod, uint64_t dims[3], uint64_t grid[3]) {
+  if (!t || !dims || !grid) {
+    tvk::set_last_error("null argument");
+    return TVK_ERR_ARG;
+  }
+  if (lod >= t->impl.n_lod) {
+    tvk::set_last_error("LOD index out of range");
+    return TVK_ERR_ARG;
+  }
+  for (int i = 0; i < 3; ++i) {
+    dims[i] = t->impl.lods[lod].dims[i];
+    grid[i] = t->impl.lods[lod].grid[i];
+  }
+  return TVK_OK;
+}
+
+int tvk_toc_read_brick(tvk_toc* t, uint32_t lod, uint32_t bx, uint32_t by, uint32_t bz, void* out,
+                       uint64_t out_bytes, uint32_t* compression, float* min_value, float* max_value) {
+  if (!t) {
+    tvk::set_last_error("null TOC handle");
+    return TVK_ERR_ARG;
+  }
+  const tvk::TocVolume& T = t->impl;
+  if (lod >= T.n_lod || bx >= T.lods[lod].grid[0] || by >= T.lods[lod].grid[1] || bz >= T.lods[lod].grid[2]) {
+    tvk::set_last_error("brick index out of range");
+    return TVK_ERR_ARG;
+  }
+  const tvk::TocEntry& E =
+      T.entries[T.lods[lod].first + ((uint64_t)bz * T.lods[lod].grid[1] + by) * T.lods[lod].grid[0] + bx];
+  if (compression) *compression = E.compression;
+  if (min_value) *min_value = E.mn;
+  if (max_value) *max_value = E.mx;
+  if (!out) return TVK_OK;
+  TVK_TRY(t->impl.ReadBrick(lod, bx, by, bz, out, out_bytes));
+}
+
+static int toc_brick_source(void* user, uint32_t lod, uint32_t bx, uint32_t by, uint32_t bz, void* slot,
+                            uint64_t slot_bytes) {
+  return static_cast<tvk::TocVolume*>(user)->ReadBrick(lod, bx, by, bz, slot, slot_bytes) == TVK_OK ? 0 : -1;
+}
+
+int tvk_set_toc_volume(tvk_renderer* r, tvk_toc* t, uint64_t pool_bytes) {
+  TVK_GUARD(r);
+  if (!t) {
+    tvk::set_last_error("null TOC handle");
+    return TVK_ERR_ARG;
+  }
+  const tvk::TocVolume& T = t->impl;
+  std::vector<float> ranges(T.entries.size() * 2);
+  for (size_t i = 0; i < T.entries.size(); ++i) {
+    ranges[2 * i] = T.entries[i].mn;
+    ranges[2 * i + 1] = T.entries[i].mx;
+  }
+  TVK_TRY(r->impl.SetPagedVolume(T.dims, T.bits, T.payload, T.overlap, T.n_lod, &toc_brick_source, &t->impl,
+                                 ranges.data(), pool_bytes));
 }
 
 uint64_t tvk_brick_slot_bytes(uint32_t brick_size, uint32_t apron, uint32_t bits) {

@@ -174,6 +174,14 @@ class Renderer:
         self._paged_keep = vol
         self._bits = bits
 
+    def setTOCVolume(self, toc, pool_bytes: int) -> None:
+        """Page a TOC volume (imagevis3d_amd.toc.TocVolume, kept open by the
+        renderer) into a brick pool of `pool_bytes`: bricks of the TOC's
+        payload with its overlap as apron, its LODs and brick ranges."""
+        call("tvk_set_toc_volume", self._h, toc._h, int(pool_bytes))
+        self._paged_keep = toc
+        self._bits = toc.info["bits"]
+
     def setPagedVolume(self, dims, bits: int, brick_size: int, source, pool_bytes: int,
                        brick_ranges=None, apron: int = 2, n_lod: int = 0) -> None:
         """Out-of-core volume from a brick source:

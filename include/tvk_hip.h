@@ -186,6 +186,57 @@ int tvk_get_brick_minmax(tvk_renderer* r, uint32_t lod, float* mins, float* maxs
  * (B+2a)^3 voxels in 4x4x4 blocks whose x-rows carry one extra voxel
  * (1.25x the voxels), 256-byte aligned; 0 for invalid arguments. */
 uint64_t tvk_brick_slot_bytes(uint32_t brick_size, uint32_t apron, uint32_t bits);
+
+/* ------------------------------------------------- TOC (UVF v3) volumes */
+/* Bricked multi-resolution volume files: TOCBlock::FlatDataToBrickedLOD
+ * (UVFReader/DataSource.h:475-491, options of ImageVis3D_Settings.cpp:
+ * 516-548 and uvfconvert, UVFReader/main.cpp:51-52,86-109) and the TOC
+ * queries GetLoDCount / GetMaxBrickSize / GetLODDomainSize / GetBrickCount /
+ * GetBrickInfo (UVFReader/BlockInfo.h:38-96).
+ *
+ * A file is a UVF version-3 container (master header and one data block,
+ * doc/import.adoc:904-1102) whose TOC block stores every LOD as bricks of
+ * max_brick^3 voxels that include `overlap` voxels on each side (payload
+ * max_brick - 2*overlap; ExtendedOctree's convention), x fastest.  LOD l+1
+ * halves LOD l (ceil) by the 2x2x2 integer mean (RasterDataBlock's rule)
+ * or, with use_median, the lower median of the existing children; levels
+ * are added until the largest dimension fits one payload.  Voxels outside
+ * the volume repeat the edge voxel (clamp_to_edge) or are 0.  Bricks are
+ * compressed one by one -- compression 0 none, 1 zlib, 2 lzma (xz), 3 lz4
+ * (block), 4 bzip2, uvfconvert's numbering, level 1..10 -- and stored in
+ * the order of `layout`: 0 scanline, 1 Morton, 2 Hilbert, 3 random.  The
+ * block payload's byte layout is this library's (ExtendedOctree's is not in
+ * the reference tree).  The pyramid and the bricks are built on the GPU,
+ * compression runs on host threads. */
+typedef struct tvk_toc tvk_toc;
+typedef struct tvk_toc_info {
+  uint64_t dims[3];
+  uint32_t bits, max_brick, overlap, n_lod;
+  uint32_t use_median, clamp_to_edge, compression, level, layout;
+  uint32_t reserved;
+  uint64_t n_bricks;
+  uint64_t stored_bytes;  /* bytes of all stored bricks */
+  uint64_t raw_bytes;     /* the same bricks uncompressed */
+} tvk_toc_info;
+int tvk_toc_build(int device, const void* voxels, uint64_t nx, uint64_t ny, uint64_t nz,
+                  uint32_t bits, uint32_t max_brick, uint32_t overlap, int use_median,
+                  int clamp_to_edge, uint32_t compression, uint32_t level, uint32_t layout,
+                  const char* path, tvk_toc_info* info);
+int tvk_toc_open(const char* path, tvk_toc** out);
+int tvk_toc_close(tvk_toc* t);
+int tvk_toc_get_info(tvk_toc* t, tvk_toc_info* info);
+/* GetLODDomainSize(lod) and GetBrickCount(lod). */
+int tvk_toc_get_lod(tvk_toc* t, uint32_t lod, uint64_t dims[3], uint64_t grid[3]);
+/* GetBrickInfo + the brick's voxels (max_brick^3 * bpv bytes, decompressed)
+ * and its value range; any pointer may be NULL. */
+int tvk_toc_read_brick(tvk_toc* t, uint32_t lod, uint32_t bx, uint32_t by, uint32_t bz, void* out,
+                       uint64_t out_bytes, uint32_t* compression, float* min_value,
+                       float* max_value);
+/* Page a TOC volume into the handle's HBM brick pool (the out-of-core
+ * GridLeaper path of tvk_set_paged_volume with the TOC as brick source:
+ * brick payload and apron = the TOC's payload and overlap).  `t` must stay
+ * open while the volume is in use. */
+int tvk_set_toc_volume(tvk_renderer* r, tvk_toc* t, uint64_t pool_bytes);
 /* Histogram1DDataBlock::Compute over LOD 0 (UVFReader/DataSource.h:595-610):
  * n_bins = 2^bits (u64 counts). */
 int tvk_compute_histogram_1d(tvk_renderer* r, uint64_t* hist, uint64_t n_bins);

@@ -420,6 +420,41 @@ void oracle_downsample(const void* src, int32_t bits, int32_t nx, int32_t ny, in
       }
 }
 
+/* Median variant of the 2x2x2 reduction (the TOC builder's UseMedian
+ * option, ImageVis3D_Settings.cpp:217,521; Tuvok's median rule is in the
+ * absent ExtendedOctree source): the lower median of the existing children,
+ * element (n-1)/2 of the n sorted values. */
+void oracle_downsample_median(const void* src, int32_t bits, int32_t nx, int32_t ny, int32_t nz,
+                              void* dst) {
+  int mx = (nx + 1) / 2, my = (ny + 1) / 2, mz = (nz + 1) / 2;
+  for (int z = 0; z < mz; ++z)
+    for (int y = 0; y < my; ++y)
+      for (int x = 0; x < mx; ++x) {
+        uint32_t v[8];
+        int n = 0;
+        for (int dz = 0; dz < 2; ++dz)
+          for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+              int sx = 2 * x + dx, sy = 2 * y + dy, sz = 2 * z + dz;
+              if (sx >= nx || sy >= ny || sz >= nz) continue;
+              int64_t i = sx + (int64_t)nx * (sy + (int64_t)ny * sz);
+              v[n++] = bits == 8 ? ((const uint8_t*)src)[i] : ((const uint16_t*)src)[i];
+            }
+        /* selection of the (n-1)/2-th smallest by repeated minimum */
+        for (int k = 0; k <= (n - 1) / 2; ++k) {
+          int m = k;
+          for (int j = k + 1; j < n; ++j)
+            if (v[j] < v[m]) m = j;
+          uint32_t t = v[k];
+          v[k] = v[m];
+          v[m] = t;
+        }
+        int64_t o = x + (int64_t)mx * (y + (int64_t)my * z);
+        if (bits == 8) ((uint8_t*)dst)[o] = (uint8_t)v[(n - 1) / 2];
+        else ((uint16_t*)dst)[o] = (uint16_t)v[(n - 1) / 2];
+      }
+}
+
 void oracle_extract_brick(const void* src, int32_t bits, int32_t nx, int32_t ny, int32_t nz,
                           int32_t B, int32_t a, int32_t bx, int32_t by, int32_t bz, void* slot) {
   int S = B + 2 * a;

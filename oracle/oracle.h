@@ -108,6 +108,10 @@ uint64_t baseline_render_1d_f32(const oracle_render_params* p, uint8_t* out_rgba
  * Output dims are ceil(n/2). */
 void oracle_downsample(const void* src, int32_t bits, int32_t nx, int32_t ny, int32_t nz,
                        void* dst);
+/* Lower median of the existing children of each 2x2x2 group (TOC builder's
+ * median option). */
+void oracle_downsample_median(const void* src, int32_t bits, int32_t nx, int32_t ny, int32_t nz,
+                              void* dst);
 
 /* Copy brick (bx,by,bz) of payload B with `apron` voxels per side into a
  * (B+2a)^3 slot; voxels outside the volume replicate the nearest edge

@@ -74,6 +74,8 @@ def lib():
         L.baseline_render_1d_f32.restype = C.c_uint64
         L.oracle_downsample.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
                                         C.c_int32, C.c_void_p]
+        L.oracle_downsample_median.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                               C.c_int32, C.c_void_p]
         L.oracle_extract_brick.argtypes = [C.c_void_p, C.c_int32] + [C.c_int32] * 8 + [C.c_void_p]
         L.oracle_brick_minmax.argtypes = [C.c_void_p, C.c_int32] + [C.c_int32] * 8 + [
             C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -205,11 +207,13 @@ def baseline_render(p: RenderParams, threads: int = 0):
 
 
 # ------------------------------------------------------------ bricking / LOD
-def downsample(vol: np.ndarray) -> np.ndarray:
+def downsample(vol: np.ndarray, median: bool = False) -> np.ndarray:
+    """One 2x2x2 reduction: integer mean, or the lower median (median=True)."""
     vol = np.ascontiguousarray(vol)
     nz, ny, nx = vol.shape
     out = np.zeros(((nz + 1) // 2, (ny + 1) // 2, (nx + 1) // 2), vol.dtype)
-    lib().oracle_downsample(_ptr(vol), bits_of(vol), nx, ny, nz, _ptr(out))
+    fn = lib().oracle_downsample_median if median else lib().oracle_downsample
+    fn(_ptr(vol), bits_of(vol), nx, ny, nz, _ptr(out))
     return out
 
 

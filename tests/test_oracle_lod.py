@@ -29,6 +29,41 @@ def test_downsample_brute_force(shape, dt):
     assert np.array_equal(O.downsample(v), brute_downsample(v))
 
 
+def brute_median(v):
+    nz, ny, nx = v.shape
+    out = np.zeros(((nz + 1) // 2, (ny + 1) // 2, (nx + 1) // 2), v.dtype)
+    for z in range(out.shape[0]):
+        for y in range(out.shape[1]):
+            for x in range(out.shape[2]):
+                blk = np.sort(v[2 * z:2 * z + 2, 2 * y:2 * y + 2, 2 * x:2 * x + 2].ravel())
+                out[z, y, x] = blk[(blk.size - 1) // 2]
+    return out
+
+
+@pytest.mark.parametrize("shape", [(3, 3, 3), (4, 5, 6), (9, 7, 8), (1, 2, 5), (2, 2, 2), (1, 1, 1)])
+@pytest.mark.parametrize("dt", [np.uint8, np.uint16])
+def test_downsample_median_brute_force(shape, dt):
+    """The median rule: element (n-1)/2 of the sorted existing children, so
+    the lower of the two middle values for 8 (and 2, 4) children."""
+    rng = np.random.default_rng(3 + sum(shape))
+    v = rng.integers(0, np.iinfo(dt).max + 1, shape).astype(dt)
+    assert np.array_equal(O.downsample(v, median=True), brute_median(v))
+
+
+def test_downsample_median_cases():
+    # 8 distinct children 0..7 -> the 4th smallest (3), not the mean (3.5)
+    w = np.arange(8, dtype=np.uint8).reshape(2, 2, 2)
+    assert O.downsample(w, median=True)[0, 0, 0] == 3
+    # a single outlier does not move the median, it does move the mean
+    w = np.full((2, 2, 2), 10, np.uint16)
+    w[1, 1, 1] = 60000
+    assert O.downsample(w, median=True)[0, 0, 0] == 10
+    assert O.downsample(w)[0, 0, 0] == (7 * 10 + 60000) // 8
+    # constant volumes stay constant, values stay within the children's range
+    c = np.full((7, 6, 5), 99, np.uint8)
+    assert np.all(O.downsample(c, median=True) == 99)
+
+
 def test_downsample_invariants():
     rng = np.random.default_rng(7)
     c = np.full((13, 11, 9), 4321, np.uint16)

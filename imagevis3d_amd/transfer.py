@@ -4,7 +4,8 @@ the benchmark workloads.
 ``.1dt`` (DemoData/c60.1dt): first line N, then N lines ``r g b a``
 (TransferFunction1D, Q1DTransferFunction.cpp:573-589).  ``.2dt``
 (DemoData/c60.2dt): ``W H``, an embedded 1D TF, then a swatch count and
-swatches (TransferFunction2D, Q2DTransferFunction.cpp:426-470).
+swatches (TransferFunction2D, Q2DTransferFunction.cpp:426-470), rasterised
+by ``rasterize_2dt``.
 """
 from __future__ import annotations
 
@@ -59,6 +60,82 @@ def parse_2dt(path: str) -> dict:
             stops.append((pos, rgba))
         swatches.append({"radial": radial, "points": pts, "gradient": grad, "stops": stops})
     return {"size": (w, h), "tf1d": tf1, "swatches": swatches}
+
+
+def _stop_colours(stops):
+    """Stops sorted by position, colours quantised as the swatch painter
+    does (QColor(int(c * 255)), Q2DTransferFunction.cpp:459-463)."""
+    st = sorted(stops, key=lambda s: s[0])
+    pos = np.asarray([p for p, _ in st], np.float64)
+    col = np.asarray([[int(c * 255) / 255.0 for c in rgba] for _, rgba in st], np.float64)
+    return pos, col
+
+
+def _even_odd(px, py, poly):
+    """Even-odd fill rule: pixel centres (px, py) inside polygon `poly`."""
+    inside = np.zeros(px.shape, bool)
+    n = len(poly)
+    for k in range(n):
+        x0, y0 = poly[k]
+        x1, y1 = poly[(k + 1) % n]
+        if y0 == y1:
+            continue
+        cross = (y0 > py) != (y1 > py)
+        xi = (x1 - x0) * (py - y0) / (y1 - y0) + x0
+        inside ^= cross & (px < xi)
+    return inside
+
+
+def rasterize_2dt(tf2d: dict, size=None) -> np.ndarray:
+    """The 2D TF raster of a parsed .2dt: (H, W, 4) float32 RGBA, column =
+    value bin, row = gradient-magnitude bin (row 0 = gradient 0), the layout
+    set2DTrans takes.
+
+    Follows the swatch painter of Q2DTransferFunction::DrawSwatches
+    (Q2DTransferFunction.cpp:426-470): swatch vertices and gradient end
+    points go to integer raster positions (truncation of x*W, y*H); each
+    swatch fills the pixels whose centres lie inside its polygon (even-odd
+    rule, no antialiasing) with its gradient -- linear from the first to the
+    second gradient point, or radial around the first with the distance to
+    the second as radius; positions outside [0, 1] take the end stops -- at
+    stop colours quantised to 8 bits, interpolated per component; swatches
+    are composited in file order with source-over on a transparent canvas
+    and the result is returned non-premultiplied.  Tuvok's own raster code
+    (TransferFunction2D) is not in the reference tree: the interpolation
+    space and the vertex snapping are this rule's choices."""
+    W, H = size or tf2d["size"]
+    jj, ii = np.mgrid[0:H, 0:W]
+    px, py = ii + 0.5, jj + 0.5
+    acc = np.zeros((H, W, 4), np.float64)  # premultiplied rgb, alpha
+    for sw in tf2d["swatches"]:
+        poly = [(float(int(x * W)), float(int(y * H))) for x, y in sw["points"]]
+        inside = _even_odd(px, py, poly)
+        if not inside.any() or not sw["stops"]:
+            continue
+        g = sw["gradient"]
+        g0 = (float(int(g[0] * W)), float(int(g[1] * H)))
+        g1 = (float(int(g[2] * W)), float(int(g[3] * H)))
+        dx, dy = g1[0] - g0[0], g1[1] - g0[1]
+        if sw["radial"]:
+            r = np.hypot(dx, dy)
+            t = np.hypot(px - g0[0], py - g0[1]) / r if r > 0 else np.ones(px.shape)
+        else:
+            ll = dx * dx + dy * dy
+            t = ((px - g0[0]) * dx + (py - g0[1]) * dy) / ll if ll > 0 else np.zeros(px.shape)
+        t = np.clip(t, 0.0, 1.0)
+        pos, col = _stop_colours(sw["stops"])
+        rgba = np.stack([np.interp(t, pos, col[:, c]) for c in range(4)], -1)
+        a = rgba[..., 3:4]
+        src = np.concatenate([rgba[..., :3] * a, a], -1)
+        m = inside[..., None]
+        acc = np.where(m, src + acc * (1.0 - a), acc)
+    out = np.zeros((H, W, 4), np.float32)
+    alpha = acc[..., 3]
+    nz = alpha > 0
+    out[..., 3] = alpha
+    for c in range(3):
+        out[..., c][nz] = acc[..., c][nz] / alpha[nz]
+    return out
 
 
 def ramp_tf(n: int = 256, lo: float = 40.0, hi: float = 220.0) -> np.ndarray:

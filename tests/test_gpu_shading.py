@@ -107,3 +107,30 @@ def test_isosurface_parity(interp):
     assert r.paint()
     assert np.array_equal(r.capture(), img)
     r.cleanup()
+
+
+@pytest.mark.parametrize("lighting", [False, True])
+def test_c60_with_its_2dt(c60, lighting):
+    """The demo dataset with its own 2D TF file: c60.2dt rasterised
+    (transfer.rasterize_2dt), gradient axis scaled by the dataset's stored
+    maximum gradient (the Histogram2D block), rendered in 2D-TF mode."""
+    print(f"case c60 + c60.2dt lighting={lighting}", flush=True)
+    import os
+    vol = c60["blocks"][0]["volume"]
+    d = transfer.parse_2dt(os.path.join(os.path.dirname(__file__), "golden", "c60.2dt"))
+    tf2 = transfer.rasterize_2dt(d)
+    _, max_grad = O.histogram_2d_exact(vol, 256, 256)
+    views = [Camera((0.0, 0.0, 2.0), (0.0, 0.0, -1.0), fov=50.0),
+             Camera((1.4, 0.6, 1.2), (-1.4, -0.6, -1.2), fov=50.0)]
+    w = workloads.Workload("c60-2dt", (128, 128, 128), 8, 0, 0, 128, 2, 1, None, 200, 150, 1.5,
+                           views, bg_top=(0.1, 0.2, 0.3), bg_bottom=(0.0, 0.0, 0.0))
+    r = make_renderer(w, vol, lighting=lighting)
+    r.set2DTrans(tf2, max_grad)
+    r.setRenderMode(RenderMode.RM_2DTRANS)
+    for i, cam in enumerate(views):
+        img, st = render(r, w, cam)
+        ref8, ns, fl = oracle_ref(vol, w, cam, None, mode=1, tf2d=tf2, max_grad=max_grad,
+                                  lighting=lighting, **LIGHT)
+        assert (ref8[..., 3] > 0).any()
+        assert_parity(img, ref8, fl, f"c60 2dt view {i} lighting={lighting}")
+    r.cleanup()

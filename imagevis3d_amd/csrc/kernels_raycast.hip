@@ -61,23 +61,38 @@ __device__ inline unsigned long long wave_atomic_add(unsigned long long* p, unsi
   return __hip_atomic_fetch_add((global_u64*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Voxel loads go through global-address-space pointers: the pool pointer
+// arrives in a generic pointer field of the job table, and a generic load
+// would compile to a FLAT instruction, which also counts against the LDS
+// wait counter that the TF lookups in LDS wait on.
+typedef __attribute__((address_space(1))) const uint8_t global_u8;
+typedef __attribute__((address_space(1))) const uint16_t global_u16;
+typedef __attribute__((address_space(1))) const uint16_t __attribute__((aligned(1))) global_u16_a1;
+typedef __attribute__((address_space(1))) const uint32_t __attribute__((aligned(2))) global_u32_a2;
+
+// Load / store of one element through a global pointer.
 template <typename T>
-__device__ inline float ld(const T* p) {
-  return (float)*p;
+__device__ inline T gload(const T* p) {
+  return *(__attribute__((address_space(1))) const T*)p;
 }
+template <typename T>
+__device__ inline void gstore(T* p, T v) {
+  *(__attribute__((address_space(1))) T*)p = v;
+}
+
+__device__ inline float ld(const uint8_t* p) { return (float)*(global_u8*)p; }
+__device__ inline float ld(const uint16_t* p) { return (float)*(global_u16*)p; }
 
 // Two x-neighbours (p[0], p[1]) with one load: a dword for 16-bit voxels, a
 // short for 8-bit ones (unaligned global loads are legal on gfx950), which
 // halves the vector-memory instructions of a trilinear fetch.
 __device__ inline void ld2(const uint16_t* p, float& a, float& b) {
-  uint32_t v;
-  __builtin_memcpy(&v, p, 4);
+  const uint32_t v = *(global_u32_a2*)p;
   a = (float)(v & 0xffffu);
   b = (float)(v >> 16);
 }
 __device__ inline void ld2(const uint8_t* p, float& a, float& b) {
-  uint16_t v;
-  __builtin_memcpy(&v, p, 2);
+  const uint16_t v = *(global_u16_a1*)p;
   a = (float)(v & 0xffu);
   b = (float)(v >> 8);
 }
@@ -209,7 +224,7 @@ __device__ inline float fetch_level(const RaycastParams& P, float qx, float qy, 
     g[i] = qi - (float)(b[i] * P.brick - P.apron) - (LINEAR ? 0.5f : 0.f);
   }
   const int gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
-  const int slot = P.page_table[gid];
+  const int slot = gload(P.page_table + gid);
   if (slot < 0) {  // paged pool: request the brick; the frame is re-rendered
     if (P.requests) P.requests[gid] = 1u;
     return 0.f;
@@ -219,13 +234,28 @@ __device__ inline float fetch_level(const RaycastParams& P, float qx, float qy, 
   return fetch<T, LINEAR>(base, slot_geo(P.slot_edge), g[0], g[1], g[2]);
 }
 
+// TF entry i: from LDS when the launch staged the TF there, else through a
+// global pointer.
+typedef float native_f4 __attribute__((ext_vector_type(4)));
+__device__ inline float4 gload_f4(const float4* p) {
+  const native_f4 v = *(__attribute__((address_space(1))) const native_f4*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
+template <bool TF_LDS>
+__device__ inline float4 tf_entry(const float4* __restrict__ tf4, int i) {
+  if constexpr (TF_LDS) return tf4[i];
+  else return gload_f4(tf4 + i);
+}
+
 // 1D TF: linear between the two entries around value * tf_scale.
+template <bool TF_LDS>
 __device__ inline void tf1d_lookup(const RaycastParams& P, const float4* __restrict__ tf4, float v,
                                    float c[4]) {
   const float x = clampf(v * P.tf_scale, 0.f, (float)(P.n_tf - 1));
   const int i = (int)x;
   const float f = x - (float)i;
-  const float4 e0 = tf4[i], e1 = tf4[min(i + 1, P.n_tf - 1)];
+  const float4 e0 = tf_entry<TF_LDS>(tf4, i), e1 = tf_entry<TF_LDS>(tf4, min(i + 1, P.n_tf - 1));
   c[0] = fmaf(e1.x - e0.x, f, e0.x);
   c[1] = fmaf(e1.y - e0.y, f, e0.y);
   c[2] = fmaf(e1.z - e0.z, f, e0.z);
@@ -241,7 +271,8 @@ __device__ inline void tf2d_lookup(const RaycastParams& P, float v, float gm, fl
   const float fx = x - (float)i, fy = y - (float)j;
   const int i1 = min(i + 1, W - 1), j1 = min(j + 1, H - 1);
   const float4* t = (const float4*)P.tf2d;
-  const float4 a = t[j * W + i], b = t[j * W + i1], e = t[j1 * W + i], f = t[j1 * W + i1];
+  const float4 a = gload_f4(t + (j * W + i)), b = gload_f4(t + (j * W + i1));
+  const float4 e = gload_f4(t + (j1 * W + i)), f = gload_f4(t + (j1 * W + i1));
   const float ax[4] = {a.x, a.y, a.z, a.w}, bx[4] = {b.x, b.y, b.z, b.w};
   const float ex[4] = {e.x, e.y, e.z, e.w}, fx4[4] = {f.x, f.y, f.z, f.w};
 #pragma unroll
@@ -271,7 +302,7 @@ __device__ inline void shade(const RaycastParams& P, const float g[3], const flo
 // or skipped when region_code[r] & 0x80 marks the level-l brick as
 // transparent under the TF.  Returns the region that added the most
 // opacity (-1 if none) for error attribution.
-template <typename T, bool LINEAR, int BATCH>
+template <typename T, bool LINEAR, int BATCH, bool TF_LDS>
 __device__ inline int march_regions(const RaycastParams& P, const float4* __restrict__ tf4,
                                     const float o[3], const float d[3], const float inv[3], float t0,
                                     float t1, float& C0, float& C1, float& C2, float& A,
@@ -298,7 +329,7 @@ __device__ inline int march_regions(const RaycastParams& P, const float4* __rest
       }
     tb = fminf(fmaxf(tb, t + 1e-3f), t1);  // always advance
     const int region = (c[2] * L0.grid[1] + c[1]) * L0.grid[0] + c[0];
-    const uint32_t code = P.region_code[region];
+    const uint32_t code = gload(P.region_code + region);
     if (!(code & 0x80u)) {
       const int l = (int)(code & 15u);
       const LodInfo& lv = P.lods[l];
@@ -314,7 +345,7 @@ __device__ inline int march_regions(const RaycastParams& P, const float4* __rest
         g0[i] = oo[i] - (float)(b[i] * P.brick - P.apron) - (LINEAR ? 0.5f : 0.f);
       }
       const int gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
-      const int slot = P.page_table[gid];
+      const int slot = gload(P.page_table + gid);
       if (slot < 0) {
         if (P.requests) P.requests[gid] = 1u;
       } else {
@@ -343,7 +374,8 @@ __device__ inline int march_regions(const RaycastParams& P, const float4* __rest
               const float x = clampf(v[j] * P.tf_scale, 0.f, tfmax);
               const int ti = (int)x;
               const float fr = x - (float)ti;
-              const float4 e0 = tf4[ti], e1 = tf4[min(ti + 1, P.n_tf - 1)];
+              const float4 e0 = tf_entry<TF_LDS>(tf4, ti);
+              const float4 e1 = tf_entry<TF_LDS>(tf4, min(ti + 1, P.n_tf - 1));
               const float a = fmaf(e1.w - e0.w, fr, e0.w);
               if (a > 0.f) {
                 const float len = (k + j == m - 1) ? last : step;
@@ -369,7 +401,7 @@ __device__ inline int march_regions(const RaycastParams& P, const float4* __rest
   return dominant;
 }
 
-template <typename T, bool LINEAR, int BATCH, int MODE>
+template <typename T, bool LINEAR, int BATCH, int MODE, bool TF_LDS>
 __device__ inline void render_tile(const RaycastParams& P, const float4* __restrict__ tf4, int tile,
                                    int lane, uint32_t& n_eval, uint32_t& n_nom) {
   const int tx = tile % P.tiles_x, ty = tile / P.tiles_x;
@@ -423,7 +455,7 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
     int dominant = -1;
     if (hit && L > 0.f) {
       n_nom += (uint32_t)floorf(L * P.nominal_rate);
-      dominant = march_regions<T, LINEAR, BATCH>(P, tf4, o, d, inv, t0, t1, C0, C1, C2, A, n_eval);
+      dominant = march_regions<T, LINEAR, BATCH, TF_LDS>(P, tf4, o, d, inv, t0, t1, C0, C1, C2, A, n_eval);
     }
     if (P.region_out) P.region_out[(int64_t)py * P.width + px] = dominant;
   } else if (hit && L > 0.f) {
@@ -486,8 +518,8 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
       const int gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
       // the cell's empty flag and the brick's slot are independent loads
       const uint8_t empty =
-          P.skip ? P.cell_empty[lv.cell_base + (c[2] * ucount[1] + c[1]) * ucount[0] + c[0]] : 0;
-      const int slot = P.page_table[gid];
+          P.skip ? gload(P.cell_empty + (lv.cell_base + (c[2] * ucount[1] + c[1]) * ucount[0] + c[0])) : 0;
+      const int slot = gload(P.page_table + gid);
       if (empty) {
         k = kend;
         continue;
@@ -542,7 +574,7 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
                 const float gm = sqrtf(g[j][0] * g[j][0] + g[j][1] * g[j][1] + g[j][2] * g[j][2]);
                 tf2d_lookup(P, v[j], gm, c);
               } else {
-                tf1d_lookup(P, tf4, v[j], c);
+                tf1d_lookup<TF_LDS>(P, tf4, v[j], c);
               }
               if (c[3] > 0.f) {
                 if (P.lighting) shade(P, g[j], Lh, c);
@@ -632,7 +664,8 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
             const float x = clampf(v[j] * P.tf_scale, 0.f, tfmax);
             const int ti = (int)x;
             const float fr = x - (float)ti;
-            const float4 e0 = tf4[ti], e1 = tf4[min(ti + 1, P.n_tf - 1)];
+            const float4 e0 = tf_entry<TF_LDS>(tf4, ti);
+            const float4 e1 = tf_entry<TF_LDS>(tf4, min(ti + 1, P.n_tf - 1));
             const float a = fmaf(e1.w - e0.w, fr, e0.w);
             if (a > 0.f) {
               const float lenj = (k + j == n - 1) ? lastlen : step;
@@ -656,8 +689,8 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
   const float bg1 = P.bg_bottom[1] + (P.bg_top[1] - P.bg_bottom[1]) * wy;
   const float bg2 = P.bg_bottom[2] + (P.bg_top[2] - P.bg_bottom[2]) * wy;
   const float T1 = 1.f - A;
-  P.framebuffer[(int64_t)py * P.width + px] =
-      pack_rgba8(fmaf(T1, bg0, C0), fmaf(T1, bg1, C1), fmaf(T1, bg2, C2), fmaf(T1, P.bg_alpha, A));
+  gstore(P.framebuffer + ((int64_t)py * P.width + px),
+         pack_rgba8(fmaf(T1, bg0, C0), fmaf(T1, bg1, C1), fmaf(T1, bg2, C2), fmaf(T1, P.bg_alpha, A)));
 }
 
 // Empty flag per empty-space cell for one TF: the TF entries (1D) or
@@ -760,7 +793,7 @@ __global__ __launch_bounds__(kThreads, 2) void raycast_kernel(const JobTable* __
         cur_job = job;
         n_eval = n_nom = 0;
       }
-      render_tile<T, LINEAR, BATCH, MODE>(P, tf4, tile - tab->tile_start[job], lane, n_eval, n_nom);
+      render_tile<T, LINEAR, BATCH, MODE, TF_LDS>(P, tf4, tile - tab->tile_start[job], lane, n_eval, n_nom);
     }
   }
   if (cur_job >= 0) {

@@ -139,6 +139,9 @@ SIGNATURES = {
     "tvk_set_1d_trans": [H, P, u32],
     "tvk_set_2d_trans": [H, P, u32, u32, f32],
     "tvk_set_iso_value": [H, f32],
+    "tvk_set_use_mip": [H, i32],
+    "tvk_get_use_mip": [H, C.POINTER(i32)],
+    "tvk_set_mip_rotation_angle": [H, f32],
     "tvk_set_isosurface_color": [H, f32, f32, f32],
     "tvk_set_view_pos": [H, f32, f32, f32],
     "tvk_set_view_dir": [H, f32, f32, f32],

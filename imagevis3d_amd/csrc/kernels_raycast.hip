@@ -206,7 +206,10 @@ __device__ inline void fetch_value_grad(const T* __restrict__ base, const SlotGe
 // Render modes of the kernel template: 1D TF without a gradient, any mode
 // that needs the gradient (1D TF with lighting, 2D TF), isosurface, 1D TF
 // with a per-region LOD.
-constexpr int kMode1D = 0, kModeGrad = 1, kModeIso = 2, kModeRegions = 3;
+// kModeMIP: maximum intensity projection -- the largest sample value along
+// the ray through the 1D TF; every sample is visited (a TF-transparent cell
+// may hold the maximum) and rays do not terminate early.
+constexpr int kMode1D = 0, kModeGrad = 1, kModeIso = 2, kModeRegions = 3, kModeMIP = 4;
 
 // Sample at level texel coordinate q through the page table (clamp to
 // edge): used off the march's current brick (isosurface refinement and
@@ -451,6 +454,7 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
   t0 = fmaxf(t0, 0.f);
   const float L = t1 - t0;
   float C0 = 0.f, C1 = 0.f, C2 = 0.f, A = 0.f;
+  float vmax = -1.f;  // MIP: largest sample value so far (-1: none)
   if (MODE == kModeRegions) {
     int dominant = -1;
     if (hit && L > 0.f) {
@@ -518,7 +522,9 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
       const int gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
       // the cell's empty flag and the brick's slot are independent loads
       const uint8_t empty =
-          P.skip ? gload(P.cell_empty + (lv.cell_base + (c[2] * ucount[1] + c[1]) * ucount[0] + c[0])) : 0;
+          (MODE != kModeMIP && P.skip)
+              ? gload(P.cell_empty + (lv.cell_base + (c[2] * ucount[1] + c[1]) * ucount[0] + c[0]))
+              : 0;
       const int slot = gload(P.page_table + gid);
       if (empty) {
         k = kend;
@@ -659,6 +665,13 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
         }
 #pragma unroll
         for (int j = 0; j < BATCH; ++j) {
+          if (MODE == kModeMIP) {
+            if (k + j < kend) {
+              ++n_eval;
+              vmax = fmaxf(vmax, v[j]);
+            }
+            continue;
+          }
           if (k + j < kend && live) {
             ++n_eval;
             const float x = clampf(v[j] * P.tf_scale, 0.f, tfmax);
@@ -683,6 +696,14 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
       }
       k = live ? kend : n;
     }
+  }
+  if (MODE == kModeMIP && vmax >= 0.f) {
+    float c[4];
+    tf1d_lookup<TF_LDS>(P, tf4, vmax, c);
+    C0 = c[0] * c[3];
+    C1 = c[1] * c[3];
+    C2 = c[2] * c[3];
+    A = c[3];
   }
   const float wy = 1.f - ((float)py + 0.5f) / (float)P.height;
   const float bg0 = P.bg_bottom[0] + (P.bg_top[0] - P.bg_bottom[0]) * wy;
@@ -825,6 +846,9 @@ hipError_t launch_typed(const LaunchInfo& li, const JobTable* tab, hipStream_t s
     return li.n_tf_lds > 0 ? launch_one<T, LINEAR, 4, true, kModeRegions>(li, tab, stream)
                            : launch_one<T, LINEAR, 4, false, kModeRegions>(li, tab, stream);
   if (li.mode == 2) return launch_one<T, LINEAR, 4, false, kModeIso>(li, tab, stream);
+  if (li.mode == 3)
+    return li.n_tf_lds > 0 ? launch_one<T, LINEAR, 4, true, kModeMIP>(li, tab, stream)
+                           : launch_one<T, LINEAR, 4, false, kModeMIP>(li, tab, stream);
   if (li.mode == 1) return launch_one<T, LINEAR, 2, false, kModeGrad>(li, tab, stream);
   const bool lds = li.n_tf_lds > 0;
   if (li.lighting)

@@ -783,6 +783,19 @@ int HipRenderer::SetIsoValue(float v) {
   return TVK_OK;
 }
 
+int HipRenderer::SetUseMIP(bool on) {
+  use_mip_ = on;
+  invalidate();
+  return TVK_OK;
+}
+
+int HipRenderer::SetMIPRotationAngle(float deg) {
+  if (!isfinite(deg)) return arg_fail("MIP rotation angle must be finite");
+  mip_angle_ = deg;
+  invalidate();
+  return TVK_OK;
+}
+
 int HipRenderer::SetIsosurfaceColor(float r, float g, float b) {
   iso_color_[0] = r;
   iso_color_[1] = g;
@@ -1001,9 +1014,34 @@ double HipRenderer::value_max() const {
   return m > 0.0 ? m : 1.0;
 }
 
-void HipRenderer::build_params(RaycastParams* p, int lod, const CameraState& cam, uint32_t width,
+// Kernel render mode of this handle's jobs: 0 1D TF, 1 2D TF, 2 isosurface,
+// 3 maximum intensity projection (setUseMIP in 1D TF mode).
+int HipRenderer::job_mode() const {
+  if (render_mode_ == TVK_RM_2DTRANS) return 1;
+  if (render_mode_ == TVK_RM_ISOSURFACE) return 2;
+  return use_mip_ ? 3 : 0;
+}
+
+void HipRenderer::build_params(RaycastParams* p, int lod, const CameraState& cam_in, uint32_t width,
                                uint32_t height, int tf_slot, uint32_t* framebuffer) const {
   memset(p, 0, sizeof(*p));
+  // MIP rotation (setMIPRotationAngle, RenderWindow.cpp:1687-1703): the view
+  // turned about its up vector around the volume centre (world origin)
+  CameraState cam = cam_in;
+  if (job_mode() == 3 && mip_angle_ != 0.f) {
+    double ax[3] = {cam.up[0], cam.up[1], cam.up[2]};
+    normalize(ax);
+    const double a = (double)mip_angle_ * M_PI / 180.0, ca = cos(a), sa = sin(a);
+    auto rot = [&](double v[3]) {  // Rodrigues
+      double kxv[3], out[3];
+      cross(ax, v, kxv);
+      const double kv = ax[0] * v[0] + ax[1] * v[1] + ax[2] * v[2];
+      for (int i = 0; i < 3; ++i) out[i] = v[i] * ca + kxv[i] * sa + ax[i] * kv * (1.0 - ca);
+      for (int i = 0; i < 3; ++i) v[i] = out[i];
+    };
+    rot(cam.pos);
+    rot(cam.dir);
+  }
   const TfSet& tfs = tfs_[tf_slot];
   double M[16], Minv[16];
   mat_mul(rot_, trans_, M);
@@ -1076,7 +1114,7 @@ void HipRenderer::build_params(RaycastParams* p, int lod, const CameraState& cam
   p->tile_group = tile_group_;
   p->xcd_split = xcd_split_;
   p->compute_units = compute_units_;
-  p->skip = skip_ ? 1 : 0;
+  p->skip = (skip_ && job_mode() != 3) ? 1 : 0;
   p->level = lods_[lod];
   p->brick = brick_;
   p->apron = apron_;
@@ -1100,7 +1138,7 @@ void HipRenderer::build_params(RaycastParams* p, int lod, const CameraState& cam
   p->framebuffer = framebuffer;
 
   // render mode, gradient units, lighting
-  p->mode = render_mode_ == TVK_RM_2DTRANS ? 1 : (render_mode_ == TVK_RM_ISOSURFACE ? 2 : 0);
+  p->mode = job_mode();
   p->lighting = (lighting_ && p->mode == 0) ? 1 : 0;
   const double maxval = (double)((1u << bits_) - 1);
   for (int i = 0; i < 3; ++i) p->grad_scale[i] = (float)((double)lods_[lod].to_level[i] / (2.0 * maxval));
@@ -1134,10 +1172,10 @@ LaunchInfo HipRenderer::launch_info(int n_tf_lds) const {
   LaunchInfo li;
   li.bpv = (int)bits_ / 8;
   li.interp = interp_;
-  li.mode = render_mode_ == TVK_RM_2DTRANS ? 1 : (render_mode_ == TVK_RM_ISOSURFACE ? 2 : 0);
+  li.mode = job_mode();
   li.lighting = (lighting_ && li.mode == 0) ? 1 : 0;
   li.batch = batch_;
-  li.n_tf_lds = (li.mode == 0 && n_tf_lds <= 4096) ? n_tf_lds : 0;
+  li.n_tf_lds = ((li.mode == 0 || li.mode == 3) && n_tf_lds <= 4096) ? n_tf_lds : 0;
   li.regions = (regions_active() && li.mode == 0 && !li.lighting) ? 1 : 0;
   li.compute_units = compute_units_;
   return li;
@@ -1249,8 +1287,9 @@ int HipRenderer::prepare_cells(int tf_slot, int mode) {
 }
 
 uint32_t HipRenderer::bricks_skipped(int lod, int tf_slot) const {
-  // bricks whose whole value range is transparent (below the iso value)
-  if (!skip_) return 0;
+  // bricks whose whole value range is transparent (below the iso value);
+  // maximum intensity projection visits every brick
+  if (!skip_ || job_mode() == 3) return 0;
   const TfSet& t = tfs_[tf_slot];
   const LodInfo& L = lods_[lod];
   const int nb = L.grid[0] * L.grid[1] * L.grid[2];

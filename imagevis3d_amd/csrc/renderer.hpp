@@ -168,6 +168,9 @@ class HipRenderer {
   int Set1DTransSlot(uint32_t slot, const float* rgba, uint32_t n);
   int Set2DTransSlot(uint32_t slot, const float* rgba, uint32_t w, uint32_t h, float max_grad);
   int SetIsoValue(float v);
+  int SetUseMIP(bool on);
+  bool GetUseMIP() const { return use_mip_; }
+  int SetMIPRotationAngle(float deg);
   int SetIsosurfaceColor(float r, float g, float b);
 
   int SetViewPos(const float v[3]);
@@ -243,6 +246,7 @@ class HipRenderer {
   int upload_lods();
   // (re)build the empty flags of TF slot `tf_slot` for the current mode
   int prepare_cells(int tf_slot, int mode);
+  int job_mode() const;
   int mode_index() const {
     return render_mode_ == TVK_RM_2DTRANS ? 1 : (render_mode_ == TVK_RM_ISOSURFACE ? 2 : 0);
   }
@@ -330,6 +334,8 @@ class HipRenderer {
   int render_mode_ = TVK_RM_1DTRANS;
   std::vector<TfSet> tfs_ = std::vector<TfSet>(1);
   float iso_value_ = 0.f;
+  bool use_mip_ = false;     // maximum intensity projection (1D TF mode)
+  float mip_angle_ = 0.f;    // MIP view rotation about the up vector, degrees
   float iso_color_[3] = {1.f, 1.f, 1.f};
 
   // camera

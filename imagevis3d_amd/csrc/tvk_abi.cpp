@@ -369,6 +369,26 @@ int tvk_set_2d_trans(tvk_renderer* r, const float* rgba, uint32_t w, uint32_t h,
   TVK_TRY(r->impl.Set2DTrans(rgba, w, h, mg));
 }
 
+int tvk_set_use_mip(tvk_renderer* r, int use_mip) {
+  TVK_GUARD(r);
+  TVK_TRY(r->impl.SetUseMIP(use_mip != 0));
+}
+
+int tvk_get_use_mip(tvk_renderer* r, int* use_mip) {
+  TVK_GUARD(r);
+  if (!use_mip) {
+    tvk::set_last_error("null output");
+    return TVK_ERR_ARG;
+  }
+  *use_mip = r->impl.GetUseMIP() ? 1 : 0;
+  return TVK_OK;
+}
+
+int tvk_set_mip_rotation_angle(tvk_renderer* r, float degrees) {
+  TVK_GUARD(r);
+  TVK_TRY(r->impl.SetMIPRotationAngle(degrees));
+}
+
 int tvk_set_iso_value(tvk_renderer* r, float iso) {
   TVK_GUARD(r);
   return r->impl.SetIsoValue(iso);

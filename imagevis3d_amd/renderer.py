@@ -331,6 +331,19 @@ class Renderer:
         h, w = a.shape[:2]
         call("tvk_set_2d_trans", self._h, a.ctypes.data, w, h, float(max_grad))
 
+    def setUseMIP(self, use_mip: bool) -> None:
+        """Maximum intensity projection in 1D TF mode (RenderWindow.cpp:281-289)."""
+        call("tvk_set_use_mip", self._h, int(bool(use_mip)))
+
+    def getUseMIP(self) -> bool:
+        v = C.c_int()
+        call("tvk_get_use_mip", self._h, C.byref(v))
+        return bool(v.value)
+
+    def setMIPRotationAngle(self, degrees: float) -> None:
+        """MIP views turned about the up vector (RenderWindow.cpp:1687-1703)."""
+        call("tvk_set_mip_rotation_angle", self._h, float(degrees))
+
     def setIsoValue(self, v: float) -> None:
         call("tvk_set_iso_value", self._h, float(v))
 

@@ -263,6 +263,17 @@ int tvk_set_2d_trans(tvk_renderer* r, const float* rgba, uint32_t w, uint32_t h,
                      float max_grad);
 /* setIsoValue -- RenderWindow.cpp:2117-2147 (raw data units). */
 int tvk_set_iso_value(tvk_renderer* r, float iso);
+/* setUseMIP / getUseMIP -- RenderWindow.cpp:281-289: maximum intensity
+ * projection in 1D TF mode: each pixel is the 1D TF at the largest sample
+ * value along its ray (every sample visited, no early termination), over
+ * the background.  ImageVis3D offers it on its 2D views (slice regions,
+ * not built here); this backend applies it to the raycast view. */
+int tvk_set_use_mip(tvk_renderer* r, int use_mip);
+int tvk_get_use_mip(tvk_renderer* r, int* use_mip);
+/* setMIPRotationAngle -- RenderWindow.cpp:1687-1703 (CaptureMIPFrame): MIP
+ * views turned by `degrees` about the camera's up vector around the volume
+ * centre. */
+int tvk_set_mip_rotation_angle(tvk_renderer* r, float degrees);
 /* setIsosurfaceColor -- RenderWindow.cpp:2329. */
 int tvk_set_isosurface_color(tvk_renderer* r, float red, float green, float blue);
 

@@ -172,12 +172,14 @@ int oracle_render_1d(const oracle_render_params* p, double* out, uint32_t* out_s
   cam_ctx c;
   if (cam_setup(p, &c)) return -1;
   int nb[3] = {0, 0, 0};
-  uint8_t* empty = p->brick_size > 0 ? oracle_brick_empty_flags(p, nb) : NULL;
+  /* MIP (mode 3) visits every sample: a TF-transparent region may hold the
+   * maximum, so the TF-based empty-space flags do not apply */
+  uint8_t* empty = p->brick_size > 0 && p->mode != 3 ? oracle_brick_empty_flags(p, nb) : NULL;
   const double lo[3] = {0, 0, 0};
   const double hi[3] = {p->n0x, p->n0y, p->n0z};
   const double lsc[3] = {(double)p->nx / p->n0x, (double)p->ny / p->n0y, (double)p->nz / p->n0z};
   const double step = ldexp(1.0, p->lod) / p->rate;
-  const int grad = p->mode == 1 || p->mode == 2 || p->lighting;
+  const int grad = p->mode == 1 || p->mode == 2 || (p->lighting && p->mode != 3);
   /* light direction in voxel space (a direction: linear part only) */
   double Lv[3] = {0, 0, 1};
   if (grad) {
@@ -199,6 +201,8 @@ int oracle_render_1d(const oracle_render_params* p, double* out, uint32_t* out_s
       uint32_t ns = 0;
       uint8_t flags = 0;
       double t0, t1;
+      double vmax = 0.0;
+      int any = 0;
       if (box_hit(o, d, lo, hi, &t0, &t1)) {
         if (t0 < 0) t0 = 0;
         double L = t1 - t0;
@@ -218,6 +222,12 @@ int oracle_render_1d(const oracle_render_params* p, double* out, uint32_t* out_s
             double v = p->interp ? sample_lin(p, q) : sample_nn(p, q, &nf);
             if (nf) flags |= 2;
             ++ns;
+            if (p->mode == 3) {
+              /* maximum intensity projection: the largest sample value */
+              if (!any || v > vmax) vmax = v;
+              any = 1;
+              continue;
+            }
             if (p->mode == 2) {
               /* first sample at or above the iso value; the hit is refined
                * linearly against the previous sample (re-fetched, so the
@@ -255,6 +265,13 @@ int oracle_render_1d(const oracle_render_params* p, double* out, uint32_t* out_s
             if (A >= p->term_alpha) break;
           }
         }
+      }
+      if (p->mode == 3 && any) {
+        /* the maximum through the 1D TF, over the background */
+        double rgba[4];
+        tf_lookup(p, vmax, rgba);
+        for (int i = 0; i < 3; ++i) C[i] = rgba[i] * rgba[3];
+        A = rgba[3];
       }
       double* px = out + 4 * ((int64_t)y * p->width + x);
       for (int i = 0; i < 3; ++i) px[i] = C[i] + (1.0 - A) * bg[i];

@@ -55,7 +55,9 @@ typedef struct oracle_render_params {
   int32_t brick_size;      /* payload B of the sampled level, 0 = no bricks */
   int32_t apron;
   /* render mode: 0 = 1D TF, 1 = 2D TF (value x gradient magnitude),
-   * 2 = isosurface (first hit) */
+   * 2 = isosurface (first hit), 3 = maximum intensity projection (the
+   * largest sample value along the ray through the 1D TF, over the
+   * background; every sample visited, no early termination) */
   int32_t mode;
   const float* tf2d;       /* tf2d_w x tf2d_h RGBA floats, value fastest */
   int32_t tf2d_w, tf2d_h;

@@ -134,3 +134,38 @@ def test_c60_with_its_2dt(c60, lighting):
         assert (ref8[..., 3] > 0).any()
         assert_parity(img, ref8, fl, f"c60 2dt view {i} lighting={lighting}")
     r.cleanup()
+
+
+@pytest.mark.parametrize("interp", [1, 0])
+def test_mip_parity(interp):
+    """Maximum intensity projection (setUseMIP) against the oracle's mode 3,
+    and setMIPRotationAngle against the same camera turned by hand."""
+    w, vol = medium_small(256, 144)
+    print(f"case MIP interp={interp}", flush=True)
+    r = make_renderer(w, vol)
+    r.set1DTrans(w.tf)
+    r.setInterpolant(interp)
+    r.setUseMIP(True)
+    assert r.getUseMIP()
+    cam = w.cameras[5]
+    img, st = render(r, w, cam)
+    ref8, ns, fl = oracle_ref(vol, w, cam, w.tf, mode=3, interp=interp)
+    assert_parity(img, ref8, fl, f"MIP interp={interp}", max_flagged_frac=1.0 if not interp else 0.02)
+    assert abs(int(st.evaluated_samples) - int(ns.sum())) <= 2e-3 * ns.sum() + 64
+    # every sample is visited: skipping changes nothing
+    r.setEmptySpaceSkipping(False)
+    assert r.paint()
+    assert np.array_equal(r.capture(), img)
+    r.setEmptySpaceSkipping(True)
+    # rotation about the up vector (here +y) through the volume centre
+    a = np.radians(90.0)
+    p = np.array(cam.view_pos)
+    rot = np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]])
+    cam2 = Camera(tuple(rot @ p), tuple(-(rot @ p)), cam.up_dir, cam.fov)
+    r.setMIPRotationAngle(90.0)
+    rot_img, _ = render(r, w, cam)
+    r.setMIPRotationAngle(0.0)
+    ref_img, _ = render(r, w, cam2)
+    d = np.abs(rot_img.astype(int) - ref_img.astype(int))
+    assert d.max() <= 2 and (d > 0).mean() < 0.01
+    r.cleanup()

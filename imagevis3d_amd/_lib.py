@@ -183,6 +183,10 @@ SIGNATURES = {
     "tvk_toc_read_brick": [C.c_void_p, u32, u32, u32, u32, P, u64, C.POINTER(u32), C.POINTER(f32),
                            C.POINTER(f32)],
     "tvk_set_toc_volume": [H, C.c_void_p, u64],
+    "tvk_toc_get_histogram_1d": [C.c_void_p, P, C.POINTER(u64)],
+    "tvk_toc_get_histogram_2d": [C.c_void_p, P, C.POINTER(u64), C.POINTER(u64), C.POINTER(f32)],
+    "tvk_toc_build_raw": [i32, C.c_char_p, u64, u64, u64, u64, u32, u32, u32, i32, i32, u32, u32, u32,
+                          C.c_char_p, C.POINTER(TocInfo)],
     "tvk_lod_features": [C.POINTER(View), P, u32, u32, P, P, P, P, P, P],
     "tvk_lod_select_octree": [P, P, P, u32, C.c_double, u64, u64, P, C.POINTER(C.c_double),
                               C.POINTER(u64)],

@@ -16,9 +16,15 @@
 //     per brick (LOD-major, then z, y, x): u64 offset (absolute), stored
 //       bytes, raw bytes; u32 compression, reserved; f32 min, max;
 //     the stored bricks, in the layout's order.
+//   Then a Histogram1D block (semantic 5: u64 n, n x u64) and a
+//   Histogram2D block (semantic 6: f32 max gradient, u64 elems_x, u64
+//   elems_y, elems_x x elems_y u64 with x = value slowest), laid out as
+//   doc/import.adoc:1021-1056 documents them; the TOC block's next-block
+//   offset points at the first of them.
 #include "toc.hpp"
 
 #include <dlfcn.h>
+#include <math.h>
 #include <fcntl.h>
 #include <hip/hip_runtime_api.h>
 #include <string.h>
@@ -196,6 +202,9 @@ struct Reader {
 
 constexpr uint64_t kEntryBytes = 40;
 const char kBlockName[] = "Tuvok MI355X TOC volume";
+const char kHist1Name[] = "1D Histogram";
+const char kHist2Name[] = "2D Histogram";
+constexpr uint64_t kMasterBytes = 8 + 1 + 8 + 8 + 8 + 8;
 
 uint64_t header_bytes(uint32_t n_lod) {
   const uint64_t master = 8 + 1 + 8 + 8 + 8 + 8;
@@ -446,6 +455,50 @@ int toc_build(int device, const void* voxels, const uint64_t dims[3], uint32_t b
     }
   }
 
+  // histogram blocks after the bricks (the renderer's histogram kernels)
+  const uint64_t n1 = 1ull << bits;
+  const int n2v = 256, n2g = 256;
+  std::vector<uint64_t> h1(n1), h2((size_t)n2v * n2g);
+  unsigned long long kmax = 0;
+  {
+    DevBuf d_h1, d_h2, d_km;
+    TOC_HIP(d_h1.alloc(n1 * 8));
+    TOC_HIP(d_h2.alloc(h2.size() * 8));
+    TOC_HIP(d_km.alloc(8));
+    const int64_t d0[3] = {(int64_t)dims[0], (int64_t)dims[1], (int64_t)dims[2]};
+    TOC_HIP(launch_histogram_1d(levels[0].p, bpv, d0[0] * d0[1] * d0[2], (unsigned long long*)d_h1.p, stream));
+    TOC_HIP(launch_gradient_kmax(levels[0].p, bpv, d0, (unsigned long long*)d_km.p, stream));
+    TOC_HIP(launch_histogram_2d(levels[0].p, bpv, d0, (const unsigned long long*)d_km.p, n2v, n2g,
+                                (unsigned long long*)d_h2.p, stream));
+    TOC_HIP(hipMemcpyAsync(h1.data(), d_h1.p, n1 * 8, hipMemcpyDeviceToHost, stream));
+    TOC_HIP(hipMemcpyAsync(h2.data(), d_h2.p, h2.size() * 8, hipMemcpyDeviceToHost, stream));
+    TOC_HIP(hipMemcpyAsync(&kmax, d_km.p, 8, hipMemcpyDeviceToHost, stream));
+    TOC_HIP(hipStreamSynchronize(stream));
+  }
+  const float max_grad = (float)(sqrt((double)kmax) / (2.0 * (double)((1u << bits) - 1)));
+  const uint64_t h1_at = offset;
+  const uint64_t h1_bytes = 8 + (sizeof(kHist1Name) - 1) + 24 + 8 + n1 * 8;
+  {
+    Writer wh{f};
+    wh.u64(sizeof(kHist1Name) - 1);
+    wh.raw(kHist1Name, sizeof(kHist1Name) - 1);
+    wh.u64(5);
+    wh.u64(0);
+    wh.u64(h1_bytes);  // the 2D histogram follows
+    wh.u64(n1);
+    wh.raw(h1.data(), n1 * 8);
+    wh.u64(sizeof(kHist2Name) - 1);
+    wh.raw(kHist2Name, sizeof(kHist2Name) - 1);
+    wh.u64(6);
+    wh.u64(0);
+    wh.u64(0);  // last block
+    wh.f32(max_grad);
+    wh.u64((uint64_t)n2v);
+    wh.u64((uint64_t)n2g);
+    wh.raw(h2.data(), h2.size() * 8);
+    if (!wh.ok) return fail(TVK_ERR_IO, "write failed on " + path);
+  }
+
   // headers and the table
   if (fseeko(f, 0, SEEK_SET) != 0) return fail(TVK_ERR_IO, "cannot write " + path);
   Writer w{f};
@@ -459,8 +512,8 @@ int toc_build(int device, const void* voxels, const uint64_t dims[3], uint32_t b
   w.u64(sizeof(kBlockName) - 1);
   w.raw(kBlockName, sizeof(kBlockName) - 1);
   w.u64(kTocSemantic);
-  w.u64(0);  // block compression: none
-  w.u64(0);  // last block
+  w.u64(0);            // block compression: none
+  w.u64(h1_at - kMasterBytes);  // next block: the 1D histogram
   w.raw(kTocMagic, 8);
   w.u32(bits);
   w.u32(1);
@@ -535,8 +588,9 @@ int TocVolume::Open(const std::string& path) {
   r.pos += ncs;
   const uint64_t skip = r.u64();
   r.pos += skip;
-  // walk the blocks to the TOC block
-  while (true) {
+  // walk the blocks: the TOC block and the histogram blocks
+  uint64_t toc_at = 0, h1_at = 0, h2_at = 0;
+  for (int guard = 0; guard < 4096; ++guard) {
     const uint64_t block_at = r.pos;
     const uint64_t name_len = r.u64();
     if (!r.ok || name_len > 4096) return fail(TVK_ERR_FORMAT, "bad UVF block header");
@@ -545,10 +599,37 @@ int TocVolume::Open(const std::string& path) {
     const uint64_t block_comp = r.u64();
     const uint64_t next = r.u64();
     if (!r.ok) return fail(TVK_ERR_FORMAT, "truncated UVF block header");
-    if (semantic == kTocSemantic && block_comp == 0) break;
-    if (next == 0) return fail(TVK_ERR_FORMAT, "no TOC block in " + path);
+    if (block_comp == 0) {
+      if (semantic == kTocSemantic && !toc_at) toc_at = r.pos;
+      if (semantic == 5 && !h1_at) h1_at = r.pos;
+      if (semantic == 6 && !h2_at) h2_at = r.pos;
+    }
+    if (next == 0) break;
+    if (next > file_bytes - block_at) return fail(TVK_ERR_FORMAT, "bad UVF block offset");
     r.pos = block_at + next;
   }
+  if (!toc_at) return fail(TVK_ERR_FORMAT, "no TOC block in " + path);
+  if (h1_at) {
+    Reader h{fd_, h1_at, file_bytes};
+    const uint64_t n = h.u64();
+    if (!h.ok || n == 0 || n > (1u << 16) || n * 8 > file_bytes - h.pos) return fail(TVK_ERR_FORMAT, "bad 1D histogram block");
+    hist1d.resize(n);
+    h.raw(hist1d.data(), n * 8);
+    if (!h.ok) return fail(TVK_ERR_FORMAT, "truncated 1D histogram block");
+  }
+  if (h2_at) {
+    Reader h{fd_, h2_at, file_bytes};
+    max_grad = h.f32();
+    hist2d_x = h.u64();
+    hist2d_y = h.u64();
+    if (!h.ok || hist2d_x == 0 || hist2d_y == 0 || hist2d_x > 65536 || hist2d_y > 65536 ||
+        hist2d_x * hist2d_y * 8 > file_bytes - h.pos)
+      return fail(TVK_ERR_FORMAT, "bad 2D histogram block");
+    hist2d.resize(hist2d_x * hist2d_y);
+    h.raw(hist2d.data(), hist2d.size() * 8);
+    if (!h.ok) return fail(TVK_ERR_FORMAT, "truncated 2D histogram block");
+  }
+  r.pos = toc_at;
   char tmagic[8];
   r.raw(tmagic, 8);
   if (!r.ok || memcmp(tmagic, kTocMagic, 8) != 0) return fail(TVK_ERR_FORMAT, "unknown TOC block payload");

@@ -46,11 +46,19 @@ class TocVolume {
   std::vector<TocLod> lods;
   std::vector<TocEntry> entries;
   uint64_t file_bytes = 0;
+  // Histogram1D / Histogram2D blocks of the file (empty when absent)
+  std::vector<uint64_t> hist1d;
+  std::vector<uint64_t> hist2d;  // hist2d_x (value, slowest) x hist2d_y
+  uint64_t hist2d_x = 0, hist2d_y = 0;
+  float max_grad = 0.f;
 
  private:
   int fd_ = -1;
 };
 
+// Writes the TOC block followed by a Histogram1D block (2^bits bins) and a
+// Histogram2D block (256 value x 256 gradient bins, the dataset's maximum
+// gradient), computed on the device with the renderer's histogram rules.
 int toc_build(int device, const void* voxels, const uint64_t dims[3], uint32_t bits,
               uint32_t max_brick, uint32_t overlap, bool median, bool clamp, uint32_t compression,
               uint32_t level, uint32_t layout, const std::string& path, tvk_toc_info* info);

@@ -2,6 +2,12 @@
 // Each function forwards to the AbstrRenderer-shaped HipRenderer; C++
 // exceptions never cross the boundary.
 #include <algorithm>
+#include <fcntl.h>
+#include <string.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <memory>
 #include <new>
 #include <string>
@@ -309,6 +315,91 @@ int tvk_toc_read_brick(tvk_toc* t, uint32_t lod, uint32_t bx, uint32_t by, uint3
   if (max_value) *max_value = E.mx;
   if (!out) return TVK_OK;
   TVK_TRY(t->impl.ReadBrick(lod, bx, by, bz, out, out_bytes));
+}
+
+int tvk_toc_get_histogram_1d(tvk_toc* t, uint64_t* out, uint64_t* n_bins) {
+  if (!t || !n_bins) {
+    tvk::set_last_error("null argument");
+    return TVK_ERR_ARG;
+  }
+  const auto& h = t->impl.hist1d;
+  if (h.empty()) {
+    tvk::set_last_error("the file has no 1D histogram block");
+    return TVK_ERR_STATE;
+  }
+  const uint64_t cap = *n_bins;
+  *n_bins = h.size();
+  if (!out) return TVK_OK;
+  if (cap < h.size()) {
+    tvk::set_last_error("histogram buffer too small");
+    return TVK_ERR_ARG;
+  }
+  memcpy(out, h.data(), h.size() * 8);
+  return TVK_OK;
+}
+
+int tvk_toc_get_histogram_2d(tvk_toc* t, uint64_t* out, uint64_t* n_value, uint64_t* n_grad,
+                             float* max_grad) {
+  if (!t || !n_value || !n_grad) {
+    tvk::set_last_error("null argument");
+    return TVK_ERR_ARG;
+  }
+  const tvk::TocVolume& T = t->impl;
+  if (T.hist2d.empty()) {
+    tvk::set_last_error("the file has no 2D histogram block");
+    return TVK_ERR_STATE;
+  }
+  const uint64_t cap = *n_value * *n_grad;
+  *n_value = T.hist2d_x;
+  *n_grad = T.hist2d_y;
+  if (max_grad) *max_grad = T.max_grad;
+  if (!out) return TVK_OK;
+  if (cap < T.hist2d.size()) {
+    tvk::set_last_error("histogram buffer too small");
+    return TVK_ERR_ARG;
+  }
+  memcpy(out, T.hist2d.data(), T.hist2d.size() * 8);
+  return TVK_OK;
+}
+
+int tvk_toc_build_raw(int device, const char* raw_path, uint64_t header_bytes, uint64_t nx, uint64_t ny,
+                      uint64_t nz, uint32_t bits, uint32_t max_brick, uint32_t overlap, int use_median,
+                      int clamp_to_edge, uint32_t compression, uint32_t level, uint32_t layout,
+                      const char* path, tvk_toc_info* info) {
+  if (!raw_path || !path || (bits != 8 && bits != 16)) {
+    tvk::set_last_error("bad raw input arguments");
+    return TVK_ERR_ARG;
+  }
+  const int fd = open(raw_path, O_RDONLY);
+  if (fd < 0) {
+    tvk::set_last_error(std::string("cannot open ") + raw_path);
+    return TVK_ERR_IO;
+  }
+  struct stat st;
+  const uint64_t need = nx * ny * nz * (bits / 8);
+  if (fstat(fd, &st) != 0 || nx == 0 || ny == 0 || nz == 0 || (uint64_t)st.st_size < header_bytes ||
+      (uint64_t)st.st_size - header_bytes < need) {
+    close(fd);
+    tvk::set_last_error(std::string(raw_path) + " is smaller than the volume it should hold");
+    return TVK_ERR_IO;
+  }
+  void* map = mmap(nullptr, (size_t)st.st_size, PROT_READ, MAP_PRIVATE, fd, 0);
+  close(fd);
+  if (map == MAP_FAILED) {
+    tvk::set_last_error(std::string("cannot map ") + raw_path);
+    return TVK_ERR_IO;
+  }
+  const uint64_t d[3] = {nx, ny, nz};
+  int rc;
+  try {
+    rc = tvk::toc_build(device, (const uint8_t*)map + header_bytes, d, bits, max_brick, overlap,
+                        use_median != 0, clamp_to_edge != 0, compression, level, layout, path, info);
+  } catch (const std::bad_alloc&) {
+    tvk::set_last_error("host allocation failed");
+    rc = TVK_ERR_OOM;
+  }
+  munmap(map, (size_t)st.st_size);
+  return rc;
 }
 
 static int toc_brick_source(void* user, uint32_t lod, uint32_t bx, uint32_t by, uint32_t bz, void* slot,

@@ -44,6 +44,20 @@ def build_toc(vol: np.ndarray, path: str, max_brick: int = 128, overlap: int = 2
     return _info_dict(info)
 
 
+def build_toc_raw(raw_path: str, header_bytes: int, dims, bits: int, path: str,
+                  max_brick: int = 128, overlap: int = 2, use_median: bool = False,
+                  clamp_to_edge: bool = True, compression: int = 1, level: int = 1,
+                  layout: int = 0, device: int = 0) -> dict:
+    """The same build from a raw file (x fastest, little endian) whose voxels
+    start `header_bytes` into the file (uvfconvert's raw input)."""
+    nx, ny, nz = (int(v) for v in dims)
+    info = TocInfo()
+    call("tvk_toc_build_raw", int(device), str(raw_path).encode(), int(header_bytes), nx, ny, nz,
+         int(bits), int(max_brick), int(overlap), int(bool(use_median)), int(bool(clamp_to_edge)),
+         int(compression), int(level), int(layout), str(path).encode(), C.byref(info))
+    return _info_dict(info)
+
+
 class TocVolume:
     """An open TOC volume file (tvk_toc_*)."""
 
@@ -71,6 +85,22 @@ class TocVolume:
         call("tvk_toc_read_brick", self._h, int(lod), int(bx), int(by), int(bz), out.ctypes.data,
              out.nbytes, C.byref(comp), C.byref(mn), C.byref(mx))
         return out, int(comp.value), float(mn.value), float(mx.value)
+
+    def histogram1D(self) -> np.ndarray:
+        """The file's Histogram1D block (u64 counts per raw value)."""
+        n = _lib.u64(0)
+        call("tvk_toc_get_histogram_1d", self._h, None, C.byref(n))
+        out = np.zeros(n.value, np.uint64)
+        call("tvk_toc_get_histogram_1d", self._h, out.ctypes.data, C.byref(n))
+        return out
+
+    def histogram2D(self):
+        """The file's Histogram2D block: (counts [value, gradient], max gradient)."""
+        nv, ng, mg = _lib.u64(0), _lib.u64(0), C.c_float()
+        call("tvk_toc_get_histogram_2d", self._h, None, C.byref(nv), C.byref(ng), C.byref(mg))
+        out = np.zeros((nv.value, ng.value), np.uint64)
+        call("tvk_toc_get_histogram_2d", self._h, out.ctypes.data, C.byref(nv), C.byref(ng), C.byref(mg))
+        return out, float(mg.value)
 
     def close(self) -> None:
         if self._h:

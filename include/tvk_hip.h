@@ -232,6 +232,23 @@ int tvk_toc_get_lod(tvk_toc* t, uint32_t lod, uint64_t dims[3], uint64_t grid[3]
 int tvk_toc_read_brick(tvk_toc* t, uint32_t lod, uint32_t bx, uint32_t by, uint32_t bz, void* out,
                        uint64_t out_bytes, uint32_t* compression, float* min_value,
                        float* max_value);
+/* The file's Histogram1DDataBlock / Histogram2DDataBlock (DataSource.h:
+ * 595-636; layouts doc/import.adoc:1021-1056), written by tvk_toc_build from
+ * the LOD-0 voxels with the c60-pinned rules: 2^bits value bins; 256 value x
+ * 256 gradient-magnitude bins (value slowest) and the maximum gradient.
+ * *n_bins / *n_value, *n_grad hold the capacity of `out` on input and the
+ * stored sizes on output (out may be NULL to query them); TVK_ERR_STATE
+ * when the file has no such block. */
+int tvk_toc_get_histogram_1d(tvk_toc* t, uint64_t* out, uint64_t* n_bins);
+int tvk_toc_get_histogram_2d(tvk_toc* t, uint64_t* out, uint64_t* n_value, uint64_t* n_grad,
+                             float* max_grad);
+/* uvfconvert's input side (UVFReader/main.cpp:143-147,372-414): the same
+ * build from a raw file of nx*ny*nz voxels (x fastest, little endian) that
+ * starts `header_bytes` into the file. */
+int tvk_toc_build_raw(int device, const char* raw_path, uint64_t header_bytes, uint64_t nx,
+                      uint64_t ny, uint64_t nz, uint32_t bits, uint32_t max_brick, uint32_t overlap,
+                      int use_median, int clamp_to_edge, uint32_t compression, uint32_t level,
+                      uint32_t layout, const char* path, tvk_toc_info* info);
 /* Page a TOC volume into the handle's HBM brick pool (the out-of-core
  * GridLeaper path of tvk_set_paged_volume with the TOC as brick source:
  * brick payload and apron = the TOC's payload and overlap).  `t` must stay

@@ -13,8 +13,9 @@ import numpy as np
 import pytest
 
 from conftest import gpu_renderer
-from imagevis3d_amd import workloads
-from imagevis3d_amd.toc import TocVolume, build_toc
+from imagevis3d_amd import TvkError, workloads
+from imagevis3d_amd.toc import TocVolume, build_toc, build_toc_raw
+from oracle import oracle as O
 from test_toc_host import toc_bricks
 
 pytestmark = pytest.mark.gpu
@@ -164,4 +165,47 @@ def test_toc_volume_renders_like_in_core(tmp_path):
     assert st.bricks_uploaded > 0
     ic.cleanup()
     pg.cleanup()
+    t.close()
+
+
+def test_c60_raw_build_histogram_blocks_match_stored_blocks(tmp_path, c60_path, c60):
+    """uvfconvert-style build straight from c60.uvf's raster payload (a raw
+    volume 484 bytes into the file): the TOC file's Histogram1D and
+    Histogram2D blocks equal the blocks the reference stored in c60.uvf,
+    bit for bit, and the bricks equal the in-memory build's."""
+    print("case toc raw c60 histograms", flush=True)
+    ras = c60["blocks"][0]
+    path = str(tmp_path / "c60_toc.uvf")
+    info = build_toc_raw(c60_path, ras["data_offset"], (128, 128, 128), 8, path, max_brick=68,
+                         overlap=2, compression=1, layout=1)
+    assert info["dims"] == (128, 128, 128) and info["n_lod"] == 2
+    t = TocVolume(path)
+    h1 = t.histogram1D()
+    assert np.array_equal(h1, c60["blocks"][1]["histogram"].astype(np.uint64))
+    h2, mg = t.histogram2D()
+    stored = c60["blocks"][2]
+    assert np.array_equal(h2, stored["histogram"].astype(np.uint64))
+    assert np.float32(mg) == np.float32(stored["max_grad"])
+    ref = toc_bricks(ras["volume"], 68, 2)
+    for lod, (dims, grid, bricks) in enumerate(ref):
+        for i, b in enumerate(bricks):
+            bx, by, bz = i % grid[0], (i // grid[0]) % grid[1], i // (grid[0] * grid[1])
+            assert np.array_equal(t.read_brick(lod, bx, by, bz)[0], b), (lod, i)
+    t.close()
+    # a raw file shorter than the volume is rejected
+    short = str(tmp_path / "short.raw")
+    open(short, "wb").write(bytes(1000))
+    with pytest.raises(TvkError):
+        build_toc_raw(short, 0, (16, 16, 16), 8, str(tmp_path / "x.uvf"))
+
+
+def test_u16_histogram_blocks_match_oracle(tmp_path):
+    vol = volume((20, 30, 40), np.uint16, seed=4)
+    path = str(tmp_path / "h.uvf")
+    build_toc(vol, path, max_brick=20, overlap=2, compression=0)
+    t = TocVolume(path)
+    assert np.array_equal(t.histogram1D(), O.histogram_1d(vol).astype(np.uint64))
+    h2, mg = t.histogram2D()
+    ref, ref_mg = O.histogram_2d_exact(vol, 256, 256)
+    assert np.array_equal(h2, ref.astype(np.uint64)) and np.float32(mg) == np.float32(ref_mg)
     t.close()

@@ -45,14 +45,18 @@ def toc_bricks(vol, max_brick, overlap, median=False, clamp=True):
     return out
 
 
-def write_toc(path, vol, max_brick, overlap, codec, order=None, version=3, magic=b"TVKTOC01"):
+def write_toc(path, vol, max_brick, overlap, codec, order=None, version=3, magic=b"TVKTOC01",
+              hists=None):
     """The documented layout: UVF master header, one block (semantic 9), the
-    TOC payload, then the stored bricks (here in reverse scanline order)."""
+    TOC payload, then the stored bricks (here in reverse scanline order);
+    with `hists` = (h1, h2, max_grad), a Histogram1D and a Histogram2D block
+    after the bricks (doc/import.adoc layouts)."""
     levels = toc_bricks(vol, max_brick, overlap)
     bits = 8 if vol.dtype == np.uint8 else 16
     name = b"test TOC volume"
     head = b"UVF-DATA" + b"\x00" + struct.pack("<QQQQ", version, 0, 0, 0)
     head += struct.pack("<Q", len(name)) + name + struct.pack("<QQQ", 9, 0, 0)
+    next_at = len(head) - 8  # the TOC block's next-block field
     toc = magic + struct.pack("<II", bits, 1) + struct.pack("<QQQ", *vol.shape[::-1])
     toc += struct.pack("<IIII", max_brick, overlap, max_brick - 2 * overlap, len(levels))
     toc += struct.pack("<IIIIII", 0, 1, codec, 1, 0, 0)
@@ -70,10 +74,24 @@ def write_toc(path, vol, max_brick, overlap, codec, order=None, version=3, magic
         pos += len(stored[i])
     table = b"".join(struct.pack("<QQQIIff", offsets[i], len(stored[i]), flat[i].nbytes, codec, 0,
                                  float(flat[i].min()), float(flat[i].max())) for i in range(len(flat)))
+    tail = b""
+    if hists is not None:
+        h1, h2, mg = hists
+        n1 = b"1D Histogram"
+        b1 = struct.pack("<Q", len(n1)) + n1
+        b1_len = len(b1) + 24 + 8 + 8 * h1.size
+        b1 += struct.pack("<QQQ", 5, 0, b1_len) + struct.pack("<Q", h1.size) + h1.astype("<u8").tobytes()
+        n2 = b"2D Histogram"
+        b2 = struct.pack("<Q", len(n2)) + n2 + struct.pack("<QQQ", 6, 0, 0)
+        b2 += struct.pack("<fQQ", mg, *h2.shape) + h2.astype("<u8").tobytes()
+        tail = b1 + b2
+        toc_start = 9 + 32
+        head = head[:next_at] + struct.pack("<Q", pos - toc_start) + head[next_at + 8:]
     with open(path, "wb") as f:
         f.write(head + toc + table)
         for i in order:
             f.write(stored[i])
+        f.write(tail)
     return levels
 
 
@@ -142,3 +160,25 @@ def test_malformed_files_are_rejected(tmp_path):
     with pytest.raises(TvkError):
         t.read_brick(0, 0, 0, 0)  # reverse order: brick 0 is stored last
     t.close()
+
+
+def test_histogram_blocks_are_read(tmp_path):
+    vol = np.arange(12 * 12 * 12, dtype=np.uint8).reshape(12, 12, 12)
+    h1 = np.bincount(vol.ravel(), minlength=256).astype(np.uint64)
+    h2 = np.arange(256 * 256, dtype=np.uint64).reshape(256, 256)
+    path = str(tmp_path / "h.uvf")
+    write_toc(path, vol, 8, 2, 1, hists=(h1, h2, 0.25))
+    t = TocVolume(path)
+    assert np.array_equal(t.histogram1D(), h1)
+    got, mg = t.histogram2D()
+    assert np.array_equal(got, h2) and mg == 0.25
+    # bricks still read after the extra blocks
+    assert np.array_equal(t.read_brick(0, 0, 0, 0)[0], toc_bricks(vol, 8, 2)[0][2][0])
+    t.close()
+    # a file without histogram blocks reports their absence
+    p2 = str(tmp_path / "n.uvf")
+    write_toc(p2, vol, 8, 2, 1)
+    t2 = TocVolume(p2)
+    with pytest.raises(TvkError):
+        t2.histogram1D()
+    t2.close()

@@ -367,6 +367,14 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
                 "brick_budget": points[k] or None,
                 "budget_factor": cal[k][0], "calibration_worst_rmse": cal[k][1]}
 
+    # the predictor alone on one frame's inputs (every region x coarser level)
+    xs, _, _ = LL.plan_features(views(1, 14, *eval_size)[0], dims, edge, n_lod, hist, mom, detail)
+    mlp_ms = []
+    for _ in range(20):
+        pred(xs)
+        mlp_ms.append(pred.kernel_ms)
+    mlp_t = float(np.median(mlp_ms)) * 1e-3
+    mlp_flops = 28800.0 * len(xs)  # 2*(32*64 + 3*64*64 + 64) per row
     out = {
         "workload": f"learned-lod-{n}^3-u8-noise-tubes-{edge}^3-bricks-{n_lod}lods-"
                     f"{eval_size[0]}x{eval_size[1]}-{eval_views}views",
@@ -376,7 +384,8 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
         "training": {"views": train_views, "tfs": 4, "size": list(train_size)},
         "target": summary("target"),
         "budgeted": summary("budgeted"),
-        "mlp_kernel_ms": pred.kernel_ms,
+        "mlp": {"rows": int(len(xs)), "kernel_ms": mlp_t * 1e3, "tflops": mlp_flops / mlp_t / 1e12,
+                "mfma_peak_frac": mlp_flops / mlp_t / 2.5e15, "dtype": "bf16 x bf16 -> f32"},
         "train_rows": int(len(X)), "train_loss": loss,
         "seconds": {"volume": round(setup_s, 2), "training_data": round(data_s, 2),
                     "training": round(train_s, 2)},

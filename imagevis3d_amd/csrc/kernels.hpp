@@ -90,10 +90,14 @@ bool is_gpu_volume_kind(int kind);
 hipError_t launch_generate_volume(int kind, uint64_t seed, const int64_t dims[3], int bits,
                                   int64_t z0, int64_t nz, void* out, hipStream_t stream);
 
-// Learned-LOD error predictor (bf16 MLP 32 -> 4 x 64 -> 1 on MFMA): w = bf16
-// weights [in][out] per layer then the 64 output weights, b = f32 biases
-// (4 x 64 + 1), x = n x 32 f32 inputs, y = n outputs.
-hipError_t launch_lod_mlp(const uint16_t* w, const float* b, const float* x, float* y, int n,
+// Learned-LOD error predictor (bf16 MLP 32 -> 4 x 64 -> 1 on MFMA).
+// lod_mlp_pack turns the weights -- bf16 [in][out] per layer, then the 64
+// output weights -- into the kernel's LDS image (lod_mlp_packed_elems()
+// elements), once per predictor; launch_lod_mlp takes the packed weights,
+// b = f32 biases (4 x 64 + 1), x = n x 32 f32 inputs, y = n outputs.
+size_t lod_mlp_packed_elems();
+void lod_mlp_pack(const uint16_t* w, uint16_t* packed);
+hipError_t launch_lod_mlp(const uint16_t* w_packed, const float* b, const float* x, float* y, int n,
                           hipStream_t stream);
 
 // Kernel variant of a raycast launch: every job of the table shares it.

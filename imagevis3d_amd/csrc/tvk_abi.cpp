@@ -766,9 +766,11 @@ int tvk_lod_mlp_create(int device, const uint16_t* w, const float* b, tvk_lod_ml
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&m->ev[0]);
   if (e == hipSuccess) e = hipEventCreate(&m->ev[1]);
-  if (e == hipSuccess) e = m->w.alloc(kMlpWeights * 2);
+  std::vector<uint16_t> packed(tvk::lod_mlp_packed_elems());
+  tvk::lod_mlp_pack(w, packed.data());
+  if (e == hipSuccess) e = m->w.alloc(packed.size() * 2);
   if (e == hipSuccess) e = m->b.alloc(kMlpBiases * 4);
-  if (e == hipSuccess) e = hipMemcpy(m->w.p, w, kMlpWeights * 2, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(m->w.p, packed.data(), packed.size() * 2, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(m->b.p, b, kMlpBiases * 4, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     tvk::set_last_error(std::string("lod mlp setup: ") + hipGetErrorString(e));

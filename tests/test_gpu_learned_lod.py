@@ -45,6 +45,24 @@ def test_mlp_predictor_matches_references():
     pred.close()
 
 
+def test_mlp_ragged_row_counts():
+    """Row counts around the 64-row wave and 256-row work-group tiles."""
+    rng = np.random.default_rng(5)
+    sizes = [(32, 64), (64, 64), (64, 64), (64, 64)]
+    ws = [rng.normal(0, 1.0 / math.sqrt(i), (i, o)).astype(np.float32) for i, o in sizes]
+    wo = rng.normal(0, 0.125, 64).astype(np.float32)
+    wbits = np.concatenate([LL.to_bf16_bits(w).ravel() for w in ws] + [LL.to_bf16_bits(wo)])
+    b = rng.normal(0, 0.1, 4 * 64 + 1).astype(np.float32)
+    pred = LL.Predictor(wbits, b)
+    for n in (1, 17, 63, 64, 65, 255, 256, 257, 4097):
+        print(f"case lod mlp {n} rows", flush=True)
+        x = rng.normal(0, 1, (n, 32)).astype(np.float32)
+        y = pred(x)
+        ref = LL.reference_forward(wbits, b, x)
+        assert y.shape == (n,) and np.abs(y - ref).max() <= 2e-2 * (1 + np.abs(ref).max()), n
+    pred.close()
+
+
 def random_views(n, seed, w, h, fov=50.0):
     rng = np.random.default_rng(seed)
     out = []

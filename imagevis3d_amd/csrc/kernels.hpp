@@ -116,4 +116,10 @@ struct LaunchInfo {
 // must be zeroed first).
 hipError_t launch_raycast(const LaunchInfo& li, const JobTable* tab, hipStream_t stream);
 
+// Background of the pixels outside each job's tile rect, for every job of
+// the device table `tab` (a fixed grid of kBackgroundBlocks work-groups that
+// strides over all jobs' pixels).
+constexpr int kBackgroundBlocks = 1024;
+hipError_t launch_background(const JobTable* tab, hipStream_t stream);
+
 }  // namespace tvk

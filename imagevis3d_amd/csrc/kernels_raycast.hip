@@ -404,10 +404,26 @@ __device__ inline int march_regions(const RaycastParams& P, const float4* __rest
   return dominant;
 }
 
+// Premultiplied ray colour (C, A) over row py's background gradient, as the
+// framebuffer's RGBA8.  A ray that misses the volume has C = A = 0 and gets
+// the background itself; the background kernel calls this with zeros too,
+// so both paths write the same bytes (explicit fmaf: no contraction choice
+// left to the compiler).
+__device__ inline uint32_t over_background(const RaycastParams& P, int py, float C0, float C1, float C2,
+                                           float A) {
+  const float wy = 1.f - ((float)py + 0.5f) / (float)P.height;
+  const float bg0 = fmaf(P.bg_top[0] - P.bg_bottom[0], wy, P.bg_bottom[0]);
+  const float bg1 = fmaf(P.bg_top[1] - P.bg_bottom[1], wy, P.bg_bottom[1]);
+  const float bg2 = fmaf(P.bg_top[2] - P.bg_bottom[2], wy, P.bg_bottom[2]);
+  const float T1 = 1.f - A;
+  return pack_rgba8(fmaf(T1, bg0, C0), fmaf(T1, bg1, C1), fmaf(T1, bg2, C2), fmaf(T1, P.bg_alpha, A));
+}
+
+// Ray tile `tile` of job P, numbered row-major over the job's tile rect.
 template <typename T, bool LINEAR, int BATCH, int MODE, bool TF_LDS>
 __device__ inline void render_tile(const RaycastParams& P, const float4* __restrict__ tf4, int tile,
                                    int lane, uint32_t& n_eval, uint32_t& n_nom) {
-  const int tx = tile % P.tiles_x, ty = tile / P.tiles_x;
+  const int tx = P.rect_x0 + tile % P.rect_w, ty = P.rect_y0 + tile / P.rect_w;
   const int sh = P.tile_shift;
   const int px = (tx << sh) + (lane & ((1 << sh) - 1));
   const int py = ty * (64 >> sh) + (lane >> sh);
@@ -705,13 +721,7 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
     C2 = c[2] * c[3];
     A = c[3];
   }
-  const float wy = 1.f - ((float)py + 0.5f) / (float)P.height;
-  const float bg0 = P.bg_bottom[0] + (P.bg_top[0] - P.bg_bottom[0]) * wy;
-  const float bg1 = P.bg_bottom[1] + (P.bg_top[1] - P.bg_bottom[1]) * wy;
-  const float bg2 = P.bg_bottom[2] + (P.bg_top[2] - P.bg_bottom[2]) * wy;
-  const float T1 = 1.f - A;
-  gstore(P.framebuffer + ((int64_t)py * P.width + px),
-         pack_rgba8(fmaf(T1, bg0, C0), fmaf(T1, bg1, C1), fmaf(T1, bg2, C2), fmaf(T1, P.bg_alpha, A)));
+  gstore(P.framebuffer + ((int64_t)py * P.width + px), over_background(P, py, C0, C1, C2, A));
 }
 
 // Empty flag per empty-space cell for one TF: the TF entries (1D) or
@@ -734,6 +744,28 @@ __global__ void cell_empty_kernel(const uint32_t* __restrict__ cell_range, int64
     empty = prefix[i1 + 1] - prefix[i0] == 0;
   }
   out[i] = empty ? 1 : 0;
+}
+
+// Background of every pixel outside its job's tile rect (such rays miss the
+// volume box): the bytes render_tile writes for a missed ray, and no
+// dominant region.  Every work-group strides over every job's pixels, so
+// one grid size serves any job count (graph replay).  Coalesced stores, no
+// queue atomics.
+__global__ __launch_bounds__(256) void background_kernel(const JobTable* __restrict__ tab) {
+  const int n_jobs = tab->h.n_jobs;
+  for (int job = 0; job < n_jobs; ++job) {
+    const RaycastParams& P = tab->jobs[job];
+    const int64_t npix = (int64_t)P.width * P.height;
+    const int th = 64 >> P.tile_shift;
+    const int x0 = P.rect_x0 << P.tile_shift, x1 = (P.rect_x0 + P.rect_w) << P.tile_shift;
+    const int y0 = P.rect_y0 * th, y1 = (P.rect_y0 + P.rect_h) * th;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < npix; i += (int64_t)gridDim.x * 256) {
+      const int py = (int)(i / P.width), px = (int)(i - (int64_t)py * P.width);
+      if (px >= x0 && px < x1 && py >= y0 && py < y1) continue;
+      gstore(P.framebuffer + i, over_background(P, py, 0.f, 0.f, 0.f, 0.f));
+      if (P.region_out) gstore(P.region_out + i, (int32_t)-1);
+    }
+  }
 }
 
 // Persistent raycast over a job table: every wave loops over ray tiles
@@ -759,7 +791,7 @@ __global__ __launch_bounds__(kThreads, 2) void raycast_kernel(const JobTable* __
   const int G = H.tile_group;
   // xcd_split regions apply to one-job launches only
   const int split = n_jobs == 1 ? H.xcd_split : 0;
-  const int tiles_x0 = tab->jobs[0].tiles_x, tiles_y0 = tab->jobs[0].tiles_y;
+  const int tiles_x0 = tab->jobs[0].rect_w, tiles_y0 = tab->jobs[0].rect_h;
   // per-lane sample counts of the job the wave is on; added to the
   // work-group's LDS counters when the wave moves to another job and at the
   // end, then the work-group adds its counters to the job's global ones
@@ -876,6 +908,11 @@ hipError_t launch_cell_empty(const uint32_t* cell_range, int64_t n_cells, const 
   const unsigned blocks = (unsigned)((n_cells + 255) / 256);
   cell_empty_kernel<<<blocks, 256, 0, stream>>>(cell_range, n_cells, prefix, n_tf, tf_scale,
                                                 mode == 2 ? kModeIso : mode, iso_value, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_background(const JobTable* tab, hipStream_t stream) {
+  background_kernel<<<kBackgroundBlocks, 256, 0, stream>>>(tab);
   return hipGetLastError();
 }
 

@@ -136,6 +136,68 @@ void normalize(double a[3]) {
 
 bool finite3(const float v[3]) { return isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]); }
 
+double dot3(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// The job's tile rect: the tiles whose rays can reach the volume box
+// [0, box].  Pixel (x, y)'s ray is eye + s (dir00 + x du + y dv), s >= 0
+// (perspective) or eye + x du + y dv + s odir (orthographic), so with the
+// basis M = [du dv c] (c = dir00 or odir) a point q has M^-1 (q - eye) =
+// (s x, s y, s) resp. (x, y, s).  When every box corner lies in front of
+// the eye plane (s > 0; always taken for orthographic), the pixels whose
+// rays meet the box are the convex hull of the projected corners, so
+// their bounding rectangle grown by 2 pixels (far beyond float rounding of
+// the rays) holds every pixel that can hit; tiles outside it get only the
+// background.  A corner on or behind the eye plane (eye inside or beside
+// the box) or a degenerate basis keeps the full frame.
+void set_tile_rect(RaycastParams* p) {
+  p->rect_x0 = p->rect_y0 = 0;
+  p->rect_w = p->tiles_x;
+  p->rect_h = p->tiles_y;
+  double c0[3], c1[3], c2[3], eye[3];
+  for (int i = 0; i < 3; ++i) {
+    c0[i] = p->du[i];
+    c1[i] = p->dv[i];
+    c2[i] = p->ortho ? p->odir[i] : p->dir00[i];
+    eye[i] = p->eye[i];
+  }
+  double r0[3], r1[3], r2[3];  // rows of det * M^-1
+  cross(c1, c2, r0);
+  cross(c2, c0, r1);
+  cross(c0, c1, r2);
+  const double det = dot3(c0, r0);
+  const double scale = sqrt(dot3(c0, c0) * dot3(c1, c1) * dot3(c2, c2));
+  if (!(fabs(det) > 1e-9 * scale)) return;
+  double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+  for (int k = 0; k < 8; ++k) {
+    double q[3];
+    for (int i = 0; i < 3; ++i) q[i] = ((k >> i) & 1 ? (double)p->box[i] : 0.0) - eye[i];
+    double x = dot3(r0, q) / det, y = dot3(r1, q) / det;
+    if (!p->ortho) {
+      const double s = dot3(r2, q) / det;
+      if (!(s > 0.0)) return;
+      x /= s;
+      y /= s;
+    }
+    if (!std::isfinite(x) || !std::isfinite(y)) return;
+    xmin = std::min(xmin, x);
+    xmax = std::max(xmax, x);
+    ymin = std::min(ymin, y);
+    ymax = std::max(ymax, y);
+  }
+  const double W = p->width, H = p->height;
+  const double x0 = std::max(floor(xmin) - 2.0, 0.0), x1 = std::min(ceil(xmax) + 2.0, W - 1.0);
+  const double y0 = std::max(floor(ymin) - 2.0, 0.0), y1 = std::min(ceil(ymax) + 2.0, H - 1.0);
+  const int tw = 1 << p->tile_shift, th = 64 >> p->tile_shift;
+  if (x0 > x1 || y0 > y1) {  // the box projects outside the image
+    p->rect_w = p->rect_h = 0;
+    return;
+  }
+  p->rect_x0 = (int)x0 / tw;
+  p->rect_y0 = (int)y0 / th;
+  p->rect_w = (int)x1 / tw + 1 - p->rect_x0;
+  p->rect_h = (int)y1 / th + 1 - p->rect_y0;
+}
+
 }  // namespace
 
 // -------------------------------------------------------------- lifecycle
@@ -1106,6 +1168,7 @@ void HipRenderer::build_params(RaycastParams* p, int lod, const CameraState& cam
   const int tile_h = 64 >> tile_shift_;
   p->tiles_y = (p->height + tile_h - 1) / tile_h;
   for (int i = 0; i < 3; ++i) p->box[i] = (float)dims0_[i];
+  set_tile_rect(p);
   p->step = (float)(ldexp(1.0, lod) / (double)rate_);
   p->nominal_rate = rate_;
   p->term_alpha = term_alpha_;
@@ -1193,9 +1256,11 @@ int launch_job_table(LaunchSlot& sl, const RaycastParams* jobs, int n, const Lau
   t->h.tf_lds = li.n_tf_lds > 0 ? jobs[0].tf : nullptr;
   t->h.counters = sl.counters.as<unsigned long long>();
   int total = 0;
+  bool culled = false;  // some job has background tiles outside its rect
   for (int k = 0; k < n; ++k) {
     t->tile_start[k] = total;
-    total += jobs[k].tiles_x * jobs[k].tiles_y;
+    total += jobs[k].rect_w * jobs[k].rect_h;
+    culled = culled || jobs[k].rect_w * jobs[k].rect_h != jobs[k].tiles_x * jobs[k].tiles_y;
     t->jobs[k] = jobs[k];
   }
   t->tile_start[n] = total;
@@ -1208,6 +1273,7 @@ int launch_job_table(LaunchSlot& sl, const RaycastParams* jobs, int n, const Lau
     TVK_HIP(hipMemcpyAsync(sl.table.p, t, table_bytes, hipMemcpyHostToDevice, stream_));
     TVK_HIP(launch_fill_u64(sl.counters.as<unsigned long long>(), words, 0ull, stream_));
     TVK_HIP(hipEventRecord(sl.ev[1], stream_));
+    if (culled) TVK_HIP(launch_background(sl.table.as<JobTable>(), stream_));
     TVK_HIP(launch_raycast(li, sl.table.as<JobTable>(), stream_));
     TVK_HIP(hipEventRecord(sl.ev[2], stream_));
     TVK_HIP(hipMemcpyAsync(sl.counters_h, sl.counters.p, words * sizeof(unsigned long long),
@@ -1230,6 +1296,7 @@ int launch_job_table(LaunchSlot& sl, const RaycastParams* jobs, int n, const Lau
     hipError_t e = hipMemcpyAsync(sl.table.p, t, sizeof(JobTable), hipMemcpyHostToDevice, stream_);
     if (e == hipSuccess)
       e = launch_fill_u64(sl.counters.as<unsigned long long>(), kCounterWords, 0ull, stream_);
+    if (e == hipSuccess) e = launch_background(sl.table.as<JobTable>(), stream_);
     if (e == hipSuccess) e = launch_raycast(li, sl.table.as<JobTable>(), stream_);
     if (e == hipSuccess)
       e = hipMemcpyAsync(sl.counters_h, sl.counters.p, kCounterWords * sizeof(unsigned long long),

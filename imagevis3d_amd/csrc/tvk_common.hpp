@@ -89,6 +89,12 @@ struct RaycastParams {
   int32_t width, height;
   int32_t tile_shift;   // ray tile = (1 << tile_shift) x (64 >> tile_shift) rays
   int32_t tiles_x, tiles_y;
+  // The tiles whose rays can reach the volume box: the rectangle
+  // [rect_x0, rect_x0 + rect_w) x [rect_y0, rect_y0 + rect_h) of the tile
+  // grid (the box's projected bounding rectangle plus a margin).  Only
+  // these enter the raycast's tile queues; the background kernel writes
+  // the job's other pixels.
+  int32_t rect_x0, rect_y0, rect_w, rect_h;
   float box[3];         // LOD-0 dims (the volume box is [0, box])
   float step;           // sample spacing in LOD-0 voxels (2^lod / rate)
   float nominal_rate;   // user rate: nominal samples per LOD-0 voxel

@@ -12,6 +12,11 @@
 //    belongs to XCD g%8, so each XCD's L2 sees neighbouring rays while all
 //    XCDs get an even share of the frame; a wave whose own queue is empty
 //    steals from the other XCDs' queues, which absorbs the long-ray tail.
+//    Only the tiles inside the volume box's projected rectangle enter the
+//    queues (a plain fill kernel writes the other pixels' background: each
+//    queue ticket is a device-scope atomic, and a frame's empty tiles used
+//    to be dealt at the atomics' rate), and their rows are dealt
+//    centre-out, long rays first.
 //  - The 1D TF is staged once per work-group into LDS (up to 4096 RGBA
 //    entries = 64 KB), so the two TF reads per sample are LDS reads and the
 //    vector-memory path only carries voxel gathers.
@@ -419,11 +424,26 @@ __device__ inline uint32_t over_background(const RaycastParams& P, int py, float
   return pack_rgba8(fmaf(T1, bg0, C0), fmaf(T1, bg1, C1), fmaf(T1, bg2, C2), fmaf(T1, P.bg_alpha, A));
 }
 
-// Ray tile `tile` of job P, numbered row-major over the job's tile rect.
+// The k-th tile row of an h-row rect in centre-out order: c, c+1, c-1,
+// c+2, c-2, ... (c = h/2), then the rest of the longer side.  The rows
+// through the middle of the box's projection hold the longest rays, so they
+// are dealt first and the rows near the rect's edges, whose rays are short,
+// end the frame (in top-down order the middle rows started about a third
+// of the way in and a few of their long tiles ran on alone at the end).
+// A bijection of [0, h).
+__device__ inline int centre_out_row(int k, int h) {
+  const int c = h >> 1, below = c, above = h - 1 - c, m = min(below, above);
+  if (k == 0) return c;
+  if (k <= 2 * m) return (k & 1) ? c + ((k + 1) >> 1) : c - (k >> 1);
+  return above > below ? c + (k - m) : c - (k - m);
+}
+
+// Ray tile `tile` of job P: row-major over the job's tile rect, rows taken
+// centre-out.
 template <typename T, bool LINEAR, int BATCH, int MODE, bool TF_LDS>
 __device__ inline void render_tile(const RaycastParams& P, const float4* __restrict__ tf4, int tile,
                                    int lane, uint32_t& n_eval, uint32_t& n_nom) {
-  const int tx = P.rect_x0 + tile % P.rect_w, ty = P.rect_y0 + tile / P.rect_w;
+  const int tx = P.rect_x0 + tile % P.rect_w, ty = P.rect_y0 + centre_out_row(tile / P.rect_w, P.rect_h);
   const int sh = P.tile_shift;
   const int px = (tx << sh) + (lane & ((1 << sh) - 1));
   const int py = ty * (64 >> sh) + (lane >> sh);

@@ -72,6 +72,13 @@ class Tuning(C.Structure):
     ]
 
 
+class ClearView(C.Structure):
+    """tvk_clearview."""
+    _fields_ = [("enabled", C.c_int32), ("iso_value", C.c_float), ("color", C.c_float * 3),
+                ("size", C.c_float), ("context_scale", C.c_float), ("border_scale", C.c_float),
+                ("focus", C.c_int32 * 2)]
+
+
 class TocInfo(C.Structure):
     """tvk_toc_info."""
     _fields_ = [
@@ -143,6 +150,14 @@ SIGNATURES = {
     "tvk_get_use_mip": [H, C.POINTER(i32)],
     "tvk_set_mip_rotation_angle": [H, f32],
     "tvk_set_isosurface_color": [H, f32, f32, f32],
+    "tvk_set_clearview_enabled": [H, i32],
+    "tvk_set_cv_iso_value": [H, f32],
+    "tvk_set_cv_color": [H, f32, f32, f32],
+    "tvk_set_cv_size": [H, f32],
+    "tvk_set_cv_context_scale": [H, f32],
+    "tvk_set_cv_border_scale": [H, f32],
+    "tvk_set_cv_focus_pos": [H, i32, i32],
+    "tvk_get_clearview": [H, C.POINTER(ClearView)],
     "tvk_set_view_pos": [H, f32, f32, f32],
     "tvk_set_view_dir": [H, f32, f32, f32],
     "tvk_set_up_dir": [H, f32, f32, f32],

@@ -302,6 +302,42 @@ __device__ inline void shade(const RaycastParams& P, const float g[3], const flo
   for (int i = 0; i < 3; ++i) c[i] = c[i] * fmaf(P.kd[i], nl, P.ka[i]) + P.ks[i] * sp;
 }
 
+// Isosurface hit at sample kh (value vh >= iso, sample kh-1 below it): the
+// crossing refined linearly against sample kh-1 (re-fetched from the level,
+// so the result does not depend on skipping), `col` shaded there with the
+// central-difference gradient into c.  Returns |N.V| (N the unit gradient,
+// V = -d): ClearView's silhouette term of the context surface.
+template <typename T, bool LINEAR>
+__device__ inline float iso_shade(const RaycastParams& P, const LodInfo& lv, const float o[3],
+                                  const float d[3], const float Lh[6], int kh, int n, float t0,
+                                  float step, float lastlen, float vh, float iso, const float col[3],
+                                  float c[4]) {
+  const float lenh = (kh == n - 1) ? lastlen : step;
+  const float th0 = fmaf((float)kh, step, t0) + 0.5f * lenh;
+  float th = th0;
+  if (kh > 0) {
+    const float tp = fmaf((float)(kh - 1), step, t0) + 0.5f * step;
+    const float vp = fetch_level<T, LINEAR>(P, (o[0] + tp * d[0]) * lv.to_level[0],
+                                            (o[1] + tp * d[1]) * lv.to_level[1],
+                                            (o[2] + tp * d[2]) * lv.to_level[2]);
+    if (vp < iso && vh > vp) th = tp + (iso - vp) / (vh - vp) * (th0 - tp);
+  }
+  const float qx = (o[0] + th * d[0]) * lv.to_level[0];
+  const float qy = (o[1] + th * d[1]) * lv.to_level[1];
+  const float qz = (o[2] + th * d[2]) * lv.to_level[2];
+  float gh[3];
+  gh[0] = (fetch_level<T, true>(P, qx + 1.f, qy, qz) - fetch_level<T, true>(P, qx - 1.f, qy, qz)) * P.grad_scale[0];
+  gh[1] = (fetch_level<T, true>(P, qx, qy + 1.f, qz) - fetch_level<T, true>(P, qx, qy - 1.f, qz)) * P.grad_scale[1];
+  gh[2] = (fetch_level<T, true>(P, qx, qy, qz + 1.f) - fetch_level<T, true>(P, qx, qy, qz - 1.f)) * P.grad_scale[2];
+  c[0] = col[0];
+  c[1] = col[1];
+  c[2] = col[2];
+  c[3] = 1.f;
+  shade(P, gh, Lh, c);
+  const float s = 1.f / sqrtf(gh[0] * gh[0] + gh[1] * gh[1] + gh[2] * gh[2] + 1e-8f);
+  return fabsf((gh[0] * d[0] + gh[1] * d[1] + gh[2] * d[2]) * s);
+}
+
 // Render the 64 rays of ray tile `tile` (lane = ray within the 8x8 tile).
 // `tf4` points at the 1D TF (LDS or global).
 // Per-region LOD march (1D TF): the ray walks the LOD-0 brick grid; region
@@ -491,6 +527,14 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
   const float L = t1 - t0;
   float C0 = 0.f, C1 = 0.f, C2 = 0.f, A = 0.f;
   float vmax = -1.f;  // MIP: largest sample value so far (-1: none)
+  // ClearView: this pixel's focus weight; after the context hit (stage 1)
+  // its shaded colour and opacity
+  float wf = 0.f, cv_ac = 1.f, cc[3] = {0.f, 0.f, 0.f};
+  int cv_stage = 0;
+  if (MODE == kModeIso && P.cv) {
+    const float dx = (float)px - P.cv_focus[0], dy = (float)py - P.cv_focus[1];
+    wf = clampf(1.f - P.cv_size * (sqrtf(dx * dx + dy * dy) / (float)P.height), 0.f, 1.f);
+  }
   if (MODE == kModeRegions) {
     int dominant = -1;
     if (hit && L > 0.f) {
@@ -651,32 +695,38 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
           for (int j = 0; j < BATCH; ++j) {
             if (k + j < kend && live) {
               ++n_eval;
-              if (v[j] >= P.iso_value) {
-                const int kh = k + j;
-                const float lenh = (kh == n - 1) ? lastlen : step;
-                const float th0 = fmaf((float)kh, step, t0) + 0.5f * lenh;
-                float th = th0;
-                if (kh > 0) {
-                  const float tp = fmaf((float)(kh - 1), step, t0) + 0.5f * step;
-                  const float vp = fetch_level<T, LINEAR>(P, (o[0] + tp * d[0]) * lv.to_level[0],
-                                                          (o[1] + tp * d[1]) * lv.to_level[1],
-                                                          (o[2] + tp * d[2]) * lv.to_level[2]);
-                  if (vp < P.iso_value && v[j] > vp)
-                    th = tp + (P.iso_value - vp) / (v[j] - vp) * (th0 - tp);
+              // stage 0: the context surface; with ClearView and the pixel
+              // in the focus (wf > 0), stage 1 then looks for the focus
+              // surface from the same sample on
+              if (cv_stage == 0 && v[j] >= P.iso_value) {
+                float c[4];
+                const float nv = iso_shade<T, LINEAR>(P, lv, o, d, Lh, k + j, n, t0, step, lastlen, v[j],
+                                                      P.iso_value, P.iso_color, c);
+                if (wf > 0.f) {
+                  cc[0] = c[0];
+                  cc[1] = c[1];
+                  cc[2] = c[2];
+                  const float x = 1.f - nv, b = P.cv_border_scale;
+                  const float pw = x > 0.f ? exp2f(b * log2f(x)) : (b == 0.f ? 1.f : 0.f);
+                  cv_ac = fmaf(wf, clampf(P.cv_context_scale * pw, 0.f, 1.f) - 1.f, 1.f);
+                  cv_stage = 1;
+                } else {
+                  C0 = c[0];
+                  C1 = c[1];
+                  C2 = c[2];
+                  A = 1.f;
+                  live = false;
                 }
-                const float qx = (o[0] + th * d[0]) * lv.to_level[0];
-                const float qy = (o[1] + th * d[1]) * lv.to_level[1];
-                const float qz = (o[2] + th * d[2]) * lv.to_level[2];
-                float gh[3];
-                gh[0] = (fetch_level<T, true>(P, qx + 1.f, qy, qz) - fetch_level<T, true>(P, qx - 1.f, qy, qz)) * P.grad_scale[0];
-                gh[1] = (fetch_level<T, true>(P, qx, qy + 1.f, qz) - fetch_level<T, true>(P, qx, qy - 1.f, qz)) * P.grad_scale[1];
-                gh[2] = (fetch_level<T, true>(P, qx, qy, qz + 1.f) - fetch_level<T, true>(P, qx, qy, qz - 1.f)) * P.grad_scale[2];
-                float c[4] = {P.iso_color[0], P.iso_color[1], P.iso_color[2], 1.f};
-                shade(P, gh, Lh, c);
-                C0 = c[0];
-                C1 = c[1];
-                C2 = c[2];
+              }
+              if (cv_stage == 1 && v[j] >= P.cv_iso_value) {
+                float c[4];
+                (void)iso_shade<T, LINEAR>(P, lv, o, d, Lh, k + j, n, t0, step, lastlen, v[j],
+                                           P.cv_iso_value, P.cv_color, c);
+                C0 = fmaf(cv_ac, cc[0] - c[0], c[0]);
+                C1 = fmaf(cv_ac, cc[1] - c[1], c[1]);
+                C2 = fmaf(cv_ac, cc[2] - c[2], c[2]);
                 A = 1.f;
+                cv_stage = 2;
                 live = false;
               }
             }
@@ -732,6 +782,12 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
       }
       k = live ? kend : n;
     }
+  }
+  if (MODE == kModeIso && cv_stage == 1) {  // context hit, no focus surface behind it
+    C0 = cv_ac * cc[0];
+    C1 = cv_ac * cc[1];
+    C2 = cv_ac * cc[2];
+    A = cv_ac;
   }
   if (MODE == kModeMIP && vmax >= 0.f) {
     float c[4];

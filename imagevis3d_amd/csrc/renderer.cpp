@@ -866,6 +866,71 @@ int HipRenderer::SetIsosurfaceColor(float r, float g, float b) {
   return TVK_OK;
 }
 
+int HipRenderer::SetClearViewEnabled(bool on) {
+  cv_enabled_ = on;
+  invalidate();
+  return TVK_OK;
+}
+
+int HipRenderer::SetCVIsoValue(float v) {
+  if (!isfinite(v)) return arg_fail("ClearView iso value must be finite");
+  cv_iso_ = v;
+  invalidate();
+  return TVK_OK;
+}
+
+int HipRenderer::SetCVColor(float r, float g, float b) {
+  if (!isfinite(r) || !isfinite(g) || !isfinite(b)) return arg_fail("ClearView colour must be finite");
+  cv_color_[0] = r;
+  cv_color_[1] = g;
+  cv_color_[2] = b;
+  invalidate();
+  return TVK_OK;
+}
+
+int HipRenderer::SetCVSize(float v) {
+  if (!isfinite(v) || v < 0.f) return arg_fail("ClearView size must be finite and >= 0");
+  cv_size_ = v;
+  invalidate();
+  return TVK_OK;
+}
+
+int HipRenderer::SetCVContextScale(float v) {
+  if (!isfinite(v) || v < 0.f) return arg_fail("ClearView context scale must be finite and >= 0");
+  cv_context_ = v;
+  invalidate();
+  return TVK_OK;
+}
+
+int HipRenderer::SetCVBorderScale(float v) {
+  if (!isfinite(v) || v < 0.f) return arg_fail("ClearView border scale must be finite and >= 0");
+  cv_border_ = v;
+  invalidate();
+  return TVK_OK;
+}
+
+int HipRenderer::SetCVFocusPos(int32_t x, int32_t y) {
+  cv_focus_[0] = x;
+  cv_focus_[1] = y;
+  invalidate();
+  return TVK_OK;
+}
+
+void HipRenderer::GetClearView(tvk_clearview* out) const {
+  out->enabled = cv_enabled_ ? 1 : 0;
+  out->iso_value = cv_iso_;
+  for (int i = 0; i < 3; ++i) out->color[i] = cv_color_[i];
+  out->size = cv_size_;
+  out->context_scale = cv_context_;
+  out->border_scale = cv_border_;
+  out->focus[0] = cv_focus_[0];
+  out->focus[1] = cv_focus_[1];
+}
+
+float HipRenderer::iso_skip_threshold() const {
+  return cv_enabled_ ? std::min(iso_value_, cv_iso_) : iso_value_;
+}
+
 int HipRenderer::upload_tf() {
   bool copied = false;
   for (TfSet& t : tfs_) {
@@ -1227,6 +1292,14 @@ void HipRenderer::build_params(RaycastParams* p, int lod, const CameraState& cam
   }
   p->iso_value = iso_value_;
   for (int i = 0; i < 3; ++i) p->iso_color[i] = iso_color_[i];
+  p->cv = (cv_enabled_ && p->mode == 2) ? 1 : 0;
+  p->cv_iso_value = cv_iso_;
+  for (int i = 0; i < 3; ++i) p->cv_color[i] = cv_color_[i];
+  p->cv_focus[0] = (float)cv_focus_[0];
+  p->cv_focus[1] = (float)cv_focus_[1];
+  p->cv_size = cv_size_;
+  p->cv_context_scale = cv_context_;
+  p->cv_border_scale = cv_border_;
   p->lods = lods_dev_.as<LodInfo>();
   if (regions_active() && p->mode == 0 && !p->lighting) p->region_code = tfs.region_code.as<uint8_t>();
 }
@@ -1335,7 +1408,8 @@ int HipRenderer::prepare_cells(int tf_slot, int mode) {
   const int n = (int)prefix.size() - 1;
   const float scale = tf_scale(n);
   uint32_t iso_bits = 0, scale_bits = 0;
-  memcpy(&iso_bits, &iso_value_, 4);
+  const float iso = iso_skip_threshold();
+  memcpy(&iso_bits, &iso, 4);
   memcpy(&scale_bits, &scale, 4);
   // FNV-1a over what the flags depend on
   uint64_t key = 1469598103934665603ull;
@@ -1348,7 +1422,7 @@ int HipRenderer::prepare_cells(int tf_slot, int mode) {
   const int32_t* prefix_dev = mode == 1 ? t.prefix2d_dev.as<int32_t>() : t.prefix1d_dev.as<int32_t>();
   if (mode != 2 && (n < 1 || !prefix_dev)) return state_fail("TF not uploaded");
   TVK_HIP(launch_cell_empty(cell_range_.as<uint32_t>(), total_cells_, prefix_dev, n, scale, mode,
-                            iso_value_, flags.as<uint8_t>(), stream_));
+                            iso, flags.as<uint8_t>(), stream_));
   t.cell_key[mode] = key;
   return TVK_OK;
 }
@@ -1369,7 +1443,7 @@ uint32_t HipRenderer::bricks_skipped(int lod, int tf_slot) const {
     const float mn = pool_.brick_range_h[2 * (L.brick_base + b)];
     const float mx = pool_.brick_range_h[2 * (L.brick_base + b) + 1];
     if (mode == 2) {
-      if (mx < iso_value_) ++skipped;
+      if (mx < iso_skip_threshold()) ++skipped;
       continue;
     }
     if (ntf < 1) continue;

@@ -172,6 +172,16 @@ class HipRenderer {
   bool GetUseMIP() const { return use_mip_; }
   int SetMIPRotationAngle(float deg);
   int SetIsosurfaceColor(float r, float g, float b);
+  // ClearView (isosurface mode): focus surface, colour, focus disc, context
+  // and border scales; the focus position is in pixels, row 0 at the top
+  int SetClearViewEnabled(bool on);
+  int SetCVIsoValue(float v);
+  int SetCVColor(float r, float g, float b);
+  int SetCVSize(float s);
+  int SetCVContextScale(float s);
+  int SetCVBorderScale(float s);
+  int SetCVFocusPos(int32_t x, int32_t y);
+  void GetClearView(tvk_clearview* out) const;
 
   int SetViewPos(const float v[3]);
   int SetViewDir(const float v[3]);
@@ -337,6 +347,14 @@ class HipRenderer {
   bool use_mip_ = false;     // maximum intensity projection (1D TF mode)
   float mip_angle_ = 0.f;    // MIP view rotation about the up vector, degrees
   float iso_color_[3] = {1.f, 1.f, 1.f};
+  bool cv_enabled_ = false;
+  float cv_iso_ = 0.f;
+  float cv_color_[3] = {1.f, 0.f, 0.f};
+  float cv_size_ = 5.5f, cv_context_ = 1.f, cv_border_ = 60.f;
+  int32_t cv_focus_[2] = {0, 0};
+  // lowest iso value the isosurface march looks for (the context's, or the
+  // focus's when ClearView is on and it is lower): cells below it are skipped
+  float iso_skip_threshold() const;
 
   // camera
   CameraState cam_;

@@ -490,6 +490,51 @@ int tvk_set_isosurface_color(tvk_renderer* r, float red, float green, float blue
   return r->impl.SetIsosurfaceColor(red, green, blue);
 }
 
+int tvk_set_clearview_enabled(tvk_renderer* r, int enabled) {
+  TVK_GUARD(r);
+  return r->impl.SetClearViewEnabled(enabled != 0);
+}
+
+int tvk_set_cv_iso_value(tvk_renderer* r, float iso) {
+  TVK_GUARD(r);
+  return r->impl.SetCVIsoValue(iso);
+}
+
+int tvk_set_cv_color(tvk_renderer* r, float red, float green, float blue) {
+  TVK_GUARD(r);
+  return r->impl.SetCVColor(red, green, blue);
+}
+
+int tvk_set_cv_size(tvk_renderer* r, float size) {
+  TVK_GUARD(r);
+  return r->impl.SetCVSize(size);
+}
+
+int tvk_set_cv_context_scale(tvk_renderer* r, float scale) {
+  TVK_GUARD(r);
+  return r->impl.SetCVContextScale(scale);
+}
+
+int tvk_set_cv_border_scale(tvk_renderer* r, float scale) {
+  TVK_GUARD(r);
+  return r->impl.SetCVBorderScale(scale);
+}
+
+int tvk_set_cv_focus_pos(tvk_renderer* r, int32_t x, int32_t y) {
+  TVK_GUARD(r);
+  return r->impl.SetCVFocusPos(x, y);
+}
+
+int tvk_get_clearview(tvk_renderer* r, tvk_clearview* out) {
+  TVK_GUARD(r);
+  if (!out) {
+    tvk::set_last_error("null output");
+    return TVK_ERR_ARG;
+  }
+  r->impl.GetClearView(out);
+  return TVK_OK;
+}
+
 int tvk_set_view_pos(tvk_renderer* r, float x, float y, float z) {
   TVK_GUARD(r);
   const float v[3] = {x, y, z};

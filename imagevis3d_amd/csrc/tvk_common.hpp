@@ -125,6 +125,17 @@ struct RaycastParams {
   float tf2d_row_scale; // (tf2d_h - 1) / max gradient
   float iso_value;      // raw data units
   float iso_color[3];
+  // ClearView (mode 2, cv != 0): the iso_value surface is the context; in a
+  // focus disc around cv_focus (pixel) it fades to its silhouettes and the
+  // first cv_iso_value crossing behind it shows in cv_color.  Focus weight
+  // w = clamp(1 - cv_size * r, 0, 1), r = distance from cv_focus in image
+  // heights; context opacity (1 - w) + w clamp(cv_context_scale
+  // (1 - |N.V|)^cv_border_scale, 0, 1).
+  int32_t cv;
+  float cv_iso_value;
+  float cv_color[3];
+  float cv_focus[2];
+  float cv_size, cv_context_scale, cv_border_scale;
   const void* pool;
   const int32_t* page_table;
   const float* brick_range;     // 2 floats per brick

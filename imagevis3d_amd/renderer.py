@@ -350,6 +350,65 @@ class Renderer:
     def setIsosurfaceColor(self, c: Sequence[float]) -> None:
         call("tvk_set_isosurface_color", self._h, *[float(x) for x in c[:3]])
 
+    # ------------------------------------------------------------ ClearView
+    def supportsClearView(self) -> bool:
+        return True
+
+    def getClearViewDisabledReason(self) -> str:
+        return ""
+
+    def setClearViewEnabled(self, on: bool) -> None:
+        """ClearView in isosurface mode (RenderWindow.cpp:2348-2356)."""
+        call("tvk_set_clearview_enabled", self._h, int(bool(on)))
+
+    def setCVIsoValue(self, v: float) -> None:
+        call("tvk_set_cv_iso_value", self._h, float(v))
+
+    def setCVColor(self, c: Sequence[float]) -> None:
+        call("tvk_set_cv_color", self._h, *[float(x) for x in c[:3]])
+
+    def setCVSize(self, v: float) -> None:
+        call("tvk_set_cv_size", self._h, float(v))
+
+    def setCVContextScale(self, v: float) -> None:
+        call("tvk_set_cv_context_scale", self._h, float(v))
+
+    def setCVBorderScale(self, v: float) -> None:
+        call("tvk_set_cv_border_scale", self._h, float(v))
+
+    def setCVFocusPos(self, *args) -> None:
+        """setCVFocusPos(region, (x, y)) as the reference calls it
+        (RenderWindow.cpp:2359-2370), or setCVFocusPos((x, y)): the focus in
+        pixels, row 0 at the top.  The region is the raycast view."""
+        x, y = args[-1]
+        call("tvk_set_cv_focus_pos", self._h, int(x), int(y))
+
+    def _clearview(self) -> _lib.ClearView:
+        cv = _lib.ClearView()
+        call("tvk_get_clearview", self._h, C.byref(cv))
+        return cv
+
+    def getClearViewEnabled(self) -> bool:
+        return bool(self._clearview().enabled)
+
+    def getCVIsoValue(self) -> float:
+        return self._clearview().iso_value
+
+    def getCVSize(self) -> float:
+        return self._clearview().size
+
+    def getCVContextScale(self) -> float:
+        return self._clearview().context_scale
+
+    def getCVBorderScale(self) -> float:
+        return self._clearview().border_scale
+
+    def getCVColor(self):
+        return tuple(self._clearview().color)
+
+    def getCVFocusPos(self):
+        return tuple(self._clearview().focus)
+
     # --------------------------------------------------------------- camera
     def setViewPos(self, v: Sequence[float]) -> None:
         call("tvk_set_view_pos", self._h, *[float(x) for x in v])

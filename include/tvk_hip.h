@@ -294,6 +294,45 @@ int tvk_set_mip_rotation_angle(tvk_renderer* r, float degrees);
 /* setIsosurfaceColor -- RenderWindow.cpp:2329. */
 int tvk_set_isosurface_color(tvk_renderer* r, float red, float green, float blue);
 
+/* ClearView in isosurface mode: setClearViewEnabled (RenderWindow.cpp:
+ * 2348-2356), setCVIsoValue (:2155-2167), setCVSize (:2169-2180),
+ * setCVContextScale (:2182-2193), setCVBorderScale (:2195-2206),
+ * setCVColor (:2337-2346), setCVFocusPos (:2359-2370, the mouse position
+ * while dragging, :908-912); getters getClearViewEnabled, getCVIsoValue,
+ * getCVSize, getCVContextScale, getCVBorderScale, getCVColor (:326-360,
+ * 2418-2422) come back together from tvk_get_clearview.  The iso-value
+ * surface is the context; around the focus position it fades to its
+ * silhouettes and the first crossing of the ClearView iso value behind it
+ * shows, shaded in the ClearView colour:
+ *   w  = clamp(1 - size * r, 0, 1), r = pixel distance from the focus in
+ *        image heights (size 0: the whole image is in focus)
+ *   ac = (1 - w) + w * clamp(context_scale * (1 - |N.V|)^border_scale, 0, 1)
+ *   colour = ac * context + (1 - ac) * focus (no focus surface: ac *
+ *   context over the background, opacity ac)
+ * N = the context surface's unit gradient, V = towards the eye.  The
+ * reference's ClearView shaders are not in its tree; this model is this
+ * library's.  Defaults: off, iso 0, red, size 5.5, context 1, border 60,
+ * focus (0, 0) -- inside the ranges of the reference's sliders
+ * (ImageVis3D.cpp:543-553: size 0..10, context 0..9.9, border 0..99).
+ * Sizes and scales must be >= 0. */
+typedef struct tvk_clearview {
+  int32_t enabled;
+  float iso_value;     /* raw data units */
+  float color[3];
+  float size;
+  float context_scale;
+  float border_scale;
+  int32_t focus[2];    /* pixel x, y (row 0 at the top) */
+} tvk_clearview;
+int tvk_set_clearview_enabled(tvk_renderer* r, int enabled);
+int tvk_set_cv_iso_value(tvk_renderer* r, float iso);
+int tvk_set_cv_color(tvk_renderer* r, float red, float green, float blue);
+int tvk_set_cv_size(tvk_renderer* r, float size);
+int tvk_set_cv_context_scale(tvk_renderer* r, float scale);
+int tvk_set_cv_border_scale(tvk_renderer* r, float scale);
+int tvk_set_cv_focus_pos(tvk_renderer* r, int32_t x, int32_t y);
+int tvk_get_clearview(tvk_renderer* r, tvk_clearview* out);
+
 /* ---------------------------------------------------------------- camera */
 /* setViewPos/setViewDir/setUpDir -- RenderWindow.cpp:1033-1082 (world space). */
 int tvk_set_view_pos(tvk_renderer* r, float x, float y, float z);

@@ -127,6 +127,32 @@ static void shade(const oracle_render_params* p, const double g[3], const double
   for (int i = 0; i < 3; ++i) rgb[i] = rgb[i] * (p->ambient[i] + p->diffuse[i] * nl) + p->specular[i] * sp;
 }
 
+/* Isosurface hit at sample k (value v >= iso, ray parameter t): the hit
+ * refined linearly against sample k-1 (re-fetched, so the result does not
+ * depend on skipping), `color` shaded there into rgb.  Returns |N.V| of the
+ * unit gradient N = g / sqrt(|g|^2 + 1e-8) (ClearView's context term). */
+static double iso_hit(const oracle_render_params* p, const double o[3], const double d[3],
+                      const double lsc[3], double t0, double step, int64_t k, double t, double v,
+                      double iso, const double color[3], const double V[3], const double Lv[3],
+                      double rgb[3], uint8_t* flags) {
+  double th = t;
+  if (k > 0) {
+    int nf = 0;
+    double tp = t0 + ((double)(k - 1) + 0.5) * step, qp[3];
+    for (int i = 0; i < 3; ++i) qp[i] = (o[i] + tp * d[i]) * lsc[i];
+    double vp = p->interp ? sample_lin(p, qp) : sample_nn(p, qp, &nf);
+    if (fabs(vp - iso) < 1e-2) *flags |= 4;
+    if (vp < iso && v > vp) th = tp + (iso - vp) / (v - vp) * (t - tp);
+  }
+  double qh[3], g[3];
+  for (int i = 0; i < 3; ++i) qh[i] = (o[i] + th * d[i]) * lsc[i];
+  gradient(p, qh, g);
+  for (int i = 0; i < 3; ++i) rgb[i] = color[i];
+  shade(p, g, V, Lv, rgb);
+  const double s = 1.0 / sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2] + 1e-8);
+  return fabs((g[0] * V[0] + g[1] * V[1] + g[2] * V[2]) * s);
+}
+
 /* TF-empty flag per brick of the sampled level: brick empty iff every TF
  * entry (1D) or column (2D, any row) its [min, max] range can reach has
  * alpha == 0 (exact skip); for isosurfaces, iff its max is below the iso
@@ -153,7 +179,10 @@ uint8_t* oracle_brick_empty_flags(const oracle_render_params* p, int nb[3]) {
         oracle_brick_minmax(p->voxels, p->bits, p->nx, p->ny, p->nz, B, a, bx, by, bz, &mn, &mx);
         int empty = 1;
         if (p->mode == 2) {
-          empty = mx < p->iso_value;
+          /* with ClearView both surfaces are searched for */
+          const double lowest =
+              p->cv_enabled && p->cv_iso_value < p->iso_value ? p->cv_iso_value : p->iso_value;
+          empty = mx < lowest;
         } else {
           double x0 = clampd(mn * p->tf_scale, 0, ncol - 1), x1 = clampd(mx * p->tf_scale, 0, ncol - 1);
           int i0 = (int)floor(x0), i1 = (int)floor(x1) + 1;
@@ -203,6 +232,14 @@ int oracle_render_1d(const oracle_render_params* p, double* out, uint32_t* out_s
       double t0, t1;
       double vmax = 0.0;
       int any = 0;
+      /* ClearView: focus weight of this pixel, context colour and opacity */
+      int cv_stage = 0;
+      double cctx[3] = {0, 0, 0}, ac = 1.0, wf = 0.0;
+      if (p->mode == 2 && p->cv_enabled) {
+        const double r = hypot(x - p->cv_focus[0], y - p->cv_focus[1]) / p->height;
+        wf = 1.0 - p->cv_size * r;
+        wf = wf < 0 ? 0 : (wf > 1 ? 1 : wf);
+      }
       if (box_hit(o, d, lo, hi, &t0, &t1)) {
         if (t0 < 0) t0 = 0;
         double L = t1 - t0;
@@ -229,25 +266,36 @@ int oracle_render_1d(const oracle_render_params* p, double* out, uint32_t* out_s
               continue;
             }
             if (p->mode == 2) {
-              /* first sample at or above the iso value; the hit is refined
-               * linearly against the previous sample (re-fetched, so the
-               * result does not depend on skipping) */
-              if (fabs(v - p->iso_value) < iso_eps) flags |= 4;
-              if (v < p->iso_value) continue;
-              double th = t;
-              if (k > 0) {
-                double tp = t0 + ((double)(k - 1) + 0.5) * step, qp[3];
-                for (int i = 0; i < 3; ++i) qp[i] = (o[i] + tp * d[i]) * lsc[i];
-                double vp = p->interp ? sample_lin(p, qp) : sample_nn(p, qp, &nf);
-                if (fabs(vp - p->iso_value) < iso_eps) flags |= 4;
-                if (vp < p->iso_value && v > vp) th = tp + (p->iso_value - vp) / (v - vp) * (t - tp);
+              /* first sample at or above the iso value, refined against
+               * the previous sample; with ClearView and the pixel in the
+               * focus, then the first sample from there on at or above
+               * the focus iso value */
+              if (cv_stage == 0) {
+                if (fabs(v - p->iso_value) < iso_eps) flags |= 4;
+                if (v < p->iso_value) continue;
+                double rgb[3];
+                const double nv = iso_hit(p, o, d, lsc, t0, step, k, t, v, p->iso_value,
+                                          p->iso_color, V, Lv, rgb, &flags);
+                if (wf <= 0.0) {
+                  for (int i = 0; i < 3; ++i) C[i] = rgb[i];
+                  A = 1.0;
+                  break;
+                }
+                for (int i = 0; i < 3; ++i) cctx[i] = rgb[i];
+                const double x = 1.0 - nv;
+                double sil = p->cv_context_scale * pow(x, p->cv_border_scale);
+                sil = sil < 0 ? 0 : (sil > 1 ? 1 : sil);
+                ac = (1.0 - wf) + wf * sil;
+                cv_stage = 1;
               }
-              double qh[3], g[3], rgb[3] = {p->iso_color[0], p->iso_color[1], p->iso_color[2]};
-              for (int i = 0; i < 3; ++i) qh[i] = (o[i] + th * d[i]) * lsc[i];
-              gradient(p, qh, g);
-              shade(p, g, V, Lv, rgb);
-              for (int i = 0; i < 3; ++i) C[i] = rgb[i];
+              if (fabs(v - p->cv_iso_value) < iso_eps) flags |= 4;
+              if (v < p->cv_iso_value) continue;
+              double rgb[3];
+              iso_hit(p, o, d, lsc, t0, step, k, t, v, p->cv_iso_value, p->cv_color, V, Lv, rgb,
+                      &flags);
+              for (int i = 0; i < 3; ++i) C[i] = ac * cctx[i] + (1.0 - ac) * rgb[i];
               A = 1.0;
+              cv_stage = 2;
               break;
             }
             double rgba[4], g[3] = {0, 0, 0};
@@ -265,6 +313,10 @@ int oracle_render_1d(const oracle_render_params* p, double* out, uint32_t* out_s
             if (A >= p->term_alpha) break;
           }
         }
+      }
+      if (cv_stage == 1) {  /* context hit, no focus surface behind it */
+        for (int i = 0; i < 3; ++i) C[i] = ac * cctx[i];
+        A = ac;
       }
       if (p->mode == 3 && any) {
         /* the maximum through the 1D TF, over the background */

@@ -69,6 +69,23 @@ typedef struct oracle_render_params {
   /* isosurface */
   double iso_value;        /* raw data units */
   double iso_color[3];
+  /* ClearView (mode 2 with cv_enabled != 0).  The iso_value surface is the
+   * context; inside a screen-space focus around cv_focus it turns
+   * transparent except at its silhouettes and reveals the focus surface,
+   * the first cv_iso_value crossing at or after the context hit:
+   *   w  = clamp(1 - cv_size * r, 0, 1), r = |pixel - cv_focus| / height
+   *   ac = (1 - w) + w * clamp(cv_context_scale * (1 - |N.V|)^cv_border_scale, 0, 1)
+   *   C  = ac * context + (1 - ac) * focus (A = 1); no focus hit: C = ac *
+   *   context, A = ac
+   * with N the context's unit gradient and V = -ray.  This is this
+   * library's model (the reference's ClearView shaders are absent). */
+  int32_t cv_enabled;
+  double cv_iso_value;     /* raw data units */
+  double cv_color[3];
+  double cv_focus[2];      /* pixel (x, y), row 0 at the top */
+  double cv_size;
+  double cv_context_scale;
+  double cv_border_scale;
 } oracle_render_params;
 
 /* FP64 reference raycast.  out_rgba: W*H*4 doubles (composited over the

@@ -51,6 +51,9 @@ class RenderParams(C.Structure):
         ("ambient", C.c_double * 3), ("diffuse", C.c_double * 3), ("specular", C.c_double * 3),
         ("shininess", C.c_double), ("light_dir", C.c_double * 3),
         ("iso_value", C.c_double), ("iso_color", C.c_double * 3),
+        ("cv_enabled", C.c_int32), ("cv_iso_value", C.c_double), ("cv_color", C.c_double * 3),
+        ("cv_focus", C.c_double * 2), ("cv_size", C.c_double), ("cv_context_scale", C.c_double),
+        ("cv_border_scale", C.c_double),
     ]
 
 
@@ -108,7 +111,7 @@ def make_params(vol: np.ndarray, tf: np.ndarray, *, dims0=None, lod=0, rescale=(
                 brick_size=0, apron=2, mode=0, tf2d=None, max_grad=1.0, lighting=False,
                 ambient=(0.1, 0.1, 0.1), diffuse=(0.9, 0.9, 0.9), specular=(0.3, 0.3, 0.3),
                 shininess=16.0, light_dir=(0.0, 0.0, 1.0), iso_value=0.0,
-                iso_color=(1.0, 1.0, 1.0)):
+                iso_color=(1.0, 1.0, 1.0), clearview=None):
     """Parameters of one render; ``vol`` is (nz, ny, nx) x-fastest.  For
     mode 1 ``tf2d`` is (H, W, 4) float32 (value along W) and ``tf_scale``
     maps values to its columns."""
@@ -157,6 +160,16 @@ def make_params(vol: np.ndarray, tf: np.ndarray, *, dims0=None, lod=0, rescale=(
     p.light_dir[:] = light_dir
     p.iso_value = iso_value
     p.iso_color[:] = iso_color
+    # ClearView (mode 2): dict(iso_value, color, focus=(x, y), size,
+    # context_scale, border_scale)
+    if clearview is not None:
+        p.cv_enabled = 1
+        p.cv_iso_value = clearview["iso_value"]
+        p.cv_color[:] = clearview.get("color", (1.0, 0.0, 0.0))
+        p.cv_focus[:] = clearview.get("focus", (0.0, 0.0))
+        p.cv_size = clearview.get("size", 5.5)
+        p.cv_context_scale = clearview.get("context_scale", 1.0)
+        p.cv_border_scale = clearview.get("border_scale", 60.0)
     # keep the buffers alive as long as the params
     p._keep = (vol, tf, t2)
     return p

@@ -23,10 +23,9 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 
 def build(force: bool = False) -> str:
-    """Compile liboracle.so with oracle/Makefile."""
-    if force or not os.path.exists(LIB_PATH):
-        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", HERE],
-                       check=True)
+    """Compile liboracle.so with oracle/Makefile (make rebuilds it when a
+    source is newer than the library, so an edited oracle is never stale)."""
+    subprocess.run(["make", "-s", "-B" if force else "-s", "-C", HERE], check=True)
     return LIB_PATH
 
 

@@ -454,7 +454,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
-              "(every GPU box here has one GPU; the sharding design is in DESIGN.md)",
+              "(the renderer runs on one GPU; sharding across GPUs is not built)",
               file=sys.stderr)
         return 2
 

@@ -462,7 +462,8 @@ def rmse255(a: np.ndarray, b: np.ndarray) -> float:
 
 def full_resolution(renderer, dims, edge) -> np.ndarray:
     """The full-resolution frame of the per-region path (every region at
-    level 0): the judge of the learned frames' error, sampled like them."""
+    level 0): the reference the learned frames' error is measured against,
+    sampled like them."""
     n_reg = int(np.prod(np.ceil(np.asarray(dims) / edge)))
     renderer.setRegionLODs(np.zeros(n_reg, np.uint8))
     img = renderer.capture()

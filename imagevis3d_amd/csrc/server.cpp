@@ -137,6 +137,11 @@ int RenderServer::Submit(const tvk_request& q, uint64_t* ticket) {
   LaunchInfo li;
   if (int e = r->ServeJob(q.view, q.render_mode, nullptr, &probe, &li)) return e;
   req.key = variant_key(li);
+  {
+    const float longest = std::max(probe.box[0], std::max(probe.box[1], probe.box[2]));
+    req.cost = (uint64_t)probe.rect_w * (uint64_t)probe.rect_h * 64ull *
+               (uint64_t)std::max(1.0f, longest / probe.step);
+  }
   req.bytes = (uint64_t)q.view.width * q.view.height * 4;
   if (req.bytes > dev_arena_.size()) return fail(TVK_ERR_OOM, "image larger than the output arena");
   req.t_submit = now_ms();
@@ -182,9 +187,14 @@ bool RenderServer::retire_locked() {
 bool RenderServer::launch_locked(Lane& L) {
   const uint64_t key = reqs_[pending_.front()].key;
   std::vector<uint64_t> pick;
+  uint64_t cost = 0;
   for (auto it = pending_.begin(); it != pending_.end() && (int)pick.size() < kMaxJobs;) {
     Request& r = reqs_[*it];
-    if (r.key != key) {
+    // same kernel variant; after the first request, no more than
+    // kLaunchCostCap estimated samples per launch, so that light requests
+    // are not held until a heavy one in the same launch finishes (a skipped
+    // request keeps its place and leads a later launch)
+    if (r.key != key || (!pick.empty() && cost + r.cost > kLaunchCostCap)) {
       ++it;
       continue;
     }
@@ -198,6 +208,7 @@ bool RenderServer::launch_locked(Lane& L) {
     r.host_off = hoff;
     r.host_held = true;
     pick.push_back(*it);
+    cost += r.cost;
     it = pending_.erase(it);
   }
   if (pick.empty()) return false;

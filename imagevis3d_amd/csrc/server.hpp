@@ -4,7 +4,9 @@
 // A worker thread owns `lanes` HIP streams, each with its own job table.
 // Requests wait in arrival order; whenever a lane is free the oldest
 // request's kernel variant picks the launch, and every waiting request of
-// that variant whose output fits the arenas joins it (up to kMaxJobs).  A
+// that variant whose output fits the arenas joins it (up to kMaxJobs, and
+// up to kLaunchCostCap estimated samples after the first request, so that a
+// heavy request does not hold light ones until it finishes).  A
 // launch renders into a device arena block per request and copies each
 // image to a pinned host arena block on the lane's stream; the lane is
 // retired when its event completes, which frees the device blocks and
@@ -26,6 +28,10 @@
 #include "renderer.hpp"
 
 namespace tvk {
+
+// Estimated samples (tile-rect rays x the volume's longest axis in steps)
+// one launch may carry beyond its first request: about 5 ms of raycasting.
+constexpr uint64_t kLaunchCostCap = 1000000000ull;
 
 // First-fit allocator over [0, size) in 256-byte units, with coalescing.
 class Arena {
@@ -55,6 +61,7 @@ class RenderServer {
     uint64_t key = 0;        // kernel variant
     double t_submit = 0, t_launch = 0, t_done = 0;
     uint64_t dev_off = 0, host_off = 0, bytes = 0;
+    uint64_t cost = 0;       // samples if every ray of its tile rect ran the longest chord
     uint64_t evaluated = 0, nominal = 0;
     uint32_t launch_jobs = 0;
     int status = TVK_OK;

@@ -570,8 +570,10 @@ int tvk_autotune(tvk_renderer* r, const tvk_view* views, uint32_t n, uint32_t fl
  * memory (device arena + pinned host arena of `arena_bytes` each) are free,
  * and overlaps launches on up to `lanes` HIP streams.  Requests that share a
  * kernel variant (bytes per voxel, interpolant, render mode, lighting) share
- * a launch.  A request's image is the same whether it runs alone or in a
- * mix.  Registered renderers must hold in-core volumes and must not be
+ * a launch, oldest first, up to 64 requests and -- after the first -- about
+ * 1e9 estimated samples (tile-rect rays x the volume's longest axis in
+ * steps), so light requests are not held until a heavy one finishes.  A
+ * request's image is the same whether it runs alone or in a mix.  Registered renderers must hold in-core volumes and must not be
  * changed or used by other threads while the server runs. */
 typedef struct tvk_server tvk_server;
 typedef struct tvk_request {

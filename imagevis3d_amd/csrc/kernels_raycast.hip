@@ -44,6 +44,13 @@ namespace {
 constexpr float kInf = __builtin_huge_valf();
 constexpr int kWaves = 8;                  // waves per work-group
 constexpr int kThreads = 64 * kWaves;
+// Work-group size of a raycast variant: kThreads when the 1D TF is staged
+// in LDS (64 KB per work-group, so two fit a CU's 160 KB), else 4 waves:
+// the variants without an LDS TF take 135-163 VGPRs (3 waves per SIMD),
+// where 8-wave work-groups fit only once per CU (8 waves) and 4-wave ones
+// three times (12 waves).
+template <bool TF_LDS>
+constexpr int raycast_threads() { return TF_LDS ? kThreads : 256; }
 constexpr int kLdsTfEntries = 4096;        // 64 KB of float4
 
 __device__ inline float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
@@ -848,16 +855,18 @@ __global__ __launch_bounds__(256) void background_kernel(const JobTable* __restr
 // taken from the per-XCD queues until all queues are drained.  Tile t of
 // the launch belongs to job j with tile_start[j] <= t < tile_start[j + 1].
 template <typename T, bool LINEAR, int BATCH, bool TF_LDS, int MODE>
-__global__ __launch_bounds__(kThreads, 2) void raycast_kernel(const JobTable* __restrict__ tab) {
+__global__ __launch_bounds__(raycast_threads<TF_LDS>(), 2) void raycast_kernel(
+    const JobTable* __restrict__ tab) {
+  constexpr int kT = raycast_threads<TF_LDS>();
   extern __shared__ float4 s_tf[];  // n_tf_lds entries when TF_LDS
   __shared__ unsigned long long s_count[2 * kMaxJobs];  // per job: evaluated, nominal
   const JobTableHeader& H = tab->h;
   const float4* tf_shared = (const float4*)H.tf_lds;
   if (TF_LDS) {
-    for (int i = (int)threadIdx.x; i < H.n_tf_lds; i += kThreads) s_tf[i] = tf_shared[i];
+    for (int i = (int)threadIdx.x; i < H.n_tf_lds; i += kT) s_tf[i] = tf_shared[i];
     tf_shared = s_tf;
   }
-  for (int i = (int)threadIdx.x; i < 2 * kMaxJobs; i += kThreads) s_count[i] = 0;
+  for (int i = (int)threadIdx.x; i < 2 * kMaxJobs; i += kT) s_count[i] = 0;
   __syncthreads();
   const int lane = (int)(threadIdx.x & 63);
   const int xcd = (int)(blockIdx.x & 7);
@@ -930,7 +939,7 @@ __global__ __launch_bounds__(kThreads, 2) void raycast_kernel(const JobTable* __
     atomicAdd(&s_count[2 * cur_job + 1], (unsigned long long)n_nom);
   }
   __syncthreads();
-  for (int i = (int)threadIdx.x; i < 2 * n_jobs; i += kThreads)
+  for (int i = (int)threadIdx.x; i < 2 * n_jobs; i += kT)
     if (s_count[i] != 0) atomicAdd(&H.counters[kQueues + i], s_count[i]);
 }
 
@@ -939,12 +948,13 @@ hipError_t launch_one(const LaunchInfo& li, const JobTable* tab, hipStream_t str
   auto kern = raycast_kernel<T, LINEAR, BATCH, TF_LDS, MODE>;
   const size_t lds = TF_LDS ? (size_t)li.n_tf_lds * sizeof(float4) : 0;
   int per_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads, lds);
+  constexpr int kT = raycast_threads<TF_LDS>();
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kT, lds);
   if (e != hipSuccess) return e;
   if (per_cu < 1) per_cu = 1;
   int grid = li.compute_units * per_cu;
   grid = ((grid + 7) / 8) * 8;
-  kern<<<grid, kThreads, lds, stream>>>(tab);
+  kern<<<grid, kT, lds, stream>>>(tab);
   return hipGetLastError();
 }
 

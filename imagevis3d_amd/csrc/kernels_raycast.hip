@@ -51,12 +51,6 @@ constexpr int kThreads = 64 * kWaves;
 // three times (12 waves).
 template <bool TF_LDS>
 constexpr int raycast_threads() { return TF_LDS ? kThreads : 256; }
-// Minimum waves per SIMD asked of the compiler: 4 (<= 128 VGPRs, a few
-// dozen bytes of spills) for the gradient variants, which otherwise land
-// a few registers above 128 (2D-TF launches measured 2-4% faster; the
-// isosurface variant measured 5% slower that way, so it keeps 2).
-template <bool TF_LDS, int MODE>
-constexpr int raycast_min_waves();
 constexpr int kLdsTfEntries = 4096;        // 64 KB of float4
 
 __device__ inline float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
@@ -228,10 +222,6 @@ __device__ inline void fetch_value_grad(const T* __restrict__ base, const SlotGe
 // the ray through the 1D TF; every sample is visited (a TF-transparent cell
 // may hold the maximum) and rays do not terminate early.
 constexpr int kMode1D = 0, kModeGrad = 1, kModeIso = 2, kModeRegions = 3, kModeMIP = 4;
-template <bool TF_LDS, int MODE>
-constexpr int raycast_min_waves() {
-  return MODE == kModeGrad ? 4 : 2;
-}
 
 // Sample at level texel coordinate q through the page table (clamp to
 // edge): used off the march's current brick (isosurface refinement and
@@ -865,7 +855,7 @@ __global__ __launch_bounds__(256) void background_kernel(const JobTable* __restr
 // taken from the per-XCD queues until all queues are drained.  Tile t of
 // the launch belongs to job j with tile_start[j] <= t < tile_start[j + 1].
 template <typename T, bool LINEAR, int BATCH, bool TF_LDS, int MODE>
-__global__ __launch_bounds__(raycast_threads<TF_LDS>(), (raycast_min_waves<TF_LDS, MODE>())) void raycast_kernel(
+__global__ __launch_bounds__(raycast_threads<TF_LDS>(), 2) void raycast_kernel(
     const JobTable* __restrict__ tab) {
   constexpr int kT = raycast_threads<TF_LDS>();
   extern __shared__ float4 s_tf[];  // n_tf_lds entries when TF_LDS

@@ -247,7 +247,7 @@ def run_batched(batches: int = 3, seed: int = 3) -> dict:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS},
         "tuned": {"samples_per_batch": t.samples_per_batch, "tile_width": t.tile_width,
-                  "tile_group": t.tile_group},
+                  "tile_group": t.tile_group, "work_group_waves": t.wg_waves},
         "pool": {"slots": int(ms.pool_slots), "resident": int(ms.resident_bricks),
                  "uploaded": int(ms.bricks_uploaded)},
         "setup_s": {"generate": round(gen_s, 2), "host_pyramid": round(plan_s, 2),
@@ -498,6 +498,7 @@ def main() -> int:
         t = r.autotune(tv, flags=r.TUNE_CELL | r.TUNE_LAUNCH)
         out = {"brick": t.brick_size, "cell": t.cell, "samples_per_batch": t.samples_per_batch,
                "tile_width": t.tile_width, "tile_group": t.tile_group,
+               "work_group_waves": t.wg_waves,
                "tuned_ms": round(t.kernel_ms, 4), "start_ms": round(t.baseline_ms, 4),
                "settings_timed": t.candidates, "seconds": round(time.perf_counter() - t0, 2)}
         print(f"[bench] autotuned for {what}: {out}", file=sys.stderr, flush=True)

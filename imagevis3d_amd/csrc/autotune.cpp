@@ -12,7 +12,7 @@ namespace tvk {
 namespace {
 
 struct Setting {
-  uint32_t brick, cell, batch, tile_width, tile_group;
+  uint32_t brick, cell, batch, tile_width, tile_group, wg_waves;
 };
 
 }  // namespace
@@ -42,12 +42,13 @@ int HipRenderer::Autotune(const tvk_view* views, uint32_t n, uint32_t flags, tvk
     return TVK_OK;
   };
   Setting cur{(uint32_t)brick_, (uint32_t)cell_, (uint32_t)batch_, 1u << tile_shift_,
-              (uint32_t)tile_group_};
+              (uint32_t)tile_group_, (uint32_t)wg_waves_};
   auto apply = [&](const Setting& s, bool rebrick) -> int {
     if (rebrick) {
       if (int e = Rebrick(s.brick, s.cell)) return e;
     }
     if (int e = SetKernelTuning(s.batch, s.tile_group, (uint32_t)xcd_split_)) return e;
+    if (int e = SetWorkGroupWaves(s.wg_waves)) return e;
     return SetRayTileShape(s.tile_width);
   };
   float best = 0.f;
@@ -98,6 +99,7 @@ int HipRenderer::Autotune(const tvk_view* views, uint32_t n, uint32_t flags, tvk
     if (int e = try_values({2u, 4u, 8u}, &Setting::batch, false)) return e;
     if (int e = try_values({4u, 8u, 16u}, &Setting::tile_width, false)) return e;
     if (int e = try_values({1u, 2u, 4u, 8u, 16u}, &Setting::tile_group, false)) return e;
+    if (int e = try_values({4u, 8u}, &Setting::wg_waves, false)) return e;
   }
   if (out) {
     *out = tvk_tuning{};
@@ -107,6 +109,7 @@ int HipRenderer::Autotune(const tvk_view* views, uint32_t n, uint32_t flags, tvk
     out->tile_width = 1u << tile_shift_;
     out->tile_group = (uint32_t)tile_group_;
     out->xcd_split = (uint32_t)xcd_split_;
+    out->wg_waves = (uint32_t)wg_waves_;
     out->kernel_ms = best;
     out->baseline_ms = baseline;
     out->candidates = (uint32_t)tried;

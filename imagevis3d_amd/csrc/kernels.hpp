@@ -110,6 +110,7 @@ struct LaunchInfo {
   int n_tf_lds = 0;     // > 0: the table's shared 1D TF goes to LDS
   int regions = 0;      // 1D TF with per-region LOD (jobs carry region codes)
   int compute_units = 0;
+  int wg_waves = 4;      // waves per work-group of variants without an LDS TF (4 or 8)
 };
 
 // One raycast launch over a device job table `tab` (its header counters

@@ -44,13 +44,6 @@ namespace {
 constexpr float kInf = __builtin_huge_valf();
 constexpr int kWaves = 8;                  // waves per work-group
 constexpr int kThreads = 64 * kWaves;
-// Work-group size of a raycast variant: kThreads when the 1D TF is staged
-// in LDS (64 KB per work-group, so two fit a CU's 160 KB), else 4 waves:
-// the variants without an LDS TF take 135-163 VGPRs (3 waves per SIMD),
-// where 8-wave work-groups fit only once per CU (8 waves) and 4-wave ones
-// three times (12 waves).
-template <bool TF_LDS>
-constexpr int raycast_threads() { return TF_LDS ? kThreads : 256; }
 constexpr int kLdsTfEntries = 4096;        // 64 KB of float4
 
 __device__ inline float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
@@ -855,9 +848,8 @@ __global__ __launch_bounds__(256) void background_kernel(const JobTable* __restr
 // taken from the per-XCD queues until all queues are drained.  Tile t of
 // the launch belongs to job j with tile_start[j] <= t < tile_start[j + 1].
 template <typename T, bool LINEAR, int BATCH, bool TF_LDS, int MODE>
-__global__ __launch_bounds__(raycast_threads<TF_LDS>(), 2) void raycast_kernel(
-    const JobTable* __restrict__ tab) {
-  constexpr int kT = raycast_threads<TF_LDS>();
+__global__ __launch_bounds__(kThreads, 2) void raycast_kernel(const JobTable* __restrict__ tab) {
+  const int kT = (int)blockDim.x;  // 256 or 512 threads (see launch_one)
   extern __shared__ float4 s_tf[];  // n_tf_lds entries when TF_LDS
   __shared__ unsigned long long s_count[2 * kMaxJobs];  // per job: evaluated, nominal
   const JobTableHeader& H = tab->h;
@@ -948,7 +940,13 @@ hipError_t launch_one(const LaunchInfo& li, const JobTable* tab, hipStream_t str
   auto kern = raycast_kernel<T, LINEAR, BATCH, TF_LDS, MODE>;
   const size_t lds = TF_LDS ? (size_t)li.n_tf_lds * sizeof(float4) : 0;
   int per_cu = 0;
-  constexpr int kT = raycast_threads<TF_LDS>();
+  // Work-group size: 8 waves when the 1D TF is staged in LDS (64 KB per
+  // work-group, so two fit a CU's 160 KB); otherwise li.wg_waves (4 or 8).
+  // The variants without an LDS TF take 135-173 VGPRs (2-3 waves per SIMD):
+  // 8-wave work-groups fit a CU once (8 waves), 4-wave ones up to three
+  // times (12 waves), which is faster on cache-friendly volumes and slower
+  // where more waves in flight thrash the caches (the autotuner picks).
+  const int kT = TF_LDS ? kThreads : 64 * (li.wg_waves == 4 ? 4 : 8);
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kT, lds);
   if (e != hipSuccess) return e;
   if (per_cu < 1) per_cu = 1;

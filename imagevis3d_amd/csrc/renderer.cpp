@@ -1098,6 +1098,12 @@ int HipRenderer::SetKernelTuning(uint32_t batch, uint32_t tile_group, uint32_t x
   return TVK_OK;
 }
 
+int HipRenderer::SetWorkGroupWaves(uint32_t waves) {
+  if (waves != 4 && waves != 8) return arg_fail("work-group waves must be 4 or 8");
+  wg_waves_ = (int)waves;
+  return TVK_OK;
+}
+
 int HipRenderer::SetRayTileShape(uint32_t tile_width) {
   if (tile_width != 4 && tile_width != 8 && tile_width != 16) return arg_fail("tile width must be 4, 8 or 16");
   tile_shift_ = tile_width == 4 ? 2 : (tile_width == 8 ? 3 : 4);
@@ -1314,6 +1320,7 @@ LaunchInfo HipRenderer::launch_info(int n_tf_lds) const {
   li.n_tf_lds = ((li.mode == 0 || li.mode == 3) && n_tf_lds <= 4096) ? n_tf_lds : 0;
   li.regions = (regions_active() && li.mode == 0 && !li.lighting) ? 1 : 0;
   li.compute_units = compute_units_;
+  li.wg_waves = wg_waves_;
   return li;
 }
 
@@ -1359,7 +1366,8 @@ int launch_job_table(LaunchSlot& sl, const RaycastParams* jobs, int n, const Lau
   // variant or its LDS size changes.
   const uint64_t key = 1ull | (uint64_t)li.bpv << 1 | (uint64_t)li.interp << 3 | (uint64_t)li.mode << 4 |
                        (uint64_t)li.lighting << 6 | (uint64_t)li.batch << 7 | (uint64_t)li.regions << 11 |
-                       (uint64_t)li.n_tf_lds << 12;
+                       (uint64_t)li.n_tf_lds << 12 |
+                       (uint64_t)(li.wg_waves == 4 ? 1 : 0) << 40;
   if (!sl.exec || sl.graph_key != key) {
     if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
     if (sl.graph) (void)hipGraphDestroy(sl.graph);

@@ -630,6 +630,11 @@ int tvk_set_kernel_tuning(tvk_renderer* r, uint32_t batch, uint32_t tile_group,
   return r->impl.SetKernelTuning(batch, tile_group, xcd_split);
 }
 
+int tvk_set_work_group_waves(tvk_renderer* r, uint32_t waves) {
+  TVK_GUARD(r);
+  return r->impl.SetWorkGroupWaves(waves);
+}
+
 int tvk_set_ray_tile_shape(tvk_renderer* r, uint32_t tile_width) {
   TVK_GUARD(r);
   return r->impl.SetRayTileShape(tile_width);

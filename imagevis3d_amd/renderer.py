@@ -581,6 +581,11 @@ class Renderer:
         call("tvk_set_kernel_tuning", self._h, int(samples_per_batch), int(tile_group),
              int(xcd_split))
 
+    def setWorkGroupWaves(self, waves: int) -> None:
+        """Waves per work-group (4 or 8) of the raycast variants without an
+        LDS TF (no image change)."""
+        call("tvk_set_work_group_waves", self._h, int(waves))
+
     # --------------------------------------------------------------- data
     def getDataset(self) -> Dataset:
         return Dataset(self, self._path)

@@ -385,6 +385,14 @@ int tvk_set_empty_space_skipping(tvk_renderer* r, int enabled);
  * any of them. */
 int tvk_set_kernel_tuning(tvk_renderer* r, uint32_t samples_per_batch, uint32_t tile_group,
                           uint32_t xcd_split);
+/* Waves per work-group (4 or 8; default 4) of the raycast variants that do
+ * not stage the 1D TF in LDS (2D TF, lighting without LDS TF, isosurface,
+ * MIP, mixed-TF launches); no counterpart in the reference, chosen by the
+ * autotuner (TVK_TUNE_LAUNCH).  Those variants fit 2-3 waves per SIMD, so
+ * 4-wave work-groups put up to 12 waves on a CU and 8-wave ones 8: more
+ * waves hide more latency on cache-friendly volumes and thrash the caches
+ * on very large ones.  The image does not depend on it. */
+int tvk_set_work_group_waves(tvk_renderer* r, uint32_t waves);
 /* Ray tile shape of one 64-lane wavefront (no counterpart in the
  * reference; chosen by the autotuner): tile_width x (64 / tile_width) rays,
  * tile_width in {4, 8, 16} (default 8).  The image does not depend on it. */
@@ -557,7 +565,7 @@ typedef struct tvk_tuning {
   float kernel_ms;          /* best summed launch time over the views   */
   float baseline_ms;        /* the same views under the starting settings */
   uint32_t candidates;      /* settings timed                            */
-  uint32_t reserved;
+  uint32_t wg_waves;        /* tvk_set_work_group_waves                  */
 } tvk_tuning;
 int tvk_autotune(tvk_renderer* r, const tvk_view* views, uint32_t n, uint32_t flags,
                  tvk_tuning* out);

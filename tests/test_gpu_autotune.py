@@ -33,6 +33,7 @@ def test_cell_and_launch_tuning_keep_images():
           f"{t.candidates} settings)")
     assert t.brick_size == 32 and 32 % t.cell == 0
     assert t.samples_per_batch in (2, 4, 8) and t.tile_width in (4, 8, 16)
+    assert t.wg_waves in (4, 8)
     assert t.kernel_ms <= t.baseline_ms and t.candidates >= 8
     got, st, _ = r.renderBatch(views)
     for i in range(len(views)):

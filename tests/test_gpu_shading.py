@@ -63,7 +63,11 @@ def test_2d_tf_parity(lighting):
                               tf_scale=scale, lighting=lighting, **LIGHT)
     assert_parity(img, ref8, fl, f"2D TF lighting={lighting}")
     assert abs(int(st.evaluated_samples) - int(ns.sum())) <= 2e-3 * ns.sum() + 64
-    # skipping on/off gives the same image
+    # 8-wave work-groups (the default is 4 for this variant) and skipping
+    # on/off give the same image
+    r.setWorkGroupWaves(8)
+    assert r.paint()
+    assert np.array_equal(r.capture(), img)
     r.setEmptySpaceSkipping(False)
     assert r.paint()
     assert np.array_equal(r.capture(), img)

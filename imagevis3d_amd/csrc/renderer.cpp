@@ -1515,7 +1515,13 @@ int HipRenderer::Paint() {
       }
     } else {
       if (int e = service_requests(lod, &pass_complete, &overflow)) return e;
-      if (overflow && lod < hi) lod_floor_ = lod + 1;  // the view's bricks do not fit: go coarser
+      if (overflow) {
+        if (lod >= hi) {
+          set_last_error("the pool cannot hold this view's bricks at the coarsest allowed LOD");
+          return TVK_ERR_OOM;
+        }
+        lod_floor_ = lod + 1;  // the view's bricks do not fit: go coarser
+      }
     }
   }
 
@@ -1619,12 +1625,20 @@ int HipRenderer::RenderBatch(const tvk_view* views, uint32_t n, tvk_view_stats* 
         if (int e = regions_active() ? service_requests(0, n_lod_ - 1, &complete, &overflow)
                                      : service_requests(lod, &complete, &overflow))
           return e;
-        if (overflow && lod < hi) {
+        if (overflow) {
+          if (regions_active() || lod >= hi) {
+            set_last_error("the pool cannot hold the batch's bricks at the coarsest allowed LOD");
+            return TVK_ERR_OOM;
+          }
           lod_floor_ = lod + 1;
           lod = lod + 1;
         }
       }
       if (complete) break;
+      if (pass + 1 == 4 * kMaxLod + 8) {
+        set_last_error("paging did not converge within the pass limit");
+        return TVK_ERR_OOM;
+      }
     }
     for (int k = 0; k < cnt; ++k) {
       const uint32_t i = first + k;
@@ -1726,6 +1740,10 @@ int HipRenderer::Capture(uint8_t* rgba, uint64_t bytes, bool preserve) {
   int rc = TVK_OK;
   // a paged pool may need a paging pass per LOD on top of the subframes
   for (int guard = 0; guard < 4 * kMaxLod + 8 && !frame_complete_ && rc == TVK_OK; ++guard) rc = Paint();
+  if (rc == TVK_OK && !frame_complete_) {
+    set_last_error("the frame did not complete within the paging pass limit");
+    rc = TVK_ERR_OOM;
+  }
   if (rc == TVK_OK) {
     hipError_t e = hipMemcpyAsync(rgba, framebuffer_.p, (size_t)width_ * height_ * 4,
                                   hipMemcpyDeviceToHost, stream_);

@@ -148,6 +148,14 @@ class HipRenderer {
                      uint64_t pool_bytes);
   int SetPagedFlatVolume(const void* voxels, const uint64_t dims[3], uint32_t bits, uint32_t brick,
                          uint32_t apron, uint32_t n_lod, uint64_t pool_bytes);
+  // bricks of every level of the pyramid SetPagedVolume would build
+  int PlannedBrickCount(const uint64_t dims[3], uint32_t bits, uint32_t brick, uint32_t apron,
+                        uint32_t n_lod, uint64_t* count) const {
+    LevelPlan plan;
+    if (int e = plan_levels(dims, bits, brick, apron, n_lod, &plan)) return e;
+    *count = (uint64_t)plan.total_bricks;
+    return 0;
+  }
   int LodLevelCount() const { return n_lod_; }
   int BitWidth(uint32_t* bits) const;
   int DomainSize(uint32_t lod, uint64_t dims[3]) const;

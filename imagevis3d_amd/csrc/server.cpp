@@ -231,8 +231,10 @@ bool RenderServer::launch_locked(Lane& L) {
     slot.reset(new LaunchSlot());
     ok = slot->alloc() == TVK_OK;
   }
+  bool enqueued = false;  // device work of this launch may be in flight
   if (ok) {
     L.cur = slot.get();
+    enqueued = true;
     ok = launch_job_table(*slot, jobs.data(), (int)jobs.size(), li, 4, 0, L.stream, true) == TVK_OK;
   }
   for (size_t k = 0; k < pick.size() && ok; ++k) {
@@ -243,6 +245,9 @@ bool RenderServer::launch_locked(Lane& L) {
   if (ok) ok = hipEventRecord(L.done, L.stream) == hipSuccess;
   const double t = now_ms();
   if (!ok) {
+    // the kernel or a copy may still write the picked arena blocks: let the
+    // lane's stream drain before they go back to the arena
+    if (enqueued) (void)hipStreamSynchronize(L.stream);
     for (uint64_t tk : pick) {
       Request& r = reqs_[tk];
       r.status = TVK_ERR_HIP;

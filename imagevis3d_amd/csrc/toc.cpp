@@ -647,16 +647,24 @@ int TocVolume::Open(const std::string& path) {
   layout = r.u32();
   r.u32();
   if (!r.ok || (bits != 8 && bits != 16) || comps != 1 || n_lod < 1 || n_lod > 24 || max_brick < 2 ||
-      max_brick > 1024 || payload != max_brick - 2 * overlap || compression >= kCodecs)
+      max_brick > 1024 || 2 * (uint64_t)overlap >= max_brick || payload != max_brick - 2 * overlap ||
+      compression >= kCodecs)
     return fail(TVK_ERR_FORMAT, "bad TOC header");
+  for (int i = 0; i < 3; ++i)
+    if (dims[i] == 0 || dims[i] > (1ull << 32)) return fail(TVK_ERR_FORMAT, "bad TOC volume size");
   lods.resize(n_lod);
   uint64_t total = 0;
-  for (auto& L : lods) {
+  for (size_t l = 0; l < lods.size(); ++l) {
+    TocLod& L = lods[l];
     for (int i = 0; i < 3; ++i) L.dims[i] = r.u64();
     for (int i = 0; i < 3; ++i) L.grid[i] = r.u64();
     L.first = r.u64();
-    for (int i = 0; i < 3; ++i)
-      if (L.dims[i] == 0 || L.grid[i] != (L.dims[i] + payload - 1) / payload) return fail(TVK_ERR_FORMAT, "bad TOC level");
+    // level 0 is the volume; each further level halves it, rounding up
+    for (int i = 0; i < 3; ++i) {
+      const uint64_t want = l == 0 ? dims[i] : (lods[l - 1].dims[i] + 1) / 2;
+      if (L.dims[i] != want || L.grid[i] != (L.dims[i] + payload - 1) / payload)
+        return fail(TVK_ERR_FORMAT, "bad TOC level");
+    }
     if (L.first != total) return fail(TVK_ERR_FORMAT, "bad TOC level table");
     total += L.grid[0] * L.grid[1] * L.grid[2];
   }

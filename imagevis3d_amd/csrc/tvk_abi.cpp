@@ -414,6 +414,12 @@ int tvk_set_toc_volume(tvk_renderer* r, tvk_toc* t, uint64_t pool_bytes) {
     return TVK_ERR_ARG;
   }
   const tvk::TocVolume& T = t->impl;
+  uint64_t planned = 0;
+  if (int e = r->impl.PlannedBrickCount(T.dims, T.bits, T.payload, T.overlap, T.n_lod, &planned)) return e;
+  if (planned != T.entries.size()) {
+    tvk::set_last_error("TOC brick table does not match its level layout");
+    return TVK_ERR_FORMAT;
+  }
   std::vector<float> ranges(T.entries.size() * 2);
   for (size_t i = 0; i < T.entries.size(); ++i) {
     ranges[2 * i] = T.entries[i].mn;

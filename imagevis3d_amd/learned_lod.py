@@ -455,8 +455,10 @@ def select_levels(pred_log_mse: np.ndarray, area: np.ndarray, budget_sse: float,
 
 # ------------------------------------------------------------ rendering
 def rmse255(a: np.ndarray, b: np.ndarray) -> float:
-    """RMSE over RGBA in units of 1/255 (as the parity tests measure it)."""
-    d = a.astype(np.float64) - b.astype(np.float64)
+    """RMSE over the colour channels (RGB) in units of 1/255.  Alpha is left
+    out: over an opaque background both frames hold 255 there, and counting
+    it would dilute the colour error by sqrt(3/4)."""
+    d = a[..., :3].astype(np.float64) - b[..., :3].astype(np.float64)
     return float(np.sqrt((d * d).mean()))
 
 

@@ -120,3 +120,10 @@ def oracle_kwargs(w: Workload, cam: Camera, lod: int = 0, level_dims=None) -> di
                 rate=w.rate, term_alpha=w.term_alpha, interp=1, bg_top=w.bg_top,
                 bg_bottom=w.bg_bottom, bg_alpha=1.0,
                 brick_size=skip_cell(w.brick) if w.skip else 0, apron=1)
+
+
+def view_of(cam: Camera, width: int, height: int, tf_slot: int = 0) -> dict:
+    """One entry of a renderBatch / serving view list for a camera."""
+    return dict(view_pos=cam.view_pos, view_dir=cam.view_dir, up_dir=cam.up_dir, fov=cam.fov,
+                ortho=cam.ortho, ortho_half_height=cam.ortho_half_height, width=width,
+                height=height, tf_slot=tf_slot)

@@ -44,30 +44,30 @@ def pixel_diff(a: np.ndarray, b: np.ndarray):
 
 
 # Parity bar for RGBA8 images against the FP64 oracle: per-pixel max |d| <=
-# 2/255 on pixels whose early-termination decision is not within 1e-5 of
-# the threshold (flagged by the oracle; there the two precisions may stop
-# one sample apart, a difference bounded by 0.01 = 2.55/255 before
-# rounding, so 4/255 is allowed), and RMSE <= 0.5/255 over all pixels.
+# 2/255 on every pixel and RMSE <= 0.5/255 over all pixels.  The oracle
+# flags pixels whose outcome hangs on a comparison within rounding of its
+# threshold (early termination within 1e-5 of alpha 0.99; an isosurface or
+# ClearView sample within 1e-2 of its iso value; nearest sampling within
+# 1e-4 of a voxel face).  Only isosurface-type renders may differ beyond the
+# bar there -- a ray that grazes the surface can hit in one precision and
+# miss in the other -- and only on at most `max_flip_frac` of the pixels.
 MAX_DIFF = 2
-MAX_DIFF_FLAGGED = 4
 MAX_RMSE = 0.5
+ISO_FLIP_FRAC = 0.02
 
 
-def assert_parity(img, ref8, flags, what="", flagged_max=MAX_DIFF_FLAGGED, max_flagged_frac=1.0):
-    """Isosurface renders pass flagged_max=255: a ray whose samples come
-    within 1e-2 of the iso value may hit in one precision and miss in the
-    other; such pixels are counted (max_flagged_frac) instead of bounded."""
+def assert_parity(img, ref8, flags, what="", max_flip_frac=0.0):
     d, rmse = pixel_diff(img, ref8)
     ok = flags == 0
-    m_ok = int(d[ok].max()) if ok.any() else 0
-    m_fl = int(d[~ok].max()) if (~ok).any() else 0
+    dp = d.max(axis=-1)
+    m_ok = int(dp[ok].max()) if ok.any() else 0
+    m_fl = int(dp[~ok].max()) if (~ok).any() else 0
+    flips = float(((dp > MAX_DIFF) & ~ok).mean())
     print(f"{what}: max|d| {m_ok}/255 (unflagged), {m_fl}/255 on {int((~ok).sum())} flagged px, "
-          f"rmse {rmse:.4f}/255")
+          f"{flips:.4%} beyond the bar, rmse {rmse:.4f}/255")
     assert m_ok <= MAX_DIFF, f"{what}: max |d| {m_ok} > {MAX_DIFF}"
-    assert m_fl <= flagged_max, f"{what}: flagged max |d| {m_fl}"
-    assert (~ok).mean() <= max_flagged_frac, f"{what}: {(~ok).mean():.4f} of the pixels flagged"
-    rmse_ok = float(np.sqrt((d[ok].astype(np.float64) ** 2).mean())) if ok.any() else 0.0
-    assert (rmse if flagged_max <= MAX_DIFF_FLAGGED else rmse_ok) <= MAX_RMSE, f"{what}: rmse {rmse}"
+    assert flips <= max_flip_frac, f"{what}: {flips:.4%} of the pixels beyond {MAX_DIFF}/255"
+    assert rmse <= MAX_RMSE, f"{what}: rmse {rmse}"
     return d, rmse
 
 

@@ -140,3 +140,31 @@ def test_paged_flat_volume_levels_match_pyramid():
     mn, mx = pg.brickMinMax(1)
     assert (mn[1, 0, 1], mx[1, 0, 1]) == O.brick_minmax(levels[1], 16, 2, 1, 0, 1)
     pg.cleanup()
+
+
+def test_pool_too_small_at_every_allowed_lod_is_oom():
+    """A view whose bricks exceed the pool at the coarsest allowed LOD
+    cannot complete: capture and batched renders report out of memory
+    instead of returning a frame with missing bricks."""
+    from imagevis3d_amd import TvkError
+    w, vol = setup()
+    print("case pool too small at every LOD", flush=True)
+    pg = renderer(w)
+    # 64 LOD-0 bricks and 8 LOD-1 bricks; 6 slots hold neither view's set
+    pg.setPagedFlatVolume(vol, w.brick, 6 * slot_bytes(w.brick), n_lod=2)
+    pg.set1DTrans(w.tf)
+    pg.setLODLimits((0, 1))
+    workloads.apply(pg, w, w.cameras[3])
+    with pytest.raises(TvkError, match="(?i)memory|pool"):
+        pg.capture()
+    views = [workloads.view_of(w.cameras[3], w.width, w.height)]
+    with pytest.raises(TvkError, match="(?i)memory|pool"):
+        pg.renderBatch(views)
+    # a pool that holds LOD 1 renders the same view at LOD 1
+    pg.setPagedFlatVolume(vol, w.brick, 12 * slot_bytes(w.brick), n_lod=2)
+    pg.set1DTrans(w.tf)
+    pg.setLODLimits((0, 1))
+    workloads.apply(pg, w, w.cameras[3])
+    pg.capture()
+    assert pg.frameStats().lod == 1
+    pg.cleanup()

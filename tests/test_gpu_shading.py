@@ -4,7 +4,7 @@ isosurface hit/miss decisions within 1e-2 of the iso value are flagged)."""
 import numpy as np
 import pytest
 
-from conftest import assert_parity
+from conftest import ISO_FLIP_FRAC, assert_parity
 from imagevis3d_amd import RenderMode, transfer, tuvok, workloads
 from imagevis3d_amd.workloads import Camera
 from oracle import oracle as O
@@ -105,8 +105,7 @@ def test_isosurface_parity(interp):
     hit = ref8[..., 3] == 255
     assert hit.mean() > 0.05
     # nearest sampling also flags samples within 1e-4 of a voxel face
-    assert_parity(img, ref8, fl, f"isosurface interp={interp}", flagged_max=255,
-                  max_flagged_frac=0.02 if interp else 0.3)
+    assert_parity(img, ref8, fl, f"isosurface interp={interp}", max_flip_frac=ISO_FLIP_FRAC)
     r.setEmptySpaceSkipping(False)
     assert r.paint()
     assert np.array_equal(r.capture(), img)
@@ -154,7 +153,7 @@ def test_mip_parity(interp):
     cam = w.cameras[5]
     img, st = render(r, w, cam)
     ref8, ns, fl = oracle_ref(vol, w, cam, w.tf, mode=3, interp=interp)
-    assert_parity(img, ref8, fl, f"MIP interp={interp}", max_flagged_frac=1.0 if not interp else 0.02)
+    assert_parity(img, ref8, fl, f"MIP interp={interp}")
     assert abs(int(st.evaluated_samples) - int(ns.sum())) <= 2e-3 * ns.sum() + 64
     # every sample is visited: skipping changes nothing
     r.setEmptySpaceSkipping(False)

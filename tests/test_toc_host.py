@@ -182,3 +182,36 @@ def test_histogram_blocks_are_read(tmp_path):
     with pytest.raises(TvkError):
         t2.histogram1D()
     t2.close()
+
+
+def test_crafted_headers_are_rejected(tmp_path):
+    """Headers whose fields contradict each other: no payload between the
+    overlaps, a level 0 that is not the volume, a level that does not halve
+    the one before (the brick grid still matching its own dims)."""
+    vol = np.arange(10 * 10 * 10, dtype=np.uint16).reshape(10, 10, 10)
+    good = str(tmp_path / "good.uvf")
+    levels = write_toc(good, vol, 8, 2, 0)
+    assert [d for d, _, _ in levels] == [(10, 10, 10), (5, 5, 5), (3, 3, 3)]
+    raw = bytearray(open(good, "rb").read())
+    toc = raw.index(b"TVKTOC01")
+    hdr = toc + 16                     # dims (3 x u64)
+    brick_at = hdr + 24                # max_brick, overlap, payload, n_lod (u32)
+    lvl = brick_at + 16 + 24           # per level: dims, grid (3 x u64 each), first (u64)
+
+    def patched(name, edits):
+        b = bytearray(raw)
+        for off, fmt, val in edits:
+            struct.pack_into(fmt, b, off, *val)
+        p = str(tmp_path / f"{name}.uvf")
+        open(p, "wb").write(bytes(b))
+        return p
+
+    cases = {
+        "no_payload": [(brick_at, "<III", (4, 2, 0))],
+        "level0_not_volume": [(hdr, "<Q", (11,))],
+        "level1_not_half": [(lvl + 56, "<Q", (6,))],  # 6 still needs 2 bricks of 4
+    }
+    for name, edits in cases.items():
+        with pytest.raises(TvkError, match="TOC"):
+            TocVolume(patched(name, edits))
+    TocVolume(patched("unchanged", [])).close()

@@ -8,8 +8,8 @@ AbstrRenderer-shaped host class.  This package holds its build script, the
 ctypes binding, and a Python mirror of Tuvok's Lua renderer interface.
 """
 from ._lib import TvkError, load  # noqa: F401
-from .renderer import (Context, Interpolant, RenderMode, Renderer, brick_slot_bytes,  # noqa: F401
+from .renderer import (BlendPrecision, Context, Interpolant, RenderMode, Renderer, brick_slot_bytes,  # noqa: F401
                        deleteClass, generate_volume, generate_volume_gpu, tuvok)
 
-__all__ = ["TvkError", "load", "Context", "Interpolant", "RenderMode", "Renderer",
+__all__ = ["TvkError", "load", "BlendPrecision", "Context", "Interpolant", "RenderMode", "Renderer",
            "brick_slot_bytes", "deleteClass", "generate_volume", "generate_volume_gpu", "tuvok"]

@@ -174,6 +174,14 @@ SIGNATURES = {
     "tvk_set_light_colors": [H, P, P, P, P],
     "tvk_set_early_ray_termination": [H, f32],
     "tvk_set_empty_space_skipping": [H, i32],
+    "tvk_set_clip_plane": [H, P],
+    "tvk_enable_clip_plane": [H, i32],
+    "tvk_is_clip_plane_enabled": [H, C.POINTER(i32)],
+    "tvk_get_clip_plane": [H, P],
+    "tvk_set_blend_precision": [H, i32],
+    "tvk_get_blend_precision": [H, C.POINTER(i32)],
+    "tvk_set_perf_measures": [H, u32, i32, f32, f32, u32],
+    "tvk_read_framebuffer": [H, P, u64],
     "tvk_set_kernel_tuning": [H, u32, u32, u32],
     "tvk_set_work_group_waves": [H, u32],
     "tvk_set_ray_tile_shape": [H, u32],

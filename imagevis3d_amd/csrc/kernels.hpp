@@ -122,5 +122,8 @@ hipError_t launch_raycast(const LaunchInfo& li, const JobTable* tab, hipStream_t
 // strides over all jobs' pixels).
 constexpr int kBackgroundBlocks = 1024;
 hipError_t launch_background(const JobTable* tab, hipStream_t stream);
+// dst (dw x dh) from src (sw x sh) by nearest pixel: dst(x, y) = src(x sw / dw, y sh / dh)
+hipError_t launch_upscale_nearest(const uint32_t* src, int sw, int sh, uint32_t* dst, int dw, int dh,
+                                  hipStream_t stream);
 
 }  // namespace tvk

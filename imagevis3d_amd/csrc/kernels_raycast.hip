@@ -36,6 +36,8 @@
 // the brick size.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace tvk {
@@ -460,6 +462,35 @@ __device__ inline uint32_t over_background(const RaycastParams& P, int py, float
   return pack_rgba8(fmaf(T1, bg0, C0), fmaf(T1, bg1, C1), fmaf(T1, bg2, C2), fmaf(T1, P.bg_alpha, A));
 }
 
+// Clip the ray interval [t0, t1] to the kept side of the clip plane
+// (clip.xyz . p + clip.w <= 0); a ray parallel to the plane is kept or
+// lost whole.
+__device__ inline void clip_ray(const RaycastParams& P, const float o[3], const float d[3], float& t0,
+                                float& t1, bool& hit) {
+  const float num = fmaf(P.clip[0], o[0], fmaf(P.clip[1], o[1], fmaf(P.clip[2], o[2], P.clip[3])));
+  const float den = fmaf(P.clip[0], d[0], fmaf(P.clip[1], d[1], P.clip[2] * d[2]));
+  if (den == 0.f) {
+    if (num > 0.f) hit = false;
+    return;
+  }
+  const float tc = -num / den;
+  if (den > 0.f) t1 = fminf(t1, tc);
+  else t0 = fmaxf(t0, tc);
+}
+
+// The running composite rounded to the blend target's precision: unorm8
+// (round to nearest of x * 255) or half float (round to nearest even).
+__device__ inline float blend_quantize(float x, int bits) {
+  if (bits == 8) return floorf(fmaf(clampf(x, 0.f, 1.f), 255.f, 0.5f)) / 255.f;
+  return (float)(_Float16)x;
+}
+__device__ inline void blend_round(int bits, float& C0, float& C1, float& C2, float& A) {
+  C0 = blend_quantize(C0, bits);
+  C1 = blend_quantize(C1, bits);
+  C2 = blend_quantize(C2, bits);
+  A = blend_quantize(A, bits);
+}
+
 // The k-th tile row of an h-row rect in centre-out order: c, c+1, c-1,
 // c+2, c-2, ... (c = h/2), then the rest of the longer side.  The rows
 // through the middle of the box's projection hold the longest rays, so they
@@ -524,6 +555,9 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
     }
   }
   t0 = fmaxf(t0, 0.f);
+  // nominal samples count the chord through the box, before clipping
+  if (hit && t1 > t0) n_nom += (uint32_t)floorf((t1 - t0) * P.nominal_rate);
+  if (P.clip_on && hit) clip_ray(P, o, d, t0, t1, hit);
   const float L = t1 - t0;
   float C0 = 0.f, C1 = 0.f, C2 = 0.f, A = 0.f;
   float vmax = -1.f;  // MIP: largest sample value so far (-1: none)
@@ -538,12 +572,10 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
   if (MODE == kModeRegions) {
     int dominant = -1;
     if (hit && L > 0.f) {
-      n_nom += (uint32_t)floorf(L * P.nominal_rate);
       dominant = march_regions<T, LINEAR, BATCH, TF_LDS>(P, tf4, o, d, inv, t0, t1, C0, C1, C2, A, n_eval);
     }
     if (P.region_out) P.region_out[(int64_t)py * P.width + px] = dominant;
   } else if (hit && L > 0.f) {
-    n_nom += (uint32_t)floorf(L * P.nominal_rate);
     const float step = P.step;
     const int n = (int)ceilf(L / step);
     const float lastlen = fmaxf(L - (float)(n - 1) * step, 0.f);
@@ -578,6 +610,8 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
 #pragma unroll
         for (int i = 0; i < 3; ++i) Lh[3 + i] = h[i] / hl;
     }
+    constexpr bool kBlends = MODE == kMode1D || MODE == kModeGrad;
+    int blend_gid = -1;  // brick of the last blend into the target
     int k = 0;
     while (k < n) {
       const float len0 = (k == n - 1) ? lastlen : step;
@@ -606,6 +640,10 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
               ? gload(P.cell_empty + (lv.cell_base + (c[2] * ucount[1] + c[1]) * ucount[0] + c[0]))
               : 0;
       const int slot = gload(P.page_table + gid);
+      if (kBlends && P.blend_bits < 32 && gid != blend_gid) {
+        blend_round(P.blend_bits, C0, C1, C2, A);
+        blend_gid = gid;
+      }
       if (empty) {
         k = kend;
         continue;
@@ -782,6 +820,7 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
       }
       k = live ? kend : n;
     }
+    if (kBlends && P.blend_bits < 32) blend_round(P.blend_bits, C0, C1, C2, A);
   }
   if (MODE == kModeIso && cv_stage == 1) {  // context hit, no focus surface behind it
     C0 = cv_ac * cc[0];
@@ -841,6 +880,20 @@ __global__ __launch_bounds__(256) void background_kernel(const JobTable* __restr
       gstore(P.framebuffer + i, over_background(P, py, 0.f, 0.f, 0.f, 0.f));
       if (P.region_out) gstore(P.region_out + i, (int32_t)-1);
     }
+  }
+}
+
+// Nearest-pixel enlargement of a reduced-resolution subframe into the
+// framebuffer (the governor's low-resolution intermediates): integer
+// arithmetic, so the host can restate it exactly.
+__global__ __launch_bounds__(256) void upscale_nearest_kernel(const uint32_t* __restrict__ src, int sw,
+                                                              int sh, uint32_t* __restrict__ dst, int dw,
+                                                              int dh) {
+  const int64_t n = (int64_t)dw * dh;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const int y = (int)(i / dw), x = (int)(i - (int64_t)y * dw);
+    const int sx = (int)((int64_t)x * sw / dw), sy = (int)((int64_t)y * sh / dh);
+    gstore(dst + i, gload(src + (int64_t)sy * sw + sx));
   }
 }
 
@@ -992,6 +1045,14 @@ hipError_t launch_cell_empty(const uint32_t* cell_range, int64_t n_cells, const 
   const unsigned blocks = (unsigned)((n_cells + 255) / 256);
   cell_empty_kernel<<<blocks, 256, 0, stream>>>(cell_range, n_cells, prefix, n_tf, tf_scale,
                                                 mode == 2 ? kModeIso : mode, iso_value, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_upscale_nearest(const uint32_t* src, int sw, int sh, uint32_t* dst, int dw, int dh,
+                                  hipStream_t stream) {
+  const int64_t n = (int64_t)dw * dh;
+  const int blocks = (int)std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(upscale_nearest_kernel, dim3(blocks), dim3(256), 0, stream, src, sw, sh, dst, dw, dh);
   return hipGetLastError();
 }
 

@@ -205,6 +205,7 @@ HipRenderer::HipRenderer(int type, bool pow2, bool to8, bool no_border, bool bia
     : type_(type), pow2_(pow2), to8_(to8), no_border_(no_border), bias_scale_tf_(bias_scale_tf) {
   mat_identity(rot_);
   mat_identity(trans_);
+  for (double& t : lod_ms_) t = -1.0;
 }
 
 HipRenderer::~HipRenderer() {
@@ -260,6 +261,8 @@ int HipRenderer::Resize(uint32_t w, uint32_t h) {
   TVK_HIP(hipSetDevice(device_));
   TVK_HIP(hipStreamSynchronize(stream_));
   TVK_HIP(framebuffer_.alloc((size_t)w * h * 4));
+  for (double& t : lod_ms_) t = -1.0;  // timings were for the old window size
+  time_pending_ = false;
   width_ = w;
   height_ = h;
   invalidate();
@@ -359,6 +362,8 @@ int HipRenderer::plan_levels(const uint64_t dims[3], uint32_t bits, uint32_t bri
 void HipRenderer::release_volume() {
   has_volume_ = false;
   paged_ = false;
+  for (double& t : lod_ms_) t = -1.0;  // the governor's timings were for the old volume
+  time_pending_ = false;
   for (auto& l : levels_) l.reset();
   pool_.slots.reset();
   upload_stage_.reset();
@@ -1081,6 +1086,95 @@ int HipRenderer::SetEarlyRayTermination(float a) {
   return TVK_OK;
 }
 
+int HipRenderer::SetClipPlane(const float plane[4]) {
+  if (!plane) return arg_fail("null clip plane");
+  for (int i = 0; i < 4; ++i)
+    if (!isfinite(plane[i])) return arg_fail("clip plane must be finite");
+  if (plane[0] == 0.f && plane[1] == 0.f && plane[2] == 0.f) return arg_fail("clip plane normal must be non-zero");
+  for (int i = 0; i < 4; ++i) clip_[i] = plane[i];
+  invalidate();
+  return TVK_OK;
+}
+
+int HipRenderer::EnableClipPlane(bool on) {
+  clip_on_ = on;
+  invalidate();
+  return TVK_OK;
+}
+
+int HipRenderer::SetBlendPrecision(int precision) {
+  if (precision != TVK_BP_8BIT && precision != TVK_BP_16BIT && precision != TVK_BP_32BIT)
+    return arg_fail("blend precision must be 0 (8 bit), 1 (16 bit) or 2 (32 bit)");
+  blend_ = precision;
+  invalidate();
+  return TVK_OK;
+}
+
+int HipRenderer::SetPerfMeasures(uint32_t min_fps, bool low_res, float screen_dec, float sample_dec,
+                                 uint32_t lod_delay) {
+  if (!(screen_dec >= 1.f && screen_dec <= 64.f) || !(sample_dec >= 1.f && sample_dec <= 64.f))
+    return arg_fail("resolution and sample-rate decrease factors must be in [1, 64]");
+  min_fps_ = min_fps;
+  low_res_ = low_res;
+  screen_dec_ = screen_dec;
+  sample_dec_ = sample_dec;
+  lod_delay_ = lod_delay;
+  invalidate();
+  return TVK_OK;
+}
+
+// Predicted kernel time of a subframe at `lod`: its own last measurement
+// (normalised to full size and rate), else the nearest measured level's
+// scaled by 2 per level (each level halves the samples along a ray); an
+// intermediate subframe (lod > lo) with low-resolution intermediates costs
+// the pixel and sample-rate fractions of that.  < 0: nothing measured yet.
+double HipRenderer::predicted_ms(int lod, int lo) const {
+  double ms = lod_ms_[lod];
+  if (ms < 0.0) {
+    for (int dl = 1; dl < n_lod_ && ms < 0.0; ++dl) {
+      if (lod + dl < n_lod_ && lod_ms_[lod + dl] >= 0.0) ms = lod_ms_[lod + dl] * ldexp(1.0, dl);
+      else if (lod - dl >= 0 && lod_ms_[lod - dl] >= 0.0) ms = lod_ms_[lod - dl] * ldexp(1.0, -dl);
+    }
+  }
+  if (ms >= 0.0 && low_res_ && lod > lo) ms /= (double)screen_dec_ * screen_dec_ * sample_dec_;
+  return ms;
+}
+
+// The finest allowed LOD whose predicted subframe time fits the frame
+// budget 1000 / min_fps ms; the coarsest when nothing fits or nothing has
+// been measured (the reference's default: start coarse, refine).
+int HipRenderer::governor_start_lod(int lo, int hi) const {
+  if (min_fps_ == 0) return hi;
+  const double budget = 1000.0 / (double)min_fps_;
+  for (int l = lo; l <= hi; ++l) {
+    const double ms = predicted_ms(l, lo);
+    if (ms < 0.0) return hi;
+    if (ms <= budget) return l;
+  }
+  return hi;
+}
+
+void HipRenderer::record_subframe_time() {
+  if (!time_pending_ || !stats_slot_ || last_lod_ < 0) return;
+  time_pending_ = false;
+  LaunchSlot& sl = *stats_slot_;
+  if (hipEventSynchronize(sl.ev[3]) != hipSuccess) return;
+  float k = 0.f;
+  if (hipEventElapsedTime(&k, sl.ev[1], sl.ev[2]) != hipSuccess) return;
+  const double full = (double)k / last_scale_;
+  double& m = lod_ms_[last_lod_];
+  m = m < 0.0 ? full : 0.5 * m + 0.5 * full;
+}
+
+int HipRenderer::ReadFramebuffer(uint8_t* rgba, uint64_t bytes) {
+  if (!initialized_ || !framebuffer_.p) return state_fail("no framebuffer (initialize + resize first)");
+  if (!rgba || bytes < (uint64_t)width_ * height_ * 4) return arg_fail("framebuffer readback buffer too small");
+  TVK_HIP(hipSetDevice(device_));
+  TVK_HIP(hipMemcpyAsync(rgba, framebuffer_.p, (size_t)width_ * height_ * 4, hipMemcpyDeviceToHost, stream_));
+  TVK_HIP(hipStreamSynchronize(stream_));
+  return TVK_OK;
+}
+
 int HipRenderer::SetEmptySpaceSkipping(bool on) {
   skip_ = on;
   invalidate();
@@ -1156,7 +1250,7 @@ int HipRenderer::job_mode() const {
 }
 
 void HipRenderer::build_params(RaycastParams* p, int lod, const CameraState& cam_in, uint32_t width,
-                               uint32_t height, int tf_slot, uint32_t* framebuffer) const {
+                               uint32_t height, int tf_slot, uint32_t* framebuffer, float rate_div) const {
   memset(p, 0, sizeof(*p));
   // MIP rotation (setMIPRotationAngle, RenderWindow.cpp:1687-1703): the view
   // turned about its up vector around the volume centre (world origin)
@@ -1240,9 +1334,27 @@ void HipRenderer::build_params(RaycastParams* p, int lod, const CameraState& cam
   p->tiles_y = (p->height + tile_h - 1) / tile_h;
   for (int i = 0; i < 3; ++i) p->box[i] = (float)dims0_[i];
   set_tile_rect(p);
-  p->step = (float)(ldexp(1.0, lod) / (double)rate_);
+  p->step = (float)(ldexp(1.0, lod) * (double)rate_div / (double)rate_);
   p->nominal_rate = rate_;
   p->term_alpha = term_alpha_;
+  if (clip_on_) {
+    // world plane n.w + d with w = m.M (row vectors) and voxel v = m * vscale
+    // + voff: the plane in voxel space is (M n) / vscale, d + M[3].n - ...
+    double nm[3], dm = clip_[3];
+    for (int i = 0; i < 3; ++i) nm[i] = M[i * 4 + 0] * clip_[0] + M[i * 4 + 1] * clip_[1] + M[i * 4 + 2] * clip_[2];
+    for (int j = 0; j < 3; ++j) dm += M[12 + j] * clip_[j];
+    double nv[3], dv = dm, len = 0.0;
+    for (int i = 0; i < 3; ++i) {
+      nv[i] = nm[i] / vscale[i];
+      dv -= nv[i] * voff[i];
+      len += nv[i] * nv[i];
+    }
+    len = sqrt(len);  // normalised: the ray parameter is in voxels either way
+    for (int i = 0; i < 3; ++i) p->clip[i] = (float)(nv[i] / len);
+    p->clip[3] = (float)(dv / len);
+    p->clip_on = 1;
+  }
+  p->blend_bits = blend_ == TVK_BP_8BIT ? 8 : (blend_ == TVK_BP_16BIT ? 16 : 32);
   p->interp = interp_;
   p->batch = batch_;
   p->tile_group = tile_group_;
@@ -1468,12 +1580,35 @@ int HipRenderer::Paint() {
   if (int e = upload_tf()) return e;
   const int lo = std::min<int>((int)lod_min_, n_lod_ - 1);
   const int hi = std::max(lo, std::min<int>((int)lod_max_, n_lod_ - 1));
+  const bool interactive = target_ == TVK_RT_INTERACTIVE;
+  const bool governed = interactive && min_fps_ > 0;
+  if (governed) record_subframe_time();
   int lod;
-  if (target_ != TVK_RT_INTERACTIVE) lod = lo;
+  if (!interactive) lod = lo;
   else if (frame_complete_) lod = lo;
-  else if (next_lod_ < 0) lod = hi;
-  else lod = std::max(lo, next_lod_);
+  else if (next_lod_ < 0) {
+    // a new frame: the governor's start LOD (the coarsest without it)
+    lod = governor_start_lod(lo, hi);
+    delay_left_ = lod_delay_;
+  } else {
+    lod = std::max(lo, next_lod_);
+    if (governed && !capturing_ && delay_left_ > 0 && lod < last_lod_) {
+      --delay_left_;  // LOD delay: hold the refinement for this many paints
+      return TVK_OK;
+    }
+  }
   if (paged_) lod = std::max(lod, std::min(lod_floor_, hi));
+  // intermediate subframes at a reduced resolution and sample rate
+  const bool lowres = governed && low_res_ && lod > lo && !regions_active() && !region_out_on_ &&
+                      (screen_dec_ > 1.f || sample_dec_ > 1.f);
+  uint32_t rw = width_, rh = height_;
+  float rdiv = 1.f;
+  if (lowres) {
+    rw = std::max<uint32_t>(1, (uint32_t)floorf((float)width_ / screen_dec_));
+    rh = std::max<uint32_t>(1, (uint32_t)floorf((float)height_ / screen_dec_));
+    rdiv = sample_dec_;
+    if (lowres_fb_.bytes < (size_t)rw * rh * 4) TVK_HIP(lowres_fb_.alloc((size_t)rw * rh * 4));
+  }
 
   if (int e = prepare_cells(0, mode_index())) return e;
   if (regions_active()) {
@@ -1482,7 +1617,8 @@ int HipRenderer::Paint() {
     if (int e = prepare_regions(0)) return e;
   }
   RaycastParams p;
-  build_params(&p, lod, cam_, width_, height_, 0, framebuffer_.as<uint32_t>());
+  build_params(&p, lod, cam_, rw, rh, 0, lowres ? lowres_fb_.as<uint32_t>() : framebuffer_.as<uint32_t>(),
+               rdiv);
   if (regions_active() && region_out_on_) {
     const size_t need = (size_t)width_ * height_ * 4;
     if (region_out_dev_.bytes < need) TVK_HIP(region_out_dev_.alloc(need));
@@ -1499,7 +1635,12 @@ int HipRenderer::Paint() {
   LaunchSlot& sl = paint_slots_[paint_cur_];
   paint_cur_ ^= 1;
   if (int e = launch_jobs(sl, &p, 1, launch_info(p.n_tf), false)) return e;
+  if (lowres)
+    TVK_HIP(launch_upscale_nearest(lowres_fb_.as<uint32_t>(), (int)rw, (int)rh, framebuffer_.as<uint32_t>(),
+                                   (int)width_, (int)height_, stream_));
   stats_slot_ = &sl;
+  last_scale_ = lowres ? (double)rw * rh / ((double)width_ * height_) / (double)rdiv : 1.0;
+  time_pending_ = governed;
   bool pass_complete = true;
   if (paged_) {
     // read the brick requests of this pass and page the bricks in
@@ -1738,8 +1879,11 @@ int HipRenderer::Capture(uint8_t* rgba, uint64_t bytes, bool preserve) {
     invalidate();
   }
   int rc = TVK_OK;
-  // a paged pool may need a paging pass per LOD on top of the subframes
+  // a paged pool may need a paging pass per LOD on top of the subframes;
+  // a capture finishes the frame without the governor's LOD delay
+  capturing_ = true;
   for (int guard = 0; guard < 4 * kMaxLod + 8 && !frame_complete_ && rc == TVK_OK; ++guard) rc = Paint();
+  capturing_ = false;
   if (rc == TVK_OK && !frame_complete_) {
     set_last_error("the frame did not complete within the paging pass limit");
     rc = TVK_ERR_OOM;

@@ -207,6 +207,23 @@ class HipRenderer {
   int SetLightingEnabled(bool on);
   int SetLightColors(const float amb[4], const float dif[4], const float spec[4], const float dir[3]);
   int SetEarlyRayTermination(float a);
+  // Clip plane (a, b, c, d) in world space: points with a x + b y + c z + d
+  // > 0 are cut away while it is enabled.
+  int SetClipPlane(const float plane[4]);
+  int EnableClipPlane(bool on);
+  bool ClipPlaneEnabled() const { return clip_on_; }
+  void GetClipPlane(float plane[4]) const {
+    for (int i = 0; i < 4; ++i) plane[i] = clip_[i];
+  }
+  // Compositing target precision: TVK_BP_8BIT, TVK_BP_16BIT, TVK_BP_32BIT.
+  int SetBlendPrecision(int precision);
+  int BlendPrecision() const { return blend_; }
+  // Interactive frame-time governor (setPerfMeasures): min_fps 0 turns it
+  // off (every interactive frame then starts at the coarsest allowed LOD).
+  int SetPerfMeasures(uint32_t min_fps, bool low_res, float screen_dec, float sample_dec,
+                      uint32_t lod_delay);
+  // the framebuffer as the last paint left it (no further painting)
+  int ReadFramebuffer(uint8_t* rgba, uint64_t bytes);
   int SetEmptySpaceSkipping(bool on);
   int SetKernelTuning(uint32_t batch, uint32_t tile_group, uint32_t xcd_split);
   int SetWorkGroupWaves(uint32_t waves);  // 4 or 8 (variants without an LDS TF)
@@ -253,7 +270,11 @@ class HipRenderer {
   int view_camera(const tvk_view& v, uint32_t index, CameraState* c) const;
   int upload_tf();
   void build_params(RaycastParams* p, int lod, const CameraState& cam, uint32_t w, uint32_t h,
-                    int tf_slot, uint32_t* framebuffer) const;
+                    int tf_slot, uint32_t* framebuffer, float rate_div = 1.f) const;
+  // governor: the interactive frame's first LOD, and the time model behind it
+  int governor_start_lod(int lo, int hi) const;
+  double predicted_ms(int lod, int lo) const;
+  void record_subframe_time();
   LaunchInfo launch_info(int n_tf_lds) const;
   // fill slot's table from jobs[0..n) and launch (graph replay if enabled)
   int launch_jobs(LaunchSlot& slot, const RaycastParams* jobs, int n, const LaunchInfo& li,
@@ -381,6 +402,9 @@ class HipRenderer {
   float light_[15] = {0.1f, 0.1f, 0.1f, 1.f, 0.9f, 0.9f, 0.9f, 1.f,
                       0.3f, 0.3f, 0.3f, 16.f, 0.f, 0.f, 1.f};
   float term_alpha_ = 0.99f;
+  float clip_[4] = {0.f, 0.f, -1.f, 0.f};  // keeps z >= 0 of the world when enabled
+  bool clip_on_ = false;
+  int blend_ = TVK_BP_32BIT;
   bool skip_ = true;
   int batch_ = 8;
   int tile_group_ = 4;
@@ -388,6 +412,17 @@ class HipRenderer {
   int wg_waves_ = 4;
   int tile_shift_ = 3;  // 8 x 8 ray tiles
   int target_ = TVK_RT_INTERACTIVE;
+
+  // frame-time governor (setPerfMeasures)
+  uint32_t min_fps_ = 0;
+  bool low_res_ = false;
+  float screen_dec_ = 2.f, sample_dec_ = 2.f;
+  uint32_t lod_delay_ = 0, delay_left_ = 0;
+  double lod_ms_[kMaxLod];             // measured full-size subframe ms per LOD (< 0: none)
+  double last_scale_ = 1.0;            // pixel x rate fraction of the last subframe
+  bool time_pending_ = false;          // the last subframe's time is not recorded yet
+  bool capturing_ = false;             // inside Capture: no LOD delay
+  DevBuf lowres_fb_;                   // reduced-resolution intermediate subframes
 
   // progress
   bool frame_complete_ = false;

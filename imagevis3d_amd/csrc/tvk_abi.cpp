@@ -630,6 +630,63 @@ int tvk_set_empty_space_skipping(tvk_renderer* r, int on) {
   return r->impl.SetEmptySpaceSkipping(on != 0);
 }
 
+int tvk_set_clip_plane(tvk_renderer* r, const float plane[4]) {
+  TVK_GUARD(r);
+  return r->impl.SetClipPlane(plane);
+}
+
+int tvk_enable_clip_plane(tvk_renderer* r, int enabled) {
+  TVK_GUARD(r);
+  return r->impl.EnableClipPlane(enabled != 0);
+}
+
+int tvk_is_clip_plane_enabled(tvk_renderer* r, int* enabled) {
+  TVK_GUARD(r);
+  if (!enabled) {
+    tvk::set_last_error("null output pointer");
+    return TVK_ERR_ARG;
+  }
+  *enabled = r->impl.ClipPlaneEnabled() ? 1 : 0;
+  return TVK_OK;
+}
+
+int tvk_get_clip_plane(tvk_renderer* r, float plane[4]) {
+  TVK_GUARD(r);
+  if (!plane) {
+    tvk::set_last_error("null output pointer");
+    return TVK_ERR_ARG;
+  }
+  r->impl.GetClipPlane(plane);
+  return TVK_OK;
+}
+
+int tvk_set_blend_precision(tvk_renderer* r, int precision) {
+  TVK_GUARD(r);
+  return r->impl.SetBlendPrecision(precision);
+}
+
+int tvk_get_blend_precision(tvk_renderer* r, int* precision) {
+  TVK_GUARD(r);
+  if (!precision) {
+    tvk::set_last_error("null output pointer");
+    return TVK_ERR_ARG;
+  }
+  *precision = r->impl.BlendPrecision();
+  return TVK_OK;
+}
+
+int tvk_set_perf_measures(tvk_renderer* r, uint32_t min_framerate, int low_res_intermediate,
+                          float screen_res_dec_factor, float sample_dec_factor, uint32_t lod_delay) {
+  TVK_GUARD(r);
+  return r->impl.SetPerfMeasures(min_framerate, low_res_intermediate != 0, screen_res_dec_factor,
+                                 sample_dec_factor, lod_delay);
+}
+
+int tvk_read_framebuffer(tvk_renderer* r, uint8_t* rgba, uint64_t rgba_bytes) {
+  TVK_GUARD(r);
+  return r->impl.ReadFramebuffer(rgba, rgba_bytes);
+}
+
 int tvk_set_kernel_tuning(tvk_renderer* r, uint32_t batch, uint32_t tile_group,
                           uint32_t xcd_split) {
   TVK_GUARD(r);

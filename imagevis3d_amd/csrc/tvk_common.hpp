@@ -99,6 +99,15 @@ struct RaycastParams {
   float step;           // sample spacing in LOD-0 voxels (2^lod / rate)
   float nominal_rate;   // user rate: nominal samples per LOD-0 voxel
   float term_alpha;     // early ray termination threshold
+  // Clip plane in LOD-0 voxel space: points with clip.xyz . p + clip.w > 0
+  // are cut away (the ray interval is clipped to the kept half-space).
+  float clip[4];
+  int32_t clip_on;
+  // Blend precision of the compositing target: 8 (unorm8), 16 (half) or 32
+  // (float).  Below 32 the running composite (C, A) is rounded to it each
+  // time the ray enters another brick of the level and at the end, as a
+  // per-brick blend into a target of that precision would (1D / 2D TF).
+  int32_t blend_bits;
   int32_t interp;       // 0 nearest, 1 trilinear
   int32_t batch;        // samples gathered per step of the march (1, 2, 4, 8)
   int32_t tile_group;   // consecutive ray tiles dealt to one XCD in turn

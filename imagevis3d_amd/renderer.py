@@ -58,6 +58,13 @@ class Interpolant:
     Linear = 1
 
 
+class BlendPrecision:
+    """AbstrRenderer::EBlendPrecision (SettingsDlg.cpp:247-251)."""
+    BP_8BIT = 0
+    BP_16BIT = 1
+    BP_32BIT = 2
+
+
 class Context:
     """What tuvok.createContext returns: the HIP device the renderer uses
     (BatchRenderer/main.cpp:49-65 makes a GL context instead)."""
@@ -574,6 +581,49 @@ class Renderer:
 
     def setEmptySpaceSkipping(self, on: bool) -> None:
         call("tvk_set_empty_space_skipping", self._h, int(on))
+
+    def setClipPlane(self, plane) -> None:
+        """World-space plane (a, b, c, d): points with a*x + b*y + c*z + d > 0
+        are cut away while the plane is enabled (RenderWindow.cpp:1920-1927)."""
+        a = np.ascontiguousarray(np.asarray(plane, np.float32).reshape(4))
+        call("tvk_set_clip_plane", self._h, a.ctypes.data)
+
+    def getClipPlane(self) -> tuple:
+        a = np.zeros(4, np.float32)
+        call("tvk_get_clip_plane", self._h, a.ctypes.data)
+        return tuple(float(x) for x in a)
+
+    def enableClipPlane(self, on: bool) -> None:
+        call("tvk_enable_clip_plane", self._h, int(on))
+
+    def isClipPlaneEnabled(self) -> bool:
+        v = C.c_int32(0)
+        call("tvk_is_clip_plane_enabled", self._h, C.byref(v))
+        return bool(v.value)
+
+    def setBlendPrecision(self, precision: int) -> None:
+        """BlendPrecision.BP_8BIT / BP_16BIT / BP_32BIT (RenderWindow.cpp:1618-1623)."""
+        call("tvk_set_blend_precision", self._h, int(precision))
+
+    def getBlendPrecision(self) -> int:
+        v = C.c_int32(0)
+        call("tvk_get_blend_precision", self._h, C.byref(v))
+        return v.value
+
+    def setPerfMeasures(self, minFramerate: int, lowResIntermediate: bool = False,
+                        screenResDecFactor: float = 2.0, sampleDecFactor: float = 2.0,
+                        LODDelay: int = 0) -> None:
+        """The interactive frame-time governor (RenderWindow.cpp:1625-1640);
+        minFramerate 0 turns it off."""
+        call("tvk_set_perf_measures", self._h, int(minFramerate), int(lowResIntermediate),
+             float(screenResDecFactor), float(sampleDecFactor), int(LODDelay))
+
+    def readFramebuffer(self) -> np.ndarray:
+        """The current subframe as (H, W, 4) uint8, without painting further."""
+        w, h = self._size
+        out = np.empty((h, w, 4), np.uint8)
+        call("tvk_read_framebuffer", self._h, out.ctypes.data, out.nbytes)
+        return out
 
     def setKernelTuning(self, samples_per_batch: int = 8, tile_group: int = 4,
                         xcd_split: int = 0) -> None:

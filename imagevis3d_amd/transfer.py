@@ -25,9 +25,19 @@ def load_1dt(path: str) -> np.ndarray:
 
 def resample_1d(tf: np.ndarray, size: int) -> np.ndarray:
     """Linear resampling of a 1D TF to `size` entries (end points kept), the
-    operation loadFromFileWithSize applies to match the histogram size."""
+    operation loadFromFileWithSize applies to match the histogram size.
+    Entry j of the result is the table read at x = j (n-1)/(size-1),
+    linearly between the two entries around x.  Tuvok's own resampling rule
+    is not in the reference tree (parity unpinned); this one is pinned by
+    closed forms in tests/test_transfer_1d.py."""
     tf = np.asarray(tf, np.float32)
     n = tf.shape[0]
+    if size < 1 or n < 1:
+        raise ValueError("a 1D TF needs at least one entry")
+    if size == 1:
+        return tf[:1].copy()
+    if n == 1:
+        return np.repeat(tf, size, axis=0)
     if n == size:
         return tf.copy()
     x = np.linspace(0.0, n - 1.0, size)
@@ -35,6 +45,64 @@ def resample_1d(tf: np.ndarray, size: int) -> np.ndarray:
     for c in range(4):
         out[:, c] = np.interp(x, np.arange(n), tf[:, c])
     return out
+
+
+class TransferFunction1D:
+    """The 1D TF proxy the editor talks to (Q1DTransferFunction.cpp): an
+    (N, 4) RGBA float table with the Lua names the reference calls --
+    ``getSize`` (:578), ``getColorData`` (:254-258; the returned array is the
+    table itself, so edits through it change the TF, as the reference's
+    shared colour vector does), ``setStdFunction`` (:441-455),
+    ``loadFromFileWithSize`` (:573-589).  ``Renderer.set1DTrans(tf.getColorData())``
+    is the upload the reference's ``tuvok.gpu.changed1DTrans`` makes (:468)."""
+
+    def __init__(self, size: int = 256):
+        if size < 1:
+            raise ValueError("a 1D TF needs at least one entry")
+        self._data = np.zeros((int(size), 4), np.float32)
+
+    def getSize(self) -> int:
+        return int(self._data.shape[0])
+
+    def getColorData(self) -> np.ndarray:
+        return self._data
+
+    def setStdFunction(self, fCenterPoint: float, fInvGradient: float, iComponent: int,
+                       bInvertedStep: bool = False) -> None:
+        """Step function in one component: 0 below and 1 above a linear ramp
+        whose width is fInvGradient (a fraction of the table) and whose
+        middle sits at fCenterPoint (a fraction of the table); inverted, 1
+        below and 0 above.  Both fractions are clamped to [0, 1].  The
+        rounding of the ramp's end points in Tuvok's TransferFunction1D is
+        not in the reference tree (parity unpinned); this ramp is exact."""
+        if iComponent not in (0, 1, 2, 3):
+            raise ValueError("component must be 0..3")
+        n = self.getSize()
+        c = min(max(float(fCenterPoint), 0.0), 1.0) * (n - 1)
+        wdt = min(max(float(fInvGradient), 0.0), 1.0) * (n - 1)
+        x = np.arange(n, dtype=np.float64)
+        if wdt > 0.0:
+            ramp = np.clip((x - (c - 0.5 * wdt)) / wdt, 0.0, 1.0)
+        else:
+            ramp = (x >= c).astype(np.float64)
+        self._data[:, iComponent] = (1.0 - ramp if bInvertedStep else ramp).astype(np.float32)
+
+    def loadFromFile(self, path: str) -> bool:
+        try:
+            self._data = load_1dt(path)
+        except (OSError, ValueError):
+            return False
+        return True
+
+    def loadFromFileWithSize(self, path: str, size: int) -> bool:
+        """Load a .1dt and resample it to `size` entries (the histogram size,
+        Q1DTransferFunction.cpp:573-589); size 0 keeps the file's size."""
+        try:
+            tf = load_1dt(path)
+        except (OSError, ValueError):
+            return False
+        self._data = resample_1d(tf, int(size)) if size else tf
+        return True
 
 
 def parse_2dt(path: str) -> dict:

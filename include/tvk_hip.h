@@ -71,6 +71,10 @@ enum tvk_target { TVK_RT_INTERACTIVE = 0, TVK_RT_CAPTURE = 1, TVK_RT_HEADLESS = 
 /* Interpolant (RenderWindow.cpp:437-446). */
 enum tvk_interpolant { TVK_NEAREST = 0, TVK_LINEAR = 1 };
 
+/* AbstrRenderer::EBlendPrecision (SettingsDlg.cpp:247-251: 0 = 8 bit,
+ * 1 = 16 bit, 2 = 32 bit; ImageVis3D_Settings.cpp:574 applies it). */
+enum tvk_blend_precision { TVK_BP_8BIT = 0, TVK_BP_16BIT = 1, TVK_BP_32BIT = 2 };
+
 typedef struct tvk_renderer tvk_renderer;
 
 /* Per-paint statistics (the progress counters of RenderWindow.cpp:453-486
@@ -375,6 +379,49 @@ int tvk_set_early_ray_termination(tvk_renderer* r, float alpha_threshold);
 /* Empty-space skipping on per-brick min/max (default on); the image is the
  * same either way, only the work changes. */
 int tvk_set_empty_space_skipping(tvk_renderer* r, int enabled);
+/* setClipPlane -- RenderWindow.cpp:1920-1927 (the region's ExtendedPlane,
+ * already in world space, RenderWindow.cpp:1852-1877): plane (a, b, c, d);
+ * points with a x + b y + c z + d > 0 are cut away while it is enabled.
+ * enableClipPlane -- RenderWindow.cpp:2229-2251; isClipPlaneEnabled --
+ * RenderWindow.cpp:613-617; getClipPlane -- RenderWindow.cpp:667-670.
+ * Clipping shortens each ray to the kept half-space (samples restart at
+ * the plane); nominal sample counts still use the whole box chord.  The
+ * default plane (0, 0, -1, 0) keeps z >= 0; clipping starts disabled. */
+int tvk_set_clip_plane(tvk_renderer* r, const float plane[4]);
+int tvk_enable_clip_plane(tvk_renderer* r, int enabled);
+int tvk_is_clip_plane_enabled(tvk_renderer* r, int* enabled);
+int tvk_get_clip_plane(tvk_renderer* r, float plane[4]);
+/* setBlendPrecision -- RenderWindow.cpp:1618-1623 (TVK_BP_*; default
+ * TVK_BP_32BIT).  Below 32 bit the running composite of the 1D / 2D TF
+ * modes is rounded to the target's precision (unorm8: round to nearest of
+ * x*255; half float: round to nearest even) each time a ray enters another
+ * brick of the rendered level and once at the end, as the per-brick blend
+ * into an 8- or 16-bit target rounds it.  Isosurface and MIP frames do not
+ * blend and are unaffected; so is the per-region LOD march. */
+int tvk_set_blend_precision(tvk_renderer* r, int precision);
+int tvk_get_blend_precision(tvk_renderer* r, int* precision);
+/* setPerfMeasures -- RenderWindow.cpp:1625-1640 (applied by
+ * ImageVis3D_Settings.cpp:575 with both decrease factors 2 and the LOD
+ * delay in redraw-timer ticks).  The interactive frame-time governor: a new
+ * interactive frame (after any change) starts at the finest allowed LOD
+ * whose predicted subframe time fits 1000 / min_framerate ms -- predicted
+ * from the kernel times this renderer measured per LOD, each level about
+ * half the work of the next finer one -- and, with nothing measured yet or
+ * nothing fitting, at the coarsest; then refines one LOD per paint.  With
+ * low_res_intermediate, every subframe coarser than the finest allowed LOD
+ * is rendered at width / screen_res_dec_factor x height /
+ * screen_res_dec_factor and sample rate / sample_dec_factor, then enlarged
+ * to the window by nearest pixel (dst(x, y) = src(x*w/W, y*h/H)).
+ * lod_delay: paints that return without rendering before each refinement
+ * (the user may still be interacting).  min_framerate 0 (the default) turns
+ * the governor off.  Capture and headless targets always render the finest
+ * allowed LOD at full resolution and rate; a capture skips the delay. */
+int tvk_set_perf_measures(tvk_renderer* r, uint32_t min_framerate, int low_res_intermediate,
+                          float screen_res_dec_factor, float sample_dec_factor, uint32_t lod_delay);
+/* The framebuffer as the last paint left it (the subframe the reference
+ * presents after each paint, RenderWindow.cpp:2453-2502), without painting
+ * further.  rgba: W*H*4 bytes, row 0 at the top. */
+int tvk_read_framebuffer(tvk_renderer* r, uint8_t* rgba, uint64_t rgba_bytes);
 
 /* Kernel launch tuning (no counterpart in the reference; chosen by the
  * autotuner): samples gathered per march step (1, 2, 4 or 8; default 8);

@@ -196,10 +196,41 @@ uint8_t* oracle_brick_empty_flags(const oracle_render_params* p, int nb[3]) {
   return e;
 }
 
+/* IEEE 754 binary16, round to nearest even: 11 significant bits for
+ * |x| >= 2^-14, multiples of 2^-24 below (subnormals); the composite stays
+ * far below the binary16 overflow. */
+static double quantize_half(double x) {
+  const double ax = fabs(x);
+  if (ax == 0.0) return x;
+  if (ax < ldexp(1.0, -14)) return nearbyint(x * ldexp(1.0, 24)) * ldexp(1.0, -24);
+  int e;
+  const double m = frexp(x, &e); /* x = m 2^e, 0.5 <= |m| < 1 */
+  return ldexp(nearbyint(ldexp(m, 11)), e - 11);
+}
+
+double oracle_blend_quantize(double x, int32_t bits) {
+  if (bits == 8) return floor(clampd(x, 0.0, 1.0) * 255.0 + 0.5) / 255.0;
+  if (bits == 16) return quantize_half(x);
+  return x;
+}
+
+static void blend_round(int bits, double C[3], double* A) {
+  for (int i = 0; i < 3; ++i) C[i] = oracle_blend_quantize(C[i], bits);
+  *A = oracle_blend_quantize(*A, bits);
+}
+
 int oracle_render_1d(const oracle_render_params* p, double* out, uint32_t* out_samples,
                      uint8_t* out_flags) {
   cam_ctx c;
   if (cam_setup(p, &c)) return -1;
+  double clip[4] = {0, 0, 0, 0};
+  if (p->clip_enabled) clip_plane_voxel(p, &c, clip);
+  const int blends = p->blend_bits > 0 && p->blend_bits < 32 && (p->mode == 0 || p->mode == 1);
+  int bnb[3] = {1, 1, 1};
+  if (blends) {
+    const int ln[3] = {p->nx, p->ny, p->nz};
+    for (int i = 0; i < 3; ++i) bnb[i] = (ln[i] + p->blend_brick - 1) / p->blend_brick;
+  }
   int nb[3] = {0, 0, 0};
   /* MIP (mode 3) visits every sample: a TF-transparent region may hold the
    * maximum, so the TF-based empty-space flags do not apply */
@@ -242,14 +273,26 @@ int oracle_render_1d(const oracle_render_params* p, double* out, uint32_t* out_s
       }
       if (box_hit(o, d, lo, hi, &t0, &t1)) {
         if (t0 < 0) t0 = 0;
+        if (p->clip_enabled && !clip_interval(clip, o, d, &t0, &t1)) t1 = t0;
         double L = t1 - t0;
         if (L > 0) {
           int64_t n = (int64_t)ceil(L / step);
+          int64_t blend_b = -1;
           for (int64_t k = 0; k < n; ++k) {
             double len = (k == n - 1) ? L - (double)(n - 1) * step : step;
             double t = t0 + (double)k * step + 0.5 * len;
             double q[3];
             for (int i = 0; i < 3; ++i) q[i] = (o[i] + t * d[i]) * lsc[i];
+            if (blends) {
+              /* the composite is blended into the target brick by brick */
+              int64_t b[3];
+              for (int i = 0; i < 3; ++i) b[i] = clampi((int)floor(q[i] / p->blend_brick), 0, bnb[i] - 1);
+              const int64_t bi = (b[2] * bnb[1] + b[1]) * bnb[0] + b[0];
+              if (bi != blend_b) {
+                blend_round(p->blend_bits, C, &A);
+                blend_b = bi;
+              }
+            }
             if (empty) {
               int b[3];
               for (int i = 0; i < 3; ++i) b[i] = clampi((int)floor(q[i] / p->brick_size), 0, nb[i] - 1);
@@ -312,6 +355,7 @@ int oracle_render_1d(const oracle_render_params* p, double* out, uint32_t* out_s
             if (fabs(A - p->term_alpha) < 1e-5) flags |= 1;
             if (A >= p->term_alpha) break;
           }
+          if (blends) blend_round(p->blend_bits, C, &A);
         }
       }
       if (cv_stage == 1) {  /* context hit, no focus surface behind it */
@@ -352,6 +396,8 @@ int oracle_render_regions(const oracle_render_params* p, int32_t n_levels,
   cam_ctx c;
   if (cam_setup(p, &c)) return -1;
   if (n_levels < 1 || n_levels > 16 || region_edge < 1) return -1;
+  double clip[4] = {0, 0, 0, 0};
+  if (p->clip_enabled) clip_plane_voxel(p, &c, clip);
   oracle_render_params lp[16];
   uint8_t* empty[16];
   int nb[16][3];
@@ -384,6 +430,7 @@ int oracle_render_regions(const oracle_render_params* p, int32_t n_levels,
       double t0, t1;
       if (box_hit(o, d, lo, hi, &t0, &t1)) {
         if (t0 < 0) t0 = 0;
+        if (p->clip_enabled && !clip_interval(clip, o, d, &t0, &t1)) t1 = t0;
         double t = t0;
         int live = t1 > t0;
         while (t < t1 && live) {

@@ -86,7 +86,22 @@ typedef struct oracle_render_params {
   double cv_size;
   double cv_context_scale;
   double cv_border_scale;
+  /* Clip plane (setClipPlane, RenderWindow.cpp:1920-1927), world space:
+   * points with a x + b y + c z + d > 0 are cut away; each ray's interval
+   * is clipped to the kept half-space and its samples start there. */
+  int32_t clip_enabled;
+  double clip_plane[4];
+  /* Blend precision (setBlendPrecision, RenderWindow.cpp:1618-1623): 8, 16
+   * or 32.  Below 32 the running composite (C, A) of modes 0 / 1 is rounded
+   * to unorm8 (floor(x * 255 + 0.5) / 255) or to binary16 (round to nearest
+   * even) whenever the next sample lies in another brick of edge
+   * blend_brick (the level's brick payload) and once at the end. */
+  int32_t blend_bits;
+  int32_t blend_brick;
 } oracle_render_params;
+
+/* The blend target's rounding of one value (bits 8 or 16; 32: unchanged). */
+double oracle_blend_quantize(double x, int32_t bits);
 
 /* FP64 reference raycast.  out_rgba: W*H*4 doubles (composited over the
  * background, [0,1]); out_samples: per-pixel evaluated samples (may be NULL);

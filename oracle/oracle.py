@@ -53,6 +53,8 @@ class RenderParams(C.Structure):
         ("cv_enabled", C.c_int32), ("cv_iso_value", C.c_double), ("cv_color", C.c_double * 3),
         ("cv_focus", C.c_double * 2), ("cv_size", C.c_double), ("cv_context_scale", C.c_double),
         ("cv_border_scale", C.c_double),
+        ("clip_enabled", C.c_int32), ("clip_plane", C.c_double * 4),
+        ("blend_bits", C.c_int32), ("blend_brick", C.c_int32),
     ]
 
 
@@ -85,6 +87,8 @@ def lib():
         L.oracle_histogram_2d.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
                                           C.c_int32, C.c_uint32, C.c_uint32, C.c_void_p]
         L.oracle_histogram_2d.restype = C.c_float
+        L.oracle_blend_quantize.argtypes = [C.c_double, C.c_int32]
+        L.oracle_blend_quantize.restype = C.c_double
         _lib = L
     return _lib
 
@@ -110,7 +114,8 @@ def make_params(vol: np.ndarray, tf: np.ndarray, *, dims0=None, lod=0, rescale=(
                 brick_size=0, apron=2, mode=0, tf2d=None, max_grad=1.0, lighting=False,
                 ambient=(0.1, 0.1, 0.1), diffuse=(0.9, 0.9, 0.9), specular=(0.3, 0.3, 0.3),
                 shininess=16.0, light_dir=(0.0, 0.0, 1.0), iso_value=0.0,
-                iso_color=(1.0, 1.0, 1.0), clearview=None):
+                iso_color=(1.0, 1.0, 1.0), clearview=None, clip_plane=None, blend_bits=32,
+                blend_brick=0):
     """Parameters of one render; ``vol`` is (nz, ny, nx) x-fastest.  For
     mode 1 ``tf2d`` is (H, W, 4) float32 (value along W) and ``tf_scale``
     maps values to its columns."""
@@ -169,9 +174,23 @@ def make_params(vol: np.ndarray, tf: np.ndarray, *, dims0=None, lod=0, rescale=(
         p.cv_size = clearview.get("size", 5.5)
         p.cv_context_scale = clearview.get("context_scale", 1.0)
         p.cv_border_scale = clearview.get("border_scale", 60.0)
+    # clip plane (a, b, c, d) in world space; blend precision 8 / 16 / 32
+    # with blend_brick = the sampled level's brick payload
+    if clip_plane is not None:
+        p.clip_enabled = 1
+        p.clip_plane[:] = clip_plane
+    if blend_bits < 32 and blend_brick <= 0:
+        raise ValueError("blend_bits < 32 needs the level's brick size (blend_brick)")
+    p.blend_bits = blend_bits
+    p.blend_brick = blend_brick
     # keep the buffers alive as long as the params
     p._keep = (vol, tf, t2)
     return p
+
+
+def blend_quantize(x: float, bits: int) -> float:
+    """The blend target's rounding of one composite value (8 or 16 bits)."""
+    return lib().oracle_blend_quantize(float(x), int(bits))
 
 
 def render(p: RenderParams):

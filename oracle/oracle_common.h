@@ -129,6 +129,38 @@ static inline int box_hit(const double o[3], const double d[3], const double lo[
 }
 
 
+/* The world clip plane (p->clip_plane) in LOD-0 voxel space.  World
+ * points are w = m.M (row vectors, M = p->model) with voxel coordinates
+ * v = m * vscale + voff, so n.w + d = (M n / vscale) . v + d + M[3].n -
+ * sum_i (M n)_i voff_i / vscale_i. */
+static inline void clip_plane_voxel(const oracle_render_params* p, const cam_ctx* c, double out[4]) {
+  const double* M = p->model;
+  const double* n = p->clip_plane;
+  double dv = n[3];
+  for (int j = 0; j < 3; ++j) dv += M[12 + j] * n[j];
+  for (int i = 0; i < 3; ++i) {
+    out[i] = (M[i * 4 + 0] * n[0] + M[i * 4 + 1] * n[1] + M[i * 4 + 2] * n[2]) / c->vscale[i];
+    dv -= out[i] * c->voff[i];
+  }
+  out[3] = dv;
+}
+
+/* Clip [t0, t1] to the kept side (pl . p <= 0) of a voxel-space plane;
+ * returns 0 when a ray parallel to the plane lies wholly on the cut side. */
+static inline int clip_interval(const double pl[4], const double o[3], const double d[3], double* t0,
+                                double* t1) {
+  const double num = pl[0] * o[0] + pl[1] * o[1] + pl[2] * o[2] + pl[3];
+  const double den = pl[0] * d[0] + pl[1] * d[1] + pl[2] * d[2];
+  if (den == 0.0) return num <= 0.0;
+  const double tc = -num / den;
+  if (den > 0.0) {
+    if (tc < *t1) *t1 = tc;
+  } else if (tc > *t0) {
+    *t0 = tc;
+  }
+  return 1;
+}
+
 /* TF-empty flag per brick of the sampled level (oracle.c). */
 uint8_t* oracle_brick_empty_flags(const oracle_render_params* p, int nb[3]);
 

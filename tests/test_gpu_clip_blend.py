@@ -1,0 +1,146 @@
+"""Clip plane (setClipPlane / enableClipPlane, RenderWindow.cpp:1920-1927,
+2229-2251) and blend precision (setBlendPrecision, RenderWindow.cpp:
+1618-1623) on the HIP raycaster against the FP64 oracle, with the
+closed-form half-image check on the device."""
+import numpy as np
+import pytest
+
+from conftest import ISO_FLIP_FRAC, assert_parity
+from imagevis3d_amd import BlendPrecision, RenderMode, TvkError, transfer, workloads
+from oracle import oracle as O
+from test_gpu_shading import LIGHT, make_renderer, medium_small, oracle_ref, render
+
+pytestmark = pytest.mark.gpu
+
+PLANES = [(1.0, 0.0, 0.0, 0.0), (0.3, -0.5, 0.8, 0.07), (0.0, 0.2, 1.0, -0.3)]
+
+
+def test_clip_setters_round_trip():
+    w, vol = medium_small(64, 48)
+    r = make_renderer(w, vol)
+    assert not r.isClipPlaneEnabled()
+    r.setClipPlane((0.0, 1.0, 0.0, 0.25))
+    r.enableClipPlane(True)
+    assert r.isClipPlaneEnabled() and r.getClipPlane() == (0.0, 1.0, 0.0, 0.25)
+    with pytest.raises(TvkError):
+        r.setClipPlane((0.0, 0.0, 0.0, 1.0))
+    with pytest.raises(TvkError):
+        r.setBlendPrecision(3)
+    assert r.getBlendPrecision() == BlendPrecision.BP_32BIT
+    r.cleanup()
+
+
+@pytest.mark.parametrize("plane", PLANES)
+def test_clip_plane_parity(plane):
+    w, vol = medium_small(256, 144)
+    print(f"case clip plane {plane}", flush=True)
+    r = make_renderer(w, vol)
+    r.set1DTrans(w.tf)
+    r.setClipPlane(plane)
+    r.enableClipPlane(True)
+    for view in (3, 17):
+        cam = w.cameras[view]
+        img, st = render(r, w, cam)
+        ref8, ns, fl = oracle_ref(vol, w, cam, w.tf, clip_plane=plane)
+        assert_parity(img, ref8, fl, f"clip {plane} view {view}")
+        assert abs(int(st.evaluated_samples) - int(ns.sum())) <= 2e-3 * ns.sum() + 64
+        # nominal samples ignore the plane
+        assert st.nominal_samples == O.nominal_samples(
+            O.make_params(vol, w.tf, **workloads.oracle_kwargs(w, cam)))
+    r.enableClipPlane(False)
+    img, _ = render(r, w, w.cameras[3])
+    ref8, _, fl = oracle_ref(vol, w, w.cameras[3], w.tf)
+    assert_parity(img, ref8, fl, "clip disabled")
+    r.cleanup()
+
+
+def test_clip_plane_through_centre_is_half_background():
+    """Ortho face-filling view along -z, plane x = 0: the right half of the
+    frame is the background exactly, the left half equals the unclipped
+    frame bit for bit."""
+    w, vol = medium_small(128, 128)
+    cam = workloads.Camera((0.0, 0.0, 1.5), (0.0, 0.0, -1.0), (0.0, 1.0, 0.0), 60.0, True, 0.5)
+    w.bg_top = w.bg_bottom = (0.2, 0.3, 0.4)
+    r = make_renderer(w, vol)
+    r.set1DTrans(w.tf)
+    full, _ = render(r, w, cam)
+    r.setClipPlane((1.0, 0.0, 0.0, 0.0))
+    r.enableClipPlane(True)
+    cut, _ = render(r, w, cam)
+    assert np.array_equal(cut[:, :64], full[:, :64])
+    bg = O.to_rgba8(np.array([[[0.2, 0.3, 0.4, 1.0]]]))[0, 0]
+    assert np.all(cut[:, 64:] == bg)
+    r.cleanup()
+
+
+@pytest.mark.parametrize("mode", ["2d_phong", "iso", "mip"])
+def test_clip_plane_other_modes(mode):
+    w, vol = medium_small(256, 144)
+    print(f"case clip plane mode {mode}", flush=True)
+    plane = PLANES[1]
+    lighting = mode == "2d_phong"
+    r = make_renderer(w, vol, bias_scale_tf=mode == "2d_phong", lighting=lighting)
+    r.setClipPlane(plane)
+    r.enableClipPlane(True)
+    cam = w.cameras[8]
+    if mode == "2d_phong":
+        _, max_grad = O.histogram_2d_exact(vol, 256, 256)
+        tf2 = transfer.gaussian_lobes_2d(256, 256, seed=3)
+        r.set2DTrans(tf2, max_grad)
+        r.setRenderMode(RenderMode.RM_2DTRANS)
+        img, _ = render(r, w, cam)
+        scale = float(np.float32(255.0 / float(vol.max())))
+        ref8, _, fl = oracle_ref(vol, w, cam, None, mode=1, tf2d=tf2, max_grad=max_grad,
+                                 tf_scale=scale, lighting=True, clip_plane=plane, **LIGHT)
+        assert_parity(img, ref8, fl, "clip 2D TF + Phong")
+    elif mode == "iso":
+        r.setLightColors((*LIGHT["ambient"], 1.0), (*LIGHT["diffuse"], 1.0),
+                         (*LIGHT["specular"], LIGHT["shininess"]), LIGHT["light_dir"])
+        r.setRenderMode(RenderMode.RM_ISOSURFACE)
+        r.setIsoValue(2700.0)
+        img, _ = render(r, w, cam)
+        ref8, _, fl = oracle_ref(vol, w, cam, None, mode=2, iso_value=2700.0, clip_plane=plane,
+                                 **LIGHT)
+        assert_parity(img, ref8, fl, "clip isosurface", max_flip_frac=ISO_FLIP_FRAC)
+    else:
+        r.set1DTrans(w.tf)
+        r.setUseMIP(True)
+        img, _ = render(r, w, cam)
+        ref8, _, fl = oracle_ref(vol, w, cam, w.tf, mode=3, clip_plane=plane)
+        assert_parity(img, ref8, fl, "clip MIP")
+    r.cleanup()
+
+
+@pytest.mark.parametrize("precision,bits", [(BlendPrecision.BP_8BIT, 8), (BlendPrecision.BP_16BIT, 16)])
+def test_blend_precision_parity(precision, bits):
+    w, vol = medium_small(256, 144)
+    print(f"case blend precision {bits} bit", flush=True)
+    r = make_renderer(w, vol)
+    r.set1DTrans(w.tf)
+    cam = w.cameras[11]
+    full, _ = render(r, w, cam)
+    r.setBlendPrecision(precision)
+    assert r.getBlendPrecision() == precision
+    img, _ = render(r, w, cam)
+    ref8, _, fl = oracle_ref(vol, w, cam, w.tf, blend_bits=bits, blend_brick=w.brick)
+    assert_parity(img, ref8, fl, f"blend {bits} bit")
+    if bits == 8:
+        # rounding per brick visibly changes an 8-bit composite
+        assert (np.abs(img.astype(int) - full.astype(int)) > 0).mean() > 0.01
+    r.setBlendPrecision(BlendPrecision.BP_32BIT)
+    again, _ = render(r, w, cam)
+    assert np.array_equal(again, full)
+    r.cleanup()
+    # the 2D TF path rounds the same way
+    r2 = make_renderer(w, vol, bias_scale_tf=True)
+    _, max_grad = O.histogram_2d_exact(vol, 256, 256)
+    tf2 = transfer.gaussian_lobes_2d(256, 256, seed=3)
+    r2.set2DTrans(tf2, max_grad)
+    r2.setRenderMode(RenderMode.RM_2DTRANS)
+    r2.setBlendPrecision(precision)
+    img2, _ = render(r2, w, cam)
+    scale = float(np.float32(255.0 / float(vol.max())))
+    ref2, _, fl2 = oracle_ref(vol, w, cam, None, mode=1, tf2d=tf2, max_grad=max_grad,
+                              tf_scale=scale, blend_bits=bits, blend_brick=w.brick)
+    assert_parity(img2, ref2, fl2, f"blend {bits} bit 2D TF")
+    r2.cleanup()

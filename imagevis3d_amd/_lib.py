@@ -46,6 +46,14 @@ class MemStats(C.Structure):
     ]
 
 
+class MemManStats(C.Structure):
+    """tvk_memman_stats: the device's memory manager as a whole."""
+    _fields_ = [(n, C.c_uint64) for n in (
+        "gpu_mem", "gpu_budget", "gpu_allocated", "cpu_mem", "cpu_budget", "cpu_allocated",
+        "pool_bytes", "pool_slots", "pool_resident", "pool_evictions", "chunks_freed",
+        "cache_bytes", "cache_hits", "cache_misses")]
+
+
 class View(C.Structure):
     """tvk_view: one view of a batched launch."""
     _fields_ = [
@@ -107,7 +115,8 @@ class ServerStats(C.Structure):
     _fields_ = [
         ("submitted", C.c_uint64), ("completed", C.c_uint64), ("launches", C.c_uint64),
         ("max_jobs_per_launch", C.c_uint64), ("max_lanes_busy", C.c_uint64),
-        ("nominal_samples", C.c_uint64),
+        ("nominal_samples", C.c_uint64), ("paged_launches", C.c_uint64),
+        ("pool_waits", C.c_uint64), ("max_pool_reserved", C.c_uint64),
     ]
 
 
@@ -234,6 +243,10 @@ SIGNATURES = {
     "tvk_synchronize": [H],
     "tvk_set_max_usable_gpu_mem": [H, u64],
     "tvk_get_mem_stats": [H, C.POINTER(MemStats)],
+    "tvk_set_max_usable_cpu_mem": [H, u64],
+    "tvk_get_memman_stats": [i32, C.POINTER(MemManStats)],
+    "tvk_set_brick_source_threads": [H, u32],
+    "tvk_get_paging_stats": [H, C.POINTER(u64), C.POINTER(C.c_double)],
     "tvk_generate_volume": [i32, u64, u64, u64, u64, u32, P, u64],
     "tvk_generate_volume_gpu": [i32, i32, u64, u64, u64, u64, u32, P, u64],
     "tvk_set_generated_volume": [H, i32, u64, u64, u64, u64, u32, u32, u32, u32],

@@ -22,6 +22,7 @@ SOURCES = [
     "kernels_raycast.hip",
     "renderer.cpp",
     "paging.cpp",
+    "memman.cpp",
     "autotune.cpp",
     "server.cpp",
     "lod_select.cpp",

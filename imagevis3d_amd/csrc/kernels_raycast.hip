@@ -218,6 +218,13 @@ __device__ inline void fetch_value_grad(const T* __restrict__ base, const SlotGe
 // may hold the maximum) and rays do not terminate early.
 constexpr int kMode1D = 0, kModeGrad = 1, kModeIso = 2, kModeRegions = 3, kModeMIP = 4;
 
+// Device address of brick gid's slot from the page table (null: the brick
+// is not resident).  Slots may live in any allocation of the device pool.
+template <typename T>
+__device__ inline const T* brick_base(const RaycastParams& P, int gid) {
+  return (const T*)gload(P.page_table + gid);
+}
+
 // Sample at level texel coordinate q through the page table (clamp to
 // edge): used off the march's current brick (isosurface refinement and
 // normal), where the neighbouring brick may hold the point.
@@ -234,13 +241,12 @@ __device__ inline float fetch_level(const RaycastParams& P, float qx, float qy, 
     g[i] = qi - (float)(b[i] * P.brick - P.apron) - (LINEAR ? 0.5f : 0.f);
   }
   const int gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
-  const int slot = gload(P.page_table + gid);
-  if (slot < 0) {  // paged pool: request the brick; the frame is re-rendered
+  const T* base = brick_base<T>(P, gid);
+  if (!base) {  // paged pool: request the brick; the frame is re-rendered
     if (P.requests) P.requests[gid] = 1u;
     return 0.f;
   }
   if (P.touch) P.touch[gid] = P.frame;
-  const T* base = (const T*)P.pool + (int64_t)slot * P.slot_elems;
   return fetch<T, LINEAR>(base, slot_geo(P.slot_edge), g[0], g[1], g[2]);
 }
 
@@ -391,12 +397,11 @@ __device__ inline int march_regions(const RaycastParams& P, const float4* __rest
         g0[i] = oo[i] - (float)(b[i] * P.brick - P.apron) - (LINEAR ? 0.5f : 0.f);
       }
       const int gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
-      const int slot = gload(P.page_table + gid);
-      if (slot < 0) {
+      const T* __restrict__ base = brick_base<T>(P, gid);
+      if (!base) {
         if (P.requests) P.requests[gid] = 1u;
       } else {
         if (P.touch) P.touch[gid] = P.frame;
-        const T* __restrict__ base = (const T*)P.pool + (int64_t)slot * P.slot_elems;
         const SlotGeo G = slot_geo(P.slot_edge);
         const float step = ldexpf(base_step, l);
         const float span = tb - t;
@@ -639,7 +644,7 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
           (MODE != kModeMIP && P.skip)
               ? gload(P.cell_empty + (lv.cell_base + (c[2] * ucount[1] + c[1]) * ucount[0] + c[0]))
               : 0;
-      const int slot = gload(P.page_table + gid);
+      const T* __restrict__ base = brick_base<T>(P, gid);
       if (kBlends && P.blend_bits < 32 && gid != blend_gid) {
         blend_round(P.blend_bits, C0, C1, C2, A);
         blend_gid = gid;
@@ -648,13 +653,12 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
         k = kend;
         continue;
       }
-      if (slot < 0) {  // not resident: ask the host for it, sample nothing
+      if (!base) {  // not resident: ask the host for it, sample nothing
         if (P.requests) P.requests[gid] = 1u;
         k = kend;
         continue;
       }
       if (P.touch) P.touch[gid] = P.frame;
-      const T* __restrict__ base = (const T*)P.pool + (int64_t)slot * P.slot_elems;
       float g0[3];  // slot-local texel coordinate at t = 0
 #pragma unroll
       for (int i = 0; i < 3; ++i)

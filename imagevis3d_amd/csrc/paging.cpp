@@ -1,27 +1,36 @@
-// paging.cpp -- out-of-core volumes on a fixed HBM brick pool (the role of
-// Tuvok's GLGridLeaper paging over GPUMemMan, ImageVis3D/UI/RenderWindowGL.cpp
-// :153-163 and ImageVis3D_Settings.cpp:557-559 for the memory budget).
+// paging.cpp -- out-of-core volumes on the device-wide brick pool (the role
+// of Tuvok's GLGridLeaper paging over GPUMemMan, ImageVis3D/UI/
+// RenderWindowGL.cpp:153-163, with the memory budgets of
+// ImageVis3D_Settings.cpp:556-559).
 //
-// The pool holds `pool_bytes / slot_bytes` brick slots.  Bricks start
-// non-resident (page table -1).  A raycast that needs a non-resident,
-// non-empty brick writes requests[brick] = 1 and samples nothing there;
-// every brick it does sample gets touch[brick] = frame.  After the launch
-// the host reads the requests, fills bricks from the brick source into
-// pinned staging buffers, copies them to the device and rearranges them
-// (blocked slot layout) into free slots or into the least recently touched
-// slots not used by this frame, refreshes those bricks'
-// empty-space cell ranges on the device, and asks for another pass.  When
-// a view needs more bricks of a LOD than the pool holds, the frame falls
-// back to the next coarser LOD.  Empty bricks (by the per-brick ranges the
-// source provides) are skipped without ever being requested.
+// A paged volume's bricks start non-resident (page table 0).  A raycast
+// that needs a non-resident, non-empty brick writes requests[brick] = 1 and
+// samples nothing there; every brick it does sample gets touch[brick] =
+// frame.  After the launch the host stamps the sampled bricks' slots in the
+// device pool (LRU), reads the requests, takes slots from the pool (MemMan:
+// free slots, new chunks within the GPU budget, else the least recently
+// sampled slots of any volume -- never this frame's own), and streams the
+// bricks in: the brick source (or the host brick cache) fills pinned staging
+// buffers on worker threads, one round of bricks while the previous round's
+// copies run; each brick is copied to a device staging buffer on the copy
+// stream, rearranged into its slot's blocked layout and its empty-space cell
+// ranges refreshed there, and the render stream waits on the copy stream's
+// event before the page table (now holding the new slots) goes up and the
+// next pass runs.  When a view needs more bricks of a LOD than the volume
+// may hold, the frame falls back to the next coarser LOD.  Empty bricks (by
+// the per-brick ranges the source provides) are skipped without ever being
+// requested.
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <numeric>
 
 #include "errors.hpp"
 #include "host_volume.hpp"
 #include "kernels.hpp"
+#include "memman.hpp"
 #include "renderer.hpp"
 
 namespace tvk {
@@ -34,6 +43,10 @@ namespace tvk {
       return e_ == hipErrorOutOfMemory ? TVK_ERR_OOM : TVK_ERR_HIP;       \
     }                                                                     \
   } while (0)
+
+namespace {
+constexpr uint64_t kStagingTarget = 128ull << 20;  // pinned bytes per staging round
+}  // namespace
 
 int HipRenderer::SetPagedVolume(const uint64_t dims[3], uint32_t bits, uint32_t brick,
                                 uint32_t apron, uint32_t n_lod, BrickSourceFn fn, void* user,
@@ -48,21 +61,14 @@ int HipRenderer::SetPagedVolume(const uint64_t dims[3], uint32_t bits, uint32_t 
   }
   LevelPlan plan;
   if (int e = plan_levels(dims, bits, brick, apron, n_lod, &plan)) return e;
-  const uint64_t n_slots = pool_bytes / plan.slot_bytes;
-  if (n_slots < 1) {
+  const uint64_t quota = pool_bytes / plan.slot_bytes;
+  if (quota < 1) {
     set_last_error("brick pool smaller than one brick slot");
     return TVK_ERR_OOM;
   }
   const int total = plan.total_bricks;
   const uint64_t S = (uint64_t)plan.slot_edge;
   const uint64_t brick_bytes = S * S * S * (bits / 8);  // one brick as the source gives it
-  const uint64_t need = n_slots * plan.slot_bytes + brick_bytes + (uint64_t)total * 20 +
-                        (uint64_t)plan.total_cells * 4 + (uint64_t)width_ * height_ * 4;
-  if (need > gpu_budget_) {
-    set_last_error("paged pool needs " + std::to_string(need) + " bytes of device memory, budget is " +
-                   std::to_string(gpu_budget_));
-    return TVK_ERR_OOM;
-  }
   TVK_HIP(hipSetDevice(device_));
   TVK_HIP(hipStreamSynchronize(stream_));
   release_volume();
@@ -88,21 +94,18 @@ int HipRenderer::SetPagedVolume(const uint64_t dims[3], uint32_t bits, uint32_t 
         }
   }
 
-  TVK_HIP(pool_.slots.alloc(n_slots * plan.slot_bytes));
-  TVK_HIP(upload_stage_.alloc(brick_bytes));
-  TVK_HIP(pool_.page_table.alloc((size_t)total * sizeof(int32_t)));
+  TVK_HIP(pool_.page_table.alloc((size_t)total * sizeof(uint64_t)));
   TVK_HIP(pool_.brick_range.alloc((size_t)total * 2 * sizeof(float)));
   TVK_HIP(cell_range_.alloc((size_t)plan.total_cells * sizeof(uint32_t)));
   const size_t req_words = ((size_t)total + 1) / 2;  // cleared as u64 words
   TVK_HIP(requests_dev_.alloc(req_words * 8));
   TVK_HIP(touch_dev_.alloc((size_t)total * sizeof(uint32_t)));
-  pool_.n_slots = n_slots;
+  pool_.n_slots = quota;
   pool_.slot_bytes = plan.slot_bytes;
   pool_.resident = 0;
-  pool_.page_table_h.assign((size_t)total, -1);
-  pool_.slot_owner.assign((size_t)n_slots, -1);
-  pool_.slot_touch.assign((size_t)n_slots, 0u);
-  TVK_HIP(hipMemcpy(pool_.page_table.p, pool_.page_table_h.data(), (size_t)total * 4,
+  pool_.page_table_h.assign((size_t)total, 0);
+  pool_.brick_slot.assign((size_t)total, -1);
+  TVK_HIP(hipMemcpy(pool_.page_table.p, pool_.page_table_h.data(), (size_t)total * 8,
                     hipMemcpyHostToDevice));
   TVK_HIP(hipMemcpy(pool_.brick_range.p, pool_.brick_range_h.data(), (size_t)total * 8,
                     hipMemcpyHostToDevice));
@@ -115,13 +118,24 @@ int HipRenderer::SetPagedVolume(const uint64_t dims[3], uint32_t bits, uint32_t 
     TVK_HIP(hipHostMalloc((void**)&requests_h_, (size_t)total * 8, hipHostMallocDefault));
     requests_h_count_ = (uint64_t)total;
   }
-  for (int i = 0; i < kStaging; ++i) {
-    if (staging_[i]) (void)hipHostFree(staging_[i]);
-    staging_[i] = nullptr;
-    TVK_HIP(hipHostMalloc(&staging_[i], plan.slot_bytes, hipHostMallocDefault));
-    if (!staging_ev_[i]) TVK_HIP(hipEventCreate(&staging_ev_[i]));
-    TVK_HIP(hipEventRecord(staging_ev_[i], stream_));
+  // staging: two rounds of pinned buffers, one round of device buffers
+  MemMan& mm = MemMan::ForDevice(device_);
+  const int round = (int)std::max<uint64_t>(2, std::min<uint64_t>(32, kStagingTarget / brick_bytes));
+  const uint64_t host_bytes = 2ull * round * brick_bytes;
+  if (!mm.ReserveCPU(host_bytes)) {
+    set_last_error("the CPU memory budget cannot hold the brick staging buffers");
+    return TVK_ERR_OOM;
   }
+  staging_host_bytes_ = host_bytes;
+  staging_.assign((size_t)2 * round, nullptr);
+  for (void*& b : staging_) TVK_HIP(hipHostMalloc(&b, brick_bytes, hipHostMallocDefault));
+  upload_stage_.clear();
+  upload_stage_.resize((size_t)round);
+  for (DevBuf& d : upload_stage_) TVK_HIP(d.alloc(brick_bytes));
+  staging_round_ = round;
+  for (hipEvent_t& ev : staging_ev_)
+    if (!ev) TVK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  if (!upload_done_) TVK_HIP(hipEventCreateWithFlags(&upload_done_, hipEventDisableTiming));
 
   bits_ = bits;
   for (int i = 0; i < 3; ++i) dims0_[i] = dims[i];
@@ -141,6 +155,10 @@ int HipRenderer::SetPagedVolume(const uint64_t dims[3], uint32_t bits, uint32_t 
   lod_floor_ = 0;
   source_fn_ = fn;
   source_user_ = user;
+  // bricks of a flat host volume are cheap to cut again; other sources
+  // (files, decompression, callbacks) keep theirs in the host cache
+  cache_bricks_ = fn != &HipRenderer::flat_source;
+  pool_client_ = mm.Register(this, plan.slot_bytes, quota);
   paged_ = true;
   has_volume_ = true;
   invalidate();
@@ -170,6 +188,18 @@ int HipRenderer::SetPagedFlatVolume(const void* voxels, const uint64_t dims[3], 
   LevelPlan plan;
   if (int e = plan_levels(dims, bits, brick, apron, n_lod, &plan)) return e;
   const int bpv = (int)bits / 8;
+  uint64_t coarse_bytes = 0;
+  for (int l = 1; l < plan.n_lod; ++l) coarse_bytes += plan.dims[l][0] * plan.dims[l][1] * plan.dims[l][2] * bpv;
+  if (!initialized_) {
+    set_last_error("setPagedFlatVolume before initialize");
+    return TVK_ERR_STATE;
+  }
+  MemMan& mm = MemMan::ForDevice(device_);
+  if (!mm.ReserveCPU(coarse_bytes)) {
+    set_last_error("the CPU memory budget cannot hold the volume's coarser levels (" +
+                   std::to_string(coarse_bytes) + " bytes)");
+    return TVK_ERR_OOM;
+  }
   // coarser levels and per-brick ranges on host threads
   std::vector<std::vector<uint8_t>> levels((size_t)plan.n_lod - 1);
   std::vector<float> ranges((size_t)plan.total_bricks * 2);
@@ -184,32 +214,104 @@ int HipRenderer::SetPagedFlatVolume(const void* voxels, const uint64_t dims[3], 
                       ranges.data() + 2 * plan.lods[l].brick_base);
   }
   if (int e = SetPagedVolume(dims, bits, brick, apron, (uint32_t)plan.n_lod, &HipRenderer::flat_source,
-                             this, ranges.data(), pool_bytes))
+                             this, ranges.data(), pool_bytes)) {
+    mm.ReleaseCPU(coarse_bytes);
     return e;
+  }
   host_level0_ = voxels;
   host_levels_ = std::move(levels);
+  host_levels_bytes_ = coarse_bytes;
   return TVK_OK;
 }
 
-int HipRenderer::upload_brick(int gid, int lod, int bx, int by, int bz, int slot, int st) {
-  TVK_HIP(hipEventSynchronize(staging_ev_[st]));  // the staging buffer's last copy is done
-  if (source_fn_(source_user_, (uint32_t)lod, (uint32_t)bx, (uint32_t)by, (uint32_t)bz, staging_[st],
-                 pool_.slot_bytes) != 0) {
-    set_last_error("brick source failed for LOD " + std::to_string(lod) + " brick " + std::to_string(gid));
-    return TVK_ERR_IO;
+void HipRenderer::PoolEvict(int gid) {
+  std::lock_guard<std::mutex> g(pool_mu_);
+  // a queued raycast may still read the slot
+  (void)hipStreamSynchronize(stream_);
+  if (gid < 0 || (size_t)gid >= pool_.page_table_h.size()) return;
+  if (pool_.page_table_h[gid]) {
+    pool_.page_table_h[gid] = 0;
+    --pool_.resident;
+    ++pool_.evicted;
   }
-  uint8_t* dst = (uint8_t*)pool_.slots.p + (uint64_t)slot * pool_.slot_bytes;
-  // the source's x-fastest brick -> device staging -> the slot's blocked
-  // layout (copies and kernels are ordered on the one stream)
-  const uint64_t n = (uint64_t)slot_edge_ * slot_edge_ * slot_edge_ * (bits_ / 8);
-  TVK_HIP(hipMemcpyAsync(upload_stage_.p, staging_[st], n, hipMemcpyHostToDevice, stream_));
-  TVK_HIP(hipEventRecord(staging_ev_[st], stream_));
-  TVK_HIP(launch_linear_to_slot(upload_stage_.p, dst, (int)bits_ / 8, slot_edge_, stream_));
-  const LodInfo& L = lods_[lod];
-  const int b[3] = {bx, by, bz};
-  TVK_HIP(launch_slot_cell_ranges(dst, (int)bits_ / 8, slot_edge_, apron_, brick_, cell_, b, L.cells,
-                                  cell_range_.as<uint32_t>() + L.cell_base, stream_));
-  ++cells_gen_;  // the cells of this brick now hold their exact ranges
+  pool_.brick_slot[gid] = -1;
+  pt_dirty_ = true;
+}
+
+int HipRenderer::sync_page_table_locked() {
+  if (!pt_dirty_ || !paged_) return TVK_OK;
+  TVK_HIP(hipMemcpyAsync(pool_.page_table.p, pool_.page_table_h.data(), pool_.page_table_h.size() * 8,
+                         hipMemcpyHostToDevice, stream_));
+  TVK_HIP(hipStreamSynchronize(stream_));  // the host copy may change once the lock is released
+  pt_dirty_ = false;
+  return TVK_OK;
+}
+
+int HipRenderer::touch_sampled() {
+  std::vector<uint32_t> touch((size_t)total_bricks_);
+  TVK_HIP(hipMemcpy(touch.data(), touch_dev_.p, touch.size() * 4, hipMemcpyDeviceToHost));
+  std::vector<int64_t> used;
+  {
+    std::lock_guard<std::mutex> g(pool_mu_);
+    for (int b = 0; b < total_bricks_; ++b)
+      if (touch[b] == frame_ && pool_.brick_slot[b] >= 0) used.push_back(pool_.brick_slot[b]);
+  }
+  if (!used.empty()) MemMan::ForDevice(device_).Touch(pool_client_, used.data(), used.size());
+  return TVK_OK;
+}
+
+int HipRenderer::upload_bricks(const int* gids, const int* lods, void* const* addrs, size_t n) {
+  MemMan& mm = MemMan::ForDevice(device_);
+  const int bpv = (int)bits_ / 8;
+  const uint64_t bb = (uint64_t)slot_edge_ * slot_edge_ * slot_edge_ * bpv;
+  const size_t R = (size_t)staging_round_;
+  const auto t0 = std::chrono::steady_clock::now();
+  std::atomic<int> failed{-1};
+  for (size_t first = 0, r = 0; first < n; first += R, ++r) {
+    const size_t cnt = std::min(R, n - first);
+    const int set = (int)(r & 1);
+    // this staging set's copies from two rounds ago are done
+    if (r >= 2) TVK_HIP(hipEventSynchronize(staging_ev_[set]));
+    mm.ParallelFor(cnt, source_threads_, [&](size_t j) {
+      const size_t i = first + j;
+      const int gid = gids[i], lod = lods[i];
+      void* buf = staging_[(size_t)set * R + j];
+      if (cache_bricks_ && mm.CacheGet(pool_client_, gid, buf, bb)) return;
+      const LodInfo& L = lods_[lod];
+      const int local = gid - L.brick_base;
+      const int bx = local % L.grid[0], by = (local / L.grid[0]) % L.grid[1],
+                bz = local / (L.grid[0] * L.grid[1]);
+      if (source_fn_(source_user_, (uint32_t)lod, (uint32_t)bx, (uint32_t)by, (uint32_t)bz, buf, bb) != 0) {
+        failed = gid;
+        return;
+      }
+      if (cache_bricks_) mm.CachePut(pool_client_, gid, buf, bb);
+    });
+    if (failed >= 0) {
+      set_last_error("brick source failed for brick " + std::to_string(failed.load()));
+      return TVK_ERR_IO;
+    }
+    // x-fastest brick -> device staging -> the slot's blocked layout, and
+    // the brick's exact empty-space cell ranges, in copy-stream order
+    for (size_t j = 0; j < cnt; ++j) {
+      const size_t i = first + j;
+      const LodInfo& L = lods_[lods[i]];
+      const int local = gids[i] - L.brick_base;
+      const int b[3] = {local % L.grid[0], (local / L.grid[0]) % L.grid[1], local / (L.grid[0] * L.grid[1])};
+      TVK_HIP(hipMemcpyAsync(upload_stage_[j].p, staging_[(size_t)set * R + j], bb, hipMemcpyHostToDevice,
+                             copy_stream_));
+      TVK_HIP(launch_linear_to_slot(upload_stage_[j].p, addrs[i], bpv, slot_edge_, copy_stream_));
+      TVK_HIP(launch_slot_cell_ranges(addrs[i], bpv, slot_edge_, apron_, brick_, cell_, b, L.cells,
+                                      cell_range_.as<uint32_t>() + L.cell_base, copy_stream_));
+    }
+    TVK_HIP(hipEventRecord(staging_ev_[set], copy_stream_));
+  }
+  // the render stream runs nothing that reads these slots before they land
+  TVK_HIP(hipEventRecord(upload_done_, copy_stream_));
+  TVK_HIP(hipStreamWaitEvent(stream_, upload_done_, 0));
+  paging_bytes_ += (uint64_t)n * bb;
+  paging_ms_ += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  ++cells_gen_;  // the cells of these bricks now hold their exact ranges
   return TVK_OK;
 }
 
@@ -220,6 +322,7 @@ int HipRenderer::service_requests(int lod, bool* complete, bool* overflow) {
 int HipRenderer::service_requests(int lod_lo, int lod_hi, bool* complete, bool* overflow) {
   *complete = true;
   *overflow = false;
+  if (int e = touch_sampled()) return e;
   std::vector<int> want, want_lod;
   for (int l = lod_lo; l <= lod_hi; ++l) {
     const LodInfo& L = lods_[l];
@@ -232,46 +335,31 @@ int HipRenderer::service_requests(int lod_lo, int lod_hi, bool* complete, bool* 
   }
   if (want.empty()) return TVK_OK;
   *complete = false;
-  // which resident bricks did this frame sample?
-  std::vector<uint32_t> touch((size_t)total_bricks_);
-  TVK_HIP(hipMemcpy(touch.data(), touch_dev_.p, touch.size() * 4, hipMemcpyDeviceToHost));
-  std::vector<int> cand;  // free slots, then LRU slots not used by this frame
-  std::vector<int> used;
-  for (uint64_t s = 0; s < pool_.n_slots; ++s) {
-    const int owner = pool_.slot_owner[s];
-    if (owner < 0) cand.push_back((int)s);
-    else {
-      pool_.slot_touch[s] = std::max(pool_.slot_touch[s], touch[owner]);
-      if (pool_.slot_touch[s] < frame_) used.push_back((int)s);
+  MemMan& mm = MemMan::ForDevice(device_);
+  std::vector<int64_t> slots(want.size());
+  std::vector<void*> addrs(want.size());
+  // may take back this volume's least recent bricks (PoolEvict) or others'
+  const size_t got = mm.Acquire(pool_client_, want.data(), want.size(), slots.data(), addrs.data());
+  if (got < want.size()) *overflow = true;
+  int rc = upload_bricks(want.data(), want_lod.data(), addrs.data(), got);
+  if (rc == TVK_OK) {
+    std::lock_guard<std::mutex> g(pool_mu_);
+    for (size_t i = 0; i < got; ++i) {
+      pool_.page_table_h[want[i]] = (uint64_t)(uintptr_t)addrs[i];
+      pool_.brick_slot[want[i]] = slots[i];
     }
+    pool_.resident += got;
+    pool_.uploaded += got;
+    pt_dirty_ = true;
+    // after the copy stream's event (upload_bricks made stream_ wait on it)
+    rc = sync_page_table_locked();
+  } else {
+    // queued copies may still write the slots: let them land first
+    (void)hipStreamSynchronize(copy_stream_);
+    for (size_t i = 0; i < got; ++i) mm.Free(pool_client_, slots[i]);
   }
-  std::stable_sort(used.begin(), used.end(),
-                   [&](int a, int b) { return pool_.slot_touch[a] < pool_.slot_touch[b]; });
-  cand.insert(cand.end(), used.begin(), used.end());
-  if (want.size() > cand.size()) *overflow = true;
-  const size_t n_up = std::min(want.size(), cand.size());
-  for (size_t i = 0; i < n_up; ++i) {
-    const int gid = want[i], s = cand[i], lod = want_lod[i];
-    const LodInfo& L = lods_[lod];
-    const int local = gid - L.brick_base;
-    const int bx = local % L.grid[0], by = (local / L.grid[0]) % L.grid[1],
-              bz = local / (L.grid[0] * L.grid[1]);
-    if (pool_.slot_owner[s] >= 0) {
-      pool_.page_table_h[pool_.slot_owner[s]] = -1;
-      ++pool_.evicted;
-      --pool_.resident;
-    }
-    if (int e = upload_brick(gid, lod, bx, by, bz, s, (int)(i % kStaging))) return e;
-    pool_.page_table_h[gid] = s;
-    pool_.slot_owner[s] = gid;
-    pool_.slot_touch[s] = frame_;
-    ++pool_.uploaded;
-    ++pool_.resident;
-  }
-  TVK_HIP(hipMemcpyAsync(pool_.page_table.p, pool_.page_table_h.data(), pool_.page_table_h.size() * 4,
-                         hipMemcpyHostToDevice, stream_));
-  TVK_HIP(hipStreamSynchronize(stream_));
-  return TVK_OK;
+  if (rc == TVK_OK) mm.Commit(pool_client_, slots.data(), got);
+  return rc;
 }
 
 }  // namespace tvk

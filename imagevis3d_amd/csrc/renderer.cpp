@@ -1,5 +1,6 @@
 // renderer.cpp -- HipRenderer (see renderer.hpp).
 #include "renderer.hpp"
+#include "memman.hpp"
 
 #include <math.h>
 #include <stddef.h>
@@ -37,14 +38,26 @@ static int state_fail(const std::string& msg) {
 hipError_t DevBuf::alloc(size_t n) {
   reset();
   if (n == 0) return hipSuccess;
+  int dev = 0;
+  if (hipError_t e = hipGetDevice(&dev)) return e;
+  MemMan& mm = MemMan::ForDevice(dev);
+  if (!mm.ReserveGPU(n)) return hipErrorOutOfMemory;  // beyond the device's budget
   hipError_t e = hipMalloc(&p, n);
-  if (e == hipSuccess) bytes = n;
-  else p = nullptr;
+  if (e == hipSuccess) {
+    bytes = n;
+    device = dev;
+  } else {
+    p = nullptr;
+    mm.ReleaseGPU(n);
+  }
   return e;
 }
 
 void DevBuf::reset() {
-  if (p) (void)hipFree(p);
+  if (p) {
+    (void)hipFree(p);
+    MemMan::ForDevice(device).ReleaseGPU(bytes);
+  }
   p = nullptr;
   bytes = 0;
 }
@@ -215,11 +228,11 @@ HipRenderer::~HipRenderer() {
     for (auto& sl : paint_slots_) sl.release();
     batch_slot_.release();
     batch_fb_.reset();
+    release_volume();  // gives a paged volume's slots back to the device pool
     if (requests_h_) (void)hipHostFree(requests_h_);
-    for (int i = 0; i < kStaging; ++i) {
-      if (staging_[i]) (void)hipHostFree(staging_[i]);
-      if (staging_ev_[i]) (void)hipEventDestroy(staging_ev_[i]);
-    }
+    for (hipEvent_t ev : staging_ev_)
+      if (ev) (void)hipEventDestroy(ev);
+    if (upload_done_) (void)hipEventDestroy(upload_done_);
     requests_dev_.reset();
     touch_dev_.reset();
     tfs_.clear();
@@ -230,6 +243,8 @@ HipRenderer::~HipRenderer() {
     scratch_.reset();
     cell_range_.reset();
     framebuffer_.reset();
+    lowres_fb_.reset();
+    if (copy_stream_) (void)hipStreamDestroy(copy_stream_);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
 }
@@ -242,6 +257,7 @@ int HipRenderer::Initialize(int device) {
   device_ = device;
   TVK_HIP(hipSetDevice(device_));
   TVK_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  TVK_HIP(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking));
   for (auto& sl : paint_slots_)
     if (int e = sl.alloc()) return e;
   if (int e = batch_slot_.alloc()) return e;
@@ -249,7 +265,6 @@ int HipRenderer::Initialize(int device) {
   size_t free_b = 0, total_b = 0;
   TVK_HIP(hipMemGetInfo(&free_b, &total_b));
   gpu_total_ = total_b;
-  if (gpu_budget_ == 0) gpu_budget_ = total_b;
   initialized_ = true;
   return TVK_OK;
 }
@@ -365,14 +380,26 @@ void HipRenderer::release_volume() {
   for (double& t : lod_ms_) t = -1.0;  // the governor's timings were for the old volume
   time_pending_ = false;
   for (auto& l : levels_) l.reset();
+  if (pool_client_ >= 0) {
+    // the copy stream may still fill this volume's slots
+    (void)hipStreamSynchronize(copy_stream_);
+    MemMan::ForDevice(device_).Unregister(pool_client_);
+    pool_client_ = -1;
+  }
+  MemMan& mm = MemMan::ForDevice(device_);
+  for (void* b : staging_) (void)hipHostFree(b);
+  staging_.clear();
+  mm.ReleaseCPU(staging_host_bytes_ + host_levels_bytes_);
+  staging_host_bytes_ = host_levels_bytes_ = 0;
+  upload_stage_.clear();
+  pt_dirty_ = false;
   pool_.slots.reset();
-  upload_stage_.reset();
   pool_.page_table.reset();
   pool_.brick_range.reset();
   pool_.n_slots = pool_.slot_bytes = pool_.resident = 0;
   pool_.page_table_h.clear();
   pool_.brick_range_h.clear();
-  pool_.slot_owner.clear();
+  pool_.brick_slot.clear();
   cell_range_.reset();
   requests_dev_.reset();
   touch_dev_.reset();
@@ -410,9 +437,10 @@ int HipRenderer::set_volume_impl(const void* voxels, int gen_kind, uint64_t seed
   const uint64_t need = level_bytes + (uint64_t)total * slot_bytes + (uint64_t)total * 12 +
                         (uint64_t)total_cells * 4 +
                         (uint64_t)total * sizeof(BrickJob) + (size_t)width_ * height_ * 4;
-  if (need > gpu_budget_) {
+  const uint64_t budget = MemMan::ForDevice(device_).GPUBudget();
+  if (need > budget) {
     set_last_error("volume needs " + std::to_string(need) + " bytes of device memory, budget is " +
-                   std::to_string(gpu_budget_));
+                   std::to_string(budget));
     return TVK_ERR_OOM;
   }
 
@@ -458,7 +486,7 @@ int HipRenderer::brick_levels(const LevelPlan& plan) {
   const int brick = plan.brick, apron = apron_;
   const LodInfo* lods = plan.lods;
   TVK_HIP(pool_.slots.alloc((size_t)total * slot_bytes));
-  TVK_HIP(pool_.page_table.alloc((size_t)total * sizeof(int32_t)));
+  TVK_HIP(pool_.page_table.alloc((size_t)total * sizeof(uint64_t)));
   TVK_HIP(pool_.brick_range.alloc((size_t)total * 2 * sizeof(float)));
   pool_.n_slots = (uint64_t)total;
   pool_.slot_bytes = slot_bytes;
@@ -492,8 +520,9 @@ int HipRenderer::brick_levels(const LevelPlan& plan) {
                                cell_range_.as<uint32_t>() + lods[l].cell_base, stream_));
   }
   pool_.page_table_h.resize(total);
-  for (int i = 0; i < total; ++i) pool_.page_table_h[i] = i;
-  TVK_HIP(hipMemcpyAsync(pool_.page_table.p, pool_.page_table_h.data(), total * sizeof(int32_t),
+  for (int i = 0; i < total; ++i)
+    pool_.page_table_h[i] = (uint64_t)(uintptr_t)((uint8_t*)pool_.slots.p + (uint64_t)i * slot_bytes);
+  TVK_HIP(hipMemcpyAsync(pool_.page_table.p, pool_.page_table_h.data(), total * sizeof(uint64_t),
                          hipMemcpyHostToDevice, stream_));
   pool_.brick_range_h.resize((size_t)total * 2);
   TVK_HIP(hipMemcpyAsync(pool_.brick_range_h.data(), pool_.brick_range.p, total * 2 * sizeof(float),
@@ -675,8 +704,13 @@ int HipRenderer::ReadBrick(uint32_t lod, uint32_t bx, uint32_t by, uint32_t bz, 
   const uint64_t n = (uint64_t)slot_edge_ * slot_edge_ * slot_edge_ * (bits_ / 8);
   if (!out || bytes < n) return arg_fail("output buffer too small");
   const int gid = L.brick_base + ((int)bz * L.grid[1] + (int)by) * L.grid[0] + (int)bx;
-  const int slot = pool_.page_table_h[gid];
-  if (slot < 0) {
+  TVK_HIP(hipSetDevice(device_));
+  DevBuf lin;
+  TVK_HIP(lin.alloc(n));
+  std::unique_lock<std::mutex> lk(pool_mu_);  // the slot stays this brick's while it is read
+  const uint64_t addr = pool_.page_table_h[gid];
+  if (!addr) {
+    lk.unlock();
     if (!paged_) return state_fail("brick is not resident");
     // not resident: the brick source produces the same slot contents
     std::vector<uint8_t> tmp(pool_.slot_bytes);
@@ -687,12 +721,8 @@ int HipRenderer::ReadBrick(uint32_t lod, uint32_t bx, uint32_t by, uint32_t bz, 
     memcpy(out, tmp.data(), n);
     return TVK_OK;
   }
-  TVK_HIP(hipSetDevice(device_));
   // the slot's blocked layout back to an x-fastest brick, then to the host
-  DevBuf lin;
-  TVK_HIP(lin.alloc(n));
-  TVK_HIP(launch_slot_to_linear((const uint8_t*)pool_.slots.p + (uint64_t)slot * pool_.slot_bytes, lin.p,
-                                (int)bits_ / 8, slot_edge_, stream_));
+  TVK_HIP(launch_slot_to_linear((const void*)(uintptr_t)addr, lin.p, (int)bits_ / 8, slot_edge_, stream_));
   TVK_HIP(hipMemcpyAsync(out, lin.p, n, hipMemcpyDeviceToHost, stream_));
   TVK_HIP(hipStreamSynchronize(stream_));
   return TVK_OK;
@@ -1365,7 +1395,6 @@ void HipRenderer::build_params(RaycastParams* p, int lod, const CameraState& cam
   p->brick = brick_;
   p->apron = apron_;
   p->slot_edge = slot_edge_;
-  p->slot_elems = (int64_t)(pool_.slot_bytes / (bits_ / 8));
   p->n_tf = (int32_t)(tfs.tf1d.size() / 4);
   p->tf_scale = tf_scale(p->n_tf);
   for (int i = 0; i < 3; ++i) {
@@ -1373,8 +1402,7 @@ void HipRenderer::build_params(RaycastParams* p, int lod, const CameraState& cam
     p->bg_bottom[i] = bg_bottom_[i];
   }
   p->bg_alpha = bg_alpha_;
-  p->pool = pool_.slots.p;
-  p->page_table = pool_.page_table.as<int32_t>();
+  p->page_table = pool_.page_table.as<uint64_t>();
   p->brick_range = pool_.brick_range.as<float>();
   p->cell_range = cell_range_.as<uint32_t>();
   p->cell_empty = tfs.cell_empty[mode_index()].as<uint8_t>();
@@ -1632,9 +1660,16 @@ int HipRenderer::Paint() {
     TVK_HIP(launch_fill_u64(requests_dev_.as<unsigned long long>(), (int64_t)(requests_dev_.bytes / 8),
                             0ull, stream_));
   }
+  // a paged volume's bricks sampled from here on are this frame's (a pass
+  // that repeats an incomplete one keeps the frame going)
+  if (paged_ && !(last_pass_incomplete_ && lod == last_lod_)) MemMan::ForDevice(device_).BeginFrame(pool_client_);
   LaunchSlot& sl = paint_slots_[paint_cur_];
   paint_cur_ ^= 1;
-  if (int e = launch_jobs(sl, &p, 1, launch_info(p.n_tf), false)) return e;
+  {
+    std::lock_guard<std::mutex> g(pool_mu_);  // no eviction between the page table and the launch
+    if (int e = sync_page_table_locked()) return e;
+    if (int e = launch_jobs(sl, &p, 1, launch_info(p.n_tf), false)) return e;
+  }
   if (lowres)
     TVK_HIP(launch_upscale_nearest(lowres_fb_.as<uint32_t>(), (int)rw, (int)rh, framebuffer_.as<uint32_t>(),
                                    (int)width_, (int)height_, stream_));
@@ -1676,6 +1711,7 @@ int HipRenderer::Paint() {
   stats_.height = height_;
   stats_pending_ = true;
   last_lod_ = lod;
+  last_pass_incomplete_ = !pass_complete;
   ++subframes_;
   if (!pass_complete) {
     // bricks were paged in: render this LOD again (a coarser one after an
@@ -1730,6 +1766,7 @@ int HipRenderer::RenderBatch(const tvk_view* views, uint32_t n, tvk_view_stats* 
     int lod = lo;
     if (paged_) lod = std::max(lod, std::min(lod_floor_, hi));
     std::vector<RaycastParams> jobs(cnt);
+    int batch_lod = -1;
     // paged volumes: a pass per set of requested bricks, as in a capture
     for (int pass = 0; pass < 4 * kMaxLod + 8; ++pass) {
       for (int k = 0; k < cnt; ++k) {
@@ -1749,10 +1786,16 @@ int HipRenderer::RenderBatch(const tvk_view* views, uint32_t n, tvk_view_stats* 
       }
       if (paged_) {
         ++frame_;
+        if (pass == 0 || lod != batch_lod) MemMan::ForDevice(device_).BeginFrame(pool_client_);
+        batch_lod = lod;
         TVK_HIP(launch_fill_u64(requests_dev_.as<unsigned long long>(),
                                 (int64_t)(requests_dev_.bytes / 8), 0ull, stream_));
       }
-      if (int e = launch_jobs(batch_slot_, jobs.data(), cnt, li, use_graphs_)) return e;
+      {
+        std::lock_guard<std::mutex> g(pool_mu_);  // no eviction between the page table and the launch
+        if (int e = sync_page_table_locked()) return e;
+        if (int e = launch_jobs(batch_slot_, jobs.data(), cnt, li, use_graphs_)) return e;
+      }
       if (paged_)
         TVK_HIP(hipMemcpyAsync(requests_h_, requests_dev_.p, requests_dev_.bytes,
                                hipMemcpyDeviceToHost, stream_));
@@ -1839,6 +1882,35 @@ int HipRenderer::ServePrepare(uint32_t tf_slot, int mode) {
   if (int e2 = prepare_cells((int)tf_slot, mode == TVK_RM_2DTRANS ? 1 : 0)) return e2;
   TVK_HIP(hipStreamSynchronize(stream_));
   return TVK_OK;
+}
+
+int HipRenderer::ServeCheck(const tvk_view& v, int mode) {
+  if (!initialized_ || !has_volume_) return state_fail("serving needs an initialized renderer with a volume");
+  if (mode != TVK_RM_1DTRANS && mode != TVK_RM_2DTRANS) return arg_fail("served render mode must be 1D or 2D TF");
+  CameraState cam;
+  if (int e = view_camera(v, 0, &cam)) return e;
+  const int saved = render_mode_;
+  render_mode_ = mode;
+  const int e = check_tf((int)v.tf_slot);
+  render_mode_ = saved;
+  return e;
+}
+
+int HipRenderer::ServeBatch(const tvk_view* views, uint32_t n, int mode, tvk_view_stats* stats) {
+  const int saved = render_mode_;
+  render_mode_ = mode;
+  float kms = 0.f;
+  const int e = RenderBatch(views, n, stats, &kms);
+  render_mode_ = saved;
+  return e;
+}
+
+uint64_t HipRenderer::PagedWorkingSetBytes() const {
+  if (!paged_) return 0;
+  const int lo = std::min<int>((int)lod_min_, n_lod_ - 1);
+  const LodInfo& L = lods_[lo];
+  const uint64_t bricks = (uint64_t)L.grid[0] * L.grid[1] * L.grid[2];
+  return std::min<uint64_t>(bricks, pool_.n_slots) * pool_.slot_bytes;
 }
 
 int HipRenderer::ServeJob(const tvk_view& v, int mode, uint32_t* fb, RaycastParams* out,
@@ -1951,7 +2023,21 @@ int HipRenderer::Synchronize() {
 // ----------------------------------------------------------------- memory
 int HipRenderer::SetMaxUsableGPUMem(uint64_t bytes) {
   if (bytes == 0) return arg_fail("GPU memory budget must be > 0");
-  gpu_budget_ = bytes;
+  if (!initialized_) return state_fail("the memory budget belongs to the device: initialize first");
+  MemMan::ForDevice(device_).SetGPUBudget(bytes);
+  return TVK_OK;
+}
+
+int HipRenderer::SetMaxUsableCPUMem(uint64_t bytes) {
+  if (bytes == 0) return arg_fail("CPU memory budget must be > 0");
+  if (!initialized_) return state_fail("the memory budget belongs to the device: initialize first");
+  MemMan::ForDevice(device_).SetCPUBudget(bytes);
+  return TVK_OK;
+}
+
+int HipRenderer::SetBrickSourceThreads(uint32_t n) {
+  if (n < 1 || n > 32) return arg_fail("brick source threads must be in [1, 32]");
+  source_threads_ = (int)n;
   return TVK_OK;
 }
 
@@ -1959,7 +2045,7 @@ int HipRenderer::MemStats(tvk_mem_stats* out) {
   if (!out) return arg_fail("null stats pointer");
   *out = tvk_mem_stats{};
   out->gpu_mem = gpu_total_;
-  out->gpu_budget = gpu_budget_;
+  out->gpu_budget = initialized_ ? MemMan::ForDevice(device_).GPUBudget() : 0;
   uint64_t a = framebuffer_.bytes + batch_fb_.bytes + pool_.slots.bytes + pool_.page_table.bytes + pool_.brick_range.bytes + scratch_.bytes +
                cell_range_.bytes;
   for (auto& l : levels_) a += l.bytes;
@@ -1969,7 +2055,9 @@ int HipRenderer::MemStats(tvk_mem_stats* out) {
   }
   for (const LaunchSlot* sl : {&paint_slots_[0], &paint_slots_[1], &batch_slot_})
     a += sl->table.bytes + sl->counters.bytes;
-  a += requests_dev_.bytes + touch_dev_.bytes;
+  a += requests_dev_.bytes + touch_dev_.bytes + lowres_fb_.bytes + region_out_dev_.bytes;
+  for (const DevBuf& d : upload_stage_) a += d.bytes;
+  if (pool_client_ >= 0) a += MemMan::ForDevice(device_).ClientSlots(pool_client_) * pool_.slot_bytes;
   out->allocated_gpu_mem = a;
   out->pool_slots = pool_.n_slots;
   out->resident_bricks = pool_.resident;

@@ -12,20 +12,26 @@
 #include <string>
 #include <vector>
 
+#include <mutex>
+
 #include "../../include/tvk_hip.h"
 #include "kernels.hpp"
+#include "memman.hpp"
 #include "tvk_common.hpp"
 
 namespace tvk {
 
-// Owning device allocation.
+// Owning device allocation, accounted against its device's memory budget
+// (MemMan): an allocation beyond the budget first frees paged-pool chunks
+// and fails with hipErrorOutOfMemory when that is not enough.
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  int device = -1;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), device(o.device) {
     o.p = nullptr;
     o.bytes = 0;
   }
@@ -34,6 +40,7 @@ struct DevBuf {
       reset();
       p = o.p;
       bytes = o.bytes;
+      device = o.device;
       o.p = nullptr;
       o.bytes = 0;
     }
@@ -48,20 +55,22 @@ struct DevBuf {
   }
 };
 
-// Device brick pool + page table + per-brick value ranges (GPUMemMan).
+// A volume's bricks on the device: in-core volumes own one allocation of
+// slots (every brick resident); paged volumes hold slots of the device-wide
+// pool (MemMan), at most n_slots of them.  The page table holds each
+// brick's slot address (0: not resident).
 struct BrickPool {
-  DevBuf slots;            // n_slots * slot_bytes
-  DevBuf page_table;       // int32 per brick
+  DevBuf slots;            // in-core: n_slots * slot_bytes
+  DevBuf page_table;       // uint64 per brick
   DevBuf brick_range;      // float2 per brick
   uint64_t n_slots = 0;
   uint64_t slot_bytes = 0;
   uint64_t resident = 0;
   uint64_t uploaded = 0;
   uint64_t evicted = 0;
-  std::vector<int32_t> page_table_h;
+  std::vector<uint64_t> page_table_h;
   std::vector<float> brick_range_h;
-  std::vector<int32_t> slot_owner;   // paged pools: brick id held by each slot, -1 free
-  std::vector<uint32_t> slot_touch;  // paged pools: last frame the slot's brick was sampled
+  std::vector<int64_t> brick_slot;  // paged: the brick's slot id in the device pool, -1
 };
 
 // Geometry of a bricked LOD pyramid (RasterDataBlock layout).
@@ -128,7 +137,7 @@ int launch_job_table(LaunchSlot& sl, const RaycastParams* jobs, int n, const Lau
 typedef int (*BrickSourceFn)(void* user, uint32_t lod, uint32_t bx, uint32_t by, uint32_t bz,
                              void* slot, uint64_t slot_bytes);
 
-class HipRenderer {
+class HipRenderer : public PoolClient {
  public:
   HipRenderer(int type, bool pow2, bool to8, bool no_border, bool bias_scale_tf);
   ~HipRenderer();
@@ -252,6 +261,14 @@ class HipRenderer {
   int ServePrepare(uint32_t tf_slot, int mode);
   int ServeJob(const tvk_view& v, int mode, uint32_t* fb, RaycastParams* out, LaunchInfo* li);
   int Device() const { return device_; }
+  // Serving paged volumes: ServeCheck validates a request (view, TF slot,
+  // mode); ServeBatch renders views in `mode` with the paging loop of
+  // RenderBatch; PagedWorkingSetBytes is the pool memory a batch may hold
+  // (the finest allowed LOD's bricks, at most the volume's share).
+  bool Paged() const { return paged_; }
+  int ServeCheck(const tvk_view& v, int mode);
+  int ServeBatch(const tvk_view* views, uint32_t n, int mode, tvk_view_stats* stats);
+  uint64_t PagedWorkingSetBytes() const;
   int CheckForRedraw() const { return frame_complete_ ? 0 : 1; }
   int ScheduleCompleteRedraw();
   int Capture(uint8_t* rgba, uint64_t bytes, bool preserve_transparency);
@@ -261,8 +278,18 @@ class HipRenderer {
   int DeviceFramebuffer(void** dptr, void** stream);
   int Synchronize();
 
+  // budgets of this renderer's device (shared by every renderer there)
   int SetMaxUsableGPUMem(uint64_t bytes);
+  int SetMaxUsableCPUMem(uint64_t bytes);
   int MemStats(tvk_mem_stats* out);
+  // threads that run a paged volume's brick source (1: one at a time)
+  int SetBrickSourceThreads(uint32_t n);
+  void PagingStats(uint64_t* bytes, double* ms) const {
+    *bytes = paging_bytes_;
+    *ms = paging_ms_;
+  }
+  // the device-wide pool takes back brick gid's slot (MemMan callback)
+  void PoolEvict(int gid) override;
 
  private:
   int check_ready() const;
@@ -302,7 +329,13 @@ class HipRenderer {
   // the bricks of this LOD for this view
   int service_requests(int lod, bool* complete, bool* overflow);
   int service_requests(int lod_lo, int lod_hi, bool* complete, bool* overflow);
-  int upload_brick(int gid, int lod, int bx, int by, int bz, int slot, int staging);
+  // fill bricks gids[0..n) from the source (or the host cache) on worker
+  // threads and stream them into their slots on the copy stream
+  int upload_bricks(const int* gids, const int* lods, void* const* addrs, size_t n);
+  // the page table to the device if evictions changed it (pool_mu_ held)
+  int sync_page_table_locked();
+  // the LRU stamps of the bricks the last pass sampled
+  int touch_sampled();
   static int flat_source(void* user, uint32_t lod, uint32_t bx, uint32_t by, uint32_t bz,
                          void* slot, uint64_t slot_bytes);
   void invalidate() {
@@ -361,10 +394,24 @@ class HipRenderer {
   DevBuf requests_dev_, touch_dev_;               // u32 per brick
   uint32_t* requests_h_ = nullptr;                // pinned
   uint64_t requests_h_count_ = 0;
-  static constexpr int kStaging = 4;
-  void* staging_[kStaging] = {nullptr, nullptr, nullptr, nullptr};  // pinned slots
-  hipEvent_t staging_ev_[kStaging] = {nullptr, nullptr, nullptr, nullptr};
-  DevBuf upload_stage_;  // one x-fastest brick on its way into a slot
+  // upload pipeline: two sets of pinned staging buffers (one filled by the
+  // source threads while the other's copies run), device staging buffers
+  // the copies land in before the layout kernel moves them into slots
+  std::vector<void*> staging_;
+  hipEvent_t staging_ev_[2] = {nullptr, nullptr};
+  std::vector<DevBuf> upload_stage_;
+  int staging_round_ = 0;              // bricks per round
+  uint64_t staging_host_bytes_ = 0;    // accounted as host memory
+  hipStream_t copy_stream_ = nullptr;  // brick uploads
+  hipEvent_t upload_done_ = nullptr;
+  int pool_client_ = -1;               // MemMan client id of a paged volume
+  std::mutex pool_mu_;                 // page table vs evictions from other threads
+  bool pt_dirty_ = false;              // page_table_h changed since its upload
+  bool cache_bricks_ = false;          // keep source bricks in the host cache
+  int source_threads_ = 8;
+  uint64_t host_levels_bytes_ = 0;     // host copies of paged levels (accounted)
+  uint64_t paging_bytes_ = 0;          // bricks uploaded (bytes) since create
+  double paging_ms_ = 0.0;             // host time spent paging them
   uint32_t frame_ = 0;
   int lod_floor_ = 0;       // coarsest-needed fallback when the pool overflows
   std::vector<uint32_t> cell_init_h_;  // brick-range initial cell ranges
@@ -422,6 +469,7 @@ class HipRenderer {
   double last_scale_ = 1.0;            // pixel x rate fraction of the last subframe
   bool time_pending_ = false;          // the last subframe's time is not recorded yet
   bool capturing_ = false;             // inside Capture: no LOD delay
+  bool last_pass_incomplete_ = false;  // the last paint paged bricks in: its LOD repeats
   DevBuf lowres_fb_;                   // reduced-resolution intermediate subframes
 
   // progress

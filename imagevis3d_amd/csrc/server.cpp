@@ -9,6 +9,7 @@
 
 #include "errors.hpp"
 #include "kernels.hpp"
+#include "memman.hpp"
 
 namespace tvk {
 
@@ -73,6 +74,11 @@ RenderServer::~RenderServer() {
   }
   cv_work_.notify_all();
   if (worker_.joinable()) worker_.join();
+  for (auto& kv : paged_) {
+    kv.second->cv.notify_all();
+    pool_cv_.notify_all();
+    if (kv.second->worker.joinable()) kv.second->worker.join();
+  }
   if (device_ >= 0) {
     (void)hipSetDevice(device_);
     for (Lane& L : lanes_) {
@@ -116,7 +122,105 @@ int RenderServer::AddVolume(HipRenderer* r, uint32_t* id) {
   std::lock_guard<std::mutex> lk(mu_);
   volumes_.push_back(r);
   *id = (uint32_t)(volumes_.size() - 1);
+  if (r->Paged()) {
+    auto pv = std::make_unique<PagedVolume>();
+    pv->id = *id;
+    pv->need = r->PagedWorkingSetBytes();
+    PagedVolume* p = pv.get();
+    paged_[*id] = std::move(pv);
+    p->worker = std::thread([this, p] { paged_loop(p); });
+  }
   return TVK_OK;
+}
+
+uint64_t RenderServer::pool_capacity() const {
+  MemManStats m;
+  MemMan::ForDevice(device_).Stats(&m);
+  const uint64_t outside = m.gpu_used > m.pool_bytes ? m.gpu_used - m.pool_bytes : 0;
+  return m.gpu_budget > outside ? m.gpu_budget - outside : 0;
+}
+
+// Batches of one paged volume's requests, one render mode each, admitted
+// by brick-pool memory and rendered with the renderer's paging loop.
+void RenderServer::paged_loop(PagedVolume* pv) {
+  (void)hipSetDevice(device_);
+  HipRenderer* r = volumes_[pv->id];
+  std::unique_lock<std::mutex> lk(mu_);
+  while (true) {
+    pv->cv.wait(lk, [&] { return stop_ || !pv->pending.empty(); });
+    if (stop_) break;
+    const int mode = reqs_[pv->pending.front()].q.render_mode;
+    std::vector<uint64_t> pick;
+    for (auto it = pv->pending.begin(); it != pv->pending.end() && (int)pick.size() < kMaxJobs;) {
+      Request& q = reqs_[*it];
+      if (q.q.render_mode != mode) {
+        ++it;
+        continue;
+      }
+      uint64_t hoff = 0;
+      if (!host_arena_.alloc(q.bytes, &hoff)) break;  // output memory, in order
+      q.host_off = hoff;
+      q.host_held = true;
+      pick.push_back(*it);
+      it = pv->pending.erase(it);
+    }
+    if (pick.empty()) {
+      // host output memory is held by finished requests not collected yet
+      cv_done_.notify_all();
+      pv->cv.wait_for(lk, std::chrono::milliseconds(1));
+      continue;
+    }
+    // admission by brick-pool memory: this batch's working set beside the
+    // paged batches in flight (one batch alone always runs)
+    bool waited = false;
+    while (!stop_ && pool_reserved_ > 0 && pool_reserved_ + pv->need > pool_capacity()) {
+      waited = true;
+      pool_cv_.wait(lk);
+    }
+    if (waited) ++stats_.pool_waits;
+    pool_reserved_ += pv->need;
+    stats_.max_pool_reserved = std::max(stats_.max_pool_reserved, pool_reserved_);
+    const double t = now_ms();
+    std::vector<tvk_view> views(pick.size());
+    for (size_t k = 0; k < pick.size(); ++k) {
+      Request& q = reqs_[pick[k]];
+      q.t_launch = t;
+      q.launch_jobs = (uint32_t)pick.size();
+      views[k] = q.q.view;
+      views[k].rgba = host_arena_mem_ + q.host_off;
+    }
+    lk.unlock();
+    std::vector<tvk_view_stats> vs(pick.size());
+    const int rc = r->ServeBatch(views.data(), (uint32_t)views.size(), mode, vs.data());
+    lk.lock();
+    pool_reserved_ -= pv->need;
+    pool_cv_.notify_all();
+    const double td = now_ms();
+    for (size_t k = 0; k < pick.size(); ++k) {
+      Request& q = reqs_[pick[k]];
+      q.done = true;
+      q.t_done = td;
+      q.status = rc;
+      if (rc == TVK_OK) {
+        q.evaluated = vs[k].evaluated_samples;
+        q.nominal = vs[k].nominal_samples;
+        stats_.nominal_samples += q.nominal;
+      }
+      ++stats_.completed;
+    }
+    ++stats_.launches;
+    ++stats_.paged_launches;
+    stats_.max_jobs_per_launch = std::max<uint64_t>(stats_.max_jobs_per_launch, pick.size());
+    cv_done_.notify_all();
+  }
+  // requests never rendered fail with a state error
+  for (uint64_t t : pv->pending) {
+    Request& q = reqs_[t];
+    q.status = TVK_ERR_STATE;
+    q.done = true;
+  }
+  pv->pending.clear();
+  cv_done_.notify_all();
 }
 
 uint64_t RenderServer::variant_key(const LaunchInfo& li) {
@@ -130,6 +234,24 @@ int RenderServer::Submit(const tvk_request& q, uint64_t* ticket) {
   if (stop_) return fail(TVK_ERR_STATE, "server is stopping");
   if (q.volume >= volumes_.size()) return fail(TVK_ERR_ARG, "unknown volume id");
   HipRenderer* r = volumes_[q.volume];
+  if (r->Paged()) {
+    if (int e = r->ServeCheck(q.view, q.render_mode)) return e;
+    Request req;
+    req.q = q;
+    req.paged = true;
+    req.bytes = (uint64_t)q.view.width * q.view.height * 4;
+    if (req.bytes > host_arena_.size()) return fail(TVK_ERR_OOM, "image larger than the output arena");
+    req.t_submit = now_ms();
+    const uint64_t t = next_ticket_++;
+    reqs_.emplace(t, req);
+    PagedVolume* pv = paged_[q.volume].get();
+    pv->pending.push_back(t);
+    ++stats_.submitted;
+    *ticket = t;
+    lk.unlock();
+    pv->cv.notify_one();
+    return TVK_OK;
+  }
   if (int e = r->ServePrepare(q.view.tf_slot, q.render_mode)) return e;
   Request req;
   req.q = q;
@@ -323,6 +445,10 @@ int RenderServer::Wait(uint64_t ticket, uint32_t timeout_ms, tvk_request_result*
     host_arena_.free(r.host_off, r.bytes);
   }
   cv_work_.notify_one();  // output memory was freed
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    for (auto& kv : paged_) kv.second->cv.notify_one();
+  }
   if (out) {
     *out = tvk_request_result{};
     out->status = r.status;

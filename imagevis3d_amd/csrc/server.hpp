@@ -67,6 +67,17 @@ class RenderServer {
     int status = TVK_OK;
     bool done = false;
     bool host_held = false;  // owns a pinned host arena block
+    bool paged = false;      // its volume pages through the device pool
+  };
+  // A paged volume: its requests are rendered in batches by a worker of its
+  // own (the renderer's multi-pass paging loop), admitted when their
+  // working set fits the device pool beside the batches already running.
+  struct PagedVolume {
+    uint32_t id = 0;
+    uint64_t need = 0;  // pool bytes a batch may hold
+    std::deque<uint64_t> pending;
+    std::condition_variable cv;
+    std::thread worker;
   };
   struct Lane {
     hipStream_t stream = nullptr;
@@ -79,12 +90,17 @@ class RenderServer {
   };
 
   void loop();
+  void paged_loop(PagedVolume* pv);
+  uint64_t pool_capacity() const;  // device pool bytes available to served batches
   bool retire_locked();  // true when a lane finished
   bool launch_locked(Lane& lane);
   static uint64_t variant_key(const LaunchInfo& li);
 
   int device_ = -1;
   std::vector<HipRenderer*> volumes_;
+  std::map<uint32_t, std::unique_ptr<PagedVolume>> paged_;
+  uint64_t pool_reserved_ = 0;  // working sets of the paged batches in flight
+  std::condition_variable pool_cv_;
   std::vector<Lane> lanes_;
   DevBuf dev_arena_mem_;
   uint8_t* host_arena_mem_ = nullptr;  // pinned

@@ -16,6 +16,7 @@
 #include "../../include/tvk_hip.h"
 #include "errors.hpp"
 #include "kernels.hpp"
+#include "memman.hpp"
 #include "renderer.hpp"
 #include "server.hpp"
 #include "toc.hpp"
@@ -949,6 +950,44 @@ int tvk_set_max_usable_gpu_mem(tvk_renderer* r, uint64_t bytes) {
 int tvk_get_mem_stats(tvk_renderer* r, tvk_mem_stats* out) {
   TVK_GUARD(r);
   return r->impl.MemStats(out);
+}
+
+int tvk_set_max_usable_cpu_mem(tvk_renderer* r, uint64_t bytes) {
+  TVK_GUARD(r);
+  return r->impl.SetMaxUsableCPUMem(bytes);
+}
+
+int tvk_get_memman_stats(int device, tvk_memman_stats* out) {
+  if (!out) {
+    tvk::set_last_error("null stats pointer");
+    return TVK_ERR_ARG;
+  }
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+    tvk::set_last_error("device ordinal out of range");
+    return TVK_ERR_ARG;
+  }
+  tvk::MemManStats m;
+  tvk::MemMan::ForDevice(device).Stats(&m);
+  *out = tvk_memman_stats{m.gpu_total, m.gpu_budget, m.gpu_used, m.cpu_total, m.cpu_budget,
+                          m.cpu_used, m.pool_bytes, m.pool_slots, m.pool_resident, m.pool_evictions,
+                          m.chunks_freed, m.cache_bytes, m.cache_hits, m.cache_misses};
+  return TVK_OK;
+}
+
+int tvk_set_brick_source_threads(tvk_renderer* r, uint32_t threads) {
+  TVK_GUARD(r);
+  return r->impl.SetBrickSourceThreads(threads);
+}
+
+int tvk_get_paging_stats(tvk_renderer* r, uint64_t* bytes, double* ms) {
+  TVK_GUARD(r);
+  if (!bytes || !ms) {
+    tvk::set_last_error("null output pointer");
+    return TVK_ERR_ARG;
+  }
+  r->impl.PagingStats(bytes, ms);
+  return TVK_OK;
 }
 
 }  // extern "C"

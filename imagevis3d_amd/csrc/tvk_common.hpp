@@ -6,8 +6,10 @@
 //                payload B^3 plus an apron of `apron` voxels on every side,
 //                S^3 voxels (S = B+2a), bpv bytes per voxel, in the blocked
 //                slot layout below.  Slots are 256-byte aligned.
-//   page table   int32 per brick of every LOD (LOD-major, then z, y, x):
-//                slot index, or -1 when the brick is not resident.
+//   page table   uint64 per brick of every LOD (LOD-major, then z, y, x):
+//                device address of the brick's slot, or 0 when the brick
+//                is not resident.  Slots live in the renderer's in-core pool
+//                or in any chunk of the device-wide paged pool.
 //   brick ranges float2 (min, max) per brick over its stored slot.
 //   cell ranges  uint32 (min | max << 16) per empty-space cell of edge C
 //                (C divides the brick size), over the voxels
@@ -118,7 +120,6 @@ struct RaycastParams {
   int32_t brick;        // payload edge B
   int32_t apron;        // apron a
   int32_t slot_edge;    // S = B + 2a
-  int64_t slot_elems;   // slot stride in voxels (S^3 rounded up to 256 bytes)
   int32_t n_tf;
   float tf_scale;
   float bg_top[3], bg_bottom[3];
@@ -145,8 +146,7 @@ struct RaycastParams {
   float cv_color[3];
   float cv_focus[2];
   float cv_size, cv_context_scale, cv_border_scale;
-  const void* pool;
-  const int32_t* page_table;
+  const uint64_t* page_table;   // per brick: device address of its slot, 0 when not resident
   const float* brick_range;     // 2 floats per brick
   const uint32_t* cell_range;   // per cell: min | max << 16 (raw values)
   const uint8_t* cell_empty;    // per cell: 1 when this job's TF is transparent over it

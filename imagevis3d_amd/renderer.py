@@ -678,10 +678,30 @@ class Renderer:
     def setMaxUsableGPUMem(self, nbytes: int) -> None:
         call("tvk_set_max_usable_gpu_mem", self._h, int(nbytes))
 
+    def setMaxUsableCPUMem(self, nbytes: int) -> None:
+        call("tvk_set_max_usable_cpu_mem", self._h, int(nbytes))
+
     def memStats(self) -> MemStats:
         s = MemStats()
         call("tvk_get_mem_stats", self._h, C.byref(s))
         return s
+
+    def setBrickSourceThreads(self, n: int) -> None:
+        call("tvk_set_brick_source_threads", self._h, int(n))
+
+    def pagingStats(self) -> tuple:
+        """(bytes of bricks streamed in, host ms spent paging) since the volume was set."""
+        b, ms = C.c_uint64(0), C.c_double(0.0)
+        call("tvk_get_paging_stats", self._h, C.byref(b), C.byref(ms))
+        return b.value, ms.value
+
+
+def memman_stats(device: int = 0):
+    """The device-wide memory manager's stats (MemManStats)."""
+    from ._lib import MemManStats
+    s = MemManStats()
+    call("tvk_get_memman_stats", int(device), C.byref(s))
+    return s
 
 
 class _RendererNS:

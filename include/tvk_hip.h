@@ -620,16 +620,26 @@ int tvk_autotune(tvk_renderer* r, const tvk_view* views, uint32_t n, uint32_t fl
 /* --------------------------------------------------------------- serving */
 /* Render server (no counterpart in the reference, where each paint serves
  * one view of one renderer): a worker thread batches requests as they
- * arrive into persistent-kernel launches over several volumes, keeps every
- * volume's bricks resident, admits a request when a launch lane and output
- * memory (device arena + pinned host arena of `arena_bytes` each) are free,
- * and overlaps launches on up to `lanes` HIP streams.  Requests that share a
- * kernel variant (bytes per voxel, interpolant, render mode, lighting) share
- * a launch, oldest first, up to 64 requests and -- after the first -- about
- * 1e9 estimated samples (tile-rect rays x the volume's longest axis in
- * steps), so light requests are not held until a heavy one finishes.  A
- * request's image is the same whether it runs alone or in a mix.  Registered renderers must hold in-core volumes and must not be
- * changed or used by other threads while the server runs. */
+ * arrive into persistent-kernel launches over several volumes and overlaps
+ * launches on up to `lanes` HIP streams.  In-core volumes keep their bricks
+ * resident; their requests are admitted when a launch lane and output
+ * memory (device arena + pinned host arena of `arena_bytes` each) are free.
+ * Requests that share a kernel variant (bytes per voxel, interpolant,
+ * render mode, lighting) share a launch, oldest first, up to 64 requests
+ * and -- after the first -- about 1e9 estimated samples (tile-rect rays x
+ * the volume's longest axis in steps), so light requests are not held
+ * until a heavy one finishes.  Paged volumes (tvk_set_paged_volume and
+ * friends) page through the device-wide brick pool: each has a worker that
+ * renders its requests in batches of up to 64 of one render mode with the
+ * renderer's multi-pass paging, on the renderer's own streams, beside the
+ * lanes; a batch is admitted by brick-pool memory -- when its working set
+ * (the bricks of the finest allowed LOD, at most the volume's pool share)
+ * fits the pool beside the paged batches already running (GPU budget less
+ * the device memory held outside the pool); one batch alone always runs.
+ * Bricks stay resident between batches until the pool needs their slots.
+ * A request's image is the same whether it runs alone or in a mix.
+ * Registered renderers must not be changed or used by other threads while
+ * the server runs. */
 typedef struct tvk_server tvk_server;
 typedef struct tvk_request {
   uint32_t volume;        /* id from tvk_server_add_volume                */
@@ -651,6 +661,9 @@ typedef struct tvk_server_stats {
   uint64_t max_jobs_per_launch;
   uint64_t max_lanes_busy;
   uint64_t nominal_samples;
+  uint64_t paged_launches;    /* batches of paged volumes rendered            */
+  uint64_t pool_waits;        /* batches that waited for device pool memory   */
+  uint64_t max_pool_reserved; /* largest pool bytes held by running batches   */
 } tvk_server_stats;
 int tvk_server_create(int device, uint64_t arena_bytes, uint32_t lanes, tvk_server** out);
 int tvk_server_destroy(tvk_server* s);
@@ -669,11 +682,49 @@ int tvk_server_get_stats(tvk_server* s, tvk_server_stats* out);
 int tvk_set_launch_graphs(tvk_renderer* r, int enabled);
 
 /* ---------------------------------------------------------------- memory */
-/* SysInfo()->SetMaxUsableGPUMem + MemMan()->MemSizesChanged()
- * (ImageVis3D_Settings.cpp:557-559). */
+/* One memory manager per device, shared by every renderer and render
+ * server on it (Tuvok's single GPUMemMan with SysInfo's budgets):
+ *  - a GPU budget over all device memory the library allocates there (an
+ *    allocation beyond it first frees paged-pool chunks, then fails with
+ *    TVK_ERR_OOM); default 90% of the device;
+ *  - one paged brick pool: paged volumes of all renderers take slots from
+ *    it, chunk by chunk within the budget; when full, the least recently
+ *    sampled brick of any volume is paged out (never one its own volume
+ *    sampled in the frame being completed), and a volume's pool_bytes caps
+ *    its own share;
+ *  - a CPU budget over host memory held for bricks (staging buffers, the
+ *    coarser levels of a flat paged volume, and a host cache of bricks
+ *    produced by file or callback sources, so that a brick paged out and
+ *    needed again is not read and decompressed again); default half of the
+ *    host's memory.
+ * SysInfo()->SetMaxUsableGPUMem / SetMaxUsableCPUMem + MemMan()->
+ * MemSizesChanged() (ImageVis3D_Settings.cpp:556-559): set the budgets of
+ * the renderer's device. */
 int tvk_set_max_usable_gpu_mem(tvk_renderer* r, uint64_t bytes);
-/* MemMan()->GetGPUMem/GetAllocatedGPUMem (ImageVis3D_DebugWindow.cpp:172-206). */
+int tvk_set_max_usable_cpu_mem(tvk_renderer* r, uint64_t bytes);
+/* MemMan()->GetGPUMem/GetAllocatedGPUMem (ImageVis3D_DebugWindow.cpp:172-206):
+ * this renderer's share (its buffers and its slots in the device pool). */
 int tvk_get_mem_stats(tvk_renderer* r, tvk_mem_stats* out);
+/* The device's memory manager as a whole: GetGPUMem, GetAllocatedGPUMem,
+ * GetCPUMem, GetAllocatedCPUMem (ImageVis3D_DebugWindow.cpp:172-206) plus
+ * the paged pool and host cache.  Valid once a renderer has initialized
+ * on the device. */
+typedef struct tvk_memman_stats {
+  uint64_t gpu_mem, gpu_budget, gpu_allocated;   /* device total, budget, allocated */
+  uint64_t cpu_mem, cpu_budget, cpu_allocated;   /* host total, budget, held for bricks */
+  uint64_t pool_bytes, pool_slots, pool_resident; /* paged pool chunks, slots, slots in use */
+  uint64_t pool_evictions, chunks_freed;
+  uint64_t cache_bytes, cache_hits, cache_misses; /* host brick cache */
+} tvk_memman_stats;
+int tvk_get_memman_stats(int device, tvk_memman_stats* out);
+/* Threads that run a paged volume's brick source in parallel (default 8;
+ * 1 for a source that must not be called concurrently).  No counterpart in
+ * the reference (its bricks load on the rendering thread). */
+int tvk_set_brick_source_threads(tvk_renderer* r, uint32_t threads);
+/* Paging since the volume was set: bytes of bricks streamed into the pool
+ * and the host milliseconds spent producing and issuing them (sources,
+ * cache, copies).  No counterpart in the reference (measurement). */
+int tvk_get_paging_stats(tvk_renderer* r, uint64_t* bytes, double* ms);
 
 /* ------------------------------------------------------ synthetic data */
 /* Seeded synthetic scalar volumes: UVFReader's GenerateVolumeData

@@ -1,0 +1,180 @@
+"""The device-wide memory manager (GPUMemMan + SysInfo budgets,
+ImageVis3D_Settings.cpp:556-559; accounting ImageVis3D_DebugWindow.cpp:
+172-206): paged volumes of several renderers share one brick pool under the
+GPU budget and page each other's least recent bricks out, a slot size in
+need takes a chunk from another, bricks from file / callback sources are
+kept in a host cache under the CPU budget, and every image stays exactly
+the in-core image."""
+import numpy as np
+import pytest
+
+from imagevis3d_amd import TvkError, brick_slot_bytes, tuvok, workloads
+from imagevis3d_amd.renderer import memman_stats
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def renderer(w):
+    r = tuvok.renderer.new(tuvok.renderer.types.OpenGL_Raycaster, False, False, False, False)
+    assert r.initialize(tuvok.createContext(w.width, w.height, device=0)), r.lastError()
+    r.setRendererTarget(tuvok.renderer.types.RT_Capture)
+    return r
+
+
+def frame(r, w, cam):
+    workloads.apply(r, w, cam)
+    return r.capture()
+
+
+def setup(n=128):
+    w = workloads.medium(dims=(n, n, n), width=256, height=144)
+    w.brick = 32
+    return w, w.volume()
+
+
+class Budget:
+    """Sets the device's GPU (and CPU) budget for a test and restores it."""
+
+    def __init__(self, r):
+        self.r = r
+        s = memman_stats(0)
+        self.gpu, self.cpu = s.gpu_budget, s.cpu_budget
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.r.setMaxUsableGPUMem(self.gpu)
+        self.r.setMaxUsableCPUMem(self.cpu)
+
+
+def test_renderers_share_the_pool_and_page_each_other_out():
+    w, vol = setup()
+    print("case shared pool, two renderers", flush=True)
+    ref = renderer(w)
+    ref.setVolume(vol, 32, 2, 1)
+    ref.set1DTrans(w.tf)
+    a, b = renderer(w), renderer(w)
+    sb = brick_slot_bytes(32, 2, 16)
+    for r in (a, b):
+        r.setPagedFlatVolume(vol, 32, 64 * sb, n_lod=1)  # each may hold all 64 bricks
+        r.set1DTrans(w.tf)
+    with Budget(a):
+        # room for about 100 slots: either volume's view fits, both do not
+        s0 = memman_stats(0)
+        a.setMaxUsableGPUMem(s0.gpu_allocated + 100 * sb + 4096)
+        views = [0, 9, 18, 27]
+        for rnd in range(2):
+            for v in views:
+                for r in (a, b):
+                    assert np.array_equal(frame(r, w, w.cameras[v]), frame(ref, w, w.cameras[v])), \
+                        f"round {rnd} view {v}"
+        s1 = memman_stats(0)
+        assert s1.pool_evictions > s0.pool_evictions
+        assert s1.gpu_allocated <= s1.gpu_budget
+        ea, eb = a.memStats().bricks_evicted, b.memStats().bricks_evicted
+        assert ea > 0 and eb > 0  # each paged the other's bricks out
+    for r in (ref, a, b):
+        r.cleanup()
+
+
+def test_a_slot_size_in_need_takes_another_sizes_chunk():
+    w, vol = setup()
+    print("case chunk reclaim across slot sizes", flush=True)
+    a, c = renderer(w), renderer(w)
+    sa, sc = brick_slot_bytes(32, 2, 16), brick_slot_bytes(16, 2, 16)
+    a.setPagedFlatVolume(vol, 32, 64 * sa, n_lod=1)
+    a.set1DTrans(w.tf)
+    c.setPagedFlatVolume(vol, 16, 512 * sc, n_lod=1)
+    c.set1DTrans(w.tf)
+    ref = renderer(w)
+    ref.setVolume(vol, 16, 2, 1)
+    ref.set1DTrans(w.tf)
+    with Budget(a):
+        s0 = memman_stats(0)
+        # A's bricks fill the budget; C (another slot size) needs it next
+        a.setMaxUsableGPUMem(s0.gpu_allocated + 70 * sa)
+        frame(a, w, w.cameras[4])
+        before = memman_stats(0).chunks_freed
+        got = frame(c, w, w.cameras[4])
+        assert np.array_equal(got, frame(ref, w, w.cameras[4]))
+        assert memman_stats(0).chunks_freed > before
+        assert a.memStats().resident_bricks == 0  # A's chunk went to C
+    for r in (a, c, ref):
+        r.cleanup()
+
+
+def test_host_cache_keeps_source_bricks_within_the_cpu_budget():
+    n, B = 96, 16
+    vol = np.zeros((n, n, n), np.uint8)
+    vol[8:80, 10:90, 20:70] = 120
+    vol[30:50, 30:60, 30:50] = 220
+    tf = np.zeros((256, 4), np.float32)
+    tf[100:] = (0.9, 0.6, 0.3, 0.05)
+    cams = [workloads.Camera((1.0, 0.5, 1.3), (-1.0, -0.5, -1.3), fov=45.0),
+            workloads.Camera((-1.2, 0.3, -1.0), (1.2, -0.3, 1.0), fov=45.0)]
+    w = workloads.Workload("cache", (n, n, n), 8, 0, 0, B, 2, 2, tf, 160, 120, 1.2, cams)
+    levels = O.lod_pyramid(vol, 2)
+    grids = [tuple(-(-d // B) for d in lv.shape) for lv in levels]  # (gz, gy, gx)
+    ranges = []
+    for lv, (gz, gy, gx) in zip(levels, grids):
+        for bz in range(gz):
+            for by in range(gy):
+                for bx in range(gx):
+                    ranges.append(O.brick_minmax(lv, B, 2, bx, by, bz))
+    ranges = np.asarray(ranges, np.float32)
+    calls = []
+
+    def source(lod, bx, by, bz):
+        calls.append((lod, bx, by, bz))
+        return O.extract_brick(levels[lod], B, 2, bx, by, bz)
+
+    ref = renderer(w)
+    ref.setVolume(vol, B, 2, 2)
+    ref.set1DTrans(tf)
+    pg = renderer(w)
+    n0 = int(np.prod(grids[0]))
+    need0 = int((ranges[:n0, 1] >= 100).sum())
+    # a share that holds LOD 0's non-empty bricks but not LOD 1's beside them
+    pg.setPagedVolume((n, n, n), 8, B, source, (need0 + 2) * brick_slot_bytes(B, 2, 8),
+                      brick_ranges=ranges, n_lod=2)
+    pg.set1DTrans(tf)
+
+    def both(lod):
+        for r in (pg, ref):
+            r.setLODLimits((lod, lod))
+        return np.array_equal(frame(pg, w, cams[0]), frame(ref, w, cams[0]))
+
+    with Budget(pg):
+        assert both(0) and both(1)
+        assert pg.memStats().bricks_evicted > 0  # LOD 1 took LOD 0 slots
+        first = len(calls)
+        hits0 = memman_stats(0).cache_hits
+        assert both(0) and both(1)  # paged out and in again: from the host cache
+        assert len(calls) == first and memman_stats(0).cache_hits > hits0
+        # a CPU budget with no room left for the cache: the source runs again
+        s = memman_stats(0)
+        pg.setMaxUsableCPUMem(s.cpu_allocated - s.cache_bytes + 1)
+        assert memman_stats(0).cache_bytes == 0
+        assert both(0)
+        assert len(calls) > first
+        b, ms = pg.pagingStats()
+        assert b > 0 and ms > 0
+    ref.cleanup()
+    pg.cleanup()
+
+
+def test_budget_refuses_what_cannot_fit():
+    w, vol = setup(64)
+    r = renderer(w)
+    with Budget(r):
+        s0 = memman_stats(0)
+        r.setMaxUsableGPUMem(s0.gpu_allocated + (1 << 20))
+        with pytest.raises(TvkError, match="(?i)memory|budget"):
+            r.setVolume(vol, 32, 2, 1)  # 512 KiB of voxels + bricks do not fit in 1 MiB
+    r.setVolume(vol, 32, 2, 1)  # the restored budget holds it
+    assert memman_stats(0).gpu_allocated <= memman_stats(0).gpu_budget
+    with pytest.raises(TvkError):
+        r.setBrickSourceThreads(0)
+    r.cleanup()

@@ -3,9 +3,10 @@ workload (512^3 uint16, 256^3 bricks + 2-voxel apron, 3 LODs, 4096-entry
 Gaussian-bump TF, 1920x1080 perspective 60 degree FoV, 36-view orbit,
 sampling rate 1.5, min/max empty-space skipping), rendered at LOD 0.
 
-A step is one frame: one raycast of the whole 1080p image for the next
-view of the orbit, with the volume already bricked into the device pool.
-`value` is nominal samples per second (sum over pixels of
+A step is one orbit: all 36 views of the 1080p orbit raycast in one
+persistent-kernel launch (tvk_render_batch), with the volume already
+bricked into the device pool and the images left in HBM; each step waits
+for its per-view sample counters.  `value` is nominal samples per second (sum over pixels of
 floor(chord * rate) through the LOD-0 box, identical for any renderer);
 frames/s and evaluated samples/s are reported beside it.  The roofline
 line prices the raycast kernel at its algorithmic gather traffic:
@@ -42,12 +43,14 @@ def cpu_baseline(vol, w, cams, threads: int) -> dict:
         t0 = time.perf_counter()
         O.baseline_render(p, threads)
         secs += time.perf_counter() - t0
+    isa = "AVX-512" if O.lib().baseline_uses_avx512() else "AVX2"
     return {"value": nominal / secs / 1e9, "unit": "Gsamples/s", "cores": threads,
             "kind": "port",
             "sample": f"{len(cams)} of the 36 orbit views at 1920x1080 on the 512^3 volume, "
-                      f"LOD 0, {secs:.1f} s wall on {threads} threads (FP32 OpenMP raycaster, "
-                      f"same model and empty-space cells)",
-            "fps": len(cams) / secs}
+                      f"LOD 0, {secs:.1f} s wall on {threads} threads = every host core "
+                      f"(nproc {threads}; FP32 OpenMP raycaster built for {isa}, same model, "
+                      f"empty-space cells skipped span by span)",
+            "fps": len(cams) / secs, "isa": isa}
 
 
 def calibrate_copy(nbytes: int = 1 << 30) -> int:
@@ -101,49 +104,26 @@ def run_serving(duration_s: float, rate_hz: float, seed: int = 5) -> dict:
     camera, a random 1D or 2D TF (two 1D slots and one 2D slot per volume)
     and an image of 256^2, 512^2, 1920x1080 or 3840x2160.  Returns latency
     percentiles (submit -> image in host memory) and aggregate samples/s."""
-    import math
     import threading
 
     import numpy as np
 
-    from imagevis3d_amd import transfer, tuvok
-    from imagevis3d_amd.renderer import VOL_FBM_SPHERES, RenderMode
+    from imagevis3d_amd import workloads
+    from imagevis3d_amd.renderer import RenderMode
     from imagevis3d_amd.serving import RenderServer
 
-    pool = [((64, 64, 64), 8, 64), ((64, 64, 64), 16, 64), ((256, 256, 256), 8, 128),
-            ((256, 256, 256), 16, 128), ((512, 512, 512), 8, 128), ((512, 512, 512), 16, 128),
-            ((2048, 2048, 2048), 16, 128)]
-    rs = []
     t0 = time.perf_counter()
-    for i, (dims, bits, brick) in enumerate(pool):
-        r = tuvok.renderer.new(tuvok.renderer.types.OpenGL_Raycaster, False, False, False, False)
-        assert r.initialize(tuvok.createContext(256, 256, device=0)), r.lastError()
-        r.setGeneratedVolume(VOL_FBM_SPHERES, 100 + i, dims, bits, brick, 2, 1)
-        n = 256 if bits == 8 else 4096
-        r.set1DTransSlot(0, transfer.gaussian_bumps_tf(n, seed=200 + i))
-        r.set1DTransSlot(1, transfer.ramp_tf(n, 0.45 * n, 0.95 * n))
-        _, mg = r.histogram2D(256, 256)
-        r.set2DTransSlot(2, transfer.gaussian_lobes_2d(256, 256, seed=300 + i), mg)
-        r.setSampleRateModifier(1.0)
-        rs.append(r)
+    rs = workloads.serving_pool()
     setup_s = time.perf_counter() - t0
     srv = RenderServer(arena_bytes=4 << 30, lanes=4)
     ids = [srv.addVolume(r) for r in rs]
     rng = np.random.default_rng(seed)
     n_req = int(duration_s * rate_hz)
     arrivals = np.cumsum(rng.exponential(1.0 / rate_hz, n_req))
-    sizes = [(256, 256), (512, 512), (1920, 1080), (3840, 2160)]
     reqs = []
     for _ in range(n_req):
-        th, ph = rng.uniform(0, 2 * math.pi), rng.uniform(-1.3, 1.3)
-        d = rng.uniform(1.5, 2.4)
-        eye = (d * math.cos(ph) * math.sin(th), d * math.sin(ph), d * math.cos(ph) * math.cos(th))
-        w, h = sizes[rng.integers(4)]
-        two_d = rng.uniform() < 0.5
-        view = dict(view_pos=eye, view_dir=tuple(-x for x in eye), fov=50.0, width=w, height=h,
-                    tf_slot=2 if two_d else int(rng.integers(2)))
-        reqs.append((ids[rng.integers(len(ids))], view,
-                     RenderMode.RM_2DTRANS if two_d else RenderMode.RM_1DTRANS))
+        v, view, mode = workloads.serving_request(rng, len(ids))
+        reqs.append((ids[v], view, mode))
     # warm every kernel variant and graph once (not timed)
     for v in range(len(ids)):
         for mode, slot in ((RenderMode.RM_1DTRANS, 0), (RenderMode.RM_2DTRANS, 2)):
@@ -194,10 +174,10 @@ def run_batched(batches: int = 3, seed: int = 3) -> dict:
     4 seeded 2D TFs (256x256 Gaussian lobes over value x gradient
     magnitude) with Phong shading; 8 orbit cameras x 4 TFs = 32 views at
     1920x1080 per persistent-kernel launch; launch shape autotuned."""
-    from imagevis3d_amd import transfer, tuvok, workloads
-    from imagevis3d_amd.renderer import VOL_FBM_PLANES, RenderMode
+    from imagevis3d_amd import tuvok, workloads
+    from imagevis3d_amd.renderer import VOL_FBM_PLANES
 
-    dims, bits, brick = (2048, 2048, 1024), 16, 128
+    dims, bits = workloads.BATCHED_DIMS, 16
     t0 = time.perf_counter()
     # exact gradient range from an in-core pass, then the host copy
     g = tuvok.renderer.new(tuvok.renderer.types.OpenGL_Raycaster, False, False, False, False)
@@ -207,24 +187,14 @@ def run_batched(batches: int = 3, seed: int = 3) -> dict:
     vol = g.readLOD(0)
     g.cleanup()
     gen_s = time.perf_counter() - t0
-    r = tuvok.renderer.new(tuvok.renderer.types.OpenGL_Raycaster, False, False, False, False)
-    assert r.initialize(tuvok.createContext(64, 64, device=0)), r.lastError()
     t0 = time.perf_counter()
-    r.setPagedFlatVolume(vol, brick, 64 * 10**9, n_lod=5)
+    r = workloads.batched_renderer(vol, max_grad, 64 * 10**9)
     plan_s = time.perf_counter() - t0
-    r.setRenderMode(RenderMode.RM_2DTRANS)
-    for k in range(4):
-        r.set2DTransSlot(k, transfer.gaussian_lobes_2d(256, 256, seed=40 + k), max_grad)
-    r.setLightingEnabled(True)
-    r.setSampleRateModifier(1.0)
-    r.setLODLimits((0, 0))
-    ext = tuple(d / max(dims) for d in dims)
-    cams = workloads.orbit_cameras(8, fov=60.0, elevation_deg=25.0, extent=ext)
-    views = [dict(view_pos=c.view_pos, view_dir=c.view_dir, up_dir=c.up_dir, fov=c.fov,
-                  width=1920, height=1080, tf_slot=k) for k in range(4) for c in cams]
+    views = workloads.batched_views(dims)
     t0 = time.perf_counter()
     _, st0, _ = r.renderBatch(views, readback=False)  # pages the view set's bricks in
     page_s = time.perf_counter() - t0
+    paged_bytes, paging_host_ms = r.pagingStats()
     t = r.autotune(views, flags=r.TUNE_LAUNCH)
     kms, nominal, evaluated = [], 0, 0
     t0 = time.perf_counter()
@@ -252,6 +222,12 @@ def run_batched(batches: int = 3, seed: int = 3) -> dict:
                  "uploaded": int(ms.bricks_uploaded)},
         "setup_s": {"generate": round(gen_s, 2), "host_pyramid": round(plan_s, 2),
                     "first_batch_paging": round(page_s, 2)},
+        # the cold first launch: every brick the 32 views sample streamed from
+        # host memory (brick sources on worker threads, copies on the copy
+        # stream) -- bytes of bricks over the launch's wall time
+        "cold_paging": {"bytes": int(paged_bytes), "wall_s": round(page_s, 3),
+                        "gb_per_s": paged_bytes / page_s / 1e9,
+                        "host_paging_ms": round(paging_host_ms, 1)},
     }
     r.cleanup()
     return out
@@ -562,7 +538,11 @@ def main() -> int:
         "kernel_ms_per_launch": ksum / args.steps * 1e3,
         "kernel_ms_per_frame": ksum / frames * 1e3,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE per launch read from "
+                                        + os.path.relpath(TRAFFIC_FILE, ROOT)
+                                        + " (a committed profile, not measured in this run)")
+                     if traffic is not None else None},
         "cpu_baseline": None,
     }
     if not args.no_interactive:
@@ -591,9 +571,9 @@ def main() -> int:
         print(f"[bench] serving {sv}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         try:
-            threads = min(16, len(os.sched_getaffinity(0)))
+            threads = len(os.sched_getaffinity(0))  # every host core this process may use
         except AttributeError:
-            threads = min(16, os.cpu_count() or 1)
+            threads = os.cpu_count() or 1
         print(f"[bench] CPU baseline on {threads} threads", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(vol, w, cams[:args.cpu_views], threads)
     print(json.dumps(out))

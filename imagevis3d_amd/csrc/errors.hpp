@@ -5,4 +5,6 @@
 namespace tvk {
 // Record the message returned by tvk_last_error() on this thread.
 void set_last_error(const std::string& msg);
+// The message last recorded on this thread.
+const std::string& last_error();
 }  // namespace tvk

@@ -40,7 +40,7 @@ hipError_t launch_slot_to_linear(const void* slot, void* dst, int bpv, int S, hi
 // Empty-space cell ranges of one flat level: out[(cz*ncy + cy)*ncx + cx] =
 // min | max << 16 over voxels [c*C - 1, (c+1)*C] (clamped to the level).
 hipError_t launch_cell_ranges(const void* level, int bpv, const int64_t dims[3], int C,
-                              uint32_t* out, hipStream_t stream);
+                              uint32_t* out, uint32_t* tmp_hi, hipStream_t stream);
 
 // Cell ranges of the cells inside one uploaded brick, computed from its pool
 // slot (slot edge S = B + 2*apron; `cells` = the level's cell grid; out =
@@ -122,6 +122,10 @@ hipError_t launch_raycast(const LaunchInfo& li, const JobTable* tab, hipStream_t
 // strides over all jobs' pixels).
 constexpr int kBackgroundBlocks = 1024;
 hipError_t launch_background(const JobTable* tab, hipStream_t stream);
+// *out += sum over the n pixels of two RGBA8 images of the squared RGB
+// differences (integer units)
+hipError_t launch_rgb_sse(const uint32_t* a, const uint32_t* b, int64_t n, unsigned long long* out,
+                          hipStream_t stream);
 // dst (dw x dh) from src (sw x sh) by nearest pixel: dst(x, y) = src(x sw / dw, y sh / dh)
 hipError_t launch_upscale_nearest(const uint32_t* src, int sw, int sh, uint32_t* dst, int dw, int dh,
                                   hipStream_t stream);

@@ -10,6 +10,8 @@
 // workgroup.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace tvk {
@@ -209,31 +211,73 @@ __global__ __launch_bounds__(kThreads) void slot_to_linear_kernel(const T* __res
   dst[i] = slot[slot_offset(slot_geo(S), x, y, z)];
 }
 
-// One wave per empty-space cell: lanes stride over the cell's (C+2)^3
-// neighbourhood row by row, then reduce.
+// Many waves per empty-space cell: the cell's neighbourhood [c*C - 1,
+// (c+1)*C] (clamped) is cut into rows of x; a wave takes max(1, 64 / w)
+// rows at a time with its lanes over x, the work-groups along grid.y split
+// the rows among them, and each work-group folds its minimum and maximum
+// into the cell with one atomic each (lo starts at ~0, hi at 0).  The work
+// per launch does not depend on the cell size.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void cell_range_kernel(const T* __restrict__ level,
-                                                              int64_t nx, int64_t ny, int64_t nz,
-                                                              int C, int ncx, int ncy, int ncz,
-                                                              uint32_t* __restrict__ out) {
-  const int wave = (int)(blockIdx.x * (kThreads / 64) + threadIdx.x / 64);
-  const int lane = (int)(threadIdx.x & 63);
-  if (wave >= ncx * ncy * ncz) return;
-  const int cx = wave % ncx, cy = (wave / ncx) % ncy, cz = wave / (ncx * ncy);
+__global__ __launch_bounds__(kThreads) void cell_range_rows_kernel(const T* __restrict__ level,
+                                                                   int64_t nx, int64_t ny, int64_t nz,
+                                                                   int C, int ncx, int ncy,
+                                                                   uint32_t* __restrict__ lo_out,
+                                                                   uint32_t* __restrict__ hi_out) {
+  const int cell = (int)blockIdx.x;
+  const int cx = cell % ncx, cy = (cell / ncx) % ncy, cz = cell / (ncx * ncy);
   const int64_t x0 = clamp64((int64_t)cx * C - 1, 0, nx - 1), x1 = clamp64((int64_t)(cx + 1) * C, 0, nx - 1);
   const int64_t y0 = clamp64((int64_t)cy * C - 1, 0, ny - 1), y1 = clamp64((int64_t)(cy + 1) * C, 0, ny - 1);
   const int64_t z0 = clamp64((int64_t)cz * C - 1, 0, nz - 1), z1 = clamp64((int64_t)(cz + 1) * C, 0, nz - 1);
   const int w = (int)(x1 - x0 + 1), h = (int)(y1 - y0 + 1), dz = (int)(z1 - z0 + 1);
+  const int rows = h * dz;
+  const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  const int rpw = w < 64 ? 64 / w : 1;            // rows per wave step
+  const int lrow = w < 64 ? lane / w : 0;          // this lane's row in the step
+  const int lx = w < 64 ? lane % w : lane;         // and its first x
+  const int xstep = w < 64 ? w : 64;
+  const int waves = (int)gridDim.y * (kThreads / 64);
+  const int wg = (int)blockIdx.y * (kThreads / 64) + wave;
   uint32_t lo = 0xffffffffu, hi = 0;
-  for (int i = lane; i < w * h * dz; i += 64) {
-    const int x = i % w, y = (i / w) % h, z = i / (w * h);
-    const uint32_t v = level[((z0 + z) * ny + (y0 + y)) * nx + (x0 + x)];
-    lo = min(lo, v);
-    hi = max(hi, v);
+  if (lrow < rpw) {
+    for (int r = wg * rpw + lrow; r < rows; r += waves * rpw) {
+      const int y = r % h, z = r / h;
+      const T* __restrict__ row = level + ((z0 + z) * ny + (y0 + y)) * nx + x0;
+      for (int x = lx; x < w; x += xstep) {
+        const uint32_t v = row[x];
+        lo = min(lo, v);
+        hi = max(hi, v);
+      }
+    }
   }
   lo = wave_min(lo);
   hi = wave_max(hi);
-  if (lane == 0) out[wave] = lo | (hi << 16);
+  __shared__ uint32_t s_lo[kThreads / 64], s_hi[kThreads / 64];
+  if (lane == 0) {
+    s_lo[wave] = lo;
+    s_hi[wave] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kThreads / 64; ++k) {
+      lo = min(lo, s_lo[k]);
+      hi = max(hi, s_hi[k]);
+    }
+    atomicMin(lo_out + cell, lo);
+    atomicMax(hi_out + cell, hi);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void fill_u32_kernel(uint32_t* __restrict__ p, int64_t n,
+                                                            uint32_t v) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    p[i] = v;
+}
+
+// out = lo | hi << 16 (16-bit voxel values)
+__global__ __launch_bounds__(kThreads) void pack_ranges_kernel(uint32_t* __restrict__ out,
+                                                               const uint32_t* __restrict__ hi, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+    out[i] |= hi[i] << 16;
 }
 
 // Cell ranges of one brick from its pool slot (paged pools, after upload):
@@ -355,6 +399,29 @@ __global__ __launch_bounds__(kThreads) void gradient_kmax_kernel(const T* __rest
   if ((threadIdx.x & 63) == 0 && m) atomicMax(kmax, m);
 }
 
+// The 2D histogram bin of voxel (x, y, z): value >> shift along the value
+// axis, b = max{ b : b^2 kmax <= F^2 K } along the gradient axis (F =
+// n_grad - 1, K and kmax squared integer gradients), decided exactly in
+// integers after a float estimate.
+template <typename T>
+__device__ inline int64_t hist2d_bin(const T* __restrict__ v, int64_t nx, int64_t ny, int64_t nz,
+                                     int64_t x, int64_t y, int64_t z, unsigned long long kmax, int shift,
+                                     int n_grad) {
+  const unsigned long long F = (unsigned long long)(n_grad - 1);
+  const unsigned long long K = grad_K(v, nx, ny, nz, x, y, z);
+  long long b = 0;
+  if (kmax > 0) {
+    b = (long long)floorf((float)F * sqrtf((float)K / (float)kmax));
+    if (b < 0) b = 0;
+    if (b > (long long)F) b = (long long)F;
+    const unsigned long long rhs = F * F * K;
+    while (b < (long long)F && (unsigned long long)(b + 1) * (unsigned long long)(b + 1) * kmax <= rhs) ++b;
+    while (b > 0 && (unsigned long long)b * (unsigned long long)b * kmax > rhs) --b;
+  }
+  const int64_t vb = (int64_t)(v[(z * ny + y) * nx + x] >> shift);
+  return vb * n_grad + b;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void hist2d_kernel(const T* __restrict__ v, int64_t nx,
                                                           int64_t ny, int64_t nz,
@@ -364,25 +431,79 @@ __global__ __launch_bounds__(kThreads) void hist2d_kernel(const T* __restrict__ 
   const int64_t x = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (x >= nx) return;
   const unsigned long long kmax = *kmax_p;
-  const unsigned long long F = (unsigned long long)(n_grad - 1);
   const uint32_t rows = (uint32_t)(ny * nz), uny = (uint32_t)ny;
   for (uint32_t row = blockIdx.y; row < rows; row += gridDim.y) {
     const int64_t y = row % uny, z = row / uny;
-    const unsigned long long K = grad_K(v, nx, ny, nz, x, y, z);
-    long long b = 0;
-    if (kmax > 0) {
-      // float estimate, then the exact integer decision of
-      // b = max{ b : b^2 * kmax <= F^2 * K }.
-      b = (long long)floorf((float)F * sqrtf((float)K / (float)kmax));
-      if (b < 0) b = 0;
-      if (b > (long long)F) b = (long long)F;
-      const unsigned long long rhs = F * F * K;
-      while (b < (long long)F && (unsigned long long)(b + 1) * (unsigned long long)(b + 1) * kmax <= rhs) ++b;
-      while (b > 0 && (unsigned long long)b * (unsigned long long)b * kmax > rhs) --b;
-    }
-    const int64_t vb = (int64_t)(v[(z * ny + y) * nx + x] >> shift);
-    atomicAdd(&hist[vb * n_grad + b], 1ull);
+    atomicAdd(&hist[hist2d_bin(v, nx, ny, nz, x, y, z, kmax, shift, n_grad)], 1ull);
   }
+}
+
+// Privatised histograms: the work-group counts in LDS, 16-bit counters two
+// to a 32-bit word (up to 65536 bins = 128 KiB), over whole rows of x and
+// at most 65535 voxels so that no counter carries into its neighbour, then
+// adds its non-zero bins to the global u64 histogram once.
+constexpr int kHistThreads = 512;
+constexpr int64_t kHistMaxVoxels = 65535;
+
+__device__ inline void lds_count(uint32_t* words, int64_t bin) {
+  atomicAdd(&words[bin >> 1], (bin & 1) ? 0x10000u : 1u);
+}
+
+__device__ inline void lds_flush(const uint32_t* words, int nwords, unsigned long long* hist) {
+  for (int k = (int)threadIdx.x; k < nwords; k += kHistThreads) {
+    const uint32_t wv = words[k];
+    if (wv & 0xffffu) atomicAdd(&hist[2 * k], (unsigned long long)(wv & 0xffffu));
+    if (wv >> 16) atomicAdd(&hist[2 * k + 1], (unsigned long long)(wv >> 16));
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kHistThreads) void hist2d_lds_kernel(const T* __restrict__ v, int64_t nx,
+                                                                  int64_t ny, int64_t nz,
+                                                                  const unsigned long long* kmax_p,
+                                                                  int shift, int n_grad, int nbins,
+                                                                  int rows_per_block,
+                                                                  unsigned long long* hist) {
+  extern __shared__ uint32_t words[];
+  const int nwords = nbins >> 1;
+  for (int k = (int)threadIdx.x; k < nwords; k += kHistThreads) words[k] = 0;
+  __syncthreads();
+  const unsigned long long kmax = *kmax_p;
+  const int64_t rows = ny * nz;
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t r1 = r0 + rows_per_block < rows ? r0 + rows_per_block : rows;
+  for (int64_t row = r0; row < r1; ++row) {
+    const int64_t y = row % ny, z = row / ny;
+    for (int64_t x = threadIdx.x; x < nx; x += kHistThreads)
+      lds_count(words, hist2d_bin(v, nx, ny, nz, x, y, z, kmax, shift, n_grad));
+  }
+  __syncthreads();
+  lds_flush(words, nwords, hist);
+}
+
+__global__ __launch_bounds__(kHistThreads) void hist1d_u16_lds_kernel(const uint16_t* __restrict__ v,
+                                                                      int64_t n,
+                                                                      unsigned long long* hist) {
+  extern __shared__ uint32_t words[];
+  constexpr int nwords = 65536 / 2;
+  for (int k = (int)threadIdx.x; k < nwords; k += kHistThreads) words[k] = 0;
+  __syncthreads();
+  const int64_t i0 = (int64_t)blockIdx.x * kHistMaxVoxels;
+  const int64_t i1 = i0 + kHistMaxVoxels < n ? i0 + kHistMaxVoxels : n;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kHistThreads) lds_count(words, v[i]);
+  __syncthreads();
+  lds_flush(words, nwords, hist);
+}
+
+// Dynamic LDS of `bytes` for kernel `k` on the current device, if it allows it.
+template <typename K>
+bool lds_allowed(K k, size_t bytes) {
+  int dev = 0, max_lds = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    return false;
+  if ((size_t)max_lds < bytes) return false;
+  return hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess;
 }
 
 inline dim3 row_grid(int64_t nx, int64_t rows) {
@@ -499,17 +620,26 @@ hipError_t launch_slot_to_linear(const void* slot, void* dst, int bpv, int S, hi
 }
 
 hipError_t launch_cell_ranges(const void* level, int bpv, const int64_t d[3], int C, uint32_t* out,
-                              hipStream_t stream) {
+                              uint32_t* tmp_hi, hipStream_t stream) {
   const int ncx = (int)((d[0] + C - 1) / C), ncy = (int)((d[1] + C - 1) / C),
             ncz = (int)((d[2] + C - 1) / C);
   const int64_t cells = (int64_t)ncx * ncy * ncz;
-  const int64_t blocks = (cells + (kThreads / 64) - 1) / (kThreads / 64);
+  const unsigned fill_blocks = (unsigned)std::min<int64_t>((cells + kThreads - 1) / kThreads, 65536);
+  fill_u32_kernel<<<fill_blocks, kThreads, 0, stream>>>(out, cells, 0xffffffffu);
+  fill_u32_kernel<<<fill_blocks, kThreads, 0, stream>>>(tmp_hi, cells, 0u);
+  // work-groups per cell: about 8 row steps per wave over the largest cell
+  const int64_t w = C + 2, rows = w * w;
+  const int64_t rpw = w < 64 ? 64 / w : 1;
+  const int64_t per_group = (kThreads / 64) * rpw * 8;
+  const unsigned gy = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (rows + per_group - 1) / per_group), 65535);
+  const dim3 grid((unsigned)cells, gy, 1);
   if (bpv == 1)
-    cell_range_kernel<uint8_t><<<(unsigned)blocks, kThreads, 0, stream>>>(
-        (const uint8_t*)level, d[0], d[1], d[2], C, ncx, ncy, ncz, out);
+    cell_range_rows_kernel<uint8_t><<<grid, kThreads, 0, stream>>>((const uint8_t*)level, d[0], d[1], d[2], C,
+                                                                   ncx, ncy, out, tmp_hi);
   else
-    cell_range_kernel<uint16_t><<<(unsigned)blocks, kThreads, 0, stream>>>(
-        (const uint16_t*)level, d[0], d[1], d[2], C, ncx, ncy, ncz, out);
+    cell_range_rows_kernel<uint16_t><<<grid, kThreads, 0, stream>>>((const uint16_t*)level, d[0], d[1], d[2], C,
+                                                                    ncx, ncy, out, tmp_hi);
+  pack_ranges_kernel<<<fill_blocks, kThreads, 0, stream>>>(out, tmp_hi, cells);
   return hipGetLastError();
 }
 
@@ -643,10 +773,17 @@ hipError_t launch_histogram_1d(const void* vol, int bpv, int64_t n, unsigned lon
   int64_t blocks = (n + kThreads * 16 - 1) / (kThreads * 16);
   if (blocks > 8192) blocks = 8192;
   if (blocks < 1) blocks = 1;
-  if (bpv == 1)
+  if (bpv == 1) {
     hist1d_u8_kernel<<<(unsigned)blocks, kThreads, 0, stream>>>((const uint8_t*)vol, n, hist);
-  else
-    hist1d_u16_kernel<<<(unsigned)blocks, kThreads, 0, stream>>>((const uint16_t*)vol, n, hist);
+    return hipGetLastError();
+  }
+  const size_t lds = 65536 * 2;
+  if (lds_allowed(hist1d_u16_lds_kernel, lds)) {
+    const int64_t groups = (n + kHistMaxVoxels - 1) / kHistMaxVoxels;
+    hist1d_u16_lds_kernel<<<(unsigned)groups, kHistThreads, lds, stream>>>((const uint16_t*)vol, n, hist);
+    if (hipGetLastError() == hipSuccess) return hipSuccess;
+  }
+  hist1d_u16_kernel<<<(unsigned)blocks, kThreads, 0, stream>>>((const uint16_t*)vol, n, hist);
   return hipGetLastError();
 }
 
@@ -671,6 +808,24 @@ hipError_t launch_histogram_2d(const void* vol, int bpv, const int64_t d[3],
   if (e != hipSuccess) return e;
   int shift = 0;
   while (((int64_t)n_value << shift) < ((int64_t)1 << (8 * bpv))) ++shift;
+  // privatised in LDS when the bins fit 128 KiB and a row fits one group
+  const int64_t nbins = (int64_t)n_value * n_grad;
+  const size_t lds = (size_t)nbins * 2;
+  if (nbins % 2 == 0 && nbins <= 65536 && d[0] <= kHistMaxVoxels) {
+    const int rows_per_block = (int)(kHistMaxVoxels / d[0]);
+    const int64_t groups = (d[1] * d[2] + rows_per_block - 1) / rows_per_block;
+    bool ok = false;
+    if (bpv == 1 && lds_allowed(hist2d_lds_kernel<uint8_t>, lds)) {
+      hist2d_lds_kernel<uint8_t><<<(unsigned)groups, kHistThreads, lds, stream>>>(
+          (const uint8_t*)vol, d[0], d[1], d[2], kmax, shift, n_grad, (int)nbins, rows_per_block, hist);
+      ok = hipGetLastError() == hipSuccess;
+    } else if (bpv == 2 && lds_allowed(hist2d_lds_kernel<uint16_t>, lds)) {
+      hist2d_lds_kernel<uint16_t><<<(unsigned)groups, kHistThreads, lds, stream>>>(
+          (const uint16_t*)vol, d[0], d[1], d[2], kmax, shift, n_grad, (int)nbins, rows_per_block, hist);
+      ok = hipGetLastError() == hipSuccess;
+    }
+    if (ok) return hipSuccess;
+  }
   const dim3 grid = row_grid(d[0], d[1] * d[2]);
   if (bpv == 1)
     hist2d_kernel<uint8_t><<<grid, kThreads, 0, stream>>>((const uint8_t*)vol, d[0], d[1], d[2],

@@ -515,6 +515,14 @@ __device__ inline int centre_out_row(int k, int h) {
 template <typename T, bool LINEAR, int BATCH, int MODE, bool TF_LDS>
 __device__ inline void render_tile(const RaycastParams& P, const float4* __restrict__ tf4, int tile,
                                    int lane, uint32_t& n_eval, uint32_t& n_nom) {
+  if (P.tile_sample > 1) {
+    // one tile of each run of tile_sample, at an offset hashed from the
+    // run and the phase (no stripes when the rect width shares a factor
+    // with tile_sample)
+    const uint32_t h = ((uint32_t)tile * 0x9E3779B1u + (uint32_t)P.tile_phase * 0x85EBCA77u) >> 16;
+    tile = tile * P.tile_sample + (int)(h % (uint32_t)P.tile_sample);
+    if (tile >= P.rect_w * P.rect_h) return;
+  }
   const int tx = P.rect_x0 + tile % P.rect_w, ty = P.rect_y0 + centre_out_row(tile / P.rect_w, P.rect_h);
   const int sh = P.tile_shift;
   const int px = (tx << sh) + (lane & ((1 << sh) - 1));
@@ -887,6 +895,24 @@ __global__ __launch_bounds__(256) void background_kernel(const JobTable* __restr
   }
 }
 
+// Sum over pixels of the squared RGB differences of two RGBA8 images
+// (integer units), folded into *out (cleared by the caller).
+__global__ __launch_bounds__(256) void rgb_sse_kernel(const uint32_t* __restrict__ a,
+                                                      const uint32_t* __restrict__ b, int64_t n,
+                                                      unsigned long long* __restrict__ out) {
+  unsigned long long acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const uint32_t x = gload(a + i), y = gload(b + i);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int d = (int)((x >> (8 * c)) & 255u) - (int)((y >> (8 * c)) & 255u);
+      acc += (unsigned long long)(d * d);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) wave_atomic_add(out, acc);  // one atomic per wave
+}
+
 // Nearest-pixel enlargement of a reduced-resolution subframe into the
 // framebuffer (the governor's low-resolution intermediates): integer
 // arithmetic, so the host can restate it exactly.
@@ -1049,6 +1075,13 @@ hipError_t launch_cell_empty(const uint32_t* cell_range, int64_t n_cells, const 
   const unsigned blocks = (unsigned)((n_cells + 255) / 256);
   cell_empty_kernel<<<blocks, 256, 0, stream>>>(cell_range, n_cells, prefix, n_tf, tf_scale,
                                                 mode == 2 ? kModeIso : mode, iso_value, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_rgb_sse(const uint32_t* a, const uint32_t* b, int64_t n, unsigned long long* out,
+                          hipStream_t stream) {
+  const int blocks = (int)std::min<int64_t>((n + 255) / 256, 2048);
+  hipLaunchKernelGGL(rgb_sse_kernel, dim3(blocks), dim3(256), 0, stream, a, b, n, out);
   return hipGetLastError();
 }
 

@@ -10,7 +10,8 @@ namespace tvk {
 
 namespace {
 
-constexpr uint64_t kChunkTarget = 512ull << 20;  // paged-pool chunk size
+constexpr uint64_t kChunkTarget = 512ull << 20;   // largest paged-pool chunk
+constexpr uint64_t kMinChunkBytes = 16ull << 20;  // smallest one the budget allows
 constexpr int kMaxWorkers = 31;
 constexpr int kGenShift = 40;  // ParallelFor ticket: generation << 40 | index
 
@@ -157,10 +158,11 @@ bool MemMan::protected_for(const Slot& sl, int client) const {
   return sl.client >= 0 && sl.client == client && sl.stamp >= clients_[client].frame_start;
 }
 
-bool MemMan::grow(int cls) {
+bool MemMan::grow(int cls, uint64_t want) {
   SlotClass& C = classes_[cls];
   const uint64_t room = gpu_budget_ > gpu_used_ ? gpu_budget_ - gpu_used_ : 0;
-  const uint64_t n = std::min<uint64_t>(C.per_chunk, room / C.slot_bytes);
+  const uint64_t min_slots = std::max<uint64_t>(1, kMinChunkBytes / C.slot_bytes);
+  const uint64_t n = std::min<uint64_t>({(uint64_t)C.per_chunk, room / C.slot_bytes, std::max(want, min_slots)});
   if (n == 0) return false;
   void* p = nullptr;
   {
@@ -176,8 +178,8 @@ bool MemMan::grow(int cls) {
   if (c == C.chunks.size()) {
     C.chunks.push_back(nullptr);
     C.chunk_slots.push_back(0);
-    C.slots.resize(C.chunks.size() * (size_t)C.per_chunk);
   }
+  if (C.slots.size() < (c + 1) * (size_t)C.per_chunk) C.slots.resize((c + 1) * (size_t)C.per_chunk);
   C.chunks[c] = p;
   C.chunk_slots[c] = (uint32_t)n;
   for (uint64_t i = 0; i < C.per_chunk; ++i) {
@@ -292,6 +294,8 @@ void MemMan::Unregister(int client) {
           C.chunks[c] = nullptr;
           C.chunk_slots[c] = 0;
         }
+      C.chunks.clear();
+      C.chunk_slots.clear();
       C.slots.clear();
       C.free_list.clear();
     }
@@ -337,13 +341,14 @@ size_t MemMan::Acquire(int client, const int* gids, size_t n, int64_t* slots, vo
     if (K.max_slots && K.held >= K.max_slots) {
       if (evict_lru(n - k, [&](const Slot& sl) { return sl.client == client; }) == 0) break;
     }
-    if (C.free_list.empty() && !grow(K.cls)) {
+    size_t want = n - k;
+    if (K.max_slots) want = std::min<uint64_t>(want, K.max_slots - std::min(K.max_slots, K.held));
+    want = std::max<size_t>(want, 1);
+    if (C.free_list.empty() && !grow(K.cls, want)) {
       // take back the least recently sampled slots of this size, else a
       // chunk of another size
-      size_t want = n - k;
-      if (K.max_slots) want = std::min<uint64_t>(want, K.max_slots - std::min(K.max_slots, K.held));
-      if (evict_lru(std::max<size_t>(want, 1), [](const Slot&) { return true; }) == 0) {
-        if (!reclaim_chunk(K.cls, C.slot_bytes) || !grow(K.cls)) break;
+      if (evict_lru(want, [](const Slot&) { return true; }) == 0) {
+        if (!reclaim_chunk(K.cls, C.slot_bytes) || !grow(K.cls, want)) break;
       }
     }
     while (k < n && !C.free_list.empty() && (!K.max_slots || K.held < K.max_slots)) {
@@ -444,6 +449,20 @@ void MemMan::CachePut(int client, int gid, const void* src, uint64_t bytes) {
 }
 
 // ----------------------------------------------------------- worker pool
+// The next index of the job tagged `tag`, if that job is still current and
+// has indices left.  A compare-exchange, so that a worker that wakes late
+// from an earlier job never consumes an index of the next one.
+bool MemMan::take_index(uint64_t tag, size_t n, size_t* i) {
+  size_t t = job_next_.load();
+  for (;;) {
+    if ((t >> kGenShift) != tag || (t & ((1ull << kGenShift) - 1)) >= n) return false;
+    if (job_next_.compare_exchange_weak(t, t + 1)) {
+      *i = t & ((1ull << kGenShift) - 1);
+      return true;
+    }
+  }
+}
+
 void MemMan::worker() {
   uint64_t seen = 0;
   for (;;) {
@@ -459,13 +478,8 @@ void MemMan::worker() {
       n = job_n_;
       ++job_active_;
     }
-    for (;;) {
-      const size_t t = job_next_.fetch_add(1);
-      if ((t >> kGenShift) != (gen & ((1ull << (64 - kGenShift)) - 1))) break;  // a later job
-      const size_t i = t & ((1ull << kGenShift) - 1);
-      if (i >= n) break;
-      (*fn)(i);
-    }
+    const uint64_t tag = gen & ((1ull << (64 - kGenShift)) - 1);
+    for (size_t i; take_index(tag, n, &i);) (*fn)(i);
     std::lock_guard<std::mutex> lk(wmu_);
     --job_active_;
     wdone_.notify_all();
@@ -488,14 +502,8 @@ void MemMan::ParallelFor(size_t n, int threads, const std::function<void(size_t)
     job_next_.store((size_t)(job_gen_ & ((1ull << (64 - kGenShift)) - 1)) << kGenShift);
   }
   wcv_.notify_all();
-  const uint64_t gen = job_gen_ & ((1ull << (64 - kGenShift)) - 1);
-  for (;;) {
-    const size_t t = job_next_.fetch_add(1);
-    if ((t >> kGenShift) != gen) break;
-    const size_t i = t & ((1ull << kGenShift) - 1);
-    if (i >= n) break;
-    fn(i);
-  }
+  const uint64_t tag = job_gen_ & ((1ull << (64 - kGenShift)) - 1);
+  for (size_t i; take_index(tag, n, &i);) fn(i);
   // every index has been taken; wait for the workers still running one
   std::unique_lock<std::mutex> lk(wmu_);
   wdone_.wait(lk, [&] { return job_active_ == 0; });

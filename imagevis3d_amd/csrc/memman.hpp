@@ -11,8 +11,10 @@
 //    (every DevBuf is accounted), and a CPU budget over host memory held
 //    for bricks (the host brick cache, host copies of paged levels);
 //  - the paged brick pool: slots of every paged volume of every renderer
-//    come from chunked arenas (one arena per slot size, chunks of up to
-//    512 MiB allocated on demand within the budget).  Page tables hold the
+//    come from chunked arenas (one arena per slot size, chunks allocated
+//    on demand within the budget, sized to the request that needs them:
+//    at least 16 MiB, at most 512 MiB, so the pool does not take the
+//    budget's headroom before it needs it).  Page tables hold the
 //    slots' device addresses, so a brick can live in any chunk.  When a
 //    request cannot be met from free slots or budget, the least recently
 //    sampled slot of any volume of that size is taken back (its owner
@@ -140,7 +142,7 @@ class MemMan {
   };
 
   int class_for(uint64_t slot_bytes);
-  bool grow(int cls);                       // one more chunk within the budget
+  bool grow(int cls, uint64_t want);        // one more chunk within the budget
   bool reclaim_chunk(int exclude_cls, uint64_t need_bytes);  // free another class's chunk
   void evict_slot(SlotClass& C, int64_t s);
   bool protected_for(const Slot& sl, int client) const;
@@ -164,6 +166,7 @@ class MemMan {
 
   // worker pool
   void worker();
+  bool take_index(uint64_t tag, size_t n, size_t* i);
   std::vector<std::thread> workers_;
   std::mutex wmu_;
   std::condition_variable wcv_, wdone_;

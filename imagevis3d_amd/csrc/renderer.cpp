@@ -166,6 +166,8 @@ void set_tile_rect(RaycastParams* p) {
   p->rect_x0 = p->rect_y0 = 0;
   p->rect_w = p->tiles_x;
   p->rect_h = p->tiles_y;
+  p->tile_sample = 1;
+  p->tile_phase = 0;
   double c0[3], c1[3], c2[3], eye[3];
   for (int i = 0; i < 3; ++i) {
     c0[i] = p->du[i];
@@ -514,10 +516,12 @@ int HipRenderer::brick_levels(const LevelPlan& plan) {
                                   pool_.brick_range.as<float>(), scratch_.as<uint32_t>(), stream_));
   }
   TVK_HIP(cell_range_.alloc((size_t)total_cells * sizeof(uint32_t)));
+  DevBuf cell_hi;  // the maxima while the kernel folds them in
+  TVK_HIP(cell_hi.alloc((size_t)total_cells * sizeof(uint32_t)));
   for (int l = 0; l < nl; ++l) {
     const int64_t d[3] = {lods[l].dims[0], lods[l].dims[1], lods[l].dims[2]};
-    TVK_HIP(launch_cell_ranges(levels_[l].p, bpv, d, cell,
-                               cell_range_.as<uint32_t>() + lods[l].cell_base, stream_));
+    TVK_HIP(launch_cell_ranges(levels_[l].p, bpv, d, cell, cell_range_.as<uint32_t>() + lods[l].cell_base,
+                               cell_hi.as<uint32_t>() + lods[l].cell_base, stream_));
   }
   pool_.page_table_h.resize(total);
   for (int i = 0; i < total; ++i)
@@ -1464,43 +1468,73 @@ LaunchInfo HipRenderer::launch_info(int n_tf_lds) const {
   return li;
 }
 
-int launch_job_table(LaunchSlot& sl, const RaycastParams* jobs, int n, const LaunchInfo& li,
-                     int tile_group, int xcd_split, hipStream_t stream_, bool graph) {
-  // the pinned table may still be read by this slot's previous upload
-  if (sl.used) TVK_HIP(hipEventSynchronize(sl.ev[3]));
-  JobTable* t = sl.table_h;
+// Tile slots of a job in the queues: its rect's tiles, or one per run of
+// tile_sample of them.
+static int job_tiles(const RaycastParams& p) {
+  const int all = p.rect_w * p.rect_h;
+  return p.tile_sample <= 1 ? all : (all + p.tile_sample - 1) / p.tile_sample;
+}
+
+// Fill the pinned table `t` from jobs[0..n) for a device table at
+// dev_table whose counters live at dev_counters; returns the tiles of the
+// launch and whether some job has background pixels outside its tile rect.
+static int fill_job_table(JobTable* t, const RaycastParams* jobs, int n, const LaunchInfo& li,
+                          int tile_group, int xcd_split, unsigned long long* dev_counters, bool* culled) {
   t->h.n_jobs = n;
   t->h.tile_group = tile_group;
   t->h.xcd_split = xcd_split;
   t->h.n_tf_lds = li.n_tf_lds;
   t->h.tf_lds = li.n_tf_lds > 0 ? jobs[0].tf : nullptr;
-  t->h.counters = sl.counters.as<unsigned long long>();
+  t->h.counters = dev_counters;
   int total = 0;
-  bool culled = false;  // some job has background tiles outside its rect
+  *culled = false;
   for (int k = 0; k < n; ++k) {
     t->tile_start[k] = total;
-    total += jobs[k].rect_w * jobs[k].rect_h;
-    culled = culled || jobs[k].rect_w * jobs[k].rect_h != jobs[k].tiles_x * jobs[k].tiles_y;
+    total += job_tiles(jobs[k]);
+    *culled = *culled || jobs[k].rect_w * jobs[k].rect_h != jobs[k].tiles_x * jobs[k].tiles_y;
+    // the frame-region split addresses tiles by rect position: sampled
+    // jobs take the round-robin runs
+    if (jobs[k].tile_sample > 1) t->h.xcd_split = 0;
     t->jobs[k] = jobs[k];
   }
   t->tile_start[n] = total;
   t->h.total_tiles = total;
+  return total;
+}
+
+int launch_job_table_at(JobTable* table_h, void* dev_table, unsigned long long* dev_counters,
+                        unsigned long long* counters_h, hipEvent_t ev[4], const RaycastParams* jobs, int n,
+                        const LaunchInfo& li, int tile_group, int xcd_split, hipStream_t stream_) {
+  bool culled = false;  // some job has background tiles outside its rect
+  fill_job_table(table_h, jobs, n, li, tile_group, xcd_split, dev_counters, &culled);
   const int words = kQueues + 2 * n;
   const size_t table_bytes = offsetof(JobTable, jobs) + (size_t)n * sizeof(RaycastParams);
+  TVK_HIP(hipEventRecord(ev[0], stream_));
+  TVK_HIP(hipMemcpyAsync(dev_table, table_h, table_bytes, hipMemcpyHostToDevice, stream_));
+  TVK_HIP(launch_fill_u64(dev_counters, words, 0ull, stream_));
+  TVK_HIP(hipEventRecord(ev[1], stream_));
+  if (culled) TVK_HIP(launch_background((const JobTable*)dev_table, stream_));
+  TVK_HIP(launch_raycast(li, (const JobTable*)dev_table, stream_));
+  TVK_HIP(hipEventRecord(ev[2], stream_));
+  if (counters_h)
+    TVK_HIP(hipMemcpyAsync(counters_h, dev_counters, words * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           stream_));
+  TVK_HIP(hipEventRecord(ev[3], stream_));
+  return TVK_OK;
+}
+
+int launch_job_table(LaunchSlot& sl, const RaycastParams* jobs, int n, const LaunchInfo& li,
+                     int tile_group, int xcd_split, hipStream_t stream_, bool graph) {
+  // the pinned table may still be read by this slot's previous upload
+  if (sl.used) TVK_HIP(hipEventSynchronize(sl.ev[3]));
   sl.used = true;
+  if (!graph)
+    return launch_job_table_at(sl.table_h, sl.table.p, sl.counters.as<unsigned long long>(), sl.counters_h,
+                               sl.ev, jobs, n, li, tile_group, xcd_split, stream_);
+  JobTable* t = sl.table_h;
+  bool culled = false;
+  fill_job_table(t, jobs, n, li, tile_group, xcd_split, sl.counters.as<unsigned long long>(), &culled);
   TVK_HIP(hipEventRecord(sl.ev[0], stream_));
-  if (!graph) {
-    TVK_HIP(hipMemcpyAsync(sl.table.p, t, table_bytes, hipMemcpyHostToDevice, stream_));
-    TVK_HIP(launch_fill_u64(sl.counters.as<unsigned long long>(), words, 0ull, stream_));
-    TVK_HIP(hipEventRecord(sl.ev[1], stream_));
-    if (culled) TVK_HIP(launch_background(sl.table.as<JobTable>(), stream_));
-    TVK_HIP(launch_raycast(li, sl.table.as<JobTable>(), stream_));
-    TVK_HIP(hipEventRecord(sl.ev[2], stream_));
-    TVK_HIP(hipMemcpyAsync(sl.counters_h, sl.counters.p, words * sizeof(unsigned long long),
-                           hipMemcpyDeviceToHost, stream_));
-    TVK_HIP(hipEventRecord(sl.ev[3], stream_));
-    return TVK_OK;
-  }
   // Graph replay: the graph copies the whole table and all counters, so one
   // capture serves any job count; it is re-captured when the kernel
   // variant or its LDS size changes.

@@ -132,6 +132,12 @@ struct LaunchSlot {
 // replay when `graph`; one graph per kernel variant, re-captured on change).
 int launch_job_table(LaunchSlot& sl, const RaycastParams* jobs, int n, const LaunchInfo& li,
                      int tile_group, int xcd_split, hipStream_t stream, bool graph);
+// The eager sequence on explicit buffers: fill the pinned table_h, copy it
+// to dev_table, clear dev_counters, background + raycast, and (when
+// counters_h) copy the counters back; ev = start, kernel in, kernel out, done.
+int launch_job_table_at(JobTable* table_h, void* dev_table, unsigned long long* dev_counters,
+                        unsigned long long* counters_h, hipEvent_t ev[4], const RaycastParams* jobs, int n,
+                        const LaunchInfo& li, int tile_group, int xcd_split, hipStream_t stream);
 
 // Out-of-core brick source (see tvk_brick_source_fn in tvk_hip.h).
 typedef int (*BrickSourceFn)(void* user, uint32_t lod, uint32_t bx, uint32_t by, uint32_t bz,

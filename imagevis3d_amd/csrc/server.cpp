@@ -192,6 +192,7 @@ void RenderServer::paged_loop(PagedVolume* pv) {
     lk.unlock();
     std::vector<tvk_view_stats> vs(pick.size());
     const int rc = r->ServeBatch(views.data(), (uint32_t)views.size(), mode, vs.data());
+    const std::string err = rc == TVK_OK ? std::string() : last_error();
     lk.lock();
     pool_reserved_ -= pv->need;
     pool_cv_.notify_all();
@@ -201,6 +202,7 @@ void RenderServer::paged_loop(PagedVolume* pv) {
       q.done = true;
       q.t_done = td;
       q.status = rc;
+      q.error = err;
       if (rc == TVK_OK) {
         q.evaluated = vs[k].evaluated_samples;
         q.nominal = vs[k].nominal_samples;
@@ -217,6 +219,7 @@ void RenderServer::paged_loop(PagedVolume* pv) {
   for (uint64_t t : pv->pending) {
     Request& q = reqs_[t];
     q.status = TVK_ERR_STATE;
+    q.error = "the server stopped before the request ran";
     q.done = true;
   }
   pv->pending.clear();
@@ -294,6 +297,7 @@ bool RenderServer::retire_locked() {
         stats_.nominal_samples += r.nominal;
       } else {
         r.status = TVK_ERR_HIP;
+        r.error = std::string("launch failed: ") + hipGetErrorString(e);
       }
       dev_arena_.free(r.dev_off, r.bytes);
       ++stats_.completed;
@@ -349,15 +353,39 @@ bool RenderServer::launch_locked(Lane& L) {
   li.n_tf_lds = shared ? jobs[0].n_tf : 0;
   const uint64_t slot_key = variant_key(li) ^ ((uint64_t)li.n_tf_lds << 32);
   auto& slot = L.slots[slot_key];
+  std::string why = ok ? std::string() : last_error();
   if (ok && !slot) {
-    slot.reset(new LaunchSlot());
-    ok = slot->alloc() == TVK_OK;
+    std::unique_ptr<LaunchSlot> fresh(new LaunchSlot());
+    const int rc = fresh->alloc();
+    if (rc == TVK_OK) {
+      slot = std::move(fresh);
+    } else {
+      bool in_flight = pool_reserved_ > 0;
+      for (const Lane& x : lanes_) in_flight |= x.busy;
+      if (rc == TVK_ERR_OOM && in_flight) {
+        // device memory is held by work in flight (a paged batch filling
+        // pool slots, other lanes): put the requests back in front and
+        // launch them once that work is done
+        for (auto it = pick.rbegin(); it != pick.rend(); ++it) {
+          Request& r = reqs_[*it];
+          dev_arena_.free(r.dev_off, r.bytes);
+          host_arena_.free(r.host_off, r.bytes);
+          r.host_held = false;
+          pending_.push_front(*it);
+        }
+        memory_wait_ = true;
+        return false;
+      }
+      ok = false;
+      why = last_error();
+    }
   }
   bool enqueued = false;  // device work of this launch may be in flight
   if (ok) {
     L.cur = slot.get();
     enqueued = true;
     ok = launch_job_table(*slot, jobs.data(), (int)jobs.size(), li, 4, 0, L.stream, true) == TVK_OK;
+    if (!ok) why = last_error();
   }
   for (size_t k = 0; k < pick.size() && ok; ++k) {
     Request& r = reqs_[pick[k]];
@@ -373,6 +401,7 @@ bool RenderServer::launch_locked(Lane& L) {
     for (uint64_t tk : pick) {
       Request& r = reqs_[tk];
       r.status = TVK_ERR_HIP;
+      r.error = why.empty() ? "launch could not be enqueued" : why;
       r.done = true;
       r.t_launch = r.t_done = t;
       dev_arena_.free(r.dev_off, r.bytes);
@@ -407,7 +436,11 @@ void RenderServer::loop() {
     for (const Lane& L : lanes_) busy |= L.busy;
     if (stop_ && !busy) break;
     if (progress) continue;
-    if (busy) {
+    if (memory_wait_) {
+      // a launch waits for device memory held by work in flight
+      memory_wait_ = false;
+      cv_work_.wait_for(lk, std::chrono::microseconds(500));
+    } else if (busy) {
       // lanes in flight: poll their events at a short interval
       cv_work_.wait_for(lk, std::chrono::microseconds(50));
     } else {
@@ -418,6 +451,7 @@ void RenderServer::loop() {
   for (uint64_t t : pending_) {
     Request& r = reqs_[t];
     r.status = TVK_ERR_STATE;
+    r.error = "the server stopped before the request ran";
     r.done = true;
   }
   pending_.clear();
@@ -458,7 +492,7 @@ int RenderServer::Wait(uint64_t ticket, uint32_t timeout_ms, tvk_request_result*
     out->queue_ms = r.t_launch > 0 ? r.t_launch - r.t_submit : 0.0;
     out->latency_ms = (copy ? now_ms() : r.t_done) - r.t_submit;
   }
-  if (r.status != TVK_OK) return fail(r.status, "request failed");
+  if (r.status != TVK_OK) return fail(r.status, "request failed" + (r.error.empty() ? "" : ": " + r.error));
   return TVK_OK;
 }
 

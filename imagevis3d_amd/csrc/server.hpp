@@ -20,6 +20,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -65,6 +66,7 @@ class RenderServer {
     uint64_t evaluated = 0, nominal = 0;
     uint32_t launch_jobs = 0;
     int status = TVK_OK;
+    std::string error;       // why it failed (status != TVK_OK)
     bool done = false;
     bool host_held = false;  // owns a pinned host arena block
     bool paged = false;      // its volume pages through the device pool
@@ -111,6 +113,7 @@ class RenderServer {
   std::deque<uint64_t> pending_;
   std::unordered_map<uint64_t, Request> reqs_;
   uint64_t next_ticket_ = 1;
+  bool memory_wait_ = false;  // the last launch was put back for device memory
   bool stop_ = false;
   std::thread worker_;
   tvk_server_stats stats_{};

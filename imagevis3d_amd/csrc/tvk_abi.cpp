@@ -26,6 +26,7 @@ namespace {
 thread_local std::string g_last_error;
 }
 void set_last_error(const std::string& msg) { g_last_error = msg; }
+const std::string& last_error() { return g_last_error; }
 }  // namespace tvk
 
 using tvk::HipRenderer;

@@ -97,6 +97,10 @@ struct RaycastParams {
   // these enter the raycast's tile queues; the background kernel writes
   // the job's other pixels.
   int32_t rect_x0, rect_y0, rect_w, rect_h;
+  // Tile sampling: with tile_sample = k > 1 the job renders one tile of
+  // each run of k consecutive rect tiles, at an offset hashed from the run
+  // index and tile_phase; the other pixels keep what the framebuffer holds.
+  int32_t tile_sample, tile_phase;
   float box[3];         // LOD-0 dims (the volume box is [0, box])
   float step;           // sample spacing in LOD-0 voxels (2^lod / rate)
   float nominal_rate;   // user rate: nominal samples per LOD-0 voxel

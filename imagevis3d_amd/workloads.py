@@ -122,6 +122,82 @@ def oracle_kwargs(w: Workload, cam: Camera, lod: int = 0, level_dims=None) -> di
                 brick_size=skip_cell(w.brick) if w.skip else 0, apron=1)
 
 
+SERVING_POOL = [((64, 64, 64), 8, 64), ((64, 64, 64), 16, 64), ((256, 256, 256), 8, 128),
+                ((256, 256, 256), 16, 128), ((512, 512, 512), 8, 128), ((512, 512, 512), 16, 128),
+                ((2048, 2048, 2048), 16, 128)]
+SERVING_SIZES = [(256, 256), (512, 512), (1920, 1080), (3840, 2160)]
+
+
+def serving_pool(pool=SERVING_POOL, seed: int = 100):
+    """The serving workload's volumes: device-generated fBm + spheres at
+    64^3, 256^3 and 512^3 in uint8 and uint16 and one 2048^3 uint16, each on
+    its own renderer with two 1D TF slots (Gaussian bumps, a ramp) and one
+    2D TF slot (Gaussian lobes), sample rate 1."""
+    from .renderer import VOL_FBM_SPHERES, tuvok
+    rs = []
+    for i, (dims, bits, brick) in enumerate(pool):
+        # the volumes span the whole 8 / 16-bit range: TF entries are spread
+        # over it (biasAndScaleTF, LuaScripts/BatchRenderer.lua:19-20)
+        r = tuvok.renderer.new(tuvok.renderer.types.OpenGL_Raycaster, False, False, False, True)
+        assert r.initialize(tuvok.createContext(256, 256, device=0)), r.lastError()
+        r.setGeneratedVolume(VOL_FBM_SPHERES, seed + i, dims, bits, brick, 2, 1)
+        n = 256 if bits == 8 else 4096
+        r.set1DTransSlot(0, transfer.gaussian_bumps_tf(n, seed=seed + 100 + i))
+        r.set1DTransSlot(1, transfer.ramp_tf(n, 0.45 * n, 0.95 * n))
+        _, mg = r.histogram2D(256, 256)
+        r.set2DTransSlot(2, transfer.gaussian_lobes_2d(256, 256, seed=seed + 200 + i), mg)
+        r.setSampleRateModifier(1.0)
+        rs.append(r)
+    return rs
+
+
+BATCHED_DIMS, BATCHED_BRICK, BATCHED_LODS = (2048, 2048, 1024), 128, 5
+
+
+def batched_renderer(vol, max_grad: float, pool_bytes: int, brick: int = BATCHED_BRICK,
+                     n_lod: int = BATCHED_LODS):
+    """The out-of-core batched workload's renderer: `vol` (u16, fBm +
+    layered slabs) paged from host memory through a device pool of
+    `pool_bytes`, 4 seeded 256x256 Gaussian-lobe 2D TFs over value x
+    gradient magnitude (value axis over the data range), Phong lighting,
+    sample rate 1, finest LOD."""
+    from .renderer import RenderMode, tuvok
+    r = tuvok.renderer.new(tuvok.renderer.types.OpenGL_Raycaster, False, False, False, True)
+    assert r.initialize(tuvok.createContext(64, 64, device=0)), r.lastError()
+    r.setPagedFlatVolume(vol, brick, pool_bytes, n_lod=n_lod)
+    r.setRenderMode(RenderMode.RM_2DTRANS)
+    for k in range(4):
+        r.set2DTransSlot(k, transfer.gaussian_lobes_2d(256, 256, seed=40 + k), max_grad)
+    r.setLightingEnabled(True)
+    r.setSampleRateModifier(1.0)
+    r.setLODLimits((0, 0))
+    return r
+
+
+def batched_views(dims=BATCHED_DIMS, width: int = 1920, height: int = 1080, n_cams: int = 8):
+    """8 orbit cameras (60 degree FoV, 25 degree elevation) x 4 TF slots."""
+    ext = tuple(d / max(dims) for d in dims)
+    cams = orbit_cameras(n_cams, fov=60.0, elevation_deg=25.0, extent=ext)
+    return [dict(view_pos=c.view_pos, view_dir=c.view_dir, up_dir=c.up_dir, fov=c.fov,
+                 width=width, height=height, tf_slot=k) for k in range(4) for c in cams]
+
+
+def serving_request(rng, n_vol: int, sizes=SERVING_SIZES):
+    """One random request of the serving workload: (volume, view, mode) with
+    a camera looking at the volume from distance U(1.5, 2.4) (50 degree FoV),
+    an image size from `sizes`, and with probability 1/2 the 2D TF (slot 2),
+    else a 1D TF (slot 0 or 1)."""
+    from .renderer import RenderMode
+    th, ph = rng.uniform(0, 2 * math.pi), rng.uniform(-1.3, 1.3)
+    d = rng.uniform(1.5, 2.4)
+    eye = (d * math.cos(ph) * math.sin(th), d * math.sin(ph), d * math.cos(ph) * math.cos(th))
+    w, h = sizes[rng.integers(len(sizes))]
+    two_d = rng.uniform() < 0.5
+    view = dict(view_pos=eye, view_dir=tuple(-x for x in eye), fov=50.0, width=w, height=h,
+                tf_slot=2 if two_d else int(rng.integers(2)))
+    return int(rng.integers(n_vol)), view, RenderMode.RM_2DTRANS if two_d else RenderMode.RM_1DTRANS
+
+
 def view_of(cam: Camera, width: int, height: int, tf_slot: int = 0) -> dict:
     """One entry of a renderBatch / serving view list for a camera."""
     return dict(view_pos=cam.view_pos, view_dir=cam.view_dir, up_dir=cam.up_dir, fov=cam.fov,

@@ -9,7 +9,14 @@
 #include <omp.h>
 #endif
 
-uint64_t baseline_render_1d_f32(const oracle_render_params* p, uint8_t* out8, int32_t n_threads) {
+/* Built twice (Makefile): once for AVX2 + FMA and once for AVX-512; the
+ * entry point baseline_render_1d_f32 (baseline_dispatch.c) picks the one
+ * the host's cores run. */
+#ifndef BASELINE_FN
+#define BASELINE_FN baseline_render_1d_f32_avx2
+#endif
+
+uint64_t BASELINE_FN(const oracle_render_params* p, uint8_t* out8, int32_t n_threads) {
   cam_ctx c;
   if (cam_setup(p, &c)) return 0;
   int nb[3] = {1, 1, 1};
@@ -62,7 +69,24 @@ uint64_t baseline_render_1d_f32(const oracle_render_params* p, uint8_t* out8, in
               int b0 = clampi((int)floorf(q0 / B), 0, nb[0] - 1);
               int b1 = clampi((int)floorf(q1 / B), 0, nb[1] - 1);
               int b2 = clampi((int)floorf(q2 / B), 0, nb[2] - 1);
-              if (empty[(b2 * nb[1] + b1) * nb[0] + b0]) continue;
+              if (empty[(b2 * nb[1] + b1) * nb[0] + b0]) {
+                /* jump to the last sample inside this empty brick: its exit
+                 * face along the ray (in LOD-0 voxels) */
+                const int bb[3] = {b0, b1, b2};
+                float texit = INFINITY;
+                for (int i = 0; i < 3; ++i) {
+                  if (d[i] == 0.0f) continue;
+                  const float face = (float)((d[i] > 0.0f ? bb[i] + 1 : bb[i]) * B) / lsc[i];
+                  const float tf = (face - o[i]) / d[i];
+                  texit = tf < texit ? tf : texit;
+                }
+                /* samples before index ceil(x) - 1 have their midpoints
+                 * inside; stop one short so rounding never skips a sample
+                 * of the next brick (the rest are tested one by one) */
+                const int kl = (int)ceilf((texit - ta) / step - 0.5f) - 2;
+                if (kl > k) k = kl < n - 1 ? kl : n - 1;
+                continue;
+              }
             }
             float v;
             if (p->interp) {

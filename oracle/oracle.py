@@ -76,6 +76,8 @@ def lib():
         L.oracle_nominal_samples.restype = C.c_uint64
         L.baseline_render_1d_f32.argtypes = [P, C.c_void_p, C.c_int32]
         L.baseline_render_1d_f32.restype = C.c_uint64
+        L.baseline_uses_avx512.argtypes = []
+        L.baseline_uses_avx512.restype = C.c_int
         L.oracle_downsample.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
                                         C.c_int32, C.c_void_p]
         L.oracle_downsample_median.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32,

@@ -44,9 +44,10 @@ def test_clip_plane_parity(plane):
         ref8, ns, fl = oracle_ref(vol, w, cam, w.tf, clip_plane=plane)
         assert_parity(img, ref8, fl, f"clip {plane} view {view}")
         assert abs(int(st.evaluated_samples) - int(ns.sum())) <= 2e-3 * ns.sum() + 64
-        # nominal samples ignore the plane
-        assert st.nominal_samples == O.nominal_samples(
-            O.make_params(vol, w.tf, **workloads.oracle_kwargs(w, cam)))
+        # nominal samples ignore the plane (FP32 vs FP64 floor(chord x rate)
+        # may differ by one sample on a few pixels)
+        nom = O.nominal_samples(O.make_params(vol, w.tf, **workloads.oracle_kwargs(w, cam)))
+        assert abs(int(st.nominal_samples) - nom) <= 1e-5 * nom + 4
     r.enableClipPlane(False)
     img, _ = render(r, w, w.cameras[3])
     ref8, _, fl = oracle_ref(vol, w, w.cameras[3], w.tf)

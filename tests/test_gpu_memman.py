@@ -63,7 +63,8 @@ def test_renderers_share_the_pool_and_page_each_other_out():
     with Budget(a):
         # room for about 100 slots: either volume's view fits, both do not
         s0 = memman_stats(0)
-        a.setMaxUsableGPUMem(s0.gpu_allocated + 100 * sb + 4096)
+        # other volumes' pool chunks give way to the budget first
+        a.setMaxUsableGPUMem(s0.gpu_allocated - s0.pool_bytes + 100 * sb + 4096)
         views = [0, 9, 18, 27]
         for rnd in range(2):
             for v in views:
@@ -93,8 +94,9 @@ def test_a_slot_size_in_need_takes_another_sizes_chunk():
     ref.set1DTrans(w.tf)
     with Budget(a):
         s0 = memman_stats(0)
-        # A's bricks fill the budget; C (another slot size) needs it next
-        a.setMaxUsableGPUMem(s0.gpu_allocated + 70 * sa)
+        # the budget holds all of C's 512 bricks (another slot size), but
+        # A's chunk takes the room first: C's frame needs it back
+        a.setMaxUsableGPUMem(s0.gpu_allocated - s0.pool_bytes + 512 * sc + (64 << 10))
         frame(a, w, w.cameras[4])
         before = memman_stats(0).chunks_freed
         got = frame(c, w, w.cameras[4])
@@ -170,7 +172,7 @@ def test_budget_refuses_what_cannot_fit():
     r = renderer(w)
     with Budget(r):
         s0 = memman_stats(0)
-        r.setMaxUsableGPUMem(s0.gpu_allocated + (1 << 20))
+        r.setMaxUsableGPUMem(s0.gpu_allocated - s0.pool_bytes + (1 << 20))
         with pytest.raises(TvkError, match="(?i)memory|budget"):
             r.setVolume(vol, 32, 2, 1)  # 512 KiB of voxels + bricks do not fit in 1 MiB
     r.setVolume(vol, 32, 2, 1)  # the restored budget holds it

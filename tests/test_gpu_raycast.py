@@ -337,12 +337,20 @@ def test_progressive_lod_subframes():
 
 
 def test_memory_budget_is_enforced():
-    r = gpu_renderer()
-    r.setMaxUsableGPUMem(1 << 20)
+    """The budget is the device's (every renderer there shares it): a volume
+    larger than it is refused with TVK_ERR_OOM; the budget is restored."""
     from imagevis3d_amd import TvkError
-    with pytest.raises(TvkError) as e:
-        r.setVolume(np.zeros((128, 128, 128), np.uint8), 64, 2, 1)
-    assert e.value.code == -5
-    ms = r.memStats()
-    assert ms.gpu_mem > (1 << 30) and ms.gpu_budget == 1 << 20
+    from imagevis3d_amd.renderer import memman_stats
+    r = gpu_renderer()
+    before = memman_stats(0).gpu_budget
+    r.setMaxUsableGPUMem(1 << 20)
+    try:
+        with pytest.raises(TvkError) as e:
+            r.setVolume(np.zeros((128, 128, 128), np.uint8), 64, 2, 1)
+        assert e.value.code == -5
+        ms = r.memStats()
+        assert ms.gpu_mem > (1 << 30) and ms.gpu_budget == 1 << 20
+        assert memman_stats(0).gpu_budget == 1 << 20
+    finally:
+        r.setMaxUsableGPUMem(before)
     r.cleanup()

@@ -96,3 +96,99 @@ def test_admission_waits_for_output_memory():
     assert srv.stats().completed == 12
     srv.close()
     vols[0][0].cleanup()
+
+
+def test_paged_volumes_served_and_admitted_by_pool_memory():
+    """Two paged volumes and an in-core one behind one server.  The GPU
+    budget leaves the pool room for one paged batch's working set but not
+    two, so paged batches never hold more pool memory at once than the
+    larger working set; every served image equals the renderer's own render
+    of the view and, for paged volumes, the in-core render."""
+    from imagevis3d_amd import brick_slot_bytes
+    from imagevis3d_amd.renderer import generate_volume_gpu, memman_stats
+    print("case serving paged volumes, pool admission", flush=True)
+    sb = brick_slot_bytes(32, 2, 16)
+    specs = [((256, 256, 256), 21, 512), ((192, 192, 192), 22, 216)]
+    paged, incore = [], []
+    for dims, seed, n in specs:
+        vol = generate_volume_gpu(VOL_FBM_SPHERES, seed, dims, 16)
+        tf = transfer.gaussian_bumps_tf(4096, seed=seed)
+        tf2 = transfer.gaussian_lobes_2d(256, 256, seed=seed)
+        for kind in ("paged", "incore"):
+            r = gpu_renderer(width=64, height=64)
+            if kind == "paged":
+                r.setPagedFlatVolume(vol, 32, n * sb, n_lod=1)
+            else:
+                r.setVolume(vol, 32, 2, 1)
+            r.set1DTransSlot(0, tf)
+            r.set1DTransSlot(1, transfer.ramp_tf(4096, 1500, 3800))
+            r.set2DTransSlot(2, tf2, 0.05)
+            r.setSampleRateModifier(1.25)
+            (paged if kind == "paged" else incore).append((r, vol))
+    other = make_volumes()[0][0]
+    srv = RenderServer(arena_bytes=64 << 20, lanes=2)
+    ids = [srv.addVolume(r) for r, _ in paged] + [srv.addVolume(other)]
+    need = [n * sb for _, _, n in specs]
+    s = memman_stats(0)
+    budget = s.gpu_budget
+    paged[0][0].setMaxUsableGPUMem(s.gpu_allocated - s.pool_bytes + max(need) + (16 << 20))
+    try:
+        rng = np.random.default_rng(3)
+        reqs = []
+        for i in range(30):
+            v = i % 3
+            _, view, mode = workloads.serving_request(rng, 1, sizes=[(160, 96), (200, 120), (96, 96)])
+            reqs.append((v, view, mode))
+        tickets = [srv.submit(ids[v], view, mode) for v, view, mode in reqs]
+        mixed = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+        print(f"paged launches {st.paged_launches} pool waits {st.pool_waits} "
+              f"max reserved {st.max_pool_reserved} need {need}")
+        assert st.completed == 30 and st.paged_launches >= 2
+        assert 0 < st.max_pool_reserved <= max(need)  # never two paged batches at once
+    finally:
+        paged[0][0].setMaxUsableGPUMem(budget)
+    srv.close()
+    for i, (v, view, mode) in enumerate(reqs):
+        assert mixed[i][1].status == 0
+        r = paged[v][0] if v < 2 else other
+        r.setRenderMode(mode)
+        alone, st_alone, _ = r.renderBatch([view])
+        assert np.array_equal(alone[0], mixed[i][0]), f"request {i} (volume {v})"
+        assert st_alone[0].evaluated_samples == mixed[i][1].evaluated_samples
+        if v < 2:
+            ic = incore[v][0]
+            ic.setRenderMode(mode)
+            ref, _, _ = ic.renderBatch([view])
+            assert np.array_equal(ref[0], mixed[i][0]), f"request {i} vs in-core"
+    for r, _ in paged + incore:
+        r.cleanup()
+    other.cleanup()
+
+
+def test_serving_mix_at_full_size_same_alone_and_in_the_mix():
+    """The serving workload's 7-volume pool (64^3 ... 2048^3, u8 / u16) with
+    64 requests of 256^2 ... 3840x2160, 1D and 2D TFs, submitted at once:
+    every image and sample count equals the same request served alone."""
+    print("case serving mix, 7 volumes incl. 2048^3, 64 requests up to 4K", flush=True)
+    rs = workloads.serving_pool()
+    srv = RenderServer(arena_bytes=4 << 30, lanes=4)
+    ids = [srv.addVolume(r) for r in rs]
+    rng = np.random.default_rng(5)
+    reqs = [workloads.serving_request(rng, len(ids)) for _ in range(64)]
+    assert {v for v, _, _ in reqs} == set(range(len(ids)))
+    assert {(view["width"], view["height"]) for _, view, _ in reqs} == set(workloads.SERVING_SIZES)
+    tickets = [srv.submit(ids[v], view, mode) for v, view, mode in reqs]
+    mixed = [srv.wait(t) for t in tickets]
+    st = srv.stats()
+    print(f"launches {st.launches} max jobs {st.max_jobs_per_launch} max lanes {st.max_lanes_busy}",
+          flush=True)
+    assert st.completed == 64 and st.launches < 64
+    for i, (v, view, mode) in enumerate(reqs):
+        img, res = srv.wait(srv.submit(ids[v], view, mode))
+        assert res.status == 0 and res.launch_jobs == 1
+        assert np.array_equal(img, mixed[i][0]), f"request {i}"
+        assert res.evaluated_samples == mixed[i][1].evaluated_samples
+    srv.close()
+    for r in rs:
+        r.cleanup()

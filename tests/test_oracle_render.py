@@ -202,3 +202,27 @@ def test_fp32_baseline_tracks_fp64_oracle():
     assert d[flags == 0].max() <= 2
     assert np.sqrt((d ** 2).mean()) <= 0.5
     assert abs(n - int(ns.sum())) <= 1e-3 * ns.sum()
+
+
+@pytest.mark.parametrize("bits", [8, 16])
+def test_fp32_baseline_skips_empty_spans_exactly(bits):
+    """A sparse volume (content in a few 16^3 cells) seen in perspective:
+    the baseline jumps across empty cells and still evaluates the samples
+    the oracle evaluates (same brick-granular rule) with the same image."""
+    n = 96
+    rng = np.random.default_rng(bits)
+    dt = np.uint8 if bits == 8 else np.uint16
+    vol = np.zeros((n, n, n), dt)
+    for _ in range(6):
+        z, y, x = rng.integers(0, n - 20, 3)
+        vol[z:z + 18, y:y + 14, x:x + 20] = rng.integers(60, 250, (18, 14, 20))
+    tf = tf_const(256 if bits == 8 else 4096, (0.8, 0.5, 0.2, 0.0))
+    tf[50 if bits == 8 else 60:, 3] = 0.08
+    p = O.make_params(vol, tf, width=160, height=120, rate=1.7, view_pos=(0.9, 0.6, 1.4),
+                      view_dir=(-0.9, -0.6, -1.4), fov=45.0, brick_size=16, apron=1)
+    ref, ns, flags = O.render(p)
+    img, cnt = O.baseline_render(p, 4)
+    d = np.abs(O.to_rgba8(ref).astype(int) - img.astype(int))
+    assert d.max() <= 2 and np.sqrt((d ** 2).mean()) <= 0.5
+    assert abs(cnt - int(ns.sum())) <= 1e-3 * ns.sum() + 16
+    assert 0 < ns.sum() < 0.5 * O.nominal_samples(p)  # most of each ray was skipped

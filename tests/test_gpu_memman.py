@@ -94,9 +94,10 @@ def test_a_slot_size_in_need_takes_another_sizes_chunk():
     ref.set1DTrans(w.tf)
     with Budget(a):
         s0 = memman_stats(0)
-        # the budget holds all of C's 512 bricks (another slot size), but
-        # A's chunk takes the room first: C's frame needs it back
-        a.setMaxUsableGPUMem(s0.gpu_allocated - s0.pool_bytes + 512 * sc + (64 << 10))
+        # the budget holds all of C's 512 bricks (another slot size) and its
+        # first paint's TF upload (80 KB), but A's chunk takes the room
+        # first: C's frame needs it back
+        a.setMaxUsableGPUMem(s0.gpu_allocated - s0.pool_bytes + 512 * sc + (1 << 20))
         frame(a, w, w.cameras[4])
         before = memman_stats(0).chunks_freed
         got = frame(c, w, w.cameras[4])

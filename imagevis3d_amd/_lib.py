@@ -183,6 +183,7 @@ SIGNATURES = {
     "tvk_set_light_colors": [H, P, P, P, P],
     "tvk_set_early_ray_termination": [H, f32],
     "tvk_set_empty_space_skipping": [H, i32],
+    "tvk_render_batch_noalloc": [H, P, u32, P, P],
     "tvk_set_clip_plane": [H, P],
     "tvk_enable_clip_plane": [H, i32],
     "tvk_is_clip_plane_enabled": [H, C.POINTER(i32)],

@@ -32,6 +32,7 @@ SOURCES = [
     "volume_gen.cpp",
     "volume_gen_gpu.hip",
     "lod_mlp.hip",
+    "noalloc.hip",
     "tvk_abi.cpp",
 ]
 

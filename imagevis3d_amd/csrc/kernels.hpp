@@ -122,6 +122,10 @@ hipError_t launch_raycast(const LaunchInfo& li, const JobTable* tab, hipStream_t
 // strides over all jobs' pixels).
 constexpr int kBackgroundBlocks = 1024;
 hipError_t launch_background(const JobTable* tab, hipStream_t stream);
+// The allocation-free raycast's device scratch (noalloc.hip): init once per
+// device with that device current; then the device's job table and counters.
+hipError_t noalloc_scratch_init(int device);
+hipError_t noalloc_scratch(int device, JobTable** table, unsigned long long** counters);
 // *out += sum over the n pixels of two RGBA8 images of the squared RGB
 // differences (integer units)
 hipError_t launch_rgb_sse(const uint32_t* a, const uint32_t* b, int64_t n, unsigned long long* out,

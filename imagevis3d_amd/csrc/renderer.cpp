@@ -260,6 +260,7 @@ int HipRenderer::Initialize(int device) {
   TVK_HIP(hipSetDevice(device_));
   TVK_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   TVK_HIP(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking));
+  TVK_HIP(noalloc_scratch_init(device_));  // this device's allocation-free launch scratch
   for (auto& sl : paint_slots_)
     if (int e = sl.alloc()) return e;
   if (int e = batch_slot_.alloc()) return e;
@@ -1874,6 +1875,79 @@ int HipRenderer::RenderBatch(const tvk_view* views, uint32_t n, tvk_view_stats* 
     TVK_HIP(hipStreamSynchronize(stream_));
   }
   if (kernel_ms) *kernel_ms = kms;
+  return TVK_OK;
+}
+
+// Would upload_tf / prepare_cells need a new device buffer before a launch
+// of these views in `mode`?  (They only copy and launch otherwise.)
+bool HipRenderer::launch_ready_without_alloc(const tvk_view* views, uint32_t n, int mode) const {
+  for (const TfSet& t : tfs_) {
+    if (t.dirty1d) {
+      const size_t m = t.tf1d.size() / 4;
+      if (t.tf1d_dev.bytes < m * 16 || t.prefix1d_dev.bytes < (m + 1) * 4) return false;
+    }
+    if (t.dirty2d) {
+      const size_t W = t.w2d, H = t.h2d;
+      if (t.tf2d_dev.bytes < W * H * 16 || t.prefix2d_dev.bytes < (W + 1) * 4) return false;
+    }
+  }
+  if (skip_ && total_cells_ > 0)
+    for (uint32_t i = 0; i < n; ++i)
+      if (tfs_[views[i].tf_slot].cell_empty[mode].bytes < (size_t)total_cells_) return false;
+  return true;
+}
+
+int HipRenderer::RenderBatchNoAlloc(const tvk_view* views, uint32_t n, void* const* dev_rgba,
+                                    tvk_view_stats* stats) {
+  if (!initialized_) return state_fail("render before initialize");
+  if (!has_volume_) return state_fail("render before a dataset is loaded");
+  if (n > (uint32_t)kMaxJobs)
+    return arg_fail("at most " + std::to_string(kMaxJobs) +
+                    " views per allocation-free launch (its device scratch holds one job table)");
+  if (n == 0) return TVK_OK;
+  if (!views || !dev_rgba) return arg_fail("null view or framebuffer array");
+  if (paged_) return state_fail("the allocation-free launch renders in-core volumes (paging takes pool memory)");
+  if (regions_active()) return state_fail("per-region LOD is not available allocation-free");
+  // fixed-size arrays: no heap memory either
+  CameraState cams[kMaxJobs];
+  RaycastParams jobs[kMaxJobs];
+  for (uint32_t i = 0; i < n; ++i) {
+    if (int e = view_camera(views[i], i, &cams[i])) return e;
+    if (int e = check_tf((int)views[i].tf_slot)) return e;
+    if (!dev_rgba[i]) return arg_fail("null device framebuffer for view " + std::to_string(i));
+  }
+  if (!launch_ready_without_alloc(views, n, mode_index()))
+    return state_fail("a TF slot of these views is not on the device for this render mode yet: render it "
+                      "once with tvk_render_batch or tvk_paint first");
+  TVK_HIP(hipSetDevice(device_));
+  if (int e = upload_tf()) return e;
+  for (uint32_t i = 0; i < n; ++i)
+    if (int e = prepare_cells((int)views[i].tf_slot, mode_index())) return e;
+  JobTable* dev_table = nullptr;
+  unsigned long long* dev_counters = nullptr;
+  TVK_HIP(noalloc_scratch(device_, &dev_table, &dev_counters));
+  const int lo = std::min<int>((int)lod_min_, n_lod_ - 1);
+  bool same_tf = true;
+  for (uint32_t k = 1; k < n; ++k) same_tf = same_tf && views[k].tf_slot == views[0].tf_slot;
+  const LaunchInfo li = launch_info(same_tf ? (int)(tfs_[views[0].tf_slot].tf1d.size() / 4) : 0);
+  for (uint32_t i = 0; i < n; ++i)
+    build_params(&jobs[i], lo, cams[i], views[i].width, views[i].height, (int)views[i].tf_slot,
+                 (uint32_t*)dev_rgba[i]);
+  // the pinned table of the batch slot may still feed its last upload
+  if (batch_slot_.used) TVK_HIP(hipEventSynchronize(batch_slot_.ev[3]));
+  batch_slot_.used = true;
+  if (int e = launch_job_table_at(batch_slot_.table_h, dev_table, dev_counters, stats ? batch_slot_.counters_h : nullptr,
+                                  batch_slot_.ev, jobs, (int)n, li, tile_group_, xcd_split_, stream_))
+    return e;
+  if (stats) {
+    TVK_HIP(hipEventSynchronize(batch_slot_.ev[3]));
+    for (uint32_t k = 0; k < n; ++k) {
+      stats[k].evaluated_samples = batch_slot_.counters_h[kQueues + 2 * k];
+      stats[k].nominal_samples = batch_slot_.counters_h[kQueues + 2 * k + 1];
+      stats[k].lod = (uint32_t)lo;
+      stats[k].reserved = 0;
+    }
+  }
   return TVK_OK;
 }
 

@@ -259,6 +259,9 @@ class HipRenderer : public PoolClient {
   // Many views (camera, size, TF slot) of this volume in one launch.
   int RenderBatch(const tvk_view* views, uint32_t n, tvk_view_stats* stats, float* kernel_ms);
   int BatchFramebuffer(uint32_t index, void** dptr);
+  // RenderBatch into caller device framebuffers with no allocation: the job
+  // table and counters in the device's fixed scratch (tvk_render_batch_noalloc)
+  int RenderBatchNoAlloc(const tvk_view* views, uint32_t n, void* const* dev_rgba, tvk_view_stats* stats);
 
   // Serving (RenderServer): ServePrepare checks the TF slot for `mode`,
   // uploads TFs and builds the slot's empty-cell flags (device work on this
@@ -300,6 +303,7 @@ class HipRenderer : public PoolClient {
  private:
   int check_ready() const;
   int check_tf(int slot) const;
+  bool launch_ready_without_alloc(const tvk_view* views, uint32_t n, int mode) const;
   int view_camera(const tvk_view& v, uint32_t index, CameraState* c) const;
   int upload_tf();
   void build_params(RaycastParams* p, int lod, const CameraState& cam, uint32_t w, uint32_t h,

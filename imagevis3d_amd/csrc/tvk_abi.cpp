@@ -632,6 +632,12 @@ int tvk_set_empty_space_skipping(tvk_renderer* r, int on) {
   return r->impl.SetEmptySpaceSkipping(on != 0);
 }
 
+int tvk_render_batch_noalloc(tvk_renderer* r, const tvk_view* views, uint32_t n, void* const* device_rgba,
+                             tvk_view_stats* stats) {
+  TVK_GUARD(r);
+  return r->impl.RenderBatchNoAlloc(views, n, device_rgba, stats);
+}
+
 int tvk_set_clip_plane(tvk_renderer* r, const float plane[4]) {
   TVK_GUARD(r);
   return r->impl.SetClipPlane(plane);

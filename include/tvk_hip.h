@@ -519,6 +519,24 @@ int tvk_render_batch(tvk_renderer* r, const tvk_view* views, uint32_t n, tvk_vie
                      float* kernel_ms);
 /* Device pointer of view `index` of the last batch (RGBA8, row 0 at top). */
 int tvk_get_batch_framebuffer(tvk_renderer* r, uint32_t index, void** dptr);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  tvk_render_batch for n views rendered into
+ * caller-provided device framebuffers (device_rgba[i]: width*height*4 bytes
+ * of RGBA8, row 0 at the top; views[i].rgba is ignored) on the renderer's
+ * stream, allocating nothing: the launch's job table and counters live in a
+ * fixed device scratch array of 64 KiB per GPU, which each GPU the library
+ * runs on holds until the process ends.  Limits: n <= 64 (TVK_ERR_ARG
+ * beyond); in-core volumes; no per-region LOD; every TF slot the views use
+ * must already be on the device for the current render mode (render it once
+ * with tvk_render_batch or tvk_paint), else TVK_ERR_STATE.  stats NULL:
+ * returns once the work is queued (tvk_synchronize waits for it); else
+ * waits and fills the per-view sample counts.  Every call on a GPU shares
+ * that GPU's scratch, which keeps the last call's table: calls whose work
+ * overlaps on one GPU (other threads, or other renderers' streams before
+ * their work is done) race. */
+int tvk_render_batch_noalloc(tvk_renderer* r, const tvk_view* views, uint32_t n, void* const* device_rgba,
+                             tvk_view_stats* stats);
 /* ---------------------------------------------------------- per-region LOD */
 /* Per-region LOD (no direct counterpart in the reference, whose LOD choice
  * is per frame: setLODLimits, RenderWindow.cpp:447-451; this is what the

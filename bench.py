@@ -151,6 +151,17 @@ def run_serving(duration_s: float, rate_hz: float, seed: int = 5) -> dict:
     st = srv.stats()
     lat = np.array([r.latency_ms for r in results])
     nominal = sum(r.nominal_samples for r in results)
+
+    def by(key):
+        # latency percentiles per request class
+        groups = {}
+        for (vid, view, mode), x in zip(reqs, lat):
+            groups.setdefault(key(vid, view, mode), []).append(x)
+        return {k: {"n": len(v), "p50_ms": round(float(np.percentile(v, 50)), 2),
+                    "p99_ms": round(float(np.percentile(v, 99)), 2)}
+                for k, v in sorted(groups.items())}
+
+    vol_name = {vid: f"{d[0]}^3-u{bits}" for vid, (d, bits, _) in zip(ids, workloads.SERVING_POOL)}
     out = {
         "workload": f"serving-poisson{rate_hz:g}hz-{duration_s:g}s-7vols",
         "requests": n_req, "offered_rate_hz": rate_hz, "completed_rate_hz": n_req / wall,
@@ -159,6 +170,11 @@ def run_serving(duration_s: float, rate_hz: float, seed: int = 5) -> dict:
         "gsamples_per_s": nominal / wall / 1e9, "launches": int(st.launches),
         "max_jobs_per_launch": int(st.max_jobs_per_launch), "max_lanes_busy": int(st.max_lanes_busy),
         "volume_setup_s": round(setup_s, 2),
+        # the tail by class: the heaviest class (4K on the 2048^3 volume)
+        # sets p99 by its own render time
+        "latency_by_image": by(lambda vid, view, mode: f"{view['width']}x{view['height']}"),
+        "latency_by_volume_and_tf": by(
+            lambda vid, view, mode: vol_name[vid] + ("-2d" if mode == RenderMode.RM_2DTRANS else "-1d")),
         "note": "image in pinned host memory counts as done; images 256^2..3840x2160",
     }
     srv.close()
@@ -173,7 +189,9 @@ def run_batched(batches: int = 3, seed: int = 3) -> dict:
     into a 64 GB device brick pool (128^3 bricks + 2-voxel apron, 5 LODs);
     4 seeded 2D TFs (256x256 Gaussian lobes over value x gradient
     magnitude) with Phong shading; 8 orbit cameras x 4 TFs = 32 views at
-    1920x1080 per persistent-kernel launch; launch shape autotuned."""
+    1920x1080 per persistent-kernel launch; brick size (32^3 ... 256^3,
+    the paged volume set up again from the host copy for each) and launch
+    shape autotuned on the 32 views."""
     from imagevis3d_amd import tuvok, workloads
     from imagevis3d_amd.renderer import VOL_FBM_PLANES
 
@@ -195,7 +213,9 @@ def run_batched(batches: int = 3, seed: int = 3) -> dict:
     _, st0, _ = r.renderBatch(views, readback=False)  # pages the view set's bricks in
     page_s = time.perf_counter() - t0
     paged_bytes, paging_host_ms = r.pagingStats()
-    t = r.autotune(views, flags=r.TUNE_LAUNCH)
+    t0 = time.perf_counter()
+    t = r.autotune(views, flags=r.TUNE_BRICK | r.TUNE_LAUNCH)
+    tune_s = time.perf_counter() - t0
     kms, nominal, evaluated = [], 0, 0
     t0 = time.perf_counter()
     for _ in range(batches):
@@ -216,8 +236,11 @@ def run_batched(batches: int = 3, seed: int = 3) -> dict:
         "evaluated_gsamples_per_s": evaluated / wall / 1e9,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS},
-        "tuned": {"samples_per_batch": t.samples_per_batch, "tile_width": t.tile_width,
-                  "tile_group": t.tile_group, "work_group_waves": t.wg_waves},
+        "tuned": {"brick": t.brick_size, "samples_per_batch": t.samples_per_batch,
+                  "tile_width": t.tile_width, "tile_group": t.tile_group,
+                  "work_group_waves": t.wg_waves, "tuned_ms": round(t.kernel_ms, 2),
+                  "start_ms": round(t.baseline_ms, 2), "settings_timed": t.candidates,
+                  "seconds": round(tune_s, 1)},
         "pool": {"slots": int(ms.pool_slots), "resident": int(ms.resident_bricks),
                  "uploaded": int(ms.bricks_uploaded)},
         "setup_s": {"generate": round(gen_s, 2), "host_pyramid": round(plan_s, 2),
@@ -244,12 +267,17 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
     calibrated on `cal_views_n` views (mean RMSE `cal_target`/255), then 24
     new random views at 3840x2160 with a new TF are rendered at full
     resolution and with the learned per-region LOD at two operating points:
-    the error target alone ("target") and under a brick budget of a quarter
-    of the LOD-0 bricks, as 16 GB of a 64 GiB volume ("budgeted"), and at
-    1/2, 1/4 and 1/8 of the calibrated error budget ("target_curve": how
-    far the budget must drop for every frame, not only their mean, to stay
-    within 1/255).  Frame time of the learned path includes features, MLP
-    and selection."""
+    the error target alone ("target") and under the 16 GB brick budget
+    ("budgeted"), and at 1/2, 1/4 and 1/8 of the calibrated error budget
+    ("target_curve": how far the budget must drop for every frame, not only
+    their mean, to stay within 1/255).  "checked" holds every frame to
+    1/255 at run time: the per-frame check estimates each learned frame's
+    RMSE from a sample of level-0 tiles and renders the frame at level 0
+    above the bound; its frame rate is that of a frame loop in which the
+    host selects the next frame while the device renders the current one.
+    "out_of_core" replays the budgeted frames from a 16 GB paged pool over a
+    host-memory copy of the volume.  Frame time of the learned path
+    includes features, MLP and selection."""
     import math
 
     import numpy as np
@@ -291,8 +319,9 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
     r.set1DTrans(tf)
     mom = LL.tf_moments(hist, tf)
     slot_bytes = brick_slot_bytes(edge, 2, 8)
-    n_reg = (n // edge) ** 3
-    mem_budget = n_reg * slot_bytes // 4
+    # the brick budget of the configuration: 16 GB of device slots (padded
+    # with their apron and blocked layout) for the chosen levels' bricks
+    mem_budget = 16 * 10**9
     cal_views = views(cal_views_n, 12, *eval_size)
     # two operating points: under the brick budget, and with the error
     # target alone (no brick cap); the budget factor is calibrated so that
@@ -301,9 +330,76 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
     cal = {k: LL.calibrate_safety(r, pred, cal_views, dims, edge, n_lod, hist, mom, cal_target,
                                   slot_bytes, mb, detail, reduce="mean")
            for k, mb in points.items()}
+    # The per-frame check: each learned frame also renders one ray tile in
+    # check_k at level 0, estimates its own RMSE from them and, above
+    # check_bound/255, is rendered again at level 0.  The budget-factor
+    # scale it runs at is picked on the calibration views (the fastest
+    # whose frames all end within 1/255), then measured on new views.
+    check_k, check_bound = 32, 0.9
+    cal_refs = []
+    for v in cal_views:
+        r.resize((v["width"], v["height"]))
+        LL.set_view(r, v)
+        cal_refs.append(LL.full_resolution(r, dims, edge))
+
+    def checked_accuracy(vs, refs, scale):
+        """Per-frame RMSE vs level 0 and fallbacks, one capture per view."""
+        r.setRegionLODCheck(check_k, check_bound)
+        errs, est, fell = [], [], 0
+        for v, ref in zip(vs, refs):
+            r.resize((v["width"], v["height"]))
+            LL.set_view(r, v)
+            lev, _, _ = LL.choose_levels(pred, v, dims, edge, n_lod, hist, mom, cal["target"][0] * scale,
+                                         slot_bytes, 0, detail)
+            r.setRegionLODs(lev)
+            img = r.capture()
+            c = r.regionLODCheck()
+            fell += c.fell_back
+            est.append(c.rmse_estimate)
+            errs.append(LL.rmse255(img, ref))
+            r.setRegionLODs(None)
+        r.setRegionLODCheck(0, 0.0)
+        return errs, est, fell
+
+    def pipelined_ms(vs, scale=None):
+        """Wall time per frame of a frame loop in which the host selects the
+        next frame's levels while the device renders the current one
+        (scale None: every frame at full resolution, no selection)."""
+        r.resize((vs[0]["width"], vs[0]["height"]))
+        if scale is not None:
+            r.setRegionLODCheck(check_k, check_bound)
+        r.synchronize()
+        t0 = time.perf_counter()
+        for v in vs:
+            LL.set_view(r, v)
+            if scale is not None:
+                lev, _, _ = LL.choose_levels(pred, v, dims, edge, n_lod, hist, mom,
+                                             cal["target"][0] * scale, slot_bytes, 0, detail)
+                r.setRegionLODs(lev)
+            else:
+                r.setRegionLODs(zero_regions)
+            assert r.paint(), r.lastError()
+        r.synchronize()
+        ms = (time.perf_counter() - t0) / len(vs) * 1e3
+        r.setRegionLODCheck(0, 0.0)
+        r.setRegionLODs(None)
+        return ms
+
+    zero_regions = np.zeros((n // edge) ** 3, np.uint8)
+    check_scales = (1.0, 0.5, 0.25)
+    check_cal = {}
+    for s in check_scales:
+        errs, _, fell = checked_accuracy(cal_views, cal_refs, s)
+        check_cal[s] = {"rmse_max": float(np.max(errs)), "fallbacks": fell,
+                        "ms": pipelined_ms(cal_views, s)}
+    ok = [s for s in check_scales if check_cal[s]["rmse_max"] <= 1.0]
+    check_scale = min(ok, key=lambda s: check_cal[s]["ms"]) if ok else min(check_scales)
+
     full_ms, s_full = [], 0
     acc = {k: {"ms": [], "kernel_ms": [], "sel_ms": [], "rmse": [], "rmse_uniform": [], "mem": [],
                "samples": 0} for k in points}
+    chk = {"rmse": [], "estimate": [], "fallbacks": 0}
+    ooc_in = []  # (view, budgeted levels, in-core image, level-0 frame, selection ms)
     # the error/speed trade-off below the calibrated point: the same views
     # at fractions of the calibrated budget factor (no brick cap)
     curve_scales = (0.5, 0.25, 0.125)
@@ -335,7 +431,13 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
             a["rmse"].append(LL.rmse255(img, ref))
             a["rmse_uniform"].append(LL.rmse255(img, uni))
             a["mem"].append(m)
+            if k == "budgeted":  # replayed out of core below
+                ooc_in.append((v, lev, img, ref, sel))
             r.setRegionLODs(None)
+        errs, est, fell = checked_accuracy([v], [ref], check_scale)
+        chk["rmse"] += errs
+        chk["estimate"] += est
+        chk["fallbacks"] += fell
         for s, c in curve.items():
             t0 = time.perf_counter()
             lev, _, _ = LL.choose_levels(pred, v, dims, edge, n_lod, hist, mom, cal["target"][0] * s,
@@ -360,6 +462,22 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
                 "brick_budget": points[k] or None,
                 "budget_factor": cal[k][0], "calibration_worst_rmse": cal[k][1]}
 
+    # frame rate of the checked loop and of full resolution, both pipelined
+    # over the same held-out views
+    eval_vs = views(eval_views, 13, *eval_size)
+    full_pipe = pipelined_ms(eval_vs)
+    checked_pipe = pipelined_ms(eval_vs, check_scale)
+    checked = {
+        "tile_sample": check_k, "rmse_bound": check_bound, "factor_scale": check_scale,
+        "fps_learned": 1e3 / checked_pipe, "fps_full": 1e3 / full_pipe,
+        "speedup": full_pipe / checked_pipe,
+        "rmse_mean": float(np.mean(chk["rmse"])), "rmse_max": float(np.max(chk["rmse"])),
+        "frames_within_1": float(np.mean(np.asarray(chk["rmse"]) <= 1.0)),
+        "estimate_mean": float(np.mean(chk["estimate"])),
+        "fallback_frames": chk["fallbacks"], "frames": len(chk["rmse"]),
+        "calibration": {str(s): c for s, c in check_cal.items()},
+    }
+
     # the predictor alone on one frame's inputs (every region x coarser level)
     xs, _, _ = LL.plan_features(views(1, 14, *eval_size)[0], dims, edge, n_lod, hist, mom, detail)
     mlp_ms = []
@@ -368,6 +486,10 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
         mlp_ms.append(pred.kernel_ms)
     mlp_t = float(np.median(mlp_ms)) * 1e-3
     mlp_flops = 28800.0 * len(xs)  # 2*(32*64 + 3*64*64 + 64) per row
+    pred.close()
+    out_of_core = run_learned_lod_out_of_core(r, ooc_in, tf, edge, n_lod, mem_budget,
+                                              float(np.mean(full_ms)))
+    r = None  # released by the out-of-core leg
     out = {
         "workload": f"learned-lod-{n}^3-u8-noise-tubes-{edge}^3-bricks-{n_lod}lods-"
                     f"{eval_size[0]}x{eval_size[1]}-{eval_views}views",
@@ -377,6 +499,9 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
         "training": {"views": train_views, "tfs": 4, "size": list(train_size)},
         "target": summary("target"),
         "budgeted": summary("budgeted"),
+        # every frame held to 1/255 by the per-frame check (frame loop with
+        # the host's selection overlapped with the device's render)
+        "checked": checked,
         # every point: speedup, mean / max RMSE and the share of frames
         # within 1/255 (the calibrated point first)
         "target_curve": [{"factor_scale": 1.0, "speedup": summary("target")["speedup"],
@@ -392,10 +517,94 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
         "train_rows": int(len(X)), "train_loss": loss,
         "seconds": {"volume": round(setup_s, 2), "training_data": round(data_s, 2),
                     "training": round(train_s, 2)},
+        "out_of_core": out_of_core,
     }
-    pred.close()
-    r.cleanup()
     return out
+
+
+def run_learned_lod_out_of_core(r, frames, tf, edge: int, n_lod: int, pool_bytes: int,
+                                full_ms_in_core: float) -> dict:
+    """The learned-LOD volume out of core: LOD 0 copied to host memory, the
+    in-core renderer `r` released (it is cleaned up here), and the bricks of
+    each budgeted selection paged from host memory into a pool of
+    `pool_bytes` (the configuration's 16 GB).  `frames` holds (view, levels,
+    in-core image, level-0 frame, selection ms) of the budgeted point: each
+    frame is rendered cold (its missing bricks streamed in, multi-pass) and
+    again warm (every brick resident), its image compared with the in-core
+    one and its RMSE with level 0."""
+    import numpy as np
+
+    from imagevis3d_amd import learned_lod as LL
+    from imagevis3d_amd import tuvok
+
+    t0 = time.perf_counter()
+    host_vol = r.readLOD(0)
+    copy_s = time.perf_counter() - t0
+    r.cleanup()
+    rp = tuvok.renderer.new(tuvok.renderer.types.OpenGL_Raycaster, False, False, False, False)
+    assert rp.initialize(tuvok.createContext(480, 270, device=0)), rp.lastError()
+    rp.setRendererTarget(tuvok.renderer.types.RT_Capture)
+    t0 = time.perf_counter()
+    rp.setPagedFlatVolume(host_vol, edge, pool_bytes, apron=2, n_lod=n_lod)
+    setup_s = time.perf_counter() - t0
+    rp.setSampleRateModifier(2.0)
+    rp.set1DTrans(tf)
+    warm_ms, cold_ms, rmse, paged, equal = [], [], [], [], 0
+    for v, lev, img_ic, ref, sel in frames:
+        rp.resize((v["width"], v["height"]))
+        LL.set_view(rp, v)
+        rp.setRegionLODs(lev)
+        b0, _ = rp.pagingStats()
+        t0 = time.perf_counter()
+        img = rp.capture()
+        cold_ms.append((time.perf_counter() - t0) * 1e3 + sel)
+        paged.append(rp.pagingStats()[0] - b0)
+        assert rp.paint(), rp.lastError()  # warm: the selection's bricks are resident
+        warm_ms.append(rp.frameStats().kernel_ms + sel)
+        equal += int(np.array_equal(img, img_ic))
+        rmse.append(LL.rmse255(img, ref))
+    ms = rp.memStats()
+    rp.cleanup()
+    del host_vol
+    return {
+        "pool_bytes": pool_bytes, "pool_slots": int(ms.pool_slots),
+        "host_copy_s": round(copy_s, 2), "paged_setup_s": round(setup_s, 2),
+        "frames": len(frames), "frames_equal_in_core": equal,
+        "fps_learned_warm": 1e3 / float(np.mean(warm_ms)),
+        # against level 0 in core (which does not fit the 16 GB pool)
+        "speedup_warm_vs_full_in_core": full_ms_in_core / float(np.mean(warm_ms)),
+        "rmse_mean": float(np.mean(rmse)), "rmse_max": float(np.max(rmse)),
+        "cold_frame_ms_mean": float(np.mean(cold_ms)),
+        "paged_gb_per_frame_mean": float(np.mean(paged)) / 1e9,
+        "paging_gb_per_s": float(np.sum(paged)) / (float(np.sum(cold_ms)) / 1e3) / 1e9,
+        "bricks_evicted": int(ms.bricks_evicted),
+    }
+
+
+def run_mlp_only(rows: int = 163840, repeats: int = 20) -> dict:
+    """The learned-LOD predictor alone (no torch in the process): random
+    bf16 weights of the 32 -> 4 x 64 -> 1 MLP over `rows` feature rows (one
+    4096^3 frame's: 32^3 regions x 5 coarser levels), timed with the
+    library's HIP events; for kernel traces of lod_mlp_kernel."""
+    import numpy as np
+
+    from imagevis3d_amd import learned_lod as LL
+
+    rng = np.random.default_rng(0)
+    n_w = LL.N_IN * LL.HIDDEN + 3 * LL.HIDDEN * LL.HIDDEN + LL.HIDDEN
+    w = LL.to_bf16_bits(rng.normal(0.0, 0.15, n_w).astype(np.float32))
+    b = rng.normal(0.0, 0.05, 4 * LL.HIDDEN + 1).astype(np.float32)
+    x = rng.normal(0.0, 1.0, (rows, LL.N_IN)).astype(np.float32)
+    pred = LL.Predictor(w, b)
+    ms = []
+    for _ in range(repeats + 1):
+        pred(x)
+        ms.append(pred.kernel_ms)
+    pred.close()
+    t = float(np.median(ms[1:])) * 1e-3
+    flops = 2.0 * (LL.N_IN * LL.HIDDEN + 3 * LL.HIDDEN * LL.HIDDEN + LL.HIDDEN) * rows
+    return {"workload": f"lod-mlp-{rows}-rows", "kernel_ms": t * 1e3, "tflops": flops / t / 1e12,
+            "mfma_peak_frac": flops / t / 2.5e15, "dtype": "bf16 x bf16 -> f32"}
 
 
 def main() -> int:
@@ -427,6 +636,8 @@ def main() -> int:
     ap.add_argument("--no-learned-lod", action="store_true", help="skip the learned-LOD workload")
     ap.add_argument("--learned-lod-only", action="store_true")
     ap.add_argument("--learned-lod-size", type=int, default=4096)
+    ap.add_argument("--mlp-only", action="store_true",
+                    help="time the learned-LOD MLP alone (random weights, no torch)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
@@ -444,6 +655,9 @@ def main() -> int:
         return 0
     if args.learned_lod_only:
         print(json.dumps(run_learned_lod(args.learned_lod_size)))
+        return 0
+    if args.mlp_only:
+        print(json.dumps(run_mlp_only()))
         return 0
 
     w = workloads.medium()
@@ -555,8 +769,9 @@ def main() -> int:
         print("[bench] learned-LOD workload", file=sys.stderr, flush=True)
         ll = run_learned_lod(args.learned_lod_size)
         out["learned_lod"] = ll
-        # FPS of the operating point held to the 1/255 RMSE target
-        out["learned_lod_fps"] = ll["target"]["fps_learned"]
+        # FPS of the frame loop whose every frame is held to 1/255 RMSE
+        # (the per-frame check with its level-0 fallback)
+        out["learned_lod_fps"] = ll["checked"]["fps_learned"]
         print(f"[bench] learned LOD {ll}", file=sys.stderr, flush=True)
     if not args.no_batched:
         print("[bench] out-of-core batched workload", file=sys.stderr, flush=True)

@@ -46,6 +46,12 @@ class MemStats(C.Structure):
     ]
 
 
+class RegionCheck(C.Structure):
+    """tvk_region_check: the per-frame check of the last per-region frame."""
+    _fields_ = [("rmse_estimate", C.c_float), ("fell_back", C.c_uint32), ("check_ms", C.c_float),
+                ("reserved", C.c_uint32), ("check_samples", C.c_uint64)]
+
+
 class MemManStats(C.Structure):
     """tvk_memman_stats: the device's memory manager as a whole."""
     _fields_ = [(n, C.c_uint64) for n in (
@@ -225,6 +231,8 @@ SIGNATURES = {
     "tvk_lod_select_octree": [P, P, P, u32, C.c_double, u64, u64, P, C.POINTER(C.c_double),
                               C.POINTER(u64)],
     "tvk_set_region_output": [H, i32],
+    "tvk_set_region_check": [H, u32, C.c_float],
+    "tvk_get_region_check": [H, P],
     "tvk_read_region_output": [H, P, u64],
     "tvk_autotune": [H, C.POINTER(View), u32, u32, C.POINTER(Tuning)],
     "tvk_server_create": [i32, u64, u32, C.POINTER(C.c_void_p)],

@@ -82,13 +82,17 @@ int HipRenderer::Autotune(const tvk_view* views, uint32_t n, uint32_t flags, tvk
   };
 
   const uint64_t maxdim = std::max(dims0_[0], std::max(dims0_[1], dims0_[2]));
-  if ((flags & TVK_TUNE_BRICK) && !paged_) {
+  // a paged volume re-bricks from the caller's flat voxels (a brick
+  // source fixes the brick size); each candidate is timed after a warm-up
+  // batch has paged its bricks in
+  const bool rebrickable = !paged_ || host_level0_ != nullptr;
+  if ((flags & TVK_TUNE_BRICK) && rebrickable) {
     std::vector<uint32_t> bricks;
     for (uint32_t b : {32u, 64u, 128u, 256u})
       if (b <= std::max<uint64_t>(32, maxdim)) bricks.push_back(b);
     if (int e = try_values(bricks, &Setting::brick, true)) return e;
   }
-  if ((flags & TVK_TUNE_CELL) && !paged_) {
+  if ((flags & TVK_TUNE_CELL) && rebrickable) {
     std::vector<uint32_t> cells;
     for (uint32_t c = 8; c <= cur.brick; c *= 2)
       if (cur.brick % c == 0) cells.push_back(c);

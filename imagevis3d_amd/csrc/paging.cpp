@@ -102,6 +102,7 @@ int HipRenderer::SetPagedVolume(const uint64_t dims[3], uint32_t bits, uint32_t 
   TVK_HIP(touch_dev_.alloc((size_t)total * sizeof(uint32_t)));
   pool_.n_slots = quota;
   pool_.slot_bytes = plan.slot_bytes;
+  pool_bytes_req_ = pool_bytes;
   pool_.resident = 0;
   pool_.page_table_h.assign((size_t)total, 0);
   pool_.brick_slot.assign((size_t)total, -1);

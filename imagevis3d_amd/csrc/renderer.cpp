@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 
 #include "errors.hpp"
 #include "kernels.hpp"
@@ -229,6 +230,12 @@ HipRenderer::~HipRenderer() {
     (void)hipStreamSynchronize(stream_);
     for (auto& sl : paint_slots_) sl.release();
     batch_slot_.release();
+    check_slot_.release();
+    if (check_sse_h_) (void)hipHostFree(check_sse_h_);
+    for (hipEvent_t ev : check_ev_)
+      if (ev) (void)hipEventDestroy(ev);
+    check_copy_.reset();
+    check_sse_.reset();
     batch_fb_.reset();
     release_volume();  // gives a paged volume's slots back to the device pool
     if (requests_h_) (void)hipHostFree(requests_h_);
@@ -276,6 +283,7 @@ int HipRenderer::Resize(uint32_t w, uint32_t h) {
   if (!initialized_) return state_fail("resize before initialize");
   if (w == 0 || h == 0 || w > 16384 || h > 16384) return arg_fail("window size out of range");
   if (w == width_ && h == height_) return TVK_OK;
+  if (int e = resolve_check()) return e;
   TVK_HIP(hipSetDevice(device_));
   TVK_HIP(hipStreamSynchronize(stream_));
   TVK_HIP(framebuffer_.alloc((size_t)w * h * 4));
@@ -378,6 +386,7 @@ int HipRenderer::plan_levels(const uint64_t dims[3], uint32_t bits, uint32_t bri
 }
 
 void HipRenderer::release_volume() {
+  (void)resolve_check();  // its launches read this volume
   has_volume_ = false;
   paged_ = false;
   for (double& t : lod_ms_) t = -1.0;  // the governor's timings were for the old volume
@@ -574,6 +583,93 @@ int HipRenderer::SetRegionLods(const uint8_t* levels, uint64_t n) {
   return TVK_OK;
 }
 
+int HipRenderer::SetRegionCheck(uint32_t tile_sample, float max_rmse) {
+  if (tile_sample == 1 || tile_sample > 4096) return arg_fail("check tile sampling must be 0 (off) or 2..4096");
+  if (!(max_rmse >= 0.f) || !std::isfinite(max_rmse)) return arg_fail("check RMSE bound must be finite and >= 0");
+  if (int e = resolve_check()) return e;  // a frame's check runs with the settings it started with
+  if (tile_sample > 0 && initialized_) {
+    TVK_HIP(hipSetDevice(device_));
+    if (!check_slot_.table.p)
+      if (int e = check_slot_.alloc()) return e;
+    if (!check_sse_.p) TVK_HIP(check_sse_.alloc(sizeof(unsigned long long)));
+    if (!check_sse_h_) TVK_HIP(hipHostMalloc((void**)&check_sse_h_, sizeof(unsigned long long), hipHostMallocDefault));
+    for (hipEvent_t& ev : check_ev_)
+      if (!ev) TVK_HIP(hipEventCreate(&ev));
+  } else if (tile_sample > 0) {
+    return state_fail("region check before initialize");
+  }
+  check_k_ = tile_sample;
+  check_max_ = max_rmse;
+  check_phase_ = 0;
+  check_stats_ = tvk_region_check{};
+  check_timing_ = false;
+  invalidate();  // the next capture renders (and checks) the frame again
+  return TVK_OK;
+}
+
+int HipRenderer::RegionCheckStats(tvk_region_check* out) {
+  if (!out) return arg_fail("null check stats");
+  if (int e = resolve_check()) return e;
+  if (check_timing_) {
+    TVK_HIP(hipSetDevice(device_));
+    TVK_HIP(hipEventSynchronize(check_ev_[1]));
+    float ms = 0.f;
+    TVK_HIP(hipEventElapsedTime(&ms, check_ev_[0], check_ev_[1]));
+    check_stats_.check_ms = ms;
+    check_timing_ = false;
+  }
+  *out = check_stats_;
+  return TVK_OK;
+}
+
+// The per-frame check of a per-region frame just launched with params p:
+// keep a copy of the frame, render the sampled level-0 tiles into it and
+// sum the squared RGB differences over the frame (non-zero on the sampled
+// tiles only).  All queued on the stream; resolve_check reads the sum.
+int HipRenderer::check_launch(const RaycastParams& p) {
+  const size_t px = (size_t)width_ * height_;
+  if (check_copy_.bytes < px * 4) TVK_HIP(check_copy_.alloc(px * 4));
+  RaycastParams q = p;
+  q.region_code = tfs_[0].region_code0.as<uint8_t>();
+  q.region_out = nullptr;
+  TVK_HIP(hipEventRecord(check_ev_[0], stream_));
+  TVK_HIP(hipMemcpyAsync(check_copy_.p, framebuffer_.p, px * 4, hipMemcpyDeviceToDevice, stream_));
+  check_params_ = q;  // the fallback: the whole frame at level 0
+  q.tile_sample = (int32_t)check_k_;
+  q.tile_phase = (int32_t)(check_phase_++);
+  if (int e = launch_jobs(check_slot_, &q, 1, launch_info(q.n_tf), false)) return e;
+  TVK_HIP(launch_fill_u64(check_sse_.as<unsigned long long>(), 1, 0ull, stream_));
+  TVK_HIP(launch_rgb_sse(framebuffer_.as<uint32_t>(), check_copy_.as<uint32_t>(), (int64_t)px,
+                         check_sse_.as<unsigned long long>(), stream_));
+  TVK_HIP(hipMemcpyAsync(check_sse_h_, check_sse_.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_));
+  TVK_HIP(hipEventRecord(check_ev_[2], stream_));
+  check_open_ = true;
+  return TVK_OK;
+}
+
+// Reads the check's error sum and, when the RMSE of the rest of the frame
+// -- estimated as (k - 1) times the sampled tiles' error, each sampled tile
+// standing for the k - 1 others of its run -- exceeds the bound, renders
+// the whole frame at level 0.
+int HipRenderer::resolve_check() {
+  if (!check_open_) return TVK_OK;
+  check_open_ = false;
+  TVK_HIP(hipSetDevice(device_));
+  TVK_HIP(hipEventSynchronize(check_ev_[2]));
+  const double px = (double)check_params_.width * (double)check_params_.height;
+  const double est = std::sqrt((double)(check_k_ - 1) * (double)*check_sse_h_ / (3.0 * px));
+  check_stats_ = tvk_region_check{};
+  check_stats_.rmse_estimate = (float)est;
+  check_stats_.check_samples = check_slot_.counters_h[kQueues];
+  if (est > (double)check_max_) {
+    if (int e = launch_jobs(check_slot_, &check_params_, 1, launch_info(check_params_.n_tf), false)) return e;
+    check_stats_.fell_back = 1;
+  }
+  TVK_HIP(hipEventRecord(check_ev_[1], stream_));
+  check_timing_ = true;
+  return TVK_OK;
+}
+
 int HipRenderer::SetRegionOutput(bool on) {
   region_out_on_ = on;
   return TVK_OK;
@@ -605,32 +701,38 @@ int HipRenderer::prepare_regions(int tf_slot) {
   if (key == t.region_key && t.region_code.p) return TVK_OK;
   const LodInfo& L0 = lods_[0];
   const int g[3] = {L0.grid[0], L0.grid[1], L0.grid[2]};
-  std::vector<uint8_t> code(region_lod_h_.size());
+  // the chosen levels' codes and, for the per-frame check, level 0's
+  std::vector<uint8_t> code(region_lod_h_.size()), code0(region_lod_h_.size());
   const float B = (float)brick_;
+  auto region_code = [&](int l, int cx, int cy, int cz) {
+    const LodInfo& lv = lods_[l];
+    const int c[3] = {cx, cy, cz};
+    int b[3];
+    for (int i = 0; i < 3; ++i) {
+      const float qc = ((float)c[i] + 0.5f) * B * lv.to_level[i];
+      b[i] = std::min(std::max((int)floorf(qc / B), 0), lv.grid[i] - 1);
+    }
+    const int gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
+    bool empty = false;
+    if (skip_ && n > 0) {
+      const float mn = pool_.brick_range_h[2 * gid], mx = pool_.brick_range_h[2 * gid + 1];
+      const int i0 = (int)std::min(std::max(mn * scale, 0.f), (float)(n - 1));
+      const int i1 = std::min((int)std::min(std::max(mx * scale, 0.f), (float)(n - 1)) + 1, n - 1);
+      empty = t.prefix1d[i1 + 1] - t.prefix1d[i0] == 0;
+    }
+    return (uint8_t)(l | (empty ? 0x80 : 0));
+  };
   for (int cz = 0; cz < g[2]; ++cz)
     for (int cy = 0; cy < g[1]; ++cy)
       for (int cx = 0; cx < g[0]; ++cx) {
         const size_t r = ((size_t)cz * g[1] + cy) * g[0] + cx;
-        const int l = region_lod_h_[r];
-        const LodInfo& lv = lods_[l];
-        const int c[3] = {cx, cy, cz};
-        int b[3];
-        for (int i = 0; i < 3; ++i) {
-          const float qc = ((float)c[i] + 0.5f) * B * lv.to_level[i];
-          b[i] = std::min(std::max((int)floorf(qc / B), 0), lv.grid[i] - 1);
-        }
-        const int gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
-        bool empty = false;
-        if (skip_ && n > 0) {
-          const float mn = pool_.brick_range_h[2 * gid], mx = pool_.brick_range_h[2 * gid + 1];
-          const int i0 = (int)std::min(std::max(mn * scale, 0.f), (float)(n - 1));
-          const int i1 = std::min((int)std::min(std::max(mx * scale, 0.f), (float)(n - 1)) + 1, n - 1);
-          empty = t.prefix1d[i1 + 1] - t.prefix1d[i0] == 0;
-        }
-        code[r] = (uint8_t)(l | (empty ? 0x80 : 0));
+        code[r] = region_code(region_lod_h_[r], cx, cy, cz);
+        code0[r] = region_lod_h_[r] == 0 ? code[r] : region_code(0, cx, cy, cz);
       }
   if (t.region_code.bytes < code.size()) TVK_HIP(t.region_code.alloc(code.size()));
+  if (t.region_code0.bytes < code0.size()) TVK_HIP(t.region_code0.alloc(code0.size()));
   TVK_HIP(hipMemcpyAsync(t.region_code.p, code.data(), code.size(), hipMemcpyHostToDevice, stream_));
+  TVK_HIP(hipMemcpyAsync(t.region_code0.p, code0.data(), code0.size(), hipMemcpyHostToDevice, stream_));
   TVK_HIP(hipStreamSynchronize(stream_));
   t.region_key = key;
   return TVK_OK;
@@ -638,8 +740,19 @@ int HipRenderer::prepare_regions(int tf_slot) {
 
 int HipRenderer::Rebrick(uint32_t brick, uint32_t cell) {
   if (!has_volume_) return state_fail("rebrick before a volume is set");
-  if (paged_) return state_fail("a paged volume keeps its brick size");
   const int saved = cell_pref_;
+  if (paged_) {
+    // a paged volume over the caller's flat voxels is set up again from
+    // them: new bricks, ranges and coarser levels, the same pool size (its
+    // bricks are paged in afresh)
+    if (!host_level0_) return state_fail("a paged volume fed by a brick source keeps its brick size");
+    if (cell) cell_pref_ = (int)cell;
+    const uint64_t dims[3] = {dims0_[0], dims0_[1], dims0_[2]};
+    const int e = SetPagedFlatVolume(host_level0_, dims, bits_, brick, (uint32_t)apron_, (uint32_t)n_lod_,
+                                     pool_bytes_req_);
+    if (e) cell_pref_ = saved;
+    return e;
+  }
   if (cell) cell_pref_ = (int)cell;
   LevelPlan plan;
   const int e0 = plan_levels(dims0_, bits_, brick, (uint32_t)apron_, (uint32_t)n_lod_, &plan);
@@ -1204,6 +1317,7 @@ void HipRenderer::record_subframe_time() {
 int HipRenderer::ReadFramebuffer(uint8_t* rgba, uint64_t bytes) {
   if (!initialized_ || !framebuffer_.p) return state_fail("no framebuffer (initialize + resize first)");
   if (!rgba || bytes < (uint64_t)width_ * height_ * 4) return arg_fail("framebuffer readback buffer too small");
+  if (int e = resolve_check()) return e;
   TVK_HIP(hipSetDevice(device_));
   TVK_HIP(hipMemcpyAsync(rgba, framebuffer_.p, (size_t)width_ * height_ * 4, hipMemcpyDeviceToHost, stream_));
   TVK_HIP(hipStreamSynchronize(stream_));
@@ -1639,6 +1753,7 @@ uint32_t HipRenderer::bricks_skipped(int lod, int tf_slot) const {
 
 int HipRenderer::Paint() {
   if (int e = check_ready()) return e;
+  if (int e = resolve_check()) return e;  // the last frame's check ends before this one starts
   TVK_HIP(hipSetDevice(device_));
   if (int e = upload_tf()) return e;
   const int lo = std::min<int>((int)lod_min_, n_lod_ - 1);
@@ -1677,6 +1792,7 @@ int HipRenderer::Paint() {
   if (regions_active()) {
     if (render_mode_ != TVK_RM_1DTRANS || lighting_)
       return state_fail("per-region LOD renders the 1D TF without lighting");
+    if (check_k_ > 0 && paged_) return state_fail("the per-frame region check renders in-core volumes");
     if (int e = prepare_regions(0)) return e;
   }
   RaycastParams p;
@@ -1705,6 +1821,8 @@ int HipRenderer::Paint() {
     if (int e = sync_page_table_locked()) return e;
     if (int e = launch_jobs(sl, &p, 1, launch_info(p.n_tf), false)) return e;
   }
+  if (check_k_ > 0 && regions_active())
+    if (int e = check_launch(p)) return e;
   if (lowres)
     TVK_HIP(launch_upscale_nearest(lowres_fb_.as<uint32_t>(), (int)rw, (int)rh, framebuffer_.as<uint32_t>(),
                                    (int)width_, (int)height_, stream_));
@@ -2068,6 +2186,7 @@ int HipRenderer::Capture(uint8_t* rgba, uint64_t bytes, bool preserve) {
     set_last_error("the frame did not complete within the paging pass limit");
     rc = TVK_ERR_OOM;
   }
+  if (rc == TVK_OK) rc = resolve_check();
   if (rc == TVK_OK) {
     hipError_t e = hipMemcpyAsync(rgba, framebuffer_.p, (size_t)width_ * height_ * 4,
                                   hipMemcpyDeviceToHost, stream_);
@@ -2097,6 +2216,7 @@ int HipRenderer::Progress(uint64_t* subframes, uint32_t* working_subframe, uint3
 
 int HipRenderer::FrameStats(tvk_frame_stats* out) {
   if (!out) return arg_fail("null stats pointer");
+  if (int e = resolve_check()) return e;
   if (stats_pending_ && stats_slot_) {
     TVK_HIP(hipSetDevice(device_));
     LaunchSlot& sl = *stats_slot_;
@@ -2116,6 +2236,7 @@ int HipRenderer::FrameStats(tvk_frame_stats* out) {
 
 int HipRenderer::DeviceFramebuffer(void** dptr, void** stream) {
   if (!initialized_ || !framebuffer_.p) return state_fail("no framebuffer (initialize + resize first)");
+  if (int e = resolve_check()) return e;  // a fallback render is queued before the caller reads
   if (dptr) *dptr = framebuffer_.p;
   if (stream) *stream = (void*)stream_;
   return TVK_OK;
@@ -2123,6 +2244,7 @@ int HipRenderer::DeviceFramebuffer(void** dptr, void** stream) {
 
 int HipRenderer::Synchronize() {
   if (!initialized_) return state_fail("not initialized");
+  if (int e = resolve_check()) return e;
   TVK_HIP(hipSetDevice(device_));
   TVK_HIP(hipStreamSynchronize(stream_));
   return TVK_OK;

@@ -96,6 +96,7 @@ struct TfSet {
   DevBuf tf2d_dev, prefix2d_dev;
   uint64_t version = 0;   // bumped by every change of this slot
   DevBuf region_code;     // per LOD-0 region: chosen level | 0x80 when TF-empty
+  DevBuf region_code0;    // the same with every region at level 0
   uint64_t region_key = 0;
   DevBuf cell_empty[3];      // per render mode: u8 per empty-space cell of every level
   uint64_t cell_key[3] = {};  // what each cell_empty was built for
@@ -247,6 +248,13 @@ class HipRenderer : public PoolClient {
   int SetRegionLods(const uint8_t* levels, uint64_t n);
   int SetRegionOutput(bool on);
   int ReadRegionOutput(int32_t* out, uint64_t n);
+  // Per-frame check of per-region LOD frames against level 0: every paint
+  // also renders one ray tile in each run of `tile_sample` at level 0 into
+  // the frame, estimates the frame's remaining RGB RMSE (1/255 units) from
+  // those tiles and, when it exceeds max_rmse, renders the whole frame at
+  // level 0.  tile_sample 0 turns it off.
+  int SetRegionCheck(uint32_t tile_sample, float max_rmse);
+  int RegionCheckStats(tvk_region_check* out);
   // In-core volumes: cut the device-resident levels into bricks of another
   // size / empty-space cell edge (cell 0 keeps the current preference).
   int Rebrick(uint32_t brick, uint32_t cell);
@@ -393,6 +401,22 @@ class HipRenderer : public PoolClient {
   uint64_t region_version_ = 0;
   bool region_out_on_ = false;
   DevBuf region_out_dev_;              // int32 per pixel of the framebuffer
+  // per-frame region check: launched by Paint, resolved (estimate read,
+  // fallback launched) by the next call that needs the frame
+  int check_launch(const RaycastParams& p);
+  int resolve_check();
+  uint32_t check_k_ = 0;               // tile sampling (0: off)
+  float check_max_ = 1.f;              // RGB RMSE bound, 1/255 units
+  uint32_t check_phase_ = 0;
+  DevBuf check_copy_;                  // the frame before the check's tiles
+  DevBuf check_sse_;                   // u64 squared-error sum
+  unsigned long long* check_sse_h_ = nullptr;  // pinned
+  LaunchSlot check_slot_;
+  hipEvent_t check_ev_[3] = {nullptr, nullptr, nullptr};  // start, end, sum copied
+  RaycastParams check_params_{};       // the frame's params (for the fallback)
+  bool check_open_ = false;            // launched, not resolved
+  bool check_timing_ = false;          // check_stats_.check_ms not read yet
+  tvk_region_check check_stats_{};
   uint64_t cells_gen_ = 1;  // bumped whenever cell ranges change
 
   // paged (out-of-core) pools: GridLeaper-style requests + LRU
@@ -400,6 +424,7 @@ class HipRenderer : public PoolClient {
   BrickSourceFn source_fn_ = nullptr;
   void* source_user_ = nullptr;
   const void* host_level0_ = nullptr;             // caller-owned LOD 0 (flat source)
+  uint64_t pool_bytes_req_ = 0;                   // the pool size the caller asked for
   std::vector<std::vector<uint8_t>> host_levels_;  // LOD >= 1 (flat source)
   DevBuf requests_dev_, touch_dev_;               // u32 per brick
   uint32_t* requests_h_ = nullptr;                // pinned

@@ -804,6 +804,16 @@ int tvk_read_region_output(tvk_renderer* r, int32_t* out, uint64_t n) {
   return r->impl.ReadRegionOutput(out, n);
 }
 
+int tvk_set_region_check(tvk_renderer* r, uint32_t tile_sample, float max_rmse) {
+  TVK_GUARD(r);
+  TVK_TRY(r->impl.SetRegionCheck(tile_sample, max_rmse));
+}
+
+int tvk_get_region_check(tvk_renderer* r, tvk_region_check* out) {
+  TVK_GUARD(r);
+  return r->impl.RegionCheckStats(out);
+}
+
 int tvk_autotune(tvk_renderer* r, const tvk_view* views, uint32_t n, uint32_t flags,
                  tvk_tuning* out) {
   TVK_GUARD(r);

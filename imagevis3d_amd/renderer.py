@@ -28,7 +28,7 @@ from typing import Sequence
 import numpy as np
 
 from . import _lib
-from ._lib import FrameStats, MemStats, Tuning, TvkError, View, ViewStats, call
+from ._lib import FrameStats, MemStats, RegionCheck, Tuning, TvkError, View, ViewStats, call
 
 
 class _Types:
@@ -554,6 +554,19 @@ class Renderer:
         out = np.empty((grid[0] * grid[1] * grid[2], nl), np.float32)
         call("tvk_region_detail", self._h, out.ctypes.data, out.size)
         return out
+
+    def setRegionLODCheck(self, tile_sample: int, max_rmse: float = 1.0) -> None:
+        """Per-frame check of per-region frames: one ray tile in each run of
+        `tile_sample` rendered at level 0, the frame's RGB RMSE estimated
+        from them (1/255 units) and, above max_rmse, the frame rendered at
+        level 0; 0 turns it off."""
+        call("tvk_set_region_check", self._h, int(tile_sample), float(max_rmse))
+
+    def regionLODCheck(self) -> RegionCheck:
+        """The last checked frame: rmse_estimate, fell_back, check_ms."""
+        c = RegionCheck()
+        call("tvk_get_region_check", self._h, C.byref(c))
+        return c
 
     def setRegionOutput(self, on: bool) -> None:
         call("tvk_set_region_output", self._h, int(bool(on)))

@@ -551,6 +551,30 @@ int tvk_set_region_lods(tvk_renderer* r, const uint8_t* levels, uint64_t n);
  * tvk_read_region_output (W*H int32). */
 int tvk_set_region_output(tvk_renderer* r, int enabled);
 int tvk_read_region_output(tvk_renderer* r, int32_t* out, uint64_t n);
+/* Per-frame check of per-region LOD frames (no counterpart in the
+ * reference; it holds each learned-LOD frame to an error bound at run
+ * time).  With tile_sample = k >= 2, every paint of a per-region frame also
+ * renders one ray tile in each run of k at level 0 into the frame (those
+ * tiles become exact), estimates the RGB RMSE of the rest of the frame
+ * against the all-level-0 frame as sqrt((k-1) * SSE / (3 * W * H)) in
+ * 1/255 units (SSE: the sampled tiles' squared differences in 8-bit
+ * units) and, above max_rmse, renders the whole frame at level 0.  The
+ * paint stays asynchronous: the next call that needs the frame (paint,
+ * capture, read/device framebuffer, frame stats, synchronize, resize)
+ * reads the estimate and queues the fallback.  The sampled tiles' offsets
+ * change from frame to frame and start over at each call of
+ * tvk_set_region_check.  In-core volumes only
+ * (TVK_ERR_STATE at paint on a paged one).  tile_sample 0 turns it off;
+ * 1 or > 4096 is TVK_ERR_ARG. */
+typedef struct tvk_region_check {
+  float rmse_estimate;     /* the last checked frame's estimate (1/255)  */
+  uint32_t fell_back;      /* 1: that frame was rendered at level 0      */
+  float check_ms;          /* device time of the check and any fallback  */
+  uint32_t reserved;
+  uint64_t check_samples;  /* samples of the check's level-0 tiles       */
+} tvk_region_check;
+int tvk_set_region_check(tvk_renderer* r, uint32_t tile_sample, float max_rmse);
+int tvk_get_region_check(tvk_renderer* r, tvk_region_check* out);
 
 /* Per-brick value histograms of LOD level `lod` (payload voxels, `bins` =
  * 2..64 equal bins over [0, 2^bits)): n_bricks x bins counts, bricks in

@@ -1,7 +1,8 @@
 """Out-of-core (paged) brick pool: a paged render equals the in-core render
 bit for bit once its bricks are resident, bricks are evicted LRU, a view
-whose bricks exceed the pool falls back to the coarser LOD, and empty
-bricks are never paged in."""
+whose bricks exceed the pool falls back to the coarser LOD, empty bricks
+are never paged in, and the autotuner's brick size on a paged flat volume
+renders like an in-core volume of that brick size."""
 import numpy as np
 import pytest
 
@@ -54,6 +55,36 @@ def test_paged_render_equals_in_core():
     up = ms.bricks_uploaded
     frame(pg, w, w.cameras[20])  # everything already resident
     assert pg.memStats().bricks_uploaded == up
+    ref.cleanup()
+    pg.cleanup()
+
+
+def test_paged_flat_volume_brick_size_autotuned():
+    """Autotuning the brick size of a paged flat volume sets it up again
+    from the caller's voxels per candidate; afterwards its frames equal an
+    in-core volume bricked at the chosen size."""
+    w, vol = setup()
+    print("case paged brick-size autotune", flush=True)
+    pg = renderer(w)
+    pg.setPagedFlatVolume(vol, 32, 64 << 20, n_lod=3)
+    pg.set1DTrans(w.tf)
+    pg.setLODLimits((0, 0))
+    workloads.apply(pg, w, w.cameras[0])
+    views = [workloads.view_of(w.cameras[i], w.width, w.height, 0) for i in (0, 9, 20)]
+    t = pg.autotune(views, flags=pg.TUNE_BRICK | pg.TUNE_LAUNCH)
+    print(f"tuned brick {t.brick_size}: {t.kernel_ms:.3f} ms (from {t.baseline_ms:.3f}, "
+          f"{t.candidates} settings)", flush=True)
+    assert t.brick_size in (32, 64, 128) and t.candidates >= 4
+    assert pg.memStats().slot_bytes == slot_bytes(t.brick_size)
+    ref = renderer(w)
+    ref.setVolume(vol, t.brick_size, w.apron, 3)
+    ref.set1DTrans(w.tf)
+    ref.setLODLimits((0, 0))
+    for v in (0, 9, 20):
+        a, sa = frame(ref, w, w.cameras[v])
+        b, sb = frame(pg, w, w.cameras[v])
+        assert np.array_equal(a, b), f"view {v}"
+        assert sa.evaluated_samples == sb.evaluated_samples
     ref.cleanup()
     pg.cleanup()
 

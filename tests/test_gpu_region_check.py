@@ -1,0 +1,112 @@
+"""The per-frame check of per-region LOD frames (tvk_set_region_check) on
+the GPU: the checked frame is the learned frame with one ray tile in each
+run of k replaced by its level-0 render; the reported estimate equals
+sqrt((k-1) SSE / (3 W H)) recomputed on the host from the two frames; a
+zero bound always falls back to the exact all-level-0 frame; off, the
+frame is the learned frame bit for bit; and painting without reading back
+(the check resolved by the next paint) gives the same frames."""
+import numpy as np
+import pytest
+
+from conftest import gpu_renderer
+from imagevis3d_amd import TvkError, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def setup():
+    w = workloads.medium(dims=(128, 128, 128), width=320, height=192)
+    w.brick = 32
+    vol = w.volume()
+    r = gpu_renderer(w)
+    r.setVolume(vol, 32, 2, 3)
+    r.set1DTrans(w.tf)
+    return w, r
+
+
+def frames(r, w, cam, levels):
+    """(learned frame, all-level-0 frame) of one view."""
+    workloads.apply(r, w, cam)
+    r.setRegionLODs(np.zeros_like(levels))
+    full = r.capture()
+    r.setRegionLODs(levels)
+    learned = r.capture()
+    return learned, full
+
+
+def sse_rgb(a, b):
+    d = a[..., :3].astype(np.int64) - b[..., :3].astype(np.int64)
+    return int((d * d).sum())
+
+
+@pytest.mark.parametrize("k", [2, 7, 16])
+def test_checked_frame_is_learned_with_level0_tiles_and_estimate_matches(k):
+    w, r = setup()
+    print(f"case region check k={k}", flush=True)
+    rng = np.random.default_rng(k)
+    levels = rng.integers(1, 3, (4, 4, 4)).astype(np.uint8)
+    for view in (3, 14):
+        learned, full = frames(r, w, w.cameras[view], levels)
+        differ = np.any(learned != full, axis=-1)
+        assert differ.mean() > 0.05  # coarse levels change the frame
+        r.setRegionLODCheck(k, 1e9)  # never falls back
+        out = r.capture()
+        c = r.regionLODCheck()
+        r.setRegionLODCheck(0, 0.0)
+        # every pixel is the learned one or the level-0 one
+        is_learned = np.all(out == learned, axis=-1)
+        is_full = np.all(out == full, axis=-1)
+        assert np.all(is_learned | is_full)
+        # about one tile in k of the differing pixels was rendered at level 0
+        took = (is_full & differ).sum() / differ.sum()
+        print(f"view {view}: level-0 share of differing pixels {took:.3f} (1/k = {1 / k:.3f}), "
+              f"estimate {c.rmse_estimate:.4f}")
+        assert 0.4 / k < took < 2.5 / k
+        # the estimate from the two frames on the host
+        h, wd = out.shape[:2]
+        est = np.sqrt((k - 1) * sse_rgb(out, learned) / (3.0 * h * wd))
+        assert abs(c.rmse_estimate - est) <= 1e-5 * est + 1e-6
+        assert c.fell_back == 0 and c.check_samples > 0 and c.check_ms > 0
+    r.cleanup()
+
+
+def test_zero_bound_falls_back_to_level_zero_and_off_is_learned():
+    w, r = setup()
+    print("case region check fallback", flush=True)
+    levels = np.full((4, 4, 4), 2, np.uint8)
+    learned, full = frames(r, w, w.cameras[9], levels)
+    r.setRegionLODCheck(8, 0.0)
+    out = r.capture()
+    c = r.regionLODCheck()
+    assert c.fell_back == 1 and c.rmse_estimate > 0
+    assert np.array_equal(out, full)
+    r.setRegionLODCheck(0, 0.0)
+    assert np.array_equal(r.capture(), learned)
+    with pytest.raises(TvkError):
+        r.setRegionLODCheck(1, 1.0)
+    r.cleanup()
+
+
+def test_pipelined_paints_resolve_like_captures():
+    """Frames painted back to back (each check resolved by the next paint)
+    equal the frames captured one at a time."""
+    w, r = setup()
+    print("case region check pipelined", flush=True)
+    rng = np.random.default_rng(5)
+    cams = [w.cameras[i] for i in (1, 8, 20)]
+    lv = [rng.integers(0, 3, (4, 4, 4)).astype(np.uint8) for _ in cams]
+    r.setRegionLODCheck(4, 0.5)
+    single = []
+    for cam, levels in zip(cams, lv):
+        workloads.apply(r, w, cam)
+        r.setRegionLODs(levels)
+        single.append((r.capture(), r.regionLODCheck().fell_back))
+    r.setRegionLODCheck(4, 0.5)  # the tile offsets start over
+    for cam, levels in zip(cams, lv):
+        workloads.apply(r, w, cam)
+        r.setRegionLODs(levels)
+        assert r.paint()
+    r.synchronize()
+    assert np.array_equal(r.readFramebuffer(), single[-1][0])
+    assert r.regionLODCheck().fell_back == single[-1][1]
+    r.cleanup()

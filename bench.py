@@ -297,20 +297,26 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
                             fov=50.0, width=w, height=h))
         return out
 
+    def say(msg):
+        print(f"[bench] learned LOD: {msg}", file=sys.stderr, flush=True)
+
     dims = (n, n, n)
     r = tuvok.renderer.new(tuvok.renderer.types.OpenGL_Raycaster, False, False, False, False)
     assert r.initialize(tuvok.createContext(480, 270, device=0)), r.lastError()
     r.setRendererTarget(tuvok.renderer.types.RT_Capture)
+    say(f"generating {n}^3")
     t0 = time.perf_counter()
     r.setGeneratedVolume(VOL_NOISE_TUBES, 4, dims, 8, edge, 2, n_lod)
     setup_s = time.perf_counter() - t0
     r.setSampleRateModifier(2.0)
     hist = LL.region_histograms(r)
     detail = LL.region_detail(r)
+    say("training data")
     t0 = time.perf_counter()
     X, Y = LL.training_rows(r, dims, edge, n_lod, views(train_views, 11, *train_size),
                             [transfer.tubes_tf(256, seed=s) for s in range(4)], hist, detail)
     data_s = time.perf_counter() - t0
+    say(f"training on {len(X)} rows")
     t0 = time.perf_counter()
     wbits, b, loss = LL.train(X, Y, steps=3000)
     train_s = time.perf_counter() - t0
@@ -327,6 +333,7 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
     # target alone (no brick cap); the budget factor is calibrated so that
     # the mean RMSE over the calibration views stays at cal_target/255
     points = {"budgeted": mem_budget, "target": 0}
+    say("calibrating the budget factors")
     cal = {k: LL.calibrate_safety(r, pred, cal_views, dims, edge, n_lod, hist, mom, cal_target,
                                   slot_bytes, mb, detail, reduce="mean")
            for k, mb in points.items()}
@@ -388,6 +395,7 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
     zero_regions = np.zeros((n // edge) ** 3, np.uint8)
     check_scales = (1.0, 0.5, 0.25)
     check_cal = {}
+    say("calibrating the checked loop")
     for s in check_scales:
         errs, _, fell = checked_accuracy(cal_views, cal_refs, s)
         check_cal[s] = {"rmse_max": float(np.max(errs)), "fallbacks": fell,
@@ -404,7 +412,8 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
     # at fractions of the calibrated budget factor (no brick cap)
     curve_scales = (0.5, 0.25, 0.125)
     curve = {s: {"ms": [], "rmse": []} for s in curve_scales}
-    for v in views(eval_views, 13, *eval_size):
+    for i, v in enumerate(views(eval_views, 13, *eval_size)):
+        say(f"view {i + 1}/{eval_views}")
         r.resize(eval_size)
         LL.set_view(r, v)
         r.setRegionLODs(None)
@@ -464,6 +473,7 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
 
     # frame rate of the checked loop and of full resolution, both pipelined
     # over the same held-out views
+    say("pipelined frame loops")
     eval_vs = views(eval_views, 13, *eval_size)
     full_pipe = pipelined_ms(eval_vs)
     checked_pipe = pipelined_ms(eval_vs, check_scale)
@@ -537,6 +547,10 @@ def run_learned_lod_out_of_core(r, frames, tf, edge: int, n_lod: int, pool_bytes
     from imagevis3d_amd import learned_lod as LL
     from imagevis3d_amd import tuvok
 
+    def say(msg):
+        print(f"[bench] learned LOD out of core: {msg}", file=sys.stderr, flush=True)
+
+    say("copying LOD 0 to host memory")
     t0 = time.perf_counter()
     host_vol = r.readLOD(0)
     copy_s = time.perf_counter() - t0
@@ -544,13 +558,15 @@ def run_learned_lod_out_of_core(r, frames, tf, edge: int, n_lod: int, pool_bytes
     rp = tuvok.renderer.new(tuvok.renderer.types.OpenGL_Raycaster, False, False, False, False)
     assert rp.initialize(tuvok.createContext(480, 270, device=0)), rp.lastError()
     rp.setRendererTarget(tuvok.renderer.types.RT_Capture)
+    say("host pyramid and brick ranges")
     t0 = time.perf_counter()
     rp.setPagedFlatVolume(host_vol, edge, pool_bytes, apron=2, n_lod=n_lod)
     setup_s = time.perf_counter() - t0
     rp.setSampleRateModifier(2.0)
     rp.set1DTrans(tf)
     warm_ms, cold_ms, rmse, paged, equal = [], [], [], [], 0
-    for v, lev, img_ic, ref, sel in frames:
+    for i, (v, lev, img_ic, ref, sel) in enumerate(frames):
+        say(f"frame {i + 1}/{len(frames)}")
         rp.resize((v["width"], v["height"]))
         LL.set_view(rp, v)
         rp.setRegionLODs(lev)

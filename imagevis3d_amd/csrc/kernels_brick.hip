@@ -216,54 +216,60 @@ __global__ __launch_bounds__(kThreads) void slot_to_linear_kernel(const T* __res
 // rows at a time with its lanes over x, the work-groups along grid.y split
 // the rows among them, and each work-group folds its minimum and maximum
 // into the cell with one atomic each (lo starts at ~0, hi at 0).  The work
-// per launch does not depend on the cell size.
+// per launch does not depend on the cell size.  Work-groups along grid.x
+// take cells blockIdx.x, blockIdx.x + gridDim.x, ... (a 4096^3 level has
+// 16.7 M cells of 16^3, more work-groups than one launch may have).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void cell_range_rows_kernel(const T* __restrict__ level,
                                                                    int64_t nx, int64_t ny, int64_t nz,
-                                                                   int C, int ncx, int ncy,
+                                                                   int C, int ncx, int ncy, int64_t n_cells,
                                                                    uint32_t* __restrict__ lo_out,
                                                                    uint32_t* __restrict__ hi_out) {
-  const int cell = (int)blockIdx.x;
-  const int cx = cell % ncx, cy = (cell / ncx) % ncy, cz = cell / (ncx * ncy);
-  const int64_t x0 = clamp64((int64_t)cx * C - 1, 0, nx - 1), x1 = clamp64((int64_t)(cx + 1) * C, 0, nx - 1);
-  const int64_t y0 = clamp64((int64_t)cy * C - 1, 0, ny - 1), y1 = clamp64((int64_t)(cy + 1) * C, 0, ny - 1);
-  const int64_t z0 = clamp64((int64_t)cz * C - 1, 0, nz - 1), z1 = clamp64((int64_t)(cz + 1) * C, 0, nz - 1);
-  const int w = (int)(x1 - x0 + 1), h = (int)(y1 - y0 + 1), dz = (int)(z1 - z0 + 1);
-  const int rows = h * dz;
+  __shared__ uint32_t s_lo[kThreads / 64], s_hi[kThreads / 64];
   const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
-  const int rpw = w < 64 ? 64 / w : 1;            // rows per wave step
-  const int lrow = w < 64 ? lane / w : 0;          // this lane's row in the step
-  const int lx = w < 64 ? lane % w : lane;         // and its first x
-  const int xstep = w < 64 ? w : 64;
   const int waves = (int)gridDim.y * (kThreads / 64);
   const int wg = (int)blockIdx.y * (kThreads / 64) + wave;
-  uint32_t lo = 0xffffffffu, hi = 0;
-  if (lrow < rpw) {
-    for (int r = wg * rpw + lrow; r < rows; r += waves * rpw) {
-      const int y = r % h, z = r / h;
-      const T* __restrict__ row = level + ((z0 + z) * ny + (y0 + y)) * nx + x0;
-      for (int x = lx; x < w; x += xstep) {
-        const uint32_t v = row[x];
-        lo = min(lo, v);
-        hi = max(hi, v);
+  // the loop bound depends on blockIdx only: every thread of the group runs
+  // the same iterations, so the barriers below are reached by all of them
+  for (int64_t cell = blockIdx.x; cell < n_cells; cell += gridDim.x) {
+    const int64_t cx = cell % ncx, cy = (cell / ncx) % ncy, cz = cell / ((int64_t)ncx * ncy);
+    const int64_t x0 = clamp64(cx * C - 1, 0, nx - 1), x1 = clamp64((cx + 1) * C, 0, nx - 1);
+    const int64_t y0 = clamp64(cy * C - 1, 0, ny - 1), y1 = clamp64((cy + 1) * C, 0, ny - 1);
+    const int64_t z0 = clamp64(cz * C - 1, 0, nz - 1), z1 = clamp64((cz + 1) * C, 0, nz - 1);
+    const int w = (int)(x1 - x0 + 1), h = (int)(y1 - y0 + 1), dz = (int)(z1 - z0 + 1);
+    const int rows = h * dz;
+    const int rpw = w < 64 ? 64 / w : 1;            // rows per wave step
+    const int lrow = w < 64 ? lane / w : 0;          // this lane's row in the step
+    const int lx = w < 64 ? lane % w : lane;         // and its first x
+    const int xstep = w < 64 ? w : 64;
+    uint32_t lo = 0xffffffffu, hi = 0;
+    if (lrow < rpw) {
+      for (int r = wg * rpw + lrow; r < rows; r += waves * rpw) {
+        const int y = r % h, z = r / h;
+        const T* __restrict__ row = level + ((z0 + z) * ny + (y0 + y)) * nx + x0;
+        for (int x = lx; x < w; x += xstep) {
+          const uint32_t v = row[x];
+          lo = min(lo, v);
+          hi = max(hi, v);
+        }
       }
     }
-  }
-  lo = wave_min(lo);
-  hi = wave_max(hi);
-  __shared__ uint32_t s_lo[kThreads / 64], s_hi[kThreads / 64];
-  if (lane == 0) {
-    s_lo[wave] = lo;
-    s_hi[wave] = hi;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int k = 1; k < kThreads / 64; ++k) {
-      lo = min(lo, s_lo[k]);
-      hi = max(hi, s_hi[k]);
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    if (lane == 0) {
+      s_lo[wave] = lo;
+      s_hi[wave] = hi;
     }
-    atomicMin(lo_out + cell, lo);
-    atomicMax(hi_out + cell, hi);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int k = 1; k < kThreads / 64; ++k) {
+        lo = min(lo, s_lo[k]);
+        hi = max(hi, s_hi[k]);
+      }
+      atomicMin(lo_out + cell, lo);
+      atomicMax(hi_out + cell, hi);
+    }
+    __syncthreads();  // s_lo / s_hi are the next cell's
   }
 }
 
@@ -632,13 +638,17 @@ hipError_t launch_cell_ranges(const void* level, int bpv, const int64_t d[3], in
   const int64_t rpw = w < 64 ? 64 / w : 1;
   const int64_t per_group = (kThreads / 64) * rpw * 8;
   const unsigned gy = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (rows + per_group - 1) / per_group), 65535);
-  const dim3 grid((unsigned)cells, gy, 1);
+  // at most 2^20 work-groups along x (a launch counts each dimension's
+  // work-items in 32 bits: one work-group per cell of a 4096^3 level would
+  // be 2^32 of them); more cells loop inside the kernel
+  const unsigned gx = (unsigned)std::min<int64_t>(cells, (int64_t)1 << 20);
+  const dim3 grid(gx, gy, 1);
   if (bpv == 1)
     cell_range_rows_kernel<uint8_t><<<grid, kThreads, 0, stream>>>((const uint8_t*)level, d[0], d[1], d[2], C,
-                                                                   ncx, ncy, out, tmp_hi);
+                                                                   ncx, ncy, cells, out, tmp_hi);
   else
     cell_range_rows_kernel<uint16_t><<<grid, kThreads, 0, stream>>>((const uint16_t*)level, d[0], d[1], d[2], C,
-                                                                    ncx, ncy, out, tmp_hi);
+                                                                    ncx, ncy, cells, out, tmp_hi);
   pack_ranges_kernel<<<fill_blocks, kThreads, 0, stream>>>(out, tmp_hi, cells);
   return hipGetLastError();
 }

@@ -354,3 +354,28 @@ def test_memory_budget_is_enforced():
     finally:
         r.setMaxUsableGPUMem(before)
     r.cleanup()
+
+
+def test_cell_ranges_beyond_one_work_group_per_cell():
+    """A 1024^3 level cut into 8^3 empty-space cells (2.1 M cells, more than
+    the cell-range kernel's work-groups, so each loops over several cells):
+    skipping on and off give the same image, so no cell was left with a
+    wrong range."""
+    from imagevis3d_amd.renderer import VOL_NOISE_TUBES
+    n = 1024
+    print("case cell ranges 1024^3 u8, 8^3 cells", flush=True)
+    r = gpu_renderer(width=384, height=256)
+    r.setEmptySpaceCell(8)
+    r.setGeneratedVolume(VOL_NOISE_TUBES, 4, (n, n, n), 8, 128, 2, 1)
+    r.set1DTrans(transfer.tubes_tf(256, seed=4))
+    workloads.set_camera(r, Camera((1.2, 0.5, 1.4), (-1.2, -0.5, -1.4), fov=45.0))
+    on = r.capture()
+    s_on = r.frameStats().evaluated_samples
+    r.setEmptySpaceSkipping(False)
+    off = r.capture()
+    s_off = r.frameStats().evaluated_samples
+    print(f"evaluated samples: skipping {s_on}, without {s_off}", flush=True)
+    assert (on[..., :3] > 0).mean() > 0.05
+    assert np.array_equal(on, off)
+    assert s_on <= s_off
+    r.cleanup()

@@ -183,7 +183,7 @@ def run_serving(duration_s: float, rate_hz: float, seed: int = 5) -> dict:
     return out
 
 
-def run_batched(batches: int = 3, seed: int = 3) -> dict:
+def run_batched(batches: int = 3, seed: int = 3, tune: bool = True) -> dict:
     """The out-of-core batched workload: a 2048x2048x1024 uint16 volume
     (device-generated fBm + layered slabs, copied to host memory) streamed
     into a 64 GB device brick pool (128^3 bricks + 2-voxel apron, 5 LODs);
@@ -214,7 +214,9 @@ def run_batched(batches: int = 3, seed: int = 3) -> dict:
     page_s = time.perf_counter() - t0
     paged_bytes, paging_host_ms = r.pagingStats()
     t0 = time.perf_counter()
-    t = r.autotune(views, flags=r.TUNE_BRICK | r.TUNE_LAUNCH)
+    # tune=False (counter runs): the configured 128^3 bricks, launch shape
+    # timed only (flags 0 times the current settings)
+    t = r.autotune(views, flags=(r.TUNE_BRICK | r.TUNE_LAUNCH) if tune else 0)
     tune_s = time.perf_counter() - t0
     kms, nominal, evaluated = [], 0, 0
     t0 = time.perf_counter()
@@ -403,7 +405,18 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
     ok = [s for s in check_scales if check_cal[s]["rmse_max"] <= 1.0]
     check_scale = min(ok, key=lambda s: check_cal[s]["ms"]) if ok else min(check_scales)
 
-    full_ms, s_full = [], 0
+    # full resolution at its best: the uniform LOD-0 path with its
+    # empty-space cells and launch shape tuned for this volume (the default
+    # 16^3 cells cost a march step each), and per view the faster of it and
+    # the per-region path with every region at level 0
+    say("tuning the full-resolution path")
+    r.resize(eval_size)
+    t_full = r.autotune(cal_views[:2], flags=r.TUNE_CELL | r.TUNE_LAUNCH)
+    full_tuning = {"cell": t_full.cell, "samples_per_batch": t_full.samples_per_batch,
+                   "tile_width": t_full.tile_width, "tile_group": t_full.tile_group,
+                   "work_group_waves": t_full.wg_waves, "brick_staging": t_full.brick_staging,
+                   "tuned_ms": round(t_full.kernel_ms, 2), "start_ms": round(t_full.baseline_ms, 2)}
+    full_ms, full_uniform_ms, full_regions_ms, s_full = [], [], [], 0
     acc = {k: {"ms": [], "kernel_ms": [], "sel_ms": [], "rmse": [], "rmse_uniform": [], "mem": [],
                "samples": 0} for k in points}
     chk = {"rmse": [], "estimate": [], "fallbacks": 0}
@@ -419,10 +432,13 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
         r.setRegionLODs(None)
         assert r.paint(), r.lastError()
         st = r.frameStats()
-        full_ms.append(st.kernel_ms)
         s_full += st.evaluated_samples
         uni = r.capture()
         ref = LL.full_resolution(r, dims, edge)
+        reg0_ms = r.frameStats().kernel_ms  # the level-0 region frame just captured
+        full_uniform_ms.append(st.kernel_ms)
+        full_regions_ms.append(reg0_ms)
+        full_ms.append(min(st.kernel_ms, reg0_ms))
         for k, mb in points.items():
             a = acc[k]
             t0 = time.perf_counter()
@@ -481,6 +497,8 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
         "tile_sample": check_k, "rmse_bound": check_bound, "factor_scale": check_scale,
         "fps_learned": 1e3 / checked_pipe, "fps_full": 1e3 / full_pipe,
         "speedup": full_pipe / checked_pipe,
+        # against the per-view best full-resolution kernel time
+        "speedup_vs_full_kernel": float(np.mean(full_ms)) / checked_pipe,
         "rmse_mean": float(np.mean(chk["rmse"])), "rmse_max": float(np.max(chk["rmse"])),
         "frames_within_1": float(np.mean(np.asarray(chk["rmse"]) <= 1.0)),
         "estimate_mean": float(np.mean(chk["estimate"])),
@@ -503,7 +521,12 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
     out = {
         "workload": f"learned-lod-{n}^3-u8-noise-tubes-{edge}^3-bricks-{n_lod}lods-"
                     f"{eval_size[0]}x{eval_size[1]}-{eval_views}views",
+        # full resolution per view: the faster of the tuned uniform LOD-0
+        # path and the per-region path at level 0 (speed-ups are against it)
         "fps_full": 1e3 / float(np.mean(full_ms)),
+        "fps_full_uniform": 1e3 / float(np.mean(full_uniform_ms)),
+        "fps_full_regions_level0": 1e3 / float(np.mean(full_regions_ms)),
+        "full_tuning": full_tuning,
         "rmse_target": 1.0,
         "calibration": {"views": cal_views_n, "mean_rmse_target": cal_target},
         "training": {"views": train_views, "tfs": 4, "size": list(train_size)},
@@ -652,6 +675,8 @@ def main() -> int:
     ap.add_argument("--no-learned-lod", action="store_true", help="skip the learned-LOD workload")
     ap.add_argument("--learned-lod-only", action="store_true")
     ap.add_argument("--learned-lod-size", type=int, default=4096)
+    ap.add_argument("--staging", type=int, default=0,
+                    help="with --no-autotune: 1 marches the 1D TF over LDS-staged cells")
     ap.add_argument("--mlp-only", action="store_true",
                     help="time the learned-LOD MLP alone (random weights, no torch)")
     args = ap.parse_args()
@@ -667,7 +692,7 @@ def main() -> int:
         print(json.dumps(run_serving(args.serving_seconds, args.serving_rate)))
         return 0
     if args.batched_only:
-        print(json.dumps(run_batched()))
+        print(json.dumps(run_batched(tune=not args.no_autotune)))
         return 0
     if args.learned_lod_only:
         print(json.dumps(run_learned_lod(args.learned_lod_size)))
@@ -690,6 +715,7 @@ def main() -> int:
     r.set1DTrans(w.tf)
     r.setLODLimits((0, 0))
     r.setKernelTuning(args.batch, args.tile_group, args.xcd_split)
+    r.setBrickStaging(bool(args.staging))
     workloads.apply(r, w, w.cameras[0])
     if args.no_skip:
         r.setEmptySpaceSkipping(False)
@@ -704,7 +730,7 @@ def main() -> int:
         t = r.autotune(tv, flags=r.TUNE_CELL | r.TUNE_LAUNCH)
         out = {"brick": t.brick_size, "cell": t.cell, "samples_per_batch": t.samples_per_batch,
                "tile_width": t.tile_width, "tile_group": t.tile_group,
-               "work_group_waves": t.wg_waves,
+               "work_group_waves": t.wg_waves, "brick_staging": t.brick_staging,
                "tuned_ms": round(t.kernel_ms, 4), "start_ms": round(t.baseline_ms, 4),
                "settings_timed": t.candidates, "seconds": round(time.perf_counter() - t0, 2)}
         print(f"[bench] autotuned for {what}: {out}", file=sys.stderr, flush=True)

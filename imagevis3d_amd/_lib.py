@@ -82,7 +82,7 @@ class Tuning(C.Structure):
         ("brick_size", C.c_uint32), ("cell", C.c_uint32), ("samples_per_batch", C.c_uint32),
         ("tile_width", C.c_uint32), ("tile_group", C.c_uint32), ("xcd_split", C.c_uint32),
         ("kernel_ms", C.c_float), ("baseline_ms", C.c_float), ("candidates", C.c_uint32),
-        ("wg_waves", C.c_uint32),
+        ("wg_waves", C.c_uint32), ("brick_staging", C.c_uint32),
     ]
 
 
@@ -200,6 +200,7 @@ SIGNATURES = {
     "tvk_read_framebuffer": [H, P, u64],
     "tvk_set_kernel_tuning": [H, u32, u32, u32],
     "tvk_set_work_group_waves": [H, u32],
+    "tvk_set_brick_staging": [H, i32],
     "tvk_set_ray_tile_shape": [H, u32],
     "tvk_set_empty_space_cell": [H, u32],
     "tvk_set_1d_trans_slot": [H, u32, P, u32],

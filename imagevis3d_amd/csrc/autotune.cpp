@@ -12,7 +12,7 @@ namespace tvk {
 namespace {
 
 struct Setting {
-  uint32_t brick, cell, batch, tile_width, tile_group, wg_waves;
+  uint32_t brick, cell, batch, tile_width, tile_group, wg_waves, stage;
 };
 
 }  // namespace
@@ -42,13 +42,14 @@ int HipRenderer::Autotune(const tvk_view* views, uint32_t n, uint32_t flags, tvk
     return TVK_OK;
   };
   Setting cur{(uint32_t)brick_, (uint32_t)cell_, (uint32_t)batch_, 1u << tile_shift_,
-              (uint32_t)tile_group_, (uint32_t)wg_waves_};
+              (uint32_t)tile_group_, (uint32_t)wg_waves_, stage_ ? 1u : 0u};
   auto apply = [&](const Setting& s, bool rebrick) -> int {
     if (rebrick) {
       if (int e = Rebrick(s.brick, s.cell)) return e;
     }
     if (int e = SetKernelTuning(s.batch, s.tile_group, (uint32_t)xcd_split_)) return e;
     if (int e = SetWorkGroupWaves(s.wg_waves)) return e;
+    if (int e = SetBrickStaging(s.stage != 0)) return e;
     return SetRayTileShape(s.tile_width);
   };
   float best = 0.f;
@@ -82,10 +83,24 @@ int HipRenderer::Autotune(const tvk_view* views, uint32_t n, uint32_t flags, tvk
   };
 
   const uint64_t maxdim = std::max(dims0_[0], std::max(dims0_[1], dims0_[2]));
+  // Launch shape first (cheap to try), then the bricking with that shape,
+  // then the launch shape again for the chosen bricks: tuning the bricks
+  // under the starting launch shape can settle on bricks that are only best
+  // for that shape.
+  auto tune_launch = [&]() -> int {
+    if (int e = try_values({2u, 4u, 8u}, &Setting::batch, false)) return e;
+    if (int e = try_values({4u, 8u, 16u}, &Setting::tile_width, false)) return e;
+    if (int e = try_values({1u, 2u, 4u, 8u, 16u}, &Setting::tile_group, false)) return e;
+    if (int e = try_values({4u, 8u}, &Setting::wg_waves, false)) return e;
+    return try_values({0u, 1u}, &Setting::stage, false);
+  };
+  if (flags & TVK_TUNE_LAUNCH)
+    if (int e = tune_launch()) return e;
   // a paged volume re-bricks from the caller's flat voxels (a brick
   // source fixes the brick size); each candidate is timed after a warm-up
   // batch has paged its bricks in
   const bool rebrickable = !paged_ || host_level0_ != nullptr;
+  const Setting before = cur;
   if ((flags & TVK_TUNE_BRICK) && rebrickable) {
     std::vector<uint32_t> bricks;
     for (uint32_t b : {32u, 64u, 128u, 256u})
@@ -99,12 +114,8 @@ int HipRenderer::Autotune(const tvk_view* views, uint32_t n, uint32_t flags, tvk
     if (std::find(cells.begin(), cells.end(), cur.brick) == cells.end()) cells.push_back(cur.brick);
     if (int e = try_values(cells, &Setting::cell, true)) return e;
   }
-  if (flags & TVK_TUNE_LAUNCH) {
-    if (int e = try_values({2u, 4u, 8u}, &Setting::batch, false)) return e;
-    if (int e = try_values({4u, 8u, 16u}, &Setting::tile_width, false)) return e;
-    if (int e = try_values({1u, 2u, 4u, 8u, 16u}, &Setting::tile_group, false)) return e;
-    if (int e = try_values({4u, 8u}, &Setting::wg_waves, false)) return e;
-  }
+  if ((flags & TVK_TUNE_LAUNCH) && (cur.brick != before.brick || cur.cell != before.cell))
+    if (int e = tune_launch()) return e;
   if (out) {
     *out = tvk_tuning{};
     out->brick_size = (uint32_t)brick_;
@@ -114,6 +125,7 @@ int HipRenderer::Autotune(const tvk_view* views, uint32_t n, uint32_t flags, tvk
     out->tile_group = (uint32_t)tile_group_;
     out->xcd_split = (uint32_t)xcd_split_;
     out->wg_waves = (uint32_t)wg_waves_;
+    out->brick_staging = stage_ ? 1u : 0u;
     out->kernel_ms = best;
     out->baseline_ms = baseline;
     out->candidates = (uint32_t)tried;

@@ -111,6 +111,7 @@ struct LaunchInfo {
   int regions = 0;      // 1D TF with per-region LOD (jobs carry region codes)
   int compute_units = 0;
   int wg_waves = 4;      // waves per work-group of variants without an LDS TF (4 or 8)
+  int stage = 0;        // 1D TF without lighting, trilinear: voxels of each cell staged in LDS
 };
 
 // One raycast launch over a device job table `tab` (its header counters

@@ -216,7 +216,9 @@ __device__ inline void fetch_value_grad(const T* __restrict__ base, const SlotGe
 // kModeMIP: maximum intensity projection -- the largest sample value along
 // the ray through the 1D TF; every sample is visited (a TF-transparent cell
 // may hold the maximum) and rays do not terminate early.
-constexpr int kMode1D = 0, kModeGrad = 1, kModeIso = 2, kModeRegions = 3, kModeMIP = 4;
+// kMode1DStaged: the 1D march with the voxels of each 8^3 slot cell staged
+// in LDS (render_tile_staged).
+constexpr int kMode1D = 0, kModeGrad = 1, kModeIso = 2, kModeRegions = 3, kModeMIP = 4, kMode1DStaged = 5;
 
 // Device address of brick gid's slot from the page table (null: the brick
 // is not resident).  Slots may live in any allocation of the device pool.
@@ -851,6 +853,294 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
   gstore(P.framebuffer + ((int64_t)py * P.width + px), over_background(P, py, C0, C1, C2, A));
 }
 
+// LDS-staged 1D march (kMode1DStaged: 1D TF without lighting, trilinear).
+// The wave marches its tile's rays in rounds.  In each round every lane
+// walks its ray over empty-space cells (skipping empty and non-resident
+// ones, exactly as render_tile does) to its next sample that needs voxels;
+// the wave then copies the 8^3 slot cell of the first such lane -- the
+// voxels [8c, 8c + 8] along each axis, 9^3 of them, from the 2 x 3 x 3
+// blocks of the blocked slot layout that hold them -- into its LDS buffer
+// with all 64 lanes, and every lane whose next sample lies in that cell
+// marches through the cell reading the 8 trilinear corners from LDS.  The
+// samples, weights, TF lookups and compositing are those of render_tile's
+// 1D march, so the image is the same bit for bit; what changes is that the
+// voxels come in as contiguous block loads shared by the lanes instead of
+// four scattered x-pair gathers per sample.
+constexpr int kStageEdge = 9;                  // cell of 8 + 1 voxel for the trilinear upper corner
+constexpr int kStageElems = 736;               // 9^3 = 729, padded
+constexpr int kStageRuns = 9;                  // (y, z) block pairs of a staged cell
+constexpr int kStageRunElems = 2 * kSlotBlockElems;  // two x-blocks per run
+
+template <typename T>
+__device__ inline void stage_cell(const T* __restrict__ base, const SlotGeo& G, int cx, int cy, int cz,
+                                  int lane, T* __restrict__ sbuf) {
+  const int nb = slot_blocks(G.S);
+  const int bx0 = 2 * cx, by0 = 2 * cy, bz0 = 2 * cz;
+  constexpr int EPW = 4 / (int)sizeof(T);            // elements per dword
+  constexpr int RUN_DW = kStageRunElems / EPW;        // dwords per run
+  constexpr int TOTAL = kStageRuns * RUN_DW;          // dwords of the cell's blocks
+  constexpr int PER_LANE = (TOTAL + 63) / 64;
+  // every load first (independent, in flight together), then the LDS writes
+  uint32_t w[PER_LANE];
+  int src[PER_LANE];  // first element of the dword in its block (z * 20 + y * 5 + x), -1: none
+  int blk[PER_LANE];  // staged block: xb | yb << 2 | zb << 4
+#pragma unroll
+  for (int m = 0; m < PER_LANE; ++m) {
+    const int dw = lane + 64 * m;
+    const int r = dw / RUN_DW, q = dw - r * RUN_DW;
+    const int yb = r % 3, zb = r / 3;
+    const int e0 = q * EPW;                          // first element of this dword in the run
+    const int xb = e0 / kSlotBlockElems;             // 0 or 1 (a dword never straddles blocks)
+    // dwords past the cell's blocks or beyond the slot (never sampled) are not loaded
+    const bool ok = dw < TOTAL && bx0 + xb < nb && by0 + yb < nb && bz0 + zb < nb;
+    src[m] = ok ? e0 - xb * kSlotBlockElems : -1;
+    blk[m] = xb | (yb << 2) | (zb << 4);
+    w[m] = 0u;
+    if (ok) {
+      const int64_t off = (int64_t)(bx0 + xb) * kSlotBlockElems + (int64_t)(by0 + yb) * G.by +
+                          (int64_t)(bz0 + zb) * G.bz + src[m];
+      w[m] = *(__attribute__((address_space(1))) const uint32_t*)(base + off);
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < PER_LANE; ++m) {
+    if (src[m] < 0) continue;
+    const int xb = blk[m] & 3, yb = (blk[m] >> 2) & 3, zb = blk[m] >> 4;
+#pragma unroll
+    for (int i = 0; i < EPW; ++i) {
+      const int j = src[m] + i;  // element in the block: z * 20 + y * 5 + x
+      const int zi = j / 20, yi = (j - zi * 20) / 5, xi = j - zi * 20 - yi * 5;
+      const int X = xb * 4 + xi, Y = yb * 4 + yi, Z = zb * 4 + zi;
+      if (X < kStageEdge && Y < kStageEdge && Z < kStageEdge)
+        sbuf[(Z * kStageEdge + Y) * kStageEdge + X] = (T)(w[m] >> (8 * sizeof(T) * i));
+    }
+  }
+}
+
+// LDS writes of one wave visible to the same wave's later reads (and the
+// reads done before the next round's writes).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T, bool TF_LDS>
+__device__ inline void render_tile_staged(const RaycastParams& P, const float4* __restrict__ tf4, int tile,
+                                          int lane, uint32_t& n_eval, uint32_t& n_nom, T* __restrict__ sbuf) {
+  if (P.tile_sample > 1) {
+    const uint32_t h = ((uint32_t)tile * 0x9E3779B1u + (uint32_t)P.tile_phase * 0x85EBCA77u) >> 16;
+    tile = tile * P.tile_sample + (int)(h % (uint32_t)P.tile_sample);
+    if (tile >= P.rect_w * P.rect_h) return;  // wave-uniform
+  }
+  const int tx = P.rect_x0 + tile % P.rect_w, ty = P.rect_y0 + centre_out_row(tile / P.rect_w, P.rect_h);
+  const int sh = P.tile_shift;
+  const int px = (tx << sh) + (lane & ((1 << sh) - 1));
+  const int py = ty * (64 >> sh) + (lane >> sh);
+  // every lane stays in the rounds (the staging needs all 64); lanes off
+  // the image only help load
+  const bool on_image = px < P.width && py < P.height;
+  float o[3], d[3];
+  const float fx = (float)px, fy = (float)py;
+  if (P.ortho) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      o[i] = P.eye[i] + fx * P.du[i] + fy * P.dv[i];
+      d[i] = P.odir[i];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      o[i] = P.eye[i];
+      d[i] = P.dir00[i] + fx * P.du[i] + fy * P.dv[i];
+    }
+    const float l = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) d[i] = d[i] / l;
+  }
+  float t0 = -kInf, t1 = kInf, inv[3];
+  bool hit = on_image;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (d[i] == 0.f) {
+      inv[i] = kInf;
+      if (o[i] < 0.f || o[i] > P.box[i]) hit = false;
+    } else {
+      inv[i] = 1.f / d[i];
+      float ta = (0.f - o[i]) * inv[i], tb = (P.box[i] - o[i]) * inv[i];
+      if (ta > tb) {
+        const float s = ta;
+        ta = tb;
+        tb = s;
+      }
+      t0 = fmaxf(t0, ta);
+      t1 = fminf(t1, tb);
+    }
+  }
+  t0 = fmaxf(t0, 0.f);
+  if (hit && t1 > t0) n_nom += (uint32_t)floorf((t1 - t0) * P.nominal_rate);
+  if (P.clip_on && hit) clip_ray(P, o, d, t0, t1, hit);
+  const float L = t1 - t0;
+  float C0 = 0.f, C1 = 0.f, C2 = 0.f, A = 0.f;
+  const float step = P.step;
+  const int n = (hit && L > 0.f) ? (int)ceilf(L / step) : 0;
+  const float lastlen = fmaxf(L - (float)(n - 1) * step, 0.f);
+  const LodInfo& lv = P.level;
+  const int U = P.cell;
+  const int per_brick = P.brick / U;
+  const float invU = 1.f / (float)U;
+  float od[3], oo[3], ul0[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    od[i] = d[i] * lv.to_level[i];
+    oo[i] = o[i] * lv.to_level[i];
+    ul0[i] = (float)U / lv.to_level[i];
+  }
+  const int ucount[3] = {(lv.dims[0] + U - 1) / U, (lv.dims[1] + U - 1) / U, (lv.dims[2] + U - 1) / U};
+  const float tfmax = (float)(P.n_tf - 1);
+  const SlotGeo G = slot_geo(P.slot_edge);
+  const int Smax = G.S - 2;
+  int k = 0, kend = 0, gid = -1, blend_gid = -1;
+  const T* __restrict__ base = nullptr;
+  float g0[3] = {0.f, 0.f, 0.f};
+  bool done = n == 0, live = true;
+  while (true) {
+    // walk to the next sample that needs voxels (a resident, non-empty unit)
+    bool need = false;
+    while (!done) {
+      if (k >= kend) {
+        if (k >= n) {
+          done = true;
+          break;
+        }
+        const float len0 = (k == n - 1) ? lastlen : step;
+        const float tk = fmaf((float)k, step, t0) + 0.5f * len0;
+        int c[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+          c[i] = clampi((int)floorf(fmaf(tk, od[i], oo[i]) * invU), 0, ucount[i] - 1);
+        float tb = kInf;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          if (d[i] != 0.f) {
+            const float face = (float)(d[i] > 0.f ? c[i] + 1 : c[i]) * ul0[i];
+            tb = fminf(tb, (face - o[i]) * inv[i]);
+          }
+        }
+        kend = min(max((int)ceilf((tb - t0) / step - 0.5f), k + 1), n);
+        int b[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) b[i] = c[i] / per_brick;
+        gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
+        const uint8_t empty =
+            P.skip ? gload(P.cell_empty + (lv.cell_base + (c[2] * ucount[1] + c[1]) * ucount[0] + c[0])) : 0;
+        base = brick_base<T>(P, gid);
+        if (P.blend_bits < 32 && gid != blend_gid) {
+          blend_round(P.blend_bits, C0, C1, C2, A);
+          blend_gid = gid;
+        }
+        if (empty) {
+          k = kend;
+          continue;
+        }
+        if (!base) {
+          if (P.requests) P.requests[gid] = 1u;
+          k = kend;
+          continue;
+        }
+        if (P.touch) P.touch[gid] = P.frame;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) g0[i] = oo[i] - (float)(b[i] * P.brick - P.apron) - 0.5f;
+      }
+      need = true;
+      break;
+    }
+    const unsigned long long want = __ballot(need);
+    if (want == 0ull) break;  // wave-uniform: every lane is done
+    // this lane's slot cell (8^3 of slot texels) at sample k
+    int cell = -1;
+    float tnext = kInf;
+    if (need) {
+      const float len0 = (k == n - 1) ? lastlen : step;
+      const float tk = fmaf((float)k, step, t0) + 0.5f * len0;
+      const int ix = clampi((int)floorf(fmaf(tk, od[0], g0[0])), 0, Smax) >> 3;
+      const int iy = clampi((int)floorf(fmaf(tk, od[1], g0[1])), 0, Smax) >> 3;
+      const int iz = clampi((int)floorf(fmaf(tk, od[2], g0[2])), 0, Smax) >> 3;
+      cell = (iz << 16) | (iy << 8) | ix;
+      tnext = tk;
+    }
+    // the leader is the lane furthest behind along its ray: the tile's rays
+    // then cross the cells front to back together, and the lanes that
+    // share the leader's cell march through it in the same round
+    float tmin = tnext;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) tmin = fminf(tmin, __shfl_xor(tmin, o, 64));
+    const unsigned long long behind = __ballot(need && tnext == tmin);
+    const int lead = __builtin_ctzll(behind ? behind : want);  // (want when no t compares equal)
+    const int lead_cell = __builtin_amdgcn_readlane(cell, lead);
+    const int lead_gid = __builtin_amdgcn_readlane(gid, lead);
+    const uint64_t bp = (uint64_t)(uintptr_t)base;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)bp, lead);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(bp >> 32), lead);
+    const T* lead_base = (const T*)(uintptr_t)(((uint64_t)hi << 32) | lo);
+    const int cx = lead_cell & 0xff, cy = (lead_cell >> 8) & 0xff, cz = lead_cell >> 16;
+    stage_cell<T>(lead_base, G, cx, cy, cz, lane, sbuf);
+    wave_lds_sync();
+    if (need && cell == lead_cell && gid == lead_gid) {
+      const int ox = 8 * cx, oy = 8 * cy, oz = 8 * cz;
+      while (k < kend) {
+        const float lenj = (k == n - 1) ? lastlen : step;
+        const float t = fmaf((float)k, step, t0) + 0.5f * lenj;
+        const float gx = fmaf(t, od[0], g0[0]), gy = fmaf(t, od[1], g0[1]), gz = fmaf(t, od[2], g0[2]);
+        const int ix = clampi((int)floorf(gx), 0, Smax);
+        const int iy = clampi((int)floorf(gy), 0, Smax);
+        const int iz = clampi((int)floorf(gz), 0, Smax);
+        if ((ix >> 3) != cx || (iy >> 3) != cy || (iz >> 3) != cz) break;  // next round
+        const float wx = clampf(gx - (float)ix, 0.f, 1.f);
+        const float wy = clampf(gy - (float)iy, 0.f, 1.f);
+        const float wz = clampf(gz - (float)iz, 0.f, 1.f);
+        const T* q = sbuf + (((iz - oz) * kStageEdge + (iy - oy)) * kStageEdge + (ix - ox));
+        const float c000 = (float)q[0], c100 = (float)q[1];
+        const float c010 = (float)q[kStageEdge], c110 = (float)q[kStageEdge + 1];
+        const float c001 = (float)q[kStageEdge * kStageEdge], c101 = (float)q[kStageEdge * kStageEdge + 1];
+        const float c011 = (float)q[kStageEdge * kStageEdge + kStageEdge];
+        const float c111 = (float)q[kStageEdge * kStageEdge + kStageEdge + 1];
+        const float c00 = fmaf(c100 - c000, wx, c000);
+        const float c10 = fmaf(c110 - c010, wx, c010);
+        const float c01 = fmaf(c101 - c001, wx, c001);
+        const float c11 = fmaf(c111 - c011, wx, c011);
+        const float c0 = fmaf(c10 - c00, wy, c00);
+        const float c1 = fmaf(c11 - c01, wy, c01);
+        const float v = fmaf(c1 - c0, wz, c0);
+        ++n_eval;
+        const float x = clampf(v * P.tf_scale, 0.f, tfmax);
+        const int ti = (int)x;
+        const float fr = x - (float)ti;
+        const float4 e0 = tf_entry<TF_LDS>(tf4, ti);
+        const float4 e1 = tf_entry<TF_LDS>(tf4, min(ti + 1, P.n_tf - 1));
+        const float a = fmaf(e1.w - e0.w, fr, e0.w);
+        ++k;
+        if (a > 0.f) {
+          const float ac = a >= 1.f ? 1.f : 1.f - exp2f(lenj * log2f(1.f - a));
+          const float w = (1.f - A) * ac;
+          C0 = fmaf(w, fmaf(e1.x - e0.x, fr, e0.x), C0);
+          C1 = fmaf(w, fmaf(e1.y - e0.y, fr, e0.y), C1);
+          C2 = fmaf(w, fmaf(e1.z - e0.z, fr, e0.z), C2);
+          A += w;
+          live = A < P.term_alpha;
+          if (!live) {
+            done = true;
+            break;
+          }
+        }
+      }
+    }
+    wave_lds_sync();  // the cell's reads end before the next round's writes
+  }
+  if (P.blend_bits < 32) blend_round(P.blend_bits, C0, C1, C2, A);
+  if (on_image) gstore(P.framebuffer + ((int64_t)py * P.width + px), over_background(P, py, C0, C1, C2, A));
+}
+
 // Empty flag per empty-space cell for one TF: the TF entries (1D) or
 // columns (2D) that the cell's value range reaches hold no alpha > 0, or
 // (isosurface) the cell's maximum is below the iso value.  Such cells
@@ -1006,7 +1296,13 @@ __global__ __launch_bounds__(kThreads, 2) void raycast_kernel(const JobTable* __
         cur_job = job;
         n_eval = n_nom = 0;
       }
-      render_tile<T, LINEAR, BATCH, MODE, TF_LDS>(P, tf4, tile - tab->tile_start[job], lane, n_eval, n_nom);
+      if constexpr (MODE == kMode1DStaged) {
+        // this wave's staging buffer follows the LDS TF
+        T* sbuf = (T*)(s_tf + (TF_LDS ? H.n_tf_lds : 0)) + (threadIdx.x >> 6) * kStageElems;
+        render_tile_staged<T, TF_LDS>(P, tf4, tile - tab->tile_start[job], lane, n_eval, n_nom, sbuf);
+      } else {
+        render_tile<T, LINEAR, BATCH, MODE, TF_LDS>(P, tf4, tile - tab->tile_start[job], lane, n_eval, n_nom);
+      }
     }
   }
   if (cur_job >= 0) {
@@ -1021,7 +1317,7 @@ __global__ __launch_bounds__(kThreads, 2) void raycast_kernel(const JobTable* __
 template <typename T, bool LINEAR, int BATCH, bool TF_LDS, int MODE>
 hipError_t launch_one(const LaunchInfo& li, const JobTable* tab, hipStream_t stream) {
   auto kern = raycast_kernel<T, LINEAR, BATCH, TF_LDS, MODE>;
-  const size_t lds = TF_LDS ? (size_t)li.n_tf_lds * sizeof(float4) : 0;
+  size_t lds = TF_LDS ? (size_t)li.n_tf_lds * sizeof(float4) : 0;
   int per_cu = 0;
   // Work-group size: 8 waves when the 1D TF is staged in LDS (64 KB per
   // work-group, so two fit a CU's 160 KB); otherwise li.wg_waves (4 or 8).
@@ -1030,6 +1326,7 @@ hipError_t launch_one(const LaunchInfo& li, const JobTable* tab, hipStream_t str
   // times (12 waves), which is faster on cache-friendly volumes and slower
   // where more waves in flight thrash the caches (the autotuner picks).
   const int kT = TF_LDS ? kThreads : 64 * (li.wg_waves == 4 ? 4 : 8);
+  if (MODE == kMode1DStaged) lds += (size_t)(kT / 64) * kStageElems * sizeof(T);  // per-wave cell buffers
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kT, lds);
   if (e != hipSuccess) return e;
   if (per_cu < 1) per_cu = 1;
@@ -1050,6 +1347,9 @@ hipError_t launch_typed(const LaunchInfo& li, const JobTable* tab, hipStream_t s
                            : launch_one<T, LINEAR, 4, false, kModeMIP>(li, tab, stream);
   if (li.mode == 1) return launch_one<T, LINEAR, 2, false, kModeGrad>(li, tab, stream);
   const bool lds = li.n_tf_lds > 0;
+  if (li.stage && LINEAR && !li.lighting)
+    return lds ? launch_one<T, true, 1, true, kMode1DStaged>(li, tab, stream)
+               : launch_one<T, true, 1, false, kMode1DStaged>(li, tab, stream);
   if (li.lighting)
     return lds ? launch_one<T, LINEAR, 2, true, kModeGrad>(li, tab, stream)
                : launch_one<T, LINEAR, 2, false, kModeGrad>(li, tab, stream);

@@ -1347,6 +1347,11 @@ int HipRenderer::SetWorkGroupWaves(uint32_t waves) {
   return TVK_OK;
 }
 
+int HipRenderer::SetBrickStaging(bool on) {
+  stage_ = on;
+  return TVK_OK;
+}
+
 int HipRenderer::SetRayTileShape(uint32_t tile_width) {
   if (tile_width != 4 && tile_width != 8 && tile_width != 16) return arg_fail("tile width must be 4, 8 or 16");
   tile_shift_ = tile_width == 4 ? 2 : (tile_width == 8 ? 3 : 4);
@@ -1580,6 +1585,7 @@ LaunchInfo HipRenderer::launch_info(int n_tf_lds) const {
   li.regions = (regions_active() && li.mode == 0 && !li.lighting) ? 1 : 0;
   li.compute_units = compute_units_;
   li.wg_waves = wg_waves_;
+  li.stage = (stage_ && li.mode == 0 && !li.lighting && !li.regions && li.interp == 1) ? 1 : 0;
   return li;
 }
 
@@ -1656,7 +1662,7 @@ int launch_job_table(LaunchSlot& sl, const RaycastParams* jobs, int n, const Lau
   const uint64_t key = 1ull | (uint64_t)li.bpv << 1 | (uint64_t)li.interp << 3 | (uint64_t)li.mode << 4 |
                        (uint64_t)li.lighting << 6 | (uint64_t)li.batch << 7 | (uint64_t)li.regions << 11 |
                        (uint64_t)li.n_tf_lds << 12 |
-                       (uint64_t)(li.wg_waves == 4 ? 1 : 0) << 40;
+                       (uint64_t)(li.wg_waves == 4 ? 1 : 0) << 40 | (uint64_t)li.stage << 41;
   if (!sl.exec || sl.graph_key != key) {
     if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
     if (sl.graph) (void)hipGraphDestroy(sl.graph);

@@ -228,7 +228,8 @@ void RenderServer::paged_loop(PagedVolume* pv) {
 
 uint64_t RenderServer::variant_key(const LaunchInfo& li) {
   return (uint64_t)li.bpv | (uint64_t)li.interp << 4 | (uint64_t)li.mode << 8 |
-         (uint64_t)li.lighting << 12 | (uint64_t)li.batch << 16 | (uint64_t)li.regions << 24;
+         (uint64_t)li.lighting << 12 | (uint64_t)li.batch << 16 | (uint64_t)li.regions << 24 |
+         (uint64_t)li.stage << 28;
 }
 
 int RenderServer::Submit(const tvk_request& q, uint64_t* ticket) {

@@ -706,6 +706,11 @@ int tvk_set_work_group_waves(tvk_renderer* r, uint32_t waves) {
   return r->impl.SetWorkGroupWaves(waves);
 }
 
+int tvk_set_brick_staging(tvk_renderer* r, int on) {
+  TVK_GUARD(r);
+  return r->impl.SetBrickStaging(on != 0);
+}
+
 int tvk_set_ray_tile_shape(tvk_renderer* r, uint32_t tile_width) {
   TVK_GUARD(r);
   return r->impl.SetRayTileShape(tile_width);

@@ -649,6 +649,10 @@ class Renderer:
         LDS TF (no image change)."""
         call("tvk_set_work_group_waves", self._h, int(waves))
 
+    def setBrickStaging(self, on: bool) -> None:
+        """1D march over LDS-staged 8^3 slot cells (no image change)."""
+        call("tvk_set_brick_staging", self._h, int(bool(on)))
+
     # --------------------------------------------------------------- data
     def getDataset(self) -> Dataset:
         return Dataset(self, self._path)

@@ -440,6 +440,14 @@ int tvk_set_kernel_tuning(tvk_renderer* r, uint32_t samples_per_batch, uint32_t 
  * waves hide more latency on cache-friendly volumes and thrash the caches
  * on very large ones.  The image does not depend on it. */
 int tvk_set_work_group_waves(tvk_renderer* r, uint32_t waves);
+/* LDS staging of the 1D march (no counterpart in the reference; chosen by
+ * the autotuner, TVK_TUNE_LAUNCH): with on = 1, 1D-TF launches without
+ * lighting and with trilinear sampling march each wave's rays in rounds
+ * over 8^3 cells of the brick slots, the voxels of each cell copied into
+ * LDS by the whole wave (contiguous block loads) and read from there,
+ * instead of four x-pair gathers per sample.  The image and the sample
+ * counts do not depend on it.  Default 0. */
+int tvk_set_brick_staging(tvk_renderer* r, int on);
 /* Ray tile shape of one 64-lane wavefront (no counterpart in the
  * reference; chosen by the autotuner): tile_width x (64 / tile_width) rays,
  * tile_width in {4, 8, 16} (default 8).  The image does not depend on it. */
@@ -655,6 +663,7 @@ typedef struct tvk_tuning {
   float baseline_ms;        /* the same views under the starting settings */
   uint32_t candidates;      /* settings timed                            */
   uint32_t wg_waves;        /* tvk_set_work_group_waves                  */
+  uint32_t brick_staging;   /* tvk_set_brick_staging                     */
 } tvk_tuning;
 int tvk_autotune(tvk_renderer* r, const tvk_view* views, uint32_t n, uint32_t flags,
                  tvk_tuning* out);

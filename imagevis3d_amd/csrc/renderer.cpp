@@ -740,6 +740,7 @@ int HipRenderer::prepare_regions(int tf_slot) {
 
 int HipRenderer::Rebrick(uint32_t brick, uint32_t cell) {
   if (!has_volume_) return state_fail("rebrick before a volume is set");
+  if (int e = resolve_check()) return e;  // its launches read the slots being replaced
   const int saved = cell_pref_;
   if (paged_) {
     // a paged volume over the caller's flat voxels is set up again from
@@ -1085,6 +1086,8 @@ float HipRenderer::iso_skip_threshold() const {
 }
 
 int HipRenderer::upload_tf() {
+  // a pending region check's fallback reads the current TF buffers
+  if (int e = resolve_check()) return e;
   bool copied = false;
   for (TfSet& t : tfs_) {
     if (t.dirty2d) {
@@ -1893,6 +1896,7 @@ int HipRenderer::RenderBatch(const tvk_view* views, uint32_t n, tvk_view_stats* 
   if (!has_volume_) return state_fail("render before a dataset is loaded");
   if (n == 0) return TVK_OK;
   if (!views) return arg_fail("null view array");
+  if (int e = resolve_check()) return e;  // its fallback reads buffers a batch may replace
   std::vector<CameraState> cams(n);
   uint64_t fb_bytes = 0;
   for (uint32_t i = 0; i < n; ++i) {

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from conftest import gpu_renderer
-from imagevis3d_amd import TvkError, workloads
+from imagevis3d_amd import TvkError, transfer, workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -109,4 +109,28 @@ def test_pipelined_paints_resolve_like_captures():
     r.synchronize()
     assert np.array_equal(r.readFramebuffer(), single[-1][0])
     assert r.regionLODCheck().fell_back == single[-1][1]
+    r.cleanup()
+
+
+def test_pending_check_resolved_before_a_batch_replaces_buffers():
+    """A checked paint whose fallback is still pending, then a larger TF and
+    a batched launch (which re-uploads the TF into a new buffer): the batch
+    resolves the check first, so the frame is the level-0 frame of the old
+    TF, and the batch renders with the new one."""
+    w, r = setup()
+    print("case region check pending across a batch", flush=True)
+    levels = np.full((4, 4, 4), 2, np.uint8)
+    cam = w.cameras[6]
+    workloads.apply(r, w, cam)
+    r.setRegionLODs(np.zeros_like(levels))
+    full = r.capture()
+    r.setRegionLODs(levels)
+    r.setRegionLODCheck(4, 0.0)  # every frame falls back
+    assert r.paint()             # check launched, not resolved
+    r.set1DTrans(transfer.ramp_tf(8192, 1000, 3000))
+    imgs, _, _ = r.renderBatch([workloads.view_of(cam, w.width, w.height, 0)])
+    assert np.array_equal(r.readFramebuffer(), full)
+    assert r.regionLODCheck().fell_back == 1
+    r.setRegionLODCheck(0, 0.0)
+    assert not np.array_equal(imgs[0], full)  # the batch used the new TF
     r.cleanup()

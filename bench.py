@@ -117,8 +117,7 @@ def run_serving(duration_s: float, rate_hz: float, seed: int = 5) -> dict:
     rs = workloads.serving_pool()
     setup_s = time.perf_counter() - t0
     # each volume's launch shape tuned once on one oblique 512^2 view (setup,
-    # not timed; the default 8-sample gather batch is the slowest choice on
-    # most volumes)
+    # not timed)
     t0 = time.perf_counter()
     tuned = []
     for r in rs:
@@ -203,8 +202,8 @@ def run_batched(batches: int = 3, seed: int = 3, tune: bool = True) -> dict:
     4 seeded 2D TFs (256x256 Gaussian lobes over value x gradient
     magnitude) with Phong shading; 8 orbit cameras x 4 TFs = 32 views at
     1920x1080 per persistent-kernel launch; brick size (32^3 ... 256^3,
-    the paged volume set up again from the host copy for each) and launch
-    shape autotuned on the 32 views."""
+    the paged volume set up again from the host copy for each), empty-space
+    cell and launch shape autotuned on the 32 views."""
     from imagevis3d_amd import tuvok, workloads
     from imagevis3d_amd.renderer import VOL_FBM_PLANES
 
@@ -229,7 +228,7 @@ def run_batched(batches: int = 3, seed: int = 3, tune: bool = True) -> dict:
     t0 = time.perf_counter()
     # tune=False (counter runs): the configured 128^3 bricks, launch shape
     # timed only (flags 0 times the current settings)
-    t = r.autotune(views, flags=(r.TUNE_BRICK | r.TUNE_LAUNCH) if tune else 0)
+    t = r.autotune(views, flags=(r.TUNE_BRICK | r.TUNE_CELL | r.TUNE_LAUNCH) if tune else 0)
     tune_s = time.perf_counter() - t0
     kms, nominal, evaluated = [], 0, 0
     t0 = time.perf_counter()
@@ -251,7 +250,7 @@ def run_batched(batches: int = 3, seed: int = 3, tune: bool = True) -> dict:
         "evaluated_gsamples_per_s": evaluated / wall / 1e9,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS},
-        "tuned": {"brick": t.brick_size, "samples_per_batch": t.samples_per_batch,
+        "tuned": {"brick": t.brick_size, "cell": t.cell, "samples_per_batch": t.samples_per_batch,
                   "tile_width": t.tile_width, "tile_group": t.tile_group,
                   "work_group_waves": t.wg_waves, "tuned_ms": round(t.kernel_ms, 2),
                   "start_ms": round(t.baseline_ms, 2), "settings_timed": t.candidates,

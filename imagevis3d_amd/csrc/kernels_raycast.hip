@@ -628,6 +628,147 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
     constexpr bool kBlends = MODE == kMode1D || MODE == kModeGrad;
     int blend_gid = -1;  // brick of the last blend into the target
     int k = 0;
+    if constexpr (MODE == kModeGrad || MODE == kMode1D) {
+      // The 1D and gradient marches, flattened: each pass of the loop first sets up
+      // the next march unit for the lanes whose unit ended (skipping empty
+      // and non-resident ones), then gathers one batch of samples for every
+      // lane inside a unit.  A nested per-unit sample loop would hold the
+      // wave at every unit boundary until its lane with the most samples
+      // left in its unit is done; here a lane whose unit ends goes on with
+      // its next unit in the next pass.  Samples, their order and the
+      // arithmetic are those of the nested form.
+      int kend = 0;
+      float g0[3] = {0.f, 0.f, 0.f};
+      const T* __restrict__ base = nullptr;
+      bool live = true;
+      while (k < n) {
+        if (k >= kend) {
+          const float len0 = (k == n - 1) ? lastlen : step;
+          const float tk = fmaf((float)k, step, t0) + 0.5f * len0;
+          int c[3];
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+            c[i] = clampi((int)floorf(fmaf(tk, od[i], oo[i]) * invU), 0, ucount[i] - 1);
+          float tb = kInf;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            if (d[i] != 0.f) {
+              const float face = (float)(d[i] > 0.f ? c[i] + 1 : c[i]) * ul0[i];
+              tb = fminf(tb, (face - o[i]) * inv[i]);
+            }
+          }
+          kend = min(max((int)ceilf((tb - t0) / step - 0.5f), k + 1), n);
+          int b[3];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) b[i] = c[i] / per_brick;
+          const int gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
+          const uint8_t empty =
+              P.skip ? gload(P.cell_empty + (lv.cell_base + (c[2] * ucount[1] + c[1]) * ucount[0] + c[0])) : 0;
+          base = brick_base<T>(P, gid);
+          if (P.blend_bits < 32 && gid != blend_gid) {
+            blend_round(P.blend_bits, C0, C1, C2, A);
+            blend_gid = gid;
+          }
+          if (empty) {
+            k = kend;
+            continue;
+          }
+          if (!base) {
+            if (P.requests) P.requests[gid] = 1u;
+            k = kend;
+            continue;
+          }
+          if (P.touch) P.touch[gid] = P.frame;
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+            g0[i] = oo[i] - (float)(b[i] * P.brick - P.apron) - (LINEAR ? 0.5f : 0.f);
+        }
+        if constexpr (MODE == kMode1D) {
+          // gather BATCH samples (clamped to the unit's last sample) so their
+          // loads are in flight together, then look each up in the TF and
+          // composite in order
+          float v[BATCH];
+#pragma unroll
+          for (int j = 0; j < BATCH; ++j) {
+            const int kk = min(k + j, kend - 1);
+            const float lenj = (kk == n - 1) ? lastlen : step;
+            const float t = fmaf((float)kk, step, t0) + 0.5f * lenj;
+            v[j] = fetch<T, LINEAR>(base, G, fmaf(t, od[0], g0[0]), fmaf(t, od[1], g0[1]),
+                                    fmaf(t, od[2], g0[2]));
+          }
+#pragma unroll
+          for (int j = 0; j < BATCH; ++j) {
+            if (k + j < kend && live) {
+              ++n_eval;
+              const float x = clampf(v[j] * P.tf_scale, 0.f, tfmax);
+              const int ti = (int)x;
+              const float fr = x - (float)ti;
+              const float4 e0 = tf_entry<TF_LDS>(tf4, ti);
+              const float4 e1 = tf_entry<TF_LDS>(tf4, min(ti + 1, P.n_tf - 1));
+              const float a = fmaf(e1.w - e0.w, fr, e0.w);
+              if (a > 0.f) {
+                const float lenj = (k + j == n - 1) ? lastlen : step;
+                const float ac = a >= 1.f ? 1.f : 1.f - exp2f(lenj * log2f(1.f - a));
+                const float w = (1.f - A) * ac;
+                C0 = fmaf(w, fmaf(e1.x - e0.x, fr, e0.x), C0);
+                C1 = fmaf(w, fmaf(e1.y - e0.y, fr, e0.y), C1);
+                C2 = fmaf(w, fmaf(e1.z - e0.z, fr, e0.z), C2);
+                A += w;
+                live = A < P.term_alpha;
+              }
+            }
+          }
+          k = live ? min(k + BATCH, kend) : n;
+          continue;
+        }
+        float v[BATCH], len[BATCH], g[BATCH][3];
+#pragma unroll
+        for (int j = 0; j < BATCH; ++j) {
+          const int kk = min(k + j, kend - 1);
+          len[j] = (kk == n - 1) ? lastlen : step;
+          const float t = fmaf((float)kk, step, t0) + 0.5f * len[j];
+          const float gx = fmaf(t, od[0], g0[0]), gy = fmaf(t, od[1], g0[1]), gz = fmaf(t, od[2], g0[2]);
+          float dd[3];
+          if (LINEAR) {
+            fetch_value_grad<T>(base, G, gx, gy, gz, v[j], dd);
+          } else {
+            v[j] = fetch<T, false>(base, G, gx, gy, gz);
+            const float lx = gx - 0.5f, ly = gy - 0.5f, lz = gz - 0.5f;
+            dd[0] = fetch<T, true>(base, G, lx + 1.f, ly, lz) - fetch<T, true>(base, G, lx - 1.f, ly, lz);
+            dd[1] = fetch<T, true>(base, G, lx, ly + 1.f, lz) - fetch<T, true>(base, G, lx, ly - 1.f, lz);
+            dd[2] = fetch<T, true>(base, G, lx, ly, lz + 1.f) - fetch<T, true>(base, G, lx, ly, lz - 1.f);
+          }
+          g[j][0] = dd[0] * P.grad_scale[0];
+          g[j][1] = dd[1] * P.grad_scale[1];
+          g[j][2] = dd[2] * P.grad_scale[2];
+        }
+#pragma unroll
+        for (int j = 0; j < BATCH; ++j) {
+          if (k + j < kend && live) {
+            ++n_eval;
+            float cs[4];
+            if (P.mode == 1) {
+              const float gm = sqrtf(g[j][0] * g[j][0] + g[j][1] * g[j][1] + g[j][2] * g[j][2]);
+              tf2d_lookup(P, v[j], gm, cs);
+            } else {
+              tf1d_lookup<TF_LDS>(P, tf4, v[j], cs);
+            }
+            if (cs[3] > 0.f) {
+              if (P.lighting) shade(P, g[j], Lh, cs);
+              const float ac = cs[3] >= 1.f ? 1.f : 1.f - exp2f(len[j] * log2f(1.f - cs[3]));
+              const float w = (1.f - A) * ac;
+              C0 = fmaf(w, cs[0], C0);
+              C1 = fmaf(w, cs[1], C1);
+              C2 = fmaf(w, cs[2], C2);
+              A += w;
+              live = A < P.term_alpha;
+            }
+          }
+        }
+        k = live ? min(k + BATCH, kend) : n;
+      }
+      k = n;  // the generic loop below has nothing left
+    }
     while (k < n) {
       const float len0 = (k == n - 1) ? lastlen : step;
       const float tk = fmaf((float)k, step, t0) + 0.5f * len0;
@@ -674,62 +815,6 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
       for (int i = 0; i < 3; ++i)
         g0[i] = oo[i] - (float)(b[i] * P.brick - P.apron) - (LINEAR ? 0.5f : 0.f);
       bool live = true;
-      if (MODE == kModeGrad) {
-        // 1D TF with lighting, or 2D TF (value x |gradient|) with or
-        // without lighting: value and central-difference gradient per
-        // sample (fetch_value_grad).
-        for (; k < kend && live; k += BATCH) {
-          float v[BATCH], len[BATCH], g[BATCH][3];
-#pragma unroll
-          for (int j = 0; j < BATCH; ++j) {
-            const int kk = min(k + j, kend - 1);
-            len[j] = (kk == n - 1) ? lastlen : step;
-            const float t = fmaf((float)kk, step, t0) + 0.5f * len[j];
-            const float gx = fmaf(t, od[0], g0[0]), gy = fmaf(t, od[1], g0[1]),
-                        gz = fmaf(t, od[2], g0[2]);
-            float d[3];
-            if (LINEAR) {
-              fetch_value_grad<T>(base, G, gx, gy, gz, v[j], d);
-            } else {
-              // nearest value; the gradient of the trilinear interpolant at
-              // the same point (texel coordinates shifted by half a voxel)
-              v[j] = fetch<T, false>(base, G, gx, gy, gz);
-              const float lx = gx - 0.5f, ly = gy - 0.5f, lz = gz - 0.5f;
-              d[0] = fetch<T, true>(base, G, lx + 1.f, ly, lz) - fetch<T, true>(base, G, lx - 1.f, ly, lz);
-              d[1] = fetch<T, true>(base, G, lx, ly + 1.f, lz) - fetch<T, true>(base, G, lx, ly - 1.f, lz);
-              d[2] = fetch<T, true>(base, G, lx, ly, lz + 1.f) - fetch<T, true>(base, G, lx, ly, lz - 1.f);
-            }
-            g[j][0] = d[0] * P.grad_scale[0];
-            g[j][1] = d[1] * P.grad_scale[1];
-            g[j][2] = d[2] * P.grad_scale[2];
-          }
-#pragma unroll
-          for (int j = 0; j < BATCH; ++j) {
-            if (k + j < kend && live) {
-              ++n_eval;
-              float c[4];
-              if (P.mode == 1) {
-                const float gm = sqrtf(g[j][0] * g[j][0] + g[j][1] * g[j][1] + g[j][2] * g[j][2]);
-                tf2d_lookup(P, v[j], gm, c);
-              } else {
-                tf1d_lookup<TF_LDS>(P, tf4, v[j], c);
-              }
-              if (c[3] > 0.f) {
-                if (P.lighting) shade(P, g[j], Lh, c);
-                const float ac = c[3] >= 1.f ? 1.f : 1.f - exp2f(len[j] * log2f(1.f - c[3]));
-                const float w = (1.f - A) * ac;
-                C0 = fmaf(w, c[0], C0);
-                C1 = fmaf(w, c[1], C1);
-                C2 = fmaf(w, c[2], C2);
-                A += w;
-                live = A < P.term_alpha;
-              }
-            }
-          }
-        }
-        k = live ? kend : n;
-        continue;
-      }
       if (MODE == kModeIso) {
         // first sample at or above the iso value; refine against the
         // previous sample, shade at the refined point

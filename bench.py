@@ -195,7 +195,7 @@ def run_serving(duration_s: float, rate_hz: float, seed: int = 5) -> dict:
     return out
 
 
-def run_batched(batches: int = 3, seed: int = 3, tune: bool = True) -> dict:
+def run_batched(batches: int = 3, seed: int = 3, tune: bool = True, groups: bool = True) -> dict:
     """The out-of-core batched workload: a 2048x2048x1024 uint16 volume
     (device-generated fBm + layered slabs, copied to host memory) streamed
     into a 64 GB device brick pool (128^3 bricks + 2-voxel apron, 5 LODs);
@@ -203,7 +203,9 @@ def run_batched(batches: int = 3, seed: int = 3, tune: bool = True) -> dict:
     magnitude) with Phong shading; 8 orbit cameras x 4 TFs = 32 views at
     1920x1080 per persistent-kernel launch; brick size (32^3 ... 256^3,
     the paged volume set up again from the host copy for each), empty-space
-    cell and launch shape autotuned on the 32 views."""
+    cell and launch shape autotuned on the 32 views.  With `groups` the 4
+    views of each camera walk their rays once (TF groups; the images equal
+    the ungrouped launch's bit for bit)."""
     from imagevis3d_amd import tuvok, workloads
     from imagevis3d_amd.renderer import VOL_FBM_PLANES
 
@@ -219,6 +221,7 @@ def run_batched(batches: int = 3, seed: int = 3, tune: bool = True) -> dict:
     gen_s = time.perf_counter() - t0
     t0 = time.perf_counter()
     r = workloads.batched_renderer(vol, max_grad, 64 * 10**9)
+    r.setTFGrouping(groups)
     plan_s = time.perf_counter() - t0
     views = workloads.batched_views(dims)
     t0 = time.perf_counter()
@@ -230,24 +233,28 @@ def run_batched(batches: int = 3, seed: int = 3, tune: bool = True) -> dict:
     # timed only (flags 0 times the current settings)
     t = r.autotune(views, flags=(r.TUNE_BRICK | r.TUNE_CELL | r.TUNE_LAUNCH) if tune else 0)
     tune_s = time.perf_counter() - t0
-    kms, nominal, evaluated = [], 0, 0
+    kms, nominal, evaluated, gathered = [], 0, 0, 0
     t0 = time.perf_counter()
     for _ in range(batches):
         _, st, k = r.renderBatch(views, readback=False)
         kms.append(k)
         nominal += sum(x.nominal_samples for x in st)
         evaluated += sum(x.evaluated_samples for x in st)
+        gathered += sum(x.gathered_samples for x in st)
     wall = time.perf_counter() - t0
     ms = r.memStats()
     ksum = sum(kms) / 1e3
-    # gather bytes: value + 6 gradient neighbours, 8 corners x 2 B each
-    achieved = (evaluated * 7 * 16 + batches * len(views) * 1920 * 1080 * 4) / ksum / 1e9
+    # gather bytes: value + 6 gradient neighbours, 8 corners x 2 B each, per
+    # sample whose voxels a march gathered (a TF group gathers each sample
+    # of its walk once for its 4 views), + 4 B per output pixel
+    achieved = (gathered * 7 * 16 + batches * len(views) * 1920 * 1080 * 4) / ksum / 1e9
     out = {
         "workload": "batched-2048x2048x1024-u16-paged64GB-2dtf-phong-8cams-x-4tfs-1080p",
-        "views_per_launch": len(views), "launches": batches,
+        "views_per_launch": len(views), "launches": batches, "tf_groups": groups,
         "gsamples_per_s": nominal / wall / 1e9, "views_per_s": batches * len(views) / wall,
         "ms_per_launch": wall / batches * 1e3, "kernel_ms_per_launch": ksum / batches * 1e3,
         "evaluated_gsamples_per_s": evaluated / wall / 1e9,
+        "gathered_gsamples_per_s": gathered / wall / 1e9,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS},
         "tuned": {"brick": t.brick_size, "cell": t.cell, "samples_per_batch": t.samples_per_batch,
@@ -689,6 +696,8 @@ def main() -> int:
     ap.add_argument("--learned-lod-size", type=int, default=4096)
     ap.add_argument("--staging", type=int, default=0,
                     help="with --no-autotune: 1 marches the 1D TF over LDS-staged cells")
+    ap.add_argument("--no-tf-groups", action="store_true",
+                    help="batched workload: render each of a camera's TF views on its own")
     ap.add_argument("--mlp-only", action="store_true",
                     help="time the learned-LOD MLP alone (random weights, no torch)")
     args = ap.parse_args()
@@ -704,7 +713,7 @@ def main() -> int:
         print(json.dumps(run_serving(args.serving_seconds, args.serving_rate)))
         return 0
     if args.batched_only:
-        print(json.dumps(run_batched(tune=not args.no_autotune)))
+        print(json.dumps(run_batched(tune=not args.no_autotune, groups=not args.no_tf_groups)))
         return 0
     if args.learned_lod_only:
         print(json.dumps(run_learned_lod(args.learned_lod_size)))
@@ -829,7 +838,7 @@ def main() -> int:
         print(f"[bench] learned LOD {ll}", file=sys.stderr, flush=True)
     if not args.no_batched:
         print("[bench] out-of-core batched workload", file=sys.stderr, flush=True)
-        out["batched"] = run_batched()
+        out["batched"] = run_batched(groups=not args.no_tf_groups)
         print(f"[bench] batched {out['batched']}", file=sys.stderr, flush=True)
     if not args.no_serving:
         print(f"[bench] serving workload ({args.serving_seconds:g} s at {args.serving_rate:g} req/s)",

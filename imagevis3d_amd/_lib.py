@@ -74,6 +74,7 @@ class ViewStats(C.Structure):
     _fields_ = [
         ("evaluated_samples", C.c_uint64), ("nominal_samples", C.c_uint64),
         ("lod", C.c_uint32), ("reserved", C.c_uint32),
+        ("gathered_samples", C.c_uint64),
     ]
 
 
@@ -201,6 +202,7 @@ SIGNATURES = {
     "tvk_set_kernel_tuning": [H, u32, u32, u32],
     "tvk_set_work_group_waves": [H, u32],
     "tvk_set_brick_staging": [H, i32],
+    "tvk_set_tf_grouping": [H, i32],
     "tvk_set_ray_tile_shape": [H, u32],
     "tvk_set_empty_space_cell": [H, u32],
     "tvk_set_1d_trans_slot": [H, u32, P, u32],

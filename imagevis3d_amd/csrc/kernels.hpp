@@ -112,6 +112,10 @@ struct LaunchInfo {
   int compute_units = 0;
   int wg_waves = 4;      // waves per work-group of variants without an LDS TF (4 or 8)
   int stage = 0;        // 1D TF without lighting, trilinear: voxels of each cell staged in LDS
+  // Gradient launches (2D TF, or 1D TF with lighting): 1 lets the table
+  // group views that differ only in their TF into one march (launch_info);
+  // the launch keeps it at 1 only when the filled table holds a group.
+  int group = 0;
 };
 
 // One raycast launch over a device job table `tab` (its header counters

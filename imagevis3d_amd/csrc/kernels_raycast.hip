@@ -218,7 +218,10 @@ __device__ inline void fetch_value_grad(const T* __restrict__ base, const SlotGe
 // may hold the maximum) and rays do not terminate early.
 // kMode1DStaged: the 1D march with the voxels of each 8^3 slot cell staged
 // in LDS (render_tile_staged).
-constexpr int kMode1D = 0, kModeGrad = 1, kModeIso = 2, kModeRegions = 3, kModeMIP = 4, kMode1DStaged = 5;
+// kModeGradGroup: the gradient march of a TF group -- one walk per ray
+// compositing up to kGroupMax TFs (render_tile_group).
+constexpr int kMode1D = 0, kModeGrad = 1, kModeIso = 2, kModeRegions = 3, kModeMIP = 4, kMode1DStaged = 5,
+              kModeGradGroup = 6;
 
 // Device address of brick gid's slot from the page table (null: the brick
 // is not resident).  Slots may live in any allocation of the device pool.
@@ -512,26 +515,39 @@ __device__ inline int centre_out_row(int k, int h) {
   return above > below ? c + (k - m) : c - (k - m);
 }
 
-// Ray tile `tile` of job P: row-major over the job's tile rect, rows taken
-// centre-out.
-template <typename T, bool LINEAR, int BATCH, int MODE, bool TF_LDS>
-__device__ inline void render_tile(const RaycastParams& P, const float4* __restrict__ tf4, int tile,
-                                   int lane, uint32_t& n_eval, uint32_t& n_nom) {
+// A lane's pixel and ray: origin o, unit direction d (LOD-0 voxel space),
+// 1/d per axis (inf on axis-parallel rays) and the interval [t0, t1] inside
+// the volume box after the clip plane; hit false when the ray misses it.
+struct Ray {
+  int px, py;
+  float o[3], d[3], inv[3];
+  float t0, t1;
+  bool hit;
+};
+
+// The ray of `lane` in ray tile `tile` of job P (row-major over the job's
+// tile rect, rows taken centre-out).  Returns false for a lane with no
+// pixel (beyond the image, or a tile that tile sampling leaves out); adds
+// the nominal samples of the ray's chord through the box to n_nom.
+__device__ inline bool setup_ray(const RaycastParams& P, int tile, int lane, Ray& r, uint32_t& n_nom) {
   if (P.tile_sample > 1) {
     // one tile of each run of tile_sample, at an offset hashed from the
     // run and the phase (no stripes when the rect width shares a factor
     // with tile_sample)
     const uint32_t h = ((uint32_t)tile * 0x9E3779B1u + (uint32_t)P.tile_phase * 0x85EBCA77u) >> 16;
     tile = tile * P.tile_sample + (int)(h % (uint32_t)P.tile_sample);
-    if (tile >= P.rect_w * P.rect_h) return;
+    if (tile >= P.rect_w * P.rect_h) return false;
   }
   const int tx = P.rect_x0 + tile % P.rect_w, ty = P.rect_y0 + centre_out_row(tile / P.rect_w, P.rect_h);
   const int sh = P.tile_shift;
   const int px = (tx << sh) + (lane & ((1 << sh) - 1));
   const int py = ty * (64 >> sh) + (lane >> sh);
-  if (px >= P.width || py >= P.height) return;
-
-  float o[3], d[3];
+  if (px >= P.width || py >= P.height) return false;
+  r.px = px;
+  r.py = py;
+  float(&o)[3] = r.o;
+  float(&d)[3] = r.d;
+  float(&inv)[3] = r.inv;
   const float fx = (float)px, fy = (float)py;
   if (P.ortho) {
 #pragma unroll
@@ -550,7 +566,7 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
     for (int i = 0; i < 3; ++i) d[i] = d[i] / l;
   }
   // ray vs LOD-0 box [0, box]; axis-parallel rays use an inside test
-  float t0 = -kInf, t1 = kInf, inv[3];
+  float t0 = -kInf, t1 = kInf;
   bool hit = true;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -573,6 +589,23 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
   // nominal samples count the chord through the box, before clipping
   if (hit && t1 > t0) n_nom += (uint32_t)floorf((t1 - t0) * P.nominal_rate);
   if (P.clip_on && hit) clip_ray(P, o, d, t0, t1, hit);
+  r.t0 = t0;
+  r.t1 = t1;
+  r.hit = hit;
+  return true;
+}
+
+template <typename T, bool LINEAR, int BATCH, int MODE, bool TF_LDS>
+__device__ inline void render_tile(const RaycastParams& P, const float4* __restrict__ tf4, int tile,
+                                   int lane, uint32_t& n_eval, uint32_t& n_nom) {
+  Ray r;
+  if (!setup_ray(P, tile, lane, r, n_nom)) return;
+  const int px = r.px, py = r.py;
+  const float(&o)[3] = r.o;
+  const float(&d)[3] = r.d;
+  const float(&inv)[3] = r.inv;
+  const float t0 = r.t0, t1 = r.t1;
+  const bool hit = r.hit;
   const float L = t1 - t0;
   float C0 = 0.f, C1 = 0.f, C2 = 0.f, A = 0.f;
   float vmax = -1.f;  // MIP: largest sample value so far (-1: none)
@@ -936,6 +969,179 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
     A = c[3];
   }
   gstore(P.framebuffer + ((int64_t)py * P.width + px), over_background(P, py, C0, C1, C2, A));
+}
+
+// Gradient march of a TF group (kModeGradGroup): jobs that differ only in
+// their TF (same camera, image, volume, level, rate and shading) walk each
+// ray once.  Every sample's value and gradient are gathered once and each
+// member's TF is looked up, shaded and composited into its own (C, A);
+// member f evaluates exactly the samples its own march would -- those of
+// the march units its TF does not skip, up to its own early termination --
+// in the same order and with the same arithmetic, so every member's image
+// and sample count equal its own render's bit for bit.  The walk skips a
+// unit only when it is empty for every member still compositing, and ends
+// when all have terminated.  A job alone (group_n <= 1) takes the same
+// path with one TF.  n_eval[f] counts member f's samples, n_walk the
+// samples of the walk that some member evaluates (the union of theirs).
+template <typename T, bool LINEAR, int BATCH>
+__device__ inline void render_tile_group(const JobTable* __restrict__ tab, const RaycastParams& P, int tile,
+                                         int lane, uint32_t (&n_eval)[kGroupMax], uint32_t& n_walk,
+                                         uint32_t& n_nom) {
+  Ray r;
+  if (!setup_ray(P, tile, lane, r, n_nom)) return;
+  const int F = P.group_n > 1 ? P.group_n : 1;
+  const RaycastParams* Q[kGroupMax];
+  Q[0] = &P;
+#pragma unroll
+  for (int f = 1; f < kGroupMax; ++f) Q[f] = f < F ? &tab->jobs[P.group_jobs[f - 1]] : &P;
+  float C[kGroupMax][3], A[kGroupMax];
+#pragma unroll
+  for (int f = 0; f < kGroupMax; ++f) C[f][0] = C[f][1] = C[f][2] = A[f] = 0.f;
+  const float t0 = r.t0, L = r.t1 - r.t0;
+  if (r.hit && L > 0.f) {
+    const float step = P.step;
+    const int n = (int)ceilf(L / step);
+    const float lastlen = fmaxf(L - (float)(n - 1) * step, 0.f);
+    const LodInfo& lv = P.level;
+    const int U = P.cell;
+    const int per_brick = P.brick / U;
+    const float invU = 1.f / (float)U;
+    float od[3], oo[3], ul0[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      od[i] = r.d[i] * lv.to_level[i];
+      oo[i] = r.o[i] * lv.to_level[i];
+      ul0[i] = (float)U / lv.to_level[i];
+    }
+    const int ucount[3] = {(lv.dims[0] + U - 1) / U, (lv.dims[1] + U - 1) / U,
+                           (lv.dims[2] + U - 1) / U};
+    const SlotGeo G = slot_geo(P.slot_edge);
+    float Lh[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // light direction, half vector
+    {
+      float h[3], hl = 0.f;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        Lh[i] = P.light[i];
+        h[i] = P.light[i] - r.d[i];
+        hl += h[i] * h[i];
+      }
+      hl = sqrtf(hl);
+      if (hl > 1e-12f)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) Lh[3 + i] = h[i] / hl;
+    }
+    uint32_t live = (1u << F) - 1;  // members still compositing
+    uint32_t need = 0;              // live members whose TF keeps the current unit
+    int k = 0, kend = 0, blend_gid = -1;
+    float g0[3] = {0.f, 0.f, 0.f};
+    const T* __restrict__ base = nullptr;
+    while (k < n) {
+      if (k >= kend) {
+        // the march unit of sample k, as in render_tile's walk
+        const float len0 = (k == n - 1) ? lastlen : step;
+        const float tk = fmaf((float)k, step, t0) + 0.5f * len0;
+        int c[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+          c[i] = clampi((int)floorf(fmaf(tk, od[i], oo[i]) * invU), 0, ucount[i] - 1);
+        float tb = kInf;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          if (r.d[i] != 0.f) {
+            const float face = (float)(r.d[i] > 0.f ? c[i] + 1 : c[i]) * ul0[i];
+            tb = fminf(tb, (face - r.o[i]) * r.inv[i]);
+          }
+        }
+        kend = min(max((int)ceilf((tb - t0) / step - 0.5f), k + 1), n);
+        int b[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) b[i] = c[i] / per_brick;
+        const int gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
+        const int64_t cidx = lv.cell_base + (c[2] * ucount[1] + c[1]) * ucount[0] + c[0];
+        need = 0;
+#pragma unroll
+        for (int f = 0; f < kGroupMax; ++f)
+          if (f < F && (live >> f & 1u) && !(P.skip && gload(Q[f]->cell_empty + cidx))) need |= 1u << f;
+        base = brick_base<T>(P, gid);
+        if (P.blend_bits < 32 && gid != blend_gid) {
+          // a member's own march stops rounding once it has terminated
+#pragma unroll
+          for (int f = 0; f < kGroupMax; ++f)
+            if (live >> f & 1u) blend_round(P.blend_bits, C[f][0], C[f][1], C[f][2], A[f]);
+          blend_gid = gid;
+        }
+        if (!need) {
+          k = kend;
+          continue;
+        }
+        if (!base) {
+          if (P.requests) P.requests[gid] = 1u;
+          k = kend;
+          continue;
+        }
+        if (P.touch) P.touch[gid] = P.frame;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+          g0[i] = oo[i] - (float)(b[i] * P.brick - P.apron) - (LINEAR ? 0.5f : 0.f);
+      }
+      float v[BATCH], len[BATCH], g[BATCH][3];
+#pragma unroll
+      for (int j = 0; j < BATCH; ++j) {
+        const int kk = min(k + j, kend - 1);
+        len[j] = (kk == n - 1) ? lastlen : step;
+        const float t = fmaf((float)kk, step, t0) + 0.5f * len[j];
+        const float gx = fmaf(t, od[0], g0[0]), gy = fmaf(t, od[1], g0[1]), gz = fmaf(t, od[2], g0[2]);
+        float dd[3];
+        if (LINEAR) {
+          fetch_value_grad<T>(base, G, gx, gy, gz, v[j], dd);
+        } else {
+          v[j] = fetch<T, false>(base, G, gx, gy, gz);
+          const float lx = gx - 0.5f, ly = gy - 0.5f, lz = gz - 0.5f;
+          dd[0] = fetch<T, true>(base, G, lx + 1.f, ly, lz) - fetch<T, true>(base, G, lx - 1.f, ly, lz);
+          dd[1] = fetch<T, true>(base, G, lx, ly + 1.f, lz) - fetch<T, true>(base, G, lx, ly - 1.f, lz);
+          dd[2] = fetch<T, true>(base, G, lx, ly, lz + 1.f) - fetch<T, true>(base, G, lx, ly, lz - 1.f);
+        }
+        g[j][0] = dd[0] * P.grad_scale[0];
+        g[j][1] = dd[1] * P.grad_scale[1];
+        g[j][2] = dd[2] * P.grad_scale[2];
+      }
+#pragma unroll
+      for (int j = 0; j < BATCH; ++j) {
+        if (k + j >= kend || !need) continue;
+        ++n_walk;  // a sample some member evaluates
+        const float gm = sqrtf(g[j][0] * g[j][0] + g[j][1] * g[j][1] + g[j][2] * g[j][2]);
+#pragma unroll
+        for (int f = 0; f < kGroupMax; ++f) {
+          if (!(need >> f & 1u)) continue;
+          ++n_eval[f];
+          float cs[4];
+          if (P.mode == 1) tf2d_lookup(*Q[f], v[j], gm, cs);
+          else tf1d_lookup<false>(*Q[f], (const float4*)Q[f]->tf, v[j], cs);
+          if (cs[3] > 0.f) {
+            if (P.lighting) shade(P, g[j], Lh, cs);
+            const float ac = cs[3] >= 1.f ? 1.f : 1.f - exp2f(len[j] * log2f(1.f - cs[3]));
+            const float w = (1.f - A[f]) * ac;
+            C[f][0] = fmaf(w, cs[0], C[f][0]);
+            C[f][1] = fmaf(w, cs[1], C[f][1]);
+            C[f][2] = fmaf(w, cs[2], C[f][2]);
+            A[f] += w;
+            if (!(A[f] < P.term_alpha)) {
+              live &= ~(1u << f);
+              need &= ~(1u << f);
+            }
+          }
+        }
+      }
+      k = need ? min(k + BATCH, kend) : (live ? kend : n);
+    }
+    if (P.blend_bits < 32)
+#pragma unroll
+      for (int f = 0; f < kGroupMax; ++f) blend_round(P.blend_bits, C[f][0], C[f][1], C[f][2], A[f]);
+  }
+  const int64_t pix = (int64_t)r.py * P.width + r.px;
+#pragma unroll
+  for (int f = 0; f < kGroupMax; ++f)
+    if (f < F) gstore(Q[f]->framebuffer + pix, over_background(P, r.py, C[f][0], C[f][1], C[f][2], A[f]));
 }
 
 // LDS-staged 1D march (kMode1DStaged: 1D TF without lighting, trilinear).
@@ -1309,14 +1515,14 @@ template <typename T, bool LINEAR, int BATCH, bool TF_LDS, int MODE>
 __global__ __launch_bounds__(kThreads, 2) void raycast_kernel(const JobTable* __restrict__ tab) {
   const int kT = (int)blockDim.x;  // 256 or 512 threads (see launch_one)
   extern __shared__ float4 s_tf[];  // n_tf_lds entries when TF_LDS
-  __shared__ unsigned long long s_count[2 * kMaxJobs];  // per job: evaluated, nominal
+  __shared__ unsigned long long s_count[kJobCounters * kMaxJobs];  // per job: evaluated, nominal, gathered
   const JobTableHeader& H = tab->h;
   const float4* tf_shared = (const float4*)H.tf_lds;
   if (TF_LDS) {
     for (int i = (int)threadIdx.x; i < H.n_tf_lds; i += kT) s_tf[i] = tf_shared[i];
     tf_shared = s_tf;
   }
-  for (int i = (int)threadIdx.x; i < 2 * kMaxJobs; i += kT) s_count[i] = 0;
+  for (int i = (int)threadIdx.x; i < kJobCounters * kMaxJobs; i += kT) s_count[i] = 0;
   __syncthreads();
   const int lane = (int)(threadIdx.x & 63);
   const int xcd = (int)(blockIdx.x & 7);
@@ -1332,6 +1538,31 @@ __global__ __launch_bounds__(kThreads, 2) void raycast_kernel(const JobTable* __
   // end, then the work-group adds its counters to the job's global ones
   int cur_job = -1;
   uint32_t n_eval = 0, n_nom = 0;
+  uint32_t n_evalg[kGroupMax] = {0, 0, 0, 0};  // kModeGradGroup: per member of the job's group
+  uint32_t n_walk = 0;                          // kModeGradGroup: samples the group's walk gathered
+  // adds the wave's counts of cur_job (and of its group's members) to the
+  // work-group's counters
+  auto flush = [&]() {
+    if constexpr (MODE == kModeGradGroup) {
+      const RaycastParams& Pc = tab->jobs[cur_job];
+      const int F = Pc.group_n > 1 ? Pc.group_n : 1;
+#pragma unroll
+      for (int f = 0; f < kGroupMax; ++f) {
+        if (f < F) {
+          const int j = f == 0 ? cur_job : Pc.group_jobs[f > 0 ? f - 1 : 0];
+          atomicAdd(&s_count[kJobCounters * j], (unsigned long long)n_evalg[f]);
+          atomicAdd(&s_count[kJobCounters * j + 1], (unsigned long long)n_nom);
+        }
+        n_evalg[f] = 0;
+      }
+      atomicAdd(&s_count[kJobCounters * cur_job + 2], (unsigned long long)n_walk);
+      n_walk = 0;
+    } else {
+      atomicAdd(&s_count[kJobCounters * cur_job], (unsigned long long)n_eval);
+      atomicAdd(&s_count[kJobCounters * cur_job + 1], (unsigned long long)n_nom);
+      atomicAdd(&s_count[kJobCounters * cur_job + 2], (unsigned long long)n_eval);
+    }
+  };
   for (int q = 0; q < kQueues; ++q) {
     const int x = (xcd + q) & 7;  // own queue first, then steal
     // Queue x enumerates either runs of G tiles dealt round-robin
@@ -1374,10 +1605,7 @@ __global__ __launch_bounds__(kThreads, 2) void raycast_kernel(const JobTable* __
       const RaycastParams& P = tab->jobs[job];
       const float4* tf4 = TF_LDS ? tf_shared : (const float4*)P.tf;
       if (job != cur_job) {  // wave-uniform
-        if (cur_job >= 0) {
-          atomicAdd(&s_count[2 * cur_job], (unsigned long long)n_eval);
-          atomicAdd(&s_count[2 * cur_job + 1], (unsigned long long)n_nom);
-        }
+        if (cur_job >= 0) flush();
         cur_job = job;
         n_eval = n_nom = 0;
       }
@@ -1385,17 +1613,16 @@ __global__ __launch_bounds__(kThreads, 2) void raycast_kernel(const JobTable* __
         // this wave's staging buffer follows the LDS TF
         T* sbuf = (T*)(s_tf + (TF_LDS ? H.n_tf_lds : 0)) + (threadIdx.x >> 6) * kStageElems;
         render_tile_staged<T, TF_LDS>(P, tf4, tile - tab->tile_start[job], lane, n_eval, n_nom, sbuf);
+      } else if constexpr (MODE == kModeGradGroup) {
+        render_tile_group<T, LINEAR, BATCH>(tab, P, tile - tab->tile_start[job], lane, n_evalg, n_walk, n_nom);
       } else {
         render_tile<T, LINEAR, BATCH, MODE, TF_LDS>(P, tf4, tile - tab->tile_start[job], lane, n_eval, n_nom);
       }
     }
   }
-  if (cur_job >= 0) {
-    atomicAdd(&s_count[2 * cur_job], (unsigned long long)n_eval);
-    atomicAdd(&s_count[2 * cur_job + 1], (unsigned long long)n_nom);
-  }
+  if (cur_job >= 0) flush();
   __syncthreads();
-  for (int i = (int)threadIdx.x; i < 2 * n_jobs; i += kT)
+  for (int i = (int)threadIdx.x; i < kJobCounters * n_jobs; i += kT)
     if (s_count[i] != 0) atomicAdd(&H.counters[kQueues + i], s_count[i]);
 }
 
@@ -1430,11 +1657,14 @@ hipError_t launch_typed(const LaunchInfo& li, const JobTable* tab, hipStream_t s
   if (li.mode == 3)
     return li.n_tf_lds > 0 ? launch_one<T, LINEAR, 4, true, kModeMIP>(li, tab, stream)
                            : launch_one<T, LINEAR, 4, false, kModeMIP>(li, tab, stream);
-  if (li.mode == 1) return launch_one<T, LINEAR, 2, false, kModeGrad>(li, tab, stream);
+  if (li.mode == 1)
+    return li.group ? launch_one<T, LINEAR, 2, false, kModeGradGroup>(li, tab, stream)
+                    : launch_one<T, LINEAR, 2, false, kModeGrad>(li, tab, stream);
   const bool lds = li.n_tf_lds > 0;
   if (li.stage && LINEAR && !li.lighting)
     return lds ? launch_one<T, true, 1, true, kMode1DStaged>(li, tab, stream)
                : launch_one<T, true, 1, false, kMode1DStaged>(li, tab, stream);
+  if (li.lighting && li.group) return launch_one<T, LINEAR, 2, false, kModeGradGroup>(li, tab, stream);
   if (li.lighting)
     return lds ? launch_one<T, LINEAR, 2, true, kModeGrad>(li, tab, stream)
                : launch_one<T, LINEAR, 2, false, kModeGrad>(li, tab, stream);

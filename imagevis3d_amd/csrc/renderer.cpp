@@ -1355,6 +1355,11 @@ int HipRenderer::SetBrickStaging(bool on) {
   return TVK_OK;
 }
 
+int HipRenderer::SetTFGrouping(bool on) {
+  group_tfs_ = on;
+  return TVK_OK;
+}
+
 int HipRenderer::SetRayTileShape(uint32_t tile_width) {
   if (tile_width != 4 && tile_width != 8 && tile_width != 16) return arg_fail("tile width must be 4, 8 or 16");
   tile_shift_ = tile_width == 4 ? 2 : (tile_width == 8 ? 3 : 4);
@@ -1589,37 +1594,86 @@ LaunchInfo HipRenderer::launch_info(int n_tf_lds) const {
   li.compute_units = compute_units_;
   li.wg_waves = wg_waves_;
   li.stage = (stage_ && li.mode == 0 && !li.lighting && !li.regions && li.interp == 1) ? 1 : 0;
+  li.group = (group_tfs_ && (li.mode == 1 || li.lighting)) ? 1 : 0;
   return li;
 }
 
 // Tile slots of a job in the queues: its rect's tiles, or one per run of
 // tile_sample of them.
 static int job_tiles(const RaycastParams& p) {
+  if (p.group_n < 0) return 0;  // a TF group member: its leader's rays render it
   const int all = p.rect_w * p.rect_h;
   return p.tile_sample <= 1 ? all : (all + p.tile_sample - 1) / p.tile_sample;
+}
+
+// Can jobs a and b share one march?  Every field equal except those of the
+// TF, the output framebuffer and the group fields.
+static bool same_but_tf(const RaycastParams& a, const RaycastParams& b) {
+  RaycastParams x, y;
+  memcpy(&x, &a, sizeof x);
+  memcpy(&y, &b, sizeof y);
+  for (RaycastParams* p : {&x, &y}) {
+    p->tf2d = nullptr;
+    p->tf2d_w = p->tf2d_h = 0;
+    p->tf2d_row_scale = p->tf_scale = 0.f;
+    p->n_tf = 0;
+    p->tf = nullptr;
+    p->tf_prefix = nullptr;
+    p->cell_empty = nullptr;
+    p->framebuffer = nullptr;
+    p->group_n = 0;
+    memset(p->group_jobs, 0, sizeof p->group_jobs);
+  }
+  return memcmp(&x, &y, sizeof x) == 0;
 }
 
 // Fill the pinned table `t` from jobs[0..n) for a device table at
 // dev_table whose counters live at dev_counters; returns the tiles of the
 // launch and whether some job has background pixels outside its tile rect.
+// With li.group, jobs that differ only in their TF are grouped (up to
+// kGroupMax, the first of them leading); *groups = the groups formed.
 static int fill_job_table(JobTable* t, const RaycastParams* jobs, int n, const LaunchInfo& li,
-                          int tile_group, int xcd_split, unsigned long long* dev_counters, bool* culled) {
+                          int tile_group, int xcd_split, unsigned long long* dev_counters, bool* culled,
+                          int* groups) {
   t->h.n_jobs = n;
   t->h.tile_group = tile_group;
   t->h.xcd_split = xcd_split;
   t->h.n_tf_lds = li.n_tf_lds;
   t->h.tf_lds = li.n_tf_lds > 0 ? jobs[0].tf : nullptr;
   t->h.counters = dev_counters;
+  *groups = 0;
+  for (int k = 0; k < n; ++k) {
+    t->jobs[k] = jobs[k];
+    t->jobs[k].group_n = 0;
+    memset(t->jobs[k].group_jobs, 0, sizeof t->jobs[k].group_jobs);
+  }
+  if (li.group) {
+    for (int k = 0; k < n; ++k) {
+      RaycastParams& lead = t->jobs[k];
+      if (lead.group_n != 0) continue;
+      int cnt = 1;
+      for (int m = k + 1; m < n && cnt < kGroupMax; ++m) {
+        if (t->jobs[m].group_n != 0 || !same_but_tf(lead, t->jobs[m])) continue;
+        lead.group_jobs[cnt - 1] = m;
+        t->jobs[m].group_n = -1;
+        ++cnt;
+      }
+      if (cnt > 1) {
+        lead.group_n = cnt;
+        ++*groups;
+      }
+    }
+  }
   int total = 0;
   *culled = false;
   for (int k = 0; k < n; ++k) {
+    const RaycastParams& j = t->jobs[k];
     t->tile_start[k] = total;
-    total += job_tiles(jobs[k]);
-    *culled = *culled || jobs[k].rect_w * jobs[k].rect_h != jobs[k].tiles_x * jobs[k].tiles_y;
+    total += job_tiles(j);
+    *culled = *culled || j.rect_w * j.rect_h != j.tiles_x * j.tiles_y;
     // the frame-region split addresses tiles by rect position: sampled
     // jobs take the round-robin runs
-    if (jobs[k].tile_sample > 1) t->h.xcd_split = 0;
-    t->jobs[k] = jobs[k];
+    if (j.tile_sample > 1) t->h.xcd_split = 0;
   }
   t->tile_start[n] = total;
   t->h.total_tiles = total;
@@ -1628,10 +1682,13 @@ static int fill_job_table(JobTable* t, const RaycastParams* jobs, int n, const L
 
 int launch_job_table_at(JobTable* table_h, void* dev_table, unsigned long long* dev_counters,
                         unsigned long long* counters_h, hipEvent_t ev[4], const RaycastParams* jobs, int n,
-                        const LaunchInfo& li, int tile_group, int xcd_split, hipStream_t stream_) {
+                        const LaunchInfo& li_in, int tile_group, int xcd_split, hipStream_t stream_) {
   bool culled = false;  // some job has background tiles outside its rect
-  fill_job_table(table_h, jobs, n, li, tile_group, xcd_split, dev_counters, &culled);
-  const int words = kQueues + 2 * n;
+  int groups = 0;
+  fill_job_table(table_h, jobs, n, li_in, tile_group, xcd_split, dev_counters, &culled, &groups);
+  LaunchInfo li = li_in;
+  li.group = groups > 0 ? 1 : 0;
+  const int words = kQueues + kJobCounters * n;
   const size_t table_bytes = offsetof(JobTable, jobs) + (size_t)n * sizeof(RaycastParams);
   TVK_HIP(hipEventRecord(ev[0], stream_));
   TVK_HIP(hipMemcpyAsync(dev_table, table_h, table_bytes, hipMemcpyHostToDevice, stream_));
@@ -1647,17 +1704,21 @@ int launch_job_table_at(JobTable* table_h, void* dev_table, unsigned long long* 
   return TVK_OK;
 }
 
-int launch_job_table(LaunchSlot& sl, const RaycastParams* jobs, int n, const LaunchInfo& li,
+int launch_job_table(LaunchSlot& sl, const RaycastParams* jobs, int n, const LaunchInfo& li_in,
                      int tile_group, int xcd_split, hipStream_t stream_, bool graph) {
   // the pinned table may still be read by this slot's previous upload
   if (sl.used) TVK_HIP(hipEventSynchronize(sl.ev[3]));
   sl.used = true;
   if (!graph)
     return launch_job_table_at(sl.table_h, sl.table.p, sl.counters.as<unsigned long long>(), sl.counters_h,
-                               sl.ev, jobs, n, li, tile_group, xcd_split, stream_);
+                               sl.ev, jobs, n, li_in, tile_group, xcd_split, stream_);
   JobTable* t = sl.table_h;
   bool culled = false;
-  fill_job_table(t, jobs, n, li, tile_group, xcd_split, sl.counters.as<unsigned long long>(), &culled);
+  int groups = 0;
+  fill_job_table(t, jobs, n, li_in, tile_group, xcd_split, sl.counters.as<unsigned long long>(), &culled,
+                 &groups);
+  LaunchInfo li = li_in;
+  li.group = groups > 0 ? 1 : 0;
   TVK_HIP(hipEventRecord(sl.ev[0], stream_));
   // Graph replay: the graph copies the whole table and all counters, so one
   // capture serves any job count; it is re-captured when the kernel
@@ -1665,7 +1726,8 @@ int launch_job_table(LaunchSlot& sl, const RaycastParams* jobs, int n, const Lau
   const uint64_t key = 1ull | (uint64_t)li.bpv << 1 | (uint64_t)li.interp << 3 | (uint64_t)li.mode << 4 |
                        (uint64_t)li.lighting << 6 | (uint64_t)li.batch << 7 | (uint64_t)li.regions << 11 |
                        (uint64_t)li.n_tf_lds << 12 |
-                       (uint64_t)(li.wg_waves == 4 ? 1 : 0) << 40 | (uint64_t)li.stage << 41;
+                       (uint64_t)(li.wg_waves == 4 ? 1 : 0) << 40 | (uint64_t)li.stage << 41 |
+                       (uint64_t)li.group << 42;
   if (!sl.exec || sl.graph_key != key) {
     if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
     if (sl.graph) (void)hipGraphDestroy(sl.graph);
@@ -1990,8 +2052,9 @@ int HipRenderer::RenderBatch(const tvk_view* views, uint32_t n, tvk_view_stats* 
     for (int k = 0; k < cnt; ++k) {
       const uint32_t i = first + k;
       if (stats) {
-        stats[i].evaluated_samples = batch_slot_.counters_h[kQueues + 2 * k];
-        stats[i].nominal_samples = batch_slot_.counters_h[kQueues + 2 * k + 1];
+        stats[i].evaluated_samples = batch_slot_.counters_h[kQueues + kJobCounters * k];
+        stats[i].nominal_samples = batch_slot_.counters_h[kQueues + kJobCounters * k + 1];
+        stats[i].gathered_samples = batch_slot_.counters_h[kQueues + kJobCounters * k + 2];
         stats[i].lod = (uint32_t)lod;
         stats[i].reserved = 0;
       }
@@ -2070,8 +2133,9 @@ int HipRenderer::RenderBatchNoAlloc(const tvk_view* views, uint32_t n, void* con
   if (stats) {
     TVK_HIP(hipEventSynchronize(batch_slot_.ev[3]));
     for (uint32_t k = 0; k < n; ++k) {
-      stats[k].evaluated_samples = batch_slot_.counters_h[kQueues + 2 * k];
-      stats[k].nominal_samples = batch_slot_.counters_h[kQueues + 2 * k + 1];
+      stats[k].evaluated_samples = batch_slot_.counters_h[kQueues + kJobCounters * k];
+      stats[k].nominal_samples = batch_slot_.counters_h[kQueues + kJobCounters * k + 1];
+      stats[k].gathered_samples = batch_slot_.counters_h[kQueues + kJobCounters * k + 2];
       stats[k].lod = (uint32_t)lo;
       stats[k].reserved = 0;
     }

@@ -244,6 +244,7 @@ class HipRenderer : public PoolClient {
   int SetKernelTuning(uint32_t batch, uint32_t tile_group, uint32_t xcd_split);
   int SetWorkGroupWaves(uint32_t waves);  // 4 or 8 (variants without an LDS TF)
   int SetBrickStaging(bool on);          // 1D march over LDS-staged cells
+  int SetTFGrouping(bool on);            // one march per TF group (gradient launches)
   int SetRayTileShape(uint32_t tile_width);
   // Per-region LOD (1D TF): one level per LOD-0 brick; n == 0 turns it off.
   int SetRegionLods(const uint8_t* levels, uint64_t n);
@@ -494,6 +495,7 @@ class HipRenderer : public PoolClient {
   int xcd_split_ = 0;
   int wg_waves_ = 4;
   bool stage_ = false;  // LDS-staged 1D march
+  bool group_tfs_ = true;  // gradient launches group views that differ only in TF
   int tile_shift_ = 3;  // 8 x 8 ray tiles
   int target_ = TVK_RT_INTERACTIVE;
 

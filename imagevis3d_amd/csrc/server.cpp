@@ -293,8 +293,8 @@ bool RenderServer::retire_locked() {
       r.done = true;
       r.t_done = t;
       if (e == hipSuccess) {
-        r.evaluated = L.cur->counters_h[kQueues + 2 * k];
-        r.nominal = L.cur->counters_h[kQueues + 2 * k + 1];
+        r.evaluated = L.cur->counters_h[kQueues + kJobCounters * k];
+        r.nominal = L.cur->counters_h[kQueues + kJobCounters * k + 1];
         stats_.nominal_samples += r.nominal;
       } else {
         r.status = TVK_ERR_HIP;

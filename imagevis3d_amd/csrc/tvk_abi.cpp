@@ -711,6 +711,11 @@ int tvk_set_brick_staging(tvk_renderer* r, int on) {
   return r->impl.SetBrickStaging(on != 0);
 }
 
+int tvk_set_tf_grouping(tvk_renderer* r, int on) {
+  TVK_GUARD(r);
+  return r->impl.SetTFGrouping(on != 0);
+}
+
 int tvk_set_ray_tile_shape(tvk_renderer* r, uint32_t tile_width) {
   TVK_GUARD(r);
   return r->impl.SetRayTileShape(tile_width);

@@ -170,7 +170,15 @@ struct RaycastParams {
   const uint8_t* region_code;
   const LodInfo* lods;
   int32_t* region_out;
+  // TF groups (gradient launches): jobs that differ only in their TF share
+  // one march.  group_n > 1: this job leads a group of group_n jobs, the
+  // others being jobs group_jobs[0 .. group_n - 2] of the table; the
+  // leader's rays composite every member's TF and write every member's
+  // framebuffer.  group_n < 0: a member (no tiles of its own).  0: alone.
+  int32_t group_n;
+  int32_t group_jobs[3];
 };
+constexpr int kGroupMax = 4;
 
 // Launch-wide part of a raycast launch.  One launch renders any number of
 // jobs (views x TFs x volumes that share a kernel variant); their ray tiles
@@ -185,8 +193,11 @@ struct JobTableHeader {
   int32_t n_tf_lds;     // > 0: every job reads this 1D TF, staged into LDS
   int32_t pad_[3];
   const float* tf_lds;
-  // [0, 8) per-XCD tile queue heads, then per job j: [8 + 2j] evaluated
-  // samples, [9 + 2j] nominal samples
+  // [0, 8) per-XCD tile queue heads, then per job j: [8 + 3j] evaluated
+  // samples, [9 + 3j] nominal samples, [10 + 3j] gathered samples (those
+  // whose voxels the march loaded: a TF group's walk is counted on its
+  // leader and 0 on its members; a job marched alone gathers the samples
+  // it evaluates)
   unsigned long long* counters;
 };
 
@@ -196,7 +207,8 @@ struct JobTable {
   int32_t tile_start[kMaxJobs + 8];  // prefix sum of the jobs' tile counts
   RaycastParams jobs[kMaxJobs];
 };
-constexpr int kCounterWords = kQueues + 2 * kMaxJobs;
+constexpr int kJobCounters = 3;
+constexpr int kCounterWords = kQueues + kJobCounters * kMaxJobs;
 
 // One brick to cut out of a flat level into a pool slot.
 struct BrickJob {

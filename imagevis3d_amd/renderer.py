@@ -653,6 +653,12 @@ class Renderer:
         """1D march over LDS-staged 8^3 slot cells (no image change)."""
         call("tvk_set_brick_staging", self._h, int(bool(on)))
 
+    def setTFGrouping(self, on: bool) -> None:
+        """Gradient-mode launches walk views that differ only in their TF
+        slot once per ray, compositing every view's TF (default on; no
+        image change)."""
+        call("tvk_set_tf_grouping", self._h, int(bool(on)))
+
     # --------------------------------------------------------------- data
     def getDataset(self) -> Dataset:
         return Dataset(self, self._path)

@@ -448,6 +448,14 @@ int tvk_set_work_group_waves(tvk_renderer* r, uint32_t waves);
  * instead of four x-pair gathers per sample.  The image and the sample
  * counts do not depend on it.  Default 0. */
 int tvk_set_brick_staging(tvk_renderer* r, int on);
+/* TF groups (no counterpart in the reference): with on = 1 (the default), a
+ * launch in a gradient mode (2D TF, or 1D TF with lighting) walks the rays
+ * of views that differ only in their TF slot (same camera, image size,
+ * volume, LOD and settings; up to 4 per group) once, gathering each
+ * sample's value and gradient once and compositing every view's TF.  Each
+ * view's image and sample counts equal its own launch's bit for bit; 0
+ * renders every view on its own. */
+int tvk_set_tf_grouping(tvk_renderer* r, int on);
 /* Ray tile shape of one 64-lane wavefront (no counterpart in the
  * reference; chosen by the autotuner): tile_width x (64 / tile_width) rays,
  * tile_width in {4, 8, 16} (default 8).  The image does not depend on it. */
@@ -511,6 +519,10 @@ typedef struct tvk_view_stats {
   uint64_t nominal_samples;
   uint32_t lod;
   uint32_t reserved;
+  /* samples whose voxels the view's march gathered: its evaluated samples
+   * when it is marched alone; for a TF group (tvk_set_tf_grouping) the
+   * group's shared walk, counted on its first view and 0 on the others */
+  uint64_t gathered_samples;
 } tvk_view_stats;
 
 /* Transfer-function slots 0..63 for batched views (slot 0 is the one

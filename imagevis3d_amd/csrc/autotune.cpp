@@ -87,8 +87,16 @@ int HipRenderer::Autotune(const tvk_view* views, uint32_t n, uint32_t flags, tvk
   // then the launch shape again for the chosen bricks: tuning the bricks
   // under the starting launch shape can settle on bricks that are only best
   // for that shape.
+  // samples per gather batch: 2, 4 or 8 for the 1D march, 1 or 2 for the
+  // gradient marches (2D TF, or 1D TF with lighting); the other modes'
+  // batch is fixed
+  const int mode = job_mode();
+  const bool grad = mode == 1 || (mode == 0 && lighting_);
+  const std::vector<uint32_t> batches =
+      grad ? std::vector<uint32_t>{1u, 2u}
+           : (mode == 0 && !regions_active() ? std::vector<uint32_t>{2u, 4u, 8u} : std::vector<uint32_t>{});
   auto tune_launch = [&]() -> int {
-    if (int e = try_values({2u, 4u, 8u}, &Setting::batch, false)) return e;
+    if (int e = try_values(batches, &Setting::batch, false)) return e;
     if (int e = try_values({4u, 8u, 16u}, &Setting::tile_width, false)) return e;
     if (int e = try_values({1u, 2u, 4u, 8u, 16u}, &Setting::tile_group, false)) return e;
     if (int e = try_values({4u, 8u}, &Setting::wg_waves, false)) return e;
@@ -107,15 +115,26 @@ int HipRenderer::Autotune(const tvk_view* views, uint32_t n, uint32_t flags, tvk
       if (b <= std::max<uint64_t>(32, maxdim)) bricks.push_back(b);
     if (int e = try_values(bricks, &Setting::brick, true)) return e;
   }
-  if ((flags & TVK_TUNE_CELL) && rebrickable) {
-    std::vector<uint32_t> cells;
-    for (uint32_t c = 8; c <= cur.brick; c *= 2)
-      if (cur.brick % c == 0) cells.push_back(c);
-    if (std::find(cells.begin(), cells.end(), cur.brick) == cells.end()) cells.push_back(cur.brick);
-    if (int e = try_values(cells, &Setting::cell, true)) return e;
-  }
-  if ((flags & TVK_TUNE_LAUNCH) && (cur.brick != before.brick || cur.cell != before.cell))
+  // cells, then the launch shape again; a second round when the launch
+  // shape moved (the best cell size depends on it: coordinate descent
+  // from one start settled 5% away from the best on the paged 2D-TF batch)
+  for (int round = 0; round < 2; ++round) {
+    const Setting start = cur;
+    if ((flags & TVK_TUNE_CELL) && rebrickable) {
+      std::vector<uint32_t> cells;
+      for (uint32_t c = 8; c <= cur.brick; c *= 2)
+        if (cur.brick % c == 0) cells.push_back(c);
+      if (std::find(cells.begin(), cells.end(), cur.brick) == cells.end()) cells.push_back(cur.brick);
+      if (int e = try_values(cells, &Setting::cell, true)) return e;
+    }
+    if (!(flags & TVK_TUNE_LAUNCH)) break;
+    if (round == 0 && cur.brick == before.brick && cur.cell == before.cell) break;
     if (int e = tune_launch()) return e;
+    const bool moved = cur.batch != start.batch || cur.tile_width != start.tile_width ||
+                       cur.tile_group != start.tile_group || cur.wg_waves != start.wg_waves ||
+                       cur.stage != start.stage;
+    if (!moved || !(flags & TVK_TUNE_CELL) || !rebrickable) break;
+  }
   if (out) {
     *out = tvk_tuning{};
     out->brick_size = (uint32_t)brick_;

@@ -44,10 +44,13 @@ hipError_t launch_cell_ranges(const void* level, int bpv, const int64_t dims[3],
 
 // Cell ranges of the cells inside one uploaded brick, computed from its pool
 // slot (slot edge S = B + 2*apron; `cells` = the level's cell grid; out =
-// the level's cell-range array).
+// the level's cell-range array).  tmp: 2 (B/C)^3 words set up once by
+// launch_slot_cell_tmp_init and left set up by every call; calls that share
+// it run in one stream.
 hipError_t launch_slot_cell_ranges(const void* slot, int bpv, int S, int apron, int B, int C,
                                    const int brick[3], const int cells[3], uint32_t* out,
-                                   hipStream_t stream);
+                                   uint32_t* tmp, hipStream_t stream);
+hipError_t launch_slot_cell_tmp_init(uint32_t* tmp, int per, hipStream_t stream);
 
 // 1D histogram of raw values, n_bins = 2^(8*bpv), u64 counts (cleared here).
 hipError_t launch_histogram_1d(const void* vol, int bpv, int64_t n, unsigned long long* hist,

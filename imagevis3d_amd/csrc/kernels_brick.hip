@@ -286,35 +286,68 @@ __global__ __launch_bounds__(kThreads) void pack_ranges_kernel(uint32_t* __restr
     out[i] |= hi[i] << 16;
 }
 
-// Cell ranges of one brick from its pool slot (paged pools, after upload):
-// one wave per cell of the brick; the slot's clamp-to-edge apron gives the
-// same values as clamping to the level.
+// Cell ranges of one brick from its pool slot (paged pools, after upload).
+// Cell c of the brick covers level voxels [c*C - 1, (c+1)*C], clamped into
+// the slot, whose clamp-to-edge apron gives the values clamping to the level
+// would.  Work-group (c, g) folds every gridDim.y-th row step of cell c (a
+// wave takes 64/w rows of w < 64 voxels at once, else one row) into
+// lo[c] / hi[c] with atomics, so a large cell spreads over many CUs
+// (one wave per 256^3 cell ran 62 ms per brick); slot_cell_pack_kernel then
+// writes the packed ranges and resets lo / hi for the next brick.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void slot_cell_range_kernel(
     const T* __restrict__ slot, int S, int apron, int B, int C, int bx, int by, int bz,
-    int ncx, int ncy, int ncz, uint32_t* __restrict__ out) {
+    int ncx, int ncy, int ncz, uint32_t* __restrict__ lo_out, uint32_t* __restrict__ hi_out) {
   const int per = B / C;
-  const int wave = (int)(blockIdx.x * (kThreads / 64) + threadIdx.x / 64);
-  const int lane = (int)(threadIdx.x & 63);
-  if (wave >= per * per * per) return;
-  const int cx = bx * per + wave % per, cy = by * per + (wave / per) % per, cz = bz * per + wave / (per * per);
-  if (cx >= ncx || cy >= ncy || cz >= ncz) return;
-  // level voxels [c*C - 1, (c+1)*C] -> slot-local (clamped into the slot)
+  const int c = (int)blockIdx.x;  // < per^3
+  const int cx = bx * per + c % per, cy = by * per + (c / per) % per, cz = bz * per + c / (per * per);
+  if (cx >= ncx || cy >= ncy || cz >= ncz) return;  // whole work-group
   const int x0 = max(cx * C - 1 - (bx * B - apron), 0), x1 = min((cx + 1) * C - (bx * B - apron), S - 1);
   const int y0 = max(cy * C - 1 - (by * B - apron), 0), y1 = min((cy + 1) * C - (by * B - apron), S - 1);
   const int z0 = max(cz * C - 1 - (bz * B - apron), 0), z1 = min((cz + 1) * C - (bz * B - apron), S - 1);
   const int w = x1 - x0 + 1, h = y1 - y0 + 1, dz = z1 - z0 + 1;
+  const int rows = h * dz;
+  const int lane = (int)(threadIdx.x & 63);
+  const int wave = (int)blockIdx.y * (kThreads / 64) + (int)(threadIdx.x >> 6);
+  const int waves = (int)gridDim.y * (kThreads / 64);
+  const int rpw = w < 64 ? 64 / w : 1;     // rows per wave step
+  const int lrow = w < 64 ? lane / w : 0;  // this lane's row in the step
+  const int lx = w < 64 ? lane % w : lane;  // and its first x
+  const int xstep = w < 64 ? w : 64;
   const SlotGeo G = slot_geo(S);
   uint32_t lo = 0xffffffffu, hi = 0;
-  for (int i = lane; i < w * h * dz; i += 64) {
-    const int x = x0 + i % w, y = y0 + (i / w) % h, z = z0 + i / (w * h);
-    const uint32_t v = slot[slot_offset(G, x, y, z)];
-    lo = min(lo, v);
-    hi = max(hi, v);
+  if (lrow < rpw) {
+    for (int r = wave * rpw + lrow; r < rows; r += waves * rpw) {
+      const int y = y0 + r % h, z = z0 + r / h;
+      const int64_t yz = slot_oy(G, y) + slot_oz(G, z);
+      for (int x = x0 + lx; x <= x1; x += xstep) {
+        const uint32_t v = slot[yz + slot_ox(x)];
+        lo = min(lo, v);
+        hi = max(hi, v);
+      }
+    }
   }
   lo = wave_min(lo);
   hi = wave_max(hi);
-  if (lane == 0) out[((int64_t)cz * ncy + cy) * ncx + cx] = lo | (hi << 16);
+  if (lane == 0 && hi >= lo) {  // a wave with no rows folds nothing
+    atomicMin(lo_out + c, lo);
+    atomicMax(hi_out + c, hi);
+  }
+}
+
+// The brick's cells' ranges into the level's cell-range array, lo / hi
+// reset for the next brick.
+__global__ __launch_bounds__(kThreads) void slot_cell_pack_kernel(uint32_t* __restrict__ lo,
+                                                                  uint32_t* __restrict__ hi, int per,
+                                                                  int bx, int by, int bz, int ncx,
+                                                                  int ncy, int ncz,
+                                                                  uint32_t* __restrict__ out) {
+  const int c = (int)(blockIdx.x * kThreads + threadIdx.x);
+  if (c >= per * per * per) return;
+  const int cx = bx * per + c % per, cy = by * per + (c / per) % per, cz = bz * per + c / (per * per);
+  if (cx < ncx && cy < ncy && cz < ncz) out[((int64_t)cz * ncy + cy) * ncx + cx] = lo[c] | (hi[c] << 16);
+  lo[c] = 0xffffffffu;
+  hi[c] = 0u;
 }
 
 __global__ void minmax_init_kernel(uint32_t* mm, int n) {
@@ -655,18 +688,34 @@ hipError_t launch_cell_ranges(const void* level, int bpv, const int64_t d[3], in
 
 hipError_t launch_slot_cell_ranges(const void* slot, int bpv, int S, int apron, int B, int C,
                                    const int brick[3], const int cells[3], uint32_t* out,
-                                   hipStream_t stream) {
+                                   uint32_t* tmp, hipStream_t stream) {
   const int per = B / C;
-  const int waves = per * per * per;
-  const unsigned blocks = (unsigned)((waves + (kThreads / 64) - 1) / (kThreads / 64));
+  const int n = per * per * per;
+  // work-groups per cell: about 8 row steps per wave over the cell
+  const int64_t w = C + 2, rows = w * w;
+  const int64_t rpw = w < 64 ? 64 / w : 1;
+  const int64_t per_group = (kThreads / 64) * rpw * 8;
+  const unsigned gy = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (rows + per_group - 1) / per_group), 65535);
+  const dim3 grid((unsigned)n, gy, 1);
+  uint32_t* lo = tmp;
+  uint32_t* hi = tmp + n;
   if (bpv == 1)
-    slot_cell_range_kernel<uint8_t><<<blocks, kThreads, 0, stream>>>(
-        (const uint8_t*)slot, S, apron, B, C, brick[0], brick[1], brick[2], cells[0], cells[1],
-        cells[2], out);
+    slot_cell_range_kernel<uint8_t><<<grid, kThreads, 0, stream>>>(
+        (const uint8_t*)slot, S, apron, B, C, brick[0], brick[1], brick[2], cells[0], cells[1], cells[2], lo, hi);
   else
-    slot_cell_range_kernel<uint16_t><<<blocks, kThreads, 0, stream>>>(
-        (const uint16_t*)slot, S, apron, B, C, brick[0], brick[1], brick[2], cells[0], cells[1],
-        cells[2], out);
+    slot_cell_range_kernel<uint16_t><<<grid, kThreads, 0, stream>>>(
+        (const uint16_t*)slot, S, apron, B, C, brick[0], brick[1], brick[2], cells[0], cells[1], cells[2], lo,
+        hi);
+  slot_cell_pack_kernel<<<(unsigned)((n + kThreads - 1) / kThreads), kThreads, 0, stream>>>(
+      lo, hi, per, brick[0], brick[1], brick[2], cells[0], cells[1], cells[2], out);
+  return hipGetLastError();
+}
+
+hipError_t launch_slot_cell_tmp_init(uint32_t* tmp, int per, hipStream_t stream) {
+  const int64_t n = (int64_t)per * per * per;
+  const unsigned blocks = (unsigned)std::min<int64_t>((n + kThreads - 1) / kThreads, 65536);
+  fill_u32_kernel<<<blocks, kThreads, 0, stream>>>(tmp, n, 0xffffffffu);
+  fill_u32_kernel<<<blocks, kThreads, 0, stream>>>(tmp + n, n, 0u);
   return hipGetLastError();
 }
 

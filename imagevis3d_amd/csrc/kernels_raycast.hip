@@ -1648,6 +1648,15 @@ hipError_t launch_one(const LaunchInfo& li, const JobTable* tab, hipStream_t str
   return hipGetLastError();
 }
 
+// The gradient variants (2D TF, or 1D TF with lighting): TF groups, or one
+// march per view with the 1D TF in LDS when the launch shares one.
+template <typename T, bool LINEAR, int BATCH>
+hipError_t launch_grad(const LaunchInfo& li, const JobTable* tab, hipStream_t stream) {
+  if (li.group) return launch_one<T, LINEAR, BATCH, false, kModeGradGroup>(li, tab, stream);
+  return li.mode == 0 && li.n_tf_lds > 0 ? launch_one<T, LINEAR, BATCH, true, kModeGrad>(li, tab, stream)
+                                         : launch_one<T, LINEAR, BATCH, false, kModeGrad>(li, tab, stream);
+}
+
 template <typename T, bool LINEAR>
 hipError_t launch_typed(const LaunchInfo& li, const JobTable* tab, hipStream_t stream) {
   if (li.regions)
@@ -1657,17 +1666,13 @@ hipError_t launch_typed(const LaunchInfo& li, const JobTable* tab, hipStream_t s
   if (li.mode == 3)
     return li.n_tf_lds > 0 ? launch_one<T, LINEAR, 4, true, kModeMIP>(li, tab, stream)
                            : launch_one<T, LINEAR, 4, false, kModeMIP>(li, tab, stream);
-  if (li.mode == 1)
-    return li.group ? launch_one<T, LINEAR, 2, false, kModeGradGroup>(li, tab, stream)
-                    : launch_one<T, LINEAR, 2, false, kModeGrad>(li, tab, stream);
+  // the gradient marches gather 1 or 2 samples per batch (li.batch 1, else 2)
+  if (li.mode == 1 || li.lighting)
+    return li.batch == 1 ? launch_grad<T, LINEAR, 1>(li, tab, stream) : launch_grad<T, LINEAR, 2>(li, tab, stream);
   const bool lds = li.n_tf_lds > 0;
   if (li.stage && LINEAR && !li.lighting)
     return lds ? launch_one<T, true, 1, true, kMode1DStaged>(li, tab, stream)
                : launch_one<T, true, 1, false, kMode1DStaged>(li, tab, stream);
-  if (li.lighting && li.group) return launch_one<T, LINEAR, 2, false, kModeGradGroup>(li, tab, stream);
-  if (li.lighting)
-    return lds ? launch_one<T, LINEAR, 2, true, kModeGrad>(li, tab, stream)
-               : launch_one<T, LINEAR, 2, false, kModeGrad>(li, tab, stream);
   switch (li.batch) {
     case 2:
       return lds ? launch_one<T, LINEAR, 2, true, kMode1D>(li, tab, stream)

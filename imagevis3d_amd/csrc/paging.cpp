@@ -267,6 +267,17 @@ int HipRenderer::upload_bricks(const int* gids, const int* lods, void* const* ad
   const uint64_t bb = (uint64_t)slot_edge_ * slot_edge_ * slot_edge_ * bpv;
   const size_t R = (size_t)staging_round_;
   const auto t0 = std::chrono::steady_clock::now();
+  // per-cell folding words of the cell-range kernels (copy-stream order)
+  const int per = brick_ / cell_;
+  if (slot_cell_tmp_per_ != per) {
+    const size_t need = 2ull * per * per * per * sizeof(uint32_t);
+    if (slot_cell_tmp_.bytes < need) {
+      TVK_HIP(hipStreamSynchronize(copy_stream_));  // earlier bricks' kernels may hold it
+      TVK_HIP(slot_cell_tmp_.alloc(need));
+    }
+    TVK_HIP(launch_slot_cell_tmp_init(slot_cell_tmp_.as<uint32_t>(), per, copy_stream_));
+    slot_cell_tmp_per_ = per;
+  }
   std::atomic<int> failed{-1};
   for (size_t first = 0, r = 0; first < n; first += R, ++r) {
     const size_t cnt = std::min(R, n - first);
@@ -303,7 +314,8 @@ int HipRenderer::upload_bricks(const int* gids, const int* lods, void* const* ad
                              copy_stream_));
       TVK_HIP(launch_linear_to_slot(upload_stage_[j].p, addrs[i], bpv, slot_edge_, copy_stream_));
       TVK_HIP(launch_slot_cell_ranges(addrs[i], bpv, slot_edge_, apron_, brick_, cell_, b, L.cells,
-                                      cell_range_.as<uint32_t>() + L.cell_base, copy_stream_));
+                                      cell_range_.as<uint32_t>() + L.cell_base, slot_cell_tmp_.as<uint32_t>(),
+                                      copy_stream_));
     }
     TVK_HIP(hipEventRecord(staging_ev_[set], copy_stream_));
   }

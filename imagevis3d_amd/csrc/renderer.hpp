@@ -437,6 +437,8 @@ class HipRenderer : public PoolClient {
   std::vector<void*> staging_;
   hipEvent_t staging_ev_[2] = {nullptr, nullptr};
   std::vector<DevBuf> upload_stage_;
+  DevBuf slot_cell_tmp_;        // per-cell min / max words of one brick's cell-range pass
+  int slot_cell_tmp_per_ = 0;   // cells per brick edge they are set up for
   int staging_round_ = 0;              // bricks per round
   uint64_t staging_host_bytes_ = 0;    // accounted as host memory
   hipStream_t copy_stream_ = nullptr;  // brick uploads

@@ -74,6 +74,13 @@ def test_groups_2d_phong_equal_separate_views():
         assert all(ga[i] == 0 for i in members)
     assert ga[6] == ev[6] and ga[0] < sum(ev[:4])
     assert np.any(got[0] != got[1])
+    # one sample per gather batch (grouped and alone) gives the same images
+    r.setKernelTuning(1, 4, 0)
+    got1, st1 = grouped_vs_alone(r, views, "2D Phong, batch 1")
+    for i in range(len(views)):
+        assert np.array_equal(got1[i], got[i]), f"batch 1: view {i}"
+        assert st1[i].evaluated_samples == st[i].evaluated_samples
+    r.setKernelTuning(8, 4, 0)
     # the group's images equal single captures too
     r.resize((120, 90))
     r.set2DTrans(lobes[2], mg)

@@ -128,40 +128,54 @@ def run_serving(duration_s: float, rate_hz: float, seed: int = 5) -> dict:
     tune_s = time.perf_counter() - t0
     srv = RenderServer(arena_bytes=4 << 30, lanes=4)
     ids = [srv.addVolume(r) for r in rs]
-    rng = np.random.default_rng(seed)
-    n_req = int(duration_s * rate_hz)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate_hz, n_req))
-    reqs = []
-    for _ in range(n_req):
-        v, view, mode = workloads.serving_request(rng, len(ids))
-        reqs.append((ids[v], view, mode))
     # warm every kernel variant and graph once (not timed)
     for v in range(len(ids)):
         for mode, slot in ((RenderMode.RM_1DTRANS, 0), (RenderMode.RM_2DTRANS, 2)):
             srv.wait(srv.submit(ids[v], dict(view_pos=(0, 0, 2), view_dir=(0, 0, -1), fov=50.0,
                                              width=64, height=64, tf_slot=slot), mode))
-    tickets = [None] * n_req
-    results = [None] * n_req
 
-    def waiter():
-        for i in range(n_req):
-            while tickets[i] is None:
-                time.sleep(0.0002)
-            results[i] = srv.wait(tickets[i])[1]
+    def drive(rate, duration, rng):
+        """Poisson arrivals at `rate` for `duration` s: (requests, results, wall s)."""
+        n = int(duration * rate)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+        rq = []
+        for _ in range(n):
+            v, view, mode = workloads.serving_request(rng, len(ids))
+            rq.append((ids[v], view, mode))
+        tickets = [None] * n
+        res = [None] * n
 
-    th_wait = threading.Thread(target=waiter)
-    th_wait.start()
-    start = time.perf_counter()
-    for i, (vid, view, mode) in enumerate(reqs):
-        delay = start + arrivals[i] - time.perf_counter()
-        if delay > 0:
-            time.sleep(delay)
-        tickets[i] = srv.submit(vid, view, mode, readback=False)
-    th_wait.join()
-    wall = time.perf_counter() - start
+        def waiter():
+            for i in range(n):
+                while tickets[i] is None:
+                    time.sleep(0.0002)
+                res[i] = srv.wait(tickets[i])[1]
+
+        th_wait = threading.Thread(target=waiter)
+        th_wait.start()
+        start = time.perf_counter()
+        for i, (vid, view, mode) in enumerate(rq):
+            delay = start + arrivals[i] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets[i] = srv.submit(vid, view, mode, readback=False)
+        th_wait.join()
+        return rq, res, time.perf_counter() - start
+
+    rng = np.random.default_rng(seed)
+    reqs, results, wall = drive(rate_hz, duration_s, rng)
+    n_req = len(reqs)
     st = srv.stats()
     lat = np.array([r.latency_ms for r in results])
     nominal = sum(r.nominal_samples for r in results)
+    # the same mix at half the rate, below this GPU's capacity: latency
+    # without the queueing of the saturated run
+    h_reqs, h_res, h_wall = drive(rate_hz / 2, duration_s / 2, np.random.default_rng(seed + 1))
+    h_lat = np.array([r.latency_ms for r in h_res])
+    half = {"offered_rate_hz": rate_hz / 2, "completed_rate_hz": len(h_reqs) / h_wall,
+            "requests": len(h_reqs), "p50_ms": float(np.percentile(h_lat, 50)),
+            "p99_ms": float(np.percentile(h_lat, 99)),
+            "mean_queue_ms": float(np.mean([r.queue_ms for r in h_res]))}
 
     def by(key):
         # latency percentiles per request class
@@ -187,6 +201,7 @@ def run_serving(duration_s: float, rate_hz: float, seed: int = 5) -> dict:
         "latency_by_image": by(lambda vid, view, mode: f"{view['width']}x{view['height']}"),
         "latency_by_volume_and_tf": by(
             lambda vid, view, mode: vol_name[vid] + ("-2d" if mode == RenderMode.RM_2DTRANS else "-1d")),
+        "half_rate": half,
         "note": "image in pinned host memory counts as done; images 256^2..3840x2160",
     }
     srv.close()
@@ -294,8 +309,11 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
     their mean, to stay within 1/255).  "checked" holds every frame to
     1/255 at run time: the per-frame check estimates each learned frame's
     RMSE from a sample of level-0 tiles and renders the frame at level 0
-    above the bound; its frame rate is that of a frame loop in which the
-    host selects the next frame while the device renders the current one.
+    above the bound; its selection (the error target at 1, 1/2 or 1/4 of
+    its factor, or the 16 GB brick budget) is the fastest whose
+    calibration frames all end within 1/255; its frame rate is that of a
+    frame loop in which the host selects the next frame while the device
+    renders the current one.
     "out_of_core" replays the budgeted frames from a 16 GB paged pool over a
     host-memory copy of the volume.  Frame time of the learned path
     includes features, MLP and selection."""
@@ -370,15 +388,17 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
         LL.set_view(r, v)
         cal_refs.append(LL.full_resolution(r, dims, edge))
 
-    def checked_accuracy(vs, refs, scale):
-        """Per-frame RMSE vs level 0 and fallbacks, one capture per view."""
+    def checked_accuracy(vs, refs, sel):
+        """Per-frame RMSE vs level 0 and fallbacks, one capture per view;
+        sel = (operating point, budget-factor scale)."""
+        point, scale = sel
         r.setRegionLODCheck(check_k, check_bound)
         errs, est, fell = [], [], 0
         for v, ref in zip(vs, refs):
             r.resize((v["width"], v["height"]))
             LL.set_view(r, v)
-            lev, _, _ = LL.choose_levels(pred, v, dims, edge, n_lod, hist, mom, cal["target"][0] * scale,
-                                         slot_bytes, 0, detail)
+            lev, _, _ = LL.choose_levels(pred, v, dims, edge, n_lod, hist, mom, cal[point][0] * scale,
+                                         slot_bytes, points[point], detail)
             r.setRegionLODs(lev)
             img = r.capture()
             c = r.regionLODCheck()
@@ -389,20 +409,20 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
         r.setRegionLODCheck(0, 0.0)
         return errs, est, fell
 
-    def pipelined_ms(vs, scale=None):
+    def pipelined_ms(vs, sel=None):
         """Wall time per frame of a frame loop in which the host selects the
         next frame's levels while the device renders the current one
-        (scale None: every frame at full resolution, no selection)."""
+        (sel None: every frame at full resolution, no selection)."""
         r.resize((vs[0]["width"], vs[0]["height"]))
-        if scale is not None:
+        if sel is not None:
             r.setRegionLODCheck(check_k, check_bound)
         r.synchronize()
         t0 = time.perf_counter()
         for v in vs:
             LL.set_view(r, v)
-            if scale is not None:
+            if sel is not None:
                 lev, _, _ = LL.choose_levels(pred, v, dims, edge, n_lod, hist, mom,
-                                             cal["target"][0] * scale, slot_bytes, 0, detail)
+                                             cal[sel[0]][0] * sel[1], slot_bytes, points[sel[0]], detail)
                 r.setRegionLODs(lev)
             else:
                 r.setRegionLODs(zero_regions)
@@ -414,15 +434,17 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
         return ms
 
     zero_regions = np.zeros((n // edge) ** 3, np.uint8)
-    check_scales = (1.0, 0.5, 0.25)
+    # candidate selections of the checked loop: the error target alone at
+    # fractions of its factor, and the 16 GB brick budget at its factor
+    check_sels = (("target", 1.0), ("target", 0.5), ("target", 0.25), ("budgeted", 1.0))
     check_cal = {}
     say("calibrating the checked loop")
-    for s in check_scales:
-        errs, _, fell = checked_accuracy(cal_views, cal_refs, s)
-        check_cal[s] = {"rmse_max": float(np.max(errs)), "fallbacks": fell,
-                        "ms": pipelined_ms(cal_views, s)}
-    ok = [s for s in check_scales if check_cal[s]["rmse_max"] <= 1.0]
-    check_scale = min(ok, key=lambda s: check_cal[s]["ms"]) if ok else min(check_scales)
+    for sel in check_sels:
+        errs, _, fell = checked_accuracy(cal_views, cal_refs, sel)
+        check_cal[sel] = {"rmse_max": float(np.max(errs)), "fallbacks": fell,
+                          "ms": pipelined_ms(cal_views, sel)}
+    ok = [sel for sel in check_sels if check_cal[sel]["rmse_max"] <= 1.0]
+    check_sel = min(ok, key=lambda sel: check_cal[sel]["ms"]) if ok else ("target", 0.25)
 
     # full resolution at its best: the uniform LOD-0 path with its
     # empty-space cells and launch shape tuned for this volume (the default
@@ -478,7 +500,7 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
             if k == "budgeted":  # replayed out of core below
                 ooc_in.append((v, lev, img, ref, sel))
             r.setRegionLODs(None)
-        errs, est, fell = checked_accuracy([v], [ref], check_scale)
+        errs, est, fell = checked_accuracy([v], [ref], check_sel)
         chk["rmse"] += errs
         chk["estimate"] += est
         chk["fallbacks"] += fell
@@ -511,9 +533,10 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
     say("pipelined frame loops")
     eval_vs = views(eval_views, 13, *eval_size)
     full_pipe = pipelined_ms(eval_vs)
-    checked_pipe = pipelined_ms(eval_vs, check_scale)
+    checked_pipe = pipelined_ms(eval_vs, check_sel)
     checked = {
-        "tile_sample": check_k, "rmse_bound": check_bound, "factor_scale": check_scale,
+        "tile_sample": check_k, "rmse_bound": check_bound, "selection": check_sel[0],
+        "factor_scale": check_sel[1],
         "fps_learned": 1e3 / checked_pipe, "fps_full": 1e3 / full_pipe,
         "speedup": full_pipe / checked_pipe,
         # against the per-view best full-resolution kernel time
@@ -522,7 +545,7 @@ def run_learned_lod(n: int = 2048, edge: int = 128, n_lod: int = 6, eval_views: 
         "frames_within_1": float(np.mean(np.asarray(chk["rmse"]) <= 1.0)),
         "estimate_mean": float(np.mean(chk["estimate"])),
         "fallback_frames": chk["fallbacks"], "frames": len(chk["rmse"]),
-        "calibration": {str(s): c for s, c in check_cal.items()},
+        "calibration": {f"{p}x{s}": c for (p, s), c in check_cal.items()},
     }
 
     # the predictor alone on one frame's inputs (every region x coarser level)

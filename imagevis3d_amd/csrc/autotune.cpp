@@ -125,7 +125,36 @@ int HipRenderer::Autotune(const tvk_view* views, uint32_t n, uint32_t flags, tvk
       for (uint32_t c = 8; c <= cur.brick; c *= 2)
         if (cur.brick % c == 0) cells.push_back(c);
       if (std::find(cells.begin(), cells.end(), cur.brick) == cells.end()) cells.push_back(cur.brick);
-      if (int e = try_values(cells, &Setting::cell, true)) return e;
+      // each cell size with both work-group sizes: large cells (long
+      // unskipped runs) and small ones favour different occupancies, and
+      // tuning them one at a time settled up to 6% away from the best pair
+      // on the paged 2D-TF batch
+      const uint32_t wgs[2] = {cur.wg_waves, cur.wg_waves == 4 ? 8u : 4u};
+      const Setting base = cur;
+      uint32_t bricked = cur.cell;  // the cell size the volume is set up with
+      for (uint32_t c : cells) {
+        for (int k = 0; k < ((flags & TVK_TUNE_LAUNCH) ? 2 : 1); ++k) {
+          Setting s = base;
+          s.cell = c;
+          s.wg_waves = wgs[k];
+          if (c == base.cell && k == 0) continue;  // the current setting, already timed
+          const bool rebrick = s.cell != bricked;
+          bricked = s.cell;
+          if (int e = apply(s, rebrick)) {
+            if (int e2 = apply(cur, true)) return e2;
+            bricked = cur.cell;
+            (void)e;
+            continue;
+          }
+          float ms = 0.f;
+          if (int e = time_now(&ms)) return e;
+          if (ms < best) {
+            best = ms;
+            cur = s;
+          }
+        }
+      }
+      if (int e = apply(cur, true)) return e;
     }
     if (!(flags & TVK_TUNE_LAUNCH)) break;
     if (round == 0 && cur.brick == before.brick && cur.cell == before.cell) break;

@@ -97,7 +97,7 @@ def run_interactive(r, w, views, tune=None, warmup: int = 4) -> dict:
             "launch": tuning}
 
 
-def run_serving(duration_s: float, rate_hz: float, seed: int = 5) -> dict:
+def run_serving(duration_s: float, rate_hz: float, seed: int = 5, lanes: int = 4) -> dict:
     """The serving workload: seeded Poisson arrivals at `rate_hz` for
     `duration_s` over a pool of 7 device-generated volumes (64^3, 256^3 and
     512^3 in uint8 and uint16, one 2048^3 uint16), each request a random
@@ -126,7 +126,7 @@ def run_serving(duration_s: float, rate_hz: float, seed: int = 5) -> dict:
         tuned.append({"samples_per_batch": t.samples_per_batch, "tile_width": t.tile_width,
                       "work_group_waves": t.wg_waves, "brick_staging": t.brick_staging})
     tune_s = time.perf_counter() - t0
-    srv = RenderServer(arena_bytes=4 << 30, lanes=4)
+    srv = RenderServer(arena_bytes=4 << 30, lanes=lanes)
     ids = [srv.addVolume(r) for r in rs]
     # warm every kernel variant and graph once (not timed)
     for v in range(len(ids)):
@@ -712,6 +712,7 @@ def main() -> int:
     ap.add_argument("--serving-seconds", type=float, default=10.0)
     ap.add_argument("--serving-rate", type=float, default=200.0)
     ap.add_argument("--serving-only", action="store_true")
+    ap.add_argument("--serving-lanes", type=int, default=4, help="streams of the render server")
     ap.add_argument("--no-batched", action="store_true", help="skip the out-of-core batched workload")
     ap.add_argument("--batched-only", action="store_true")
     ap.add_argument("--no-learned-lod", action="store_true", help="skip the learned-LOD workload")
@@ -733,7 +734,7 @@ def main() -> int:
     from imagevis3d_amd import tuvok, workloads
 
     if args.serving_only:
-        print(json.dumps(run_serving(args.serving_seconds, args.serving_rate)))
+        print(json.dumps(run_serving(args.serving_seconds, args.serving_rate, lanes=args.serving_lanes)))
         return 0
     if args.batched_only:
         print(json.dumps(run_batched(tune=not args.no_autotune, groups=not args.no_tf_groups)))
@@ -866,7 +867,7 @@ def main() -> int:
     if not args.no_serving:
         print(f"[bench] serving workload ({args.serving_seconds:g} s at {args.serving_rate:g} req/s)",
               file=sys.stderr, flush=True)
-        sv = run_serving(args.serving_seconds, args.serving_rate)
+        sv = run_serving(args.serving_seconds, args.serving_rate, lanes=args.serving_lanes)
         out["serving"] = sv
         out["p99_ms"] = sv["p99_ms"]
         print(f"[bench] serving {sv}", file=sys.stderr, flush=True)

@@ -1666,9 +1666,9 @@ hipError_t launch_typed(const LaunchInfo& li, const JobTable* tab, hipStream_t s
   if (li.mode == 3)
     return li.n_tf_lds > 0 ? launch_one<T, LINEAR, 4, true, kModeMIP>(li, tab, stream)
                            : launch_one<T, LINEAR, 4, false, kModeMIP>(li, tab, stream);
-  // the gradient marches gather 1 or 2 samples per batch (li.batch 1, else 2)
+  // the gradient marches gather 2 samples per batch when li.batch is 2, else 1
   if (li.mode == 1 || li.lighting)
-    return li.batch == 1 ? launch_grad<T, LINEAR, 1>(li, tab, stream) : launch_grad<T, LINEAR, 2>(li, tab, stream);
+    return li.batch == 2 ? launch_grad<T, LINEAR, 2>(li, tab, stream) : launch_grad<T, LINEAR, 1>(li, tab, stream);
   const bool lds = li.n_tf_lds > 0;
   if (li.stage && LINEAR && !li.lighting)
     return lds ? launch_one<T, true, 1, true, kMode1DStaged>(li, tab, stream)

@@ -62,6 +62,7 @@ def test_groups_2d_phong_equal_separate_views():
     views = [view(CAMS[0], (120, 90), s) for s in range(6)]
     views += [view(CAMS[0], (90, 120), 1)]
     views += [view(CAMS[1], (120, 90), s) for s in (0, 4, 2)]
+    r.setKernelTuning(2, 4, 0)  # 2 samples per gather batch
     got, st = grouped_vs_alone(r, views, "2D Phong")
     assert st[4].evaluated_samples == 0  # transparent TF: every cell skipped
     # the walk of a group gathers the union of its members' samples, counted

@@ -97,7 +97,8 @@ def run_interactive(r, w, views, tune=None, warmup: int = 4) -> dict:
             "launch": tuning}
 
 
-def run_serving(duration_s: float, rate_hz: float, seed: int = 5, lanes: int = 4) -> dict:
+def run_serving(duration_s: float, rate_hz: float, seed: int = 5, lanes: int = 4,
+                tune_size=(512, 512)) -> dict:
     """The serving workload: seeded Poisson arrivals at `rate_hz` for
     `duration_s` over a pool of 7 device-generated volumes (64^3, 256^3 and
     512^3 in uint8 and uint16, one 2048^3 uint16), each request a random
@@ -116,13 +117,13 @@ def run_serving(duration_s: float, rate_hz: float, seed: int = 5, lanes: int = 4
     t0 = time.perf_counter()
     rs = workloads.serving_pool()
     setup_s = time.perf_counter() - t0
-    # each volume's launch shape tuned once on one oblique 512^2 view (setup,
-    # not timed)
+    # each volume's launch shape tuned once on one oblique view (setup, not
+    # timed) of tune_size
     t0 = time.perf_counter()
     tuned = []
     for r in rs:
         t = r.autotune([dict(view_pos=(0.9, 0.6, 1.4), view_dir=(-0.9, -0.6, -1.4), fov=50.0,
-                             width=512, height=512, tf_slot=0)], flags=r.TUNE_LAUNCH)
+                             width=tune_size[0], height=tune_size[1], tf_slot=0)], flags=r.TUNE_LAUNCH)
         tuned.append({"samples_per_batch": t.samples_per_batch, "tile_width": t.tile_width,
                       "work_group_waves": t.wg_waves, "brick_staging": t.brick_staging})
     tune_s = time.perf_counter() - t0
@@ -713,6 +714,8 @@ def main() -> int:
     ap.add_argument("--serving-rate", type=float, default=200.0)
     ap.add_argument("--serving-only", action="store_true")
     ap.add_argument("--serving-lanes", type=int, default=4, help="streams of the render server")
+    ap.add_argument("--serving-tune-size", type=int, nargs=2, default=(512, 512),
+                    help="image size of the view each serving volume's launch shape is tuned on")
     ap.add_argument("--no-batched", action="store_true", help="skip the out-of-core batched workload")
     ap.add_argument("--batched-only", action="store_true")
     ap.add_argument("--no-learned-lod", action="store_true", help="skip the learned-LOD workload")
@@ -734,7 +737,8 @@ def main() -> int:
     from imagevis3d_amd import tuvok, workloads
 
     if args.serving_only:
-        print(json.dumps(run_serving(args.serving_seconds, args.serving_rate, lanes=args.serving_lanes)))
+        print(json.dumps(run_serving(args.serving_seconds, args.serving_rate, lanes=args.serving_lanes,
+                                     tune_size=tuple(args.serving_tune_size))))
         return 0
     if args.batched_only:
         print(json.dumps(run_batched(tune=not args.no_autotune, groups=not args.no_tf_groups)))
@@ -867,7 +871,8 @@ def main() -> int:
     if not args.no_serving:
         print(f"[bench] serving workload ({args.serving_seconds:g} s at {args.serving_rate:g} req/s)",
               file=sys.stderr, flush=True)
-        sv = run_serving(args.serving_seconds, args.serving_rate, lanes=args.serving_lanes)
+        sv = run_serving(args.serving_seconds, args.serving_rate, lanes=args.serving_lanes,
+                         tune_size=tuple(args.serving_tune_size))
         out["serving"] = sv
         out["p99_ms"] = sv["p99_ms"]
         print(f"[bench] serving {sv}", file=sys.stderr, flush=True)

@@ -1653,7 +1653,8 @@ static int fill_job_table(JobTable* t, const RaycastParams* jobs, int n, const L
       if (lead.group_n != 0) continue;
       int cnt = 1;
       for (int m = k + 1; m < n && cnt < kGroupMax; ++m) {
-        if (t->jobs[m].group_n != 0 || !same_but_tf(lead, t->jobs[m])) continue;
+        // compared as the caller built them (zeroed, so padding compares equal)
+        if (t->jobs[m].group_n != 0 || !same_but_tf(jobs[k], jobs[m])) continue;
         lead.group_jobs[cnt - 1] = m;
         t->jobs[m].group_n = -1;
         ++cnt;

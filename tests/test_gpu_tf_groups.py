@@ -166,3 +166,27 @@ def test_groups_on_paged_pool():
     assert pg.memStats().bricks_uploaded > 0
     pg.cleanup()
     ic.cleanup()
+
+
+def test_groups_allocation_free_launch():
+    from test_zz_no_alloc import render_noalloc
+    vol = volume(64)
+    print("case TF groups: allocation-free launch vs ungrouped batch", flush=True)
+    _, mg = O.histogram_2d_exact(vol, 128, 128)
+    lobes = [transfer.gaussian_lobes_2d(128, 128, seed=s) for s in (11, 12, 13)]
+    r = gpu_renderer(width=80, height=60)
+    r.setVolume(vol, 32, 2, 1)
+    r.setRenderMode(RenderMode.RM_2DTRANS)
+    r.setLightingEnabled(True)
+    for s, t in enumerate(lobes):
+        r.set2DTransSlot(s, t, mg)
+    views = [view(c, (80, 60), s) for c in CAMS for s in range(3)]
+    r.setTFGrouping(False)
+    ref, rst, _ = r.renderBatch(views)  # also puts every TF slot on the device
+    r.setTFGrouping(True)
+    got, gst = render_noalloc(r, views)
+    for i in range(len(views)):
+        assert np.array_equal(got[i], ref[i]), f"view {i}"
+        assert gst[i].evaluated_samples == rst[i].evaluated_samples
+    assert gst[0].gathered_samples > 0 and gst[1].gathered_samples == 0  # view 0 leads a group
+    r.cleanup()

@@ -1633,10 +1633,11 @@ hipError_t launch_one(const LaunchInfo& li, const JobTable* tab, hipStream_t str
   int per_cu = 0;
   // Work-group size: 8 waves when the 1D TF is staged in LDS (64 KB per
   // work-group, so two fit a CU's 160 KB); otherwise li.wg_waves (4 or 8).
-  // The variants without an LDS TF take 135-173 VGPRs (2-3 waves per SIMD):
-  // 8-wave work-groups fit a CU once (8 waves), 4-wave ones up to three
-  // times (12 waves), which is faster on cache-friendly volumes and slower
-  // where more waves in flight thrash the caches (the autotuner picks).
+  // The variants without an LDS TF take 114-173 VGPRs (2-4 waves per SIMD;
+  // the gradient marches 114-128 at 1 sample per batch): 8-wave
+  // work-groups fit a CU once or twice, 4-wave ones up to four times,
+  // which is faster on cache-friendly volumes and slower where more waves
+  // in flight thrash the caches (the autotuner picks).
   const int kT = TF_LDS ? kThreads : 64 * (li.wg_waves == 4 ? 4 : 8);
   if (MODE == kMode1DStaged) lds += (size_t)(kT / 64) * kStageElems * sizeof(T);  // per-wave cell buffers
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kT, lds);

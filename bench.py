@@ -211,7 +211,8 @@ def run_serving(duration_s: float, rate_hz: float, seed: int = 5, lanes: int = 4
     return out
 
 
-def run_batched(batches: int = 3, seed: int = 3, tune: bool = True, groups: bool = True) -> dict:
+def run_batched(batches: int = 3, seed: int = 3, tune: bool = True, groups: bool = True,
+                batch: int = 0) -> dict:
     """The out-of-core batched workload: a 2048x2048x1024 uint16 volume
     (device-generated fBm + layered slabs, copied to host memory) streamed
     into a 64 GB device brick pool (128^3 bricks + 2-voxel apron, 5 LODs);
@@ -238,6 +239,8 @@ def run_batched(batches: int = 3, seed: int = 3, tune: bool = True, groups: bool
     t0 = time.perf_counter()
     r = workloads.batched_renderer(vol, max_grad, 64 * 10**9)
     r.setTFGrouping(groups)
+    if batch:  # samples per gather batch of the untuned launch (0: the default)
+        r.setKernelTuning(batch, 4, 0)
     plan_s = time.perf_counter() - t0
     views = workloads.batched_views(dims)
     t0 = time.perf_counter()
@@ -723,6 +726,8 @@ def main() -> int:
     ap.add_argument("--learned-lod-size", type=int, default=4096)
     ap.add_argument("--staging", type=int, default=0,
                     help="with --no-autotune: 1 marches the 1D TF over LDS-staged cells")
+    ap.add_argument("--batched-batch", type=int, default=0,
+                    help="batched workload: samples per gather batch to start from (0: default)")
     ap.add_argument("--no-tf-groups", action="store_true",
                     help="batched workload: render each of a camera's TF views on its own")
     ap.add_argument("--mlp-only", action="store_true",
@@ -741,7 +746,8 @@ def main() -> int:
                                      tune_size=tuple(args.serving_tune_size))))
         return 0
     if args.batched_only:
-        print(json.dumps(run_batched(tune=not args.no_autotune, groups=not args.no_tf_groups)))
+        print(json.dumps(run_batched(tune=not args.no_autotune, groups=not args.no_tf_groups,
+                                     batch=args.batched_batch)))
         return 0
     if args.learned_lod_only:
         print(json.dumps(run_learned_lod(args.learned_lod_size)))

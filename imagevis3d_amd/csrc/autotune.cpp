@@ -87,13 +87,14 @@ int HipRenderer::Autotune(const tvk_view* views, uint32_t n, uint32_t flags, tvk
   // then the launch shape again for the chosen bricks: tuning the bricks
   // under the starting launch shape can settle on bricks that are only best
   // for that shape.
-  // samples per gather batch: 2, 4 or 8 for the 1D march, 1 or 2 for the
-  // gradient marches (2D TF, or 1D TF with lighting); the other modes'
+  // samples per gather batch: 2, 4 or 8 for the 1D march; 1 or 2 for the
+  // gradient marches (2D TF, or 1D TF with lighting), 4 = TF groups
+  // gathering runs of 4 samples per ray across lanes; the other modes'
   // batch is fixed
   const int mode = job_mode();
   const bool grad = mode == 1 || (mode == 0 && lighting_);
   const std::vector<uint32_t> batches =
-      grad ? std::vector<uint32_t>{1u, 2u}
+      grad ? std::vector<uint32_t>{1u, 2u, 4u}
            : (mode == 0 && !regions_active() ? std::vector<uint32_t>{2u, 4u, 8u} : std::vector<uint32_t>{});
   auto tune_launch = [&]() -> int {
     if (int e = try_values(batches, &Setting::batch, false)) return e;

@@ -1508,6 +1508,220 @@ __global__ __launch_bounds__(256) void upscale_nearest_kernel(const uint32_t* __
   }
 }
 
+// Gradient march of a TF group in runs (kModeGradGroup at a gather batch of
+// kRun): the group march above, with the samples gathered by lanes other
+// than the ray's.  Rays about 2 voxels apart (1080p over a 2048-voxel
+// volume) touch a cache line per lane in every voxel gather; here each
+// lane walks its own ray to its next unit with samples to gather, as
+// render_tile_group does, and then the wave gathers in 4 passes: in pass
+// p, lane l takes sample (l & 3) of the run of ray 16p + (l >> 2) -- kRun
+// consecutive samples, 2/3 of a voxel apart, of one ray per 4 lanes, which
+// share their lines -- looks up every member's TF, shades, and leaves
+// (colour, opacity-corrected alpha) per member in the wave's LDS slab;
+// the ray's own lane then composites its run member by member in sample
+// order.  Sample positions, values, lookups and the compositing arithmetic
+// are those of render_tile_group, and a member's samples and termination
+// are its own, so images and counts are the same bit for bit.  Every lane
+// takes part in the passes, rays finished or not; the loop ends when no
+// lane has a run left (a wave-wide ballot).
+constexpr int kRun = 4;
+template <typename T, bool LINEAR>
+__device__ inline void render_tile_group_runs(const JobTable* __restrict__ tab, const RaycastParams& P,
+                                              int tile, int lane, uint32_t (&n_eval)[kGroupMax],
+                                              uint32_t& n_walk, uint32_t& n_nom, float4* __restrict__ slab) {
+  Ray r = {};
+  const bool mine = setup_ray(P, tile, lane, r, n_nom);
+  const int F = P.group_n > 1 ? P.group_n : 1;
+  const RaycastParams* Q[kGroupMax];
+  Q[0] = &P;
+#pragma unroll
+  for (int f = 1; f < kGroupMax; ++f) Q[f] = f < F ? &tab->jobs[P.group_jobs[f - 1]] : &P;
+  float C[kGroupMax][3], A[kGroupMax];
+#pragma unroll
+  for (int f = 0; f < kGroupMax; ++f) C[f][0] = C[f][1] = C[f][2] = A[f] = 0.f;
+  const float step = P.step;
+  const float t0 = r.t0, L = r.t1 - r.t0;
+  const int n = (mine && r.hit && L > 0.f) ? (int)ceilf(L / step) : 0;
+  const float lastlen = n > 0 ? fmaxf(L - (float)(n - 1) * step, 0.f) : 0.f;
+  const LodInfo& lv = P.level;
+  const int U = P.cell;
+  const int per_brick = P.brick / U;
+  const float invU = 1.f / (float)U;
+  float od[3], oo[3], ul0[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    od[i] = r.d[i] * lv.to_level[i];
+    oo[i] = r.o[i] * lv.to_level[i];
+    ul0[i] = (float)U / lv.to_level[i];
+  }
+  const int ucount[3] = {(lv.dims[0] + U - 1) / U, (lv.dims[1] + U - 1) / U, (lv.dims[2] + U - 1) / U};
+  const SlotGeo G = slot_geo(P.slot_edge);
+  uint32_t live = n > 0 ? (1u << F) - 1 : 0u;  // members still compositing
+  uint32_t need = 0;                           // live members whose TF keeps the current unit
+  int k = 0, kend = 0, blend_gid = -1;
+  float g0[3] = {0.f, 0.f, 0.f};
+  const T* __restrict__ base = nullptr;
+  while (true) {
+    // this lane's ray: on to the next unit with samples to gather
+    bool ready = false;
+    while (live && k < n) {
+      if (k < kend) {
+        ready = true;
+        break;
+      }
+      // the march unit of sample k, as in render_tile_group
+      const float len0 = (k == n - 1) ? lastlen : step;
+      const float tk = fmaf((float)k, step, t0) + 0.5f * len0;
+      int c[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        c[i] = clampi((int)floorf(fmaf(tk, od[i], oo[i]) * invU), 0, ucount[i] - 1);
+      float tb = kInf;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        if (r.d[i] != 0.f) {
+          const float face = (float)(r.d[i] > 0.f ? c[i] + 1 : c[i]) * ul0[i];
+          tb = fminf(tb, (face - r.o[i]) * r.inv[i]);
+        }
+      }
+      kend = min(max((int)ceilf((tb - t0) / step - 0.5f), k + 1), n);
+      int b[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) b[i] = c[i] / per_brick;
+      const int gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
+      const int64_t cidx = lv.cell_base + (c[2] * ucount[1] + c[1]) * ucount[0] + c[0];
+      need = 0;
+#pragma unroll
+      for (int f = 0; f < kGroupMax; ++f)
+        if (f < F && (live >> f & 1u) && !(P.skip && gload(Q[f]->cell_empty + cidx))) need |= 1u << f;
+      base = brick_base<T>(P, gid);
+      if (P.blend_bits < 32 && gid != blend_gid) {
+#pragma unroll
+        for (int f = 0; f < kGroupMax; ++f)
+          if (live >> f & 1u) blend_round(P.blend_bits, C[f][0], C[f][1], C[f][2], A[f]);
+        blend_gid = gid;
+      }
+      if (!need) {
+        k = kend;
+        continue;
+      }
+      if (!base) {
+        if (P.requests) P.requests[gid] = 1u;
+        k = kend;
+        continue;
+      }
+      if (P.touch) P.touch[gid] = P.frame;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        g0[i] = oo[i] - (float)(b[i] * P.brick - P.apron) - (LINEAR ? 0.5f : 0.f);
+    }
+    if (__ballot(ready) == 0) break;  // wave-uniform: every lane reaches it
+    const int cnt = ready ? min(kRun, kend - k) : 0;
+    const uint32_t run_need = ready ? need : 0u;
+    const int ib0 = (int)(uint32_t)(uint64_t)base, ib1 = (int)(uint32_t)((uint64_t)base >> 32);
+    for (int p = 0; p < 64 / 16; ++p) {
+      // this lane gathers sample q of the run of ray `src`
+      const int src = 16 * p + (lane >> 2), q = lane & 3;
+      const int s_cnt = __shfl(cnt, src, 64);
+      const uint32_t s_need = (uint32_t)__shfl((int)run_need, src, 64);
+      const int s_k = __shfl(k, src, 64), s_n = __shfl(n, src, 64);
+      const float s_t0 = __shfl(t0, src, 64), s_last = __shfl(lastlen, src, 64);
+      float s_od[3], s_g0[3], s_d[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        s_od[i] = __shfl(od[i], src, 64);
+        s_g0[i] = __shfl(g0[i], src, 64);
+        s_d[i] = __shfl(r.d[i], src, 64);
+      }
+      const uint64_t sb = (uint64_t)(uint32_t)__shfl(ib0, src, 64) | (uint64_t)(uint32_t)__shfl(ib1, src, 64) << 32;
+      if (q < s_cnt) {
+        const T* __restrict__ sbase = (const T*)sb;
+        const int kk = s_k + q;
+        const float len = (kk == s_n - 1) ? s_last : step;
+        const float t = fmaf((float)kk, step, s_t0) + 0.5f * len;
+        const float gx = fmaf(t, s_od[0], s_g0[0]), gy = fmaf(t, s_od[1], s_g0[1]), gz = fmaf(t, s_od[2], s_g0[2]);
+        float v, dd[3];
+        if (LINEAR) {
+          fetch_value_grad<T>(sbase, G, gx, gy, gz, v, dd);
+        } else {
+          v = fetch<T, false>(sbase, G, gx, gy, gz);
+          const float lx = gx - 0.5f, ly = gy - 0.5f, lz = gz - 0.5f;
+          dd[0] = fetch<T, true>(sbase, G, lx + 1.f, ly, lz) - fetch<T, true>(sbase, G, lx - 1.f, ly, lz);
+          dd[1] = fetch<T, true>(sbase, G, lx, ly + 1.f, lz) - fetch<T, true>(sbase, G, lx, ly - 1.f, lz);
+          dd[2] = fetch<T, true>(sbase, G, lx, ly, lz + 1.f) - fetch<T, true>(sbase, G, lx, ly, lz - 1.f);
+        }
+        float g[3] = {dd[0] * P.grad_scale[0], dd[1] * P.grad_scale[1], dd[2] * P.grad_scale[2]};
+        const float gm = sqrtf(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+        // the source ray's light direction and half vector, as its own march sets them up
+        float Lh[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        {
+          float h[3], hl = 0.f;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            Lh[i] = P.light[i];
+            h[i] = P.light[i] - s_d[i];
+            hl += h[i] * h[i];
+          }
+          hl = sqrtf(hl);
+          if (hl > 1e-12f)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) Lh[3 + i] = h[i] / hl;
+        }
+#pragma unroll
+        for (int f = 0; f < kGroupMax; ++f) {
+          if (!(s_need >> f & 1u)) continue;
+          float cs[4];
+          if (P.mode == 1) tf2d_lookup(*Q[f], v, gm, cs);
+          else tf1d_lookup<false>(*Q[f], (const float4*)Q[f]->tf, v, cs);
+          float ac = 0.f;  // compositing with 0 leaves (C, A) as skipping the sample does
+          if (cs[3] > 0.f) {
+            if (P.lighting) shade(P, g, Lh, cs);
+            ac = cs[3] >= 1.f ? 1.f : 1.f - exp2f(len * log2f(1.f - cs[3]));
+          }
+          slab[f * 64 + lane] = make_float4(cs[0], cs[1], cs[2], ac);
+        }
+      }
+      wave_lds_sync();
+      if ((lane >> 4) == p && cnt > 0) {
+        // this lane's ray was gathered in this pass by lanes 4 (lane - 16p) + q
+        const int w0 = 4 * (lane - 16 * p);
+        int most = 0;  // samples of the run some member evaluates
+#pragma unroll
+        for (int f = 0; f < kGroupMax; ++f) {
+          if (!(run_need >> f & 1u)) continue;
+          int e = 0;
+          for (int j = 0; j < cnt && (live >> f & 1u); ++j) {
+            const float4 x = slab[f * 64 + w0 + j];
+            ++e;
+            const float w = (1.f - A[f]) * x.w;
+            C[f][0] = fmaf(w, x.x, C[f][0]);
+            C[f][1] = fmaf(w, x.y, C[f][1]);
+            C[f][2] = fmaf(w, x.z, C[f][2]);
+            A[f] += w;
+            if (!(A[f] < P.term_alpha)) {
+              live &= ~(1u << f);
+              need &= ~(1u << f);
+            }
+          }
+          n_eval[f] += (uint32_t)e;
+          most = max(most, e);
+        }
+        n_walk += (uint32_t)most;
+      }
+      wave_lds_sync();  // the slab is the next pass's
+    }
+    if (ready) k = live ? (need ? k + cnt : kend) : n;
+  }
+  if (n > 0 && P.blend_bits < 32)
+#pragma unroll
+    for (int f = 0; f < kGroupMax; ++f) blend_round(P.blend_bits, C[f][0], C[f][1], C[f][2], A[f]);
+  if (!mine) return;
+  const int64_t pix = (int64_t)r.py * P.width + r.px;
+#pragma unroll
+  for (int f = 0; f < kGroupMax; ++f)
+    if (f < F) gstore(Q[f]->framebuffer + pix, over_background(P, r.py, C[f][0], C[f][1], C[f][2], A[f]));
+}
+
 // Persistent raycast over a job table: every wave loops over ray tiles
 // taken from the per-XCD queues until all queues are drained.  Tile t of
 // the launch belongs to job j with tile_start[j] <= t < tile_start[j + 1].
@@ -1613,6 +1827,11 @@ __global__ __launch_bounds__(kThreads, 2) void raycast_kernel(const JobTable* __
         // this wave's staging buffer follows the LDS TF
         T* sbuf = (T*)(s_tf + (TF_LDS ? H.n_tf_lds : 0)) + (threadIdx.x >> 6) * kStageElems;
         render_tile_staged<T, TF_LDS>(P, tf4, tile - tab->tile_start[job], lane, n_eval, n_nom, sbuf);
+      } else if constexpr (MODE == kModeGradGroup && BATCH == kRun) {
+        // this wave's slab follows the work-group's counters in LDS
+        float4* slab = s_tf + (threadIdx.x >> 6) * 64 * kGroupMax;
+        render_tile_group_runs<T, LINEAR>(tab, P, tile - tab->tile_start[job], lane, n_evalg, n_walk, n_nom,
+                                          slab);
       } else if constexpr (MODE == kModeGradGroup) {
         render_tile_group<T, LINEAR, BATCH>(tab, P, tile - tab->tile_start[job], lane, n_evalg, n_walk, n_nom);
       } else {
@@ -1640,6 +1859,7 @@ hipError_t launch_one(const LaunchInfo& li, const JobTable* tab, hipStream_t str
   // in flight thrash the caches (the autotuner picks).
   const int kT = TF_LDS ? kThreads : 64 * (li.wg_waves == 4 ? 4 : 8);
   if (MODE == kMode1DStaged) lds += (size_t)(kT / 64) * kStageElems * sizeof(T);  // per-wave cell buffers
+  if (MODE == kModeGradGroup && BATCH == kRun) lds += (size_t)(kT / 64) * 64 * kGroupMax * sizeof(float4);
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kT, lds);
   if (e != hipSuccess) return e;
   if (per_cu < 1) per_cu = 1;
@@ -1667,9 +1887,12 @@ hipError_t launch_typed(const LaunchInfo& li, const JobTable* tab, hipStream_t s
   if (li.mode == 3)
     return li.n_tf_lds > 0 ? launch_one<T, LINEAR, 4, true, kModeMIP>(li, tab, stream)
                            : launch_one<T, LINEAR, 4, false, kModeMIP>(li, tab, stream);
-  // the gradient marches gather 2 samples per batch when li.batch is 2, else 1
-  if (li.mode == 1 || li.lighting)
+  // the gradient marches gather 2 samples per batch when li.batch is 2, else
+  // 1; TF groups at li.batch kRun gather runs of kRun samples per ray
+  if (li.mode == 1 || li.lighting) {
+    if (li.group && li.batch == kRun) return launch_one<T, LINEAR, kRun, false, kModeGradGroup>(li, tab, stream);
     return li.batch == 2 ? launch_grad<T, LINEAR, 2>(li, tab, stream) : launch_grad<T, LINEAR, 1>(li, tab, stream);
+  }
   const bool lds = li.n_tf_lds > 0;
   if (li.stage && LINEAR && !li.lighting)
     return lds ? launch_one<T, true, 1, true, kMode1DStaged>(li, tab, stream)

@@ -81,6 +81,14 @@ def test_groups_2d_phong_equal_separate_views():
     for i in range(len(views)):
         assert np.array_equal(got1[i], got[i]), f"batch 1: view {i}"
         assert st1[i].evaluated_samples == st[i].evaluated_samples
+    # runs of 4 samples per ray gathered across lanes (grouped), against
+    # the march alone
+    r.setKernelTuning(4, 4, 0)
+    got4, st4 = grouped_vs_alone(r, views, "2D Phong, runs of 4")
+    for i in range(len(views)):
+        assert np.array_equal(got4[i], got[i]), f"runs of 4: view {i}"
+        assert st4[i].evaluated_samples == st[i].evaluated_samples
+        assert st4[i].gathered_samples == st[i].gathered_samples
     r.setKernelTuning(8, 4, 0)
     # the group's images equal single captures too
     r.resize((120, 90))
@@ -97,10 +105,10 @@ def test_groups_2d_phong_equal_separate_views():
     r.cleanup()
 
 
-@pytest.mark.parametrize("interp", ["linear", "nearest"])
-def test_groups_1d_lighting_equal_separate_views(interp):
+@pytest.mark.parametrize("interp,batch", [("linear", 8), ("nearest", 8), ("linear", 4), ("nearest", 4)])
+def test_groups_1d_lighting_equal_separate_views(interp, batch):
     vol = volume(80)
-    print(f"case TF groups: 1D TF + lighting, {interp}", flush=True)
+    print(f"case TF groups: 1D TF + lighting, {interp}, batch {batch}", flush=True)
     tfs = [transfer.gaussian_bumps_tf(4096, seed=s) for s in (3, 5)]
     tfs.append(transfer.ramp_tf(4096, 500.0, 9000.0))  # dense: rays end early
     r = gpu_renderer(width=96, height=72)
@@ -111,8 +119,9 @@ def test_groups_1d_lighting_equal_separate_views(interp):
         r.setInterpolant(Interpolant.NearestNeighbor)
     for s, t in enumerate(tfs):
         r.set1DTransSlot(s, t)
+    r.setKernelTuning(batch, 4, 0)
     views = [view(c, (96, 72), s) for c in CAMS for s in (0, 1, 2)]
-    grouped_vs_alone(r, views, f"1D lighting {interp}")
+    grouped_vs_alone(r, views, f"1D lighting {interp} batch {batch}")
     r.cleanup()
 
 
@@ -158,6 +167,7 @@ def test_groups_on_paged_pool():
         for s, t in enumerate(lobes):
             r.set2DTransSlot(s, t, mg)
     views = [view(c, (112, 80), s) for c in (CAMS[0], w.cameras[9]) for s in range(3)]
+    pg.setKernelTuning(4, 4, 0)  # grouped: runs of 4 samples per ray across lanes
     a, sa = grouped_vs_alone(pg, views, "paged")
     b, sb_, _ = ic.renderBatch(views)
     for i in range(len(views)):

@@ -87,10 +87,10 @@ int HipRenderer::Autotune(const tvk_view* views, uint32_t n, uint32_t flags, tvk
   // then the launch shape again for the chosen bricks: tuning the bricks
   // under the starting launch shape can settle on bricks that are only best
   // for that shape.
-  // samples per gather batch: 2, 4 or 8 for the 1D march; 1 or 2 for the
-  // gradient marches (2D TF, or 1D TF with lighting), 4 = TF groups
-  // gathering runs of 4 samples per ray across lanes; the other modes'
-  // batch is fixed
+  // samples per gather batch: 2, 4 or 8 for the 1D march; for the gradient
+  // marches (2D TF, or 1D TF with lighting) 1 or 2 per lane, or 4 = runs of
+  // 4 samples per ray gathered across lanes; the other modes' batch is
+  // fixed
   const int mode = job_mode();
   const bool grad = mode == 1 || (mode == 0 && lighting_);
   const std::vector<uint32_t> batches =

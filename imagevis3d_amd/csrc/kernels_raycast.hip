@@ -1887,11 +1887,14 @@ hipError_t launch_typed(const LaunchInfo& li, const JobTable* tab, hipStream_t s
   if (li.mode == 3)
     return li.n_tf_lds > 0 ? launch_one<T, LINEAR, 4, true, kModeMIP>(li, tab, stream)
                            : launch_one<T, LINEAR, 4, false, kModeMIP>(li, tab, stream);
-  // the gradient marches gather 2 samples per batch when li.batch is 2, else
-  // 1; TF groups at li.batch kRun gather runs of kRun samples per ray
+  // the gradient marches: li.batch 1 or 2 samples per gather batch, each
+  // lane its own ray; otherwise runs of kRun samples per ray gathered
+  // across lanes (the TF-group march, which takes lone views as groups of
+  // one)
   if (li.mode == 1 || li.lighting) {
-    if (li.group && li.batch == kRun) return launch_one<T, LINEAR, kRun, false, kModeGradGroup>(li, tab, stream);
-    return li.batch == 2 ? launch_grad<T, LINEAR, 2>(li, tab, stream) : launch_grad<T, LINEAR, 1>(li, tab, stream);
+    if (li.batch == 1) return launch_grad<T, LINEAR, 1>(li, tab, stream);
+    if (li.batch == 2) return launch_grad<T, LINEAR, 2>(li, tab, stream);
+    return launch_one<T, LINEAR, kRun, false, kModeGradGroup>(li, tab, stream);
   }
   const bool lds = li.n_tf_lds > 0;
   if (li.stage && LINEAR && !li.lighting)

@@ -425,9 +425,10 @@ int tvk_read_framebuffer(tvk_renderer* r, uint8_t* rgba, uint64_t rgba_bytes);
 
 /* Kernel launch tuning (no counterpart in the reference; chosen by the
  * autotuner): samples gathered per march step (1, 2, 4 or 8; default 8:
- * the 1D march takes 2, 4 or 8 (1 as 8), the gradient marches of the 2D TF
- * and of lighting 2 when it is 2 and 1 otherwise (with 4, TF groups gather
- * runs of 4 consecutive samples of a ray on 4 lanes), the other modes 4);
+ * the 1D march takes 2, 4 or 8 (1 as 8); the gradient marches of the 2D TF
+ * and of lighting take 1 or 2 samples per lane, and any other value
+ * gathers runs of 4 consecutive samples of a ray on 4 lanes (the default);
+ * the other modes take 4);
  * how 8x8 ray tiles are dealt to the 8 XCDs: xcd_split 0 deals runs of
  * tile_group (1..4096) consecutive tiles round-robin, xcd_split rx in
  * {1, 2, 4, 8} gives each XCD one region of an rx x (8/rx) split of the

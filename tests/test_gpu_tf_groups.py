@@ -105,7 +105,7 @@ def test_groups_2d_phong_equal_separate_views():
     r.cleanup()
 
 
-@pytest.mark.parametrize("interp,batch", [("linear", 8), ("nearest", 8), ("linear", 4), ("nearest", 4)])
+@pytest.mark.parametrize("interp,batch", [("linear", 1), ("nearest", 1), ("linear", 4), ("nearest", 4)])
 def test_groups_1d_lighting_equal_separate_views(interp, batch):
     vol = volume(80)
     print(f"case TF groups: 1D TF + lighting, {interp}, batch {batch}", flush=True)

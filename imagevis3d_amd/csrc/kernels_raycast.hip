@@ -1517,7 +1517,8 @@ __global__ __launch_bounds__(256) void upscale_nearest_kernel(const uint32_t* __
 // p, lane l takes sample (l & 3) of the run of ray 16p + (l >> 2) -- kRun
 // consecutive samples, 2/3 of a voxel apart, of one ray per 4 lanes, which
 // share their lines -- looks up every member's TF, shades, and leaves
-// (colour, opacity-corrected alpha) per member in the wave's LDS slab;
+// (colour, opacity-corrected alpha) per member in the wave's LDS slab
+// (kRun passes of 64 / kRun rays in general);
 // the ray's own lane then composites its run member by member in sample
 // order.  Sample positions, values, lookups and the compositing arithmetic
 // are those of render_tile_group, and a member's samples and termination
@@ -1619,9 +1620,10 @@ __device__ inline void render_tile_group_runs(const JobTable* __restrict__ tab, 
     const int cnt = ready ? min(kRun, kend - k) : 0;
     const uint32_t run_need = ready ? need : 0u;
     const int ib0 = (int)(uint32_t)(uint64_t)base, ib1 = (int)(uint32_t)((uint64_t)base >> 32);
-    for (int p = 0; p < 64 / 16; ++p) {
+    constexpr int kRays = 64 / kRun;  // rays whose runs a pass gathers
+    for (int p = 0; p < kRun; ++p) {
       // this lane gathers sample q of the run of ray `src`
-      const int src = 16 * p + (lane >> 2), q = lane & 3;
+      const int src = kRays * p + lane / kRun, q = lane % kRun;
       const int s_cnt = __shfl(cnt, src, 64);
       const uint32_t s_need = (uint32_t)__shfl((int)run_need, src, 64);
       const int s_k = __shfl(k, src, 64), s_n = __shfl(n, src, 64);
@@ -1682,9 +1684,9 @@ __device__ inline void render_tile_group_runs(const JobTable* __restrict__ tab, 
         }
       }
       wave_lds_sync();
-      if ((lane >> 4) == p && cnt > 0) {
-        // this lane's ray was gathered in this pass by lanes 4 (lane - 16p) + q
-        const int w0 = 4 * (lane - 16 * p);
+      if (lane / kRays == p && cnt > 0) {
+        // this lane's ray was gathered in this pass by lanes kRun (lane - kRays p) + q
+        const int w0 = kRun * (lane - kRays * p);
         int most = 0;  // samples of the run some member evaluates
 #pragma unroll
         for (int f = 0; f < kGroupMax; ++f) {

@@ -971,6 +971,62 @@ __device__ inline void render_tile(const RaycastParams& P, const float4* __restr
   gstore(P.framebuffer + ((int64_t)py * P.width + px), over_background(P, py, C0, C1, C2, A));
 }
 
+// The march unit of sample k on a TF group's ray (render_tile_group and
+// render_tile_group_runs), as render_tile's walk finds it: its end kend,
+// the live members whose TF keeps it (need), its brick slot `base` and the
+// slot-local texel origin g0.  Rounds each live member's blend when the
+// brick changes (a member's own march stops rounding once it has
+// terminated), asks for a non-resident brick and touches a resident one
+// that some member needs.  Returns whether the unit has samples to gather.
+template <typename T, bool LINEAR>
+__device__ inline bool group_unit(const RaycastParams& P, const RaycastParams* const (&Q)[kGroupMax], int F,
+                                  const Ray& r, const float (&od)[3], const float (&oo)[3],
+                                  const float (&ul0)[3], const int (&ucount)[3], float invU, int per_brick,
+                                  int k, int n, float t0, float step, float lastlen, uint32_t live, int& kend,
+                                  uint32_t& need, const T* __restrict__& base, float (&g0)[3], int& blend_gid,
+                                  float (&C)[kGroupMax][3], float (&A)[kGroupMax]) {
+  const LodInfo& lv = P.level;
+  const float len0 = (k == n - 1) ? lastlen : step;
+  const float tk = fmaf((float)k, step, t0) + 0.5f * len0;
+  int c[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) c[i] = clampi((int)floorf(fmaf(tk, od[i], oo[i]) * invU), 0, ucount[i] - 1);
+  float tb = kInf;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (r.d[i] != 0.f) {
+      const float face = (float)(r.d[i] > 0.f ? c[i] + 1 : c[i]) * ul0[i];
+      tb = fminf(tb, (face - r.o[i]) * r.inv[i]);
+    }
+  }
+  kend = min(max((int)ceilf((tb - t0) / step - 0.5f), k + 1), n);
+  int b[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) b[i] = c[i] / per_brick;
+  const int gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
+  const int64_t cidx = lv.cell_base + (c[2] * ucount[1] + c[1]) * ucount[0] + c[0];
+  need = 0;
+#pragma unroll
+  for (int f = 0; f < kGroupMax; ++f)
+    if (f < F && (live >> f & 1u) && !(P.skip && gload(Q[f]->cell_empty + cidx))) need |= 1u << f;
+  base = brick_base<T>(P, gid);
+  if (P.blend_bits < 32 && gid != blend_gid) {
+#pragma unroll
+    for (int f = 0; f < kGroupMax; ++f)
+      if (live >> f & 1u) blend_round(P.blend_bits, C[f][0], C[f][1], C[f][2], A[f]);
+    blend_gid = gid;
+  }
+  if (!need) return false;
+  if (!base) {
+    if (P.requests) P.requests[gid] = 1u;
+    return false;
+  }
+  if (P.touch) P.touch[gid] = P.frame;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) g0[i] = oo[i] - (float)(b[i] * P.brick - P.apron) - (LINEAR ? 0.5f : 0.f);
+  return true;
+}
+
 // Gradient march of a TF group (kModeGradGroup): jobs that differ only in
 // their TF (same camera, image, volume, level, rate and shading) walk each
 // ray once.  Every sample's value and gradient are gathered once and each
@@ -1036,53 +1092,10 @@ __device__ inline void render_tile_group(const JobTable* __restrict__ tab, const
     float g0[3] = {0.f, 0.f, 0.f};
     const T* __restrict__ base = nullptr;
     while (k < n) {
-      if (k >= kend) {
-        // the march unit of sample k, as in render_tile's walk
-        const float len0 = (k == n - 1) ? lastlen : step;
-        const float tk = fmaf((float)k, step, t0) + 0.5f * len0;
-        int c[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-          c[i] = clampi((int)floorf(fmaf(tk, od[i], oo[i]) * invU), 0, ucount[i] - 1);
-        float tb = kInf;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          if (r.d[i] != 0.f) {
-            const float face = (float)(r.d[i] > 0.f ? c[i] + 1 : c[i]) * ul0[i];
-            tb = fminf(tb, (face - r.o[i]) * r.inv[i]);
-          }
-        }
-        kend = min(max((int)ceilf((tb - t0) / step - 0.5f), k + 1), n);
-        int b[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) b[i] = c[i] / per_brick;
-        const int gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
-        const int64_t cidx = lv.cell_base + (c[2] * ucount[1] + c[1]) * ucount[0] + c[0];
-        need = 0;
-#pragma unroll
-        for (int f = 0; f < kGroupMax; ++f)
-          if (f < F && (live >> f & 1u) && !(P.skip && gload(Q[f]->cell_empty + cidx))) need |= 1u << f;
-        base = brick_base<T>(P, gid);
-        if (P.blend_bits < 32 && gid != blend_gid) {
-          // a member's own march stops rounding once it has terminated
-#pragma unroll
-          for (int f = 0; f < kGroupMax; ++f)
-            if (live >> f & 1u) blend_round(P.blend_bits, C[f][0], C[f][1], C[f][2], A[f]);
-          blend_gid = gid;
-        }
-        if (!need) {
-          k = kend;
-          continue;
-        }
-        if (!base) {
-          if (P.requests) P.requests[gid] = 1u;
-          k = kend;
-          continue;
-        }
-        if (P.touch) P.touch[gid] = P.frame;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-          g0[i] = oo[i] - (float)(b[i] * P.brick - P.apron) - (LINEAR ? 0.5f : 0.f);
+      if (k >= kend && !group_unit<T, LINEAR>(P, Q, F, r, od, oo, ul0, ucount, invU, per_brick, k, n, t0, step,
+                                              lastlen, live, kend, need, base, g0, blend_gid, C, A)) {
+        k = kend;
+        continue;
       }
       float v[BATCH], len[BATCH], g[BATCH][3];
 #pragma unroll
@@ -1570,51 +1583,9 @@ __device__ inline void render_tile_group_runs(const JobTable* __restrict__ tab, 
         ready = true;
         break;
       }
-      // the march unit of sample k, as in render_tile_group
-      const float len0 = (k == n - 1) ? lastlen : step;
-      const float tk = fmaf((float)k, step, t0) + 0.5f * len0;
-      int c[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-        c[i] = clampi((int)floorf(fmaf(tk, od[i], oo[i]) * invU), 0, ucount[i] - 1);
-      float tb = kInf;
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        if (r.d[i] != 0.f) {
-          const float face = (float)(r.d[i] > 0.f ? c[i] + 1 : c[i]) * ul0[i];
-          tb = fminf(tb, (face - r.o[i]) * r.inv[i]);
-        }
-      }
-      kend = min(max((int)ceilf((tb - t0) / step - 0.5f), k + 1), n);
-      int b[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) b[i] = c[i] / per_brick;
-      const int gid = lv.brick_base + (b[2] * lv.grid[1] + b[1]) * lv.grid[0] + b[0];
-      const int64_t cidx = lv.cell_base + (c[2] * ucount[1] + c[1]) * ucount[0] + c[0];
-      need = 0;
-#pragma unroll
-      for (int f = 0; f < kGroupMax; ++f)
-        if (f < F && (live >> f & 1u) && !(P.skip && gload(Q[f]->cell_empty + cidx))) need |= 1u << f;
-      base = brick_base<T>(P, gid);
-      if (P.blend_bits < 32 && gid != blend_gid) {
-#pragma unroll
-        for (int f = 0; f < kGroupMax; ++f)
-          if (live >> f & 1u) blend_round(P.blend_bits, C[f][0], C[f][1], C[f][2], A[f]);
-        blend_gid = gid;
-      }
-      if (!need) {
+      if (!group_unit<T, LINEAR>(P, Q, F, r, od, oo, ul0, ucount, invU, per_brick, k, n, t0, step, lastlen, live,
+                                 kend, need, base, g0, blend_gid, C, A))
         k = kend;
-        continue;
-      }
-      if (!base) {
-        if (P.requests) P.requests[gid] = 1u;
-        k = kend;
-        continue;
-      }
-      if (P.touch) P.touch[gid] = P.frame;
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-        g0[i] = oo[i] - (float)(b[i] * P.brick - P.apron) - (LINEAR ? 0.5f : 0.f);
     }
     if (__ballot(ready) == 0) break;  // wave-uniform: every lane reaches it
     const int cnt = ready ? min(kRun, kend - k) : 0;

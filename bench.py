@@ -98,7 +98,7 @@ def run_interactive(r, w, views, tune=None, warmup: int = 4) -> dict:
 
 
 def run_serving(duration_s: float, rate_hz: float, seed: int = 5, lanes: int = 4,
-                tune_size=(512, 512)) -> dict:
+                tune_size=(512, 512), part_cost: int = 0) -> dict:
     """The serving workload: seeded Poisson arrivals at `rate_hz` for
     `duration_s` over a pool of 7 device-generated volumes (64^3, 256^3 and
     512^3 in uint8 and uint16, one 2048^3 uint16), each request a random
@@ -128,6 +128,7 @@ def run_serving(duration_s: float, rate_hz: float, seed: int = 5, lanes: int = 4
                       "work_group_waves": t.wg_waves, "brick_staging": t.brick_staging})
     tune_s = time.perf_counter() - t0
     srv = RenderServer(arena_bytes=4 << 30, lanes=lanes)
+    srv.setPartCost(part_cost)  # heavy requests render in parts (0: the default cost)
     ids = [srv.addVolume(r) for r in rs]
     # warm every kernel variant and graph once (not timed)
     for v in range(len(ids)):
@@ -195,6 +196,7 @@ def run_serving(duration_s: float, rate_hz: float, seed: int = 5, lanes: int = 4
         "mean_queue_ms": float(np.mean([r.queue_ms for r in results])),
         "gsamples_per_s": nominal / wall / 1e9, "launches": int(st.launches),
         "max_jobs_per_launch": int(st.max_jobs_per_launch), "max_lanes_busy": int(st.max_lanes_busy),
+        "split_requests": int(st.split_requests), "parts_launched": int(st.parts_launched),
         "volume_setup_s": round(setup_s, 2), "launch_tuning_s": round(tune_s, 2),
         "launch_tuning": tuned,
         # the tail by class: the heaviest class (4K on the 2048^3 volume)
@@ -719,6 +721,9 @@ def main() -> int:
     ap.add_argument("--serving-lanes", type=int, default=4, help="streams of the render server")
     ap.add_argument("--serving-tune-size", type=int, nargs=2, default=(512, 512),
                     help="image size of the view each serving volume's launch shape is tuned on")
+    ap.add_argument("--serving-part-cost", type=float, default=0,
+                    help="estimated samples above which a request renders in parts (0: the server's "
+                         "default; 1.8e19: never split)")
     ap.add_argument("--no-batched", action="store_true", help="skip the out-of-core batched workload")
     ap.add_argument("--batched-only", action="store_true")
     ap.add_argument("--no-learned-lod", action="store_true", help="skip the learned-LOD workload")
@@ -743,7 +748,8 @@ def main() -> int:
 
     if args.serving_only:
         print(json.dumps(run_serving(args.serving_seconds, args.serving_rate, lanes=args.serving_lanes,
-                                     tune_size=tuple(args.serving_tune_size))))
+                                     tune_size=tuple(args.serving_tune_size),
+                                     part_cost=min(int(args.serving_part_cost), 2**64 - 1))))
         return 0
     if args.batched_only:
         print(json.dumps(run_batched(tune=not args.no_autotune, groups=not args.no_tf_groups,
@@ -878,7 +884,8 @@ def main() -> int:
         print(f"[bench] serving workload ({args.serving_seconds:g} s at {args.serving_rate:g} req/s)",
               file=sys.stderr, flush=True)
         sv = run_serving(args.serving_seconds, args.serving_rate, lanes=args.serving_lanes,
-                         tune_size=tuple(args.serving_tune_size))
+                         tune_size=tuple(args.serving_tune_size),
+                         part_cost=min(int(args.serving_part_cost), 2**64 - 1))
         out["serving"] = sv
         out["p99_ms"] = sv["p99_ms"]
         print(f"[bench] serving {sv}", file=sys.stderr, flush=True)

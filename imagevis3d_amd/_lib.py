@@ -115,6 +115,7 @@ class RequestResult(C.Structure):
         ("status", C.c_int32), ("launch_jobs", C.c_uint32),
         ("evaluated_samples", C.c_uint64), ("nominal_samples", C.c_uint64),
         ("queue_ms", C.c_double), ("latency_ms", C.c_double),
+        ("parts", C.c_uint32), ("reserved_", C.c_uint32),
     ]
 
 
@@ -124,6 +125,7 @@ class ServerStats(C.Structure):
         ("max_jobs_per_launch", C.c_uint64), ("max_lanes_busy", C.c_uint64),
         ("nominal_samples", C.c_uint64), ("paged_launches", C.c_uint64),
         ("pool_waits", C.c_uint64), ("max_pool_reserved", C.c_uint64),
+        ("split_requests", C.c_uint64), ("parts_launched", C.c_uint64),
     ]
 
 
@@ -244,6 +246,7 @@ SIGNATURES = {
     "tvk_server_submit": [C.c_void_p, C.POINTER(Request), C.POINTER(u64)],
     "tvk_server_wait": [C.c_void_p, u64, u32, C.POINTER(RequestResult)],
     "tvk_server_get_stats": [C.c_void_p, C.POINTER(ServerStats)],
+    "tvk_server_set_part_cost": [C.c_void_p, u64],
     "tvk_paint": [H],
     "tvk_check_for_redraw": [H],
     "tvk_schedule_complete_redraw": [H],

@@ -1476,6 +1476,7 @@ __global__ __launch_bounds__(256) void background_kernel(const JobTable* __restr
   const int n_jobs = tab->h.n_jobs;
   for (int job = 0; job < n_jobs; ++job) {
     const RaycastParams& P = tab->jobs[job];
+    if (P.tile_first > 0) continue;  // a later part of a split job
     const int64_t npix = (int64_t)P.width * P.height;
     const int th = 64 >> P.tile_shift;
     const int x0 = P.rect_x0 << P.tile_shift, x1 = (P.rect_x0 + P.rect_w) << P.tile_shift;
@@ -1796,19 +1797,21 @@ __global__ __launch_bounds__(kThreads, 2) void raycast_kernel(const JobTable* __
         cur_job = job;
         n_eval = n_nom = 0;
       }
+      // the job's own tile slot (a part's slots start at tile_first)
+      const int slot = tile - tab->tile_start[job] + P.tile_first;
       if constexpr (MODE == kMode1DStaged) {
         // this wave's staging buffer follows the LDS TF
         T* sbuf = (T*)(s_tf + (TF_LDS ? H.n_tf_lds : 0)) + (threadIdx.x >> 6) * kStageElems;
-        render_tile_staged<T, TF_LDS>(P, tf4, tile - tab->tile_start[job], lane, n_eval, n_nom, sbuf);
+        render_tile_staged<T, TF_LDS>(P, tf4, slot, lane, n_eval, n_nom, sbuf);
       } else if constexpr (MODE == kModeGradGroup && BATCH == kRun) {
         // this wave's slab follows the work-group's counters in LDS
         float4* slab = s_tf + (threadIdx.x >> 6) * 64 * kGroupMax;
-        render_tile_group_runs<T, LINEAR>(tab, P, tile - tab->tile_start[job], lane, n_evalg, n_walk, n_nom,
+        render_tile_group_runs<T, LINEAR>(tab, P, slot, lane, n_evalg, n_walk, n_nom,
                                           slab);
       } else if constexpr (MODE == kModeGradGroup) {
-        render_tile_group<T, LINEAR, BATCH>(tab, P, tile - tab->tile_start[job], lane, n_evalg, n_walk, n_nom);
+        render_tile_group<T, LINEAR, BATCH>(tab, P, slot, lane, n_evalg, n_walk, n_nom);
       } else {
-        render_tile<T, LINEAR, BATCH, MODE, TF_LDS>(P, tf4, tile - tab->tile_start[job], lane, n_eval, n_nom);
+        render_tile<T, LINEAR, BATCH, MODE, TF_LDS>(P, tf4, slot, lane, n_eval, n_nom);
       }
     }
   }

@@ -1599,9 +1599,10 @@ LaunchInfo HipRenderer::launch_info(int n_tf_lds) const {
 }
 
 // Tile slots of a job in the queues: its rect's tiles, or one per run of
-// tile_sample of them.
-static int job_tiles(const RaycastParams& p) {
+// tile_sample of them (a part: its own slots).
+int job_tiles(const RaycastParams& p) {
   if (p.group_n < 0) return 0;  // a TF group member: its leader's rays render it
+  if (p.tile_count > 0) return p.tile_count;  // a part of a split job
   const int all = p.rect_w * p.rect_h;
   return p.tile_sample <= 1 ? all : (all + p.tile_sample - 1) / p.tile_sample;
 }
@@ -1673,8 +1674,8 @@ static int fill_job_table(JobTable* t, const RaycastParams* jobs, int n, const L
     total += job_tiles(j);
     *culled = *culled || j.rect_w * j.rect_h != j.tiles_x * j.tiles_y;
     // the frame-region split addresses tiles by rect position: sampled
-    // jobs take the round-robin runs
-    if (j.tile_sample > 1) t->h.xcd_split = 0;
+    // jobs and parts take the round-robin runs
+    if (j.tile_sample > 1 || j.tile_count > 0) t->h.xcd_split = 0;
   }
   t->tile_start[n] = total;
   t->h.total_tiles = total;

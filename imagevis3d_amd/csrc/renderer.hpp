@@ -129,6 +129,11 @@ struct LaunchSlot {
   void release();
 };
 
+// Tile slots a job puts in the launch's queues: its rect's tiles, or one
+// per run of tile_sample of them; a part's tile_count; 0 for a TF group
+// member.
+int job_tiles(const RaycastParams& p);
+
 // Fill `sl`'s job table from jobs[0..n) and launch it on `stream` (graph
 // replay when `graph`; one graph per kernel variant, re-captured on change).
 int launch_job_table(LaunchSlot& sl, const RaycastParams* jobs, int n, const LaunchInfo& li,

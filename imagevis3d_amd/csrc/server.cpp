@@ -268,6 +268,12 @@ int RenderServer::Submit(const tvk_request& q, uint64_t* ticket) {
     req.cost = (uint64_t)probe.rect_w * (uint64_t)probe.rect_h * 64ull *
                (uint64_t)std::max(1.0f, longest / probe.step);
   }
+  req.tiles = job_tiles(probe);
+  if (req.cost > part_cost_ && req.tiles > 1) {
+    const uint64_t n = (req.cost + part_cost_ - 1) / part_cost_;
+    req.parts = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(n, kMaxParts), (uint64_t)req.tiles);
+    ++stats_.split_requests;
+  }
   req.bytes = (uint64_t)q.view.width * q.view.height * 4;
   if (req.bytes > dev_arena_.size()) return fail(TVK_ERR_OOM, "image larger than the output arena");
   req.t_submit = now_ms();
@@ -288,21 +294,31 @@ bool RenderServer::retire_locked() {
     const hipError_t e = hipEventQuery(L.done);
     if (e == hipErrorNotReady) continue;
     const double t = now_ms();
+    bool failed = false;
     for (size_t k = 0; k < L.tickets.size(); ++k) {
       Request& r = reqs_[L.tickets[k]];
-      r.done = true;
-      r.t_done = t;
+      ++r.parts_retired;
       if (e == hipSuccess) {
-        r.evaluated = L.cur->counters_h[kQueues + kJobCounters * k];
-        r.nominal = L.cur->counters_h[kQueues + kJobCounters * k + 1];
-        stats_.nominal_samples += r.nominal;
-      } else {
+        const uint64_t nominal = L.cur->counters_h[kQueues + kJobCounters * k + 1];
+        r.evaluated += L.cur->counters_h[kQueues + kJobCounters * k];
+        r.nominal += nominal;
+        stats_.nominal_samples += nominal;
+      } else if (r.status == TVK_OK) {
         r.status = TVK_ERR_HIP;
         r.error = std::string("launch failed: ") + hipGetErrorString(e);
+        failed = true;
       }
-      dev_arena_.free(r.dev_off, r.bytes);
-      ++stats_.completed;
+      // done after its last part (whose launch copied the image out), or
+      // at the first part that failed
+      if (r.parts_retired == r.parts || r.status != TVK_OK) {
+        r.done = true;
+        r.t_done = t;
+        dev_arena_.free(r.dev_off, r.bytes);
+        ++stats_.completed;
+      }
     }
+    // a failed request's parts not yet launched never run
+    if (failed) drop_done_locked();
     L.tickets.clear();
     L.busy = false;
     any = true;
@@ -312,31 +328,40 @@ bool RenderServer::retire_locked() {
 }
 
 bool RenderServer::launch_locked(Lane& L) {
-  const uint64_t key = reqs_[pending_.front()].key;
+  // The launch's kernel variant is that of the oldest request without a
+  // part on the GPU; every waiting request of that variant joins it, oldest
+  // first, with its next part: after the first, no more than
+  // kLaunchCostCap estimated samples per launch, so that light requests
+  // are not held until a heavy one in the same launch finishes (a skipped
+  // request keeps its place and leads a later launch).
   std::vector<uint64_t> pick;
-  uint64_t cost = 0;
-  for (auto it = pending_.begin(); it != pending_.end() && (int)pick.size() < kMaxJobs;) {
-    Request& r = reqs_[*it];
-    // same kernel variant; after the first request, no more than
-    // kLaunchCostCap estimated samples per launch, so that light requests
-    // are not held until a heavy one in the same launch finishes (a skipped
-    // request keeps its place and leads a later launch)
-    if (r.key != key || (!pick.empty() && cost + r.cost > kLaunchCostCap)) {
-      ++it;
-      continue;
+  std::vector<uint32_t> part;
+  uint64_t key = 0, cost = 0;
+  bool keyed = false;
+  for (uint64_t tk : pending_) {
+    if ((int)pick.size() >= kMaxJobs) break;
+    Request& r = reqs_[tk];
+    if (r.parts_launched > r.parts_retired) continue;  // its previous part is on the GPU
+    if (!keyed) {
+      key = r.key;
+      keyed = true;
     }
-    uint64_t doff = 0, hoff = 0;
-    if (!dev_arena_.alloc(r.bytes, &doff)) break;  // admission by output memory, in order
-    if (!host_arena_.alloc(r.bytes, &hoff)) {
-      dev_arena_.free(doff, r.bytes);
-      break;
+    const uint64_t c = r.cost / r.parts;
+    if (r.key != key || (!pick.empty() && cost + c > kLaunchCostCap)) continue;
+    if (r.parts_launched == 0) {
+      uint64_t doff = 0, hoff = 0;
+      if (!dev_arena_.alloc(r.bytes, &doff)) break;  // admission by output memory, in order
+      if (!host_arena_.alloc(r.bytes, &hoff)) {
+        dev_arena_.free(doff, r.bytes);
+        break;
+      }
+      r.dev_off = doff;
+      r.host_off = hoff;
+      r.host_held = true;
     }
-    r.dev_off = doff;
-    r.host_off = hoff;
-    r.host_held = true;
-    pick.push_back(*it);
-    cost += r.cost;
-    it = pending_.erase(it);
+    pick.push_back(tk);
+    part.push_back(r.parts_launched);
+    cost += c;
   }
   if (pick.empty()) return false;
   std::vector<RaycastParams> jobs(pick.size());
@@ -346,6 +371,11 @@ bool RenderServer::launch_locked(Lane& L) {
     Request& r = reqs_[pick[k]];
     uint32_t* fb = (uint32_t*)((uint8_t*)dev_arena_mem_.p + r.dev_off);
     ok = volumes_[r.q.volume]->ServeJob(r.q.view, r.q.render_mode, fb, &jobs[k], &li) == TVK_OK;
+    if (ok && r.parts > 1) {
+      const int64_t first = (int64_t)part[k] * r.tiles / r.parts;
+      jobs[k].tile_first = (int32_t)first;
+      jobs[k].tile_count = (int32_t)((int64_t)(part[k] + 1) * r.tiles / r.parts - first);
+    }
   }
   // a 1D TF shared by every job of the launch is staged in LDS
   bool shared = ok && li.mode == 0 && jobs[0].n_tf <= 4096;
@@ -365,14 +395,14 @@ bool RenderServer::launch_locked(Lane& L) {
       for (const Lane& x : lanes_) in_flight |= x.busy;
       if (rc == TVK_ERR_OOM && in_flight) {
         // device memory is held by work in flight (a paged batch filling
-        // pool slots, other lanes): put the requests back in front and
-        // launch them once that work is done
-        for (auto it = pick.rbegin(); it != pick.rend(); ++it) {
-          Request& r = reqs_[*it];
+        // pool slots, other lanes): the requests keep their places and
+        // launch once that work is done
+        for (size_t k = 0; k < pick.size(); ++k) {
+          if (part[k] != 0) continue;
+          Request& r = reqs_[pick[k]];
           dev_arena_.free(r.dev_off, r.bytes);
           host_arena_.free(r.host_off, r.bytes);
           r.host_held = false;
-          pending_.push_front(*it);
         }
         memory_wait_ = true;
         return false;
@@ -388,8 +418,10 @@ bool RenderServer::launch_locked(Lane& L) {
     ok = launch_job_table(*slot, jobs.data(), (int)jobs.size(), li, 4, 0, L.stream, true) == TVK_OK;
     if (!ok) why = last_error();
   }
+  // the images finished by this launch (a request's last part) go to host memory
   for (size_t k = 0; k < pick.size() && ok; ++k) {
     Request& r = reqs_[pick[k]];
+    if (part[k] + 1 < r.parts) continue;
     ok = hipMemcpyAsync(host_arena_mem_ + r.host_off, (uint8_t*)dev_arena_mem_.p + r.dev_off, r.bytes,
                         hipMemcpyDeviceToHost, L.stream) == hipSuccess;
   }
@@ -404,18 +436,29 @@ bool RenderServer::launch_locked(Lane& L) {
       r.status = TVK_ERR_HIP;
       r.error = why.empty() ? "launch could not be enqueued" : why;
       r.done = true;
-      r.t_launch = r.t_done = t;
+      r.t_done = t;
+      if (r.t_launch == 0) r.t_launch = t;
       dev_arena_.free(r.dev_off, r.bytes);
       ++stats_.completed;
     }
+    drop_done_locked();
     cv_done_.notify_all();
     return true;
   }
-  for (uint64_t tk : pick) {
-    Request& r = reqs_[tk];
-    r.t_launch = t;
+  for (size_t k = 0; k < pick.size(); ++k) {
+    Request& r = reqs_[pick[k]];
+    if (part[k] == 0) r.t_launch = t;
     r.launch_jobs = (uint32_t)pick.size();
+    ++r.parts_launched;
   }
+  // requests whose last part is on the GPU leave the queue
+  pending_.erase(std::remove_if(pending_.begin(), pending_.end(),
+                                [&](uint64_t tk) {
+                                  const Request& r = reqs_[tk];
+                                  return r.parts_launched == r.parts;
+                                }),
+                 pending_.end());
+  stats_.parts_launched += pick.size();
   L.tickets = pick;
   L.busy = true;
   ++stats_.launches;
@@ -488,12 +531,24 @@ int RenderServer::Wait(uint64_t ticket, uint32_t timeout_ms, tvk_request_result*
     *out = tvk_request_result{};
     out->status = r.status;
     out->launch_jobs = r.launch_jobs;
+    out->parts = r.parts;
     out->evaluated_samples = r.evaluated;
     out->nominal_samples = r.nominal;
     out->queue_ms = r.t_launch > 0 ? r.t_launch - r.t_submit : 0.0;
     out->latency_ms = (copy ? now_ms() : r.t_done) - r.t_submit;
   }
   if (r.status != TVK_OK) return fail(r.status, "request failed" + (r.error.empty() ? "" : ": " + r.error));
+  return TVK_OK;
+}
+
+void RenderServer::drop_done_locked() {
+  pending_.erase(std::remove_if(pending_.begin(), pending_.end(), [&](uint64_t tk) { return reqs_[tk].done; }),
+                 pending_.end());
+}
+
+int RenderServer::SetPartCost(uint64_t samples) {
+  std::lock_guard<std::mutex> lk(mu_);
+  part_cost_ = samples == 0 ? kPartCost : samples;
   return TVK_OK;
 }
 

@@ -6,12 +6,16 @@
 // request's kernel variant picks the launch, and every waiting request of
 // that variant whose output fits the arenas joins it (up to kMaxJobs, and
 // up to kLaunchCostCap estimated samples after the first request, so that a
-// heavy request does not hold light ones until it finishes).  A
-// launch renders into a device arena block per request and copies each
-// image to a pinned host arena block on the lane's stream; the lane is
-// retired when its event completes, which frees the device blocks and
-// wakes the waiters.  Lanes run concurrently, so requests that arrive while
-// others are on the GPU start as soon as a lane and memory free up.
+// heavy request does not hold light ones until it finishes).  A request
+// heavier than the part cost renders in parts: its tile slots split evenly
+// over several launches, one part on the GPU at a time, so that the
+// requests behind it run between its parts instead of waiting for the whole
+// image.  A launch renders into a device arena block per request and copies
+// each finished image to a pinned host arena block on the lane's stream;
+// the lane is retired when its event completes, which frees the device
+// blocks of finished requests and wakes the waiters.  Lanes run
+// concurrently, so requests that arrive while others are on the GPU start
+// as soon as a lane and memory free up.
 #pragma once
 #include <hip/hip_runtime_api.h>
 
@@ -33,6 +37,10 @@ namespace tvk {
 // Estimated samples (tile-rect rays x the volume's longest axis in steps)
 // one launch may carry beyond its first request: about 5 ms of raycasting.
 constexpr uint64_t kLaunchCostCap = 1000000000ull;
+// Default part cost: a request of more estimated samples renders in parts of
+// at most this many (about 20 ms of raycasting each), and at most kMaxParts.
+constexpr uint64_t kPartCost = 4000000000ull;
+constexpr uint32_t kMaxParts = 64;
 
 // First-fit allocator over [0, size) in 256-byte units, with coalescing.
 class Arena {
@@ -55,6 +63,7 @@ class RenderServer {
   int Submit(const tvk_request& q, uint64_t* ticket);
   int Wait(uint64_t ticket, uint32_t timeout_ms, tvk_request_result* out);
   int Stats(tvk_server_stats* out);
+  int SetPartCost(uint64_t samples);
 
  private:
   struct Request {
@@ -63,8 +72,11 @@ class RenderServer {
     double t_submit = 0, t_launch = 0, t_done = 0;
     uint64_t dev_off = 0, host_off = 0, bytes = 0;
     uint64_t cost = 0;       // samples if every ray of its tile rect ran the longest chord
-    uint64_t evaluated = 0, nominal = 0;
+    uint64_t evaluated = 0, nominal = 0;  // summed over its parts
     uint32_t launch_jobs = 0;
+    int tiles = 0;           // its tile slots
+    uint32_t parts = 1;      // launches it renders in: part p takes slots [p T / parts, (p+1) T / parts)
+    uint32_t parts_launched = 0, parts_retired = 0;
     int status = TVK_OK;
     std::string error;       // why it failed (status != TVK_OK)
     bool done = false;
@@ -96,6 +108,7 @@ class RenderServer {
   uint64_t pool_capacity() const;  // device pool bytes available to served batches
   bool retire_locked();  // true when a lane finished
   bool launch_locked(Lane& lane);
+  void drop_done_locked();  // finished (or failed) requests leave the queue
   static uint64_t variant_key(const LaunchInfo& li);
 
   int device_ = -1;
@@ -114,6 +127,7 @@ class RenderServer {
   std::unordered_map<uint64_t, Request> reqs_;
   uint64_t next_ticket_ = 1;
   bool memory_wait_ = false;  // the last launch was put back for device memory
+  uint64_t part_cost_ = kPartCost;
   bool stop_ = false;
   std::thread worker_;
   tvk_server_stats stats_{};

@@ -887,6 +887,14 @@ int tvk_server_get_stats(tvk_server* s, tvk_server_stats* out) {
   return s->impl.Stats(out);
 }
 
+int tvk_server_set_part_cost(tvk_server* s, uint64_t samples) {
+  if (!s) {
+    tvk::set_last_error("null server handle");
+    return TVK_ERR_ARG;
+  }
+  return s->impl.SetPartCost(samples);
+}
+
 static const uint64_t kMlpWeights = 32 * 64 + 3 * 64 * 64 + 64, kMlpBiases = 4 * 64 + 1;
 
 int tvk_lod_mlp_create(int device, const uint16_t* w, const float* b, tvk_lod_mlp** out) {

@@ -177,6 +177,14 @@ struct RaycastParams {
   // framebuffer.  group_n < 0: a member (no tiles of its own).  0: alone.
   int32_t group_n;
   int32_t group_jobs[3];
+  // Parts (the render server's split of a heavy request over launches):
+  // tile_count > 0 renders only the job's tile slots [tile_first,
+  // tile_first + tile_count) -- slots as the queues count them (rect tiles,
+  // or runs of tile_sample) -- and a part with tile_first > 0 leaves the
+  // pixels outside the tile rect to the part that starts at slot 0.  Both
+  // 0: every slot.
+  int32_t tile_first;
+  int32_t tile_count;
 };
 constexpr int kGroupMax = 4;
 

@@ -53,6 +53,11 @@ class RenderServer:
         call("tvk_server_get_stats", self._h, C.byref(st))
         return st
 
+    def setPartCost(self, samples: int) -> None:
+        """Requests above `samples` estimated samples render in parts (0:
+        the default, 4e9; 2**64 - 1: never split)."""
+        call("tvk_server_set_part_cost", self._h, int(samples))
+
     def close(self) -> None:
         if self._h:
             _lib.load().tvk_server_destroy(self._h)

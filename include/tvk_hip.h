@@ -718,8 +718,10 @@ typedef struct tvk_request_result {
   uint32_t launch_jobs;   /* jobs in the launch that rendered it          */
   uint64_t evaluated_samples;
   uint64_t nominal_samples;
-  double queue_ms;        /* submit -> launch                             */
+  double queue_ms;        /* submit -> launch (of its first part)         */
   double latency_ms;      /* submit -> image in host memory               */
+  uint32_t parts;         /* launches it was rendered in (1: not split)   */
+  uint32_t reserved_;
 } tvk_request_result;
 typedef struct tvk_server_stats {
   uint64_t submitted;
@@ -731,6 +733,8 @@ typedef struct tvk_server_stats {
   uint64_t paged_launches;    /* batches of paged volumes rendered            */
   uint64_t pool_waits;        /* batches that waited for device pool memory   */
   uint64_t max_pool_reserved; /* largest pool bytes held by running batches   */
+  uint64_t split_requests;    /* requests rendered in parts                   */
+  uint64_t parts_launched;    /* jobs launched (one per part of a request)    */
 } tvk_server_stats;
 int tvk_server_create(int device, uint64_t arena_bytes, uint32_t lanes, tvk_server** out);
 int tvk_server_destroy(tvk_server* s);
@@ -741,6 +745,15 @@ int tvk_server_submit(tvk_server* s, const tvk_request* req, uint64_t* ticket);
  * ticket.  Returns TVK_ERR_STATE on time-out (the ticket stays valid). */
 int tvk_server_wait(tvk_server* s, uint64_t ticket, uint32_t timeout_ms, tvk_request_result* out);
 int tvk_server_get_stats(tvk_server* s, tvk_server_stats* out);
+/* Requests of more than `samples` estimated samples (as above) render in
+ * parts: their tile slots split evenly over ceil(estimate / samples)
+ * launches (at most 64, and at most one per tile), one part on the GPU at
+ * a time, so that the requests behind a heavy one run between its parts;
+ * the image is copied out after the last part.  Images and sample counts
+ * do not depend on it.  0 restores the default, 4e9 (about 20 ms of
+ * raycasting per part); UINT64_MAX never splits.  Applies to requests
+ * submitted afterwards. */
+int tvk_server_set_part_cost(tvk_server* s, uint64_t samples);
 
 /* Replay batched launches from captured HIP graphs (default on): table
  * upload, counter clear, raycast and counter readback are one graph per

@@ -83,6 +83,40 @@ def test_results_same_alone_and_in_the_mix():
         r.cleanup()
 
 
+def test_requests_in_parts_equal_whole_requests():
+    """A request split over launches (its tile slots in parts, one part on
+    the GPU at a time, light requests launched between its parts) gives the
+    image and sample counts of the whole request."""
+    vols = make_volumes()
+    print("case serving: 40 requests in parts vs whole", flush=True)
+    reqs = random_requests(40, len(vols), seed=9)
+    out = {}
+    for label, cost in (("whole", 2**64 - 1), ("parts", 100_000)):
+        srv = RenderServer(arena_bytes=64 << 20, lanes=3)
+        srv.setPartCost(cost)
+        ids = [srv.addVolume(r) for r, _ in vols]
+        tickets = [srv.submit(ids[v], view, mode) for v, view, mode in reqs]
+        out[label] = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+        print(f"{label}: launches {st.launches} split {st.split_requests} parts {st.parts_launched}",
+              flush=True)
+        assert st.completed == 40
+        if label == "whole":
+            assert st.split_requests == 0 and st.parts_launched == 40
+        else:
+            assert st.split_requests > 20 and st.parts_launched > 100
+        srv.close()
+    assert max(res.parts for _, res in out["parts"]) > 8
+    for i in range(len(reqs)):
+        (a, ra), (b, rb) = out["whole"][i], out["parts"][i]
+        assert ra.parts == 1
+        assert np.array_equal(a, b), f"request {i} ({rb.parts} parts)"
+        assert ra.evaluated_samples == rb.evaluated_samples, f"request {i}"
+        assert ra.nominal_samples == rb.nominal_samples > 0, f"request {i}"
+    for r, _ in vols:
+        r.cleanup()
+
+
 def test_admission_waits_for_output_memory():
     vols = make_volumes()[:1]
     print("case serving admission with a 1 MiB arena", flush=True)

@@ -110,12 +110,11 @@ int HipRenderer::Autotune(const tvk_view* views, uint32_t n, uint32_t flags, tvk
   // batch has paged its bricks in
   const bool rebrickable = !paged_ || host_level0_ != nullptr;
   const Setting before = cur;
-  if ((flags & TVK_TUNE_BRICK) && rebrickable) {
-    std::vector<uint32_t> bricks;
-    for (uint32_t b : {32u, 64u, 128u, 256u})
-      if (b <= std::max<uint64_t>(32, maxdim)) bricks.push_back(b);
+  std::vector<uint32_t> bricks;
+  for (uint32_t b : {32u, 64u, 128u, 256u})
+    if (b <= std::max<uint64_t>(32, maxdim)) bricks.push_back(b);
+  if ((flags & TVK_TUNE_BRICK) && rebrickable)
     if (int e = try_values(bricks, &Setting::brick, true)) return e;
-  }
   // cells, then the launch shape again; a second round when the launch
   // shape moved (the best cell size depends on it: coordinate descent
   // from one start settled 5% away from the best on the paged 2D-TF batch)
@@ -164,6 +163,40 @@ int HipRenderer::Autotune(const tvk_view* views, uint32_t n, uint32_t flags, tvk
                        cur.tile_group != start.tile_group || cur.wg_waves != start.wg_waves ||
                        cur.stage != start.stage;
     if (!moved || !(flags & TVK_TUNE_CELL) || !rebrickable) break;
+  }
+  // the brick sizes once more with the tuned cells (cells as large as their
+  // brick grow with it): the first pass timed each brick size with the
+  // starting cells, which settled the paged 2D-TF batch on 128^3 bricks and
+  // cells in one run (126 ms) and on 256^3 ones (121 ms) in another
+  if ((flags & TVK_TUNE_BRICK) && (flags & TVK_TUNE_CELL) && rebrickable) {
+    const Setting tuned = cur;
+    Setting set = cur;  // the bricking the volume is set up with
+    for (uint32_t b : bricks) {
+      if (b == tuned.brick) continue;
+      Setting s = tuned;
+      s.brick = b;
+      s.cell = tuned.cell == tuned.brick ? b : tuned.cell;
+      if (s.cell > s.brick || s.brick % s.cell != 0) continue;
+      set = s;
+      if (int e = apply(s, true)) {
+        if (int e2 = apply(cur, true)) return e2;
+        set = cur;
+        (void)e;
+        continue;
+      }
+      float ms = 0.f;
+      if (int e = time_now(&ms)) return e;
+      if (ms < best) {
+        best = ms;
+        cur = s;
+      }
+    }
+    if (set.brick != cur.brick || set.cell != cur.cell) {
+      // set the chosen bricking up again, and page its bricks in (a paged
+      // volume) with one batch, as every timed setting was
+      if (int e = apply(cur, true)) return e;
+      if (int e = RenderBatch(v.data(), n, nullptr, nullptr)) return e;
+    }
   }
   if (out) {
     *out = tvk_tuning{};

@@ -660,8 +660,10 @@ int tvk_lod_select_octree(const float* pred, const double* area, const uint32_t 
 /* Launch autotuner (no counterpart in the reference).  Times the given
  * views (tvk_render_batch, images stay on the device) under candidate
  * settings by coordinate descent and leaves the fastest applied:
- *   TVK_TUNE_BRICK  brick size in {32, 64, 128, 256} (in-core volumes;
- *                   the device-resident levels are re-bricked),
+ *   TVK_TUNE_BRICK  brick size in {32, 64, 128, 256} (in-core volumes
+ *                   and paged volumes set from flat voxels, re-bricked
+ *                   from them), tried with the starting cells and again
+ *                   with the tuned ones,
  *   TVK_TUNE_CELL   empty-space cell edge (divisors of the brick size),
  *   TVK_TUNE_LAUNCH samples per batch, ray tile width, XCD tile group.
  * Cell and launch settings leave the images bit-identical; a different

@@ -715,7 +715,8 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the given launch settings instead of autotuning them")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
-    ap.add_argument("--serving-seconds", type=float, default=10.0)
+    ap.add_argument("--serving-seconds", type=float, default=20.0,
+                    help="seconds of arrivals at the full rate (then half as long at half the rate)")
     ap.add_argument("--serving-rate", type=float, default=200.0)
     ap.add_argument("--serving-only", action="store_true")
     ap.add_argument("--serving-lanes", type=int, default=4, help="streams of the render server")
